@@ -1,0 +1,252 @@
+"""gstore_amd -- MI355X G-Store engine, Python host side.
+
+A thin ctypes layer over the C ABI in include/gstore_amd.h (libgstore_amd.so,
+built by gstore_amd.build).  It mirrors the reference's job surface
+(gstore.cpp:235-333): ``bfs`` is -j 0/20, ``pagerank`` -j 1/21, ``kcore``
+-j 2/22, ``wcc`` -j 3/23, with the reference's argument defaults (PageRank 5
+iterations, BFS root 0, k = 10; gstore.cpp:2194-2196, 2307, 2353).
+
+There is no CPU fallback: if the HIP library is missing, importing works but
+every call raises GStoreError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import tiles
+
+__all__ = ["Graph", "GStoreError", "kron_edges", "convert", "load_library", "tiles", "LIB_PATH",
+           "PR_COMPAT", "PR_INTENDED"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgstore_amd.so")
+
+PR_COMPAT = 0
+PR_INTENDED = 1
+
+GS_OK = 0
+_STATUS = {1: "GS_ERR_ARG", 2: "GS_ERR_IO", 3: "GS_ERR_FORMAT", 4: "GS_ERR_NOMEM",
+           5: "GS_ERR_DEVICE", 6: "GS_ERR_UNSUPPORTED", 7: "GS_ERR_DEPTH_OVERFLOW"}
+KERNELS = ["pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end", "wcc_hook", "wcc_compress",
+           "wcc_check", "kcore_peel", "kcore_scan", "kron_gen", "convert"]
+
+
+class GStoreError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("vert_count", ctypes.c_uint64), ("edge_count", ctypes.c_uint64),
+                ("tile_slots", ctypes.c_uint64), ("nonempty_tiles", ctypes.c_uint64),
+                ("groups", ctypes.c_uint64), ("tiles_per_side", ctypes.c_uint64),
+                ("device_bytes", ctypes.c_uint64), ("directed", ctypes.c_int32),
+                ("resident", ctypes.c_int32)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double), ("iterations", ctypes.c_uint64),
+                ("edges_scanned", ctypes.c_uint64), ("tiles_scanned", ctypes.c_uint64),
+                ("kern_ms", ctypes.c_double * len(KERNELS)),
+                ("kern_calls", ctypes.c_uint64 * len(KERNELS))]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+_P = ctypes.c_void_p
+_SIGS = {
+    "gs_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "gs_open_arrays": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, _P, _P, _P, ctypes.POINTER(_P)]),
+    "gs_open_kron": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.POINTER(_P)]),
+    "gs_kron_edges": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
+                                     ctypes.c_int64, _P]),
+    "gs_convert": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p]),
+    "gs_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "gs_get_info": (ctypes.c_int, [_P, ctypes.POINTER(_Info)]),
+    "gs_get_arrays": (ctypes.c_int, [_P, _P, _P, _P]),
+    "gs_pagerank": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P]),
+    "gs_bfs": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "gs_wcc": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64)]),
+    "gs_kcore": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_uint64),
+                                ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
+    "gs_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(_Stats)]),
+    "gs_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "gs_close": (None, [_P]),
+    "gs_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
+}
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libgstore_amd.so (raises GStoreError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GStoreError(5, f"{path} is not built; run gstore_amd.build.build_all()")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != GS_OK:
+        buf = ctypes.create_string_buffer(1024)
+        _lib.gs_last_error(buf, 1024)
+        raise GStoreError(status, buf.value.decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class BfsResult:
+    depth: np.ndarray          # u8, 0 = unreached, root = 1
+    level_counts: list         # vertices reached at level 1, 2, ... (last entry 0)
+
+
+class Graph:
+    """A tile grid resident in HBM (one gs_graph handle)."""
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self._h = handle
+        info = _Info()
+        _check(_lib.gs_get_info(self._h, ctypes.byref(info)))
+        self.n = int(info.vert_count)
+        self.E = int(info.edge_count)
+        self.T = int(info.tile_slots)
+        self.nonempty_tiles = int(info.nonempty_tiles)
+        self.p = int(info.groups)
+        self.p_s = int(info.tiles_per_side)
+        self.directed = bool(info.directed)
+
+    # --- construction -------------------------------------------------------
+    @classmethod
+    def open(cls, prefix: str, vert_count: int, directed: bool = False, device_budget: int = 0) -> "Graph":
+        load_library()
+        h = ctypes.c_void_p()
+        _check(_lib.gs_open(prefix.encode(), vert_count, int(directed), device_budget, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, n: int, start: np.ndarray, grid: np.ndarray, degree: np.ndarray,
+                    directed: bool = False) -> "Graph":
+        load_library()
+        start = np.ascontiguousarray(start, dtype=np.uint64)
+        grid = np.ascontiguousarray(grid, dtype=np.uint32)
+        degree = np.ascontiguousarray(degree, dtype=np.int32)
+        T = tiles.total_slots(tiles.tiles_per_side(n), directed)
+        if start.size != T + 1 or degree.size != n or grid.size != int(start[-1]):
+            raise GStoreError(1, f"array sizes do not match N={n}: start {start.size} (want {T + 1}), "
+                                 f"grid {grid.size} (want {int(start[-1])}), degree {degree.size}")
+        h = ctypes.c_void_p()
+        _check(_lib.gs_open_arrays(n, int(directed), _ptr(start), _ptr(grid) if grid.size else None,
+                                   _ptr(degree), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def kron(cls, scale: int, edge_factor: int = 16, seed1: int = 1, seed2: int = 2,
+             directed: bool = False) -> "Graph":
+        load_library()
+        h = ctypes.c_void_p()
+        _check(_lib.gs_open_kron(scale, edge_factor, seed1, seed2, int(directed), ctypes.byref(h)))
+        return cls(h)
+
+    def close(self) -> None:
+        if self._h:
+            _lib.gs_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- data ---------------------------------------------------------------
+    def arrays(self):
+        start = np.empty(self.T + 1, np.uint64)
+        grid = np.empty(self.E, np.uint32)
+        degree = np.empty(self.n, np.int32)
+        _check(_lib.gs_get_arrays(self._h, _ptr(start), _ptr(grid) if self.E else None, _ptr(degree)))
+        return start, grid, degree
+
+    def degree(self) -> np.ndarray:
+        degree = np.empty(self.n, np.int32)
+        _check(_lib.gs_get_arrays(self._h, None, None, _ptr(degree)))
+        return degree
+
+    def save(self, prefix: str) -> None:
+        _check(_lib.gs_save(self._h, prefix.encode()))
+
+    def set_kernel_timing(self, on: bool) -> None:
+        _check(_lib.gs_set_kernel_timing(self._h, int(on)))
+
+    def stats(self) -> dict:
+        st = _Stats()
+        _check(_lib.gs_get_stats(self._h, ctypes.byref(st)))
+        return {"ms": st.ms, "iterations": int(st.iterations), "edges_scanned": int(st.edges_scanned),
+                "tiles_scanned": int(st.tiles_scanned),
+                "kernels": {KERNELS[i]: {"ms": st.kern_ms[i], "calls": int(st.kern_calls[i])}
+                            for i in range(len(KERNELS)) if st.kern_calls[i]}}
+
+    # --- jobs -----------------------------------------------------------------
+    def pagerank(self, iters: int = 5, mode: int = PR_COMPAT, copy_out: bool = True) -> Optional[np.ndarray]:
+        out = np.empty(self.n, np.float32) if copy_out else None
+        _check(_lib.gs_pagerank(self._h, iters, mode, _ptr(out) if copy_out else None))
+        return out
+
+    def bfs(self, root: int = 0, copy_out: bool = True, max_levels: int = 256) -> BfsResult:
+        depth = np.empty(self.n, np.uint8) if copy_out else None
+        counts = np.zeros(max_levels, np.uint64)
+        nlev = ctypes.c_int(0)
+        _check(_lib.gs_bfs(self._h, root, _ptr(depth) if copy_out else None, _ptr(counts), max_levels,
+                           ctypes.byref(nlev)))
+        return BfsResult(depth, [int(c) for c in counts[:min(nlev.value, max_levels)]])
+
+    def wcc(self, copy_out: bool = True):
+        labels = np.empty(self.n, np.uint32) if copy_out else None
+        nc = ctypes.c_uint64(0)
+        _check(_lib.gs_wcc(self._h, _ptr(labels) if copy_out else None, ctypes.byref(nc)))
+        return labels, int(nc.value)
+
+    def kcore(self, k: int = 10, copy_out: bool = True):
+        deg = np.empty(self.n, np.int32) if copy_out else None
+        ne0, pos, rounds = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_int(0)
+        _check(_lib.gs_kcore(self._h, k, _ptr(deg) if copy_out else None, ctypes.byref(ne0), ctypes.byref(pos),
+                             ctypes.byref(rounds)))
+        return deg, int(ne0.value), int(pos.value), int(rounds.value)
+
+
+def kron_edges(scale: int, e0: int, e1: int, seed1: int = 1, seed2: int = 2) -> np.ndarray:
+    """Graph500 tuples [e0, e1) generated on the device, shape (e1-e0, 2) u32."""
+    load_library()
+    out = np.empty((e1 - e0, 2), np.uint32)
+    _check(_lib.gs_kron_edges(scale, seed1, seed2, e0, e1, _ptr(out)))
+    return out
+
+
+def convert(edges: np.ndarray, vert_count: int, prefix: str, directed: bool = False) -> None:
+    """-c 1: tuples (M, 2) u32 -> PREFIX.grid/.start/.degree, converted on the device."""
+    load_library()
+    edges = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+    _check(_lib.gs_convert(_ptr(edges) if edges.size else None, edges.shape[0], vert_count, int(directed),
+                           prefix.encode()))

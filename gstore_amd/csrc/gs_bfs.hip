@@ -1,0 +1,168 @@
+// gs_bfs.hip -- level-synchronous BFS over the resident tile grid.
+//
+// Restates bfs_t (bfs.cpp:56-338) driven as bfs_mmap (gstore.cpp:2631-2713):
+// depth is u8 with 0 = unreached and root = 1; level L reaches v1 from v0
+// when d[v0] == L and d[v1] == 0, and v0 from v1 the other way round
+// (bfs.cpp:158-177); a tile is swept only when its tile row or tile column
+// holds a frontier vertex (gstore.cpp:2687-2698).
+//
+// Device form: the frontier (depth == L), the visited set (depth != 0) and
+// the next frontier are N-bit bitmaps, so the per-edge test reads two bits
+// from 8 KiB tile windows that stay in cache; newly reached vertices are set
+// with one atomicOr per (vertex word) and their depth bytes are written once
+// by the level-end pass, which also rebuilds the tile-row activity bits and
+// counts the new frontier exactly.
+#include "gs_internal.h"
+
+namespace gs {
+
+__global__ void k_bfs_root(uint32_t root, uint8_t *depth, uint32_t *vis, uint32_t *front, uint32_t *rowact) {
+    depth[root] = 1;
+    vis[root >> 5] |= 1u << (root & 31);
+    front[root >> 5] |= 1u << (root & 31);
+    uint32_t r = root >> 16;
+    rowact[r >> 5] |= 1u << (r & 31);
+}
+
+template <bool DIRECTED>
+__global__ void __launch_bounds__(SWEEP_WG) k_bfs_expand(SweepArgs a, const uint32_t *__restrict__ front,
+                                                         const uint32_t *__restrict__ vis, uint32_t *next,
+                                                         const uint32_t *__restrict__ rowact,
+                                                         unsigned long long *scanned) {
+    __shared__ ChunkTiles s;
+    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
+    if (chunk >= a.nchunk) return;
+    uint32_t len = stage_chunk(a, chunk, s, [&](uint32_t rc) {
+        return bit_of(rowact, rc >> 16) || (!DIRECTED && bit_of(rowact, rc & 0xFFFFu));
+    });
+    if (!s.any) return;
+    count_active_edges(s, scanned);
+    auto reach = [&](uint32_t v) {
+        uint32_t wi = v >> 5, b = 1u << (v & 31);
+        if ((vis[wi] & b) == 0 && (next[wi] & b) == 0) atomicOr(&next[wi], b);
+    };
+    auto op = [&](uint32_t w, uint32_t rc) {
+        uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
+        if ((front[v0 >> 5] >> (v0 & 31)) & 1u) {
+            reach(v1);
+        } else if (!DIRECTED && ((front[v1 >> 5] >> (v1 & 31)) & 1u)) {
+            reach(v0);
+        }
+    };
+    sweep_chunk<true>(a, chunk, s, len, op);
+}
+
+// One workgroup per tile row (2048 bitmap words).
+__global__ void __launch_bounds__(256) k_bfs_level_end(uint32_t *vis, uint32_t *front, uint32_t *next,
+                                                       uint32_t *rowact, uint8_t *depth, uint32_t newdepth,
+                                                       unsigned long long *count) {
+    const uint32_t r = blockIdx.x;
+    const uint64_t base = (uint64_t)r * ROW_WORDS;
+    uint32_t cnt = 0;
+    for (uint32_t w = threadIdx.x; w < ROW_WORDS; w += blockDim.x) {
+        uint64_t gw = base + w;
+        uint32_t nx = next[gw];
+        front[gw] = nx;
+        if (nx) {
+            next[gw] = 0;
+            vis[gw] |= nx;
+            cnt += __popc(nx);
+            uint32_t m = nx;
+            while (m) {
+                uint32_t b = __ffs(m) - 1;
+                m &= m - 1;
+                depth[gw * 32 + b] = (uint8_t)newdepth;
+            }
+        }
+    }
+    int any = __syncthreads_or(cnt != 0);
+    // block sum of cnt
+    __shared__ unsigned long long tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    if (cnt) atomicAdd(&tot, (unsigned long long)cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (any) atomicOr(&rowact[r >> 5], 1u << (r & 31));
+        else atomicAnd(&rowact[r >> 5], ~(1u << (r & 31)));
+        if (tot) atomicAdd(count, tot);
+    }
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" int gs_bfs(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts, int max_levels,
+                      int *levels) {
+    if (!g || root >= g->n || max_levels < 0) {
+        set_error("gs_bfs: bad arguments (root %llu, N %llu)", (unsigned long long)root,
+                  g ? (unsigned long long)g->n : 0ull);
+        return GS_ERR_ARG;
+    }
+    GS_TRY(ensure_bitmaps(g));
+    if (g->depth.n != g->n) GS_TRY(g->depth.alloc(g->n));
+    const uint64_t words = bitmap_words(g);
+    g->timer.reset();
+    hipEvent_t t0, t1;
+    GS_HIP(hipEventCreate(&t0));
+    GS_HIP(hipEventCreate(&t1));
+    GS_HIP(hipEventRecord(t0, g->stream));
+    GS_HIP(hipMemsetAsync(g->depth.ptr, 0, g->n, g->stream));
+    GS_HIP(hipMemsetAsync(g->bm_vis.ptr, 0, words * 4, g->stream));
+    GS_HIP(hipMemsetAsync(g->bm_front.ptr, 0, words * 4, g->stream));
+    GS_HIP(hipMemsetAsync(g->bm_next.ptr, 0, words * 4, g->stream));
+    GS_HIP(hipMemsetAsync(g->rowact.ptr, 0, g->rowact.bytes(), g->stream));
+    GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 16, g->stream));
+    k_bfs_root<<<1, 1, 0, g->stream>>>((uint32_t)root, g->depth.ptr, g->bm_vis.ptr, g->bm_front.ptr, g->rowact.ptr);
+    GS_HIP(hipGetLastError());
+    int nlev = 0, status = GS_OK;
+    for (uint32_t level = 1;; ++level) {
+        GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 8, g->stream));
+        hipEvent_t kb;
+        if (g->nchunk) {
+            g->timer.begin(GS_K_BFS_EXPAND, g->stream, &kb);
+            if (g->directed)
+                k_bfs_expand<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(
+                    sweep_args(g), g->bm_front.ptr, g->bm_vis.ptr, g->bm_next.ptr, g->rowact.ptr,
+                    g->counters.ptr + 1);
+            else
+                k_bfs_expand<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(
+                    sweep_args(g), g->bm_front.ptr, g->bm_vis.ptr, g->bm_next.ptr, g->rowact.ptr,
+                    g->counters.ptr + 1);
+            GS_HIP(hipGetLastError());
+            g->timer.end(GS_K_BFS_EXPAND, g->stream, kb);
+        }
+        g->timer.begin(GS_K_BFS_LEVEL_END, g->stream, &kb);
+        k_bfs_level_end<<<(unsigned)g->rows, 256, 0, g->stream>>>(g->bm_vis.ptr, g->bm_front.ptr, g->bm_next.ptr,
+                                                                   g->rowact.ptr, g->depth.ptr, level + 1,
+                                                                   g->counters.ptr);
+        GS_HIP(hipGetLastError());
+        g->timer.end(GS_K_BFS_LEVEL_END, g->stream, kb);
+        GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 16, hipMemcpyDeviceToHost, g->stream));
+        GS_HIP(hipStreamSynchronize(g->stream));
+        uint64_t cnt = g->h_counters[0];
+        if (level_counts && nlev < max_levels) level_counts[nlev] = cnt;
+        ++nlev;
+        if (cnt == 0) break;
+        if (level + 1 > 255) {
+            set_error("BFS from %llu needs depth > 255, beyond the u8 depth encoding", (unsigned long long)root);
+            status = GS_ERR_DEPTH_OVERFLOW;
+            break;
+        }
+    }
+    GS_HIP(hipEventRecord(t1, g->stream));
+    GS_TRY(sync_and_collect(g));
+    float ms = 0.f;
+    GS_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    g->stats = gs_stats{};
+    g->stats.ms = ms;
+    g->stats.iterations = (uint64_t)nlev;
+    g->stats.edges_scanned = g->h_counters[1];
+    if (levels) *levels = nlev;
+    if (status != GS_OK) return status;
+    if (depth_out) GS_HIP(hipMemcpy(depth_out, g->depth.ptr, g->n, hipMemcpyDeviceToHost));
+    return GS_OK;
+}

@@ -1,0 +1,267 @@
+// gs_internal.h -- shared state, tile geometry and the edge-chunk sweep used
+// by every kernel of the engine.
+//
+// HBM layout of a resident graph (N vertices, E stored edges, T tile slots):
+//   grid     u32[E]        one word per stored edge: v0_local | v1_local << 16,
+//                          tiles back to back in .start order (as on disk)
+//   tz_beg   u64[nz + 1]   first edge of every non-empty tile, tz_beg[nz] = E
+//   tz_rc    u32[nz]       tile row << 16 | tile column (global tile coords)
+//   chunk_tz u32[nchunk+1] non-empty tile holding edge c*CHUNK; last = nz-1
+//   degree   i32[N]        the .degree file
+// Edges are swept in fixed chunks of CHUNK edges, one workgroup per chunk;
+// the tiles a chunk touches are staged in LDS so each lane finds its tile by
+// a short search and decodes the 16-bit local ids in registers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/gstore_amd.h"
+
+namespace gs {
+
+constexpr int CHUNK = 4096;       // edges per workgroup sweep
+constexpr int SWEEP_WG = 256;     // threads per sweep workgroup (4 waves)
+constexpr int QUADS = CHUNK / 4 / SWEEP_WG;  // 16-byte loads per lane
+constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr uint32_t ROW_WORDS = 65536 / 32;   // bitmap words per tile row
+
+void set_error(const char *fmt, ...);
+
+#define GS_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            gs::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,              \
+                          hipGetErrorString(e_));                                 \
+            return GS_ERR_DEVICE;                                                 \
+        }                                                                         \
+    } while (0)
+
+#define GS_TRY(expr)                 \
+    do {                             \
+        int s_ = (expr);             \
+        if (s_ != GS_OK) return s_;  \
+    } while (0)
+
+// ---- tile geometry (gstore.h:109-141 undirected, :176-198 directed) ------
+__host__ __device__ inline uint64_t tri_index(uint64_t i, uint64_t j, uint64_t n) {
+    return ((i * (2 * n - i - 1)) >> 1) + j;
+}
+__host__ __device__ inline uint64_t groups_for(uint64_t n) {
+    return (n >> 24) + ((n & 0xFFFFFFull) != 0);
+}
+__host__ __device__ inline uint64_t total_slots(uint64_t p_s, bool directed) {
+    return directed ? p_s * p_s : (p_s * (p_s + 1)) >> 1;
+}
+// .start index of tile (r, c); undirected requires r <= c.
+__host__ __device__ inline uint64_t tile_slot(uint64_t r, uint64_t c, uint64_t p, bool directed) {
+    uint64_t bi = r >> 8, si = r & 255, bj = c >> 8, sj = c & 255;
+    if (directed) return ((bi * p + bj) << 16) + si * 256 + sj;
+    if (bi == bj) return ((tri_index(bi, bi, p) - bi) << 16) + ((bi * 257) << 7) + tri_index(si, sj, 256);
+    return ((tri_index(bi, bj, p) - bi - 1) << 16) + (((bi + 1) * 257) << 7) + si * 256 + sj;
+}
+
+// ---- device buffers --------------------------------------------------------
+template <class T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        if (hipMalloc(&ptr, count * sizeof(T)) != hipSuccess) {
+            ptr = nullptr;
+            set_error("hipMalloc of %zu bytes failed", count * sizeof(T));
+            return GS_ERR_NOMEM;
+        }
+        n = count;
+        return GS_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Per-graph timing of kernel classes (HIP events on the graph's stream).
+struct KernelTimer {
+    bool on = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    double ms[GS_K_COUNT] = {};
+    uint64_t calls[GS_K_COUNT] = {};
+    void begin(int k, hipStream_t s, hipEvent_t *out);
+    void end(int k, hipStream_t s, hipEvent_t b);
+    int collect();  // after a stream sync
+    void reset();
+};
+
+}  // namespace gs
+
+struct gs_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool directed = false;
+    uint64_t n = 0, p = 0, p_s = 0, T = 0, E = 0, nz = 0, nchunk = 0;
+    uint64_t rows = 0;                       // tile rows holding vertices: ceil(N/65536)
+    gs::DevBuf<uint32_t> grid;
+    gs::DevBuf<uint64_t> tz_beg;
+    gs::DevBuf<uint32_t> tz_rc;
+    gs::DevBuf<uint32_t> chunk_tz;
+    gs::DevBuf<int32_t> degree;
+    // algorithm state, allocated on first use and kept
+    gs::DevBuf<float> pr_prior, pr_acc, pr_fac;
+    gs::DevBuf<uint8_t> depth;
+    gs::DevBuf<uint32_t> bm_vis, bm_front, bm_next, rowact;  // bitmaps, words
+    gs::DevBuf<uint32_t> wcc_parent;
+    gs::DevBuf<int32_t> kc_deg;
+    gs::DevBuf<unsigned long long> counters;
+    unsigned long long *h_counters = nullptr;  // pinned mirror of counters
+    gs::KernelTimer timer;
+    gs_stats stats{};
+    // last results live in: pr_prior (rank), depth (BFS), wcc_parent (labels), kc_deg
+};
+
+namespace gs {
+
+int graph_create(gs_graph **out);
+int graph_build_meta(gs_graph *g, const uint64_t *h_start);  // tz/chunk lists, from host .start
+int ensure_bitmaps(gs_graph *g);
+uint64_t bitmap_words(const gs_graph *g);  // rows * ROW_WORDS
+int sync_and_collect(gs_graph *g);
+
+// XCD-aware chunk order: workgroup b runs on XCD b % 8; give each XCD a
+// contiguous run of chunks so its L2 sees consecutive tiles.
+__device__ inline uint64_t xcd_chunk(uint64_t bid, uint64_t nchunk) {
+    uint64_t per = (nchunk + NUM_XCD - 1) / NUM_XCD;
+    return (bid % NUM_XCD) * per + bid / NUM_XCD;
+}
+inline uint64_t xcd_grid(uint64_t nchunk) {
+    uint64_t per = (nchunk + NUM_XCD - 1) / NUM_XCD;
+    return per * NUM_XCD;
+}
+
+// Tiles of one chunk, staged in LDS.
+struct ChunkTiles {
+    uint16_t beg[CHUNK + 2];  // first edge of tile k relative to the chunk (clamped), beg[nt] = len
+    uint32_t rc[CHUNK + 1];
+    uint8_t act[CHUNK + 1];   // per-tile activity flag (frontier/peel sweeps)
+    int nt;
+    int any;
+};
+
+struct SweepArgs {
+    const uint32_t *grid;
+    const uint64_t *tz_beg;
+    const uint32_t *tz_rc;
+    const uint32_t *chunk_tz;
+    uint64_t E, nchunk;
+};
+
+inline SweepArgs sweep_args(const gs_graph *g) {
+    return SweepArgs{g->grid.ptr, g->tz_beg.ptr, g->tz_rc.ptr, g->chunk_tz.ptr, g->E, g->nchunk};
+}
+
+__device__ inline bool bit_of(const uint32_t *bm, uint32_t rc_row) {
+    return (bm[rc_row >> 5] >> (rc_row & 31)) & 1u;
+}
+
+// Stage the chunk's tiles.  ActFn(rc) -> bool gives a tile's activity.
+// Returns chunk length (0 if the chunk is out of range).
+template <class ActFn>
+__device__ inline uint32_t stage_chunk(const SweepArgs &a, uint64_t chunk, ChunkTiles &s, ActFn act_fn) {
+    uint64_t base = chunk * (uint64_t)CHUNK;
+    uint32_t len = (uint32_t)min<uint64_t>((uint64_t)CHUNK, a.E - base);
+    uint32_t t0 = a.chunk_tz[chunk], t1 = a.chunk_tz[chunk + 1];
+    int nt = (int)(t1 - t0) + 1;
+    if (threadIdx.x == 0) {
+        s.nt = nt;
+        s.any = 0;
+        s.beg[nt] = (uint16_t)len;
+    }
+    int any = 0;
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+        uint64_t b = a.tz_beg[t0 + k];
+        s.beg[k] = (uint16_t)(b <= base ? 0 : min<uint64_t>(b - base, len));
+        uint32_t rc = a.tz_rc[t0 + k];
+        s.rc[k] = rc;
+        bool ac = act_fn(rc);
+        s.act[k] = ac;
+        any |= ac;
+    }
+    __syncthreads();
+    if (any) s.any = 1;  // benign same-value race
+    __syncthreads();
+    return len;
+}
+
+// Adds the chunk's edges in active tiles to *out (one atomic per workgroup).
+__device__ inline void count_active_edges(const ChunkTiles &s, unsigned long long *out) {
+    __shared__ uint32_t acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    uint32_t c = 0;
+    for (int k = threadIdx.x; k < s.nt; k += blockDim.x)
+        if (s.act[k]) c += (uint32_t)s.beg[k + 1] - s.beg[k];
+    if (c) atomicAdd(&acc, c);
+    __syncthreads();
+    if (threadIdx.x == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
+__device__ inline int find_tile(const ChunkTiles &s, uint32_t off) {
+    int lo = 0, hi = s.nt - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (s.beg[mid] <= off) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Visit every edge of the chunk: op(word, rc, tile_k).  Lanes read 16-byte
+// quads so a wave streams 1 KiB of contiguous edges per load.  Edges of
+// inactive tiles are skipped when skip_inactive is set (quads fully inside
+// one inactive tile are not loaded at all).
+template <bool SKIP_INACTIVE, class Op>
+__device__ inline void sweep_chunk(const SweepArgs &a, uint64_t chunk, const ChunkTiles &s,
+                                   uint32_t len, Op &op) {
+    const uint64_t base = chunk * (uint64_t)CHUNK;
+    const uint32_t *g = a.grid + base;
+#pragma unroll 1
+    for (int q = 0; q < QUADS; ++q) {
+        uint32_t off = (uint32_t)(q * SWEEP_WG + threadIdx.x) * 4u;
+        if (off >= len) break;
+        int k = find_tile(s, off);
+        if (SKIP_INACTIVE) {
+            if (!s.act[k] && s.beg[k + 1] >= off + 4) continue;
+        }
+        uint32_t w[4];
+        if (off + 4 <= len) {
+            uint4 v = *reinterpret_cast<const uint4 *>(g + off);
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+            for (int i = 0; i < 4; ++i) w[i] = (off + i < len) ? g[off + i] : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t e = off + i;
+            if (e >= len) break;
+            while (s.beg[k + 1] <= e) ++k;
+            if (SKIP_INACTIVE && !s.act[k]) continue;
+            op(w[i], s.rc[k]);
+        }
+    }
+}
+
+// Global ids of an edge word in tile rc: row << 16 | v0_local, col << 16 | v1_local.
+__device__ inline uint32_t gv0(uint32_t w, uint32_t rc) { return (rc & 0xFFFF0000u) | (w & 0xFFFFu); }
+__device__ inline uint32_t gv1(uint32_t w, uint32_t rc) { return (rc << 16) | (w >> 16); }
+
+}  // namespace gs

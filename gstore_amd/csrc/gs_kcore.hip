@@ -1,0 +1,150 @@
+// gs_kcore.hip -- k-core peeling over the resident tile grid.
+//
+// Restates kcore_t (kcore.cpp:55-217) driven as kcore_mmap
+// (gstore.cpp:2716-2804):
+//   peel   m = (int16)deg[v]; if 0 < m < k: deg[v] = 0, v active
+//          (kcore.cpp:186-196); the first peel runs before any edge pass
+//   scan   for every stored edge in a tile whose row or column holds an
+//          active vertex: if active(v0) --deg[v1]; if active(v1) --deg[v0]
+//          (kcore.cpp:69-103, tile skip gstore.cpp:2768-2781)
+//   until a peel removes nothing; result count(deg != 0) (gstore.cpp:2794-2800)
+// Decrements are integer atomics, so the final degree array is independent
+// of order and bit-exact against the CPU restatement.  The active set is an
+// N-bit bitmap built with one wave ballot per 64 vertices.
+#include "gs_internal.h"
+
+namespace gs {
+
+// One workgroup per tile row (65536 vertices, 256 steps of 256).
+__global__ void __launch_bounds__(256) k_kc_peel(int32_t *deg, uint64_t n, int k, uint32_t *act, uint32_t *rowact,
+                                                 unsigned long long *count) {
+    const uint32_t r = blockIdx.x;
+    const uint64_t rbase = (uint64_t)r << 16;
+    uint32_t cnt = 0;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t step = 0; step < 256; ++step) {
+        uint64_t v = rbase + step * 256 + threadIdx.x;
+        bool pred = false;
+        if (v < n) {
+            int16_t m = (int16_t)deg[v];
+            if (m > 0 && m < k) {
+                deg[v] = 0;
+                pred = true;
+            }
+        }
+        unsigned long long mask = __ballot(pred);
+        if (lane == 0) {
+            uint64_t wv = v >> 5;  // v is the wave's first vertex here, a multiple of 64
+            act[wv] = (uint32_t)mask;
+            act[wv + 1] = (uint32_t)(mask >> 32);
+            cnt += __popcll(mask);
+        }
+    }
+    int any = __syncthreads_or(cnt != 0);
+    __shared__ unsigned long long tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    if (cnt) atomicAdd(&tot, (unsigned long long)cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (any) atomicOr(&rowact[r >> 5], 1u << (r & 31));
+        else atomicAnd(&rowact[r >> 5], ~(1u << (r & 31)));
+        if (tot) atomicAdd(count, tot);
+    }
+}
+
+__global__ void __launch_bounds__(SWEEP_WG) k_kc_scan(SweepArgs a, const uint32_t *__restrict__ act, int32_t *deg,
+                                                      const uint32_t *__restrict__ rowact,
+                                                      unsigned long long *scanned) {
+    __shared__ ChunkTiles s;
+    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
+    if (chunk >= a.nchunk) return;
+    uint32_t len = stage_chunk(a, chunk, s, [&](uint32_t rc) {
+        return bit_of(rowact, rc >> 16) || bit_of(rowact, rc & 0xFFFFu);
+    });
+    if (!s.any) return;
+    count_active_edges(s, scanned);
+    auto op = [&](uint32_t w, uint32_t rc) {
+        uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
+        if ((act[v0 >> 5] >> (v0 & 31)) & 1u) atomicSub(&deg[v1], 1);
+        if ((act[v1 >> 5] >> (v1 & 31)) & 1u) atomicSub(&deg[v0], 1);
+    };
+    sweep_chunk<true>(a, chunk, s, len, op);
+}
+
+__global__ void k_kc_count(const int32_t *__restrict__ deg, uint64_t n, unsigned long long *out) {
+    uint32_t ne = 0, pos = 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        int32_t d = deg[v];
+        ne += d != 0;
+        pos += d > 0;
+    }
+    if (ne) atomicAdd(&out[0], (unsigned long long)ne);
+    if (pos) atomicAdd(&out[1], (unsigned long long)pos);
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" int gs_kcore(gs_graph *g, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
+                        int *rounds) {
+    if (!g) { set_error("gs_kcore: null graph"); return GS_ERR_ARG; }
+    if (g->directed) {
+        set_error("k-core is not applicable to directed graphs");
+        return GS_ERR_UNSUPPORTED;
+    }
+    GS_TRY(ensure_bitmaps(g));
+    if (g->kc_deg.n != g->n) GS_TRY(g->kc_deg.alloc(g->n));
+    g->timer.reset();
+    hipEvent_t t0, t1;
+    GS_HIP(hipEventCreate(&t0));
+    GS_HIP(hipEventCreate(&t1));
+    GS_HIP(hipEventRecord(t0, g->stream));
+    GS_HIP(hipMemcpyAsync(g->kc_deg.ptr, g->degree.ptr, g->n * 4, hipMemcpyDeviceToDevice, g->stream));
+    GS_HIP(hipMemsetAsync(g->rowact.ptr, 0, g->rowact.bytes(), g->stream));
+    GS_HIP(hipMemsetAsync(g->counters.ptr + 2, 0, 8, g->stream));
+    int nr = 0;
+    for (;;) {
+        GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 8, g->stream));
+        hipEvent_t kb;
+        g->timer.begin(GS_K_KCORE_PEEL, g->stream, &kb);
+        k_kc_peel<<<(unsigned)g->rows, 256, 0, g->stream>>>(g->kc_deg.ptr, g->n, k, g->bm_front.ptr, g->rowact.ptr,
+                                                            g->counters.ptr);
+        GS_HIP(hipGetLastError());
+        g->timer.end(GS_K_KCORE_PEEL, g->stream, kb);
+        GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 8, hipMemcpyDeviceToHost, g->stream));
+        GS_HIP(hipStreamSynchronize(g->stream));
+        ++nr;
+        if (g->h_counters[0] == 0 || !g->nchunk) break;
+        if (nr >= 1000000) {
+            set_error("gs_kcore: still removing vertices after %d rounds", nr);
+            return GS_ERR_DEVICE;
+        }
+        g->timer.begin(GS_K_KCORE_SCAN, g->stream, &kb);
+        k_kc_scan<<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), g->bm_front.ptr, g->kc_deg.ptr,
+                                                                   g->rowact.ptr, g->counters.ptr + 2);
+        GS_HIP(hipGetLastError());
+        g->timer.end(GS_K_KCORE_SCAN, g->stream, kb);
+    }
+    GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 16, g->stream));
+    k_kc_count<<<(unsigned)std::min<uint64_t>((g->n + 255) / 256, 8192), 256, 0, g->stream>>>(g->kc_deg.ptr, g->n,
+                                                                                            g->counters.ptr);
+    GS_HIP(hipGetLastError());
+    GS_HIP(hipEventRecord(t1, g->stream));
+    GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 24, hipMemcpyDeviceToHost, g->stream));
+    GS_TRY(sync_and_collect(g));
+    float ms = 0.f;
+    GS_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    g->stats = gs_stats{};
+    g->stats.ms = ms;
+    g->stats.iterations = (uint64_t)nr;
+    g->stats.edges_scanned = g->h_counters[2];
+    if (count_ne0) *count_ne0 = g->h_counters[0];
+    if (count_pos) *count_pos = g->h_counters[1];
+    if (rounds) *rounds = nr;
+    if (deg_out) GS_HIP(hipMemcpy(deg_out, g->kc_deg.ptr, g->n * 4, hipMemcpyDeviceToHost));
+    return GS_OK;
+}

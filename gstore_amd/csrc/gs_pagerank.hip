@@ -1,0 +1,115 @@
+// gs_pagerank.hip -- PageRank over the resident tile grid.
+//
+// Restates pr_t (pr.cpp:82-303) with the resident iteration structure of
+// pagerank_mmap (gstore.cpp:2572-2629):
+//   init      prior = (float)(1.0/N); factor = (int32)(1.0/deg) (compat) or
+//             (float)(1.0/deg) (intended); acc = 0            pr.cpp:82-117
+//   scatter   for every stored edge: acc[v1] += prior[v0], and for the
+//             undirected grid acc[v0] += prior[v1]             pr.cpp:119-154
+//   finalize  prior = (float)((0.15 + 0.85*(double)acc) * factor); acc = 0
+//                                                              pr.cpp:281-302
+// The scatter is one FP32 hardware atomic add per direction (global_atomic_
+// add_f32 on coarse-grained HBM, no CAS loop).  Finalize uses explicit
+// round-to-nearest double ops so the compiler cannot fuse 0.85*x + 0.15.
+#include "gs_internal.h"
+
+namespace gs {
+
+__global__ void k_pr_init(const int32_t *__restrict__ deg, uint64_t n, int mode, float inv_n, float *prior,
+                          float *acc, float *fac) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        int32_t d = deg[v];
+        float f = 0.0f;
+        if (d != 0) {
+            double q = 1.0 / (double)d;
+            f = mode == GS_PR_COMPAT ? (float)(int32_t)q : (float)q;
+        }
+        fac[v] = f;
+        prior[v] = inv_n;
+        acc[v] = 0.0f;
+    }
+}
+
+template <bool DIRECTED>
+__global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter(SweepArgs a, const float *__restrict__ prior, float *acc) {
+    __shared__ ChunkTiles s;
+    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
+    if (chunk >= a.nchunk) return;
+    uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
+    auto op = [&](uint32_t w, uint32_t rc) {
+        uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
+        float p0 = prior[v0];
+        if (!DIRECTED) {
+            float p1 = prior[v1];
+            unsafeAtomicAdd(&acc[v0], p1);
+        }
+        unsafeAtomicAdd(&acc[v1], p0);
+    };
+    sweep_chunk<false>(a, chunk, s, len, op);
+}
+
+__global__ void k_pr_finalize(uint64_t n, float *__restrict__ prior, float *__restrict__ acc,
+                              const float *__restrict__ fac) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        double r = __dadd_rn(0.15, __dmul_rn(0.85, (double)acc[v]));
+        prior[v] = __double2float_rn(__dmul_rn(r, (double)fac[v]));
+        acc[v] = 0.0f;
+    }
+}
+
+static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" int gs_pagerank(gs_graph *g, int iters, int mode, float *rank_out) {
+    if (!g || iters < 0 || (mode != GS_PR_COMPAT && mode != GS_PR_INTENDED)) {
+        set_error("gs_pagerank: bad arguments");
+        return GS_ERR_ARG;
+    }
+    if (g->pr_prior.n != g->n) {
+        GS_TRY(g->pr_prior.alloc(g->n));
+        GS_TRY(g->pr_acc.alloc(g->n));
+        GS_TRY(g->pr_fac.alloc(g->n));
+    }
+    g->timer.reset();
+    hipEvent_t t0, t1;
+    GS_HIP(hipEventCreate(&t0));
+    GS_HIP(hipEventCreate(&t1));
+    GS_HIP(hipEventRecord(t0, g->stream));
+    k_pr_init<<<vgrid(g->n), 256, 0, g->stream>>>(g->degree.ptr, g->n, mode, (float)(1.0 / (double)g->n),
+                                                  g->pr_prior.ptr, g->pr_acc.ptr, g->pr_fac.ptr);
+    GS_HIP(hipGetLastError());
+    for (int it = 0; it < iters; ++it) {
+        hipEvent_t kb;
+        if (g->nchunk) {
+            g->timer.begin(GS_K_PR_SCATTER, g->stream, &kb);
+            if (g->directed)
+                k_pr_scatter<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), g->pr_prior.ptr,
+                                                                                    g->pr_acc.ptr);
+            else
+                k_pr_scatter<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), g->pr_prior.ptr,
+                                                                                     g->pr_acc.ptr);
+            GS_HIP(hipGetLastError());
+            g->timer.end(GS_K_PR_SCATTER, g->stream, kb);
+        }
+        g->timer.begin(GS_K_PR_FINALIZE, g->stream, &kb);
+        k_pr_finalize<<<vgrid(g->n), 256, 0, g->stream>>>(g->n, g->pr_prior.ptr, g->pr_acc.ptr, g->pr_fac.ptr);
+        GS_HIP(hipGetLastError());
+        g->timer.end(GS_K_PR_FINALIZE, g->stream, kb);
+    }
+    GS_HIP(hipEventRecord(t1, g->stream));
+    GS_TRY(sync_and_collect(g));
+    float ms = 0.f;
+    GS_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    g->stats = gs_stats{};
+    g->stats.ms = ms;
+    g->stats.iterations = (uint64_t)iters;
+    g->stats.edges_scanned = g->E * (uint64_t)iters;
+    g->stats.tiles_scanned = g->nz * (uint64_t)iters;
+    if (rank_out) GS_HIP(hipMemcpy(rank_out, g->pr_prior.ptr, g->n * 4, hipMemcpyDeviceToHost));
+    return GS_OK;
+}

@@ -1,0 +1,156 @@
+/*
+ * gstore_amd.h -- C ABI of the MI355X G-Store engine.
+ *
+ * The reference (the-data-lab/gstore) has no library interface: its drop-in
+ * surface is the gstoreu/gstored command line and the .grid/.start/.degree
+ * tile files, with algorithms plugged in through algo_t (algo.h:38-44).  Each
+ * entry point below replaces one reference function or job, cited by
+ * file:line.  Conventions: integer status codes (GS_OK = 0), no exceptions
+ * across the ABI, caller-owned host output buffers (a NULL output pointer
+ * leaves the result on the device), device memory owned by the gs_graph
+ * handle.  One call at a time per handle; handles are independent.
+ */
+#ifndef GSTORE_AMD_H
+#define GSTORE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gs_graph gs_graph;
+
+enum gs_status {
+    GS_OK = 0,
+    GS_ERR_ARG = 1,            /* bad argument */
+    GS_ERR_IO = 2,             /* file missing, short read/write */
+    GS_ERR_FORMAT = 3,         /* tile files inconsistent with each other or with N */
+    GS_ERR_NOMEM = 4,          /* device or host allocation failed */
+    GS_ERR_DEVICE = 5,         /* HIP runtime error */
+    GS_ERR_UNSUPPORTED = 6,    /* e.g. k-core on a directed grid (gstore.cpp:2348-2351) */
+    GS_ERR_DEPTH_OVERFLOW = 7  /* BFS would need depth > 255 (u8 depth, gstore.h:49) */
+};
+
+enum gs_pr_mode {
+    GS_PR_COMPAT = 0,   /* degree factor (int32)(1.0/deg) as pr.cpp:112-114 stores it */
+    GS_PR_INTENDED = 1  /* degree factor (float)(1.0/deg) */
+};
+
+/* Kernel classes whose device time is recorded when kernel timing is on. */
+enum gs_kernel {
+    GS_K_PR_SCATTER = 0, GS_K_PR_FINALIZE, GS_K_BFS_EXPAND, GS_K_BFS_LEVEL_END,
+    GS_K_WCC_HOOK, GS_K_WCC_COMPRESS, GS_K_WCC_CHECK, GS_K_KCORE_PEEL, GS_K_KCORE_SCAN,
+    GS_K_GEN, GS_K_CONVERT, GS_K_COUNT
+};
+
+typedef struct gs_info {
+    uint64_t vert_count;      /* N */
+    uint64_t edge_count;      /* E = stored edges = start[T] */
+    uint64_t tile_slots;      /* T (calc_total_part, gstore.h:121-125) */
+    uint64_t nonempty_tiles;
+    uint64_t groups;          /* p, physical groups per side */
+    uint64_t tiles_per_side;  /* p_s = 256 p */
+    uint64_t device_bytes;    /* device memory held by the handle */
+    int32_t directed;
+    int32_t resident;         /* 1: whole grid in HBM */
+} gs_info;
+
+typedef struct gs_stats {
+    double ms;                     /* device time of the last algorithm call */
+    uint64_t iterations;           /* PR iterations, BFS levels, WCC passes, k-core rounds */
+    uint64_t edges_scanned;        /* stored edges visited by edge sweeps (all iterations) */
+    uint64_t tiles_scanned;
+    double kern_ms[GS_K_COUNT];    /* per-class device time (kernel timing on) */
+    uint64_t kern_calls[GS_K_COUNT];
+} gs_stats;
+
+/* Open a converted graph: PREFIX.grid, PREFIX.start, PREFIX.degree.  Replaces
+ * read_grid_in_mem / read_start_in_mem / read_degree_in_mem
+ * (gstore.cpp:2953-3011, 3799-3813) and the -v/-s geometry
+ * (gstore.cpp:143-161): vert_count is N (the -v value).  The whole grid is
+ * placed in HBM when it fits device_budget_bytes (0 = no limit).  The files
+ * are checked against each other (sizes, start monotone, sentinel, degree
+ * sums, local ids < N). */
+int gs_open(const char *prefix, uint64_t vert_count, int directed,
+            uint64_t device_budget_bytes, gs_graph **out);
+
+/* Same as gs_open from host arrays already in memory (start: T+1 u64,
+ * grid: E u32 words {u16 v0, u16 v1}, degree: N i32). */
+int gs_open_arrays(uint64_t vert_count, int directed, const uint64_t *start,
+                   const uint32_t *grid, const int32_t *degree, gs_graph **out);
+
+/* Generate the Graph500 Kronecker graph of 2^scale vertices and
+ * edge_factor * 2^scale tuples with user seeds (seed1, seed2) on the device
+ * and convert it to the tile grid there.  Replaces make_graph
+ * (graph500-generator/make_graph.c:42-59) followed by the -c 1 converter
+ * proc_grid (gstore.cpp:1329-1458). */
+int gs_open_kron(int scale, int edge_factor, uint64_t seed1, uint64_t seed2, int directed,
+                 gs_graph **out);
+
+/* Generator alone: tuples [e0, e1) into out (u32 pairs), computed on the
+ * device (graph500-generator/graph_generator.c:163-198). */
+int gs_kron_edges(int scale, uint64_t seed1, uint64_t seed2, int64_t e0, int64_t e1,
+                  uint32_t *out);
+
+/* Converter alone (-c 1, gstore.cpp:1245-1458, 2818-2935): tuples (u32
+ * pairs) -> PREFIX.grid/.start/.degree written to disk.  Self-loops are
+ * dropped, duplicates kept, undirected edges stored as (min, max). */
+int gs_convert(const uint32_t *edges, uint64_t tuples, uint64_t vert_count, int directed,
+               const char *out_prefix);
+
+/* Write an open graph's tile files (save_grid, gstore.cpp:2818-2935; the
+ * unused .sdegree/.bdegree are not written). */
+int gs_save(gs_graph *g, const char *prefix);
+
+int gs_get_info(gs_graph *g, gs_info *info);
+
+/* Copy the tile arrays back to host buffers (start: T+1, grid: E, degree: N). */
+int gs_get_arrays(gs_graph *g, uint64_t *start, uint32_t *grid, int32_t *degree);
+
+/* PageRank, `iters` iterations (pr_t, pr.cpp:82-303; resident driver
+ * pagerank_mmap gstore.cpp:2572-2629; -j 1/11/21 -a iters, default 5). */
+int gs_pagerank(gs_graph *g, int iters, int mode, float *rank_out);
+
+/* BFS from root (bfs_t, bfs.cpp:56-338; driver bfs_mmap gstore.cpp:2631-2713;
+ * -j 0/10/20 -a root).  depth_out: u8, 0 = unreached, root = 1.
+ * level_counts[L-1] = vertices reached at level L (up to max_levels).
+ * *levels = number of levels run (including the last, empty one). */
+int gs_bfs(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
+           int max_levels, int *levels);
+
+/* Weakly connected components (wcc2_t, wcc.cpp:57-395; driver wcc_mmap
+ * gstore.cpp:2502-2570; -j 3/13/23).  label_out[v] = smallest vertex id of
+ * v's component for vertices with a stored edge, 0xFFFFFFFF otherwise.
+ * *ncomp = components with at least one stored edge ("WCC count"). */
+int gs_wcc(gs_graph *g, uint32_t *label_out, uint64_t *ncomp);
+
+/* k-core peeling (kcore_t, kcore.cpp:55-300; driver kcore_mmap
+ * gstore.cpp:2716-2804; -j 2/12/22 -a k, default 10).  deg_out = final
+ * degree array (removed vertices <= 0).  *count_ne0 = #(deg != 0), the
+ * reference's printed count; *count_pos = #(deg > 0).  *rounds = removal
+ * rounds run, including the first. */
+int gs_kcore(gs_graph *g, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
+             int *rounds);
+
+/* Stats of the last algorithm call on this handle. */
+int gs_get_stats(gs_graph *g, gs_stats *st);
+
+/* 1: record per-kernel-class device time with HIP events (adds a little
+ * launch overhead); 0: off (default). */
+int gs_set_kernel_timing(gs_graph *g, int on);
+
+/* Name of a gs_kernel class. */
+const char *gs_kernel_name(int k);
+
+void gs_close(gs_graph *g);
+
+/* Message of the last failed call on this thread; returns its length. */
+size_t gs_last_error(char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSTORE_AMD_H */

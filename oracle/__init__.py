@@ -1,0 +1,132 @@
+"""CPU oracle for the G-Store hot path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrappers over oracle/liboracle.so (gs_oracle.c, a C restatement of the
+reference's algorithms; see that file's header for what pins each function).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this package, and only as the checker or the timed CPU baseline; the engine
+(gstore_amd/) never imports it.
+
+``ref_generator()`` loads the reference's own Graph500 generator compiled from
+its source files into oracle/_ref/ (make -C oracle ref), when present.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_GEN = os.path.join(HERE, "_ref", "libref_graph500.so")
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", HERE, "-B", "liboracle.so"], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        u64, i64, p, i = ctypes.c_uint64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int
+        L.or_kron_range.argtypes = [i, u64, u64, i64, i64, p]
+        L.or_convert_count.argtypes = [p, u64, u64, i, p, p]
+        L.or_convert_count.restype = i64
+        L.or_convert_scatter.argtypes = [p, u64, u64, i, p, p]
+        L.or_pagerank.argtypes = [u64, i, p, p, p, i, i, i, p]
+        L.or_bfs.argtypes = [u64, i, p, p, u64, p, p, i]
+        L.or_wcc.argtypes = [u64, i, p, p, p, p]
+        L.or_kcore.argtypes = [u64, p, p, p, i, p, p, p]
+        L.or_total_tiles.argtypes = [u64, i]
+        L.or_total_tiles.restype = u64
+        L.or_tile_slot.argtypes = [u64, u64, u64, i]
+        L.or_tile_slot.restype = u64
+        L.or_groups.argtypes = [u64]
+        L.or_groups.restype = u64
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def kron(scale: int, e0: int, e1: int, seed1: int = 1, seed2: int = 2) -> np.ndarray:
+    out = np.empty((e1 - e0, 2), np.uint32)
+    lib().or_kron_range(scale, seed1, seed2, e0, e1, _p(out))
+    return out
+
+
+def convert(edges: np.ndarray, n: int, directed: bool = False):
+    """-> (start u64[T+1], grid u32[E], degree i32[N]); in-tile order = input order."""
+    edges = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+    L = lib()
+    T = L.or_total_tiles(256 * L.or_groups(n), int(directed))
+    start = np.empty(T + 1, np.uint64)
+    degree = np.empty(n, np.int32)
+    E = L.or_convert_count(_p(edges), edges.shape[0], n, int(directed), _p(start), _p(degree))
+    if E < 0:
+        raise ValueError("edge list holds a vertex id >= N")
+    grid = np.empty(E, np.uint32)
+    L.or_convert_scatter(_p(edges), edges.shape[0], n, int(directed), _p(start), _p(grid))
+    return start, grid, degree
+
+
+def pagerank(n, start, grid, degree, iters=5, mode=0, nthreads=1, directed=False) -> np.ndarray:
+    out = np.empty(n, np.float32)
+    rc = lib().or_pagerank(n, int(directed), _p(start), _p(grid), _p(degree), iters, mode, nthreads, _p(out))
+    if rc:
+        raise RuntimeError("or_pagerank failed")
+    return out
+
+
+def bfs(n, start, grid, root, directed=False):
+    depth = np.empty(n, np.uint8)
+    counts = np.zeros(256, np.uint64)
+    nl = lib().or_bfs(n, int(directed), _p(start), _p(grid), root, _p(depth), _p(counts), 256)
+    if nl < 0:
+        raise RuntimeError(f"or_bfs failed ({nl})")
+    return depth, [int(c) for c in counts[:nl]]
+
+
+def wcc(n, start, grid, directed=False):
+    label = np.empty(n, np.uint32)
+    nc = np.zeros(1, np.uint64)
+    if lib().or_wcc(n, int(directed), _p(start), _p(grid), _p(label), _p(nc)):
+        raise RuntimeError("or_wcc failed")
+    return label, int(nc[0])
+
+
+def kcore(n, start, grid, degree, k):
+    deg = np.empty(n, np.int32)
+    c = np.zeros(2, np.uint64)
+    rounds = lib().or_kcore(n, _p(start), _p(grid), _p(degree), k, _p(deg), _p(c[:1]),
+                            ctypes.c_void_p(c.ctypes.data + 8))
+    if rounds < 0:
+        raise RuntimeError("or_kcore failed")
+    return deg, int(c[0]), int(c[1]), rounds
+
+
+def ref_generator():
+    """The reference's generator library built from source, or None."""
+    if not os.path.exists(REF_GEN):
+        return None
+    L = ctypes.CDLL(REF_GEN)
+    L.make_mrg_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p]
+    L.generate_kronecker_range.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.c_void_p]
+    return L
+
+
+def ref_kron(L, scale: int, e0: int, e1: int, seed1: int = 1, seed2: int = 2) -> np.ndarray:
+    seed = (ctypes.c_uint64 * 5)()   # uint_fast32_t is 64-bit on x86-64 glibc
+    L.make_mrg_seed(seed1, seed2, seed)
+    out = np.empty((e1 - e0, 2), np.uint32)
+    L.generate_kronecker_range(seed, scale, e0, e1, _p(out))
+    return out

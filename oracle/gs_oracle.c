@@ -1,0 +1,564 @@
+/*
+ * gs_oracle.c -- CPU restatement of the G-Store tile-grid hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP engine in
+ * gstore_amd/ is compared against, and the CPU baseline that bench.py times.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it; the engine never links or calls it.
+ *
+ * Every function cites the reference (the-data-lab/gstore) file:line whose
+ * behaviour it restates.  Nothing here is copied: the loops are written from
+ * the described semantics, and the Kronecker generator is restated from the
+ * Graph500 specification (MRG32k5 of L'Ecuyer-Blouin-Couture 1993, table 3,
+ * m = 2^31-1, k = 5) with its skip-ahead matrices recomputed by exponentiation.
+ *
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *   or_kron_range      -- reference generator built from source (oracle/_ref)
+ *                         and the edge listing in graph500-generator/README.md
+ *   or_convert_*       -- invariants (multiset of (min,max) non-self-loop edges,
+ *                         degree = endpoint count, start monotone, sentinel)
+ *   or_pagerank compat -- closed form (only degree-1 vertices are non-zero)
+ *   or_pagerank intd.  -- float64 scipy recurrence in tests (tolerance stated)
+ *   or_bfs             -- scipy.sparse.csgraph breadth-first distances
+ *   or_wcc             -- scipy.sparse.csgraph.connected_components
+ *   or_kcore           -- independent numpy peeling in tests + brute force
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Tile addressing (gstore.h:63-141, HALF_GRID branch; full grid :176-198)   */
+/* ------------------------------------------------------------------------ */
+
+/* Upper-triangle row-major index of (i, j), i <= j, in an n x n triangle
+ * (gstore.h:109-113). */
+static uint64_t tri_index(uint64_t i, uint64_t j, uint64_t n)
+{
+    return ((i * (2 * n - i - 1)) >> 1) + j;
+}
+
+/* Number of tile slots for p_s tiles per side (gstore.h:121-125). */
+uint64_t or_total_tiles(uint64_t p_s, int directed)
+{
+    return directed ? p_s * p_s : (p_s * (p_s + 1)) >> 1;
+}
+
+/* Physical groups per side: ceil(N / 2^24) (gstore.cpp:152-157). */
+uint64_t or_groups(uint64_t n)
+{
+    return (n >> 24) + ((n & 0xFFFFFFull) != 0);
+}
+
+/* Index in .start of tile (r, c), where r, c are global tile coordinates
+ * (tile = 2^16 vertices, group = 256 tiles).  Undirected: r <= c, groups in
+ * upper-triangle order, a diagonal group holds its 32896 upper-triangle tiles,
+ * an off-diagonal group 65536 tiles (gstore.h:128-141, matrix::get :365-369,
+ * matrix_f::get :494-497).  Directed: groups p x p row-major, tiles row-major
+ * (gstore.h:176-198). */
+uint64_t or_tile_slot(uint64_t r, uint64_t c, uint64_t p, int directed)
+{
+    uint64_t bi = r >> 8, si = r & 255, bj = c >> 8, sj = c & 255;
+    if (directed)
+        return ((bi * p + bj) << 16) + si * 256 + sj;
+    if (bi == bj) {
+        uint64_t base = ((tri_index(bi, bi, p) - bi) << 16) + ((bi * 257) << 7);
+        return base + tri_index(si, sj, 256);
+    }
+    uint64_t base = ((tri_index(bi, bj, p) - bi - 1) << 16) + (((bi + 1) * 257) << 7);
+    return base + si * 256 + sj;
+}
+
+/* Inverse of or_tile_slot over a whole grid: fills row[t], col[t] for every
+ * slot t in .start order.  Used by the tile walks below. */
+void or_slot_coords(uint64_t p, int directed, uint32_t *row, uint32_t *col)
+{
+    uint64_t t = 0;
+    for (uint64_t bi = 0; bi < p; ++bi)
+        for (uint64_t bj = directed ? 0 : bi; bj < p; ++bj)
+            for (uint64_t si = 0; si < 256; ++si)
+                for (uint64_t sj = (!directed && bi == bj) ? si : 0; sj < 256; ++sj) {
+                    row[t] = (uint32_t)(bi * 256 + si);
+                    col[t] = (uint32_t)(bj * 256 + sj);
+                    ++t;
+                }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Graph500 Kronecker generator (graph500-generator/graph_generator.c:41-198, */
+/* splittable_mrg.c:116-285, utils.c:51-57)                                 */
+/* ------------------------------------------------------------------------ */
+
+#define MRG_P 0x7FFFFFFFull
+#define MRG_X 107374182ull
+#define MRG_Y 104480ull
+
+static uint64_t mrg_mod(uint64_t v)
+{
+    v = (v & MRG_P) + (v >> 31);
+    v = (v & MRG_P) + (v >> 31);
+    return v >= MRG_P ? v - MRG_P : v;
+}
+
+typedef struct { uint64_t m[5][5]; } mat5;
+
+static void mat5_mul(const mat5 *a, const mat5 *b, mat5 *out)
+{
+    mat5 r;
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) {
+            uint64_t s = 0;
+            for (int k = 0; k < 5; ++k) s = mrg_mod(s + mrg_mod(a->m[i][k] * b->m[k][j]));
+            r.m[i][j] = s;
+        }
+    *out = r;
+}
+
+static void mat5_identity(mat5 *a)
+{
+    memset(a, 0, sizeof *a);
+    for (int i = 0; i < 5; ++i) a->m[i][i] = 1;
+}
+
+/* One MRG step on the column state (z1..z5): z1' = x*z1 + y*z5, shift down. */
+static void mat5_step(mat5 *a)
+{
+    memset(a, 0, sizeof *a);
+    a->m[0][0] = MRG_X;
+    a->m[0][4] = MRG_Y;
+    for (int i = 1; i < 5; ++i) a->m[i][i - 1] = 1;
+}
+
+static void mat5_pow2k(const mat5 *a, int k, mat5 *out) /* a^(2^k) */
+{
+    mat5 r = *a;
+    for (int i = 0; i < k; ++i) mat5_mul(&r, &r, &r);
+    *out = r;
+}
+
+static void mat5_pow(const mat5 *a, uint64_t e, mat5 *out)
+{
+    mat5 r, b = *a;
+    mat5_identity(&r);
+    while (e) {
+        if (e & 1) mat5_mul(&r, &b, &r);
+        mat5_mul(&b, &b, &b);
+        e >>= 1;
+    }
+    *out = r;
+}
+
+static void mat5_apply(const mat5 *a, uint64_t z[5])
+{
+    uint64_t o[5];
+    for (int i = 0; i < 5; ++i) {
+        uint64_t s = 0;
+        for (int k = 0; k < 5; ++k) s = mrg_mod(s + mrg_mod(a->m[i][k] * z[k]));
+        o[i] = s;
+    }
+    memcpy(z, o, sizeof o);
+}
+
+static uint32_t mrg_next(uint64_t z[5])
+{
+    uint64_t nz = mrg_mod(mrg_mod(MRG_X * z[0]) + mrg_mod(MRG_Y * z[4]));
+    z[4] = z[3]; z[3] = z[2]; z[2] = z[1]; z[1] = z[0]; z[0] = nz;
+    return (uint32_t)nz;
+}
+
+static uint64_t bitrev64(uint64_t x)
+{
+    uint64_t r = 0;
+    for (int i = 0; i < 64; ++i) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+/* Vertex scramble (graph_generator.c:120-130). */
+static uint64_t kron_scramble(uint64_t v, int lg, uint64_t k0, uint64_t k1)
+{
+    v += k0 + k1;
+    v *= (k0 | 0x4519840211493211ull);
+    v = bitrev64(v) >> (64 - lg);
+    v *= (k1 | 0x3050852102C843A5ull);
+    v = bitrev64(v) >> (64 - lg);
+    return v;
+}
+
+/* Generate tuples [e0, e1) of the scale-lg Kronecker graph for user seeds
+ * (s1, s2) into out[2*(e-e0)+{0,1}] (u32 pairs, the gedge_t layout of
+ * gstore.h:254-279).  Tuple e starts from the seed state advanced by e*2^64
+ * steps (graph_generator.c:189-197); each of the lg levels draws one
+ * quadrant with A=.57, B=C=.19, D=.05 by rejection of values below
+ * 2^32-1 mod 10000 (graph_generator.c:41-68), clip-and-flip on the diagonal
+ * (:141-148), then both endpoints are scrambled (:155-156). */
+void or_kron_range(int lg, uint64_t s1, uint64_t s2, int64_t e0, int64_t e1, uint32_t *out)
+{
+    uint64_t seed[5] = {
+        (s1 & 0x3FFFFFFF) + 1, ((s1 >> 30) & 0x3FFFFFFF) + 1,
+        (s2 & 0x3FFFFFFF) + 1, ((s2 >> 30) & 0x3FFFFFFF) + 1,
+        ((s2 >> 60) << 4) + (s1 >> 60) + 1 };
+    mat5 a1, a64, a128, t;
+    mat5_step(&a1);
+    mat5_pow2k(&a1, 64, &a64);
+    mat5_pow2k(&a64, 64, &a128);
+    /* scramble keys: state advanced by 50*2^128 + 7*2^64 steps */
+    uint64_t z[5];
+    memcpy(z, seed, sizeof z);
+    mat5_pow(&a64, 7, &t);   mat5_apply(&t, z);
+    mat5_pow(&a128, 50, &t); mat5_apply(&t, z);
+    uint64_t k0 = mrg_next(z); k0 *= 0xFFFFFFFFull; k0 += mrg_next(z);
+    uint64_t k1 = mrg_next(z); k1 *= 0xFFFFFFFFull; k1 += mrg_next(z);
+
+    /* byte tables: bt[b][v] = a64^(v * 256^b) */
+    mat5 *bt = (mat5 *)malloc(sizeof(mat5) * 8 * 256);
+    mat5 base = a64;
+    for (int b = 0; b < 8; ++b) {
+        mat5_identity(&bt[b * 256]);
+        for (int v = 1; v < 256; ++v) mat5_mul(&bt[b * 256 + v - 1], &base, &bt[b * 256 + v]);
+        mat5_pow(&base, 256, &base);
+    }
+    const uint32_t limit = 0xFFFFFFFFu % 10000u;
+    #pragma omp parallel for schedule(static)
+    for (int64_t e = e0; e < e1; ++e) {
+        uint64_t st[5];
+        memcpy(st, seed, sizeof st);
+        uint64_t ee = (uint64_t)e;
+        for (int b = 0; ee; ++b, ee >>= 8)
+            if (ee & 255) mat5_apply(&bt[b * 256 + (ee & 255)], st);
+        uint64_t src = 0, dst = 0, half = 1ull << lg;
+        for (int l = 0; l < lg; ++l) {
+            uint32_t v = mrg_next(st);
+            while (v < limit) v = mrg_next(st);
+            v %= 10000u;
+            int q;
+            if (v < 1900) q = 1;
+            else if (v < 3800) q = 2;
+            else if (v < 9500) q = 0;
+            else q = 3;
+            int so = q >> 1, to = q & 1;
+            if (src == dst && so > to) { so = 0; to = 1; }
+            half >>= 1;
+            src += half * so;
+            dst += half * to;
+        }
+        out[2 * (e - e0)]     = (uint32_t)kron_scramble(src, lg, k0, k1);
+        out[2 * (e - e0) + 1] = (uint32_t)kron_scramble(dst, lg, k0, k1);
+    }
+    free(bt);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Converter -c 1 (gstore.cpp:1245-1458): drop self-loops, store (min,max), */
+/* per-tile count -> prefix -> scatter, degree += 1 on both endpoints.       */
+/* The reference's in-tile order depends on thread timing; this one keeps    */
+/* input order inside a tile.                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Pass 1: fills start[0..T] (prefix offsets, start[T] = E) and degree[0..N).
+ * Returns E, or -1 if a vertex id is >= n. */
+int64_t or_convert_count(const uint32_t *edges, uint64_t m, uint64_t n, int directed,
+                         uint64_t *start, int32_t *degree)
+{
+    uint64_t p = or_groups(n), T = or_total_tiles(256 * p, directed);
+    memset(start, 0, sizeof(uint64_t) * (T + 1));
+    memset(degree, 0, sizeof(int32_t) * n);
+    for (uint64_t k = 0; k < m; ++k) {
+        uint64_t a = edges[2 * k], b = edges[2 * k + 1];
+        if (a == b) continue;
+        if (a >= n || b >= n) return -1;
+        uint64_t v0 = a, v1 = b;
+        if (!directed && v0 > v1) { v0 = b; v1 = a; }
+        start[or_tile_slot(v0 >> 16, v1 >> 16, p, directed)]++;
+        degree[a]++;
+        if (!directed) degree[b]++;
+    }
+    uint64_t s = 0;
+    for (uint64_t t = 0; t <= T; ++t) { uint64_t c = start[t]; start[t] = s; s += c; }
+    return (int64_t)s;
+}
+
+/* Pass 2: scatter into grid[0..E) as u32 words (v0 & 0xFFFF) | (v1 << 16)
+ * (edge_t {u16 v0, v1}, gstore.h:311-334). */
+void or_convert_scatter(const uint32_t *edges, uint64_t m, uint64_t n, int directed,
+                        const uint64_t *start, uint32_t *grid)
+{
+    uint64_t p = or_groups(n), T = or_total_tiles(256 * p, directed);
+    uint64_t *cur = (uint64_t *)malloc(sizeof(uint64_t) * T);
+    memcpy(cur, start, sizeof(uint64_t) * T);
+    for (uint64_t k = 0; k < m; ++k) {
+        uint64_t a = edges[2 * k], b = edges[2 * k + 1];
+        if (a == b) continue;
+        uint64_t v0 = a, v1 = b;
+        if (!directed && v0 > v1) { v0 = b; v1 = a; }
+        uint64_t t = or_tile_slot(v0 >> 16, v1 >> 16, p, directed);
+        grid[cur[t]++] = (uint32_t)(v0 & 0xFFFF) | ((uint32_t)(v1 & 0xFFFF) << 16);
+    }
+    free(cur);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Common tile walk: every tile slot in .start order with its coordinates.   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint64_t n, p, T;
+    int directed;
+    const uint64_t *start;
+    const uint32_t *grid;
+    uint32_t *row, *col;
+} walk_t;
+
+static int walk_init(walk_t *w, uint64_t n, int directed, const uint64_t *start, const uint32_t *grid)
+{
+    w->n = n;
+    w->directed = directed;
+    w->p = or_groups(n);
+    w->T = or_total_tiles(256 * w->p, directed);
+    w->start = start;
+    w->grid = grid;
+    w->row = (uint32_t *)malloc(sizeof(uint32_t) * w->T);
+    w->col = (uint32_t *)malloc(sizeof(uint32_t) * w->T);
+    if (!w->row || !w->col) return -1;
+    or_slot_coords(w->p, directed, w->row, w->col);
+    return 0;
+}
+
+static void walk_free(walk_t *w)
+{
+    free(w->row);
+    free(w->col);
+}
+
+static inline float atomic_add_f32(float *a, float v)
+{
+    /* CAS loop, like qthread_dincr (pr.cpp:60-80) */
+    union { float f; uint32_t u; } o, nw;
+    o.u = __atomic_load_n((uint32_t *)a, __ATOMIC_RELAXED);
+    do { nw.f = o.f + v; } while (!__atomic_compare_exchange_n((uint32_t *)a, &o.u, nw.u, 1,
+                                                               __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+    return o.f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* PageRank (pr.cpp:82-154, 229-303; driver gstore.cpp:2572-2629)           */
+/* ------------------------------------------------------------------------ */
+
+/* mode 0 (compat): factor = (int32)(1.0/deg) as the reference stores it in
+ * its int32 degree array (pr.cpp:112-114) -> 1 for degree 1, else 0.
+ * mode 1 (intended): factor = (float)(1.0/deg).
+ * prior0 = (float)(1.0/N) (pr.cpp:110); each iteration scatters
+ * acc[v1] += prior[v0], acc[v0] += prior[v1] for every stored edge
+ * (pr.cpp:141-152), then prior = (float)((0.15 + 0.85*(double)acc) * factor)
+ * (pr.cpp:286-290) with acc cleared.  Output is prior after `iters`
+ * iterations.  nthreads == 1 gives a fixed summation order (tile order,
+ * edge order); nthreads > 1 sweeps tile rows in parallel with CAS adds like
+ * the reference.  Build with -ffp-contract=off so 0.15 + 0.85*x is not fused.
+ * directed != 0: full grid, acc[v1] += prior[v0] only (pr.cpp without
+ * HALF_GRID). */
+int or_pagerank(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid,
+                const int32_t *degree, int iters, int mode, int nthreads, float *out)
+{
+    walk_t w;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    float *prior = out, *acc = (float *)calloc(n, sizeof(float));
+    float *fac = (float *)malloc(sizeof(float) * n);
+    double inv_n = 1.0 / (double)n;
+    for (uint64_t v = 0; v < n; ++v) {
+        prior[v] = (float)inv_n;
+        int32_t d = degree[v];
+        if (d == 0) fac[v] = 0.0f;
+        else if (mode == 0) fac[v] = (float)(int32_t)(1.0 / (double)d);
+        else fac[v] = (float)(1.0 / (double)d);
+    }
+    for (int it = 0; it < iters; ++it) {
+        if (nthreads <= 1) {
+            for (uint64_t t = 0; t < w.T; ++t) {
+                uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+                for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                    uint64_t v0 = o0 + (grid[k] & 0xFFFF), v1 = o1 + (grid[k] >> 16);
+                    acc[v1] += prior[v0];
+                    if (!directed) acc[v0] += prior[v1];
+                }
+            }
+        } else {
+            #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+            for (uint64_t t = 0; t < w.T; ++t) {
+                uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+                for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                    uint64_t v0 = o0 + (grid[k] & 0xFFFF), v1 = o1 + (grid[k] >> 16);
+                    atomic_add_f32(&acc[v1], prior[v0]);
+                    if (!directed) atomic_add_f32(&acc[v0], prior[v1]);
+                }
+            }
+        }
+        #pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+        for (uint64_t v = 0; v < n; ++v) {
+            double r = 0.15 + 0.85 * (double)acc[v];
+            prior[v] = (float)(r * (double)fac[v]);
+            acc[v] = 0.0f;
+        }
+    }
+    free(acc);
+    free(fac);
+    walk_free(&w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BFS (bfs.cpp:56-224; resident driver gstore.cpp:2631-2713)                */
+/* ------------------------------------------------------------------------ */
+
+/* depth is u8, 0 = unreached, root = 1 (bfs.cpp:54, 70-71).  Level L visits
+ * every stored edge once: if d[v0] == L and d[v1] == 0 then d[v1] = L+1,
+ * else if d[v1] == L and d[v0] == 0 then d[v0] = L+1 (bfs.cpp:158-177).  The
+ * run stops at the first level that reaches nothing (bfs.cpp:196-214).  The
+ * reference skips tiles whose row and column hold no frontier vertex
+ * (gstore.cpp:2687-2698); such tiles cannot produce an update, so visiting
+ * all tiles gives the same depths.  level_counts[L-1] receives the exact
+ * number of vertices reached at level L (the reference's printed count can
+ * over-count under races).  Returns the number of levels run, or -2 when a
+ * level reaches vertices at depth 256, which the u8 depth of the reference
+ * cannot hold.  directed != 0:
+ * full grid, v0 -> v1 only (bfs.cpp without HALF_GRID). */
+int or_bfs(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, uint64_t root,
+           uint8_t *depth, uint64_t *level_counts, int max_levels)
+{
+    walk_t w;
+    if (root >= n) return -1;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    memset(depth, 0, n);
+    depth[root] = 1;
+    int level = 1, nlev = 0;
+    for (;;) {
+        uint64_t cnt = 0;
+        for (uint64_t t = 0; t < w.T; ++t) {
+            uint8_t *d0 = depth + ((uint64_t)w.row[t] << 16);
+            uint8_t *d1 = depth + ((uint64_t)w.col[t] << 16);
+            for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                uint32_t a = grid[k] & 0xFFFF, b = grid[k] >> 16;
+                if (d0[a] == level && d1[b] == 0) { d1[b] = (uint8_t)(level + 1); ++cnt; }
+                else if (!directed && d1[b] == level && d0[a] == 0) { d0[a] = (uint8_t)(level + 1); ++cnt; }
+            }
+        }
+        if (nlev < max_levels && level_counts) level_counts[nlev] = cnt;
+        ++nlev;
+        if (cnt == 0) break;
+        if (level + 1 > 255) { walk_free(&w); return -2; }  /* reached vertices got depth 256 */
+        ++level;
+    }
+    walk_free(&w);
+    return nlev;
+}
+
+/* ------------------------------------------------------------------------ */
+/* WCC (wcc.cpp:57-395; resident driver gstore.cpp:2502-2570)                */
+/* ------------------------------------------------------------------------ */
+
+static uint32_t uf_find(uint32_t *par, uint32_t x)
+{
+    while (par[x] != x) { par[x] = par[par[x]]; x = par[x]; }
+    return x;
+}
+
+/* The reference labels components with atomic-counter group ids
+ * (wcc.cpp:204) and prints the number of distinct ids, which counts only
+ * components with at least one stored edge (wcc.cpp:324-337).  Its numbering
+ * depends on thread timing, so the canonical form here is: label[v] = the
+ * minimum vertex id of v's component for every vertex with a stored edge,
+ * 0xFFFFFFFF (the reference's invalid_cid, wcc.cpp:54) otherwise.
+ * *ncomp = number of components with at least one stored edge.  A directed
+ * grid gives its weakly connected components. */
+int or_wcc(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, uint32_t *label,
+           uint64_t *ncomp)
+{
+    walk_t w;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    uint32_t *par = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint8_t *touched = (uint8_t *)calloc(n, 1);
+    for (uint64_t v = 0; v < n; ++v) par[v] = (uint32_t)v;
+    for (uint64_t t = 0; t < w.T; ++t) {
+        uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+        for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+            uint32_t a = (uint32_t)(o0 + (grid[k] & 0xFFFF)), b = (uint32_t)(o1 + (grid[k] >> 16));
+            touched[a] = touched[b] = 1;
+            uint32_t ra = uf_find(par, a), rb = uf_find(par, b);
+            if (ra < rb) par[rb] = ra;
+            else if (rb < ra) par[ra] = rb;
+        }
+    }
+    uint64_t c = 0;
+    for (uint64_t v = 0; v < n; ++v) {
+        if (!touched[v]) { label[v] = 0xFFFFFFFFu; continue; }
+        label[v] = uf_find(par, (uint32_t)v);
+        c += (label[v] == v);
+    }
+    *ncomp = c;
+    free(par);
+    free(touched);
+    walk_free(&w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* k-core (kcore.cpp:55-217; drivers gstore.cpp:2346-2500, 2716-2804)        */
+/* ------------------------------------------------------------------------ */
+
+/* deg starts as the .degree array (multigraph degree without self-loops).
+ * Each round tests m = (int16)deg[v] (sdegree_t, kcore.cpp:189) and removes
+ * v when 0 < m < k by setting deg[v] = 0 and marking it active
+ * (kcore.cpp:190-195).  The first round runs before any edge pass
+ * (gstore.cpp:2355-2357).  An edge pass then applies, for every stored edge,
+ * if active(v0) --deg[v1]; if active(v1) --deg[v0] (kcore.cpp:69-103; removed
+ * vertices keep being decremented).  Tiles with no active endpoint are
+ * skipped by the reference (gstore.cpp:2768-2781), which changes nothing.
+ * Stops at the first round that removes nothing.  *count_ne0 = number of
+ * vertices with deg != 0, the reference's printed ">= k vertices count"
+ * (gstore.cpp:2794-2800); *count_pos = number with deg > 0 (the true core).
+ * Returns the number of removal rounds (including the first), or -1. */
+int or_kcore(uint64_t n, const uint64_t *start, const uint32_t *grid, const int32_t *degree,
+             int k, int32_t *deg, uint64_t *count_ne0, uint64_t *count_pos)
+{
+    walk_t w;
+    if (walk_init(&w, n, 0, start, grid)) return -1;
+    uint8_t *act = (uint8_t *)calloc(n, 1);
+    memcpy(deg, degree, sizeof(int32_t) * n);
+    int rounds = 0;
+    for (;;) {
+        uint64_t f = 0;
+        for (uint64_t v = 0; v < n; ++v) {
+            int16_t m = (int16_t)deg[v];
+            act[v] = 0;
+            if (m < k && m > 0) { deg[v] = 0; act[v] = 1; ++f; }
+        }
+        ++rounds;
+        if (f == 0) break;
+        for (uint64_t t = 0; t < w.T; ++t) {
+            uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+            for (uint64_t e = w.start[t]; e < w.start[t + 1]; ++e) {
+                uint64_t a = o0 + (grid[e] & 0xFFFF), b = o1 + (grid[e] >> 16);
+                if (act[a]) deg[b]--;
+                if (act[b]) deg[a]--;
+            }
+        }
+    }
+    uint64_t ne0 = 0, pos = 0;
+    for (uint64_t v = 0; v < n; ++v) { ne0 += (deg[v] != 0); pos += (deg[v] > 0); }
+    *count_ne0 = ne0;
+    *count_pos = pos;
+    free(act);
+    walk_free(&w);
+    return rounds;
+}
+
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
