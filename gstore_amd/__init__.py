@@ -72,7 +72,9 @@ _SIGS = {
     "gs_get_info": (ctypes.c_int, [_P, ctypes.POINTER(_Info)]),
     "gs_get_arrays": (ctypes.c_int, [_P, _P, _P, _P]),
     "gs_pagerank": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P]),
+    "gs_ppr": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_int, _P]),
     "gs_bfs": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "gs_bfs_batch": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "gs_wcc": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64)]),
     "gs_kcore": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_uint64),
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
@@ -214,6 +216,12 @@ class Graph:
         _check(_lib.gs_pagerank(self._h, iters, mode, _ptr(out) if copy_out else None))
         return out
 
+    def ppr(self, source: int, iters: int = 20, copy_out: bool = True) -> Optional[np.ndarray]:
+        """Personalised PageRank from `source` (FP32 ranks, damping 0.85)."""
+        out = np.empty(self.n, np.float32) if copy_out else None
+        _check(_lib.gs_ppr(self._h, source, iters, _ptr(out) if copy_out else None))
+        return out
+
     def bfs(self, root: int = 0, copy_out: bool = True, max_levels: int = 256) -> BfsResult:
         depth = np.empty(self.n, np.uint8) if copy_out else None
         counts = np.zeros(max_levels, np.uint64)
@@ -221,6 +229,17 @@ class Graph:
         _check(_lib.gs_bfs(self._h, root, _ptr(depth) if copy_out else None, _ptr(counts), max_levels,
                            ctypes.byref(nlev)))
         return BfsResult(depth, [int(c) for c in counts[:min(nlev.value, max_levels)]])
+
+    def bfs_batch(self, roots, copy_out: bool = True, max_levels: int = 256):
+        """Up to 64 roots in one multi-source sweep -> (depth[n_roots, N] u8, level_counts per root)."""
+        r = np.ascontiguousarray(roots, dtype=np.uint64)
+        depth = np.empty((r.size, self.n), np.uint8) if copy_out else None
+        counts = np.zeros((r.size, max_levels), np.uint64)
+        nlev = ctypes.c_int(0)
+        _check(_lib.gs_bfs_batch(self._h, _ptr(r), r.size, _ptr(depth) if copy_out else None, _ptr(counts),
+                                 max_levels, ctypes.byref(nlev)))
+        lv = min(nlev.value, max_levels)
+        return depth, [[int(c) for c in counts[i, :lv]] for i in range(r.size)]
 
     def wcc(self, copy_out: bool = True):
         labels = np.empty(self.n, np.uint32) if copy_out else None

@@ -6,14 +6,14 @@ gstoreu / gstored command-line tools (C++ host over the C ABI).
 from __future__ import annotations
 
 import os
-import shutil
 import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["gs_graph.hip", "gs_pagerank.hip", "gs_bfs.hip", "gs_wcc.hip", "gs_kcore.hip"]
+HIP_SOURCES = ["gs_graph.hip", "gs_pagerank.hip", "gs_bfs.hip", "gs_bfs_batch.hip", "gs_wcc.hip", "gs_kcore.hip",
+               "gs_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

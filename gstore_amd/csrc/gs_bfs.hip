@@ -93,55 +93,55 @@ __global__ void __launch_bounds__(256) k_bfs_level_end(uint32_t *vis, uint32_t *
 
 using namespace gs;
 
-extern "C" int gs_bfs(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts, int max_levels,
-                      int *levels) {
+int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
+                 int max_levels, int *levels) {
     if (!g || root >= g->n || max_levels < 0) {
         set_error("gs_bfs: bad arguments (root %llu, N %llu)", (unsigned long long)root,
                   g ? (unsigned long long)g->n : 0ull);
         return GS_ERR_ARG;
     }
-    GS_TRY(ensure_bitmaps(g));
-    if (g->depth.n != g->n) GS_TRY(g->depth.alloc(g->n));
+    GS_TRY(ensure_bitmaps(g, w));
+    if (w->depth.n != g->n) GS_TRY(w->depth.alloc(g->n));
     const uint64_t words = bitmap_words(g);
-    g->timer.reset();
+    w->timer.reset();
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
-    GS_HIP(hipEventRecord(t0, g->stream));
-    GS_HIP(hipMemsetAsync(g->depth.ptr, 0, g->n, g->stream));
-    GS_HIP(hipMemsetAsync(g->bm_vis.ptr, 0, words * 4, g->stream));
-    GS_HIP(hipMemsetAsync(g->bm_front.ptr, 0, words * 4, g->stream));
-    GS_HIP(hipMemsetAsync(g->bm_next.ptr, 0, words * 4, g->stream));
-    GS_HIP(hipMemsetAsync(g->rowact.ptr, 0, g->rowact.bytes(), g->stream));
-    GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 16, g->stream));
-    k_bfs_root<<<1, 1, 0, g->stream>>>((uint32_t)root, g->depth.ptr, g->bm_vis.ptr, g->bm_front.ptr, g->rowact.ptr);
+    GS_HIP(hipEventRecord(t0, w->stream));
+    GS_HIP(hipMemsetAsync(w->depth.ptr, 0, g->n, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_vis.ptr, 0, words * 4, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_front.ptr, 0, words * 4, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_next.ptr, 0, words * 4, w->stream));
+    GS_HIP(hipMemsetAsync(w->rowact.ptr, 0, w->rowact.bytes(), w->stream));
+    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 16, w->stream));
+    k_bfs_root<<<1, 1, 0, w->stream>>>((uint32_t)root, w->depth.ptr, w->bm_vis.ptr, w->bm_front.ptr, w->rowact.ptr);
     GS_HIP(hipGetLastError());
     int nlev = 0, status = GS_OK;
     for (uint32_t level = 1;; ++level) {
-        GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 8, g->stream));
+        GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
         hipEvent_t kb;
         if (g->nchunk) {
-            g->timer.begin(GS_K_BFS_EXPAND, g->stream, &kb);
+            w->timer.begin(GS_K_BFS_EXPAND, w->stream, &kb);
             if (g->directed)
-                k_bfs_expand<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(
-                    sweep_args(g), g->bm_front.ptr, g->bm_vis.ptr, g->bm_next.ptr, g->rowact.ptr,
-                    g->counters.ptr + 1);
+                k_bfs_expand<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(
+                    sweep_args(g), w->bm_front.ptr, w->bm_vis.ptr, w->bm_next.ptr, w->rowact.ptr,
+                    w->counters.ptr + 1);
             else
-                k_bfs_expand<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(
-                    sweep_args(g), g->bm_front.ptr, g->bm_vis.ptr, g->bm_next.ptr, g->rowact.ptr,
-                    g->counters.ptr + 1);
+                k_bfs_expand<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(
+                    sweep_args(g), w->bm_front.ptr, w->bm_vis.ptr, w->bm_next.ptr, w->rowact.ptr,
+                    w->counters.ptr + 1);
             GS_HIP(hipGetLastError());
-            g->timer.end(GS_K_BFS_EXPAND, g->stream, kb);
+            w->timer.end(GS_K_BFS_EXPAND, w->stream, kb);
         }
-        g->timer.begin(GS_K_BFS_LEVEL_END, g->stream, &kb);
-        k_bfs_level_end<<<(unsigned)g->rows, 256, 0, g->stream>>>(g->bm_vis.ptr, g->bm_front.ptr, g->bm_next.ptr,
-                                                                   g->rowact.ptr, g->depth.ptr, level + 1,
-                                                                   g->counters.ptr);
+        w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
+        k_bfs_level_end<<<(unsigned)g->rows, 256, 0, w->stream>>>(w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
+                                                                   w->rowact.ptr, w->depth.ptr, level + 1,
+                                                                   w->counters.ptr);
         GS_HIP(hipGetLastError());
-        g->timer.end(GS_K_BFS_LEVEL_END, g->stream, kb);
-        GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 16, hipMemcpyDeviceToHost, g->stream));
-        GS_HIP(hipStreamSynchronize(g->stream));
-        uint64_t cnt = g->h_counters[0];
+        w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
+        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 16, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        uint64_t cnt = w->h_counters[0];
         if (level_counts && nlev < max_levels) level_counts[nlev] = cnt;
         ++nlev;
         if (cnt == 0) break;
@@ -151,18 +151,27 @@ extern "C" int gs_bfs(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *
             break;
         }
     }
-    GS_HIP(hipEventRecord(t1, g->stream));
-    GS_TRY(sync_and_collect(g));
+    GS_HIP(hipEventRecord(t1, w->stream));
+    GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    g->stats = gs_stats{};
-    g->stats.ms = ms;
-    g->stats.iterations = (uint64_t)nlev;
-    g->stats.edges_scanned = g->h_counters[1];
+    w->stats = gs_stats{};
+    w->stats.ms = ms;
+    w->stats.iterations = (uint64_t)nlev;
+    w->stats.edges_scanned = w->h_counters[1];
     if (levels) *levels = nlev;
     if (status != GS_OK) return status;
-    if (depth_out) GS_HIP(hipMemcpy(depth_out, g->depth.ptr, g->n, hipMemcpyDeviceToHost));
+    if (depth_out) {
+        GS_HIP(hipMemcpyAsync(depth_out, w->depth.ptr, g->n, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+    }
     return GS_OK;
+}
+
+extern "C" int gs_bfs(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts, int max_levels,
+                      int *levels) {
+    if (!g) { set_error("gs_bfs: null graph"); return GS_ERR_ARG; }
+    return run_bfs(g, g->ws, root, depth_out, level_counts, max_levels, levels);
 }

@@ -55,22 +55,31 @@ void KernelTimer::reset() {
     for (int i = 0; i < GS_K_COUNT; ++i) { ms[i] = 0; calls[i] = 0; }
 }
 
-int sync_and_collect(gs_graph *g) {
-    GS_HIP(hipStreamSynchronize(g->stream));
-    return g->timer.collect();
+int sync_and_collect(gs_ws *w) {
+    GS_HIP(hipStreamSynchronize(w->stream));
+    return w->timer.collect();
 }
 
 uint64_t bitmap_words(const gs_graph *g) { return g->rows * ROW_WORDS; }
 
-int ensure_bitmaps(gs_graph *g) {
-    uint64_t w = bitmap_words(g);
-    if (g->bm_vis.n != w) {
-        GS_TRY(g->bm_vis.alloc(w));
-        GS_TRY(g->bm_front.alloc(w));
-        GS_TRY(g->bm_next.alloc(w));
+int ensure_bitmaps(const gs_graph *g, gs_ws *w) {
+    uint64_t wd = bitmap_words(g);
+    if (w->bm_vis.n != wd) {
+        GS_TRY(w->bm_vis.alloc(wd));
+        GS_TRY(w->bm_front.alloc(wd));
+        GS_TRY(w->bm_next.alloc(wd));
     }
     uint64_t rw = (g->p_s + 31) / 32;
-    if (g->rowact.n != rw) GS_TRY(g->rowact.alloc(rw));
+    if (w->rowact.n != rw) GS_TRY(w->rowact.alloc(rw));
+    return GS_OK;
+}
+
+int ws_create(gs_ws **out) {
+    std::unique_ptr<gs_ws> w(new gs_ws());
+    GS_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    GS_TRY(w->counters.alloc(16));
+    GS_HIP(hipHostMalloc((void **)&w->h_counters, 16 * sizeof(unsigned long long), hipHostMallocDefault));
+    *out = w.release();
     return GS_OK;
 }
 
@@ -80,9 +89,35 @@ int graph_create(gs_graph **out) {
     GS_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     GS_TRY(g->counters.alloc(16));
     GS_HIP(hipHostMalloc((void **)&g->h_counters, 16 * sizeof(unsigned long long), hipHostMallocDefault));
+    GS_TRY(ws_create(&g->ws));
     *out = g.release();
     return GS_OK;
 }
+
+}  // namespace gs
+
+gs_ws::~gs_ws() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    timer.collect();
+    if (h_counters) (void)hipHostFree(h_counters);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+size_t gs_ws::device_bytes() const {
+    return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
+           bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + kc_deg.bytes() + ms_vis.bytes() +
+           ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_counts.bytes() +
+           counters.bytes();
+}
+
+gs_graph::~gs_graph() {
+    delete ws;
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (h_counters) (void)hipHostFree(h_counters);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace gs {
 
 static int set_geometry(gs_graph *g, uint64_t n, bool directed) {
     if (n == 0 || n > (1ull << 32)) {
@@ -482,14 +517,7 @@ const char *gs_kernel_name(int k) {
     return (k >= 0 && k < GS_K_COUNT) ? names[k] : "?";
 }
 
-void gs_close(gs_graph *g) {
-    if (!g) return;
-    if (g->stream) (void)hipStreamSynchronize(g->stream);
-    g->timer.collect();
-    if (g->h_counters) (void)hipHostFree(g->h_counters);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
-    delete g;  // DevBuf members are released by their owners below
-}
+void gs_close(gs_graph *g) { delete g; }
 
 int gs_open_arrays(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, const int32_t *degree,
                    gs_graph **out) {
@@ -643,9 +671,7 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->directed = g->directed;
     info->resident = 1;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
-                         g->degree.bytes() + g->pr_prior.bytes() + g->pr_acc.bytes() + g->pr_fac.bytes() +
-                         g->depth.bytes() + g->bm_vis.bytes() * 3 + g->rowact.bytes() + g->wcc_parent.bytes() +
-                         g->kc_deg.bytes();
+                         g->degree.bytes() + g->ws->device_bytes();
     return GS_OK;
 }
 
@@ -694,17 +720,17 @@ int gs_save(gs_graph *g, const char *prefix) {
 
 int gs_get_stats(gs_graph *g, gs_stats *st) {
     if (!g || !st) { set_error("null argument"); return GS_ERR_ARG; }
-    *st = g->stats;
+    *st = g->ws->stats;
     for (int i = 0; i < GS_K_COUNT; ++i) {
-        st->kern_ms[i] = g->timer.ms[i];
-        st->kern_calls[i] = g->timer.calls[i];
+        st->kern_ms[i] = g->ws->timer.ms[i];
+        st->kern_calls[i] = g->ws->timer.calls[i];
     }
     return GS_OK;
 }
 
 int gs_set_kernel_timing(gs_graph *g, int on) {
     if (!g) { set_error("null graph"); return GS_ERR_ARG; }
-    g->timer.on = on != 0;
+    g->ws->timer.on = on != 0;
     return GS_OK;
 }
 

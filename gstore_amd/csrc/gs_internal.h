@@ -106,9 +106,30 @@ struct KernelTimer {
 
 }  // namespace gs
 
+// State of one in-flight algorithm call: its stream, work arrays and stats.
+// A graph owns one default workspace; the serving engine makes more so calls
+// on the same graph can run concurrently.
+struct gs_ws {
+    hipStream_t stream = nullptr;
+    gs::DevBuf<float> pr_prior, pr_acc, pr_fac, ppr_rank;
+    gs::DevBuf<uint8_t> depth;
+    gs::DevBuf<uint32_t> bm_vis, bm_front, bm_next, rowact;  // bitmaps, words
+    gs::DevBuf<uint32_t> wcc_parent;
+    gs::DevBuf<int32_t> kc_deg;
+    gs::DevBuf<uint64_t> ms_vis, ms_front, ms_next, ms_roots;  // batched BFS lane masks
+    gs::DevBuf<uint8_t> ms_depth;                             // lane-major depth, lanes x N
+    gs::DevBuf<unsigned long long> ms_counts;
+    gs::DevBuf<unsigned long long> counters;
+    unsigned long long *h_counters = nullptr;  // pinned mirror of counters
+    gs::KernelTimer timer;
+    gs_stats stats{};
+    ~gs_ws();
+    size_t device_bytes() const;
+};
+
 struct gs_graph {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;            // loading / conversion
     bool directed = false;
     uint64_t n = 0, p = 0, p_s = 0, T = 0, E = 0, nz = 0, nchunk = 0;
     uint64_t rows = 0;                       // tile rows holding vertices: ceil(N/65536)
@@ -117,26 +138,31 @@ struct gs_graph {
     gs::DevBuf<uint32_t> tz_rc;
     gs::DevBuf<uint32_t> chunk_tz;
     gs::DevBuf<int32_t> degree;
-    // algorithm state, allocated on first use and kept
-    gs::DevBuf<float> pr_prior, pr_acc, pr_fac;
-    gs::DevBuf<uint8_t> depth;
-    gs::DevBuf<uint32_t> bm_vis, bm_front, bm_next, rowact;  // bitmaps, words
-    gs::DevBuf<uint32_t> wcc_parent;
-    gs::DevBuf<int32_t> kc_deg;
     gs::DevBuf<unsigned long long> counters;
-    unsigned long long *h_counters = nullptr;  // pinned mirror of counters
-    gs::KernelTimer timer;
-    gs_stats stats{};
-    // last results live in: pr_prior (rank), depth (BFS), wcc_parent (labels), kc_deg
+    unsigned long long *h_counters = nullptr;
+    gs_ws *ws = nullptr;                     // default workspace of the plain entry points
+    ~gs_graph();
 };
 
 namespace gs {
 
 int graph_create(gs_graph **out);
+int ws_create(gs_ws **out);
 int graph_build_meta(gs_graph *g, const uint64_t *h_start);  // tz/chunk lists, from host .start
-int ensure_bitmaps(gs_graph *g);
+int ensure_bitmaps(const gs_graph *g, gs_ws *w);
 uint64_t bitmap_words(const gs_graph *g);  // rows * ROW_WORDS
-int sync_and_collect(gs_graph *g);
+int sync_and_collect(gs_ws *w);
+
+// Algorithm bodies on an explicit workspace (the extern "C" entry points use g->ws).
+int run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *rank_out);
+int run_ppr(const gs_graph *g, gs_ws *w, uint64_t source, int iters, float *out);
+int run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
+            int max_levels, int *levels);
+int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
+                  uint64_t *level_counts, int max_levels, int *levels);
+int run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp);
+int run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
+              int *rounds);
 
 // XCD-aware chunk order: workgroup b runs on XCD b % 8; give each XCD a
 // contiguous run of chunks so its L2 sees consecutive tiles.

@@ -87,64 +87,73 @@ __global__ void k_kc_count(const int32_t *__restrict__ deg, uint64_t n, unsigned
 
 using namespace gs;
 
-extern "C" int gs_kcore(gs_graph *g, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
-                        int *rounds) {
+int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t *count_ne0,
+                   uint64_t *count_pos, int *rounds) {
     if (!g) { set_error("gs_kcore: null graph"); return GS_ERR_ARG; }
     if (g->directed) {
         set_error("k-core is not applicable to directed graphs");
         return GS_ERR_UNSUPPORTED;
     }
-    GS_TRY(ensure_bitmaps(g));
-    if (g->kc_deg.n != g->n) GS_TRY(g->kc_deg.alloc(g->n));
-    g->timer.reset();
+    GS_TRY(ensure_bitmaps(g, w));
+    if (w->kc_deg.n != g->n) GS_TRY(w->kc_deg.alloc(g->n));
+    w->timer.reset();
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
-    GS_HIP(hipEventRecord(t0, g->stream));
-    GS_HIP(hipMemcpyAsync(g->kc_deg.ptr, g->degree.ptr, g->n * 4, hipMemcpyDeviceToDevice, g->stream));
-    GS_HIP(hipMemsetAsync(g->rowact.ptr, 0, g->rowact.bytes(), g->stream));
-    GS_HIP(hipMemsetAsync(g->counters.ptr + 2, 0, 8, g->stream));
+    GS_HIP(hipEventRecord(t0, w->stream));
+    GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, g->degree.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+    GS_HIP(hipMemsetAsync(w->rowact.ptr, 0, w->rowact.bytes(), w->stream));
+    GS_HIP(hipMemsetAsync(w->counters.ptr + 2, 0, 8, w->stream));
     int nr = 0;
     for (;;) {
-        GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 8, g->stream));
+        GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
         hipEvent_t kb;
-        g->timer.begin(GS_K_KCORE_PEEL, g->stream, &kb);
-        k_kc_peel<<<(unsigned)g->rows, 256, 0, g->stream>>>(g->kc_deg.ptr, g->n, k, g->bm_front.ptr, g->rowact.ptr,
-                                                            g->counters.ptr);
+        w->timer.begin(GS_K_KCORE_PEEL, w->stream, &kb);
+        k_kc_peel<<<(unsigned)g->rows, 256, 0, w->stream>>>(w->kc_deg.ptr, g->n, k, w->bm_front.ptr, w->rowact.ptr,
+                                                            w->counters.ptr);
         GS_HIP(hipGetLastError());
-        g->timer.end(GS_K_KCORE_PEEL, g->stream, kb);
-        GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 8, hipMemcpyDeviceToHost, g->stream));
-        GS_HIP(hipStreamSynchronize(g->stream));
+        w->timer.end(GS_K_KCORE_PEEL, w->stream, kb);
+        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
         ++nr;
-        if (g->h_counters[0] == 0 || !g->nchunk) break;
+        if (w->h_counters[0] == 0 || !g->nchunk) break;
         if (nr >= 1000000) {
             set_error("gs_kcore: still removing vertices after %d rounds", nr);
             return GS_ERR_DEVICE;
         }
-        g->timer.begin(GS_K_KCORE_SCAN, g->stream, &kb);
-        k_kc_scan<<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), g->bm_front.ptr, g->kc_deg.ptr,
-                                                                   g->rowact.ptr, g->counters.ptr + 2);
+        w->timer.begin(GS_K_KCORE_SCAN, w->stream, &kb);
+        k_kc_scan<<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->bm_front.ptr, w->kc_deg.ptr,
+                                                                   w->rowact.ptr, w->counters.ptr + 2);
         GS_HIP(hipGetLastError());
-        g->timer.end(GS_K_KCORE_SCAN, g->stream, kb);
+        w->timer.end(GS_K_KCORE_SCAN, w->stream, kb);
     }
-    GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 16, g->stream));
-    k_kc_count<<<(unsigned)std::min<uint64_t>((g->n + 255) / 256, 8192), 256, 0, g->stream>>>(g->kc_deg.ptr, g->n,
-                                                                                            g->counters.ptr);
+    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 16, w->stream));
+    k_kc_count<<<(unsigned)std::min<uint64_t>((g->n + 255) / 256, 8192), 256, 0, w->stream>>>(w->kc_deg.ptr, g->n,
+                                                                                            w->counters.ptr);
     GS_HIP(hipGetLastError());
-    GS_HIP(hipEventRecord(t1, g->stream));
-    GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 24, hipMemcpyDeviceToHost, g->stream));
-    GS_TRY(sync_and_collect(g));
+    GS_HIP(hipEventRecord(t1, w->stream));
+    GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 24, hipMemcpyDeviceToHost, w->stream));
+    GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    g->stats = gs_stats{};
-    g->stats.ms = ms;
-    g->stats.iterations = (uint64_t)nr;
-    g->stats.edges_scanned = g->h_counters[2];
-    if (count_ne0) *count_ne0 = g->h_counters[0];
-    if (count_pos) *count_pos = g->h_counters[1];
+    w->stats = gs_stats{};
+    w->stats.ms = ms;
+    w->stats.iterations = (uint64_t)nr;
+    w->stats.edges_scanned = w->h_counters[2];
+    if (count_ne0) *count_ne0 = w->h_counters[0];
+    if (count_pos) *count_pos = w->h_counters[1];
     if (rounds) *rounds = nr;
-    if (deg_out) GS_HIP(hipMemcpy(deg_out, g->kc_deg.ptr, g->n * 4, hipMemcpyDeviceToHost));
+    if (deg_out) {
+        GS_HIP(hipMemcpyAsync(deg_out, w->kc_deg.ptr, g->n * 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+    }
     return GS_OK;
+}
+
+extern "C" int gs_kcore(gs_graph *g, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
+                        int *rounds) {
+    if (!g) { set_error("gs_kcore: null graph"); return GS_ERR_ARG; }
+    return run_kcore(g, g->ws, k, deg_out, count_ne0, count_pos, rounds);
 }

@@ -57,59 +57,151 @@ __global__ void k_pr_finalize(uint64_t n, float *__restrict__ prior, float *__re
     }
 }
 
+// Personalised PageRank from one source s (no reference counterpart; same
+// scatter as above): r0 = e_s, prior = r * (float)(1/deg),
+// r' = 0.15 [v == s] + 0.85 * sum of prior over in-edges.
+__global__ void k_ppr_init(const int32_t *__restrict__ deg, uint64_t n, uint64_t src, float *prior, float *acc,
+                           float *fac) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        int32_t d = deg[v];
+        float f = d != 0 ? (float)(1.0 / (double)d) : 0.0f;
+        fac[v] = f;
+        prior[v] = v == src ? f : 0.0f;
+        acc[v] = 0.0f;
+    }
+}
+
+__global__ void k_ppr_finalize(uint64_t n, uint64_t src, float *__restrict__ prior, float *__restrict__ acc,
+                               const float *__restrict__ fac, float *__restrict__ rank) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        float r = __double2float_rn(__dadd_rn(v == src ? 0.15 : 0.0, __dmul_rn(0.85, (double)acc[v])));
+        prior[v] = __fmul_rn(r, fac[v]);
+        acc[v] = 0.0f;
+        if (rank) rank[v] = r;
+    }
+}
+
 static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
 
 }  // namespace gs
 
 using namespace gs;
 
-extern "C" int gs_pagerank(gs_graph *g, int iters, int mode, float *rank_out) {
+int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *rank_out) {
     if (!g || iters < 0 || (mode != GS_PR_COMPAT && mode != GS_PR_INTENDED)) {
         set_error("gs_pagerank: bad arguments");
         return GS_ERR_ARG;
     }
-    if (g->pr_prior.n != g->n) {
-        GS_TRY(g->pr_prior.alloc(g->n));
-        GS_TRY(g->pr_acc.alloc(g->n));
-        GS_TRY(g->pr_fac.alloc(g->n));
+    if (w->pr_prior.n != g->n) {
+        GS_TRY(w->pr_prior.alloc(g->n));
+        GS_TRY(w->pr_acc.alloc(g->n));
+        GS_TRY(w->pr_fac.alloc(g->n));
     }
-    g->timer.reset();
+    w->timer.reset();
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
-    GS_HIP(hipEventRecord(t0, g->stream));
-    k_pr_init<<<vgrid(g->n), 256, 0, g->stream>>>(g->degree.ptr, g->n, mode, (float)(1.0 / (double)g->n),
-                                                  g->pr_prior.ptr, g->pr_acc.ptr, g->pr_fac.ptr);
+    GS_HIP(hipEventRecord(t0, w->stream));
+    k_pr_init<<<vgrid(g->n), 256, 0, w->stream>>>(g->degree.ptr, g->n, mode, (float)(1.0 / (double)g->n),
+                                                  w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
     GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
         if (g->nchunk) {
-            g->timer.begin(GS_K_PR_SCATTER, g->stream, &kb);
+            w->timer.begin(GS_K_PR_SCATTER, w->stream, &kb);
             if (g->directed)
-                k_pr_scatter<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), g->pr_prior.ptr,
-                                                                                    g->pr_acc.ptr);
+                k_pr_scatter<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->pr_prior.ptr,
+                                                                                    w->pr_acc.ptr);
             else
-                k_pr_scatter<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), g->pr_prior.ptr,
-                                                                                     g->pr_acc.ptr);
+                k_pr_scatter<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->pr_prior.ptr,
+                                                                                     w->pr_acc.ptr);
             GS_HIP(hipGetLastError());
-            g->timer.end(GS_K_PR_SCATTER, g->stream, kb);
+            w->timer.end(GS_K_PR_SCATTER, w->stream, kb);
         }
-        g->timer.begin(GS_K_PR_FINALIZE, g->stream, &kb);
-        k_pr_finalize<<<vgrid(g->n), 256, 0, g->stream>>>(g->n, g->pr_prior.ptr, g->pr_acc.ptr, g->pr_fac.ptr);
+        w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
+        k_pr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
         GS_HIP(hipGetLastError());
-        g->timer.end(GS_K_PR_FINALIZE, g->stream, kb);
+        w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
     }
-    GS_HIP(hipEventRecord(t1, g->stream));
-    GS_TRY(sync_and_collect(g));
+    GS_HIP(hipEventRecord(t1, w->stream));
+    GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    g->stats = gs_stats{};
-    g->stats.ms = ms;
-    g->stats.iterations = (uint64_t)iters;
-    g->stats.edges_scanned = g->E * (uint64_t)iters;
-    g->stats.tiles_scanned = g->nz * (uint64_t)iters;
-    if (rank_out) GS_HIP(hipMemcpy(rank_out, g->pr_prior.ptr, g->n * 4, hipMemcpyDeviceToHost));
+    w->stats = gs_stats{};
+    w->stats.ms = ms;
+    w->stats.iterations = (uint64_t)iters;
+    w->stats.edges_scanned = g->E * (uint64_t)iters;
+    w->stats.tiles_scanned = g->nz * (uint64_t)iters;
+    if (rank_out) {
+        GS_HIP(hipMemcpyAsync(rank_out, w->pr_prior.ptr, g->n * 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+    }
     return GS_OK;
+}
+
+int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out) {
+    if (!g || iters < 1 || src >= g->n) {
+        set_error("gs_ppr: bad arguments");
+        return GS_ERR_ARG;
+    }
+    if (w->pr_prior.n != g->n) {
+        GS_TRY(w->pr_prior.alloc(g->n));
+        GS_TRY(w->pr_acc.alloc(g->n));
+        GS_TRY(w->pr_fac.alloc(g->n));
+    }
+    if (w->ppr_rank.n != g->n) GS_TRY(w->ppr_rank.alloc(g->n));
+    w->timer.reset();
+    hipEvent_t t0, t1;
+    GS_HIP(hipEventCreate(&t0));
+    GS_HIP(hipEventCreate(&t1));
+    GS_HIP(hipEventRecord(t0, w->stream));
+    k_ppr_init<<<vgrid(g->n), 256, 0, w->stream>>>(g->degree.ptr, g->n, src, w->pr_prior.ptr, w->pr_acc.ptr,
+                                                   w->pr_fac.ptr);
+    GS_HIP(hipGetLastError());
+    for (int it = 0; it < iters; ++it) {
+        hipEvent_t kb;
+        if (g->nchunk) {
+            w->timer.begin(GS_K_PR_SCATTER, w->stream, &kb);
+            if (g->directed)
+                k_pr_scatter<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->pr_prior.ptr,
+                                                                                    w->pr_acc.ptr);
+            else
+                k_pr_scatter<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->pr_prior.ptr,
+                                                                                     w->pr_acc.ptr);
+            GS_HIP(hipGetLastError());
+            w->timer.end(GS_K_PR_SCATTER, w->stream, kb);
+        }
+        w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
+        k_ppr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, src, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr,
+                                                           it + 1 == iters ? w->ppr_rank.ptr : nullptr);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
+    }
+    GS_HIP(hipEventRecord(t1, w->stream));
+    GS_TRY(sync_and_collect(w));
+    float ms = 0.f;
+    GS_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    w->stats = gs_stats{};
+    w->stats.ms = ms;
+    w->stats.iterations = (uint64_t)iters;
+    w->stats.edges_scanned = g->E * (uint64_t)iters;
+    if (out) {
+        GS_HIP(hipMemcpyAsync(out, w->ppr_rank.ptr, g->n * 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+    }
+    return GS_OK;
+}
+
+extern "C" int gs_ppr(gs_graph *g, uint64_t source, int iters, float *rank_out) {
+    if (!g) { set_error("gs_ppr: null graph"); return GS_ERR_ARG; }
+    return run_ppr(g, g->ws, source, iters, rank_out);
+}
+
+extern "C" int gs_pagerank(gs_graph *g, int iters, int mode, float *rank_out) {
+    if (!g) { set_error("gs_pagerank: null graph"); return GS_ERR_ARG; }
+    return run_pagerank(g, g->ws, iters, mode, rank_out);
 }

@@ -1,0 +1,343 @@
+// gs_serve.hip -- serving loop with continuous batching (no reference
+// counterpart; the reference runs one job per process, gstore.cpp:235-333).
+//
+// Several graphs stay resident; queries (BFS, personalised PageRank, WCC,
+// PageRank) are submitted at any time and served by a pool of worker threads,
+// each with its own HIP stream and one workspace per graph, so the work of
+// different queries overlaps on the device.  A worker that comes free takes
+// the oldest pending query; if it is a BFS it also takes every other pending
+// BFS on the same graph, up to 64, and runs them as the bit lanes of one
+// batched sweep -- queries that arrived while earlier batches were on the GPU
+// join the next batch.  A query is admitted only when the workspace memory it
+// needs fits the HBM budget next to the memory already held; idle workspaces
+// give their memory back when a query needs room.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "gs_internal.h"
+
+namespace gs {
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Query {
+    uint64_t id;
+    int graph, kind;
+    uint64_t arg;
+    void *out;
+    double t_submit;
+};
+
+struct Worker;
+
+}  // namespace gs
+
+struct gs_server {
+    uint64_t budget = 0;       // bytes of workspace memory allowed
+    uint64_t held = 0;         // bytes held by all workspaces
+    int max_batch = 64;
+    int ppr_iters = 20, pr_iters = 20;
+    std::vector<gs_graph *> graphs;
+    std::deque<gs::Query> pending;
+    std::vector<gs_query_result> done;
+    std::vector<std::unique_ptr<gs::Worker>> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    uint64_t next_id = 1, inflight = 0;
+    bool stop = false;
+    double t0 = 0;
+};
+
+namespace gs {
+
+struct Worker {
+    gs_server *srv;
+    int index;
+    std::map<int, std::unique_ptr<gs_ws>> ws;  // per graph
+    std::thread th;
+    bool busy = false;
+};
+
+// Workspace bytes a query needs on graph g (what run_* allocates).
+static uint64_t need_bytes(const gs_graph *g, int kind, int lanes) {
+    uint64_t n = g->n, bm = g->rows * ROW_WORDS * 4 * 3 + (g->p_s + 31) / 32 * 4;
+    switch (kind) {
+        case GS_Q_BFS: return 24 * n + (uint64_t)lanes * n + bm + 1024;
+        case GS_Q_PPR: return 16 * n;
+        case GS_Q_PAGERANK: return 12 * n;
+        case GS_Q_WCC: return 4 * n;
+        default: return 0;
+    }
+}
+
+static uint64_t ws_bytes(const gs_ws *w) { return w->device_bytes(); }
+
+// Free idle workspaces (not the caller's) until `want` more bytes fit.  Lock held.
+static void make_room(gs_server *s, Worker *self, uint64_t want) {
+    for (auto &wk : s->workers) {
+        if (s->held + want <= s->budget) return;
+        if (wk.get() == self || wk->busy) continue;
+        for (auto &kv : wk->ws) {
+            uint64_t b = ws_bytes(kv.second.get());
+            kv.second.reset();
+            s->held -= std::min(s->held, b);
+        }
+        wk->ws.clear();
+    }
+}
+
+static uint64_t fnv(const void *p, size_t n) {
+    const unsigned char *c = (const unsigned char *)p;
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    return h;
+}
+
+static void worker_main(Worker *wk) {
+    gs_server *s = wk->srv;
+    std::vector<uint64_t> roots;
+    std::vector<uint64_t> counts;
+    for (;;) {
+        std::vector<Query> batch;
+        uint64_t need = 0;
+        gs_ws *w = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (;;) {
+                if (s->stop) return;
+                if (!s->pending.empty()) {
+                    // oldest query first; a BFS collects the pending BFS of its graph
+                    const Query &q = s->pending.front();
+                    gs_graph *g = s->graphs[q.graph];
+                    int lanes = 1;
+                    if (q.kind == GS_Q_BFS) {
+                        lanes = 0;
+                        for (auto &p : s->pending)
+                            if (p.kind == GS_Q_BFS && p.graph == q.graph && lanes < s->max_batch) ++lanes;
+                        while (lanes > 1 && need_bytes(g, GS_Q_BFS, lanes) > s->budget) --lanes;
+                    }
+                    auto it = wk->ws.find(q.graph);
+                    uint64_t have = it != wk->ws.end() && it->second ? ws_bytes(it->second.get()) : 0;
+                    uint64_t want = need_bytes(g, q.kind, lanes);
+                    need = want > have ? want - have : 0;
+                    if (s->held + need > s->budget) make_room(s, wk, need);
+                    if (s->held + need <= s->budget) {
+                        s->held += need;
+                        int gid = q.graph, kind = q.kind;
+                        if (kind == GS_Q_BFS) {
+                            for (auto p = s->pending.begin(); p != s->pending.end() && (int)batch.size() < lanes;) {
+                                if (p->kind == GS_Q_BFS && p->graph == gid) {
+                                    batch.push_back(*p);
+                                    p = s->pending.erase(p);
+                                } else {
+                                    ++p;
+                                }
+                            }
+                        } else {
+                            batch.push_back(q);
+                            s->pending.pop_front();
+                        }
+                        ++s->inflight;
+                        wk->busy = true;
+                        break;
+                    }
+                }
+                s->cv_work.wait(lk);
+            }
+        }
+        const int gid = batch[0].graph, kind = batch[0].kind;
+        const gs_graph *g = s->graphs[gid];
+        auto &slot = wk->ws[gid];
+        int status = GS_OK;
+        if (!slot) {
+            gs_ws *nw = nullptr;
+            status = ws_create(&nw);
+            slot.reset(nw);
+        }
+        w = slot.get();
+        double t_start = now_ms();
+        std::vector<gs_query_result> res(batch.size());
+        for (size_t i = 0; i < batch.size(); ++i) {
+            res[i] = gs_query_result{};
+            res[i].id = batch[i].id;
+            res[i].graph = gid;
+            res[i].kind = kind;
+            res[i].arg = batch[i].arg;
+            res[i].batch_size = (int)batch.size();
+            res[i].t_submit_ms = batch[i].t_submit - s->t0;
+            res[i].t_start_ms = t_start - s->t0;
+        }
+        uint64_t before = w ? ws_bytes(w) : 0;
+        if (status == GS_OK) {
+            switch (kind) {
+                case GS_Q_BFS: {
+                    const int nr = (int)batch.size();
+                    roots.resize(nr);
+                    for (int i = 0; i < nr; ++i) roots[i] = batch[i].arg;
+                    counts.assign((size_t)nr * 256, 0);
+                    int levels = 0;
+                    status = run_bfs_batch(g, w, roots.data(), nr, nullptr, counts.data(), 256, &levels);
+                    for (int i = 0; i < nr && status == GS_OK; ++i) {
+                        uint64_t reached = 1;
+                        for (int l = 0; l < levels && l < 256; ++l) reached += counts[(size_t)i * 256 + l];
+                        res[i].value = reached;
+                        res[i].aux = (uint64_t)levels;
+                        if (batch[i].out) {
+                            if (hipMemcpyAsync(batch[i].out, w->ms_depth.ptr + (size_t)i * g->n, g->n,
+                                               hipMemcpyDeviceToHost, w->stream) != hipSuccess ||
+                                hipStreamSynchronize(w->stream) != hipSuccess)
+                                status = GS_ERR_DEVICE;
+                            else
+                                res[i].checksum = fnv(batch[i].out, g->n);
+                        }
+                    }
+                    break;
+                }
+                case GS_Q_PPR:
+                    status = run_ppr(g, w, batch[0].arg, s->ppr_iters, (float *)batch[0].out);
+                    if (status == GS_OK && batch[0].out) res[0].checksum = fnv(batch[0].out, g->n * 4);
+                    break;
+                case GS_Q_PAGERANK:
+                    status = run_pagerank(g, w, s->pr_iters, (int)batch[0].arg, (float *)batch[0].out);
+                    if (status == GS_OK && batch[0].out) res[0].checksum = fnv(batch[0].out, g->n * 4);
+                    break;
+                case GS_Q_WCC: {
+                    uint64_t nc = 0;
+                    status = run_wcc(g, w, (uint32_t *)batch[0].out, &nc);
+                    res[0].value = nc;
+                    if (status == GS_OK && batch[0].out) res[0].checksum = fnv(batch[0].out, g->n * 4);
+                    break;
+                }
+                default:
+                    status = GS_ERR_ARG;
+            }
+        }
+        double t_done = now_ms();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            uint64_t after = w ? ws_bytes(w) : 0;
+            // settle the reservation against what the workspace really holds
+            s->held = s->held - std::min(s->held, need) - std::min(s->held - std::min(s->held, need), before) + after;
+            for (auto &r : res) {
+                r.status = status;
+                r.t_done_ms = t_done - s->t0;
+                s->done.push_back(r);
+            }
+            --s->inflight;
+            wk->busy = false;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" {
+
+int gs_server_create(uint64_t budget, int workers, gs_server **out) {
+    if (!out || workers < 1 || workers > 16) { set_error("gs_server_create: 1..16 workers"); return GS_ERR_ARG; }
+    std::unique_ptr<gs_server> s(new gs_server());
+    s->budget = budget ? budget : ~0ull;
+    s->t0 = now_ms();
+    for (int i = 0; i < workers; ++i) {
+        s->workers.emplace_back(new Worker());
+        s->workers.back()->srv = s.get();
+        s->workers.back()->index = i;
+    }
+    for (auto &wk : s->workers) wk->th = std::thread(worker_main, wk.get());
+    *out = s.release();
+    return GS_OK;
+}
+
+int gs_server_add_graph(gs_server *s, gs_graph *g, int *graph_id) {
+    if (!s || !g || !graph_id) { set_error("null argument"); return GS_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->graphs.push_back(g);
+    *graph_id = (int)s->graphs.size() - 1;
+    return GS_OK;
+}
+
+int gs_server_set(gs_server *s, int max_batch, int ppr_iters, int pr_iters) {
+    if (!s || max_batch < 1 || max_batch > 64 || ppr_iters < 1 || pr_iters < 0) {
+        set_error("gs_server_set: bad arguments");
+        return GS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->max_batch = max_batch;
+    s->ppr_iters = ppr_iters;
+    s->pr_iters = pr_iters;
+    return GS_OK;
+}
+
+int gs_server_submit(gs_server *s, int graph, int kind, uint64_t arg, void *out, uint64_t *id) {
+    if (!s) { set_error("null server"); return GS_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (graph < 0 || graph >= (int)s->graphs.size()) { set_error("unknown graph %d", graph); return GS_ERR_ARG; }
+    const gs_graph *g = s->graphs[graph];
+    if (kind < GS_Q_BFS || kind > GS_Q_PAGERANK) { set_error("unknown query kind %d", kind); return GS_ERR_ARG; }
+    if ((kind == GS_Q_BFS || kind == GS_Q_PPR) && arg >= g->n) {
+        set_error("query source %llu >= N", (unsigned long long)arg);
+        return GS_ERR_ARG;
+    }
+    if (need_bytes(g, kind, 1) > s->budget) {
+        set_error("query needs more workspace memory than the server budget");
+        return GS_ERR_NOMEM;
+    }
+    Query q{s->next_id++, graph, kind, arg, out, now_ms()};
+    s->pending.push_back(q);
+    if (id) *id = q.id;
+    s->cv_work.notify_one();
+    return GS_OK;
+}
+
+int gs_server_poll(gs_server *s, gs_query_result *out, int max, int wait_ms, int *n) {
+    if (!s || !n || (max > 0 && !out)) { set_error("null argument"); return GS_ERR_ARG; }
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !s->done.empty(); });
+    int k = std::min<int>(max, (int)s->done.size());
+    for (int i = 0; i < k; ++i) out[i] = s->done[i];
+    s->done.erase(s->done.begin(), s->done.begin() + k);
+    *n = k;
+    return GS_OK;
+}
+
+int gs_server_drain(gs_server *s) {
+    if (!s) { set_error("null server"); return GS_ERR_ARG; }
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->inflight == 0; });
+    return GS_OK;
+}
+
+int gs_server_info(gs_server *s, uint64_t *held_bytes, uint64_t *pending, uint64_t *inflight) {
+    if (!s) { set_error("null server"); return GS_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (held_bytes) *held_bytes = s->held;
+    if (pending) *pending = s->pending.size();
+    if (inflight) *inflight = s->inflight;
+    return GS_OK;
+}
+
+void gs_server_destroy(gs_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &wk : s->workers)
+        if (wk->th.joinable()) wk->th.join();
+    delete s;
+}
+
+}  // extern "C"

@@ -93,14 +93,14 @@ static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255
 
 using namespace gs;
 
-extern "C" int gs_wcc(gs_graph *g, uint32_t *label_out, uint64_t *ncomp) {
+int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp) {
     if (!g) { set_error("gs_wcc: null graph"); return GS_ERR_ARG; }
-    if (g->wcc_parent.n != g->n) GS_TRY(g->wcc_parent.alloc(g->n));
-    g->timer.reset();
+    if (w->wcc_parent.n != g->n) GS_TRY(w->wcc_parent.alloc(g->n));
+    w->timer.reset();
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventRecord(t0, g->stream));
-    k_wcc_init<<<vgrid(g->n), 256, 0, g->stream>>>(g->wcc_parent.ptr, g->n);
+    GS_HIP(hipEventRecord(t0, w->stream));
+    k_wcc_init<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
     GS_HIP(hipGetLastError());
     uint64_t passes = 0;
     for (;;) {
@@ -110,42 +110,50 @@ extern "C" int gs_wcc(gs_graph *g, uint32_t *label_out, uint64_t *ncomp) {
         }
         hipEvent_t kb;
         if (g->nchunk) {
-            g->timer.begin(GS_K_WCC_HOOK, g->stream, &kb);
-            k_wcc_hook<<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), g->wcc_parent.ptr);
+            w->timer.begin(GS_K_WCC_HOOK, w->stream, &kb);
+            k_wcc_hook<<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->wcc_parent.ptr);
             GS_HIP(hipGetLastError());
-            g->timer.end(GS_K_WCC_HOOK, g->stream, kb);
+            w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
         }
         ++passes;
-        g->timer.begin(GS_K_WCC_COMPRESS, g->stream, &kb);
-        k_wcc_compress<<<vgrid(g->n), 256, 0, g->stream>>>(g->wcc_parent.ptr, g->n);
+        w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
+        k_wcc_compress<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
         GS_HIP(hipGetLastError());
-        g->timer.end(GS_K_WCC_COMPRESS, g->stream, kb);
+        w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
         if (!g->nchunk) break;
-        GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 8, g->stream));
-        g->timer.begin(GS_K_WCC_CHECK, g->stream, &kb);
-        k_wcc_check<<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), g->wcc_parent.ptr, g->counters.ptr);
+        GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
+        w->timer.begin(GS_K_WCC_CHECK, w->stream, &kb);
+        k_wcc_check<<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->wcc_parent.ptr, w->counters.ptr);
         GS_HIP(hipGetLastError());
-        g->timer.end(GS_K_WCC_CHECK, g->stream, kb);
-        GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 8, hipMemcpyDeviceToHost, g->stream));
-        GS_HIP(hipStreamSynchronize(g->stream));
-        if (g->h_counters[0] == 0) break;
+        w->timer.end(GS_K_WCC_CHECK, w->stream, kb);
+        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        if (w->h_counters[0] == 0) break;
     }
-    GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 8, g->stream));
-    k_wcc_label<<<vgrid(g->n), 256, 0, g->stream>>>(g->wcc_parent.ptr, g->degree.ptr, g->n, g->counters.ptr);
+    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
+    k_wcc_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->degree.ptr, g->n, w->counters.ptr);
     GS_HIP(hipGetLastError());
     GS_HIP(hipEventCreate(&t1));
-    GS_HIP(hipEventRecord(t1, g->stream));
-    GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 8, hipMemcpyDeviceToHost, g->stream));
-    GS_TRY(sync_and_collect(g));
+    GS_HIP(hipEventRecord(t1, w->stream));
+    GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 8, hipMemcpyDeviceToHost, w->stream));
+    GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    g->stats = gs_stats{};
-    g->stats.ms = ms;
-    g->stats.iterations = passes;
-    g->stats.edges_scanned = g->E * passes * 2;
-    if (ncomp) *ncomp = g->h_counters[0];
-    if (label_out) GS_HIP(hipMemcpy(label_out, g->wcc_parent.ptr, g->n * 4, hipMemcpyDeviceToHost));
+    w->stats = gs_stats{};
+    w->stats.ms = ms;
+    w->stats.iterations = passes;
+    w->stats.edges_scanned = g->E * passes * 2;
+    if (ncomp) *ncomp = w->h_counters[0];
+    if (label_out) {
+        GS_HIP(hipMemcpyAsync(label_out, w->wcc_parent.ptr, g->n * 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+    }
     return GS_OK;
+}
+
+extern "C" int gs_wcc(gs_graph *g, uint32_t *label_out, uint64_t *ncomp) {
+    if (!g) { set_error("gs_wcc: null graph"); return GS_ERR_ARG; }
+    return run_wcc(g, g->ws, label_out, ncomp);
 }

@@ -113,12 +113,26 @@ int gs_get_arrays(gs_graph *g, uint64_t *start, uint32_t *grid, int32_t *degree)
  * pagerank_mmap gstore.cpp:2572-2629; -j 1/11/21 -a iters, default 5). */
 int gs_pagerank(gs_graph *g, int iters, int mode, float *rank_out);
 
+/* Personalised PageRank from one source (no reference counterpart; same
+ * tile scatter as gs_pagerank): r_0 = e_source, prior = r * (float)(1/deg),
+ * r_{t+1} = 0.15 [v == source] + 0.85 * sum of prior over neighbours.
+ * rank_out[v] = r after `iters` iterations (FP32). */
+int gs_ppr(gs_graph *g, uint64_t source, int iters, float *rank_out);
+
 /* BFS from root (bfs_t, bfs.cpp:56-338; driver bfs_mmap gstore.cpp:2631-2713;
  * -j 0/10/20 -a root).  depth_out: u8, 0 = unreached, root = 1.
  * level_counts[L-1] = vertices reached at level L (up to max_levels).
  * *levels = number of levels run (including the last, empty one). */
 int gs_bfs(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
            int max_levels, int *levels);
+
+/* Batched BFS: up to 64 roots run as bit lanes of one multi-source sweep
+ * (no reference counterpart; each lane applies the rule of bfs.cpp:158-177).
+ * depth_out: n_roots x N bytes, lane i at depth_out + i*N, same encoding as
+ * gs_bfs and bit-identical to gs_bfs(roots[i]).  level_counts: n_roots x
+ * max_levels (lane-major).  *levels = levels run for the whole batch. */
+int gs_bfs_batch(gs_graph *g, const uint64_t *roots, int n_roots, uint8_t *depth_out,
+                 uint64_t *level_counts, int max_levels, int *levels);
 
 /* Weakly connected components (wcc2_t, wcc.cpp:57-395; driver wcc_mmap
  * gstore.cpp:2502-2570; -j 3/13/23).  label_out[v] = smallest vertex id of
@@ -133,6 +147,44 @@ int gs_wcc(gs_graph *g, uint32_t *label_out, uint64_t *ncomp);
  * rounds run, including the first. */
 int gs_kcore(gs_graph *g, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
              int *rounds);
+
+/* ---- Serving (no reference counterpart) ------------------------------------
+ * Several resident graphs, queries submitted at any time, a pool of worker
+ * threads each with its own stream and per-graph workspaces so queries overlap
+ * on the device.  Continuous batching: a free worker takes the oldest pending
+ * query; a BFS takes every pending BFS of its graph along (up to max_batch,
+ * default 64) into one gs_bfs_batch sweep.  A query starts only when the
+ * workspace memory it needs fits hbm_budget_bytes (0 = unlimited) beside the
+ * memory other workspaces hold; idle workspaces are freed to make room.
+ * Results: `out` (optional, caller-owned, must stay valid until the query
+ * completes) receives the BFS depth (N bytes), the PPR / PageRank ranks or the
+ * WCC labels (4N bytes); the result record carries latency stamps. */
+enum gs_query_kind { GS_Q_BFS = 0, GS_Q_PPR = 1, GS_Q_WCC = 2, GS_Q_PAGERANK = 3 };
+
+typedef struct gs_query_result {
+    uint64_t id;
+    uint64_t arg;        /* BFS root / PPR source / PageRank mode */
+    uint64_t value;      /* BFS: reached vertices; WCC: component count */
+    uint64_t aux;        /* BFS: levels of its batch */
+    uint64_t checksum;   /* FNV-1a of `out` when given */
+    double t_submit_ms, t_start_ms, t_done_ms;  /* server clock */
+    int32_t graph, kind, status, batch_size;
+} gs_query_result;
+
+typedef struct gs_server gs_server;
+
+int gs_server_create(uint64_t hbm_budget_bytes, int workers, gs_server **out);
+/* The server does not own the graph; keep it open until gs_server_destroy. */
+int gs_server_add_graph(gs_server *s, gs_graph *g, int *graph_id);
+/* max_batch 1..64 BFS lanes; iterations of PPR and PageRank queries (default 20). */
+int gs_server_set(gs_server *s, int max_batch, int ppr_iters, int pr_iters);
+int gs_server_submit(gs_server *s, int graph_id, int kind, uint64_t arg, void *out, uint64_t *id);
+/* Completed results (up to max), waiting up to wait_ms for the first. */
+int gs_server_poll(gs_server *s, gs_query_result *out, int max, int wait_ms, int *n);
+/* Block until nothing is pending or running. */
+int gs_server_drain(gs_server *s);
+int gs_server_info(gs_server *s, uint64_t *held_bytes, uint64_t *pending, uint64_t *inflight);
+void gs_server_destroy(gs_server *s);
 
 /* Stats of the last algorithm call on this handle. */
 int gs_get_stats(gs_graph *g, gs_stats *st);

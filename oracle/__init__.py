@@ -40,6 +40,7 @@ def lib() -> ctypes.CDLL:
         L.or_convert_count.restype = i64
         L.or_convert_scatter.argtypes = [p, u64, u64, i, p, p]
         L.or_pagerank.argtypes = [u64, i, p, p, p, i, i, i, p]
+        L.or_ppr.argtypes = [u64, i, p, p, p, u64, i, p]
         L.or_bfs.argtypes = [u64, i, p, p, u64, p, p, i]
         L.or_wcc.argtypes = [u64, i, p, p, p, p]
         L.or_kcore.argtypes = [u64, p, p, p, i, p, p, p]
@@ -83,6 +84,14 @@ def pagerank(n, start, grid, degree, iters=5, mode=0, nthreads=1, directed=False
     rc = lib().or_pagerank(n, int(directed), _p(start), _p(grid), _p(degree), iters, mode, nthreads, _p(out))
     if rc:
         raise RuntimeError("or_pagerank failed")
+    return out
+
+
+def ppr(n, start, grid, degree, source, iters=20, directed=False) -> np.ndarray:
+    """FP64 personalised PageRank."""
+    out = np.empty(n, np.float64)
+    if lib().or_ppr(n, int(directed), _p(start), _p(grid), _p(degree), source, iters, _p(out)):
+        raise RuntimeError("or_ppr failed")
     return out
 
 

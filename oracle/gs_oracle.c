@@ -408,6 +408,40 @@ int or_pagerank(uint64_t n, int directed, const uint64_t *start, const uint32_t 
     return 0;
 }
 
+/* Personalised PageRank in FP64 (no reference counterpart; uses the tile
+ * scatter of pr.cpp:141-152): r_0 = e_src, prior = r / deg,
+ * r_{t+1}[v] = 0.15 [v == src] + 0.85 * sum of prior over in-edges. */
+int or_ppr(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, const int32_t *degree,
+           uint64_t src, int iters, double *out)
+{
+    walk_t w;
+    if (src >= n) return -1;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    double *prior = (double *)calloc(n, sizeof(double)), *acc = (double *)calloc(n, sizeof(double));
+    prior[src] = degree[src] ? 1.0 / degree[src] : 0.0;
+    for (int it = 0; it < iters; ++it) {
+        for (uint64_t t = 0; t < w.T; ++t) {
+            uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+            for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                uint64_t v0 = o0 + (grid[k] & 0xFFFF), v1 = o1 + (grid[k] >> 16);
+                acc[v1] += prior[v0];
+                if (!directed) acc[v0] += prior[v1];
+            }
+        }
+        for (uint64_t v = 0; v < n; ++v) {
+            double r = (v == src ? 0.15 : 0.0) + 0.85 * acc[v];
+            out[v] = r;
+            prior[v] = degree[v] ? r / degree[v] : 0.0;
+            acc[v] = 0.0;
+        }
+    }
+    if (iters < 1) for (uint64_t v = 0; v < n; ++v) out[v] = v == src;
+    free(prior);
+    free(acc);
+    walk_free(&w);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* BFS (bfs.cpp:56-224; resident driver gstore.cpp:2631-2713)                */
 /* ------------------------------------------------------------------------ */

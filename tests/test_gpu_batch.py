@@ -1,0 +1,84 @@
+"""Batched (multi-source, 64 bit-lane) BFS: every lane must equal the
+single-root BFS of the oracle bit for bit, whatever else shares the batch."""
+import numpy as np
+import pytest
+
+import gstore_amd as ga
+import oracle
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-batch]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def kron14():
+    n, e = graphs.kron(14)
+    arrays = oracle.convert(e, n)
+    g = ga.Graph.from_arrays(n, *arrays)
+    yield n, arrays, g
+    g.close()
+
+
+def _check_batch(g, n, start, grid, roots):
+    depth, counts = g.bfs_batch(roots)
+    for i, r in enumerate(roots):
+        d, lc = oracle.bfs(n, start, grid, int(r))
+        assert np.array_equal(depth[i], d), (i, r)
+        assert counts[i][:len(lc)] == lc and all(c == 0 for c in counts[i][len(lc):])
+
+
+def test_batch_64_roots_equal_single(kron14):
+    n, (start, grid, deg), g = kron14
+    rng = np.random.default_rng(1)
+    roots = rng.choice(np.flatnonzero(deg), 64, replace=False)
+    _say("64 roots")
+    _check_batch(g, n, start, grid, roots)
+
+
+@pytest.mark.parametrize("nr", [1, 7, 33])
+def test_batch_partial_and_duplicate_roots(kron14, nr):
+    n, (start, grid, deg), g = kron14
+    rng = np.random.default_rng(nr)
+    roots = rng.integers(0, n, nr)          # isolated roots and repeats allowed
+    if nr > 1:
+        roots[-1] = roots[0]
+    _say("roots", nr)
+    _check_batch(g, n, start, grid, roots)
+
+
+def test_batch_lane_independent_of_mix(kron14):
+    n, (start, grid, deg), g = kron14
+    a, _ = g.bfs_batch([5])
+    b, _ = g.bfs_batch(list(range(60, 123)) + [5])
+    assert np.array_equal(a[0], b[-1])
+
+
+@pytest.mark.parametrize("name", ["path", "tiles", "components"])
+def test_batch_hand_graphs(name):
+    n, e = graphs.HAND[name]()
+    start, grid, deg = oracle.convert(e, n)
+    with ga.Graph.from_arrays(n, start, grid, deg) as g:
+        _say(name)
+        _check_batch(g, n, start, grid, [0, 1, n - 1, int(np.argmax(deg))])
+
+
+def test_batch_directed():
+    n, e = graphs.kron(12)
+    start, grid, deg = oracle.convert(e, n, directed=True)
+    with ga.Graph.from_arrays(n, start, grid, deg, directed=True) as g:
+        roots = [0, 3, 9, int(np.argmax(deg))]
+        depth, _ = g.bfs_batch(roots)
+        for i, r in enumerate(roots):
+            assert np.array_equal(depth[i], oracle.bfs(n, start, grid, r, directed=True)[0])
+
+
+def test_batch_errors(kron14):
+    n, _, g = kron14
+    with pytest.raises(ga.GStoreError):
+        g.bfs_batch(list(range(65)))
+    with pytest.raises(ga.GStoreError):
+        g.bfs_batch([n])

@@ -173,6 +173,22 @@ def test_pagerank_intended_vs_float64(name, n, e, directed):
         assert np.max(np.abs(wrong - want)) > 1e-3
 
 
+@pytest.mark.parametrize("name,n,e", CASES, ids=[c[0] for c in CASES])
+def test_ppr_vs_scipy(name, n, e):
+    start, grid, degree = oracle.convert(e, n)
+    A = _adj(n, start, grid)
+    inv = np.where(degree > 0, 1.0 / np.maximum(degree, 1), 0.0)
+    for src in (0, int(np.argmax(degree))):
+        r = np.zeros(n)
+        r[src] = 1.0
+        for _ in range(20):
+            r = 0.85 * (A.T @ (r * inv))
+            r[src] += 0.15
+        got = oracle.ppr(n, start, grid, degree, src, 20)
+        assert np.max(np.abs(got - r)) <= 1e-12
+        assert got.sum() <= 1.0 + 1e-9
+
+
 def test_pagerank_parallel_matches_sequential_within_fp32():
     n, e = graphs.kron(12)
     start, grid, degree = oracle.convert(e, n)
