@@ -20,7 +20,7 @@ import numpy as np
 
 from . import tiles
 
-__all__ = ["Graph", "GStoreError", "kron_edges", "convert", "load_library", "tiles", "LIB_PATH",
+__all__ = ["Graph", "Server", "QueryResult", "Q_BFS", "Q_PPR", "Q_WCC", "Q_PAGERANK", "GStoreError", "kron_edges", "convert", "load_library", "tiles", "LIB_PATH",
            "PR_COMPAT", "PR_INTENDED"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -33,7 +33,7 @@ GS_OK = 0
 _STATUS = {1: "GS_ERR_ARG", 2: "GS_ERR_IO", 3: "GS_ERR_FORMAT", 4: "GS_ERR_NOMEM",
            5: "GS_ERR_DEVICE", 6: "GS_ERR_UNSUPPORTED", 7: "GS_ERR_DEPTH_OVERFLOW"}
 KERNELS = ["pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end", "wcc_hook", "wcc_compress",
-           "wcc_check", "kcore_peel", "kcore_scan", "kron_gen", "convert"]
+           "wcc_check", "kcore_peel", "kcore_scan", "kron_gen", "convert", "h2d_copy"]
 
 
 class GStoreError(RuntimeError):
@@ -46,16 +46,29 @@ class _Info(ctypes.Structure):
     _fields_ = [("vert_count", ctypes.c_uint64), ("edge_count", ctypes.c_uint64),
                 ("tile_slots", ctypes.c_uint64), ("nonempty_tiles", ctypes.c_uint64),
                 ("groups", ctypes.c_uint64), ("tiles_per_side", ctypes.c_uint64),
-                ("device_bytes", ctypes.c_uint64), ("directed", ctypes.c_int32),
+                ("device_bytes", ctypes.c_uint64), ("resident_edges", ctypes.c_uint64),
+                ("segment_edges", ctypes.c_uint64), ("directed", ctypes.c_int32),
                 ("resident", ctypes.c_int32)]
 
 
 class _Stats(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double), ("iterations", ctypes.c_uint64),
                 ("edges_scanned", ctypes.c_uint64), ("tiles_scanned", ctypes.c_uint64),
-                ("kern_ms", ctypes.c_double * len(KERNELS)),
+                ("h2d_bytes", ctypes.c_uint64), ("kern_ms", ctypes.c_double * len(KERNELS)),
                 ("kern_calls", ctypes.c_uint64 * len(KERNELS))]
 
+
+class QueryResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("arg", ctypes.c_uint64), ("value", ctypes.c_uint64),
+                ("aux", ctypes.c_uint64), ("checksum", ctypes.c_uint64), ("t_submit_ms", ctypes.c_double),
+                ("t_start_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double), ("graph", ctypes.c_int32),
+                ("kind", ctypes.c_int32), ("status", ctypes.c_int32), ("batch_size", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+Q_BFS, Q_PPR, Q_WCC, Q_PAGERANK = 0, 1, 2, 3
 
 _lib: Optional[ctypes.CDLL] = None
 
@@ -70,6 +83,7 @@ _SIGS = {
     "gs_convert": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p]),
     "gs_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "gs_get_info": (ctypes.c_int, [_P, ctypes.POINTER(_Info)]),
+    "gs_set_device_budget": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64]),
     "gs_get_arrays": (ctypes.c_int, [_P, _P, _P, _P]),
     "gs_pagerank": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P]),
     "gs_ppr": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_int, _P]),
@@ -82,6 +96,17 @@ _SIGS = {
     "gs_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
     "gs_close": (None, [_P]),
+    "gs_server_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_P)]),
+    "gs_server_add_graph": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int)]),
+    "gs_server_set": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "gs_server_submit": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _P,
+                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "gs_server_poll": (ctypes.c_int, [_P, ctypes.POINTER(QueryResult), ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int)]),
+    "gs_server_drain": (ctypes.c_int, [_P]),
+    "gs_server_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64)]),
+    "gs_server_destroy": (None, [_P]),
     "gs_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 
@@ -133,6 +158,9 @@ class Graph:
         self.p = int(info.groups)
         self.p_s = int(info.tiles_per_side)
         self.directed = bool(info.directed)
+        self.resident = bool(info.resident)
+        self.resident_edges = int(info.resident_edges)
+        self.segment_edges = int(info.segment_edges)
 
     # --- construction -------------------------------------------------------
     @classmethod
@@ -191,6 +219,11 @@ class Graph:
         _check(_lib.gs_get_arrays(self._h, _ptr(start), _ptr(grid) if self.E else None, _ptr(degree)))
         return start, grid, degree
 
+    def set_device_budget(self, budget_bytes: int, segment_bytes: int = 0) -> None:
+        """Semi-external mode: keep at most budget_bytes of the grid in HBM."""
+        _check(_lib.gs_set_device_budget(self._h, budget_bytes, segment_bytes))
+        self.__init__(self._h)
+
     def degree(self) -> np.ndarray:
         degree = np.empty(self.n, np.int32)
         _check(_lib.gs_get_arrays(self._h, None, None, _ptr(degree)))
@@ -206,7 +239,7 @@ class Graph:
         st = _Stats()
         _check(_lib.gs_get_stats(self._h, ctypes.byref(st)))
         return {"ms": st.ms, "iterations": int(st.iterations), "edges_scanned": int(st.edges_scanned),
-                "tiles_scanned": int(st.tiles_scanned),
+                "tiles_scanned": int(st.tiles_scanned), "h2d_bytes": int(st.h2d_bytes),
                 "kernels": {KERNELS[i]: {"ms": st.kern_ms[i], "calls": int(st.kern_calls[i])}
                             for i in range(len(KERNELS)) if st.kern_calls[i]}}
 
@@ -253,6 +286,62 @@ class Graph:
         _check(_lib.gs_kcore(self._h, k, _ptr(deg) if copy_out else None, ctypes.byref(ne0), ctypes.byref(pos),
                              ctypes.byref(rounds)))
         return deg, int(ne0.value), int(pos.value), int(rounds.value)
+
+
+class Server:
+    """Continuous-batching query server over resident graphs (gs_server_*)."""
+
+    def __init__(self, hbm_budget: int = 0, workers: int = 4):
+        load_library()
+        self._h = ctypes.c_void_p()
+        _check(_lib.gs_server_create(hbm_budget, workers, ctypes.byref(self._h)))
+        self._graphs = []
+        self._keep = {}   # query id -> output buffer kept alive until polled
+
+    def add_graph(self, g: "Graph") -> int:
+        gid = ctypes.c_int(0)
+        _check(_lib.gs_server_add_graph(self._h, g._h, ctypes.byref(gid)))
+        self._graphs.append(g)
+        return gid.value
+
+    def configure(self, max_batch: int = 64, ppr_iters: int = 20, pr_iters: int = 20) -> None:
+        _check(_lib.gs_server_set(self._h, max_batch, ppr_iters, pr_iters))
+
+    def submit(self, graph: int, kind: int, arg: int = 0, out: Optional[np.ndarray] = None) -> int:
+        qid = ctypes.c_uint64(0)
+        _check(_lib.gs_server_submit(self._h, graph, kind, arg, _ptr(out) if out is not None else None,
+                                     ctypes.byref(qid)))
+        if out is not None:
+            self._keep[qid.value] = out
+        return qid.value
+
+    def poll(self, max_results: int = 4096, wait_ms: int = 0) -> list:
+        buf = (QueryResult * max_results)()
+        n = ctypes.c_int(0)
+        _check(_lib.gs_server_poll(self._h, buf, max_results, wait_ms, ctypes.byref(n)))
+        out = [buf[i].as_dict() for i in range(n.value)]
+        for r in out:
+            self._keep.pop(r["id"], None)
+        return out
+
+    def drain(self) -> None:
+        _check(_lib.gs_server_drain(self._h))
+
+    def info(self) -> dict:
+        h, p, i = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(_lib.gs_server_info(self._h, ctypes.byref(h), ctypes.byref(p), ctypes.byref(i)))
+        return {"held_bytes": h.value, "pending": p.value, "inflight": i.value}
+
+    def close(self) -> None:
+        if self._h:
+            _lib.gs_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def kron_edges(scale: int, e0: int, e1: int, seed1: int = 1, seed2: int = 2) -> np.ndarray:

@@ -30,8 +30,8 @@ __global__ void __launch_bounds__(SWEEP_WG) k_bfs_expand(SweepArgs a, const uint
                                                          const uint32_t *__restrict__ rowact,
                                                          unsigned long long *scanned) {
     __shared__ ChunkTiles s;
-    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
-    if (chunk >= a.nchunk) return;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [&](uint32_t rc) {
         return bit_of(rowact, rc >> 16) || (!DIRECTED && bit_of(rowact, rc & 0xFFFFu));
     });
@@ -120,19 +120,14 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     for (uint32_t level = 1;; ++level) {
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
         hipEvent_t kb;
-        if (g->nchunk) {
-            w->timer.begin(GS_K_BFS_EXPAND, w->stream, &kb);
+        GS_TRY(sweep(g, w, GS_K_BFS_EXPAND, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
             if (g->directed)
-                k_bfs_expand<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(
-                    sweep_args(g), w->bm_front.ptr, w->bm_vis.ptr, w->bm_next.ptr, w->rowact.ptr,
-                    w->counters.ptr + 1);
+                k_bfs_expand<true><<<grid, SWEEP_WG, 0, st>>>(a, w->bm_front.ptr, w->bm_vis.ptr, w->bm_next.ptr,
+                                                             w->rowact.ptr, w->counters.ptr + 1);
             else
-                k_bfs_expand<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(
-                    sweep_args(g), w->bm_front.ptr, w->bm_vis.ptr, w->bm_next.ptr, w->rowact.ptr,
-                    w->counters.ptr + 1);
-            GS_HIP(hipGetLastError());
-            w->timer.end(GS_K_BFS_EXPAND, w->stream, kb);
-        }
+                k_bfs_expand<false><<<grid, SWEEP_WG, 0, st>>>(a, w->bm_front.ptr, w->bm_vis.ptr, w->bm_next.ptr,
+                                                              w->rowact.ptr, w->counters.ptr + 1);
+        }));
         w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
         k_bfs_level_end<<<(unsigned)g->rows, 256, 0, w->stream>>>(w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
                                                                    w->rowact.ptr, w->depth.ptr, level + 1,

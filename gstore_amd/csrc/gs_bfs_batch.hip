@@ -31,8 +31,8 @@ __global__ void __launch_bounds__(SWEEP_WG) k_msbfs_expand(SweepArgs a, const un
                                                            const uint32_t *__restrict__ rowact,
                                                            unsigned long long *scanned) {
     __shared__ ChunkTiles s;
-    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
-    if (chunk >= a.nchunk) return;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [&](uint32_t rc) {
         return bit_of(rowact, rc >> 16) || (!DIRECTED && bit_of(rowact, rc & 0xFFFFu));
     });
@@ -115,7 +115,7 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
             set_error("gs_bfs_batch: root %llu >= N", (unsigned long long)roots[i]);
             return GS_ERR_ARG;
         }
-    GS_TRY(ensure_bitmaps(g, w));  // rowact
+    GS_TRY(ensure_rowact(g, w));
     if (w->ms_vis.n != g->n) {
         GS_TRY(w->ms_vis.alloc(g->n));
         GS_TRY(w->ms_front.alloc(g->n));
@@ -123,7 +123,7 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     }
     if (w->ms_depth.n < (size_t)nr * g->n) GS_TRY(w->ms_depth.alloc((size_t)nr * g->n));
     if (w->ms_roots.n < 64) GS_TRY(w->ms_roots.alloc(64));
-    GS_TRY(w->ms_counts.alloc(64));
+    if (w->ms_counts.n < 64) GS_TRY(w->ms_counts.alloc(64));
     w->timer.reset();
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
@@ -145,19 +145,14 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     for (uint32_t level = 1;; ++level) {
         GS_HIP(hipMemsetAsync(w->ms_counts.ptr, 0, 64 * 8, w->stream));
         hipEvent_t kb;
-        if (g->nchunk) {
-            w->timer.begin(GS_K_BFS_EXPAND, w->stream, &kb);
+        auto *fr = (unsigned long long *)w->ms_front.ptr, *vi = (unsigned long long *)w->ms_vis.ptr,
+             *nx = (unsigned long long *)w->ms_next.ptr;
+        GS_TRY(sweep(g, w, GS_K_BFS_EXPAND, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
             if (g->directed)
-                k_msbfs_expand<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(
-                    sweep_args(g), (unsigned long long *)w->ms_front.ptr, (unsigned long long *)w->ms_vis.ptr,
-                    (unsigned long long *)w->ms_next.ptr, w->rowact.ptr, w->counters.ptr);
+                k_msbfs_expand<true><<<grid, SWEEP_WG, 0, st>>>(a, fr, vi, nx, w->rowact.ptr, w->counters.ptr);
             else
-                k_msbfs_expand<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(
-                    sweep_args(g), (unsigned long long *)w->ms_front.ptr, (unsigned long long *)w->ms_vis.ptr,
-                    (unsigned long long *)w->ms_next.ptr, w->rowact.ptr, w->counters.ptr);
-            GS_HIP(hipGetLastError());
-            w->timer.end(GS_K_BFS_EXPAND, w->stream, kb);
-        }
+                k_msbfs_expand<false><<<grid, SWEEP_WG, 0, st>>>(a, fr, vi, nx, w->rowact.ptr, w->counters.ptr);
+        }));
         w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
         k_msbfs_level_end<<<(unsigned)g->rows, 256, 0, w->stream>>>(
             (unsigned long long *)w->ms_vis.ptr, (unsigned long long *)w->ms_front.ptr,

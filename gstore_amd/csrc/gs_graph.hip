@@ -62,6 +62,12 @@ int sync_and_collect(gs_ws *w) {
 
 uint64_t bitmap_words(const gs_graph *g) { return g->rows * ROW_WORDS; }
 
+int ensure_rowact(const gs_graph *g, gs_ws *w) {
+    uint64_t rw = (g->p_s + 31) / 32;
+    if (w->rowact.n != rw) GS_TRY(w->rowact.alloc(rw));
+    return GS_OK;
+}
+
 int ensure_bitmaps(const gs_graph *g, gs_ws *w) {
     uint64_t wd = bitmap_words(g);
     if (w->bm_vis.n != wd) {
@@ -69,9 +75,7 @@ int ensure_bitmaps(const gs_graph *g, gs_ws *w) {
         GS_TRY(w->bm_front.alloc(wd));
         GS_TRY(w->bm_next.alloc(wd));
     }
-    uint64_t rw = (g->p_s + 31) / 32;
-    if (w->rowact.n != rw) GS_TRY(w->rowact.alloc(rw));
-    return GS_OK;
+    return ensure_rowact(g, w);
 }
 
 int ws_create(gs_ws **out) {
@@ -112,6 +116,7 @@ size_t gs_ws::device_bytes() const {
 
 gs_graph::~gs_graph() {
     delete ws;
+    delete ss;
     if (stream) (void)hipStreamSynchronize(stream);
     if (h_counters) (void)hipHostFree(h_counters);
     if (stream) (void)hipStreamDestroy(stream);
@@ -176,8 +181,8 @@ int graph_build_meta(gs_graph *g, const uint64_t *st) {
 __global__ void __launch_bounds__(SWEEP_WG) k_validate(SweepArgs a, uint32_t n, int directed,
                                                        int32_t *gdeg, unsigned long long *bad) {
     __shared__ ChunkTiles s;
-    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
-    if (chunk >= a.nchunk) return;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
     unsigned long long nbad = 0;
     auto op = [&](uint32_t w, uint32_t rc) {
@@ -202,9 +207,18 @@ static int validate(gs_graph *g) {
     GS_TRY(gdeg.alloc(g->n));
     GS_HIP(hipMemsetAsync(gdeg.ptr, 0, g->n * 4, g->stream));
     GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 16 * 8, g->stream));
-    if (g->nchunk)
-        k_validate<<<xcd_grid(g->nchunk), SWEEP_WG, 0, g->stream>>>(sweep_args(g), (uint32_t)std::min<uint64_t>(g->n, 0xFFFFFFFFull),
-                                                                    g->directed, gdeg.ptr, g->counters.ptr);
+    {
+        // the validation sweep runs on the default workspace's stream
+        gs_ws *w = g->ws;
+        GS_HIP(hipStreamSynchronize(g->stream));
+        uint32_t n32 = (uint32_t)std::min<uint64_t>(g->n, 0xFFFFFFFFull);
+        GS_TRY(sweep(g, w, GS_K_CONVERT, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
+            k_validate<<<grid, SWEEP_WG, 0, st>>>(a, n32, g->directed, gdeg.ptr, g->counters.ptr);
+        }));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        w->timer.collect();
+        w->timer.reset();
+    }
     k_count_mismatch<<<1024, 256, 0, g->stream>>>(gdeg.ptr, g->degree.ptr, g->n, g->counters.ptr + 1);
     GS_HIP(hipGetLastError());
     GS_HIP(hipMemcpyAsync(g->h_counters, g->counters.ptr, 16, hipMemcpyDeviceToHost, g->stream));
@@ -513,7 +527,7 @@ size_t gs_last_error(char *buf, size_t len) {
 const char *gs_kernel_name(int k) {
     static const char *names[GS_K_COUNT] = {"pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end",
                                             "wcc_hook", "wcc_compress", "wcc_check", "kcore_peel",
-                                            "kcore_scan", "kron_gen", "convert"};
+                                            "kcore_scan", "kron_gen", "convert", "h2d_copy"};
     return (k >= 0 && k < GS_K_COUNT) ? names[k] : "?";
 }
 
@@ -554,6 +568,7 @@ int gs_open(const char *prefix, uint64_t n, int directed, uint64_t budget, gs_gr
     std::vector<uint64_t> st(g->T + 1);
     GS_TRY(read_file(pre + ".start", st.data(), sz));
     g->E = st[g->T];
+    g->nchunk = (g->E + CHUNK - 1) / CHUNK;
     GS_TRY(file_size(pre + ".grid", &sz));
     if (sz != g->E * 4) {
         set_error("%s.grid holds %llu bytes; .start sentinel says %llu edges", prefix, (unsigned long long)sz,
@@ -566,14 +581,22 @@ int gs_open(const char *prefix, uint64_t n, int directed, uint64_t budget, gs_gr
                   (unsigned long long)(n * 4));
         return GS_ERR_FORMAT;
     }
-    if (budget && g->E * 4 > budget) {
-        set_error("grid of %llu bytes exceeds the device budget of %llu bytes; streaming mode is not built yet",
-                  (unsigned long long)(g->E * 4), (unsigned long long)budget);
-        return GS_ERR_UNSUPPORTED;
+    const bool streaming = budget && g->E * 4 > budget;
+    GS_TRY(g->degree.alloc(n));
+    if (streaming) {
+        // semi-external: the grid goes to pinned host memory, a prefix to HBM
+        uint32_t *h = nullptr;
+        GS_HIP(hipHostMalloc((void **)&h, std::max<uint64_t>(g->E, 1) * 4, hipHostMallocDefault));
+        int rs = read_file(pre + ".grid", h, g->E * 4);
+        if (rs == GS_OK) rs = stream_setup(g, h, budget, 0);
+        if (rs != GS_OK) {
+            (void)hipHostFree(h);
+            return rs;
+        }
+    } else {
+        GS_TRY(g->grid.alloc(g->E));
     }
     // stage through pinned host memory in 256 MiB pieces
-    GS_TRY(g->grid.alloc(g->E));
-    GS_TRY(g->degree.alloc(n));
     {
         const uint64_t piece = 1ull << 28;
         char *pin = nullptr;
@@ -600,7 +623,7 @@ int gs_open(const char *prefix, uint64_t n, int directed, uint64_t budget, gs_gr
             fclose(f);
             return GS_OK;
         };
-        GS_TRY(stage(pre + ".grid", g->grid.ptr, g->E * 4));
+        if (!streaming) GS_TRY(stage(pre + ".grid", g->grid.ptr, g->E * 4));
         GS_TRY(stage(pre + ".degree", g->degree.ptr, n * 4));
     }
     GS_TRY(open_common(g, st.data()));
@@ -669,9 +692,12 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->groups = g->p;
     info->tiles_per_side = g->p_s;
     info->directed = g->directed;
-    info->resident = 1;
+    info->resident = g->ss ? 0 : 1;
+    info->resident_edges = g->ss ? std::min<uint64_t>(g->E, g->ss->res_chunks * CHUNK) : g->E;
+    info->segment_edges = g->ss ? g->ss->seg_chunks * CHUNK : 0;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
-                         g->degree.bytes() + g->ws->device_bytes();
+                         g->degree.bytes() + g->ws->device_bytes() +
+                         (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
     return GS_OK;
 }
 
@@ -698,7 +724,10 @@ int gs_get_arrays(gs_graph *g, uint64_t *start, uint32_t *grid, int32_t *degree)
         GS_TRY(rebuild_start(g, &st));
         memcpy(start, st.data(), (g->T + 1) * 8);
     }
-    if (grid && g->E) GS_HIP(hipMemcpy(grid, g->grid.ptr, g->E * 4, hipMemcpyDeviceToHost));
+    if (grid && g->E) {
+        if (g->ss) memcpy(grid, g->ss->h_grid, g->E * 4);
+        else GS_HIP(hipMemcpy(grid, g->grid.ptr, g->E * 4, hipMemcpyDeviceToHost));
+    }
     if (degree) GS_HIP(hipMemcpy(degree, g->degree.ptr, g->n * 4, hipMemcpyDeviceToHost));
     return GS_OK;
 }
@@ -707,12 +736,12 @@ int gs_save(gs_graph *g, const char *prefix) {
     if (!g || !prefix) { set_error("null argument"); return GS_ERR_ARG; }
     std::vector<uint64_t> st;
     GS_TRY(rebuild_start(g, &st));
-    std::vector<uint32_t> grid(g->E);
+    std::vector<uint32_t> grid(g->ss ? 0 : g->E);
     std::vector<int32_t> deg(g->n);
-    if (g->E) GS_HIP(hipMemcpy(grid.data(), g->grid.ptr, g->E * 4, hipMemcpyDeviceToHost));
+    if (g->E && !g->ss) GS_HIP(hipMemcpy(grid.data(), g->grid.ptr, g->E * 4, hipMemcpyDeviceToHost));
     GS_HIP(hipMemcpy(deg.data(), g->degree.ptr, g->n * 4, hipMemcpyDeviceToHost));
     std::string pre(prefix);
-    GS_TRY(write_file(pre + ".grid", grid.data(), g->E * 4));
+    GS_TRY(write_file(pre + ".grid", g->ss ? g->ss->h_grid : grid.data(), g->E * 4));
     GS_TRY(write_file(pre + ".start", st.data(), (g->T + 1) * 8));
     GS_TRY(write_file(pre + ".degree", deg.data(), g->n * 4));
     return GS_OK;

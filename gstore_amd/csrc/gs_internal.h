@@ -127,6 +127,22 @@ struct gs_ws {
     size_t device_bytes() const;
 };
 
+namespace gs {
+// Semi-external mode: the grid lives in pinned host memory; the first
+// res_chunks chunks stay resident in HBM and the rest pass through two
+// device segment buffers, copied on a side stream while the previous
+// segment is being swept.
+struct StreamState {
+    uint32_t *h_grid = nullptr;  // pinned host copy of the whole grid
+    uint64_t res_chunks = 0;     // chunks [0, res_chunks) resident in gs_graph::grid
+    uint64_t seg_chunks = 0;     // chunks per segment buffer
+    DevBuf<uint32_t> seg[2];
+    hipStream_t copy = nullptr;
+    hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
+    ~StreamState();
+};
+}  // namespace gs
+
 struct gs_graph {
     int device = 0;
     hipStream_t stream = nullptr;            // loading / conversion
@@ -141,6 +157,7 @@ struct gs_graph {
     gs::DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;
     gs_ws *ws = nullptr;                     // default workspace of the plain entry points
+    gs::StreamState *ss = nullptr;           // non-null in semi-external mode
     ~gs_graph();
 };
 
@@ -150,6 +167,8 @@ int graph_create(gs_graph **out);
 int ws_create(gs_ws **out);
 int graph_build_meta(gs_graph *g, const uint64_t *h_start);  // tz/chunk lists, from host .start
 int ensure_bitmaps(const gs_graph *g, gs_ws *w);
+int ensure_rowact(const gs_graph *g, gs_ws *w);
+int stream_setup(gs_graph *g, uint32_t *h_grid, uint64_t budget, uint64_t seg_bytes);
 uint64_t bitmap_words(const gs_graph *g);  // rows * ROW_WORDS
 int sync_and_collect(gs_ws *w);
 
@@ -184,16 +203,29 @@ struct ChunkTiles {
     int any;
 };
 
+// One sweep launch covers chunks [chunk0, chunk0 + nlaunch); `grid` holds the
+// edges of chunks from grid_chunk0 on (the whole grid when resident, one
+// staged segment when streaming).
 struct SweepArgs {
     const uint32_t *grid;
     const uint64_t *tz_beg;
     const uint32_t *tz_rc;
     const uint32_t *chunk_tz;
     uint64_t E, nchunk;
+    uint64_t grid_chunk0, chunk0, nlaunch;
 };
 
-inline SweepArgs sweep_args(const gs_graph *g) {
-    return SweepArgs{g->grid.ptr, g->tz_beg.ptr, g->tz_rc.ptr, g->chunk_tz.ptr, g->E, g->nchunk};
+inline SweepArgs sweep_args(const gs_graph *g, const uint32_t *grid, uint64_t grid_chunk0, uint64_t chunk0,
+                            uint64_t nlaunch) {
+    return SweepArgs{grid, g->tz_beg.ptr, g->tz_rc.ptr, g->chunk_tz.ptr, g->E, g->nchunk, grid_chunk0, chunk0, nlaunch};
+}
+
+// Chunk of this workgroup, false if the workgroup is past the launch's range.
+__device__ inline bool launch_chunk(const SweepArgs &a, uint64_t &chunk) {
+    uint64_t c = xcd_chunk(blockIdx.x, a.nlaunch);
+    if (c >= a.nlaunch) return false;
+    chunk = a.chunk0 + c;
+    return true;
 }
 
 __device__ inline bool bit_of(const uint32_t *bm, uint32_t rc_row) {
@@ -258,8 +290,7 @@ __device__ inline int find_tile(const ChunkTiles &s, uint32_t off) {
 template <bool SKIP_INACTIVE, class Op>
 __device__ inline void sweep_chunk(const SweepArgs &a, uint64_t chunk, const ChunkTiles &s,
                                    uint32_t len, Op &op) {
-    const uint64_t base = chunk * (uint64_t)CHUNK;
-    const uint32_t *g = a.grid + base;
+    const uint32_t *g = a.grid + (chunk - a.grid_chunk0) * (uint64_t)CHUNK;
 #pragma unroll 1
     for (int q = 0; q < QUADS; ++q) {
         uint32_t off = (uint32_t)(q * SWEEP_WG + threadIdx.x) * 4u;
@@ -289,5 +320,41 @@ __device__ inline void sweep_chunk(const SweepArgs &a, uint64_t chunk, const Chu
 // Global ids of an edge word in tile rc: row << 16 | v0_local, col << 16 | v1_local.
 __device__ inline uint32_t gv0(uint32_t w, uint32_t rc) { return (rc & 0xFFFF0000u) | (w & 0xFFFFu); }
 __device__ inline uint32_t gv1(uint32_t w, uint32_t rc) { return (rc << 16) | (w >> 16); }
+
+// Launch one sweep over every chunk of the grid: one launch when resident;
+// when streaming, the resident prefix first, then each segment after its
+// host-to-device copy, double-buffered so copy k+1 overlaps sweep k.
+// launch(SweepArgs, grid_size, stream) issues the kernel.
+template <class F>
+int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch) {
+    if (!g->nchunk) return GS_OK;
+    auto run = [&](const uint32_t *grid, uint64_t gc0, uint64_t c0, uint64_t cn) -> int {
+        hipEvent_t kb;
+        w->timer.begin(kclass, w->stream, &kb);
+        launch(sweep_args(g, grid, gc0, c0, cn), (unsigned)xcd_grid(cn), w->stream);
+        GS_HIP(hipGetLastError());
+        w->timer.end(kclass, w->stream, kb);
+        return GS_OK;
+    };
+    StreamState *ss = g->ss;
+    if (!ss) return run(g->grid.ptr, 0, 0, g->nchunk);
+    if (ss->res_chunks) GS_TRY(run(g->grid.ptr, 0, 0, ss->res_chunks));
+    int k = 0;
+    for (uint64_t c0 = ss->res_chunks; c0 < g->nchunk; c0 += ss->seg_chunks, k ^= 1) {
+        uint64_t cn = std::min(ss->seg_chunks, g->nchunk - c0);
+        uint64_t e0 = c0 * CHUNK, e1 = std::min(g->E, (c0 + cn) * CHUNK);
+        GS_HIP(hipStreamWaitEvent(ss->copy, ss->ev_done[k], 0));
+        hipEvent_t cb;
+        w->timer.begin(GS_K_H2D, ss->copy, &cb);
+        GS_HIP(hipMemcpyAsync(ss->seg[k].ptr, ss->h_grid + e0, (e1 - e0) * 4, hipMemcpyHostToDevice, ss->copy));
+        w->timer.end(GS_K_H2D, ss->copy, cb);
+        GS_HIP(hipEventRecord(ss->ev_copied[k], ss->copy));
+        GS_HIP(hipStreamWaitEvent(w->stream, ss->ev_copied[k], 0));
+        GS_TRY(run(ss->seg[k].ptr, c0, c0, cn));
+        GS_HIP(hipEventRecord(ss->ev_done[k], w->stream));
+        w->stats.h2d_bytes += (e1 - e0) * 4;
+    }
+    return GS_OK;
+}
 
 }  // namespace gs

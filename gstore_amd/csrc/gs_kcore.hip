@@ -57,8 +57,8 @@ __global__ void __launch_bounds__(SWEEP_WG) k_kc_scan(SweepArgs a, const uint32_
                                                       const uint32_t *__restrict__ rowact,
                                                       unsigned long long *scanned) {
     __shared__ ChunkTiles s;
-    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
-    if (chunk >= a.nchunk) return;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [&](uint32_t rc) {
         return bit_of(rowact, rc >> 16) || bit_of(rowact, rc & 0xFFFFu);
     });
@@ -121,11 +121,9 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
             set_error("gs_kcore: still removing vertices after %d rounds", nr);
             return GS_ERR_DEVICE;
         }
-        w->timer.begin(GS_K_KCORE_SCAN, w->stream, &kb);
-        k_kc_scan<<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->bm_front.ptr, w->kc_deg.ptr,
-                                                                   w->rowact.ptr, w->counters.ptr + 2);
-        GS_HIP(hipGetLastError());
-        w->timer.end(GS_K_KCORE_SCAN, w->stream, kb);
+        GS_TRY(sweep(g, w, GS_K_KCORE_SCAN, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
+            k_kc_scan<<<grid, SWEEP_WG, 0, st>>>(a, w->bm_front.ptr, w->kc_deg.ptr, w->rowact.ptr, w->counters.ptr + 2);
+        }));
     }
     GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 16, w->stream));
     k_kc_count<<<(unsigned)std::min<uint64_t>((g->n + 255) / 256, 8192), 256, 0, w->stream>>>(w->kc_deg.ptr, g->n,

@@ -33,8 +33,8 @@ __global__ void k_pr_init(const int32_t *__restrict__ deg, uint64_t n, int mode,
 template <bool DIRECTED>
 __global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter(SweepArgs a, const float *__restrict__ prior, float *acc) {
     __shared__ ChunkTiles s;
-    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
-    if (chunk >= a.nchunk) return;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
     auto op = [&](uint32_t w, uint32_t rc) {
         uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
@@ -107,17 +107,10 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
     GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
-        if (g->nchunk) {
-            w->timer.begin(GS_K_PR_SCATTER, w->stream, &kb);
-            if (g->directed)
-                k_pr_scatter<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->pr_prior.ptr,
-                                                                                    w->pr_acc.ptr);
-            else
-                k_pr_scatter<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->pr_prior.ptr,
-                                                                                     w->pr_acc.ptr);
-            GS_HIP(hipGetLastError());
-            w->timer.end(GS_K_PR_SCATTER, w->stream, kb);
-        }
+        GS_TRY(sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
+            if (g->directed) k_pr_scatter<true><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
+            else k_pr_scatter<false><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
+        }));
         w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
         k_pr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
         GS_HIP(hipGetLastError());
@@ -162,17 +155,10 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
     GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
-        if (g->nchunk) {
-            w->timer.begin(GS_K_PR_SCATTER, w->stream, &kb);
-            if (g->directed)
-                k_pr_scatter<true><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->pr_prior.ptr,
-                                                                                    w->pr_acc.ptr);
-            else
-                k_pr_scatter<false><<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->pr_prior.ptr,
-                                                                                     w->pr_acc.ptr);
-            GS_HIP(hipGetLastError());
-            w->timer.end(GS_K_PR_SCATTER, w->stream, kb);
-        }
+        GS_TRY(sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
+            if (g->directed) k_pr_scatter<true><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
+            else k_pr_scatter<false><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
+        }));
         w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
         k_ppr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, src, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr,
                                                            it + 1 == iters ? w->ppr_rank.ptr : nullptr);

@@ -65,19 +65,48 @@ struct Worker {
     bool busy = false;
 };
 
-// Workspace bytes a query needs on graph g (what run_* allocates).
+// Workspace bytes a query of `kind` needs on graph g: exactly what run_*
+// allocates (plus the 128-byte counter block every workspace has).
 static uint64_t need_bytes(const gs_graph *g, int kind, int lanes) {
-    uint64_t n = g->n, bm = g->rows * ROW_WORDS * 4 * 3 + (g->p_s + 31) / 32 * 4;
+    const uint64_t n = g->n, rowact = (g->p_s + 31) / 32 * 4, base = 16 * 8;
     switch (kind) {
-        case GS_Q_BFS: return 24 * n + (uint64_t)lanes * n + bm + 1024;
-        case GS_Q_PPR: return 16 * n;
-        case GS_Q_PAGERANK: return 12 * n;
-        case GS_Q_WCC: return 4 * n;
-        default: return 0;
+        case GS_Q_BFS: return base + 24 * n + (uint64_t)lanes * n + rowact + 64 * 8 * 2;
+        case GS_Q_PPR: return base + 16 * n;
+        case GS_Q_PAGERANK: return base + 12 * n;
+        case GS_Q_WCC: return base + 4 * n;
+        default: return base;
     }
 }
 
 static uint64_t ws_bytes(const gs_ws *w) { return w->device_bytes(); }
+
+// Bytes of the buffers a query of `kind` reuses.
+static uint64_t kind_bytes(const gs_ws *w, int kind) {
+    uint64_t b = w->counters.bytes();
+    switch (kind) {
+        case GS_Q_BFS:
+            return b + w->ms_vis.bytes() + w->ms_front.bytes() + w->ms_next.bytes() + w->ms_depth.bytes() +
+                   w->rowact.bytes() + w->ms_roots.bytes() + w->ms_counts.bytes();
+        case GS_Q_PPR: return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes() + w->ppr_rank.bytes();
+        case GS_Q_PAGERANK: return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes();
+        case GS_Q_WCC: return b + w->wcc_parent.bytes();
+        default: return b;
+    }
+}
+
+// Release the buffers a query of `kind` does not use.
+static void trim(gs_ws *w, int kind) {
+    if (kind != GS_Q_BFS) {
+        w->ms_vis.release(); w->ms_front.release(); w->ms_next.release(); w->ms_depth.release();
+        w->rowact.release(); w->ms_roots.release(); w->ms_counts.release();
+    }
+    if (kind != GS_Q_PPR && kind != GS_Q_PAGERANK) {
+        w->pr_prior.release(); w->pr_acc.release(); w->pr_fac.release();
+    }
+    if (kind != GS_Q_PPR) w->ppr_rank.release();
+    if (kind != GS_Q_WCC) w->wcc_parent.release();
+    w->depth.release(); w->bm_vis.release(); w->bm_front.release(); w->bm_next.release(); w->kc_deg.release();
+}
 
 // Free idle workspaces (not the caller's) until `want` more bytes fit.  Lock held.
 static void make_room(gs_server *s, Worker *self, uint64_t want) {
@@ -106,7 +135,7 @@ static void worker_main(Worker *wk) {
     std::vector<uint64_t> counts;
     for (;;) {
         std::vector<Query> batch;
-        uint64_t need = 0;
+        uint64_t need = 0, have_all = 0;
         gs_ws *w = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
@@ -124,7 +153,9 @@ static void worker_main(Worker *wk) {
                         while (lanes > 1 && need_bytes(g, GS_Q_BFS, lanes) > s->budget) --lanes;
                     }
                     auto it = wk->ws.find(q.graph);
-                    uint64_t have = it != wk->ws.end() && it->second ? ws_bytes(it->second.get()) : 0;
+                    const gs_ws *cur = it != wk->ws.end() ? it->second.get() : nullptr;
+                    have_all = cur ? ws_bytes(cur) : 0;
+                    uint64_t have = cur ? kind_bytes(cur, q.kind) : 0;
                     uint64_t want = need_bytes(g, q.kind, lanes);
                     need = want > have ? want - have : 0;
                     if (s->held + need > s->budget) make_room(s, wk, need);
@@ -162,6 +193,7 @@ static void worker_main(Worker *wk) {
             slot.reset(nw);
         }
         w = slot.get();
+        if (w) trim(w, kind);
         double t_start = now_ms();
         std::vector<gs_query_result> res(batch.size());
         for (size_t i = 0; i < batch.size(); ++i) {
@@ -174,7 +206,6 @@ static void worker_main(Worker *wk) {
             res[i].t_submit_ms = batch[i].t_submit - s->t0;
             res[i].t_start_ms = t_start - s->t0;
         }
-        uint64_t before = w ? ws_bytes(w) : 0;
         if (status == GS_OK) {
             switch (kind) {
                 case GS_Q_BFS: {
@@ -222,9 +253,10 @@ static void worker_main(Worker *wk) {
         double t_done = now_ms();
         {
             std::lock_guard<std::mutex> lk(s->mu);
+            // replace the reservation and the workspace's old size by what it holds now
             uint64_t after = w ? ws_bytes(w) : 0;
-            // settle the reservation against what the workspace really holds
-            s->held = s->held - std::min(s->held, need) - std::min(s->held - std::min(s->held, need), before) + after;
+            uint64_t drop = std::min(s->held, need + have_all);
+            s->held = s->held - drop + after;
             for (auto &r : res) {
                 r.status = status;
                 r.t_done_ms = t_done - s->t0;
@@ -261,6 +293,7 @@ int gs_server_create(uint64_t budget, int workers, gs_server **out) {
 
 int gs_server_add_graph(gs_server *s, gs_graph *g, int *graph_id) {
     if (!s || !g || !graph_id) { set_error("null argument"); return GS_ERR_ARG; }
+    if (g->ss) { set_error("the server keeps graphs resident; this one is in semi-external mode"); return GS_ERR_UNSUPPORTED; }
     std::lock_guard<std::mutex> lk(s->mu);
     s->graphs.push_back(g);
     *graph_id = (int)s->graphs.size() - 1;

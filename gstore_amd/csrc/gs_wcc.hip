@@ -41,8 +41,8 @@ __global__ void k_wcc_init(uint32_t *par, uint64_t n) {
 
 __global__ void __launch_bounds__(SWEEP_WG) k_wcc_hook(SweepArgs a, uint32_t *par) {
     __shared__ ChunkTiles s;
-    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
-    if (chunk >= a.nchunk) return;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
     auto op = [&](uint32_t w, uint32_t rc) {
         uint32_t ru = wcc_find(par, gv0(w, rc)), rv = wcc_find(par, gv1(w, rc));
@@ -68,8 +68,8 @@ __global__ void k_wcc_compress(uint32_t *par, uint64_t n) {
 __global__ void __launch_bounds__(SWEEP_WG) k_wcc_check(SweepArgs a, const uint32_t *__restrict__ par,
                                                         unsigned long long *bad) {
     __shared__ ChunkTiles s;
-    uint64_t chunk = xcd_chunk(blockIdx.x, a.nchunk);
-    if (chunk >= a.nchunk) return;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
     uint32_t nb = 0;
     auto op = [&](uint32_t w, uint32_t rc) { nb += par[gv0(w, rc)] != par[gv1(w, rc)]; };
@@ -109,12 +109,9 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
             return GS_ERR_DEVICE;
         }
         hipEvent_t kb;
-        if (g->nchunk) {
-            w->timer.begin(GS_K_WCC_HOOK, w->stream, &kb);
-            k_wcc_hook<<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->wcc_parent.ptr);
-            GS_HIP(hipGetLastError());
-            w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
-        }
+        GS_TRY(sweep(g, w, GS_K_WCC_HOOK, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
+            k_wcc_hook<<<grid, SWEEP_WG, 0, st>>>(a, w->wcc_parent.ptr);
+        }));
         ++passes;
         w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
         k_wcc_compress<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
@@ -122,10 +119,9 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
         w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
         if (!g->nchunk) break;
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
-        w->timer.begin(GS_K_WCC_CHECK, w->stream, &kb);
-        k_wcc_check<<<xcd_grid(g->nchunk), SWEEP_WG, 0, w->stream>>>(sweep_args(g), w->wcc_parent.ptr, w->counters.ptr);
-        GS_HIP(hipGetLastError());
-        w->timer.end(GS_K_WCC_CHECK, w->stream, kb);
+        GS_TRY(sweep(g, w, GS_K_WCC_CHECK, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
+            k_wcc_check<<<grid, SWEEP_WG, 0, st>>>(a, w->wcc_parent.ptr, w->counters.ptr);
+        }));
         GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 8, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
         if (w->h_counters[0] == 0) break;

@@ -42,7 +42,7 @@ enum gs_pr_mode {
 enum gs_kernel {
     GS_K_PR_SCATTER = 0, GS_K_PR_FINALIZE, GS_K_BFS_EXPAND, GS_K_BFS_LEVEL_END,
     GS_K_WCC_HOOK, GS_K_WCC_COMPRESS, GS_K_WCC_CHECK, GS_K_KCORE_PEEL, GS_K_KCORE_SCAN,
-    GS_K_GEN, GS_K_CONVERT, GS_K_COUNT
+    GS_K_GEN, GS_K_CONVERT, GS_K_H2D, GS_K_COUNT
 };
 
 typedef struct gs_info {
@@ -53,6 +53,8 @@ typedef struct gs_info {
     uint64_t groups;          /* p, physical groups per side */
     uint64_t tiles_per_side;  /* p_s = 256 p */
     uint64_t device_bytes;    /* device memory held by the handle */
+    uint64_t resident_edges;  /* edges kept in HBM (all of them when resident) */
+    uint64_t segment_edges;   /* edges per streamed segment (0 when resident) */
     int32_t directed;
     int32_t resident;         /* 1: whole grid in HBM */
 } gs_info;
@@ -62,6 +64,7 @@ typedef struct gs_stats {
     uint64_t iterations;           /* PR iterations, BFS levels, WCC passes, k-core rounds */
     uint64_t edges_scanned;        /* stored edges visited by edge sweeps (all iterations) */
     uint64_t tiles_scanned;
+    uint64_t h2d_bytes;            /* grid bytes streamed from host memory (semi-external mode) */
     double kern_ms[GS_K_COUNT];    /* per-class device time (kernel timing on) */
     uint64_t kern_calls[GS_K_COUNT];
 } gs_stats;
@@ -105,6 +108,18 @@ int gs_convert(const uint32_t *edges, uint64_t tuples, uint64_t vert_count, int 
 int gs_save(gs_graph *g, const char *prefix);
 
 int gs_get_info(gs_graph *g, gs_info *info);
+
+/* Cap the HBM used by the tile grid (semi-external mode, replacing the
+ * reference's libaio segment streaming, gstore.cpp:3044-3946, and its
+ * cross-iteration cache, :1491-1768).  If the grid is larger than
+ * budget_bytes it moves to pinned host memory; the first
+ * budget - 2 * segment_bytes stay resident in HBM (swept first, never
+ * copied again) and the rest is copied through two segment buffers on a
+ * side stream, each copy overlapping the sweep of the previous segment.
+ * segment_bytes 0 = min(512 MiB, budget / 8).  gs_open with a budget below the
+ * grid size opens in this mode directly.  A streaming graph serves one call
+ * at a time. */
+int gs_set_device_budget(gs_graph *g, uint64_t budget_bytes, uint64_t segment_bytes);
 
 /* Copy the tile arrays back to host buffers (start: T+1, grid: E, degree: N). */
 int gs_get_arrays(gs_graph *g, uint64_t *start, uint32_t *grid, int32_t *degree);

@@ -1,0 +1,65 @@
+"""Semi-external mode: the grid in pinned host memory, a resident prefix in
+HBM and the rest streamed through two segment buffers.  Every algorithm must
+give the resident result (integer outputs bit for bit, PageRank compat bit for
+bit, intended within FP32 tolerance)."""
+import numpy as np
+import pytest
+
+import gstore_amd as ga
+import oracle
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+CH = 4096 * 4   # bytes per chunk
+
+
+def _say(*a):
+    print("[gpu-stream]", *a, flush=True)
+
+
+@pytest.mark.parametrize("res_chunks", [0, 5])
+def test_streaming_equals_resident(res_chunks):
+    n, e = graphs.kron(15)
+    start, grid, deg = oracle.convert(e, n)
+    seg = 3 * CH
+    budget = 2 * seg + res_chunks * CH + CH
+    with ga.Graph.from_arrays(n, start, grid, deg) as ref, ga.Graph.from_arrays(n, start, grid, deg) as g:
+        g.set_device_budget(budget, seg)
+        assert not g.resident and g.resident_edges == min(g.E, (res_chunks + 1) * 4096)
+        _say("streaming, resident edges", g.resident_edges, "of", g.E)
+        assert np.array_equal(g.pagerank(5, ga.PR_COMPAT), ref.pagerank(5, ga.PR_COMPAT))
+        assert np.max(np.abs(g.pagerank(10, ga.PR_INTENDED) - ref.pagerank(10, ga.PR_INTENDED))) <= 1e-6
+        st = g.stats()
+        assert st["h2d_bytes"] >= 10 * 4 * (g.E - g.resident_edges)
+        for root in (0, int(np.argmax(deg))):
+            assert np.array_equal(g.bfs(root).depth, ref.bfs(root).depth)
+        assert np.array_equal(g.bfs_batch([1, 2, 3])[0], ref.bfs_batch([1, 2, 3])[0])
+        assert np.array_equal(g.wcc()[0], ref.wcc()[0])
+        a, b = g.kcore(8), ref.kcore(8)
+        assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
+        assert np.max(np.abs(g.ppr(5) - ref.ppr(5))) <= 1e-6
+        s2, g2, d2 = g.arrays()
+        assert np.array_equal(s2, start) and np.array_equal(g2, grid)
+
+
+def test_open_with_budget(tmp_path):
+    n, e = graphs.kron(14)
+    pre = str(tmp_path / "k14")
+    ga.convert(e, n, pre)
+    grid = np.fromfile(pre + ".grid", np.uint32)
+    with ga.Graph.open(pre, n, device_budget=4 * CH) as g, ga.Graph.open(pre, n) as ref:
+        assert not g.resident and ref.resident
+        assert np.array_equal(g.bfs(3).depth, ref.bfs(3).depth)
+        assert np.array_equal(g.pagerank(3, ga.PR_COMPAT), ref.pagerank(3, ga.PR_COMPAT))
+    with pytest.raises(ga.GStoreError):
+        ga.Graph.open(pre, n, device_budget=CH)          # below two segments
+    assert grid.size > 4 * 4096
+
+
+def test_streaming_graph_not_served():
+    n, e = graphs.kron(12)
+    arrays = oracle.convert(e, n)
+    with ga.Graph.from_arrays(n, *arrays) as g, ga.Server(workers=1) as srv:
+        g.set_device_budget(2 * CH + CH, CH)
+        with pytest.raises(ga.GStoreError):
+            srv.add_graph(g)
