@@ -69,8 +69,162 @@ def cpu_baseline(scale, threads):
                       f"{dt:.2f} s"}
 
 
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    return round(xs[min(len(xs) - 1, int(q * len(xs)))], 3)
+
+
+def serve_mode(args):
+    """Batched serving (Kronecker scales --serve-scales resident together): BFS
+    saturation throughput, batched vs one root per launch, then Poisson arrivals
+    of a 70/20/10 BFS/PPR/WCC mix at swept rates, p50/p99 latency per kind."""
+    import threading
+    import gstore_amd as ga
+    scales = [int(x) for x in args.serve_scales.split(",")]
+    graphs = []
+    for s in scales:
+        t0 = time.time()
+        graphs.append(ga.Graph.kron(s))
+        log(f"serve: Kron-{s} resident ({time.time() - t0:.1f}s)")
+    degs = [g.degree() for g in graphs]
+    nz = [np.flatnonzero(d) for d in degs]
+    rng = np.random.default_rng(4242)
+    out = {"metric": "batched BFS queries/s, p50/p99 latency (serving)", "unit": "queries/s", "n_gpus": 1,
+           "higher_is_better": True, "data": "synthetic (Graph500 Kronecker, seeds 1,2)",
+           "config": {"workload": "serving", "scales": scales, "workers": args.workers}}
+
+    # 1. BFS saturation: the same K roots, one root per launch vs 64-lane batches
+    big = len(graphs) - 1
+    K = args.sat_queries
+    roots = rng.integers(0, graphs[big].n, K)
+    sat = {}
+    for label, max_batch, workers in (("one_root_per_launch", 1, 1), ("batched_64", 64, args.workers)):
+        with ga.Server(workers=workers) as srv:
+            gid = srv.add_graph(graphs[big])
+            srv.configure(max_batch=max_batch)
+            srv.submit(gid, ga.Q_BFS, int(roots[0]))     # warm the workspace
+            srv.drain()
+            srv.poll()
+            t = time.perf_counter()
+            for r in roots:
+                srv.submit(gid, ga.Q_BFS, int(r))
+            srv.drain()
+            dt = time.perf_counter() - t
+            res = srv.poll(K + 10)
+            lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
+            sat[label] = {"queries": K, "seconds": round(dt, 3), "qps": round(K / dt, 2),
+                          "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                          "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 2)}
+            log(f"serve: BFS saturation {label}: {K / dt:.1f} q/s")
+    sat["speedup"] = round(sat["batched_64"]["qps"] / sat["one_root_per_launch"]["qps"], 2)
+    out["bfs_saturation"] = {"graph_scale": scales[big], **sat}
+
+    # 2. Poisson mix
+    rates = [float(x) for x in args.serve_rates.split(",")]
+    mix = []
+    for lam in rates:
+        n_q = max(10, int(lam * args.serve_seconds))
+        gaps = rng.exponential(1.0 / lam, n_q)
+        plan = []
+        for i in range(n_q):
+            gi = int(rng.integers(0, len(graphs)))
+            u = rng.random()
+            kind = ga.Q_BFS if u < 0.7 else ga.Q_PPR if u < 0.9 else ga.Q_WCC
+            arg = int(rng.choice(nz[gi])) if kind == ga.Q_PPR else int(rng.integers(0, graphs[gi].n))
+            plan.append((gi, kind, arg))
+        with ga.Server(workers=args.workers) as srv:
+            gids = [srv.add_graph(g) for g in graphs]
+            t0 = time.perf_counter()
+            due = t0
+            for (gi, kind, arg), gap in zip(plan, gaps):
+                due += gap
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                srv.submit(gids[gi], kind, arg)
+            srv.drain()
+            dt = time.perf_counter() - t0
+            res = srv.poll(n_q + 10)
+        by = {}
+        for r in res:
+            by.setdefault(r["kind"], []).append(r["t_done_ms"] - r["t_submit_ms"])
+        names = {ga.Q_BFS: "bfs", ga.Q_PPR: "ppr", ga.Q_WCC: "wcc"}
+        allat = [x for v in by.values() for x in v]
+        mix.append({"offered_qps": lam, "queries": n_q, "achieved_qps": round(n_q / dt, 2),
+                    "p50_ms": pct(allat, 0.5), "p99_ms": pct(allat, 0.99),
+                    "per_kind": {names[k]: {"n": len(v), "p50_ms": pct(v, 0.5), "p99_ms": pct(v, 0.99)}
+                                 for k, v in by.items()},
+                    "bfs_mean_batch": round(float(np.mean([r["batch_size"] for r in res if r["kind"] == 0])), 2)})
+        log(f"serve: rate {lam}/s -> {n_q / dt:.1f} q/s, p99 {mix[-1]['p99_ms']} ms")
+    out["poisson_mix"] = mix
+    out["value"] = sat["batched_64"]["qps"]
+    print(json.dumps(out), flush=True)
+    for g in graphs:
+        g.close()
+
+
+def stream_mode(args):
+    """Semi-external: Kron-S grid in pinned host memory with the device tile
+    budget capped; PageRank iterations and BFS roots streamed."""
+    import gstore_amd as ga
+    t0 = time.time()
+    g = ga.Graph.kron(args.scale)
+    log(f"stream: Kron-{args.scale} generated ({time.time() - t0:.1f}s), E={g.E}")
+    t0 = time.time()
+    g.set_device_budget(args.budget_gb << 30, args.segment_mb << 20)
+    log(f"stream: moved to pinned host, resident edges {g.resident_edges} of {g.E} ({time.time() - t0:.1f}s)")
+    g.set_kernel_timing(True)
+    res = {}
+    g.pagerank(args.stream_pr_iters, ga.PR_INTENDED, copy_out=False)
+    st = g.stats()
+    k = st["kernels"]
+    kern_ms = sum(v["ms"] for n_, v in k.items() if n_ != "h2d_copy")
+    h2d_ms = k.get("h2d_copy", {}).get("ms", 0.0)
+    exposed = max(0.0, st["ms"] - kern_ms)
+    res["pagerank"] = {"iterations": args.stream_pr_iters, "ms": round(st["ms"], 1),
+                       "ms_per_iter": round(st["ms"] / args.stream_pr_iters, 1),
+                       "h2d_gb": round(st["h2d_bytes"] / 1e9, 2),
+                       "h2d_gbs_wall": round(st["h2d_bytes"] / (st["ms"] / 1e3) / 1e9, 2),
+                       "h2d_gbs_copy": round(st["h2d_bytes"] / (h2d_ms / 1e3) / 1e9, 2) if h2d_ms else None,
+                       "kernel_ms": round(kern_ms, 1), "copy_ms": round(h2d_ms, 1),
+                       "overlap_frac": round(1 - exposed / st["ms"], 4),
+                       "copy_hidden_frac": round(1 - exposed / h2d_ms, 4) if h2d_ms else None,
+                       "gteps": round(2 * g.E * args.stream_pr_iters / (st["ms"] / 1e3) / 1e9, 3)}
+    log(f"stream: pagerank {res['pagerank']}")
+    deg = g.degree()
+    roots = [int(r) for r in np.random.default_rng(5).choice(np.flatnonzero(deg), args.stream_bfs_roots, replace=False)]
+    bfs = []
+    for r in roots:
+        d = g.bfs(r, copy_out=False)
+        st = g.stats()
+        bfs.append({"root": r, "levels": st["iterations"], "ms": round(st["ms"], 1),
+                    "h2d_gb": round(st["h2d_bytes"] / 1e9, 2)})
+    res["bfs"] = bfs
+    out = {"metric": "semi-external PageRank GTEPS, H2D GB/s and overlap", "value": res["pagerank"]["gteps"],
+           "unit": "GTEPS", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic (Graph500 Kronecker, seeds 1,2)",
+           "config": {"workload": f"Kron-{args.scale} semi-external, device tile budget {args.budget_gb} GB",
+                      "scale": args.scale, "stored_edges": g.E, "resident_edges": g.resident_edges,
+                      "segment_edges": g.segment_edges},
+           **res}
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", choices=["main", "serve", "stream"], default="main")
+    ap.add_argument("--serve-scales", default="18,20,22,24,26,28")
+    ap.add_argument("--serve-rates", default="20,80,320")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--sat-queries", type=int, default=512)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--budget-gb", type=int, default=16)
+    ap.add_argument("--segment-mb", type=int, default=512)
+    ap.add_argument("--stream-pr-iters", type=int, default=10)
+    ap.add_argument("--stream-bfs-roots", type=int, default=4)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -86,6 +240,11 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.mode == "serve":
+        return serve_mode(args)
+    if args.mode == "stream":
+        return stream_mode(args)
 
     import torch
     import gstore_amd as ga

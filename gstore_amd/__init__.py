@@ -94,6 +94,8 @@ _SIGS = {
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
     "gs_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(_Stats)]),
     "gs_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_set_sweep_order": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_set_chunk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "gs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
     "gs_close": (None, [_P]),
     "gs_server_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_P)]),
@@ -231,6 +233,12 @@ class Graph:
 
     def save(self, prefix: str) -> None:
         _check(_lib.gs_save(self._h, prefix.encode()))
+
+    def set_chunk(self, edges: int) -> None:
+        _check(_lib.gs_set_chunk(self._h, edges))
+
+    def set_sweep_order(self, xcd_contiguous: bool) -> None:
+        _check(_lib.gs_set_sweep_order(self._h, int(xcd_contiguous)))
 
     def set_kernel_timing(self, on: bool) -> None:
         _check(_lib.gs_set_kernel_timing(self._h, int(on)))

@@ -159,20 +159,25 @@ int graph_build_meta(gs_graph *g, const uint64_t *st) {
     }
     g->nz = beg.size();
     beg.push_back(g->E);
-    g->nchunk = (g->E + CHUNK - 1) / CHUNK;
+    GS_TRY(g->tz_beg.alloc(beg.size()));
+    GS_TRY(g->tz_rc.alloc(std::max<size_t>(rc.size(), 1)));
+    GS_HIP(hipMemcpy(g->tz_beg.ptr, beg.data(), beg.size() * 8, hipMemcpyHostToDevice));
+    if (!rc.empty()) GS_HIP(hipMemcpy(g->tz_rc.ptr, rc.data(), rc.size() * 4, hipMemcpyHostToDevice));
+    return build_chunk_map(g, beg);
+}
+
+// chunk_tz[c] = non-empty tile holding edge c * g->chunk; chunk_tz[nchunk] = nz - 1.
+int build_chunk_map(gs_graph *g, const std::vector<uint64_t> &beg) {
+    g->nchunk = (g->E + g->chunk - 1) / g->chunk;
     std::vector<uint32_t> ct(g->nchunk + 1);
     uint64_t k = 0;
     for (uint64_t c = 0; c < g->nchunk; ++c) {
-        uint64_t e = c * CHUNK;
+        uint64_t e = c * g->chunk;
         while (beg[k + 1] <= e) ++k;
         ct[c] = (uint32_t)k;
     }
     ct[g->nchunk] = g->nz ? (uint32_t)(g->nz - 1) : 0;
-    GS_TRY(g->tz_beg.alloc(beg.size()));
-    GS_TRY(g->tz_rc.alloc(std::max<size_t>(rc.size(), 1)));
     GS_TRY(g->chunk_tz.alloc(ct.size()));
-    GS_HIP(hipMemcpy(g->tz_beg.ptr, beg.data(), beg.size() * 8, hipMemcpyHostToDevice));
-    if (!rc.empty()) GS_HIP(hipMemcpy(g->tz_rc.ptr, rc.data(), rc.size() * 4, hipMemcpyHostToDevice));
     GS_HIP(hipMemcpy(g->chunk_tz.ptr, ct.data(), ct.size() * 4, hipMemcpyHostToDevice));
     return GS_OK;
 }
@@ -568,7 +573,7 @@ int gs_open(const char *prefix, uint64_t n, int directed, uint64_t budget, gs_gr
     std::vector<uint64_t> st(g->T + 1);
     GS_TRY(read_file(pre + ".start", st.data(), sz));
     g->E = st[g->T];
-    g->nchunk = (g->E + CHUNK - 1) / CHUNK;
+    g->nchunk = (g->E + g->chunk - 1) / g->chunk;
     GS_TRY(file_size(pre + ".grid", &sz));
     if (sz != g->E * 4) {
         set_error("%s.grid holds %llu bytes; .start sentinel says %llu edges", prefix, (unsigned long long)sz,
@@ -693,8 +698,8 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->tiles_per_side = g->p_s;
     info->directed = g->directed;
     info->resident = g->ss ? 0 : 1;
-    info->resident_edges = g->ss ? std::min<uint64_t>(g->E, g->ss->res_chunks * CHUNK) : g->E;
-    info->segment_edges = g->ss ? g->ss->seg_chunks * CHUNK : 0;
+    info->resident_edges = g->ss ? std::min<uint64_t>(g->E, g->ss->res_chunks * g->chunk) : g->E;
+    info->segment_edges = g->ss ? g->ss->seg_chunks * g->chunk : 0;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
                          g->degree.bytes() + g->ws->device_bytes() +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
@@ -754,6 +759,25 @@ int gs_get_stats(gs_graph *g, gs_stats *st) {
         st->kern_ms[i] = g->ws->timer.ms[i];
         st->kern_calls[i] = g->ws->timer.calls[i];
     }
+    return GS_OK;
+}
+
+int gs_set_chunk(gs_graph *g, uint32_t edges) {
+    if (!g || edges < 4 * SWEEP_WG || edges > (uint32_t)CHUNK || edges % (4 * SWEEP_WG)) {
+        set_error("chunk must be a multiple of %d in [%d, %d]", 4 * SWEEP_WG, 4 * SWEEP_WG, CHUNK);
+        return GS_ERR_ARG;
+    }
+    if (g->ss) { set_error("the chunk size of a streaming graph is fixed"); return GS_ERR_UNSUPPORTED; }
+    if (edges == g->chunk) return GS_OK;
+    std::vector<uint64_t> beg(g->nz + 1);
+    GS_HIP(hipMemcpy(beg.data(), g->tz_beg.ptr, (g->nz + 1) * 8, hipMemcpyDeviceToHost));
+    g->chunk = edges;
+    return build_chunk_map(g, beg);
+}
+
+int gs_set_sweep_order(gs_graph *g, int xcd_contiguous) {
+    if (!g) { set_error("null graph"); return GS_ERR_ARG; }
+    g->xcd_order = xcd_contiguous ? 1 : 0;
     return GS_OK;
 }
 

@@ -6,9 +6,10 @@
 //                          tiles back to back in .start order (as on disk)
 //   tz_beg   u64[nz + 1]   first edge of every non-empty tile, tz_beg[nz] = E
 //   tz_rc    u32[nz]       tile row << 16 | tile column (global tile coords)
-//   chunk_tz u32[nchunk+1] non-empty tile holding edge c*CHUNK; last = nz-1
+//   chunk_tz u32[nchunk+1] non-empty tile holding edge c*chunk; last = nz-1
 //   degree   i32[N]        the .degree file
-// Edges are swept in fixed chunks of CHUNK edges, one workgroup per chunk;
+// Edges are swept in fixed chunks of `chunk` edges (1024..4096, a tuning
+// knob), one workgroup per chunk;
 // the tiles a chunk touches are staged in LDS so each lane finds its tile by
 // a short search and decodes the 16-bit local ids in registers.
 #pragma once
@@ -22,9 +23,8 @@
 
 namespace gs {
 
-constexpr int CHUNK = 4096;       // edges per workgroup sweep
+constexpr int CHUNK = 4096;       // largest (and default) edges per workgroup sweep
 constexpr int SWEEP_WG = 256;     // threads per sweep workgroup (4 waves)
-constexpr int QUADS = CHUNK / 4 / SWEEP_WG;  // 16-byte loads per lane
 constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr uint32_t ROW_WORDS = 65536 / 32;   // bitmap words per tile row
 
@@ -148,6 +148,7 @@ struct gs_graph {
     hipStream_t stream = nullptr;            // loading / conversion
     bool directed = false;
     uint64_t n = 0, p = 0, p_s = 0, T = 0, E = 0, nz = 0, nchunk = 0;
+    uint32_t chunk = gs::CHUNK;              // edges per sweep chunk (workgroup)
     uint64_t rows = 0;                       // tile rows holding vertices: ceil(N/65536)
     gs::DevBuf<uint32_t> grid;
     gs::DevBuf<uint64_t> tz_beg;
@@ -158,6 +159,7 @@ struct gs_graph {
     unsigned long long *h_counters = nullptr;
     gs_ws *ws = nullptr;                     // default workspace of the plain entry points
     gs::StreamState *ss = nullptr;           // non-null in semi-external mode
+    int xcd_order = 1;                       // sweep order, see gs_set_sweep_order
     ~gs_graph();
 };
 
@@ -166,6 +168,7 @@ namespace gs {
 int graph_create(gs_graph **out);
 int ws_create(gs_ws **out);
 int graph_build_meta(gs_graph *g, const uint64_t *h_start);  // tz/chunk lists, from host .start
+int build_chunk_map(gs_graph *g, const std::vector<uint64_t> &beg);  // chunk_tz for g->chunk
 int ensure_bitmaps(const gs_graph *g, gs_ws *w);
 int ensure_rowact(const gs_graph *g, gs_ws *w);
 int stream_setup(gs_graph *g, uint32_t *h_grid, uint64_t budget, uint64_t seg_bytes);
@@ -213,16 +216,19 @@ struct SweepArgs {
     const uint32_t *chunk_tz;
     uint64_t E, nchunk;
     uint64_t grid_chunk0, chunk0, nlaunch;
+    uint32_t chunk;  // edges per chunk, a multiple of 4 * SWEEP_WG, at most CHUNK
+    int xcd_order;   // 1: each XCD sweeps a contiguous eighth; 0: chunks in launch order
 };
 
 inline SweepArgs sweep_args(const gs_graph *g, const uint32_t *grid, uint64_t grid_chunk0, uint64_t chunk0,
                             uint64_t nlaunch) {
-    return SweepArgs{grid, g->tz_beg.ptr, g->tz_rc.ptr, g->chunk_tz.ptr, g->E, g->nchunk, grid_chunk0, chunk0, nlaunch};
+    return SweepArgs{grid, g->tz_beg.ptr, g->tz_rc.ptr, g->chunk_tz.ptr, g->E, g->nchunk,
+                     grid_chunk0, chunk0, nlaunch, g->chunk, g->xcd_order};
 }
 
 // Chunk of this workgroup, false if the workgroup is past the launch's range.
 __device__ inline bool launch_chunk(const SweepArgs &a, uint64_t &chunk) {
-    uint64_t c = xcd_chunk(blockIdx.x, a.nlaunch);
+    uint64_t c = a.xcd_order ? xcd_chunk(blockIdx.x, a.nlaunch) : blockIdx.x;
     if (c >= a.nlaunch) return false;
     chunk = a.chunk0 + c;
     return true;
@@ -236,8 +242,8 @@ __device__ inline bool bit_of(const uint32_t *bm, uint32_t rc_row) {
 // Returns chunk length (0 if the chunk is out of range).
 template <class ActFn>
 __device__ inline uint32_t stage_chunk(const SweepArgs &a, uint64_t chunk, ChunkTiles &s, ActFn act_fn) {
-    uint64_t base = chunk * (uint64_t)CHUNK;
-    uint32_t len = (uint32_t)min<uint64_t>((uint64_t)CHUNK, a.E - base);
+    uint64_t base = chunk * (uint64_t)a.chunk;
+    uint32_t len = (uint32_t)min<uint64_t>((uint64_t)a.chunk, a.E - base);
     uint32_t t0 = a.chunk_tz[chunk], t1 = a.chunk_tz[chunk + 1];
     int nt = (int)(t1 - t0) + 1;
     if (threadIdx.x == 0) {
@@ -290,9 +296,10 @@ __device__ inline int find_tile(const ChunkTiles &s, uint32_t off) {
 template <bool SKIP_INACTIVE, class Op>
 __device__ inline void sweep_chunk(const SweepArgs &a, uint64_t chunk, const ChunkTiles &s,
                                    uint32_t len, Op &op) {
-    const uint32_t *g = a.grid + (chunk - a.grid_chunk0) * (uint64_t)CHUNK;
+    const uint32_t *g = a.grid + (chunk - a.grid_chunk0) * (uint64_t)a.chunk;
+    const int quads = (int)(a.chunk / (4 * SWEEP_WG));
 #pragma unroll 1
-    for (int q = 0; q < QUADS; ++q) {
+    for (int q = 0; q < quads; ++q) {
         uint32_t off = (uint32_t)(q * SWEEP_WG + threadIdx.x) * 4u;
         if (off >= len) break;
         int k = find_tile(s, off);
@@ -331,7 +338,7 @@ int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch) {
     auto run = [&](const uint32_t *grid, uint64_t gc0, uint64_t c0, uint64_t cn) -> int {
         hipEvent_t kb;
         w->timer.begin(kclass, w->stream, &kb);
-        launch(sweep_args(g, grid, gc0, c0, cn), (unsigned)xcd_grid(cn), w->stream);
+        launch(sweep_args(g, grid, gc0, c0, cn), (unsigned)(g->xcd_order ? xcd_grid(cn) : cn), w->stream);
         GS_HIP(hipGetLastError());
         w->timer.end(kclass, w->stream, kb);
         return GS_OK;
@@ -342,7 +349,7 @@ int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch) {
     int k = 0;
     for (uint64_t c0 = ss->res_chunks; c0 < g->nchunk; c0 += ss->seg_chunks, k ^= 1) {
         uint64_t cn = std::min(ss->seg_chunks, g->nchunk - c0);
-        uint64_t e0 = c0 * CHUNK, e1 = std::min(g->E, (c0 + cn) * CHUNK);
+        uint64_t e0 = c0 * g->chunk, e1 = std::min(g->E, (c0 + cn) * g->chunk);
         GS_HIP(hipStreamWaitEvent(ss->copy, ss->ev_done[k], 0));
         hipEvent_t cb;
         w->timer.begin(GS_K_H2D, ss->copy, &cb);

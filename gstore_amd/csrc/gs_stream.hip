@@ -31,7 +31,7 @@ StreamState::~StreamState() {
 // (pinned, ownership taken).  Uploads the resident prefix.
 int stream_setup(gs_graph *g, uint32_t *h_grid, uint64_t budget, uint64_t seg_bytes) {
     std::unique_ptr<StreamState> ss(new StreamState());  // takes h_grid only on success
-    const uint64_t chunk_bytes = (uint64_t)CHUNK * 4;
+    const uint64_t chunk_bytes = (uint64_t)g->chunk * 4;
     if (!seg_bytes) seg_bytes = std::min<uint64_t>(512ull << 20, budget / 8);  // default: 512 MiB or budget/8
     ss->seg_chunks = std::max<uint64_t>(1, seg_bytes / chunk_bytes);
     uint64_t two = 2 * ss->seg_chunks * chunk_bytes;
@@ -41,11 +41,11 @@ int stream_setup(gs_graph *g, uint32_t *h_grid, uint64_t budget, uint64_t seg_by
         return GS_ERR_ARG;
     }
     ss->res_chunks = std::min<uint64_t>(g->nchunk, (budget - two) / chunk_bytes);
-    uint64_t res_edges = std::min<uint64_t>(g->E, ss->res_chunks * CHUNK);
+    uint64_t res_edges = std::min<uint64_t>(g->E, ss->res_chunks * g->chunk);
     GS_TRY(g->grid.alloc(res_edges));
     if (res_edges) GS_HIP(hipMemcpy(g->grid.ptr, h_grid, res_edges * 4, hipMemcpyHostToDevice));
-    GS_TRY(ss->seg[0].alloc(ss->seg_chunks * CHUNK));
-    GS_TRY(ss->seg[1].alloc(ss->seg_chunks * CHUNK));
+    GS_TRY(ss->seg[0].alloc(ss->seg_chunks * g->chunk));
+    GS_TRY(ss->seg[1].alloc(ss->seg_chunks * g->chunk));
     GS_HIP(hipStreamCreateWithFlags(&ss->copy, hipStreamNonBlocking));
     for (int i = 0; i < 2; ++i) {
         GS_HIP(hipEventCreateWithFlags(&ss->ev_copied[i], hipEventDisableTiming));

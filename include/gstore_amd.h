@@ -201,6 +201,17 @@ int gs_server_drain(gs_server *s);
 int gs_server_info(gs_server *s, uint64_t *held_bytes, uint64_t *pending, uint64_t *inflight);
 void gs_server_destroy(gs_server *s);
 
+/* Edges per workgroup in every edge sweep: a multiple of 1024 up to 4096
+ * (default 4096).  Results do not depend on it; gs_tune picks it. */
+int gs_set_chunk(gs_graph *g, uint32_t edges);
+
+/* Order in which workgroups take edge chunks: 1 (default) gives each of the
+ * 8 XCDs a contiguous eighth of the grid (its L2 sees consecutive tiles);
+ * 0 deals chunks to XCDs round-robin (all XCDs work in one region of the
+ * grid at a time, sharing the last-level cache).  Results do not depend on
+ * it; gs_tune picks the faster one. */
+int gs_set_sweep_order(gs_graph *g, int xcd_contiguous);
+
 /* Stats of the last algorithm call on this handle. */
 int gs_get_stats(gs_graph *g, gs_stats *st);
 
