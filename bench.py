@@ -213,9 +213,27 @@ def stream_mode(args):
     g.close()
 
 
+def copy_mode(args):
+    """Reference copy for counter calibration: 5 device-to-device copies of 1 GiB.
+    Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE each dispatch should read and
+    write 1 GiB; the ratio to what the counters report scales kernel traffic."""
+    import torch
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / 5
+    print(json.dumps({"metric": "device copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                      "ms": round(dt * 1e3, 3), "gbs": round(2 * (1 << 30) / dt / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["main", "serve", "stream"], default="main")
+    ap.add_argument("--mode", choices=["main", "serve", "stream", "copy"], default="main")
     ap.add_argument("--serve-scales", default="18,20,22,24,26,28")
     ap.add_argument("--serve-rates", default="20,80,320")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
@@ -234,6 +252,10 @@ def main():
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--cpu-scale", type=int, default=22)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--algos", default="pr,bfs,wcc,kcore",
+                    help="subset of the step's jobs (profiling runs); the default is the full step")
+    ap.add_argument("--bfs-mode", type=int, default=-1, help="-1: autotune; 0: tile sweep; 1: direction-optimizing")
+    ap.add_argument("--tune", type=int, default=1, help="run the sweep autotuner before timing")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -245,6 +267,8 @@ def main():
         return serve_mode(args)
     if args.mode == "stream":
         return stream_mode(args)
+    if args.mode == "copy":
+        return copy_mode(args)
 
     import torch
     import gstore_amd as ga
@@ -268,6 +292,24 @@ def main():
         m_reach[r] = int(deg64[d > 0].sum() // 2)
     log("BFS M_reach per root computed")
 
+    algos = set(args.algos.split(","))
+    tune = None
+    if args.tune:
+        tune = g.tune()
+        log(f"autotune: chunk {tune['chunk']} xcd_order {tune['xcd_order']} ({tune['best_ms']:.2f} ms/PR iteration)")
+    bfs_mode = args.bfs_mode
+    if bfs_mode < 0 and "bfs" in algos:
+        best = None
+        for mode in (0, 1):
+            g.set_bfs_mode(mode)
+            g.bfs(roots[0], copy_out=False)
+            g.bfs(roots[1], copy_out=False)
+            ms = g.stats()["ms"]
+            log(f"bfs mode {mode}: {ms:.2f} ms (root {roots[1]})")
+            if best is None or ms < best[1]:
+                best = (mode, ms)
+        bfs_mode = best[0]
+    g.set_bfs_mode(max(bfs_mode, 0))
     g.set_kernel_timing(True)
     kern = {}
     per = {"pagerank": [0.0, 0.0], "bfs": [0.0, 0.0, 0.0], "wcc": [0.0, 0.0], "kcore": [0.0, 0.0]}
@@ -279,13 +321,14 @@ def main():
             a[1] += v["calls"]
 
     def step(record):
-        g.pagerank(args.pr_iters, ga.PR_INTENDED, copy_out=False)
-        st = g.stats()
-        if record:
-            absorb(st)
-            per["pagerank"][0] += st["ms"]
-            per["pagerank"][1] += 2 * E * args.pr_iters
-        for r in roots:
+        if "pr" in algos:
+            g.pagerank(args.pr_iters, ga.PR_INTENDED, copy_out=False)
+            st = g.stats()
+            if record:
+                absorb(st)
+                per["pagerank"][0] += st["ms"]
+                per["pagerank"][1] += 2 * E * args.pr_iters
+        for r in (roots if "bfs" in algos else []):
             g.bfs(r, copy_out=False)
             st = g.stats()
             if record:
@@ -294,12 +337,15 @@ def main():
                 per["bfs"][0] += st["ms"]
                 per["bfs"][1] += m_reach[r]
                 per["bfs"][2] += 2 * st["edges_scanned"]
-        g.wcc(copy_out=False)
-        st = g.stats()
-        if record:
-            absorb(st)
-            per["wcc"][0] += st["ms"]
-            per["wcc"][1] += 2 * st["edges_scanned"]
+        if "wcc" in algos:
+            g.wcc(copy_out=False)
+            st = g.stats()
+            if record:
+                absorb(st)
+                per["wcc"][0] += st["ms"]
+                per["wcc"][1] += 2 * st["edges_scanned"]
+        if "kcore" not in algos:
+            return
         g.kcore(args.k, copy_out=False)
         st = g.stats()
         if record:

@@ -58,6 +58,12 @@ class _Stats(ctypes.Structure):
                 ("kern_calls", ctypes.c_uint64 * len(KERNELS))]
 
 
+class _Tune(ctypes.Structure):
+    _fields_ = [("chunk", ctypes.c_uint32), ("xcd_order", ctypes.c_int32), ("from_cache", ctypes.c_int32),
+                ("ncand", ctypes.c_int32), ("best_ms", ctypes.c_double), ("cand_chunk", ctypes.c_uint32 * 6),
+                ("cand_order", ctypes.c_int32 * 6), ("cand_ms", ctypes.c_double * 6)]
+
+
 class QueryResult(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("arg", ctypes.c_uint64), ("value", ctypes.c_uint64),
                 ("aux", ctypes.c_uint64), ("checksum", ctypes.c_uint64), ("t_submit_ms", ctypes.c_double),
@@ -96,6 +102,9 @@ _SIGS = {
     "gs_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_sweep_order": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_chunk": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "gs_set_pagerank_heavy": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_set_bfs_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "gs_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_Tune)]),
     "gs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
     "gs_close": (None, [_P]),
     "gs_server_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_P)]),
@@ -233,6 +242,22 @@ class Graph:
 
     def save(self, prefix: str) -> None:
         _check(_lib.gs_save(self._h, prefix.encode()))
+
+    def tune(self, cache_path: Optional[str] = None) -> dict:
+        """Autotune edges-per-workgroup and sweep order (gs_tune)."""
+        r = _Tune()
+        _check(_lib.gs_tune(self._h, cache_path.encode() if cache_path else None, ctypes.byref(r)))
+        return {"chunk": r.chunk, "xcd_order": r.xcd_order, "from_cache": bool(r.from_cache),
+                "best_ms": r.best_ms,
+                "candidates": [{"chunk": r.cand_chunk[i], "xcd_order": r.cand_order[i], "ms": r.cand_ms[i]}
+                               for i in range(r.ncand)]}
+
+    def set_bfs_mode(self, mode: int, alpha: int = 14, beta: int = 24) -> None:
+        """0: tile sweep; 1: direction-optimizing over a neighbour-list index."""
+        _check(_lib.gs_set_bfs_mode(self._h, mode, alpha, beta))
+
+    def set_pagerank_heavy(self, on: bool) -> None:
+        _check(_lib.gs_set_pagerank_heavy(self._h, int(on)))
 
     def set_chunk(self, edges: int) -> None:
         _check(_lib.gs_set_chunk(self._h, edges))

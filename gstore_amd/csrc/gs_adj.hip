@@ -1,0 +1,303 @@
+// gs_adj.hip -- per-vertex adjacency index built from the tile grid, and the
+// direction-optimizing BFS that uses it.
+//
+// The tile grid stores each undirected edge once, in the tile of (min, max);
+// a vertex's neighbours are spread over its whole tile row and tile column,
+// so a BFS level over tiles has to read every edge of every active tile.
+// For levels with a small frontier (top-down) and levels where most vertices
+// are already reached (bottom-up, each unreached vertex stops at its first
+// frontier neighbour) an index of neighbour lists is far cheaper.  The index
+// is built once from the grid on the device (offsets = exclusive scan of the
+// .degree file, one sweep scattering both directions of every stored edge)
+// and kept with the graph.  Level structure, frontier bitmaps and the depth
+// encoding are those of the tile BFS (bfs.cpp:56-224), so depths are
+// bit-identical whichever direction each level takes.
+#include "gs_internal.h"
+
+namespace gs {
+
+// ---- index build -------------------------------------------------------------
+
+// Per-block sums of degree (blocks of 1024 vertices).
+__global__ void k_deg_block_sums(const int32_t *__restrict__ deg, uint64_t n, unsigned long long *bsum) {
+    __shared__ unsigned long long s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * 1024;
+    unsigned long long c = 0;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x)
+        if (base + i < n) c += (uint32_t)deg[base + i];
+    atomicAdd(&s, c);
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = s;
+}
+
+// off[v] = bstart[block] + exclusive prefix of degree within the block of 1024
+// (4 entries per thread, then a Hillis-Steele scan of the 256 thread sums).
+__global__ void k_deg_scan(const int32_t *__restrict__ deg, uint64_t n, const unsigned long long *__restrict__ bstart,
+                           uint64_t *off) {
+    __shared__ uint32_t d[1024];
+    uint64_t base = (uint64_t)blockIdx.x * 1024;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) d[i] = base + i < n ? (uint32_t)deg[base + i] : 0;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 entries with 256 threads (4 per thread)
+    __shared__ unsigned long long part[256];
+    unsigned long long loc[4], run = 0;
+    for (int j = 0; j < 4; ++j) {
+        loc[j] = run;
+        run += d[threadIdx.x * 4 + j];
+    }
+    part[threadIdx.x] = run;
+    __syncthreads();
+    for (int step = 1; step < 256; step <<= 1) {
+        unsigned long long v = threadIdx.x >= step ? part[threadIdx.x - step] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long before = (threadIdx.x ? part[threadIdx.x - 1] : 0) + bstart[blockIdx.x];
+    for (int j = 0; j < 4; ++j) {
+        uint64_t v = base + threadIdx.x * 4 + j;
+        if (v < n) off[v] = before + loc[j];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) off[n] = before + run;
+}
+
+__global__ void __launch_bounds__(SWEEP_WG) k_adj_scatter(SweepArgs a, const uint64_t *__restrict__ off,
+                                                          uint32_t *cur, uint32_t *nbr) {
+    __shared__ ChunkTiles s;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
+    uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
+    auto op = [&](uint32_t w, uint32_t rc) {
+        uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
+        nbr[off[v0] + atomicAdd(&cur[v0], 1u)] = v1;
+        nbr[off[v1] + atomicAdd(&cur[v1], 1u)] = v0;
+    };
+    sweep_chunk<false>(a, chunk, s, len, op);
+}
+
+// ---- direction-optimizing BFS kernels ------------------------------------------
+
+__global__ void k_do_root(uint32_t root, uint8_t *depth, uint32_t *vis, uint32_t *front, uint32_t *queue) {
+    depth[root] = 1;
+    vis[root >> 5] |= 1u << (root & 31);
+    front[root >> 5] |= 1u << (root & 31);
+    queue[0] = root;
+}
+
+// Top-down: one wave per frontier vertex from the queue, lanes over its neighbours.
+__global__ void __launch_bounds__(256) k_td_expand(const uint32_t *__restrict__ queue, uint64_t nq,
+                                                   const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                                                   const uint32_t *__restrict__ vis, uint32_t *next) {
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    for (uint64_t qi = wave; qi < nq; qi += nwaves) {
+        uint32_t u = queue[qi];
+        uint64_t b = off[u], e = off[u + 1];
+        for (uint64_t k = b + lane; k < e; k += 64) {
+            uint32_t v = nbr[k];
+            uint32_t wi = v >> 5, bit = 1u << (v & 31);
+            if ((vis[wi] & bit) == 0 && (next[wi] & bit) == 0) atomicOr(&next[wi], bit);
+        }
+    }
+}
+
+// Bottom-up: one lane per vertex; an unreached vertex with neighbours scans
+// them until it meets a frontier vertex.  Each wave owns 64 consecutive
+// vertices, so it writes its two next-bitmap words without atomics.
+__global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint64_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
+                                                   const uint32_t *__restrict__ front, uint32_t *next,
+                                                   unsigned long long *edges_checked) {
+    const int lane = threadIdx.x & 63;
+    uint32_t checked = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull; base < n; base += stride) {
+        uint64_t v = base + lane;
+        bool found = false;
+        if (v < n && !((vis[v >> 5] >> (v & 31)) & 1u)) {
+            uint64_t b = off[v], e = off[v + 1];
+            for (uint64_t k = b; k < e; ++k) {
+                uint32_t u = nbr[k];
+                ++checked;
+                if ((front[u >> 5] >> (u & 31)) & 1u) {
+                    found = true;
+                    break;
+                }
+            }
+        }
+        unsigned long long m = __ballot(found);
+        if (lane == 0 && m) {
+            next[base >> 5] = (uint32_t)m;
+            next[(base >> 5) + 1] = (uint32_t)(m >> 32);
+        }
+    }
+    if (checked) atomicAdd(edges_checked, (unsigned long long)checked);
+}
+
+// Level end for the direction-optimizing BFS: as k_bfs_level_end, plus the
+// degree sum of the new frontier and its vertex queue for a top-down level.
+__global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *front, uint32_t *next, uint8_t *depth,
+                                                      uint32_t newdepth, const int32_t *__restrict__ deg,
+                                                      uint64_t nwords, uint32_t *queue,
+                                                      unsigned long long *count) {
+    unsigned long long cnt = 0, msum = 0;
+    for (uint64_t gw = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; gw < nwords;
+         gw += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t nx = next[gw];
+        front[gw] = nx;
+        if (!nx) continue;
+        next[gw] = 0;
+        vis[gw] |= nx;
+        uint32_t c = __popc(nx);
+        unsigned long long pos = atomicAdd(&count[2], (unsigned long long)c);
+        cnt += c;
+        uint32_t m = nx;
+        while (m) {
+            uint32_t b = __ffs(m) - 1;
+            m &= m - 1;
+            uint64_t v = gw * 32 + b;
+            depth[v] = (uint8_t)newdepth;
+            msum += (uint32_t)deg[v];
+            queue[pos++] = (uint32_t)v;
+        }
+    }
+    if (cnt) {
+        atomicAdd(&count[0], cnt);
+        atomicAdd(&count[1], msum);
+    }
+}
+
+static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+
+int build_adjacency(gs_graph *g) {
+    if (g->adj_nbr.ptr && g->adj_off.n == g->n + 1) return GS_OK;
+    if (g->directed) { set_error("the adjacency index is built for undirected grids"); return GS_ERR_UNSUPPORTED; }
+    gs_ws *w = g->ws;
+    const uint64_t nb = (g->n + 1023) / 1024;
+    DevBuf<unsigned long long> bsum;
+    GS_TRY(bsum.alloc(nb));
+    GS_TRY(g->adj_off.alloc(g->n + 1));
+    GS_TRY(g->adj_nbr.alloc(2 * g->E));
+    k_deg_block_sums<<<(unsigned)nb, 256, 0, w->stream>>>(g->degree.ptr, g->n, bsum.ptr);
+    GS_HIP(hipGetLastError());
+    std::vector<unsigned long long> h(nb);
+    GS_HIP(hipMemcpyAsync(h.data(), bsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
+    GS_HIP(hipStreamSynchronize(w->stream));
+    unsigned long long run = 0;
+    for (uint64_t i = 0; i < nb; ++i) { unsigned long long c = h[i]; h[i] = run; run += c; }
+    if (run != 2 * g->E) {
+        set_error("degree sum %llu differs from twice the stored edges %llu", run, (unsigned long long)(2 * g->E));
+        return GS_ERR_FORMAT;
+    }
+    GS_HIP(hipMemcpyAsync(bsum.ptr, h.data(), nb * 8, hipMemcpyHostToDevice, w->stream));
+    k_deg_scan<<<(unsigned)nb, 256, 0, w->stream>>>(g->degree.ptr, g->n, bsum.ptr, g->adj_off.ptr);
+    GS_HIP(hipGetLastError());
+    DevBuf<uint32_t> cur;
+    GS_TRY(cur.alloc(g->n));
+    GS_HIP(hipMemsetAsync(cur.ptr, 0, g->n * 4, w->stream));
+    GS_TRY(sweep(g, w, GS_K_CONVERT, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
+        k_adj_scatter<<<grid, SWEEP_WG, 0, st>>>(a, g->adj_off.ptr, cur.ptr, g->adj_nbr.ptr);
+    }));
+    GS_HIP(hipStreamSynchronize(w->stream));
+    w->timer.collect();
+    w->timer.reset();
+    w->stats = gs_stats{};
+    return GS_OK;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
+                   int max_levels, int *levels) {
+    gs_graph *g = const_cast<gs_graph *>(gc);
+    if (!g || root >= g->n || max_levels < 0) {
+        set_error("gs_bfs: bad arguments");
+        return GS_ERR_ARG;
+    }
+    GS_TRY(build_adjacency(g));
+    GS_TRY(ensure_bitmaps(g, w));
+    if (w->depth.n != g->n) GS_TRY(w->depth.alloc(g->n));
+    if (w->queue.n != g->n) GS_TRY(w->queue.alloc(g->n));
+    const uint64_t words = bitmap_words(g);
+    w->timer.reset();
+    w->stats = gs_stats{};
+    hipEvent_t t0, t1;
+    GS_HIP(hipEventCreate(&t0));
+    GS_HIP(hipEventCreate(&t1));
+    GS_HIP(hipEventRecord(t0, w->stream));
+    GS_HIP(hipMemsetAsync(w->depth.ptr, 0, g->n, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_vis.ptr, 0, words * 4, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_front.ptr, 0, words * 4, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_next.ptr, 0, words * 4, w->stream));
+    k_do_root<<<1, 1, 0, w->stream>>>((uint32_t)root, w->depth.ptr, w->bm_vis.ptr, w->bm_front.ptr, w->queue.ptr);
+    GS_HIP(hipGetLastError());
+    // heuristic state (Beamer et al.): edges still to check from the unreached side
+    uint64_t m_u = 2 * g->E, n_f = 1, m_f = 0;
+    {
+        int32_t d = 0;
+        GS_HIP(hipMemcpyAsync(&d, g->degree.ptr + root, 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        m_f = (uint32_t)d;
+        m_u -= m_f;
+    }
+    bool bottom_up = false;
+    int nlev = 0, status = GS_OK;
+    uint64_t checked = 0;
+    for (uint32_t level = 1;; ++level) {
+        const uint64_t m_f_prev = m_f;  // edges a top-down level examines
+        if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha) bottom_up = true;
+        else if (bottom_up && n_f < g->n / (uint64_t)g->do_beta) bottom_up = false;
+        GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 32, w->stream));
+        hipEvent_t kb;
+        w->timer.begin(GS_K_BFS_EXPAND, w->stream, &kb);
+        if (bottom_up)
+            k_bu_expand<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, g->adj_off.ptr, g->adj_nbr.ptr, w->bm_vis.ptr,
+                                                            w->bm_front.ptr, w->bm_next.ptr, w->counters.ptr + 3);
+        else if (n_f)
+            k_td_expand<<<(unsigned)std::min<uint64_t>((n_f + 3) / 4, 8192), 256, 0, w->stream>>>(
+                w->queue.ptr, n_f, g->adj_off.ptr, g->adj_nbr.ptr, w->bm_vis.ptr, w->bm_next.ptr);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_BFS_EXPAND, w->stream, kb);
+        w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
+        k_do_level_end<<<vgrid(words), 256, 0, w->stream>>>(w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
+                                                            w->depth.ptr, level + 1, g->degree.ptr, words,
+                                                            w->queue.ptr, w->counters.ptr);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
+        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 32, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        n_f = w->h_counters[0];
+        m_f = w->h_counters[1];
+        checked += bottom_up ? w->h_counters[3] : m_f_prev;
+        m_u -= std::min(m_u, m_f);
+        if (level_counts && nlev < max_levels) level_counts[nlev] = n_f;
+        ++nlev;
+        if (n_f == 0) break;
+        if (level + 1 > 255) {
+            set_error("BFS from %llu needs depth > 255, beyond the u8 depth encoding", (unsigned long long)root);
+            status = GS_ERR_DEPTH_OVERFLOW;
+            break;
+        }
+    }
+    GS_HIP(hipEventRecord(t1, w->stream));
+    GS_TRY(sync_and_collect(w));
+    float ms = 0.f;
+    GS_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    w->stats.ms = ms;
+    w->stats.iterations = (uint64_t)nlev;
+    w->stats.edges_scanned = checked;
+    if (levels) *levels = nlev;
+    if (status != GS_OK) return status;
+    if (depth_out) {
+        GS_HIP(hipMemcpyAsync(depth_out, w->depth.ptr, g->n, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+    }
+    return GS_OK;
+}

@@ -104,6 +104,7 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     if (w->depth.n != g->n) GS_TRY(w->depth.alloc(g->n));
     const uint64_t words = bitmap_words(g);
     w->timer.reset();
+    w->stats = gs_stats{};
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
@@ -152,7 +153,6 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    w->stats = gs_stats{};
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nlev;
     w->stats.edges_scanned = w->h_counters[1];
@@ -168,5 +168,6 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
 extern "C" int gs_bfs(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts, int max_levels,
                       int *levels) {
     if (!g) { set_error("gs_bfs: null graph"); return GS_ERR_ARG; }
+    if (g->bfs_mode == 1) return run_bfs_do(g, g->ws, root, depth_out, level_counts, max_levels, levels);
     return run_bfs(g, g->ws, root, depth_out, level_counts, max_levels, levels);
 }

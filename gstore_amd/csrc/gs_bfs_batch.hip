@@ -125,6 +125,7 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     if (w->ms_roots.n < 64) GS_TRY(w->ms_roots.alloc(64));
     if (w->ms_counts.n < 64) GS_TRY(w->ms_counts.alloc(64));
     w->timer.reset();
+    w->stats = gs_stats{};
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
@@ -182,7 +183,6 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    w->stats = gs_stats{};
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nlev;
     w->stats.edges_scanned = w->h_counters[0];

@@ -111,7 +111,7 @@ size_t gs_ws::device_bytes() const {
     return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
            bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + kc_deg.bytes() + ms_vis.bytes() +
            ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_counts.bytes() +
-           counters.bytes();
+           queue.bytes() + counters.bytes();
 }
 
 gs_graph::~gs_graph() {
@@ -184,7 +184,7 @@ int build_chunk_map(gs_graph *g, const std::vector<uint64_t> &beg) {
 
 // ---- validation: ids < N and degree file == endpoint counts -----------------
 __global__ void __launch_bounds__(SWEEP_WG) k_validate(SweepArgs a, uint32_t n, int directed,
-                                                       int32_t *gdeg, unsigned long long *bad) {
+                                                       int32_t *gdeg, uint32_t *in_touch, unsigned long long *bad) {
     __shared__ ChunkTiles s;
     uint64_t chunk;
     if (!launch_chunk(a, chunk)) return;
@@ -195,6 +195,7 @@ __global__ void __launch_bounds__(SWEEP_WG) k_validate(SweepArgs a, uint32_t n, 
         if (v0 >= n || v1 >= n) { ++nbad; return; }
         atomicAdd(&gdeg[v0], 1);
         if (!directed) atomicAdd(&gdeg[v1], 1);
+        else if (!((in_touch[v1 >> 5] >> (v1 & 31)) & 1u)) atomicOr(&in_touch[v1 >> 5], 1u << (v1 & 31));
     };
     sweep_chunk<false>(a, chunk, s, len, op);
     if (nbad) atomicAdd(bad, nbad);
@@ -210,6 +211,11 @@ __global__ void k_count_mismatch(const int32_t *x, const int32_t *y, uint64_t n,
 static int validate(gs_graph *g) {
     DevBuf<int32_t> gdeg;
     GS_TRY(gdeg.alloc(g->n));
+    if (g->directed) {
+        // vertices with an in-edge: with .degree (out-degree) they are the vertices with a stored edge
+        GS_TRY(g->in_touch.alloc(bitmap_words(g)));
+        GS_HIP(hipMemsetAsync(g->in_touch.ptr, 0, g->in_touch.bytes(), g->stream));
+    }
     GS_HIP(hipMemsetAsync(gdeg.ptr, 0, g->n * 4, g->stream));
     GS_HIP(hipMemsetAsync(g->counters.ptr, 0, 16 * 8, g->stream));
     {
@@ -218,7 +224,7 @@ static int validate(gs_graph *g) {
         GS_HIP(hipStreamSynchronize(g->stream));
         uint32_t n32 = (uint32_t)std::min<uint64_t>(g->n, 0xFFFFFFFFull);
         GS_TRY(sweep(g, w, GS_K_CONVERT, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
-            k_validate<<<grid, SWEEP_WG, 0, st>>>(a, n32, g->directed, gdeg.ptr, g->counters.ptr);
+            k_validate<<<grid, SWEEP_WG, 0, st>>>(a, n32, g->directed, gdeg.ptr, g->in_touch.ptr, g->counters.ptr);
         }));
         GS_HIP(hipStreamSynchronize(w->stream));
         w->timer.collect();
@@ -701,7 +707,7 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->resident_edges = g->ss ? std::min<uint64_t>(g->E, g->ss->res_chunks * g->chunk) : g->E;
     info->segment_edges = g->ss ? g->ss->seg_chunks * g->chunk : 0;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
-                         g->degree.bytes() + g->ws->device_bytes() +
+                         g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
     return GS_OK;
 }
@@ -773,6 +779,22 @@ int gs_set_chunk(gs_graph *g, uint32_t edges) {
     GS_HIP(hipMemcpy(beg.data(), g->tz_beg.ptr, (g->nz + 1) * 8, hipMemcpyDeviceToHost));
     g->chunk = edges;
     return build_chunk_map(g, beg);
+}
+
+int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta) {
+    if (!g || mode < 0 || mode > 1 || alpha < 1 || beta < 1) { set_error("gs_set_bfs_mode: bad arguments"); return GS_ERR_ARG; }
+    if (mode == 1 && g->directed) { set_error("direction-optimizing BFS needs an undirected grid"); return GS_ERR_UNSUPPORTED; }
+    g->bfs_mode = mode;
+    g->do_alpha = alpha;
+    g->do_beta = beta;
+    return mode == 1 ? build_adjacency(g) : GS_OK;
+}
+
+int gs_set_pagerank_heavy(gs_graph *g, int on) {
+    if (!g) { set_error("null graph"); return GS_ERR_ARG; }
+    g->heavy_on = on != 0;
+    g->heavy_ready = false;
+    return GS_OK;
 }
 
 int gs_set_sweep_order(gs_graph *g, int xcd_contiguous) {

@@ -118,6 +118,7 @@ struct gs_ws {
     gs::DevBuf<int32_t> kc_deg;
     gs::DevBuf<uint64_t> ms_vis, ms_front, ms_next, ms_roots;  // batched BFS lane masks
     gs::DevBuf<uint8_t> ms_depth;                             // lane-major depth, lanes x N
+    gs::DevBuf<uint32_t> queue;                               // frontier vertices (direction-optimizing BFS)
     gs::DevBuf<unsigned long long> ms_counts;
     gs::DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;  // pinned mirror of counters
@@ -155,11 +156,22 @@ struct gs_graph {
     gs::DevBuf<uint32_t> tz_rc;
     gs::DevBuf<uint32_t> chunk_tz;
     gs::DevBuf<int32_t> degree;
+    gs::DevBuf<uint32_t> in_touch;           // directed grids: bitmap of vertices with an in-edge
     gs::DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;
     gs_ws *ws = nullptr;                     // default workspace of the plain entry points
     gs::StreamState *ss = nullptr;           // non-null in semi-external mode
     int xcd_order = 1;                       // sweep order, see gs_set_sweep_order
+    int bfs_mode = 0;                        // 0: tile sweep, 1: direction-optimizing over adj_*
+    bool heavy_on = true;                    // PageRank: accumulate the heaviest vertices in LDS
+    bool heavy_ready = false;
+    uint32_t heavy_dthr = 0xFFFFFFFFu;       // degree threshold of the heavy set
+    int nheavy = 0;
+    gs::DevBuf<uint32_t> heavy_ids;          // sorted heavy vertex ids
+    int persistent_wgs = 1024;               // workgroups of the persistent PageRank scatter
+    int do_alpha = 14, do_beta = 24;         // top-down <-> bottom-up switch thresholds
+    gs::DevBuf<uint64_t> adj_off;            // neighbour-list offsets, N + 1 (built on demand)
+    gs::DevBuf<uint32_t> adj_nbr;            // neighbour ids, 2E
     ~gs_graph();
 };
 
@@ -180,6 +192,9 @@ int run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *rank_o
 int run_ppr(const gs_graph *g, gs_ws *w, uint64_t source, int iters, float *out);
 int run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
             int max_levels, int *levels);
+int run_bfs_do(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
+               int max_levels, int *levels);
+int build_adjacency(gs_graph *g);
 int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
                   uint64_t *level_counts, int max_levels, int *levels);
 int run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp);

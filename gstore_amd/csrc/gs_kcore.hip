@@ -97,6 +97,7 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     GS_TRY(ensure_bitmaps(g, w));
     if (w->kc_deg.n != g->n) GS_TRY(w->kc_deg.alloc(g->n));
     w->timer.reset();
+    w->stats = gs_stats{};
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
@@ -136,7 +137,6 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    w->stats = gs_stats{};
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nr;
     w->stats.edges_scanned = w->h_counters[2];

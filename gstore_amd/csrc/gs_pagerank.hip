@@ -11,12 +11,22 @@
 // The scatter is one FP32 hardware atomic add per direction (global_atomic_
 // add_f32 on coarse-grained HBM, no CAS loop).  Finalize uses explicit
 // round-to-nearest double ops so the compiler cannot fuse 0.85*x + 0.15.
+#include <algorithm>
+
 #include "gs_internal.h"
 
 namespace gs {
 
-__global__ void k_pr_init(const int32_t *__restrict__ deg, uint64_t n, int mode, float inv_n, float *prior,
-                          float *acc, float *fac) {
+// Heavy vertices (degree >= dthr, the few thousand largest) carry the sign
+// bit on fac and prior: the scatter then knows from the prior it already
+// reads that the vertex's adds go to the workgroup's LDS table instead of a
+// contended global atomic.  Values themselves are never negative.
+__device__ inline float mark(float x, bool heavy) { return heavy ? __int_as_float(__float_as_int(x) | 0x80000000) : x; }
+__device__ inline bool is_heavy(float x) { return __float_as_int(x) < 0; }
+__device__ inline float unmark(float x) { return __int_as_float(__float_as_int(x) & 0x7FFFFFFF); }
+
+__global__ void k_pr_init(const int32_t *__restrict__ deg, uint64_t n, int mode, float inv_n, uint32_t dthr,
+                          float *prior, float *acc, float *fac) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         int32_t d = deg[v];
         float f = 0.0f;
@@ -24,8 +34,9 @@ __global__ void k_pr_init(const int32_t *__restrict__ deg, uint64_t n, int mode,
             double q = 1.0 / (double)d;
             f = mode == GS_PR_COMPAT ? (float)(int32_t)q : (float)q;
         }
-        fac[v] = f;
-        prior[v] = inv_n;
+        bool h = (uint32_t)d >= dthr;
+        fac[v] = mark(f, h);
+        prior[v] = mark(inv_n, h);
         acc[v] = 0.0f;
     }
 }
@@ -38,9 +49,9 @@ __global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter(SweepArgs a, const floa
     uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
     auto op = [&](uint32_t w, uint32_t rc) {
         uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
-        float p0 = prior[v0];
+        float p0 = unmark(prior[v0]);
         if (!DIRECTED) {
-            float p1 = prior[v1];
+            float p1 = unmark(prior[v1]);
             unsafeAtomicAdd(&acc[v0], p1);
         }
         unsafeAtomicAdd(&acc[v1], p0);
@@ -48,11 +59,81 @@ __global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter(SweepArgs a, const floa
     sweep_chunk<false>(a, chunk, s, len, op);
 }
 
-__global__ void k_pr_finalize(uint64_t n, float *__restrict__ prior, float *__restrict__ acc,
-                              const float *__restrict__ fac) {
+// Undirected scatter with heavy vertices accumulated in LDS.  Persistent:
+// gridDim.x workgroups, each sweeping a contiguous run of chunks (runs ordered
+// so the workgroups of one XCD hold one contiguous region), then adding its
+// LDS table to the heavy vertices' acc once.
+constexpr int HEAVY_MAX = 2048;
+
+__global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter_heavy(SweepArgs a, const float *__restrict__ prior,
+                                                               float *acc, const uint32_t *__restrict__ heavy,
+                                                               int nheavy) {
+    __shared__ ChunkTiles s;
+    __shared__ uint32_t hid[HEAVY_MAX];
+    __shared__ float hacc[HEAVY_MAX];
+    for (int i = threadIdx.x; i < nheavy; i += blockDim.x) {
+        hid[i] = heavy[i];
+        hacc[i] = 0.0f;
+    }
+    const uint64_t G = gridDim.x;
+    const uint64_t rank = (blockIdx.x % NUM_XCD) * ((G + NUM_XCD - 1) / NUM_XCD) + blockIdx.x / NUM_XCD;
+    const uint64_t per = (a.nlaunch + G - 1) / G;
+    const uint64_t lo = min(a.nlaunch, rank * per), hi = min(a.nlaunch, lo + per);
+    auto add = [&](uint32_t v, float x, bool h) {
+        if (h) {
+            int l = 0, r = nheavy - 1;
+            while (l < r) {
+                int m = (l + r) >> 1;
+                if (hid[m] < v) l = m + 1; else r = m;
+            }
+            atomicAdd(&hacc[l], x);
+        } else {
+            unsafeAtomicAdd(&acc[v], x);
+        }
+    };
+    auto op = [&](uint32_t w, uint32_t rc) {
+        uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
+        float p0 = prior[v0], p1 = prior[v1];
+        add(v1, unmark(p0), is_heavy(p1));
+        add(v0, unmark(p1), is_heavy(p0));
+    };
+    for (uint64_t c = lo; c < hi; ++c) {
+        uint64_t chunk = a.chunk0 + c;
+        __syncthreads();  // previous chunk's tiles are no longer read
+        uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
+        sweep_chunk<false>(a, chunk, s, len, op);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nheavy; i += blockDim.x)
+        if (hacc[i] != 0.0f) unsafeAtomicAdd(&acc[hid[i]], hacc[i]);
+}
+
+// Degree threshold selection: histogram of floor(log2(degree)).
+__global__ void k_deg_log_hist(const int32_t *__restrict__ deg, uint64_t n, unsigned long long *hist) {
+    __shared__ uint32_t h[33];
+    if (threadIdx.x < 33) h[threadIdx.x] = 0;
+    __syncthreads();
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t d = (uint32_t)deg[v];
+        if (d) atomicAdd(&h[31 - __clz(d)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 33 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ void k_collect_heavy(const int32_t *__restrict__ deg, uint64_t n, uint32_t dthr, uint32_t *out,
+                                unsigned long long *count) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        if ((uint32_t)deg[v] >= dthr) out[atomicAdd(count, 1ull)] = (uint32_t)v;
+}
+
+__global__ void k_pr_finalize(uint64_t n, float *__restrict__ prior, float *__restrict__ acc,
+                              const float *__restrict__ fac, int last) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        float f = fac[v];
         double r = __dadd_rn(0.15, __dmul_rn(0.85, (double)acc[v]));
-        prior[v] = __double2float_rn(__dmul_rn(r, (double)fac[v]));
+        float x = __double2float_rn(__dmul_rn(r, (double)unmark(f)));
+        prior[v] = last ? x : mark(x, is_heavy(f));
         acc[v] = 0.0f;
     }
 }
@@ -60,13 +141,14 @@ __global__ void k_pr_finalize(uint64_t n, float *__restrict__ prior, float *__re
 // Personalised PageRank from one source s (no reference counterpart; same
 // scatter as above): r0 = e_s, prior = r * (float)(1/deg),
 // r' = 0.15 [v == s] + 0.85 * sum of prior over in-edges.
-__global__ void k_ppr_init(const int32_t *__restrict__ deg, uint64_t n, uint64_t src, float *prior, float *acc,
-                           float *fac) {
+__global__ void k_ppr_init(const int32_t *__restrict__ deg, uint64_t n, uint64_t src, uint32_t dthr, float *prior,
+                           float *acc, float *fac) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         int32_t d = deg[v];
         float f = d != 0 ? (float)(1.0 / (double)d) : 0.0f;
-        fac[v] = f;
-        prior[v] = v == src ? f : 0.0f;
+        bool h = (uint32_t)d >= dthr;
+        fac[v] = mark(f, h);
+        prior[v] = mark(v == src ? f : 0.0f, h);
         acc[v] = 0.0f;
     }
 }
@@ -74,14 +156,66 @@ __global__ void k_ppr_init(const int32_t *__restrict__ deg, uint64_t n, uint64_t
 __global__ void k_ppr_finalize(uint64_t n, uint64_t src, float *__restrict__ prior, float *__restrict__ acc,
                                const float *__restrict__ fac, float *__restrict__ rank) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        float f = fac[v];
         float r = __double2float_rn(__dadd_rn(v == src ? 0.15 : 0.0, __dmul_rn(0.85, (double)acc[v])));
-        prior[v] = __fmul_rn(r, fac[v]);
+        prior[v] = mark(__fmul_rn(r, unmark(f)), is_heavy(f));
         acc[v] = 0.0f;
         if (rank) rank[v] = r;
     }
 }
 
 static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+
+// Heavy set: vertices of degree >= 2^b for the smallest b leaving at most
+// HEAVY_MAX of them (none below degree 256).  Built once per graph.
+int ensure_heavy(gs_graph *g, hipStream_t st) {
+    if (g->heavy_ready) return GS_OK;
+    g->heavy_ready = true;
+    g->heavy_dthr = 0xFFFFFFFFu;
+    g->nheavy = 0;
+    if (g->directed || !g->heavy_on) return GS_OK;
+    DevBuf<unsigned long long> hist;
+    GS_TRY(hist.alloc(34));
+    GS_HIP(hipMemsetAsync(hist.ptr, 0, 34 * 8, st));
+    k_deg_log_hist<<<vgrid(g->n), 256, 0, st>>>(g->degree.ptr, g->n, hist.ptr);
+    GS_HIP(hipGetLastError());
+    unsigned long long h[34];
+    GS_HIP(hipMemcpyAsync(h, hist.ptr, 34 * 8, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    unsigned long long above = 0;
+    int b = 32;
+    while (b > 8 && above + h[b - 1] <= (unsigned long long)HEAVY_MAX) above += h[--b];
+    if (above == 0) return GS_OK;
+    const uint32_t dthr = 1u << b;
+    GS_TRY(g->heavy_ids.alloc(HEAVY_MAX));
+    GS_HIP(hipMemsetAsync(hist.ptr, 0, 8, st));
+    k_collect_heavy<<<vgrid(g->n), 256, 0, st>>>(g->degree.ptr, g->n, dthr, g->heavy_ids.ptr, hist.ptr);
+    GS_HIP(hipGetLastError());
+    std::vector<uint32_t> ids(above);
+    GS_HIP(hipMemcpyAsync(ids.data(), g->heavy_ids.ptr, above * 4, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    std::sort(ids.begin(), ids.end());
+    GS_HIP(hipMemcpyAsync(g->heavy_ids.ptr, ids.data(), above * 4, hipMemcpyHostToDevice, st));
+    GS_HIP(hipStreamSynchronize(st));
+    g->heavy_dthr = dthr;
+    g->nheavy = (int)above;
+    return GS_OK;
+}
+
+// One scatter pass: the heavy-vertex kernel when the graph has a heavy set.
+static int scatter(const gs_graph *g, gs_ws *w) {
+    if (g->nheavy && !g->directed) {
+        return sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned, hipStream_t st) {
+            unsigned grid = (unsigned)std::min<uint64_t>(a.nlaunch, (uint64_t)g->persistent_wgs);
+            k_pr_scatter_heavy<<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr, g->heavy_ids.ptr,
+                                                          g->nheavy);
+        });
+    }
+    return sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
+        if (g->directed) k_pr_scatter<true><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
+        else k_pr_scatter<false><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
+    });
+}
 
 }  // namespace gs
 
@@ -97,22 +231,23 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
         GS_TRY(w->pr_acc.alloc(g->n));
         GS_TRY(w->pr_fac.alloc(g->n));
     }
+    GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream));
     w->timer.reset();
+    w->stats = gs_stats{};
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t0, w->stream));
     k_pr_init<<<vgrid(g->n), 256, 0, w->stream>>>(g->degree.ptr, g->n, mode, (float)(1.0 / (double)g->n),
-                                                  w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
+                                                  iters ? g->heavy_dthr : 0xFFFFFFFFu, w->pr_prior.ptr,
+                                                  w->pr_acc.ptr, w->pr_fac.ptr);
     GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
-        GS_TRY(sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
-            if (g->directed) k_pr_scatter<true><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
-            else k_pr_scatter<false><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
-        }));
+        GS_TRY(scatter(g, w));
         w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
-        k_pr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
+        k_pr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr,
+                                                          it + 1 == iters);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
     }
@@ -122,7 +257,6 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    w->stats = gs_stats{};
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)iters;
     w->stats.edges_scanned = g->E * (uint64_t)iters;
@@ -145,20 +279,19 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
         GS_TRY(w->pr_fac.alloc(g->n));
     }
     if (w->ppr_rank.n != g->n) GS_TRY(w->ppr_rank.alloc(g->n));
+    GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream));
     w->timer.reset();
+    w->stats = gs_stats{};
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t0, w->stream));
-    k_ppr_init<<<vgrid(g->n), 256, 0, w->stream>>>(g->degree.ptr, g->n, src, w->pr_prior.ptr, w->pr_acc.ptr,
-                                                   w->pr_fac.ptr);
+    k_ppr_init<<<vgrid(g->n), 256, 0, w->stream>>>(g->degree.ptr, g->n, src, g->heavy_dthr, w->pr_prior.ptr,
+                                                   w->pr_acc.ptr, w->pr_fac.ptr);
     GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
-        GS_TRY(sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
-            if (g->directed) k_pr_scatter<true><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
-            else k_pr_scatter<false><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
-        }));
+        GS_TRY(scatter(g, w));
         w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
         k_ppr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, src, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr,
                                                            it + 1 == iters ? w->ppr_rank.ptr : nullptr);
@@ -171,7 +304,6 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    w->stats = gs_stats{};
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)iters;
     w->stats.edges_scanned = g->E * (uint64_t)iters;

@@ -1,0 +1,110 @@
+// gs_tune.hip -- autotuner for the sweep knobs of one graph.
+//
+// Knobs: edges per workgroup (the tile chunk each workgroup stages, 1024 /
+// 2048 / 4096) and the chunk-to-XCD order (contiguous eighths vs round-robin).
+// Each candidate is timed on two PageRank iterations (the sweep every job
+// shares); the fastest is applied.  With a cache path, results are kept per
+// graph signature (N, E, non-empty tiles, directed) so a later open of the
+// same graph applies them without timing.
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "gs_internal.h"
+
+using namespace gs;
+
+namespace {
+
+std::string signature(const gs_graph *g) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "n=%llu e=%llu nz=%llu d=%d", (unsigned long long)g->n, (unsigned long long)g->E,
+             (unsigned long long)g->nz, (int)g->directed);
+    return buf;
+}
+
+bool cache_lookup(const char *path, const std::string &sig, uint32_t *chunk, int *order) {
+    FILE *f = fopen(path, "r");
+    if (!f) return false;
+    char line[512];
+    bool hit = false;
+    while (fgets(line, sizeof line, f)) {
+        // format: <signature> | chunk=<c> order=<o>
+        char *bar = strchr(line, '|');
+        if (!bar) continue;
+        std::string key(line, bar - line);
+        while (!key.empty() && key.back() == ' ') key.pop_back();
+        unsigned c;
+        int o;
+        if (key == sig && sscanf(bar + 1, " chunk=%u order=%d", &c, &o) == 2) {
+            *chunk = c;
+            *order = o;
+            hit = true;  // last entry wins
+        }
+    }
+    fclose(f);
+    return hit;
+}
+
+}  // namespace
+
+extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out) {
+    if (!g) { set_error("null graph"); return GS_ERR_ARG; }
+    gs_tune_result r{};
+    const std::string sig = signature(g);
+    uint32_t c;
+    int o;
+    if (cache_path && cache_lookup(cache_path, sig, &c, &o)) {
+        if (!g->ss) GS_TRY(gs_set_chunk(g, c));
+        g->xcd_order = o;
+        r.chunk = g->chunk;
+        r.xcd_order = g->xcd_order;
+        r.from_cache = 1;
+        if (out) *out = r;
+        return GS_OK;
+    }
+    const uint32_t chunks[3] = {1024, 2048, 4096};
+    double best = 1e30;
+    uint32_t best_c = g->chunk;
+    int best_o = g->xcd_order, k = 0;
+    const bool timing = g->ws->timer.on;
+    g->ws->timer.on = false;
+    for (int ci = 0; ci < 3; ++ci) {
+        if (g->ss && chunks[ci] != g->chunk) continue;  // streaming graphs keep their chunk
+        if (!g->ss) GS_TRY(gs_set_chunk(g, chunks[ci]));
+        for (int order = 0; order < 2; ++order, ++k) {
+            g->xcd_order = order;
+            GS_TRY(run_pagerank(g, g->ws, 1, GS_PR_INTENDED, nullptr));  // warm
+            GS_TRY(run_pagerank(g, g->ws, 2, GS_PR_INTENDED, nullptr));
+            double ms = g->ws->stats.ms / 2;
+            if (k < 6) {
+                r.cand_chunk[k] = chunks[ci];
+                r.cand_order[k] = order;
+                r.cand_ms[k] = ms;
+            }
+            if (ms < best) {
+                best = ms;
+                best_c = chunks[ci];
+                best_o = order;
+            }
+        }
+    }
+    g->ws->timer.on = timing;
+    if (!g->ss) GS_TRY(gs_set_chunk(g, best_c));
+    g->xcd_order = best_o;
+    r.chunk = g->chunk;
+    r.xcd_order = g->xcd_order;
+    r.ncand = k < 6 ? k : 6;
+    r.best_ms = best;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "a");
+        if (f) {
+            fprintf(f, "%s | chunk=%u order=%d pr_iter_ms=%.3f\n", sig.c_str(), g->chunk, g->xcd_order, best);
+            fclose(f);
+        }
+    }
+    if (out) *out = r;
+    return GS_OK;
+}

@@ -78,10 +78,14 @@ __global__ void __launch_bounds__(SWEEP_WG) k_wcc_check(SweepArgs a, const uint3
 }
 
 // label = root for vertices with a stored edge, 0xFFFFFFFF otherwise; count roots.
-__global__ void k_wcc_label(uint32_t *par, const int32_t *__restrict__ deg, uint64_t n, unsigned long long *ncomp) {
+// A vertex has a stored edge when its .degree is non-zero (undirected: endpoint
+// count) or, on a directed grid (.degree = out-degree), when it has an in-edge.
+__global__ void k_wcc_label(uint32_t *par, const int32_t *__restrict__ deg, const uint32_t *__restrict__ in_touch,
+                            uint64_t n, unsigned long long *ncomp) {
     uint32_t c = 0;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
-        if (deg[v] == 0) par[v] = 0xFFFFFFFFu;
+        bool touched = deg[v] != 0 || (in_touch && ((in_touch[v >> 5] >> (v & 31)) & 1u));
+        if (!touched) par[v] = 0xFFFFFFFFu;
         else c += (par[v] == (uint32_t)v);
     }
     if (c) atomicAdd(ncomp, (unsigned long long)c);
@@ -97,6 +101,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     if (!g) { set_error("gs_wcc: null graph"); return GS_ERR_ARG; }
     if (w->wcc_parent.n != g->n) GS_TRY(w->wcc_parent.alloc(g->n));
     w->timer.reset();
+    w->stats = gs_stats{};
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventRecord(t0, w->stream));
@@ -127,7 +132,8 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
         if (w->h_counters[0] == 0) break;
     }
     GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
-    k_wcc_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->degree.ptr, g->n, w->counters.ptr);
+    k_wcc_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->degree.ptr,
+                                                    g->directed ? g->in_touch.ptr : nullptr, g->n, w->counters.ptr);
     GS_HIP(hipGetLastError());
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t1, w->stream));
@@ -137,7 +143,6 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    w->stats = gs_stats{};
     w->stats.ms = ms;
     w->stats.iterations = passes;
     w->stats.edges_scanned = g->E * passes * 2;

@@ -201,6 +201,23 @@ int gs_server_drain(gs_server *s);
 int gs_server_info(gs_server *s, uint64_t *held_bytes, uint64_t *pending, uint64_t *inflight);
 void gs_server_destroy(gs_server *s);
 
+/* BFS method for gs_bfs: 0 (default) sweeps the active tiles of the grid each
+ * level; 1 is direction-optimizing: it builds (once, kept with the graph) a
+ * neighbour-list index from the grid -- 8(N+1) + 8E bytes of HBM -- and runs
+ * each level top-down over the frontier's lists or bottom-up (each unreached
+ * vertex stops at its first frontier neighbour), switching to bottom-up when
+ * the frontier's edges exceed the unreached side's edges / alpha and back when
+ * the frontier holds fewer than N / beta vertices (defaults 14, 24).  Depths
+ * and level counts are identical in both modes.  Undirected grids only. */
+int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta);
+
+/* PageRank / personalised PageRank on undirected grids: 1 (default) adds
+ * into the (at most 2048) highest-degree vertices through a per-workgroup LDS
+ * table flushed once per workgroup, instead of one contended global atomic
+ * per edge; 0 uses plain global atomics for every vertex.  Only the FP32
+ * summation order of those vertices changes. */
+int gs_set_pagerank_heavy(gs_graph *g, int on);
+
 /* Edges per workgroup in every edge sweep: a multiple of 1024 up to 4096
  * (default 4096).  Results do not depend on it; gs_tune picks it. */
 int gs_set_chunk(gs_graph *g, uint32_t edges);
@@ -211,6 +228,23 @@ int gs_set_chunk(gs_graph *g, uint32_t edges);
  * grid at a time, sharing the last-level cache).  Results do not depend on
  * it; gs_tune picks the faster one. */
 int gs_set_sweep_order(gs_graph *g, int xcd_contiguous);
+
+/* Autotuner: times two PageRank iterations for each edges-per-workgroup
+ * value (1024, 2048, 4096) and both sweep orders, applies the fastest, and
+ * appends it to cache_path (optional) keyed by N, E, non-empty tiles and
+ * directedness; a cached entry is applied without timing. */
+typedef struct gs_tune_result {
+    uint32_t chunk;
+    int32_t xcd_order;
+    int32_t from_cache;
+    int32_t ncand;
+    double best_ms;           /* ms per PageRank iteration of the chosen setting */
+    uint32_t cand_chunk[6];
+    int32_t cand_order[6];
+    double cand_ms[6];
+} gs_tune_result;
+
+int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out);
 
 /* Stats of the last algorithm call on this handle. */
 int gs_get_stats(gs_graph *g, gs_stats *st);
