@@ -278,7 +278,6 @@ def main():
     g = ga.Graph.kron(args.scale)
     gen_s = time.time() - t0
     n, E = g.n, g.E
-    nchunk = (E + 4095) // 4096
     log(f"graph ready in {gen_s:.1f}s: N={n} E={E} nonempty tiles={g.nonempty_tiles}")
     deg = g.degree()
     rng = np.random.default_rng(20240601)
@@ -368,21 +367,30 @@ def main():
     traversed = per["pagerank"][1] + per["bfs"][1] + per["wcc"][1] + per["kcore"][1]
     value = traversed / elapsed / 1e9
 
-    per_algo = {
-        "pagerank": {"ms": per["pagerank"][0] / args.steps, "iterations": args.pr_iters,
-                     "gteps": per["pagerank"][1] / (per["pagerank"][0] / 1e3) / 1e9},
-        "bfs": {"ms_per_root": per["bfs"][0] / args.steps / len(roots), "roots": len(roots),
-                "gteps_graph500": per["bfs"][1] / (per["bfs"][0] / 1e3) / 1e9,
-                "gteps_scanned": per["bfs"][2] / (per["bfs"][0] / 1e3) / 1e9},
-        "wcc": {"ms": per["wcc"][0] / args.steps, "gteps": per["wcc"][1] / (per["wcc"][0] / 1e3) / 1e9},
-        "kcore": {"ms": per["kcore"][0] / args.steps, "k": args.k,
-                  "gteps": per["kcore"][1] / (per["kcore"][0] / 1e3) / 1e9},
-    }
+    def rate(edges, ms):
+        return edges / (ms / 1e3) / 1e9 if ms > 0 else None
+
+    per_algo = {}
+    if "pr" in algos:
+        per_algo["pagerank"] = {"ms": per["pagerank"][0] / args.steps, "iterations": args.pr_iters,
+                                "gteps": rate(per["pagerank"][1], per["pagerank"][0])}
+    if "bfs" in algos:
+        per_algo["bfs"] = {"ms_per_root": per["bfs"][0] / args.steps / len(roots), "roots": len(roots),
+                           "mode": ["tile sweep", "direction-optimizing"][max(bfs_mode, 0)],
+                           "gteps_graph500": rate(per["bfs"][1], per["bfs"][0]),
+                           "gteps_scanned": rate(per["bfs"][2], per["bfs"][0])}
+    if "wcc" in algos:
+        per_algo["wcc"] = {"ms": per["wcc"][0] / args.steps, "gteps": rate(per["wcc"][1], per["wcc"][0])}
+    if "kcore" in algos:
+        per_algo["kcore"] = {"ms": per["kcore"][0] / args.steps, "k": args.k,
+                             "gteps": rate(per["kcore"][1], per["kcore"][0])}
     for a in per_algo.values():
         for k2 in list(a):
             if isinstance(a[k2], float):
                 a[k2] = round(a[k2], 4)
 
+    chunk = tune["chunk"] if tune else 4096
+    nchunk = (E + chunk - 1) // chunk
     kernels = {}
     for name, (ms, calls, scanned) in kern.items():
         avg_ms = ms / max(calls, 1)
@@ -413,7 +421,8 @@ def main():
         "config": {"workload": f"Kron-{args.scale} ef16 undirected, resident; PR {args.pr_iters} it + "
                                f"BFS {len(roots)} roots + WCC + k-core k={args.k}",
                    "scale": args.scale, "vertices": n, "stored_edges": E, "nonempty_tiles": g.nonempty_tiles,
-                   "gen_convert_s": round(gen_s, 2), "parallelism": "single GPU"},
+                   "gen_convert_s": round(gen_s, 2), "parallelism": "single GPU",
+                   "tuned": {"chunk": chunk, "xcd_order": tune["xcd_order"] if tune else 1}},
         "per_algo": per_algo, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -180,9 +180,12 @@ def test_directed_parity(name):
     with ga.Graph.from_arrays(n, start, grid, deg, directed=True) as g:
         # directed compat ranks sum several in-neighbours (FP32 order-dependent): tolerance, not bit-exact
         got = g.pagerank(5, ga.PR_COMPAT)
-        assert np.max(np.abs(got - oracle.pagerank(n, start, grid, deg, 5, 0, directed=True))) <= PR_TOL
+        want = oracle.pagerank(n, start, grid, deg, 5, 0, directed=True)
+        assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) <= PR_TOL
+        # out-degree normalisation leaves ranks above 1 on a directed grid: relative FP32 tolerance
         got = g.pagerank(5, ga.PR_INTENDED)
-        assert np.max(np.abs(got - oracle.pagerank(n, start, grid, deg, 5, 1, directed=True))) <= PR_TOL
+        want = oracle.pagerank(n, start, grid, deg, 5, 1, directed=True)
+        assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) <= PR_TOL
         for root in (0, 1, int(np.argmax(deg))):
             assert np.array_equal(g.bfs(root).depth, oracle.bfs(n, start, grid, root, directed=True)[0])
         lab, nc = g.wcc()
