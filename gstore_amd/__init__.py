@@ -103,6 +103,7 @@ _SIGS = {
     "gs_set_sweep_order": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_chunk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "gs_set_pagerank_heavy": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_set_pagerank_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_bfs_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "gs_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_Tune)]),
     "gs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -255,6 +256,10 @@ class Graph:
     def set_bfs_mode(self, mode: int, alpha: int = 14, beta: int = 24) -> None:
         """0: tile sweep; 1: direction-optimizing over a neighbour-list index."""
         _check(_lib.gs_set_bfs_mode(self._h, mode, alpha, beta))
+
+    def set_pagerank_method(self, pull: bool) -> None:
+        """True: pull over the neighbour-list index; False: tile scatter with atomics."""
+        _check(_lib.gs_set_pagerank_method(self._h, int(pull)))
 
     def set_pagerank_heavy(self, on: bool) -> None:
         _check(_lib.gs_set_pagerank_heavy(self._h, int(on)))

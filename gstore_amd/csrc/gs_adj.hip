@@ -170,9 +170,61 @@ __global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *f
     }
 }
 
+// ---- compact list of vertices with neighbours (the pull sweep's segments) ---------
+
+__global__ void k_nz_block_counts(const int32_t *__restrict__ deg, uint64_t n, unsigned long long *bcnt) {
+    __shared__ uint32_t c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * 1024;
+    uint32_t k = 0;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) k += base + i < n && deg[base + i] != 0;
+    atomicAdd(&c, k);
+    __syncthreads();
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = c;
+}
+
+// Order-preserving compaction of one block of 1024 vertices (4 rounds of 256).
+__global__ void __launch_bounds__(256) k_nz_compact(const int32_t *__restrict__ deg, uint64_t n,
+                                                    const uint64_t *__restrict__ off,
+                                                    const unsigned long long *__restrict__ bstart, uint32_t *nzv,
+                                                    uint64_t *offz) {
+    __shared__ uint32_t wsum[4];
+    uint64_t pos = bstart[blockIdx.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int round = 0; round < 4; ++round) {
+        uint64_t v = (uint64_t)blockIdx.x * 1024 + round * 256 + threadIdx.x;
+        bool keep = v < n && deg[v] != 0;
+        unsigned long long m = __ballot(keep);
+        if (lane == 0) wsum[wv] = __popcll(m);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int i = 0; i < 4; ++i) {
+            before += i < wv ? wsum[i] : 0;
+            total += wsum[i];
+        }
+        if (keep) {
+            uint64_t p = pos + before + __popcll(m & ((1ull << lane) - 1));
+            nzv[p] = (uint32_t)v;
+            offz[p] = off[v];
+        }
+        pos += total;
+        __syncthreads();
+    }
+}
+
+// chunkz[c] = index (in nzv) of the vertex whose neighbour list holds entry c*PULL_CHUNK.
+__global__ void k_pull_chunk_map(const uint64_t *__restrict__ offz, uint64_t nzn, uint32_t *chunkz) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nzn; k += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t b = offz[k], e = offz[k + 1];
+        for (uint64_t c = (b + PULL_CHUNK - 1) / PULL_CHUNK; c * PULL_CHUNK < e; ++c) chunkz[c] = (uint32_t)k;
+    }
+}
+
 static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
 
 int build_adjacency(gs_graph *g) {
+    std::lock_guard<std::mutex> lk(g->build_mu);
     if (g->adj_nbr.ptr && g->adj_off.n == g->n + 1) return GS_OK;
     if (g->directed) { set_error("the adjacency index is built for undirected grids"); return GS_ERR_UNSUPPORTED; }
     gs_ws *w = g->ws;
@@ -201,10 +253,129 @@ int build_adjacency(gs_graph *g) {
     GS_TRY(sweep(g, w, GS_K_CONVERT, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
         k_adj_scatter<<<grid, SWEEP_WG, 0, st>>>(a, g->adj_off.ptr, cur.ptr, g->adj_nbr.ptr);
     }));
+    cur.release();
+    // vertices with neighbours, in id order, and the pull chunk map over their lists
+    k_nz_block_counts<<<(unsigned)nb, 256, 0, w->stream>>>(g->degree.ptr, g->n, bsum.ptr);
+    GS_HIP(hipGetLastError());
+    GS_HIP(hipMemcpyAsync(h.data(), bsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
+    GS_HIP(hipStreamSynchronize(w->stream));
+    run = 0;
+    for (uint64_t i = 0; i < nb; ++i) { unsigned long long c = h[i]; h[i] = run; run += c; }
+    g->nzn = run;
+    GS_HIP(hipMemcpyAsync(bsum.ptr, h.data(), nb * 8, hipMemcpyHostToDevice, w->stream));
+    GS_TRY(g->nzv.alloc(std::max<uint64_t>(g->nzn, 1)));
+    GS_TRY(g->offz.alloc(g->nzn + 1));
+    k_nz_compact<<<(unsigned)nb, 256, 0, w->stream>>>(g->degree.ptr, g->n, g->adj_off.ptr, bsum.ptr, g->nzv.ptr,
+                                                      g->offz.ptr);
+    GS_HIP(hipGetLastError());
+    uint64_t total = 2 * g->E;
+    GS_HIP(hipMemcpyAsync(g->offz.ptr + g->nzn, &total, 8, hipMemcpyHostToDevice, w->stream));
+    g->npull = (total + PULL_CHUNK - 1) / PULL_CHUNK;
+    GS_TRY(g->chunkz.alloc(g->npull + 1));
+    if (g->nzn) {
+        k_pull_chunk_map<<<vgrid(g->nzn), 256, 0, w->stream>>>(g->offz.ptr, g->nzn, g->chunkz.ptr);
+        GS_HIP(hipGetLastError());
+        uint32_t last = (uint32_t)(g->nzn - 1);
+        GS_HIP(hipMemcpyAsync(g->chunkz.ptr + g->npull, &last, 4, hipMemcpyHostToDevice, w->stream));
+    }
     GS_HIP(hipStreamSynchronize(w->stream));
     w->timer.collect();
     w->timer.reset();
     w->stats = gs_stats{};
+    return GS_OK;
+}
+
+// ---- PageRank pull over the neighbour lists ----------------------------------------
+// acc[v] = sum of prior over v's neighbours: one workgroup per PULL_CHUNK
+// consecutive list entries; lanes read 16 entries each (4 x 16-byte loads), find
+// their vertex by a search over the chunk's segment starts in LDS, and add
+// runs of entries of one vertex in registers, then into an LDS slot per vertex.
+// A vertex whose list lies inside the chunk is stored; one crossing a chunk
+// edge is added atomically (at most two per chunk).  No per-edge atomics.
+struct PullTiles {
+    uint16_t beg[PULL_CHUNK + 2];
+    uint32_t vid[PULL_CHUNK + 1];
+    float sum[PULL_CHUNK + 1];
+    uint8_t full[PULL_CHUNK + 1];
+    int nt;
+};
+
+__global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nbr, uint64_t nent,
+                                                 const uint32_t *__restrict__ nzv, const uint64_t *__restrict__ offz,
+                                                 const uint32_t *__restrict__ chunkz, uint64_t nchunk,
+                                                 const float *__restrict__ prior, float *acc) {
+    __shared__ PullTiles s;
+    const uint64_t chunk = blockIdx.x;
+    if (chunk >= nchunk) return;
+    const uint64_t base = chunk * PULL_CHUNK;
+    const uint32_t len = (uint32_t)min<uint64_t>(PULL_CHUNK, nent - base);
+    const uint32_t k0 = chunkz[chunk], k1 = chunkz[chunk + 1];
+    const int nt = (int)(k1 - k0) + 1;
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+        uint64_t b = offz[k0 + k], e = offz[k0 + k + 1];
+        s.beg[k] = (uint16_t)(b <= base ? 0 : min<uint64_t>(b - base, len));
+        s.vid[k] = nzv[k0 + k];
+        s.sum[k] = 0.0f;
+        s.full[k] = b >= base && e <= base + len;
+    }
+    if (threadIdx.x == 0) {
+        s.nt = nt;
+        s.beg[nt] = (uint16_t)len;
+    }
+    __syncthreads();
+    const uint32_t e0 = threadIdx.x * 16;
+    if (e0 < len) {
+        uint32_t idx[16];
+        const uint32_t *p = nbr + base + e0;
+        if (e0 + 16 <= len) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint4 v = reinterpret_cast<const uint4 *>(p)[q];
+                idx[4 * q] = v.x; idx[4 * q + 1] = v.y; idx[4 * q + 2] = v.z; idx[4 * q + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) idx[i] = e0 + i < len ? p[i] : 0u;
+        }
+        float val[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) val[i] = e0 + i < len ? prior[idx[i]] : 0.0f;
+        int lo = 0, hi = nt - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (s.beg[mid] <= e0) lo = mid; else hi = mid - 1;
+        }
+        int k = lo;
+        float run = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint32_t e = e0 + i;
+            if (e >= len) break;
+            while (s.beg[k + 1] <= e) {
+                if (run != 0.0f) atomicAdd(&s.sum[k], run);
+                run = 0.0f;
+                ++k;
+            }
+            run += val[i];
+        }
+        if (run != 0.0f) atomicAdd(&s.sum[k], run);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+        float x = s.sum[k];
+        if (s.full[k]) acc[s.vid[k]] = x;
+        else if (x != 0.0f) unsafeAtomicAdd(&acc[s.vid[k]], x);
+    }
+}
+
+int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc) {
+    if (!g->npull) return GS_OK;
+    hipEvent_t kb;
+    w->timer.begin(GS_K_PR_SCATTER, w->stream, &kb);
+    k_pr_pull<<<(unsigned)g->npull, 256, 0, w->stream>>>(g->adj_nbr.ptr, 2 * g->E, g->nzv.ptr, g->offz.ptr,
+                                                         g->chunkz.ptr, g->npull, prior, acc);
+    GS_HIP(hipGetLastError());
+    w->timer.end(GS_K_PR_SCATTER, w->stream, kb);
     return GS_OK;
 }
 

@@ -708,6 +708,7 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->segment_edges = g->ss ? g->ss->seg_chunks * g->chunk : 0;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
                          g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() +
+                         g->nzv.bytes() + g->offz.bytes() + g->chunkz.bytes() +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
     return GS_OK;
 }
@@ -793,6 +794,13 @@ int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta) {
 int gs_set_pagerank_heavy(gs_graph *g, int on) {
     if (!g) { set_error("null graph"); return GS_ERR_ARG; }
     g->heavy_on = on != 0;
+    g->heavy_ready = false;
+    return GS_OK;
+}
+
+int gs_set_pagerank_method(gs_graph *g, int pull) {
+    if (!g) { set_error("null graph"); return GS_ERR_ARG; }
+    g->pr_pull = pull != 0;
     g->heavy_ready = false;
     return GS_OK;
 }

@@ -16,6 +16,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,6 +26,7 @@
 namespace gs {
 
 constexpr int CHUNK = 4096;       // largest (and default) edges per workgroup sweep
+constexpr int PULL_CHUNK = 4096;  // neighbour-list entries per workgroup of the PageRank pull
 constexpr int SWEEP_WG = 256;     // threads per sweep workgroup (4 waves)
 constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr uint32_t ROW_WORDS = 65536 / 32;   // bitmap words per tile row
@@ -172,6 +175,12 @@ struct gs_graph {
     int do_alpha = 14, do_beta = 24;         // top-down <-> bottom-up switch thresholds
     gs::DevBuf<uint64_t> adj_off;            // neighbour-list offsets, N + 1 (built on demand)
     gs::DevBuf<uint32_t> adj_nbr;            // neighbour ids, 2E
+    uint64_t nzn = 0, npull = 0;             // vertices with neighbours; pull chunks
+    gs::DevBuf<uint32_t> nzv;                // those vertices, ascending
+    gs::DevBuf<uint64_t> offz;               // their list offsets, nzn + 1
+    gs::DevBuf<uint32_t> chunkz;             // nzv index holding entry c*PULL_CHUNK, npull + 1
+    int pr_pull = 1;                         // PageRank by pull over the lists (undirected, resident)
+    std::mutex build_mu;                     // lazy builds (index, heavy set) from concurrent workspaces
     ~gs_graph();
 };
 
@@ -195,6 +204,7 @@ int run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint
 int run_bfs_do(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
                int max_levels, int *levels);
 int build_adjacency(gs_graph *g);
+int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc);
 int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
                   uint64_t *level_counts, int max_levels, int *levels);
 int run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp);

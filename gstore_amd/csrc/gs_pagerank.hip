@@ -166,9 +166,18 @@ __global__ void k_ppr_finalize(uint64_t n, uint64_t src, float *__restrict__ pri
 
 static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
 
+static bool use_pull(const gs_graph *g);
+
 // Heavy set: vertices of degree >= 2^b for the smallest b leaving at most
 // HEAVY_MAX of them (none below degree 256).  Built once per graph.
 int ensure_heavy(gs_graph *g, hipStream_t st) {
+    if (use_pull(g)) {
+        g->heavy_dthr = 0xFFFFFFFFu;  // the pull has no atomics to relieve
+        g->nheavy = 0;
+        g->heavy_ready = false;
+        return build_adjacency(g);
+    }
+    std::lock_guard<std::mutex> lk(g->build_mu);
     if (g->heavy_ready) return GS_OK;
     g->heavy_ready = true;
     g->heavy_dthr = 0xFFFFFFFFu;
@@ -202,8 +211,13 @@ int ensure_heavy(gs_graph *g, hipStream_t st) {
     return GS_OK;
 }
 
-// One scatter pass: the heavy-vertex kernel when the graph has a heavy set.
+static bool use_pull(const gs_graph *g) { return g->pr_pull && !g->directed && !g->ss; }
+
+// One accumulation pass: pull over the neighbour lists when the graph has
+// them, else the tile scatter (with the heavy-vertex kernel when there is a
+// heavy set).
 static int scatter(const gs_graph *g, gs_ws *w) {
+    if (use_pull(g)) return pr_pull(g, w, w->pr_prior.ptr, w->pr_acc.ptr);
     if (g->nheavy && !g->directed) {
         return sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned, hipStream_t st) {
             unsigned grid = (unsigned)std::min<uint64_t>(a.nlaunch, (uint64_t)g->persistent_wgs);

@@ -211,6 +211,14 @@ void gs_server_destroy(gs_server *s);
  * and level counts are identical in both modes.  Undirected grids only. */
 int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta);
 
+/* PageRank / personalised PageRank method on undirected resident grids:
+ * 1 (default) pulls over the neighbour-list index (built once from the grid,
+ * as for direction-optimizing BFS): each vertex's sum is formed in registers
+ * and LDS and stored, with no per-edge atomics; 0 scatters over the tile grid
+ * with FP32 atomics, the reference's scheme (pr.cpp:119-154).  Directed and
+ * semi-external grids always scatter.  Only the FP32 summation order differs. */
+int gs_set_pagerank_method(gs_graph *g, int pull);
+
 /* PageRank / personalised PageRank on undirected grids: 1 (default) adds
  * into the (at most 2048) highest-degree vertices through a per-workgroup LDS
  * table flushed once per workgroup, instead of one contended global atomic

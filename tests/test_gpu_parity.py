@@ -309,17 +309,23 @@ def test_bfs_direction_optimizing_bit_exact(loaded, name):
     g.set_bfs_mode(0)
 
 
-def test_pagerank_heavy_table_same_results(loaded):
-    """LDS accumulation of the heaviest vertices changes only FP32 summation order."""
-    n, (start, grid, deg), g = loaded["kron16"]
+@pytest.mark.parametrize("name", ["kron16", "two_groups", "star", "tiles"])
+def test_pagerank_methods_same_results(loaded, name):
+    """Tile scatter (plain atomics or heavy-vertex LDS table) and the pull over
+    neighbour lists change only the FP32 summation order."""
+    n, (start, grid, deg), g = loaded[name]
     want_c = oracle.pagerank(n, start, grid, deg, 10, 0)
     want_i = oracle.pagerank(n, start, grid, deg, 10, 1)
-    for on in (False, True):
-        g.set_pagerank_heavy(on)
-        _say("heavy", on)
+    src = int(np.argmax(deg))
+    want_p = oracle.ppr(n, start, grid, deg, src, 20)
+    for pull, heavy in ((False, False), (False, True), (True, True)):
+        g.set_pagerank_method(pull)
+        g.set_pagerank_heavy(heavy)
+        _say("pr method", name, "pull", pull, "heavy", heavy)
         assert np.array_equal(g.pagerank(10, ga.PR_COMPAT), want_c)
         assert np.max(np.abs(g.pagerank(10, ga.PR_INTENDED) - want_i)) <= PR_TOL
         assert np.array_equal(g.pagerank(0, ga.PR_COMPAT), np.full(n, np.float32(1.0 / n)))
-        p = g.ppr(int(np.argmax(deg)), 20)
-        assert p.min() >= 0
+        p = g.ppr(src, 20)
+        assert p.min() >= 0 and float(np.abs(p - want_p).sum()) <= 1e-6
+    g.set_pagerank_method(True)
     g.set_pagerank_heavy(True)
