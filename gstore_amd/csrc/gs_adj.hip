@@ -1,17 +1,21 @@
-// gs_adj.hip -- per-vertex adjacency index built from the tile grid, and the
-// direction-optimizing BFS that uses it.
+// gs_adj.hip -- neighbour-list index built from the tile grid, and the
+// direction-optimizing BFS and PageRank pull that use it.
 //
 // The tile grid stores each undirected edge once, in the tile of (min, max);
 // a vertex's neighbours are spread over its whole tile row and tile column,
-// so a BFS level over tiles has to read every edge of every active tile.
-// For levels with a small frontier (top-down) and levels where most vertices
-// are already reached (bottom-up, each unreached vertex stops at its first
-// frontier neighbour) an index of neighbour lists is far cheaper.  The index
-// is built once from the grid on the device (offsets = exclusive scan of the
-// .degree file, one sweep scattering both directions of every stored edge)
-// and kept with the graph.  Level structure, frontier bitmaps and the depth
-// encoding are those of the tile BFS (bfs.cpp:56-224), so depths are
-// bit-identical whichever direction each level takes.
+// so a BFS level over tiles has to read every edge of every active tile, and
+// a PageRank iteration over tiles needs an atomic add per edge endpoint.  The
+// index is built once from the grid on the device and kept with the graph:
+//   * vertices are relabelled in descending degree order (log-spaced degree
+//     buckets), so the per-vertex values of the heavily shared vertices sit in
+//     few cache lines that stay in L2 / the Infinity Cache; vertices without
+//     edges come last;
+//   * off_r (u64, N+1) is the exclusive scan of the relabelled degrees and
+//     nbr_r (u32, 2E) holds every stored edge in both directions, as
+//     relabelled ids, filled by one sweep over the grid.
+// Results are mapped back to the original ids at the end of each call.  BFS
+// keeps the level structure, frontier bitmaps and depth encoding of the tile
+// BFS (bfs.cpp:56-224), so depths are bit-identical in either mode.
 #include "gs_internal.h"
 
 namespace gs {
@@ -63,16 +67,17 @@ __global__ void k_deg_scan(const int32_t *__restrict__ deg, uint64_t n, const un
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) off[n] = before + run;
 }
 
-__global__ void __launch_bounds__(SWEEP_WG) k_adj_scatter(SweepArgs a, const uint64_t *__restrict__ off,
-                                                          uint32_t *cur, uint32_t *nbr) {
+__global__ void __launch_bounds__(SWEEP_WG) k_adj_scatter(SweepArgs a, const uint32_t *__restrict__ new_id,
+                                                          const uint64_t *__restrict__ off, uint32_t *cur,
+                                                          uint32_t *nbr) {
     __shared__ ChunkTiles s;
     uint64_t chunk;
     if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
     auto op = [&](uint32_t w, uint32_t rc) {
-        uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
-        nbr[off[v0] + atomicAdd(&cur[v0], 1u)] = v1;
-        nbr[off[v1] + atomicAdd(&cur[v1], 1u)] = v0;
+        uint32_t p0 = new_id[gv0(w, rc)], p1 = new_id[gv1(w, rc)];
+        nbr[off[p0] + atomicAdd(&cur[p0], 1u)] = p1;
+        nbr[off[p1] + atomicAdd(&cur[p1], 1u)] = p0;
     };
     sweep_chunk<false>(a, chunk, s, len, op);
 }
@@ -170,47 +175,70 @@ __global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *f
     }
 }
 
-// ---- compact list of vertices with neighbours (the pull sweep's segments) ---------
+// ---- relabelling by descending degree ------------------------------------------------
 
-__global__ void k_nz_block_counts(const int32_t *__restrict__ deg, uint64_t n, unsigned long long *bcnt) {
-    __shared__ uint32_t c;
-    if (threadIdx.x == 0) c = 0;
+// Bucket of a degree: 8 sub-buckets per power of two; 0 for no edges.  Higher
+// bucket = earlier position.
+__device__ inline uint32_t deg_bucket(uint32_t d) {
+    if (!d) return 0;
+    uint32_t lg = 31 - __clz(d);
+    uint32_t sub = lg >= 3 ? (d >> (lg - 3)) & 7 : 0;
+    return 1 + lg * 8 + sub;  // 1 .. 256
+}
+constexpr int NBUCKET = 257;
+
+__global__ void k_bucket_hist(const int32_t *__restrict__ deg, uint64_t n, unsigned long long *hist) {
+    __shared__ uint32_t h[NBUCKET];
+    for (int i = threadIdx.x; i < NBUCKET; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    uint64_t base = (uint64_t)blockIdx.x * 1024;
-    uint32_t k = 0;
-    for (int i = threadIdx.x; i < 1024; i += blockDim.x) k += base + i < n && deg[base + i] != 0;
-    atomicAdd(&c, k);
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[deg_bucket((uint32_t)deg[v])], 1u);
     __syncthreads();
-    if (threadIdx.x == 0) bcnt[blockIdx.x] = c;
+    for (int i = threadIdx.x; i < NBUCKET; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
 }
 
-// Order-preserving compaction of one block of 1024 vertices (4 rounds of 256).
-__global__ void __launch_bounds__(256) k_nz_compact(const int32_t *__restrict__ deg, uint64_t n,
-                                                    const uint64_t *__restrict__ off,
-                                                    const unsigned long long *__restrict__ bstart, uint32_t *nzv,
-                                                    uint64_t *offz) {
-    __shared__ uint32_t wsum[4];
-    uint64_t pos = bstart[blockIdx.x];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int round = 0; round < 4; ++round) {
-        uint64_t v = (uint64_t)blockIdx.x * 1024 + round * 256 + threadIdx.x;
-        bool keep = v < n && deg[v] != 0;
-        unsigned long long m = __ballot(keep);
-        if (lane == 0) wsum[wv] = __popcll(m);
-        __syncthreads();
-        uint32_t before = 0, total = 0;
-        for (int i = 0; i < 4; ++i) {
-            before += i < wv ? wsum[i] : 0;
-            total += wsum[i];
-        }
-        if (keep) {
-            uint64_t p = pos + before + __popcll(m & ((1ull << lane) - 1));
-            nzv[p] = (uint32_t)v;
-            offz[p] = off[v];
-        }
-        pos += total;
-        __syncthreads();
+// Each workgroup takes 4096 vertices: local ranks per bucket in LDS, one global
+// reservation per (workgroup, bucket), then new_id / old_of.
+__global__ void __launch_bounds__(256) k_relabel(const int32_t *__restrict__ deg, uint64_t n,
+                                                 unsigned long long *cursor, uint32_t *new_id, uint32_t *old_of) {
+    __shared__ uint32_t h[NBUCKET];
+    __shared__ unsigned long long base[NBUCKET];
+    for (int i = threadIdx.x; i < NBUCKET; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint64_t v0 = (uint64_t)blockIdx.x * 4096;
+    uint32_t key[16], rank[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        uint64_t v = v0 + i * 256 + threadIdx.x;
+        key[i] = v < n ? deg_bucket((uint32_t)deg[v]) : 0;
+        rank[i] = v < n ? atomicAdd(&h[key[i]], 1u) : 0;
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NBUCKET; i += blockDim.x)
+        if (h[i]) base[i] = atomicAdd(&cursor[i], (unsigned long long)h[i]);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        uint64_t v = v0 + i * 256 + threadIdx.x;
+        if (v < n) {
+            uint64_t p = base[key[i]] + rank[i];
+            new_id[v] = (uint32_t)p;
+            old_of[p] = (uint32_t)v;
+        }
+    }
+}
+
+__global__ void k_gather_deg(const int32_t *__restrict__ deg, const uint32_t *__restrict__ old_of, uint64_t n,
+                             int32_t *deg_r) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < n; p += (uint64_t)gridDim.x * blockDim.x)
+        deg_r[p] = deg[old_of[p]];
+}
+
+template <class T>
+__global__ void k_unpermute(const T *__restrict__ x_r, const uint32_t *__restrict__ new_id, uint64_t n, T *out) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = x_r[new_id[v]];
 }
 
 // chunkz[c] = index (in nzv) of the vertex whose neighbour list holds entry c*PULL_CHUNK.
@@ -228,57 +256,69 @@ int build_adjacency(gs_graph *g) {
     if (g->adj_nbr.ptr && g->adj_off.n == g->n + 1) return GS_OK;
     if (g->directed) { set_error("the adjacency index is built for undirected grids"); return GS_ERR_UNSUPPORTED; }
     gs_ws *w = g->ws;
-    const uint64_t nb = (g->n + 1023) / 1024;
+    hipStream_t st = w->stream;
+    const uint64_t n = g->n;
+    // 1. relabel: histogram of degree buckets, descending bucket order
+    DevBuf<unsigned long long> cur64;
+    GS_TRY(cur64.alloc(NBUCKET));
+    GS_HIP(hipMemsetAsync(cur64.ptr, 0, NBUCKET * 8, st));
+    k_bucket_hist<<<vgrid(n), 256, 0, st>>>(g->degree.ptr, n, cur64.ptr);
+    GS_HIP(hipGetLastError());
+    std::vector<unsigned long long> h(NBUCKET);
+    GS_HIP(hipMemcpyAsync(h.data(), cur64.ptr, NBUCKET * 8, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    g->nzn = n - h[0];
+    unsigned long long run = 0;
+    for (int b = NBUCKET - 1; b >= 0; --b) { unsigned long long c = h[b]; h[b] = run; run += c; }
+    GS_HIP(hipMemcpyAsync(cur64.ptr, h.data(), NBUCKET * 8, hipMemcpyHostToDevice, st));
+    GS_TRY(g->new_id.alloc(n));
+    DevBuf<uint32_t> old_of;
+    GS_TRY(old_of.alloc(n));
+    k_relabel<<<(unsigned)((n + 4095) / 4096), 256, 0, st>>>(g->degree.ptr, n, cur64.ptr, g->new_id.ptr, old_of.ptr);
+    GS_HIP(hipGetLastError());
+    GS_TRY(g->deg_r.alloc(n));
+    k_gather_deg<<<vgrid(n), 256, 0, st>>>(g->degree.ptr, old_of.ptr, n, g->deg_r.ptr);
+    GS_HIP(hipGetLastError());
+    old_of.release();
+    // 2. offsets = exclusive scan of the relabelled degrees
+    const uint64_t nb = (n + 1023) / 1024;
     DevBuf<unsigned long long> bsum;
     GS_TRY(bsum.alloc(nb));
-    GS_TRY(g->adj_off.alloc(g->n + 1));
-    GS_TRY(g->adj_nbr.alloc(2 * g->E));
-    k_deg_block_sums<<<(unsigned)nb, 256, 0, w->stream>>>(g->degree.ptr, g->n, bsum.ptr);
+    GS_TRY(g->adj_off.alloc(n + 1));
+    GS_TRY(g->adj_nbr.alloc(std::max<uint64_t>(2 * g->E, 1)));
+    k_deg_block_sums<<<(unsigned)nb, 256, 0, st>>>(g->deg_r.ptr, n, bsum.ptr);
     GS_HIP(hipGetLastError());
-    std::vector<unsigned long long> h(nb);
-    GS_HIP(hipMemcpyAsync(h.data(), bsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
-    GS_HIP(hipStreamSynchronize(w->stream));
-    unsigned long long run = 0;
-    for (uint64_t i = 0; i < nb; ++i) { unsigned long long c = h[i]; h[i] = run; run += c; }
+    std::vector<unsigned long long> hb(nb);
+    GS_HIP(hipMemcpyAsync(hb.data(), bsum.ptr, nb * 8, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    run = 0;
+    for (uint64_t i = 0; i < nb; ++i) { unsigned long long c = hb[i]; hb[i] = run; run += c; }
     if (run != 2 * g->E) {
         set_error("degree sum %llu differs from twice the stored edges %llu", run, (unsigned long long)(2 * g->E));
         return GS_ERR_FORMAT;
     }
-    GS_HIP(hipMemcpyAsync(bsum.ptr, h.data(), nb * 8, hipMemcpyHostToDevice, w->stream));
-    k_deg_scan<<<(unsigned)nb, 256, 0, w->stream>>>(g->degree.ptr, g->n, bsum.ptr, g->adj_off.ptr);
+    GS_HIP(hipMemcpyAsync(bsum.ptr, hb.data(), nb * 8, hipMemcpyHostToDevice, st));
+    k_deg_scan<<<(unsigned)nb, 256, 0, st>>>(g->deg_r.ptr, n, bsum.ptr, g->adj_off.ptr);
     GS_HIP(hipGetLastError());
+    // 3. neighbour lists, relabelled, by one sweep over the grid
     DevBuf<uint32_t> cur;
-    GS_TRY(cur.alloc(g->n));
-    GS_HIP(hipMemsetAsync(cur.ptr, 0, g->n * 4, w->stream));
-    GS_TRY(sweep(g, w, GS_K_CONVERT, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
-        k_adj_scatter<<<grid, SWEEP_WG, 0, st>>>(a, g->adj_off.ptr, cur.ptr, g->adj_nbr.ptr);
+    GS_TRY(cur.alloc(n));
+    GS_HIP(hipMemsetAsync(cur.ptr, 0, n * 4, st));
+    GS_TRY(sweep(g, w, GS_K_CONVERT, [&](const SweepArgs &a, unsigned grid, hipStream_t s2) {
+        k_adj_scatter<<<grid, SWEEP_WG, 0, s2>>>(a, g->new_id.ptr, g->adj_off.ptr, cur.ptr, g->adj_nbr.ptr);
     }));
     cur.release();
-    // vertices with neighbours, in id order, and the pull chunk map over their lists
-    k_nz_block_counts<<<(unsigned)nb, 256, 0, w->stream>>>(g->degree.ptr, g->n, bsum.ptr);
-    GS_HIP(hipGetLastError());
-    GS_HIP(hipMemcpyAsync(h.data(), bsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
-    GS_HIP(hipStreamSynchronize(w->stream));
-    run = 0;
-    for (uint64_t i = 0; i < nb; ++i) { unsigned long long c = h[i]; h[i] = run; run += c; }
-    g->nzn = run;
-    GS_HIP(hipMemcpyAsync(bsum.ptr, h.data(), nb * 8, hipMemcpyHostToDevice, w->stream));
-    GS_TRY(g->nzv.alloc(std::max<uint64_t>(g->nzn, 1)));
-    GS_TRY(g->offz.alloc(g->nzn + 1));
-    k_nz_compact<<<(unsigned)nb, 256, 0, w->stream>>>(g->degree.ptr, g->n, g->adj_off.ptr, bsum.ptr, g->nzv.ptr,
-                                                      g->offz.ptr);
-    GS_HIP(hipGetLastError());
-    uint64_t total = 2 * g->E;
-    GS_HIP(hipMemcpyAsync(g->offz.ptr + g->nzn, &total, 8, hipMemcpyHostToDevice, w->stream));
+    // 4. pull chunk map over the vertices with edges (relabelled ids 0 .. nzn-1)
+    const uint64_t total = 2 * g->E;
     g->npull = (total + PULL_CHUNK - 1) / PULL_CHUNK;
     GS_TRY(g->chunkz.alloc(g->npull + 1));
     if (g->nzn) {
-        k_pull_chunk_map<<<vgrid(g->nzn), 256, 0, w->stream>>>(g->offz.ptr, g->nzn, g->chunkz.ptr);
+        k_pull_chunk_map<<<vgrid(g->nzn), 256, 0, st>>>(g->adj_off.ptr, g->nzn, g->chunkz.ptr);
         GS_HIP(hipGetLastError());
         uint32_t last = (uint32_t)(g->nzn - 1);
-        GS_HIP(hipMemcpyAsync(g->chunkz.ptr + g->npull, &last, 4, hipMemcpyHostToDevice, w->stream));
+        GS_HIP(hipMemcpyAsync(g->chunkz.ptr + g->npull, &last, 4, hipMemcpyHostToDevice, st));
     }
-    GS_HIP(hipStreamSynchronize(w->stream));
+    GS_HIP(hipStreamSynchronize(st));
     w->timer.collect();
     w->timer.reset();
     w->stats = gs_stats{};
@@ -292,6 +332,8 @@ int build_adjacency(gs_graph *g) {
 // runs of entries of one vertex in registers, then into an LDS slot per vertex.
 // A vertex whose list lies inside the chunk is stored; one crossing a chunk
 // edge is added atomically (at most two per chunk).  No per-edge atomics.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 struct PullTiles {
     uint16_t beg[PULL_CHUNK + 2];
     uint32_t vid[PULL_CHUNK + 1];
@@ -301,7 +343,7 @@ struct PullTiles {
 };
 
 __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nbr, uint64_t nent,
-                                                 const uint32_t *__restrict__ nzv, const uint64_t *__restrict__ offz,
+                                                 const uint64_t *__restrict__ offz,
                                                  const uint32_t *__restrict__ chunkz, uint64_t nchunk,
                                                  const float *__restrict__ prior, float *acc) {
     __shared__ PullTiles s;
@@ -314,7 +356,7 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
     for (int k = threadIdx.x; k < nt; k += blockDim.x) {
         uint64_t b = offz[k0 + k], e = offz[k0 + k + 1];
         s.beg[k] = (uint16_t)(b <= base ? 0 : min<uint64_t>(b - base, len));
-        s.vid[k] = nzv[k0 + k];
+        s.vid[k] = k0 + k;
         s.sum[k] = 0.0f;
         s.full[k] = b >= base && e <= base + len;
     }
@@ -330,8 +372,9 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
         if (e0 + 16 <= len) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                uint4 v = reinterpret_cast<const uint4 *>(p)[q];
-                idx[4 * q] = v.x; idx[4 * q + 1] = v.y; idx[4 * q + 2] = v.z; idx[4 * q + 3] = v.w;
+                // streamed once per iteration: non-temporal, so it does not evict the gathered values
+                u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p) + q);
+                idx[4 * q] = v[0]; idx[4 * q + 1] = v[1]; idx[4 * q + 2] = v[2]; idx[4 * q + 3] = v[3];
             }
         } else {
 #pragma unroll
@@ -368,12 +411,18 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
     }
 }
 
+int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out) {
+    k_unpermute<float><<<vgrid(g->n), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, out);
+    GS_HIP(hipGetLastError());
+    return GS_OK;
+}
+
 int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc) {
     if (!g->npull) return GS_OK;
     hipEvent_t kb;
     w->timer.begin(GS_K_PR_SCATTER, w->stream, &kb);
-    k_pr_pull<<<(unsigned)g->npull, 256, 0, w->stream>>>(g->adj_nbr.ptr, 2 * g->E, g->nzv.ptr, g->offz.ptr,
-                                                         g->chunkz.ptr, g->npull, prior, acc);
+    k_pr_pull<<<(unsigned)g->npull, 256, 0, w->stream>>>(g->adj_nbr.ptr, 2 * g->E, g->adj_off.ptr, g->chunkz.ptr,
+                                                         g->npull, prior, acc);
     GS_HIP(hipGetLastError());
     w->timer.end(GS_K_PR_SCATTER, w->stream, kb);
     return GS_OK;
@@ -393,6 +442,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     GS_TRY(build_adjacency(g));
     GS_TRY(ensure_bitmaps(g, w));
     if (w->depth.n != g->n) GS_TRY(w->depth.alloc(g->n));
+    if (w->depth_r.n != g->n) GS_TRY(w->depth_r.alloc(g->n));
     if (w->queue.n != g->n) GS_TRY(w->queue.alloc(g->n));
     const uint64_t words = bitmap_words(g);
     w->timer.reset();
@@ -401,17 +451,20 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t0, w->stream));
-    GS_HIP(hipMemsetAsync(w->depth.ptr, 0, g->n, w->stream));
+    GS_HIP(hipMemsetAsync(w->depth_r.ptr, 0, g->n, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_vis.ptr, 0, words * 4, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_front.ptr, 0, words * 4, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_next.ptr, 0, words * 4, w->stream));
-    k_do_root<<<1, 1, 0, w->stream>>>((uint32_t)root, w->depth.ptr, w->bm_vis.ptr, w->bm_front.ptr, w->queue.ptr);
+    uint32_t root_r = 0;
+    GS_HIP(hipMemcpyAsync(&root_r, g->new_id.ptr + root, 4, hipMemcpyDeviceToHost, w->stream));
+    GS_HIP(hipStreamSynchronize(w->stream));
+    k_do_root<<<1, 1, 0, w->stream>>>(root_r, w->depth_r.ptr, w->bm_vis.ptr, w->bm_front.ptr, w->queue.ptr);
     GS_HIP(hipGetLastError());
     // heuristic state (Beamer et al.): edges still to check from the unreached side
     uint64_t m_u = 2 * g->E, n_f = 1, m_f = 0;
     {
         int32_t d = 0;
-        GS_HIP(hipMemcpyAsync(&d, g->degree.ptr + root, 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipMemcpyAsync(&d, g->deg_r.ptr + root_r, 4, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
         m_f = (uint32_t)d;
         m_u -= m_f;
@@ -436,7 +489,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
         w->timer.end(GS_K_BFS_EXPAND, w->stream, kb);
         w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
         k_do_level_end<<<vgrid(words), 256, 0, w->stream>>>(w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
-                                                            w->depth.ptr, level + 1, g->degree.ptr, words,
+                                                            w->depth_r.ptr, level + 1, g->deg_r.ptr, words,
                                                             w->queue.ptr, w->counters.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
@@ -455,6 +508,8 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
             break;
         }
     }
+    k_unpermute<uint8_t><<<vgrid(g->n), 256, 0, w->stream>>>(w->depth_r.ptr, g->new_id.ptr, g->n, w->depth.ptr);
+    GS_HIP(hipGetLastError());
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;

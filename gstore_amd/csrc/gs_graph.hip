@@ -111,7 +111,7 @@ size_t gs_ws::device_bytes() const {
     return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
            bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + kc_deg.bytes() + ms_vis.bytes() +
            ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_counts.bytes() +
-           queue.bytes() + counters.bytes();
+           queue.bytes() + depth_r.bytes() + counters.bytes();
 }
 
 gs_graph::~gs_graph() {
@@ -708,7 +708,7 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->segment_edges = g->ss ? g->ss->seg_chunks * g->chunk : 0;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
                          g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() +
-                         g->nzv.bytes() + g->offz.bytes() + g->chunkz.bytes() +
+                         g->new_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
     return GS_OK;
 }

@@ -122,6 +122,7 @@ struct gs_ws {
     gs::DevBuf<uint64_t> ms_vis, ms_front, ms_next, ms_roots;  // batched BFS lane masks
     gs::DevBuf<uint8_t> ms_depth;                             // lane-major depth, lanes x N
     gs::DevBuf<uint32_t> queue;                               // frontier vertices (direction-optimizing BFS)
+    gs::DevBuf<uint8_t> depth_r;                              // depth by relabelled id
     gs::DevBuf<unsigned long long> ms_counts;
     gs::DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;  // pinned mirror of counters
@@ -173,12 +174,13 @@ struct gs_graph {
     gs::DevBuf<uint32_t> heavy_ids;          // sorted heavy vertex ids
     int persistent_wgs = 1024;               // workgroups of the persistent PageRank scatter
     int do_alpha = 14, do_beta = 24;         // top-down <-> bottom-up switch thresholds
-    gs::DevBuf<uint64_t> adj_off;            // neighbour-list offsets, N + 1 (built on demand)
-    gs::DevBuf<uint32_t> adj_nbr;            // neighbour ids, 2E
-    uint64_t nzn = 0, npull = 0;             // vertices with neighbours; pull chunks
-    gs::DevBuf<uint32_t> nzv;                // those vertices, ascending
-    gs::DevBuf<uint64_t> offz;               // their list offsets, nzn + 1
-    gs::DevBuf<uint32_t> chunkz;             // nzv index holding entry c*PULL_CHUNK, npull + 1
+    // neighbour-list index (built on demand), in relabelled ids (descending degree)
+    gs::DevBuf<uint32_t> new_id;             // relabelled id of each vertex, N
+    gs::DevBuf<int32_t> deg_r;               // degree by relabelled id, N
+    gs::DevBuf<uint64_t> adj_off;            // list offsets by relabelled id, N + 1
+    gs::DevBuf<uint32_t> adj_nbr;            // neighbour ids (relabelled), 2E
+    uint64_t nzn = 0, npull = 0;             // vertices with edges (ids 0 .. nzn-1); pull chunks
+    gs::DevBuf<uint32_t> chunkz;             // vertex whose list holds entry c*PULL_CHUNK, npull + 1
     int pr_pull = 1;                         // PageRank by pull over the lists (undirected, resident)
     std::mutex build_mu;                     // lazy builds (index, heavy set) from concurrent workspaces
     ~gs_graph();
@@ -205,6 +207,7 @@ int run_bfs_do(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, u
                int max_levels, int *levels);
 int build_adjacency(gs_graph *g);
 int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc);
+int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out);
 int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
                   uint64_t *level_counts, int max_levels, int *levels);
 int run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp);

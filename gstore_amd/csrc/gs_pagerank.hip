@@ -252,9 +252,11 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t0, w->stream));
-    k_pr_init<<<vgrid(g->n), 256, 0, w->stream>>>(g->degree.ptr, g->n, mode, (float)(1.0 / (double)g->n),
-                                                  iters ? g->heavy_dthr : 0xFFFFFFFFu, w->pr_prior.ptr,
-                                                  w->pr_acc.ptr, w->pr_fac.ptr);
+    // the pull works in the index's relabelled ids; results are mapped back at the end
+    const bool pull = use_pull(g);
+    k_pr_init<<<vgrid(g->n), 256, 0, w->stream>>>(pull ? g->deg_r.ptr : g->degree.ptr, g->n, mode,
+                                                  (float)(1.0 / (double)g->n), iters ? g->heavy_dthr : 0xFFFFFFFFu,
+                                                  w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
     GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
@@ -264,6 +266,10 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
                                                           it + 1 == iters);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
+    }
+    if (pull) {  // prior (relabelled) -> acc (original order) -> prior
+        GS_TRY(unpermute_f32(g, w->stream, w->pr_prior.ptr, w->pr_acc.ptr));
+        GS_HIP(hipMemcpyAsync(w->pr_prior.ptr, w->pr_acc.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
     }
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_TRY(sync_and_collect(w));
@@ -300,17 +306,33 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t0, w->stream));
-    k_ppr_init<<<vgrid(g->n), 256, 0, w->stream>>>(g->degree.ptr, g->n, src, g->heavy_dthr, w->pr_prior.ptr,
-                                                   w->pr_acc.ptr, w->pr_fac.ptr);
+    const bool pull = use_pull(g);
+    uint64_t s_id = src;
+    if (pull) {
+        uint32_t r = 0;
+        GS_HIP(hipMemcpyAsync(&r, g->new_id.ptr + src, 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        s_id = r;
+    }
+    k_ppr_init<<<vgrid(g->n), 256, 0, w->stream>>>(pull ? g->deg_r.ptr : g->degree.ptr, g->n, s_id, g->heavy_dthr,
+                                                   w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
     GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
         GS_TRY(scatter(g, w));
         w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
-        k_ppr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, src, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr,
-                                                           it + 1 == iters ? w->ppr_rank.ptr : nullptr);
+        k_ppr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, s_id, w->pr_prior.ptr, w->pr_acc.ptr,
+                                                           w->pr_fac.ptr, it + 1 == iters ? w->ppr_rank.ptr : nullptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
+    }
+    if (pull) {  // rank (relabelled) -> acc (original order) -> rank
+        GS_TRY(unpermute_f32(g, w->stream, w->ppr_rank.ptr, w->pr_acc.ptr));
+        GS_HIP(hipMemcpyAsync(w->ppr_rank.ptr, w->pr_acc.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+    }
+    if (pull) {  // prior (relabelled) -> acc (original order) -> prior
+        GS_TRY(unpermute_f32(g, w->stream, w->pr_prior.ptr, w->pr_acc.ptr));
+        GS_HIP(hipMemcpyAsync(w->pr_prior.ptr, w->pr_acc.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
     }
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_TRY(sync_and_collect(w));
