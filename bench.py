@@ -295,7 +295,7 @@ def main():
     tune = None
     if args.tune:
         tune = g.tune()
-        log(f"autotune: chunk {tune['chunk']} xcd_order {tune['xcd_order']} ({tune['best_ms']:.2f} ms/PR iteration)")
+        log(f"autotune: chunk {tune['chunk']} xcd_order {tune['xcd_order']} ({tune['best_ms']:.2f} ms per WCC)")
     bfs_mode = args.bfs_mode
     if bfs_mode < 0 and "bfs" in algos:
         best = None

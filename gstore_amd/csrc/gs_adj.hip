@@ -334,19 +334,21 @@ int build_adjacency(gs_graph *g) {
 // edge is added atomically (at most two per chunk).  No per-edge atomics.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+template <class A>  // accumulator type: float (PageRank) or double (personalised PageRank)
 struct PullTiles {
     uint16_t beg[PULL_CHUNK + 2];
     uint32_t vid[PULL_CHUNK + 1];
-    float sum[PULL_CHUNK + 1];
+    A sum[PULL_CHUNK + 1];
     uint8_t full[PULL_CHUNK + 1];
     int nt;
 };
 
+template <class A>
 __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nbr, uint64_t nent,
                                                  const uint64_t *__restrict__ offz,
                                                  const uint32_t *__restrict__ chunkz, uint64_t nchunk,
-                                                 const float *__restrict__ prior, float *acc) {
-    __shared__ PullTiles s;
+                                                 const float *__restrict__ prior, A *acc) {
+    __shared__ PullTiles<A> s;
     const uint64_t chunk = blockIdx.x;
     if (chunk >= nchunk) return;
     const uint64_t base = chunk * PULL_CHUNK;
@@ -357,7 +359,7 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
         uint64_t b = offz[k0 + k], e = offz[k0 + k + 1];
         s.beg[k] = (uint16_t)(b <= base ? 0 : min<uint64_t>(b - base, len));
         s.vid[k] = k0 + k;
-        s.sum[k] = 0.0f;
+        s.sum[k] = A(0);
         s.full[k] = b >= base && e <= base + len;
     }
     if (threadIdx.x == 0) {
@@ -389,25 +391,25 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
             if (s.beg[mid] <= e0) lo = mid; else hi = mid - 1;
         }
         int k = lo;
-        float run = 0.0f;
+        A run = A(0);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             uint32_t e = e0 + i;
             if (e >= len) break;
             while (s.beg[k + 1] <= e) {
-                if (run != 0.0f) atomicAdd(&s.sum[k], run);
-                run = 0.0f;
+                if (run != A(0)) atomicAdd(&s.sum[k], run);
+                run = A(0);
                 ++k;
             }
-            run += val[i];
+            run += (A)val[i];
         }
-        if (run != 0.0f) atomicAdd(&s.sum[k], run);
+        if (run != A(0)) atomicAdd(&s.sum[k], run);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < nt; k += blockDim.x) {
-        float x = s.sum[k];
+        A x = s.sum[k];
         if (s.full[k]) acc[s.vid[k]] = x;
-        else if (x != 0.0f) unsafeAtomicAdd(&acc[s.vid[k]], x);
+        else if (x != A(0)) unsafeAtomicAdd(&acc[s.vid[k]], x);
     }
 }
 
@@ -417,12 +419,16 @@ int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *ou
     return GS_OK;
 }
 
-int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc) {
+int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double *acc64) {
     if (!g->npull) return GS_OK;
     hipEvent_t kb;
     w->timer.begin(GS_K_PR_SCATTER, w->stream, &kb);
-    k_pr_pull<<<(unsigned)g->npull, 256, 0, w->stream>>>(g->adj_nbr.ptr, 2 * g->E, g->adj_off.ptr, g->chunkz.ptr,
-                                                         g->npull, prior, acc);
+    if (acc64)
+        k_pr_pull<double><<<(unsigned)g->npull, 256, 0, w->stream>>>(g->adj_nbr.ptr, 2 * g->E, g->adj_off.ptr,
+                                                                     g->chunkz.ptr, g->npull, prior, acc64);
+    else
+        k_pr_pull<float><<<(unsigned)g->npull, 256, 0, w->stream>>>(g->adj_nbr.ptr, 2 * g->E, g->adj_off.ptr,
+                                                                    g->chunkz.ptr, g->npull, prior, acc);
     GS_HIP(hipGetLastError());
     w->timer.end(GS_K_PR_SCATTER, w->stream, kb);
     return GS_OK;

@@ -108,7 +108,7 @@ gs_ws::~gs_ws() {
 }
 
 size_t gs_ws::device_bytes() const {
-    return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
+    return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + ppr_acc.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
            bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + kc_deg.bytes() + ms_vis.bytes() +
            ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_counts.bytes() +
            queue.bytes() + depth_r.bytes() + counters.bytes();

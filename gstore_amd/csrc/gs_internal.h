@@ -115,6 +115,7 @@ struct KernelTimer {
 struct gs_ws {
     hipStream_t stream = nullptr;
     gs::DevBuf<float> pr_prior, pr_acc, pr_fac, ppr_rank;
+    gs::DevBuf<double> ppr_acc;                                // personalised PageRank accumulates in FP64
     gs::DevBuf<uint8_t> depth;
     gs::DevBuf<uint32_t> bm_vis, bm_front, bm_next, rowact;  // bitmaps, words
     gs::DevBuf<uint32_t> wcc_parent;
@@ -206,7 +207,7 @@ int run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint
 int run_bfs_do(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
                int max_levels, int *levels);
 int build_adjacency(gs_graph *g);
-int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc);
+int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double *acc64);
 int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out);
 int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
                   uint64_t *level_counts, int max_levels, int *levels);

@@ -41,8 +41,8 @@ __global__ void k_pr_init(const int32_t *__restrict__ deg, uint64_t n, int mode,
     }
 }
 
-template <bool DIRECTED>
-__global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter(SweepArgs a, const float *__restrict__ prior, float *acc) {
+template <bool DIRECTED, class A>
+__global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter(SweepArgs a, const float *__restrict__ prior, A *acc) {
     __shared__ ChunkTiles s;
     uint64_t chunk;
     if (!launch_chunk(a, chunk)) return;
@@ -52,9 +52,9 @@ __global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter(SweepArgs a, const floa
         float p0 = unmark(prior[v0]);
         if (!DIRECTED) {
             float p1 = unmark(prior[v1]);
-            unsafeAtomicAdd(&acc[v0], p1);
+            unsafeAtomicAdd(&acc[v0], (A)p1);
         }
-        unsafeAtomicAdd(&acc[v1], p0);
+        unsafeAtomicAdd(&acc[v1], (A)p0);
     };
     sweep_chunk<false>(a, chunk, s, len, op);
 }
@@ -65,15 +65,16 @@ __global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter(SweepArgs a, const floa
 // LDS table to the heavy vertices' acc once.
 constexpr int HEAVY_MAX = 2048;
 
+template <class A>
 __global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter_heavy(SweepArgs a, const float *__restrict__ prior,
-                                                               float *acc, const uint32_t *__restrict__ heavy,
+                                                               A *acc, const uint32_t *__restrict__ heavy,
                                                                int nheavy) {
     __shared__ ChunkTiles s;
     __shared__ uint32_t hid[HEAVY_MAX];
-    __shared__ float hacc[HEAVY_MAX];
+    __shared__ A hacc[HEAVY_MAX];
     for (int i = threadIdx.x; i < nheavy; i += blockDim.x) {
         hid[i] = heavy[i];
-        hacc[i] = 0.0f;
+        hacc[i] = A(0);
     }
     const uint64_t G = gridDim.x;
     const uint64_t rank = (blockIdx.x % NUM_XCD) * ((G + NUM_XCD - 1) / NUM_XCD) + blockIdx.x / NUM_XCD;
@@ -86,9 +87,9 @@ __global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter_heavy(SweepArgs a, cons
                 int m = (l + r) >> 1;
                 if (hid[m] < v) l = m + 1; else r = m;
             }
-            atomicAdd(&hacc[l], x);
+            atomicAdd(&hacc[l], (A)x);
         } else {
-            unsafeAtomicAdd(&acc[v], x);
+            unsafeAtomicAdd(&acc[v], (A)x);
         }
     };
     auto op = [&](uint32_t w, uint32_t rc) {
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(SWEEP_WG) k_pr_scatter_heavy(SweepArgs a, cons
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nheavy; i += blockDim.x)
-        if (hacc[i] != 0.0f) unsafeAtomicAdd(&acc[hid[i]], hacc[i]);
+        if (hacc[i] != A(0)) unsafeAtomicAdd(&acc[hid[i]], hacc[i]);
 }
 
 // Degree threshold selection: histogram of floor(log2(degree)).
@@ -140,26 +141,28 @@ __global__ void k_pr_finalize(uint64_t n, float *__restrict__ prior, float *__re
 
 // Personalised PageRank from one source s (no reference counterpart; same
 // scatter as above): r0 = e_s, prior = r * (float)(1/deg),
-// r' = 0.15 [v == s] + 0.85 * sum of prior over in-edges.
+// r' = 0.15 [v == s] + 0.85 * sum of prior over in-edges.  Ranks and priors
+// are FP32; the per-vertex sums accumulate in FP64 so the L1 error against an
+// FP64 recurrence stays below 1e-6 whatever the summation order.
 __global__ void k_ppr_init(const int32_t *__restrict__ deg, uint64_t n, uint64_t src, uint32_t dthr, float *prior,
-                           float *acc, float *fac) {
+                           double *acc, float *fac) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         int32_t d = deg[v];
         float f = d != 0 ? (float)(1.0 / (double)d) : 0.0f;
         bool h = (uint32_t)d >= dthr;
         fac[v] = mark(f, h);
         prior[v] = mark(v == src ? f : 0.0f, h);
-        acc[v] = 0.0f;
+        acc[v] = 0.0;
     }
 }
 
-__global__ void k_ppr_finalize(uint64_t n, uint64_t src, float *__restrict__ prior, float *__restrict__ acc,
+__global__ void k_ppr_finalize(uint64_t n, uint64_t src, float *__restrict__ prior, double *__restrict__ acc,
                                const float *__restrict__ fac, float *__restrict__ rank) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         float f = fac[v];
-        float r = __double2float_rn(__dadd_rn(v == src ? 0.15 : 0.0, __dmul_rn(0.85, (double)acc[v])));
+        float r = __double2float_rn(__dadd_rn(v == src ? 0.15 : 0.0, __dmul_rn(0.85, acc[v])));
         prior[v] = mark(__fmul_rn(r, unmark(f)), is_heavy(f));
-        acc[v] = 0.0f;
+        acc[v] = 0.0;
         if (rank) rank[v] = r;
     }
 }
@@ -216,19 +219,25 @@ static bool use_pull(const gs_graph *g) { return g->pr_pull && !g->directed && !
 // One accumulation pass: pull over the neighbour lists when the graph has
 // them, else the tile scatter (with the heavy-vertex kernel when there is a
 // heavy set).
-static int scatter(const gs_graph *g, gs_ws *w) {
-    if (use_pull(g)) return pr_pull(g, w, w->pr_prior.ptr, w->pr_acc.ptr);
+template <class A>
+static int scatter_t(const gs_graph *g, gs_ws *w, A *acc) {
     if (g->nheavy && !g->directed) {
         return sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned, hipStream_t st) {
             unsigned grid = (unsigned)std::min<uint64_t>(a.nlaunch, (uint64_t)g->persistent_wgs);
-            k_pr_scatter_heavy<<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr, g->heavy_ids.ptr,
-                                                          g->nheavy);
+            k_pr_scatter_heavy<A><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, acc, g->heavy_ids.ptr, g->nheavy);
         });
     }
     return sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
-        if (g->directed) k_pr_scatter<true><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
-        else k_pr_scatter<false><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, w->pr_acc.ptr);
+        if (g->directed) k_pr_scatter<true, A><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, acc);
+        else k_pr_scatter<false, A><<<grid, SWEEP_WG, 0, st>>>(a, w->pr_prior.ptr, acc);
     });
+}
+
+// acc64 non-null: accumulate in FP64 (personalised PageRank).
+static int scatter(const gs_graph *g, gs_ws *w, double *acc64 = nullptr) {
+    if (use_pull(g)) return pr_pull(g, w, w->pr_prior.ptr, w->pr_acc.ptr, acc64);
+    if (acc64) return scatter_t<double>(g, w, acc64);
+    return scatter_t<float>(g, w, w->pr_acc.ptr);
 }
 
 }  // namespace gs
@@ -299,6 +308,7 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
         GS_TRY(w->pr_fac.alloc(g->n));
     }
     if (w->ppr_rank.n != g->n) GS_TRY(w->ppr_rank.alloc(g->n));
+    if (w->ppr_acc.n != g->n) GS_TRY(w->ppr_acc.alloc(g->n));
     GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream));
     w->timer.reset();
     w->stats = gs_stats{};
@@ -315,13 +325,13 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
         s_id = r;
     }
     k_ppr_init<<<vgrid(g->n), 256, 0, w->stream>>>(pull ? g->deg_r.ptr : g->degree.ptr, g->n, s_id, g->heavy_dthr,
-                                                   w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
+                                                   w->pr_prior.ptr, w->ppr_acc.ptr, w->pr_fac.ptr);
     GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
-        GS_TRY(scatter(g, w));
+        GS_TRY(scatter(g, w, w->ppr_acc.ptr));
         w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
-        k_ppr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, s_id, w->pr_prior.ptr, w->pr_acc.ptr,
+        k_ppr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, s_id, w->pr_prior.ptr, w->ppr_acc.ptr,
                                                            w->pr_fac.ptr, it + 1 == iters ? w->ppr_rank.ptr : nullptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);

@@ -2,8 +2,9 @@
 //
 // Knobs: edges per workgroup (the tile chunk each workgroup stages, 1024 /
 // 2048 / 4096) and the chunk-to-XCD order (contiguous eighths vs round-robin).
-// Each candidate is timed on two PageRank iterations (the sweep every job
-// shares); the fastest is applied.  With a cache path, results are kept per
+// Each candidate is timed on one WCC call, whose hook and check passes are
+// plain full sweeps of the grid (the sweep every tile job shares); the
+// fastest is applied.  With a cache path, results are kept per
 // graph signature (N, E, non-empty tiles, directed) so a later open of the
 // same graph applies them without timing.
 #include <stdio.h>
@@ -76,9 +77,9 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
         if (!g->ss) GS_TRY(gs_set_chunk(g, chunks[ci]));
         for (int order = 0; order < 2; ++order, ++k) {
             g->xcd_order = order;
-            GS_TRY(run_pagerank(g, g->ws, 1, GS_PR_INTENDED, nullptr));  // warm
-            GS_TRY(run_pagerank(g, g->ws, 2, GS_PR_INTENDED, nullptr));
-            double ms = g->ws->stats.ms / 2;
+            GS_TRY(run_wcc(g, g->ws, nullptr, nullptr));  // warm
+            GS_TRY(run_wcc(g, g->ws, nullptr, nullptr));
+            double ms = g->ws->stats.ms;
             if (k < 6) {
                 r.cand_chunk[k] = chunks[ci];
                 r.cand_order[k] = order;
@@ -101,7 +102,7 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%s | chunk=%u order=%d pr_iter_ms=%.3f\n", sig.c_str(), g->chunk, g->xcd_order, best);
+            fprintf(f, "%s | chunk=%u order=%d wcc_ms=%.3f\n", sig.c_str(), g->chunk, g->xcd_order, best);
             fclose(f);
         }
     }

@@ -131,7 +131,8 @@ int gs_pagerank(gs_graph *g, int iters, int mode, float *rank_out);
 /* Personalised PageRank from one source (no reference counterpart; same
  * tile scatter as gs_pagerank): r_0 = e_source, prior = r * (float)(1/deg),
  * r_{t+1} = 0.15 [v == source] + 0.85 * sum of prior over neighbours.
- * rank_out[v] = r after `iters` iterations (FP32). */
+ * rank_out[v] = r after `iters` iterations (FP32 ranks, FP64 per-vertex
+ * sums). */
 int gs_ppr(gs_graph *g, uint64_t source, int iters, float *rank_out);
 
 /* BFS from root (bfs_t, bfs.cpp:56-338; driver bfs_mmap gstore.cpp:2631-2713;
@@ -237,8 +238,9 @@ int gs_set_chunk(gs_graph *g, uint32_t edges);
  * it; gs_tune picks the faster one. */
 int gs_set_sweep_order(gs_graph *g, int xcd_contiguous);
 
-/* Autotuner: times two PageRank iterations for each edges-per-workgroup
- * value (1024, 2048, 4096) and both sweep orders, applies the fastest, and
+/* Autotuner: times one WCC call (two full grid sweeps) for each
+ * edges-per-workgroup value (1024, 2048, 4096) and both sweep orders,
+ * applies the fastest, and
  * appends it to cache_path (optional) keyed by N, E, non-empty tiles and
  * directedness; a cached entry is applied without timing. */
 typedef struct gs_tune_result {
@@ -246,7 +248,7 @@ typedef struct gs_tune_result {
     int32_t xcd_order;
     int32_t from_cache;
     int32_t ncand;
-    double best_ms;           /* ms per PageRank iteration of the chosen setting */
+    double best_ms;           /* ms of the timed WCC call for the chosen setting */
     uint32_t cand_chunk[6];
     int32_t cand_order[6];
     double cand_ms[6];
