@@ -33,8 +33,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, n, E, nz, nchunk, calls, scanned):
-    """Algorithmic HBM bytes per launch of a kernel class (averaged over calls)."""
+def kernel_bytes(name, n, E, nz, nchunk, calls, scanned, idx=None):
+    """Algorithmic HBM bytes per launch of a kernel class (averaged over calls).
+    idx: neighbour-list index sizes {"entries", "vertices"} once it is built."""
+    if name == "pr_pull" and idx and idx["entries"]:
+        # neighbour ids + offsets + prior (each value once) + acc write + chunk map
+        nv, npull = idx["vertices"], (idx["entries"] + 4095) // 4096
+        return 4 * idx["entries"] + 8 * (nv + 1) + 4 * nv + 4 * nv + 4 * (npull + 1)
     if name == "pr_scatter":      # edges + tile metadata + prior read + acc read-modify-write
         return 4 * E + 12 * nz + 4 * (nchunk + 1) + 12 * n
     if name == "pr_finalize":     # acc read + factor read + prior write + acc clear
@@ -256,6 +261,8 @@ def main():
                     help="subset of the step's jobs (profiling runs); the default is the full step")
     ap.add_argument("--bfs-mode", type=int, default=-1, help="-1: autotune; 0: tile sweep; 1: direction-optimizing")
     ap.add_argument("--tune", type=int, default=1, help="run the sweep autotuner before timing")
+    ap.add_argument("--pr-method", choices=["pull", "scatter"], default="pull",
+                    help="PageRank over the neighbour-list index (pull) or the tile scatter with atomics")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -309,6 +316,7 @@ def main():
                 best = (mode, ms)
         bfs_mode = best[0]
     g.set_bfs_mode(max(bfs_mode, 0))
+    g.set_pagerank_method(args.pr_method == "pull")
     g.set_kernel_timing(True)
     kern = {}
     per = {"pagerank": [0.0, 0.0], "bfs": [0.0, 0.0, 0.0], "wcc": [0.0, 0.0], "kcore": [0.0, 0.0]}
@@ -373,6 +381,7 @@ def main():
     per_algo = {}
     if "pr" in algos:
         per_algo["pagerank"] = {"ms": per["pagerank"][0] / args.steps, "iterations": args.pr_iters,
+                                "method": args.pr_method,
                                 "gteps": rate(per["pagerank"][1], per["pagerank"][0])}
     if "bfs" in algos:
         per_algo["bfs"] = {"ms_per_root": per["bfs"][0] / args.steps / len(roots), "roots": len(roots),
@@ -391,17 +400,18 @@ def main():
 
     chunk = tune["chunk"] if tune else 4096
     nchunk = (E + chunk - 1) // chunk
+    idx = g.index_info()
     kernels = {}
     for name, (ms, calls, scanned) in kern.items():
         avg_ms = ms / max(calls, 1)
-        b = kernel_bytes(name, n, E, g.nonempty_tiles, nchunk, calls, scanned)
+        b = kernel_bytes(name, n, E, g.nonempty_tiles, nchunk, calls, scanned, idx)
         kernels[name] = {"total_ms": round(ms / args.steps, 3), "launches_per_step": calls / args.steps,
                          "avg_ms": round(avg_ms, 4),
                          "alg_gbs": round(b / (avg_ms / 1e3) / 1e9, 1) if b else None}
     dom = max(kern, key=lambda k: kern[k][0])
     d_ms, d_calls, d_scanned = kern[dom]
     d_avg = d_ms / d_calls
-    d_bytes = kernel_bytes(dom, n, E, g.nonempty_tiles, nchunk, d_calls, d_scanned)
+    d_bytes = kernel_bytes(dom, n, E, g.nonempty_tiles, nchunk, d_calls, d_scanned, idx)
     achieved = d_bytes / (d_avg / 1e3) / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -422,6 +432,7 @@ def main():
                                f"BFS {len(roots)} roots + WCC + k-core k={args.k}",
                    "scale": args.scale, "vertices": n, "stored_edges": E, "nonempty_tiles": g.nonempty_tiles,
                    "gen_convert_s": round(gen_s, 2), "parallelism": "single GPU",
+                   "index": {"entries": idx["entries"], "build_ms": round(idx["build_ms"], 1)},
                    "tuned": {"chunk": chunk, "xcd_order": tune["xcd_order"] if tune else 1}},
         "per_algo": per_algo, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -33,7 +33,7 @@ GS_OK = 0
 _STATUS = {1: "GS_ERR_ARG", 2: "GS_ERR_IO", 3: "GS_ERR_FORMAT", 4: "GS_ERR_NOMEM",
            5: "GS_ERR_DEVICE", 6: "GS_ERR_UNSUPPORTED", 7: "GS_ERR_DEPTH_OVERFLOW"}
 KERNELS = ["pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end", "wcc_hook", "wcc_compress",
-           "wcc_check", "kcore_peel", "kcore_scan", "kron_gen", "convert", "h2d_copy"]
+           "wcc_check", "kcore_peel", "kcore_scan", "kron_gen", "convert", "h2d_copy", "pr_pull", "index_build"]
 
 
 class GStoreError(RuntimeError):
@@ -48,7 +48,8 @@ class _Info(ctypes.Structure):
                 ("groups", ctypes.c_uint64), ("tiles_per_side", ctypes.c_uint64),
                 ("device_bytes", ctypes.c_uint64), ("resident_edges", ctypes.c_uint64),
                 ("segment_edges", ctypes.c_uint64), ("directed", ctypes.c_int32),
-                ("resident", ctypes.c_int32)]
+                ("resident", ctypes.c_int32), ("index_entries", ctypes.c_uint64),
+                ("index_vertices", ctypes.c_uint64), ("index_ms", ctypes.c_double)]
 
 
 class _Stats(ctypes.Structure):
@@ -173,6 +174,13 @@ class Graph:
         self.resident = bool(info.resident)
         self.resident_edges = int(info.resident_edges)
         self.segment_edges = int(info.segment_edges)
+
+    def index_info(self) -> dict:
+        """Neighbour-list index: entries (2E), vertices with edges, build ms (zeros until built)."""
+        info = _Info()
+        _check(_lib.gs_get_info(self._h, ctypes.byref(info)))
+        return {"entries": int(info.index_entries), "vertices": int(info.index_vertices),
+                "build_ms": float(info.index_ms), "device_bytes": int(info.device_bytes)}
 
     # --- construction -------------------------------------------------------
     @classmethod

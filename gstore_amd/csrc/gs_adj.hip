@@ -258,6 +258,10 @@ int build_adjacency(gs_graph *g) {
     gs_ws *w = g->ws;
     hipStream_t st = w->stream;
     const uint64_t n = g->n;
+    hipEvent_t t0, t1;
+    GS_HIP(hipEventCreate(&t0));
+    GS_HIP(hipEventCreate(&t1));
+    GS_HIP(hipEventRecord(t0, st));
     // 1. relabel: histogram of degree buckets, descending bucket order
     DevBuf<unsigned long long> cur64;
     GS_TRY(cur64.alloc(NBUCKET));
@@ -318,7 +322,13 @@ int build_adjacency(gs_graph *g) {
         uint32_t last = (uint32_t)(g->nzn - 1);
         GS_HIP(hipMemcpyAsync(g->chunkz.ptr + g->npull, &last, 4, hipMemcpyHostToDevice, st));
     }
+    GS_HIP(hipEventRecord(t1, st));
     GS_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    GS_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    g->index_ms = ms;
     w->timer.collect();
     w->timer.reset();
     w->stats = gs_stats{};
@@ -422,7 +432,7 @@ int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *ou
 int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double *acc64) {
     if (!g->npull) return GS_OK;
     hipEvent_t kb;
-    w->timer.begin(GS_K_PR_SCATTER, w->stream, &kb);
+    w->timer.begin(GS_K_PR_PULL, w->stream, &kb);
     if (acc64)
         k_pr_pull<double><<<(unsigned)g->npull, 256, 0, w->stream>>>(g->adj_nbr.ptr, 2 * g->E, g->adj_off.ptr,
                                                                      g->chunkz.ptr, g->npull, prior, acc64);
@@ -430,7 +440,7 @@ int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double 
         k_pr_pull<float><<<(unsigned)g->npull, 256, 0, w->stream>>>(g->adj_nbr.ptr, 2 * g->E, g->adj_off.ptr,
                                                                     g->chunkz.ptr, g->npull, prior, acc);
     GS_HIP(hipGetLastError());
-    w->timer.end(GS_K_PR_SCATTER, w->stream, kb);
+    w->timer.end(GS_K_PR_PULL, w->stream, kb);
     return GS_OK;
 }
 

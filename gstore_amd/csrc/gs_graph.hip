@@ -538,7 +538,8 @@ size_t gs_last_error(char *buf, size_t len) {
 const char *gs_kernel_name(int k) {
     static const char *names[GS_K_COUNT] = {"pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end",
                                             "wcc_hook", "wcc_compress", "wcc_check", "kcore_peel",
-                                            "kcore_scan", "kron_gen", "convert", "h2d_copy"};
+                                            "kcore_scan", "kron_gen", "convert", "h2d_copy", "pr_pull",
+                                            "index_build"};
     return (k >= 0 && k < GS_K_COUNT) ? names[k] : "?";
 }
 
@@ -710,6 +711,10 @@ int gs_get_info(gs_graph *g, gs_info *info) {
                          g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() +
                          g->new_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
+    const bool built = g->adj_nbr.ptr && g->adj_off.n == g->n + 1;
+    info->index_entries = built ? 2 * g->E : 0;
+    info->index_vertices = built ? g->nzn : 0;
+    info->index_ms = built ? g->index_ms : 0.0;
     return GS_OK;
 }
 

@@ -181,6 +181,7 @@ struct gs_graph {
     gs::DevBuf<uint64_t> adj_off;            // list offsets by relabelled id, N + 1
     gs::DevBuf<uint32_t> adj_nbr;            // neighbour ids (relabelled), 2E
     uint64_t nzn = 0, npull = 0;             // vertices with edges (ids 0 .. nzn-1); pull chunks
+    double index_ms = 0;                     // device time of the index build
     gs::DevBuf<uint32_t> chunkz;             // vertex whose list holds entry c*PULL_CHUNK, npull + 1
     int pr_pull = 1;                         // PageRank by pull over the lists (undirected, resident)
     std::mutex build_mu;                     // lazy builds (index, heavy set) from concurrent workspaces

@@ -2,9 +2,10 @@
 //
 // Knobs: edges per workgroup (the tile chunk each workgroup stages, 1024 /
 // 2048 / 4096) and the chunk-to-XCD order (contiguous eighths vs round-robin).
-// Each candidate is timed on one WCC call, whose hook and check passes are
-// plain full sweeps of the grid (the sweep every tile job shares); the
-// fastest is applied.  With a cache path, results are kept per
+// Each candidate is timed on one WCC call (its hook and check passes are full
+// sweeps of the grid) plus, for undirected graphs, one k-core call with k = 8
+// (sweeps that skip inactive tiles, whose cost depends on the knobs
+// differently); the smallest sum is applied.  With a cache path, results are kept per
 // graph signature (N, E, non-empty tiles, directed) so a later open of the
 // same graph applies them without timing.
 #include <stdio.h>
@@ -80,6 +81,10 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
             GS_TRY(run_wcc(g, g->ws, nullptr, nullptr));  // warm
             GS_TRY(run_wcc(g, g->ws, nullptr, nullptr));
             double ms = g->ws->stats.ms;
+            if (!g->directed) {
+                GS_TRY(run_kcore(g, g->ws, 8, nullptr, nullptr, nullptr, nullptr));
+                ms += g->ws->stats.ms;
+            }
             if (k < 6) {
                 r.cand_chunk[k] = chunks[ci];
                 r.cand_order[k] = order;
@@ -102,7 +107,7 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%s | chunk=%u order=%d wcc_ms=%.3f\n", sig.c_str(), g->chunk, g->xcd_order, best);
+            fprintf(f, "%s | chunk=%u order=%d sweep_ms=%.3f\n", sig.c_str(), g->chunk, g->xcd_order, best);
             fclose(f);
         }
     }

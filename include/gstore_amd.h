@@ -42,7 +42,7 @@ enum gs_pr_mode {
 enum gs_kernel {
     GS_K_PR_SCATTER = 0, GS_K_PR_FINALIZE, GS_K_BFS_EXPAND, GS_K_BFS_LEVEL_END,
     GS_K_WCC_HOOK, GS_K_WCC_COMPRESS, GS_K_WCC_CHECK, GS_K_KCORE_PEEL, GS_K_KCORE_SCAN,
-    GS_K_GEN, GS_K_CONVERT, GS_K_H2D, GS_K_COUNT
+    GS_K_GEN, GS_K_CONVERT, GS_K_H2D, GS_K_PR_PULL, GS_K_INDEX, GS_K_COUNT
 };
 
 typedef struct gs_info {
@@ -57,6 +57,9 @@ typedef struct gs_info {
     uint64_t segment_edges;   /* edges per streamed segment (0 when resident) */
     int32_t directed;
     int32_t resident;         /* 1: whole grid in HBM */
+    uint64_t index_entries;   /* neighbour-list index entries (2E once built, else 0) */
+    uint64_t index_vertices;  /* vertices with at least one stored edge (index built) */
+    double index_ms;          /* device time of the index build (0 until built) */
 } gs_info;
 
 typedef struct gs_stats {
