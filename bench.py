@@ -11,7 +11,10 @@ Workload (one "step" = one job of each algorithm on a resident graph):
     k-core        k = 8 (-j 22 -a 8)
 value = aggregate GTEPS over the step, using per algorithm: PageRank 2E per
 iteration; BFS Graph500 M_reach per root (stored edges in the reached
-component); WCC 2 x edges swept; k-core 2 x edges in swept tiles.
+component); WCC and k-core 2E per pass / round.  Each algorithm also reports
+its fraction of HBM peak from fixed algorithmic-byte formulas (independent of
+the GPU method; BFS and k-core scans counted with the reference's tile-skip
+rule), and the roofline entry is one PageRank iteration.
 
 Prints one JSON line (stdout).  Progress goes to stderr.
 """
@@ -33,26 +36,45 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, n, E, nz, nchunk, calls, scanned, idx=None):
-    """Algorithmic HBM bytes per launch of a kernel class (averaged over calls).
-    idx: neighbour-list index sizes {"entries", "vertices"} once it is built."""
-    if name == "pr_pull" and idx and idx["entries"]:
-        # neighbour ids + offsets + prior (each value once) + acc write + chunk map
-        nv, npull = idx["vertices"], (idx["entries"] + 4095) // 4096
-        return 4 * idx["entries"] + 8 * (nv + 1) + 4 * nv + 4 * nv + 4 * (npull + 1)
-    if name == "pr_scatter":      # edges + tile metadata + prior read + acc read-modify-write
-        return 4 * E + 12 * nz + 4 * (nchunk + 1) + 12 * n
-    if name == "pr_finalize":     # acc read + factor read + prior write + acc clear
-        return 16 * n
-    if name == "wcc_hook":        # edges + metadata + parent read/write
-        return 4 * E + 12 * nz + 8 * n
-    if name == "wcc_check":
-        return 4 * E + 12 * nz + 4 * n
-    if name == "bfs_expand":      # active edges + frontier/visited/next bitmaps
-        return (4 * scanned) / max(calls, 1) + 3 * n / 8
-    if name == "kcore_scan":      # active edges + active bitmap + degree read-modify-write
-        return (4 * scanned) / max(calls, 1) + n / 8 + 8 * n
-    return None
+def alg_bytes_pr_iter(n, E, T):
+    """PageRank, one iteration: tiles + offsets + 24 B per vertex (prior read,
+    accumulator read-modify-write and read, degree factor read, new prior
+    write).  Fixed formula, independent of the GPU method."""
+    return 4 * E + 8 * (T + 1) + 24 * n
+
+
+def alg_bytes_bfs(n, e_scan, t_scan, levels, reached):
+    """BFS, one root: per level 4 B per scanned edge + 8 B per scanned tile
+    (reference tile-skip rule) + three N-bit bitmaps, plus 1 B per reached vertex."""
+    return 4 * e_scan + 8 * t_scan + levels * 3 * n / 8 + reached
+
+
+def alg_bytes_wcc(n, E, T, passes, jumps):
+    return passes * (4 * E + 8 * (T + 1) + 8 * n) + jumps * 8 * n
+
+
+def alg_bytes_kcore(n, e_scan, t_scan, rounds):
+    return 4 * e_scan + 8 * t_scan + rounds * (4 * n + n / 8)
+
+
+def measured_traffic(workload, method, iter_ms):
+    """HBM-side bytes of one PageRank iteration measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration), kept in
+    profiles/traffic.json with the iteration time of that run.  Used only when
+    this run's iteration time is within 30% of it (same kernels)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload, {}).get(method)
+    except (OSError, ValueError):
+        return None
+    if not rec or abs(iter_ms - rec["iteration_ms"]) > 0.3 * rec["iteration_ms"]:
+        return None
+    return {"bytes": int(rec["fetch_bytes"] + rec["write_bytes"]), "source": rec["source"]}
+
+
+# kernel classes of one PageRank iteration, per method
+PR_ITER_KERNELS = ("pb_gather", "pb_accum", "pb_reduce", "pr_pull", "pr_scatter", "pr_finalize")
 
 
 def cpu_baseline(scale, threads):
@@ -261,8 +283,9 @@ def main():
                     help="subset of the step's jobs (profiling runs); the default is the full step")
     ap.add_argument("--bfs-mode", type=int, default=-1, help="-1: autotune; 0: tile sweep; 1: direction-optimizing")
     ap.add_argument("--tune", type=int, default=1, help="run the sweep autotuner before timing")
-    ap.add_argument("--pr-method", choices=["pull", "scatter"], default="pull",
-                    help="PageRank over the neighbour-list index (pull) or the tile scatter with atomics")
+    ap.add_argument("--pr-method", choices=["blocked", "pull", "scatter"], default="blocked",
+                    help="PageRank: two streamed passes over the blocked index copy, pull over the "
+                         "neighbour-list index, or the tile scatter with atomics")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -290,13 +313,19 @@ def main():
     rng = np.random.default_rng(20240601)
     roots = [int(r) for r in rng.choice(np.flatnonzero(deg), args.bfs_roots, replace=False)]
 
-    # Graph500 M_reach per root (untimed pre-pass; BFS output is deterministic)
-    m_reach = {}
+    # Untimed pre-pass with the tile BFS (its scans follow the reference's
+    # tile-skip rule): Graph500 M_reach and the scanned edges / tiles of each
+    # root, for the fixed algorithmic-bytes formula.  BFS output is
+    # deterministic, so these hold for every BFS mode.
+    m_reach, ref_scan = {}, {}
     deg64 = deg.astype(np.int64)
+    g.set_bfs_mode(0)
     for r in roots:
         d = g.bfs(r).depth
+        st = g.stats()
         m_reach[r] = int(deg64[d > 0].sum() // 2)
-    log("BFS M_reach per root computed")
+        ref_scan[r] = (st["edges_scanned"], st["tiles_scanned"], st["iterations"], int(np.count_nonzero(d)))
+    log("BFS M_reach and reference scan counts per root computed")
 
     algos = set(args.algos.split(","))
     tune = None
@@ -316,14 +345,17 @@ def main():
                 best = (mode, ms)
         bfs_mode = best[0]
     g.set_bfs_mode(max(bfs_mode, 0))
-    g.set_pagerank_method(args.pr_method == "pull")
+    g.set_pagerank_method({"scatter": 0, "pull": 1, "blocked": 2}[args.pr_method])
     g.set_kernel_timing(True)
     kern = {}
-    per = {"pagerank": [0.0, 0.0], "bfs": [0.0, 0.0, 0.0], "wcc": [0.0, 0.0], "kcore": [0.0, 0.0]}
+    T = g.T
+    # per algorithm: device ms, traversed edges (GTEPS convention), algorithmic bytes
+    per = {k: [0.0, 0.0, 0.0] for k in ("pagerank", "bfs", "wcc", "kcore")}
+    bfs_scanned = [0.0]
 
     def absorb(st):
         for k, v in st["kernels"].items():
-            a = kern.setdefault(k, [0.0, 0, 0])
+            a = kern.setdefault(k, [0.0, 0])
             a[0] += v["ms"]
             a[1] += v["calls"]
 
@@ -335,31 +367,36 @@ def main():
                 absorb(st)
                 per["pagerank"][0] += st["ms"]
                 per["pagerank"][1] += 2 * E * args.pr_iters
+                per["pagerank"][2] += args.pr_iters * alg_bytes_pr_iter(n, E, T)
         for r in (roots if "bfs" in algos else []):
             g.bfs(r, copy_out=False)
             st = g.stats()
             if record:
                 absorb(st)
-                kern.setdefault("bfs_expand", [0.0, 0, 0])[2] += st["edges_scanned"]
+                e_scan, t_scan, levels, reached = ref_scan[r]
                 per["bfs"][0] += st["ms"]
                 per["bfs"][1] += m_reach[r]
-                per["bfs"][2] += 2 * st["edges_scanned"]
+                per["bfs"][2] += alg_bytes_bfs(n, e_scan, t_scan, levels, reached)
+                bfs_scanned[0] += 2 * e_scan
         if "wcc" in algos:
             g.wcc(copy_out=False)
             st = g.stats()
             if record:
                 absorb(st)
+                passes = st["iterations"]
+                jumps = st["kernels"].get("wcc_compress", {}).get("calls", 0)
                 per["wcc"][0] += st["ms"]
-                per["wcc"][1] += 2 * st["edges_scanned"]
+                per["wcc"][1] += 2 * E * passes
+                per["wcc"][2] += alg_bytes_wcc(n, E, T, passes, jumps)
         if "kcore" not in algos:
             return
         g.kcore(args.k, copy_out=False)
         st = g.stats()
         if record:
             absorb(st)
-            kern.setdefault("kcore_scan", [0.0, 0, 0])[2] += st["edges_scanned"]
             per["kcore"][0] += st["ms"]
-            per["kcore"][1] += 2 * st["edges_scanned"]
+            per["kcore"][1] += 2 * E * st["iterations"]
+            per["kcore"][2] += alg_bytes_kcore(n, st["edges_scanned"], st["tiles_scanned"], st["iterations"])
 
     for i in range(args.warmup):
         step(False)
@@ -372,50 +409,50 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t
     ms_per_step = elapsed * 1e3 / args.steps
-    traversed = per["pagerank"][1] + per["bfs"][1] + per["wcc"][1] + per["kcore"][1]
-    value = traversed / elapsed / 1e9
+    value = sum(v[1] for v in per.values()) / elapsed / 1e9
 
-    def rate(edges, ms):
-        return edges / (ms / 1e3) / 1e9 if ms > 0 else None
+    def rate(x, ms):
+        return x / (ms / 1e3) / 1e9 if ms > 0 else None
 
+    names = {"pagerank": "pr", "bfs": "bfs", "wcc": "wcc", "kcore": "kcore"}
     per_algo = {}
-    if "pr" in algos:
-        per_algo["pagerank"] = {"ms": per["pagerank"][0] / args.steps, "iterations": args.pr_iters,
-                                "method": args.pr_method,
-                                "gteps": rate(per["pagerank"][1], per["pagerank"][0])}
-    if "bfs" in algos:
-        per_algo["bfs"] = {"ms_per_root": per["bfs"][0] / args.steps / len(roots), "roots": len(roots),
-                           "mode": ["tile sweep", "direction-optimizing"][max(bfs_mode, 0)],
-                           "gteps_graph500": rate(per["bfs"][1], per["bfs"][0]),
-                           "gteps_scanned": rate(per["bfs"][2], per["bfs"][0])}
-    if "wcc" in algos:
-        per_algo["wcc"] = {"ms": per["wcc"][0] / args.steps, "gteps": rate(per["wcc"][1], per["wcc"][0])}
-    if "kcore" in algos:
-        per_algo["kcore"] = {"ms": per["kcore"][0] / args.steps, "k": args.k,
-                             "gteps": rate(per["kcore"][1], per["kcore"][0])}
-    for a in per_algo.values():
-        for k2 in list(a):
-            if isinstance(a[k2], float):
-                a[k2] = round(a[k2], 4)
+    for algo, (ms, teps, ab) in per.items():
+        if names[algo] not in algos:
+            continue
+        d = {"ms": ms / args.steps, "gteps": rate(teps, ms), "alg_gbs": rate(ab, ms),
+             "hbm_frac": rate(ab, ms) / HBM_PEAK_GBS if ms > 0 else None}
+        if algo == "pagerank":
+            d.update(iterations=args.pr_iters, method=args.pr_method)
+        if algo == "bfs":
+            d.update(ms_per_root=ms / args.steps / len(roots), roots=len(roots),
+                     mode=["tile sweep", "direction-optimizing"][max(bfs_mode, 0)],
+                     gteps_scanned=rate(bfs_scanned[0], ms))
+        if algo == "kcore":
+            d.update(k=args.k)
+        per_algo[algo] = {k2: round(v, 4) if isinstance(v, float) else v for k2, v in d.items()}
 
     chunk = tune["chunk"] if tune else 4096
-    nchunk = (E + chunk - 1) // chunk
     idx = g.index_info()
-    kernels = {}
-    for name, (ms, calls, scanned) in kern.items():
-        avg_ms = ms / max(calls, 1)
-        b = kernel_bytes(name, n, E, g.nonempty_tiles, nchunk, calls, scanned, idx)
-        kernels[name] = {"total_ms": round(ms / args.steps, 3), "launches_per_step": calls / args.steps,
-                         "avg_ms": round(avg_ms, 4),
-                         "alg_gbs": round(b / (avg_ms / 1e3) / 1e9, 1) if b else None}
-    dom = max(kern, key=lambda k: kern[k][0])
-    d_ms, d_calls, d_scanned = kern[dom]
-    d_avg = d_ms / d_calls
-    d_bytes = kernel_bytes(dom, n, E, g.nonempty_tiles, nchunk, d_calls, d_scanned, idx)
-    achieved = d_bytes / (d_avg / 1e3) / 1e9
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": int(d_bytes), "avg_launch_ms": round(d_avg, 4)}
+    kernels = {name: {"total_ms": round(ms / args.steps, 3), "launches_per_step": calls / args.steps,
+                      "avg_ms": round(ms / max(calls, 1), 4)} for name, (ms, calls) in kern.items()}
+    # roofline: one PageRank iteration (all its kernels) against the fixed formula
+    roofline = None
+    if "pr" in algos and per["pagerank"][0] > 0:
+        iters = args.pr_iters * args.steps
+        it_kern = {k: kern[k][0] / iters for k in PR_ITER_KERNELS if k in kern}
+        it_ms = sum(it_kern.values())
+        b = alg_bytes_pr_iter(n, E, T)
+        achieved = b / (it_ms / 1e3) / 1e9
+        roofline = {"kernel": "pagerank iteration (" + " + ".join(it_kern) + ")", "bound": "hbm",
+                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_launch": int(b), "avg_launch_ms": round(it_ms, 4),
+                    "kernel_ms": {k: round(v, 4) for k, v in it_kern.items()}}
+        trf = measured_traffic(f"kron{args.scale}", args.pr_method, it_ms)
+        if trf:
+            roofline["traffic"] = trf["bytes"]
+            roofline["traffic_frac"] = round(trf["bytes"] / (it_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+            roofline["traffic_source"] = trf["source"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -432,7 +469,8 @@ def main():
                                f"BFS {len(roots)} roots + WCC + k-core k={args.k}",
                    "scale": args.scale, "vertices": n, "stored_edges": E, "nonempty_tiles": g.nonempty_tiles,
                    "gen_convert_s": round(gen_s, 2), "parallelism": "single GPU",
-                   "index": {"entries": idx["entries"], "build_ms": round(idx["build_ms"], 1)},
+                   "index": {"entries": idx["entries"], "build_ms": round(idx["build_ms"], 1),
+                             "device_gb": round(idx["device_bytes"] / 1e9, 1)},
                    "tuned": {"chunk": chunk, "xcd_order": tune["xcd_order"] if tune else 1}},
         "per_algo": per_algo, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -28,12 +28,14 @@ LIB_PATH = os.path.join(_HERE, "libgstore_amd.so")
 
 PR_COMPAT = 0
 PR_INTENDED = 1
+PR_SCATTER, PR_PULL, PR_BLOCKED = 0, 1, 2  # PageRank methods (set_pagerank_method)
 
 GS_OK = 0
 _STATUS = {1: "GS_ERR_ARG", 2: "GS_ERR_IO", 3: "GS_ERR_FORMAT", 4: "GS_ERR_NOMEM",
            5: "GS_ERR_DEVICE", 6: "GS_ERR_UNSUPPORTED", 7: "GS_ERR_DEPTH_OVERFLOW"}
 KERNELS = ["pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end", "wcc_hook", "wcc_compress",
-           "wcc_check", "kcore_peel", "kcore_scan", "kron_gen", "convert", "h2d_copy", "pr_pull", "index_build"]
+           "wcc_check", "kcore_peel", "kcore_scan", "kron_gen", "convert", "h2d_copy", "pr_pull", "index_build",
+           "pb_gather", "pb_accum", "pb_reduce"]
 
 
 class GStoreError(RuntimeError):
@@ -105,6 +107,7 @@ _SIGS = {
     "gs_set_chunk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "gs_set_pagerank_heavy": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_pagerank_method": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_set_pagerank_blocking": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]),
     "gs_set_bfs_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "gs_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_Tune)]),
     "gs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -265,9 +268,16 @@ class Graph:
         """0: tile sweep; 1: direction-optimizing over a neighbour-list index."""
         _check(_lib.gs_set_bfs_mode(self._h, mode, alpha, beta))
 
-    def set_pagerank_method(self, pull: bool) -> None:
-        """True: pull over the neighbour-list index; False: tile scatter with atomics."""
-        _check(_lib.gs_set_pagerank_method(self._h, int(pull)))
+    def set_pagerank_method(self, method) -> None:
+        """PR_BLOCKED (2, default): two streamed passes over the blocked copy of the
+        neighbour-list index; PR_PULL (1): pull over the lists; PR_SCATTER (0):
+        tile scatter with atomics.  True / False mean PR_PULL / PR_SCATTER."""
+        _check(_lib.gs_set_pagerank_method(self._h, int(method)))
+
+    def set_pagerank_blocking(self, gather_entries: int = 1 << 20, gather_runs: int = 2048,
+                              accum_entries: int = 1 << 20) -> None:
+        """Work split of the blocked PageRank (small values exercise the split paths)."""
+        _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries))
 
     def set_pagerank_heavy(self, on: bool) -> None:
         _check(_lib.gs_set_pagerank_heavy(self._h, int(on)))

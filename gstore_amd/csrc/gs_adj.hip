@@ -394,7 +394,10 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
         }
         float val[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) val[i] = e0 + i < len ? prior[idx[i]] : 0.0f;
+        for (int i = 0; i < 16; ++i) val[i] = prior[idx[i]];  // idx is 0 past the end: in bounds, all loads in flight
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (e0 + i >= len) val[i] = 0.0f;
         int lo = 0, hi = nt - 1;
         while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(SWEEP_WG) k_bfs_expand(SweepArgs a, const uint
         return bit_of(rowact, rc >> 16) || (!DIRECTED && bit_of(rowact, rc & 0xFFFFu));
     });
     if (!s.any) return;
-    count_active_edges(s, scanned);
+    count_active_edges(a, chunk, s, scanned);
     auto reach = [&](uint32_t v) {
         uint32_t wi = v >> 5, b = 1u << (v & 31);
         if ((vis[wi] & b) == 0 && (next[wi] & b) == 0) atomicOr(&next[wi], b);
@@ -114,7 +114,7 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     GS_HIP(hipMemsetAsync(w->bm_front.ptr, 0, words * 4, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_next.ptr, 0, words * 4, w->stream));
     GS_HIP(hipMemsetAsync(w->rowact.ptr, 0, w->rowact.bytes(), w->stream));
-    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 16, w->stream));
+    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 24, w->stream));
     k_bfs_root<<<1, 1, 0, w->stream>>>((uint32_t)root, w->depth.ptr, w->bm_vis.ptr, w->bm_front.ptr, w->rowact.ptr);
     GS_HIP(hipGetLastError());
     int nlev = 0, status = GS_OK;
@@ -135,7 +135,7 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
                                                                    w->counters.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
-        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 16, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 24, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
         uint64_t cnt = w->h_counters[0];
         if (level_counts && nlev < max_levels) level_counts[nlev] = cnt;
@@ -156,6 +156,7 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nlev;
     w->stats.edges_scanned = w->h_counters[1];
+    w->stats.tiles_scanned = w->h_counters[2];
     if (levels) *levels = nlev;
     if (status != GS_OK) return status;
     if (depth_out) {

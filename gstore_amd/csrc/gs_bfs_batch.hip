@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(SWEEP_WG) k_msbfs_expand(SweepArgs a, const un
         return bit_of(rowact, rc >> 16) || (!DIRECTED && bit_of(rowact, rc & 0xFFFFu));
     });
     if (!s.any) return;
-    count_active_edges(s, scanned);
+    count_active_edges(a, chunk, s, scanned);
     auto push = [&](unsigned long long f, uint32_t v) {
         unsigned long long m = f & ~vis[v];
         if (m && (m & ~next[v])) atomicOr(&next[v], m);
@@ -136,7 +136,7 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     GS_HIP(hipMemsetAsync(w->ms_next.ptr, 0, g->n * 8, w->stream));
     GS_HIP(hipMemsetAsync(w->ms_depth.ptr, 0, (size_t)nr * g->n, w->stream));
     GS_HIP(hipMemsetAsync(w->rowact.ptr, 0, w->rowact.bytes(), w->stream));
-    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
+    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 16, w->stream));
     k_msbfs_roots<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->n, w->ms_depth.ptr,
                                            (unsigned long long *)w->ms_vis.ptr,
                                            (unsigned long long *)w->ms_front.ptr, w->rowact.ptr);
@@ -177,7 +177,7 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
         }
     }
     GS_HIP(hipEventRecord(t1, w->stream));
-    GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 8, hipMemcpyDeviceToHost, w->stream));
+    GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 16, hipMemcpyDeviceToHost, w->stream));
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
@@ -186,6 +186,7 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nlev;
     w->stats.edges_scanned = w->h_counters[0];
+    w->stats.tiles_scanned = w->h_counters[1];
     if (levels) *levels = nlev;
     if (status != GS_OK) return status;
     if (depth_out) {

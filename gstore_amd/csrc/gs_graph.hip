@@ -539,7 +539,7 @@ const char *gs_kernel_name(int k) {
     static const char *names[GS_K_COUNT] = {"pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end",
                                             "wcc_hook", "wcc_compress", "wcc_check", "kcore_peel",
                                             "kcore_scan", "kron_gen", "convert", "h2d_copy", "pr_pull",
-                                            "index_build"};
+                                            "index_build", "pb_gather", "pb_accum", "pb_reduce"};
     return (k >= 0 && k < GS_K_COUNT) ? names[k] : "?";
 }
 
@@ -709,12 +709,12 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->segment_edges = g->ss ? g->ss->seg_chunks * g->chunk : 0;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
                          g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() +
-                         g->new_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() +
+                         g->new_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() + gs::pb_bytes(g) +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
     const bool built = g->adj_nbr.ptr && g->adj_off.n == g->n + 1;
     info->index_entries = built ? 2 * g->E : 0;
     info->index_vertices = built ? g->nzn : 0;
-    info->index_ms = built ? g->index_ms : 0.0;
+    info->index_ms = built ? g->index_ms + g->pb_ms : 0.0;
     return GS_OK;
 }
 
@@ -803,10 +803,26 @@ int gs_set_pagerank_heavy(gs_graph *g, int on) {
     return GS_OK;
 }
 
-int gs_set_pagerank_method(gs_graph *g, int pull) {
+int gs_set_pagerank_method(gs_graph *g, int method) {
     if (!g) { set_error("null graph"); return GS_ERR_ARG; }
-    g->pr_pull = pull != 0;
+    if (method < 0 || method > 2) { set_error("PageRank method %d: expected 0, 1 or 2", method); return GS_ERR_ARG; }
+    g->pr_method = method;
     g->heavy_ready = false;
+    return GS_OK;
+}
+
+int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries) {
+    if (!g) { set_error("null graph"); return GS_ERR_ARG; }
+    if (gather_entries < 1 || gather_entries > (1ull << 30) || gather_runs < 1 || gather_runs > gs::PB_RUNS ||
+        accum_entries < 1 || accum_entries > (1ull << 30)) {
+        set_error("blocking limits out of range");
+        return GS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g->build_mu);
+    g->pb_unit1 = gather_entries;
+    g->pb_runs = gather_runs;
+    g->pb_unit2 = accum_entries;
+    g->pb_ready = false;  // tables rebuilt at the next PageRank
     return GS_OK;
 }
 

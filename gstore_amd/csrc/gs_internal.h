@@ -31,6 +31,16 @@ constexpr int SWEEP_WG = 256;     // threads per sweep workgroup (4 waves)
 constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr uint32_t ROW_WORDS = 65536 / 32;   // bitmap words per tile row
 
+// Source-blocked copy of the neighbour-list index (PageRank), gs_pb.hip.
+constexpr int PB_SHIFT = 15;                 // vertices per block: 2^15, FP32 values fill 128 KiB of LDS
+constexpr uint32_t PB_B = 1u << PB_SHIFT;
+constexpr uint32_t PB_KMAX = 8192;           // blocks at most (vertices with edges <= 2^28)
+constexpr uint32_t PB_RUNS = 2048;           // runs staged per gather workgroup at most
+struct PbSlice { uint64_t e0, e1; uint32_t q, pad; };               // build: list entries of block q
+struct PbUnit1 { uint64_t pm0, pm1; uint32_t r0, r1, p, pad; };     // gather: source-major entries of block p
+struct PbUnit2 { uint64_t e0, e1; uint32_t q, slot; };              // accum: slot ~0u = the block's only unit
+struct PbRed { uint32_t q, slot0, nslot, pad; };                    // split block: its partial slots
+
 void set_error(const char *fmt, ...);
 
 #define GS_HIP(call)                                                              \
@@ -183,7 +193,22 @@ struct gs_graph {
     uint64_t nzn = 0, npull = 0;             // vertices with edges (ids 0 .. nzn-1); pull chunks
     double index_ms = 0;                     // device time of the index build
     gs::DevBuf<uint32_t> chunkz;             // vertex whose list holds entry c*PULL_CHUNK, npull + 1
-    int pr_pull = 1;                         // PageRank by pull over the lists (undirected, resident)
+    int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy
+    // source-blocked copy of the lists (built on demand), gs_pb.hip
+    bool pb_ready = false;
+    uint32_t pb_k = 0;                       // blocks; 0 = not available (PageRank pulls)
+    uint64_t pb_nu1 = 0, pb_nu2 = 0, pb_nred = 0, pb_nruns = 0;
+    double pb_ms = 0;                        // build time
+    uint64_t pb_unit1 = 1u << 20;            // gather: entries per workgroup at most
+    uint32_t pb_runs = gs::PB_RUNS;          // gather: runs per workgroup at most
+    uint64_t pb_unit2 = 1u << 20;            // accum: entries per workgroup at most
+    gs::DevBuf<uint16_t> pb_src, pb_dst;     // 2E: offsets of u in its block and v in its block
+    gs::DevBuf<float> pb_val;                // 2E: prior[u] per entry, rewritten every iteration
+    gs::DevBuf<uint64_t> pb_run_pm, pb_run_qm;  // non-empty runs, source-major: prefix, q-major start
+    gs::DevBuf<gs::PbUnit1> pb_u1;
+    gs::DevBuf<gs::PbUnit2> pb_u2;
+    gs::DevBuf<gs::PbRed> pb_red;
+    gs::DevBuf<float> pb_part;               // partial sums of split blocks, PB_B per slot
     std::mutex build_mu;                     // lazy builds (index, heavy set) from concurrent workspaces
     ~gs_graph();
 };
@@ -209,6 +234,9 @@ int run_bfs_do(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, u
                int max_levels, int *levels);
 int build_adjacency(gs_graph *g);
 int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double *acc64);
+int build_blocked(gs_graph *g);  // the blocked copy (after the lists); pb_k = 0 when too many blocks
+int pr_blocked(const gs_graph *g, gs_ws *w, const float *prior, float *acc);
+size_t pb_bytes(const gs_graph *g);
 int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out);
 int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
                   uint64_t *level_counts, int max_levels, int *levels);
@@ -297,17 +325,28 @@ __device__ inline uint32_t stage_chunk(const SweepArgs &a, uint64_t chunk, Chunk
     return len;
 }
 
-// Adds the chunk's edges in active tiles to *out (one atomic per workgroup).
-__device__ inline void count_active_edges(const ChunkTiles &s, unsigned long long *out) {
-    __shared__ uint32_t acc;
-    if (threadIdx.x == 0) acc = 0;
+// Adds the chunk's edges in active tiles to out[0] and the active tiles that
+// start in this chunk to out[1] (each tile counted once over the sweep; one
+// atomic each per workgroup).
+__device__ inline void count_active_edges(const SweepArgs &a, uint64_t chunk, const ChunkTiles &s,
+                                          unsigned long long *out) {
+    __shared__ uint32_t acc, tiles;
+    if (threadIdx.x == 0) acc = tiles = 0;
     __syncthreads();
-    uint32_t c = 0;
+    const uint32_t len = s.beg[s.nt];
+    uint32_t c = 0, t = 0;
     for (int k = threadIdx.x; k < s.nt; k += blockDim.x)
-        if (s.act[k]) c += (uint32_t)s.beg[k + 1] - s.beg[k];
+        if (s.act[k]) {
+            c += (uint32_t)s.beg[k + 1] - s.beg[k];
+            const bool starts_here = k ? s.beg[k] < len
+                                       : len && a.tz_beg[a.chunk_tz[chunk]] == chunk * (uint64_t)a.chunk;
+            t += starts_here;
+        }
     if (c) atomicAdd(&acc, c);
+    if (t) atomicAdd(&tiles, t);
     __syncthreads();
     if (threadIdx.x == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+    if (threadIdx.x == 0 && tiles) atomicAdd(out + 1, (unsigned long long)tiles);
 }
 
 __device__ inline int find_tile(const ChunkTiles &s, uint32_t off) {

@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(SWEEP_WG) k_kc_scan(SweepArgs a, const uint32_
         return bit_of(rowact, rc >> 16) || bit_of(rowact, rc & 0xFFFFu);
     });
     if (!s.any) return;
-    count_active_edges(s, scanned);
+    count_active_edges(a, chunk, s, scanned);
     auto op = [&](uint32_t w, uint32_t rc) {
         uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
         if ((act[v0 >> 5] >> (v0 & 31)) & 1u) atomicSub(&deg[v1], 1);
@@ -104,7 +104,7 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     GS_HIP(hipEventRecord(t0, w->stream));
     GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, g->degree.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
     GS_HIP(hipMemsetAsync(w->rowact.ptr, 0, w->rowact.bytes(), w->stream));
-    GS_HIP(hipMemsetAsync(w->counters.ptr + 2, 0, 8, w->stream));
+    GS_HIP(hipMemsetAsync(w->counters.ptr + 2, 0, 16, w->stream));
     int nr = 0;
     for (;;) {
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
@@ -131,7 +131,7 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
                                                                                             w->counters.ptr);
     GS_HIP(hipGetLastError());
     GS_HIP(hipEventRecord(t1, w->stream));
-    GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 24, hipMemcpyDeviceToHost, w->stream));
+    GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 32, hipMemcpyDeviceToHost, w->stream));
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
@@ -140,6 +140,7 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nr;
     w->stats.edges_scanned = w->h_counters[2];
+    w->stats.tiles_scanned = w->h_counters[3];
     if (count_ne0) *count_ne0 = w->h_counters[0];
     if (count_pos) *count_pos = w->h_counters[1];
     if (rounds) *rounds = nr;

@@ -173,12 +173,13 @@ static bool use_pull(const gs_graph *g);
 
 // Heavy set: vertices of degree >= 2^b for the smallest b leaving at most
 // HEAVY_MAX of them (none below degree 256).  Built once per graph.
-int ensure_heavy(gs_graph *g, hipStream_t st) {
+// blocked: the caller sums in FP32 and may use the blocked copy of the lists.
+int ensure_heavy(gs_graph *g, hipStream_t st, bool blocked) {
     if (use_pull(g)) {
         g->heavy_dthr = 0xFFFFFFFFu;  // the pull has no atomics to relieve
         g->nheavy = 0;
         g->heavy_ready = false;
-        return build_adjacency(g);
+        return blocked && g->pr_method == 2 ? build_blocked(g) : build_adjacency(g);
     }
     std::lock_guard<std::mutex> lk(g->build_mu);
     if (g->heavy_ready) return GS_OK;
@@ -214,7 +215,7 @@ int ensure_heavy(gs_graph *g, hipStream_t st) {
     return GS_OK;
 }
 
-static bool use_pull(const gs_graph *g) { return g->pr_pull && !g->directed && !g->ss; }
+static bool use_pull(const gs_graph *g) { return g->pr_method >= 1 && !g->directed && !g->ss; }
 
 // One accumulation pass: pull over the neighbour lists when the graph has
 // them, else the tile scatter (with the heavy-vertex kernel when there is a
@@ -235,7 +236,10 @@ static int scatter_t(const gs_graph *g, gs_ws *w, A *acc) {
 
 // acc64 non-null: accumulate in FP64 (personalised PageRank).
 static int scatter(const gs_graph *g, gs_ws *w, double *acc64 = nullptr) {
-    if (use_pull(g)) return pr_pull(g, w, w->pr_prior.ptr, w->pr_acc.ptr, acc64);
+    if (use_pull(g)) {
+        if (!acc64 && g->pr_method == 2 && g->pb_k) return pr_blocked(g, w, w->pr_prior.ptr, w->pr_acc.ptr);
+        return pr_pull(g, w, w->pr_prior.ptr, w->pr_acc.ptr, acc64);
+    }
     if (acc64) return scatter_t<double>(g, w, acc64);
     return scatter_t<float>(g, w, w->pr_acc.ptr);
 }
@@ -254,7 +258,7 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
         GS_TRY(w->pr_acc.alloc(g->n));
         GS_TRY(w->pr_fac.alloc(g->n));
     }
-    GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream));
+    GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream, true));
     w->timer.reset();
     w->stats = gs_stats{};
     hipEvent_t t0, t1;
@@ -309,7 +313,7 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
     }
     if (w->ppr_rank.n != g->n) GS_TRY(w->ppr_rank.alloc(g->n));
     if (w->ppr_acc.n != g->n) GS_TRY(w->ppr_acc.alloc(g->n));
-    GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream));
+    GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream, false));
     w->timer.reset();
     w->stats = gs_stats{};
     hipEvent_t t0, t1;

@@ -1,0 +1,506 @@
+// gs_pb.hip -- PageRank over a source-blocked copy of the neighbour-list index.
+//
+// The pull over neighbour lists (gs_adj.hip) reads prior[u] for every list
+// entry at a random address; on a Kronecker graph of 2^28 vertices nearly
+// every such read misses L2 and moves a whole cache line for 4 useful bytes.
+// Propagation blocking (Beamer, Asanovic and Patterson, IPDPS 2017) turns the
+// random reads into two streamed passes.  The relabelled ids with edges
+// (0 .. nzn-1) are cut into blocks of PB_B = 2^15 vertices, whose FP32 values
+// fill 128 KiB of LDS.  Every list entry (v <- u) is copied once into an array
+// ordered by destination block q = v / PB_B, then source block p = u / PB_B
+// (the run (p, q)), keeping only the two 15-bit offsets inside the blocks:
+//   gather, per source block p: stage prior[p*PB_B ..] in LDS and write
+//           val[e] = prior[u] at every entry of the runs (p, *) -- 2 bytes read
+//           and 4 bytes written per entry, contiguous within each run;
+//   accum,  per destination block q: add val[e] into an LDS array over the
+//           block at the entry's destination offset (one LDS atomic per run of
+//           equal offsets in a wave) and store the block's sums.
+// A destination block with more entries than one workgroup takes is split
+// into units whose sums go to a partial buffer, added by k_pb_reduce.  The
+// order of the FP32 additions is the only difference from the other methods.
+#include <algorithm>
+#include <vector>
+
+#include "gs_internal.h"
+
+namespace gs {
+
+namespace {
+
+constexpr int PB_WG = 1024;            // threads per workgroup of the three per-iteration kernels
+constexpr int PB_PASS = 4096;          // build: list entries staged per pass
+constexpr uint64_t PB_SLICE = 65536;   // build: list entries per workgroup
+
+unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+
+// ---- build -------------------------------------------------------------------
+
+__global__ void k_pb_bin_begin(const uint64_t *__restrict__ off, uint64_t nzn, uint32_t K, uint64_t *beg) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q <= K; q += gridDim.x * blockDim.x)
+        beg[q] = off[std::min<uint64_t>((uint64_t)q << PB_SHIFT, nzn)];
+}
+
+// Entries per run: cnt[q*K + p] over one slice of the lists of block q.
+__global__ void __launch_bounds__(PB_WG) k_pb_count(const PbSlice *__restrict__ sl, const uint32_t *__restrict__ nbr,
+                                                    uint32_t K, unsigned long long *cnt) {
+    __shared__ uint32_t hist[PB_KMAX];
+    const PbSlice s = sl[blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < K; k += PB_WG) hist[k] = 0;
+    __syncthreads();
+    for (uint64_t e = s.e0 + threadIdx.x; e < s.e1; e += PB_WG) atomicAdd(&hist[nbr[e] >> PB_SHIFT], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < K; k += PB_WG)
+        if (hist[k]) atomicAdd(&cnt[(uint64_t)s.q * K + k], (unsigned long long)hist[k]);
+}
+
+// Copy one slice of the lists of block q into the runs (p, q): each workgroup
+// reserves a contiguous stretch of every run it feeds (one global atomic per
+// run), then places entries with LDS cursors.  cur[q*K + p] starts at the
+// run's first position.
+__global__ void __launch_bounds__(PB_WG) k_pb_fill(const PbSlice *__restrict__ sl, const uint64_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ nbr, uint64_t nzn, uint32_t K,
+                                                   unsigned long long *cur, uint16_t *__restrict__ src_out,
+                                                   uint16_t *__restrict__ dst_out) {
+    __shared__ uint32_t hist[PB_KMAX];
+    __shared__ unsigned long long base[PB_KMAX];
+    __shared__ uint64_t voff[PB_PASS + 2];
+    __shared__ uint64_t v_first;
+    __shared__ int nvs;
+    const PbSlice s = sl[blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < K; k += PB_WG) hist[k] = 0;
+    __syncthreads();
+    for (uint64_t e = s.e0 + threadIdx.x; e < s.e1; e += PB_WG) atomicAdd(&hist[nbr[e] >> PB_SHIFT], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < K; k += PB_WG) {
+        uint32_t c = hist[k];
+        base[k] = c ? atomicAdd(&cur[(uint64_t)s.q * K + k], (unsigned long long)c) : 0ull;
+        hist[k] = 0;
+    }
+    const uint64_t vlo = (uint64_t)s.q << PB_SHIFT, vhi = std::min<uint64_t>(vlo + PB_B, nzn);
+    for (uint64_t a = s.e0; a < s.e1; a += PB_PASS) {
+        const uint64_t b = std::min<uint64_t>(a + PB_PASS, s.e1);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // largest v in [vlo, vhi) with off[v] <= a, then with off[v] <= b - 1
+            uint64_t lo = vlo, hi = vhi - 1;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= a) lo = mid; else hi = mid - 1;
+            }
+            uint64_t v0 = lo;
+            hi = vhi - 1;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= b - 1) lo = mid; else hi = mid - 1;
+            }
+            v_first = v0;
+            nvs = (int)(lo - v0 + 1);
+        }
+        __syncthreads();
+        const int nv = nvs;
+        const uint64_t v0 = v_first;
+        for (int k = threadIdx.x; k <= nv; k += PB_WG) voff[k] = off[v0 + k];
+        __syncthreads();
+        for (uint64_t e = a + threadIdx.x; e < b; e += PB_WG) {
+            int lo = 0, hi = nv - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (voff[mid] <= e) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t u = nbr[e];
+            const uint32_t p = u >> PB_SHIFT;
+            const unsigned long long pos = base[p] + atomicAdd(&hist[p], 1u);
+            src_out[pos] = (uint16_t)(u & (PB_B - 1));
+            dst_out[pos] = (uint16_t)((v0 + lo) & (PB_B - 1));
+        }
+    }
+}
+
+// ---- per iteration -----------------------------------------------------------
+
+constexpr int PB_PER = 16;                         // entries per lane per step
+constexpr uint32_t PB_WAVE_SPAN = 64 * PB_PER;     // a wave's entries per step, lane-interleaved
+constexpr uint32_t PB_STEP = PB_WAVE_SPAN * (PB_WG / 64);
+
+// prior[pbase .. pbase + pn) into LDS with 16-byte loads, all issued before
+// the stores (pbase is a multiple of 2^15, so the loads are aligned).
+__device__ inline void stage_block(const float *__restrict__ prior, uint64_t pbase, uint32_t pn, float4 *pri4) {
+    const float4 *p4 = reinterpret_cast<const float4 *>(prior + pbase);
+    const uint32_t n4 = pn >> 2;
+    float4 v[PB_B / 4 / PB_WG];
+#pragma unroll
+    for (uint32_t j = 0; j < PB_B / 4 / PB_WG; ++j) {
+        const uint32_t i = threadIdx.x + j * PB_WG;
+        v[j] = i < n4 ? p4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < PB_B / 4 / PB_WG; ++j) {
+        const uint32_t i = threadIdx.x + j * PB_WG;
+        if (i < n4) pri4[i] = v[j];
+    }
+    float *pri = reinterpret_cast<float *>(pri4);
+    for (uint32_t i = (n4 << 2) + threadIdx.x; i < pn; i += PB_WG) pri[i] = prior[pbase + i];
+}
+
+// val[e] = prior[p*PB_B + src[e]] for the entries [pm0, pm1) of source block p,
+// counted in source-major order; the unit's runs are staged in LDS.  A wave
+// takes 1024 consecutive entries per step, lane-interleaved, so each lane's
+// run index only moves forward (at most 64 runs per 64 entries).
+__global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__ units,
+                                                     const uint64_t *__restrict__ run_pm,
+                                                     const uint64_t *__restrict__ run_qm,
+                                                     const uint16_t *__restrict__ src,
+                                                     const float *__restrict__ prior, uint64_t nzn,
+                                                     float *__restrict__ val) {
+    __shared__ float4 pri4[PB_B / 4];
+    __shared__ uint32_t rs[PB_RUNS + 1];
+    __shared__ uint64_t rq[PB_RUNS];
+    const PbUnit1 u = units[blockIdx.x];
+    const uint64_t pbase = (uint64_t)u.p << PB_SHIFT;
+    const uint32_t pn = (uint32_t)std::min<uint64_t>(PB_B, nzn - pbase);
+    stage_block(prior, pbase, pn, pri4);
+    const float *pri = reinterpret_cast<const float *>(pri4);
+    const int nr = (int)(u.r1 - u.r0);
+    for (int k = threadIdx.x; k < nr; k += PB_WG) {
+        const uint64_t r0 = run_pm[u.r0 + k];
+        const uint64_t s = r0 > u.pm0 ? r0 : u.pm0;
+        rs[k] = (uint32_t)(s - u.pm0);
+        rq[k] = run_qm[u.r0 + k] + (s - r0);
+    }
+    const uint32_t len = (uint32_t)(u.pm1 - u.pm0);
+    if (threadIdx.x == 0) rs[nr] = len;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t sb = 0; sb < len; sb += PB_STEP) {
+        const uint32_t wb = sb + wave * PB_WAVE_SPAN;
+        if (wb >= len) break;  // wave-uniform, no barrier follows
+        // largest k < nr with rs[k] <= e, by fixed-step searches (rs[0] = 0)
+        int k = 0;
+        {
+            const uint32_t e = wb + lane;
+#pragma unroll
+            for (int step = PB_RUNS / 2; step > 0; step >>= 1)
+                if (k + step < nr && rs[k + step] <= e) k += step;
+        }
+        uint64_t pos[PB_PER];
+#pragma unroll
+        for (int j = 0; j < PB_PER; ++j) {
+            const uint32_t e = wb + j * 64 + lane;
+            if (e < len && rs[k + 1] <= e) {
+                // 64 entries on, at most 64 runs on
+#pragma unroll
+                for (int step = 64; step > 0; step >>= 1)
+                    if (k + step < nr && rs[k + step] <= e) k += step;
+            }
+            pos[j] = e < len ? rq[k] + (e - rs[k]) : ~0ull;
+        }
+        uint32_t sv[PB_PER];
+#pragma unroll
+        for (int j = 0; j < PB_PER; ++j) sv[j] = pos[j] != ~0ull ? src[pos[j]] : 0u;
+#pragma unroll
+        for (int j = 0; j < PB_PER; ++j)
+            if (pos[j] != ~0ull) val[pos[j]] = pri[sv[j]];
+    }
+}
+
+// Sums of val over the entries [e0, e1) of destination block q, in LDS.  Each
+// lane takes 16 consecutive entries (16-entry aligned, so the loads are two
+// and four 16-byte loads; both arrays carry 16 entries of padding), adds runs
+// of equal destination offsets in registers and each run into LDS once.
+typedef unsigned int pb_u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(PB_WG) k_pb_accum(const PbUnit2 *__restrict__ units,
+                                                    const uint16_t *__restrict__ dst,
+                                                    const float *__restrict__ val, uint64_t nzn,
+                                                    float *__restrict__ acc, float *__restrict__ part) {
+    __shared__ float sum[PB_B];
+    const PbUnit2 u = units[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < PB_B; i += PB_WG) sum[i] = 0.0f;
+    __syncthreads();
+    for (uint64_t sb = u.e0 & ~15ull; sb < u.e1; sb += 16 * PB_WG) {
+        const uint64_t e0 = sb + threadIdx.x * 16;
+        if (e0 >= u.e1) break;
+        pb_u32x4 dq[2], vq[4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) dq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(dst + e0) + k);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(val + e0) + k);
+        uint32_t cur = 0xFFFFFFFFu;
+        float run = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint64_t e = e0 + i;
+            if (e < u.e0 || e >= u.e1) continue;
+            const uint32_t d = (dq[i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
+            const float x = __uint_as_float(vq[i >> 2][i & 3]);
+            if (d != cur) {
+                if (cur != 0xFFFFFFFFu) atomicAdd(&sum[cur], run);
+                cur = d;
+                run = x;
+            } else {
+                run += x;
+            }
+        }
+        if (cur != 0xFFFFFFFFu) atomicAdd(&sum[cur], run);
+    }
+    __syncthreads();
+    const uint64_t qbase = (uint64_t)u.q << PB_SHIFT;
+    const uint32_t qn = (uint32_t)std::min<uint64_t>(PB_B, nzn - qbase);
+    if (u.slot == 0xFFFFFFFFu) {
+        for (uint32_t i = threadIdx.x; i < qn; i += PB_WG) acc[qbase + i] = sum[i];
+    } else {
+        float *pp = part + (uint64_t)u.slot * PB_B;
+        for (uint32_t i = threadIdx.x; i < qn; i += PB_WG) pp[i] = sum[i];
+    }
+}
+
+// acc over a split destination block = sum of its units' partial sums.
+__global__ void __launch_bounds__(256) k_pb_reduce(const PbRed *__restrict__ red, const float *__restrict__ part,
+                                                   uint64_t nzn, float *__restrict__ acc) {
+    const PbRed r = red[blockIdx.y];
+    const uint64_t qbase = (uint64_t)r.q << PB_SHIFT;
+    const uint32_t qn = (uint32_t)std::min<uint64_t>(PB_B, nzn - qbase);
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= qn) return;
+    const float *pp = part + (uint64_t)r.slot0 * PB_B + i;
+    float s = 0.0f;
+    for (uint32_t k = 0; k < r.nslot; ++k) s += pp[(uint64_t)k * PB_B];
+    acc[qbase + i] = s;
+}
+
+}  // namespace
+
+// The unit tables must tile the entry ranges exactly and stay within the
+// LDS tables of the kernels: checked on the host before the first launch.
+static int check_units(const std::vector<PbUnit1> &u1, const std::vector<PbUnit2> &u2,
+                       const std::vector<PbRed> &red, const std::vector<uint64_t> &rpm, uint64_t total,
+                       uint32_t runs_max, uint32_t slots) {
+    uint64_t pm = 0;
+    const uint64_t nr = rpm.size() - 1;
+    for (const PbUnit1 &x : u1) {
+        const bool ok = x.pm0 == pm && x.pm1 > x.pm0 && x.r0 < x.r1 && x.r1 <= nr && x.r1 - x.r0 <= runs_max &&
+                        x.r1 - x.r0 <= PB_RUNS && rpm[x.r0] <= x.pm0 && x.pm0 < rpm[x.r0 + 1] &&
+                        rpm[x.r1 - 1] < x.pm1 && x.pm1 <= rpm[x.r1] && x.pm1 - x.pm0 < (1ull << 32);
+        if (!ok) { set_error("blocked index: bad gather unit at entry %llu", (unsigned long long)pm); return GS_ERR_FORMAT; }
+        pm = x.pm1;
+    }
+    if (pm != total) { set_error("blocked index: gather units end at %llu", (unsigned long long)pm); return GS_ERR_FORMAT; }
+    uint64_t e = 0;
+    for (const PbUnit2 &x : u2) {
+        if (x.e0 != e || x.e1 <= x.e0 || (x.slot != 0xFFFFFFFFu && x.slot >= slots)) {
+            set_error("blocked index: bad accum unit at entry %llu", (unsigned long long)e);
+            return GS_ERR_FORMAT;
+        }
+        e = x.e1;
+    }
+    if (e != total) { set_error("blocked index: accum units end at %llu", (unsigned long long)e); return GS_ERR_FORMAT; }
+    for (const PbRed &r : red)
+        if (r.nslot < 2 || r.slot0 + r.nslot > slots) { set_error("blocked index: bad partial slots"); return GS_ERR_FORMAT; }
+    return GS_OK;
+}
+
+size_t pb_bytes(const gs_graph *g) {
+    return g->pb_src.bytes() + g->pb_dst.bytes() + g->pb_val.bytes() + g->pb_run_pm.bytes() +
+           g->pb_run_qm.bytes() + g->pb_u1.bytes() + g->pb_u2.bytes() + g->pb_red.bytes() + g->pb_part.bytes();
+}
+
+// Builds the blocked copy from the neighbour-list index (built first if
+// needed).  Leaves pb_k = 0 -- PageRank then pulls over the lists -- when the
+// graph has more than PB_KMAX blocks of vertices with edges.
+int build_blocked(gs_graph *g) {
+    GS_TRY(build_adjacency(g));
+    std::lock_guard<std::mutex> lk(g->build_mu);
+    if (g->pb_ready) return GS_OK;
+    const uint64_t nzn = g->nzn;
+    const uint64_t K64 = (nzn + PB_B - 1) >> PB_SHIFT;
+    if (K64 == 0 || K64 > PB_KMAX) {
+        g->pb_k = 0;
+        g->pb_ready = true;
+        return GS_OK;
+    }
+    const uint32_t K = (uint32_t)K64;
+    gs_ws *w = g->ws;
+    hipStream_t st = w->stream;
+    hipEvent_t t0, t1;
+    GS_HIP(hipEventCreate(&t0));
+    GS_HIP(hipEventCreate(&t1));
+    GS_HIP(hipEventRecord(t0, st));
+    // 1. list range of every destination block, cut into build slices
+    std::vector<uint64_t> beg(K + 1);
+    {
+        DevBuf<uint64_t> d_beg;
+        GS_TRY(d_beg.alloc(K + 1));
+        k_pb_bin_begin<<<vgrid(K + 1), 256, 0, st>>>(g->adj_off.ptr, nzn, K, d_beg.ptr);
+        GS_HIP(hipGetLastError());
+        GS_HIP(hipMemcpyAsync(beg.data(), d_beg.ptr, (K + 1) * 8, hipMemcpyDeviceToHost, st));
+        GS_HIP(hipStreamSynchronize(st));
+    }
+    std::vector<PbSlice> slices;
+    for (uint32_t q = 0; q < K; ++q)
+        for (uint64_t a = beg[q]; a < beg[q + 1]; a += PB_SLICE)
+            slices.push_back(PbSlice{a, std::min(a + PB_SLICE, beg[q + 1]), q, 0});
+    DevBuf<PbSlice> d_sl;
+    GS_TRY(d_sl.alloc(slices.size()));
+    GS_HIP(hipMemcpyAsync(d_sl.ptr, slices.data(), slices.size() * sizeof(PbSlice), hipMemcpyHostToDevice, st));
+    // 2. entries per run (q-major table)
+    const uint64_t KK = (uint64_t)K * K;
+    DevBuf<unsigned long long> cnt;
+    GS_TRY(cnt.alloc(KK));
+    GS_HIP(hipMemsetAsync(cnt.ptr, 0, KK * 8, st));
+    if (!slices.empty()) {
+        k_pb_count<<<(unsigned)slices.size(), PB_WG, 0, st>>>(d_sl.ptr, g->adj_nbr.ptr, K, cnt.ptr);
+        GS_HIP(hipGetLastError());
+    }
+    std::vector<unsigned long long> c(KK);
+    GS_HIP(hipMemcpyAsync(c.data(), cnt.ptr, KK * 8, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    // 3. run starts: q-major positions, and the non-empty runs in source-major order
+    std::vector<unsigned long long> qm(KK);
+    unsigned long long run = 0;
+    for (uint64_t i = 0; i < KK; ++i) { qm[i] = run; run += c[i]; }
+    if (run != 2 * g->E) {
+        set_error("blocked index: %llu entries, expected %llu", run, (unsigned long long)(2 * g->E));
+        return GS_ERR_FORMAT;
+    }
+    std::vector<uint64_t> rpm, rqm;
+    std::vector<uint32_t> prun(K + 1);
+    {
+        // source-major walk of a q-major table, in 64 x 64 tiles for the host caches
+        std::vector<uint64_t> tmp_c(KK), tmp_q(KK);
+        constexpr uint32_t TB = 64;
+        for (uint32_t q0 = 0; q0 < K; q0 += TB)
+            for (uint32_t p0 = 0; p0 < K; p0 += TB)
+                for (uint32_t q = q0; q < std::min(K, q0 + TB); ++q)
+                    for (uint32_t p = p0; p < std::min(K, p0 + TB); ++p) {
+                        tmp_c[(uint64_t)p * K + q] = c[(uint64_t)q * K + p];
+                        tmp_q[(uint64_t)p * K + q] = qm[(uint64_t)q * K + p];
+                    }
+        uint64_t pm = 0;
+        for (uint32_t p = 0; p < K; ++p) {
+            prun[p] = (uint32_t)rpm.size();
+            for (uint32_t q = 0; q < K; ++q) {
+                const uint64_t n_ = tmp_c[(uint64_t)p * K + q];
+                if (!n_) continue;
+                rpm.push_back(pm);
+                rqm.push_back(tmp_q[(uint64_t)p * K + q]);
+                pm += n_;
+            }
+        }
+        prun[K] = (uint32_t)rpm.size();
+        rpm.push_back(pm);
+    }
+    const uint64_t nruns = rqm.size();
+    const uint64_t unit1 = g->pb_unit1, unit2 = g->pb_unit2;
+    const uint32_t runs_max = g->pb_runs;
+    // 4. gather units: source-major entry ranges within one source block, at
+    //    most pb_unit1 entries and pb_runs runs each
+    std::vector<PbUnit1> u1;
+    for (uint32_t p = 0; p < K; ++p) {
+        uint32_t r = prun[p];
+        const uint32_t rend = prun[p + 1];
+        uint64_t pm = rpm[r];
+        const uint64_t pend = rpm[rend];
+        while (pm < pend) {
+            PbUnit1 x{};
+            x.p = p;
+            x.pm0 = pm;
+            x.r0 = r;
+            uint64_t lim = std::min(pm + unit1, pend);
+            if (r + runs_max < rend && rpm[r + runs_max] < lim) lim = rpm[r + runs_max];
+            x.pm1 = lim;
+            // runs overlapping [pm, lim): r .. the run holding lim - 1
+            uint32_t rl = r;
+            while (rl + 1 < rend && rpm[rl + 1] <= lim - 1) ++rl;
+            x.r1 = rl + 1;
+            u1.push_back(x);
+            pm = lim;
+            r = (rl + 1 < rend && rpm[rl + 1] <= pm) ? rl + 1 : rl;
+        }
+    }
+    // 5. accum units: each destination block in equal parts of at most
+    //    pb_unit2 entries; split blocks get partial slots
+    std::vector<PbUnit2> u2;
+    std::vector<PbRed> red;
+    uint32_t slots = 0;
+    for (uint32_t q = 0; q < K; ++q) {
+        const uint64_t e0 = qm[(uint64_t)q * K], e1 = q + 1 < K ? qm[(uint64_t)(q + 1) * K] : run;
+        if (e1 == e0) continue;
+        const uint64_t parts = (e1 - e0 + unit2 - 1) / unit2;
+        if (parts == 1) {
+            u2.push_back(PbUnit2{e0, e1, q, 0xFFFFFFFFu});
+            continue;
+        }
+        red.push_back(PbRed{q, slots, (uint32_t)parts, 0});
+        for (uint64_t k = 0; k < parts; ++k)
+            u2.push_back(PbUnit2{e0 + (e1 - e0) * k / parts, e0 + (e1 - e0) * (k + 1) / parts, q, slots++});
+    }
+    GS_TRY(check_units(u1, u2, red, rpm, run, runs_max, slots));
+    GS_TRY(g->pb_src.alloc(2 * g->E));
+    GS_TRY(g->pb_dst.alloc(2 * g->E + 16));  // padding: the accumulation reads whole 16-entry groups
+    GS_TRY(g->pb_val.alloc(2 * g->E + 16));
+    GS_TRY(g->pb_run_pm.alloc(rpm.size()));
+    GS_TRY(g->pb_run_qm.alloc(std::max<uint64_t>(nruns, 1)));
+    GS_TRY(g->pb_u1.alloc(u1.size()));
+    GS_TRY(g->pb_u2.alloc(u2.size()));
+    GS_TRY(g->pb_red.alloc(red.size()));
+    GS_TRY(g->pb_part.alloc((uint64_t)slots * PB_B));
+    // 6. the copy, and the unit tables
+    GS_HIP(hipMemcpyAsync(cnt.ptr, qm.data(), KK * 8, hipMemcpyHostToDevice, st));
+    if (!slices.empty()) {
+        k_pb_fill<<<(unsigned)slices.size(), PB_WG, 0, st>>>(d_sl.ptr, g->adj_off.ptr, g->adj_nbr.ptr, nzn, K,
+                                                             cnt.ptr, g->pb_src.ptr, g->pb_dst.ptr);
+        GS_HIP(hipGetLastError());
+    }
+    GS_HIP(hipMemcpyAsync(g->pb_run_pm.ptr, rpm.data(), rpm.size() * 8, hipMemcpyHostToDevice, st));
+    if (nruns) GS_HIP(hipMemcpyAsync(g->pb_run_qm.ptr, rqm.data(), nruns * 8, hipMemcpyHostToDevice, st));
+    if (!u1.empty())
+        GS_HIP(hipMemcpyAsync(g->pb_u1.ptr, u1.data(), u1.size() * sizeof(PbUnit1), hipMemcpyHostToDevice, st));
+    if (!u2.empty())
+        GS_HIP(hipMemcpyAsync(g->pb_u2.ptr, u2.data(), u2.size() * sizeof(PbUnit2), hipMemcpyHostToDevice, st));
+    if (!red.empty())
+        GS_HIP(hipMemcpyAsync(g->pb_red.ptr, red.data(), red.size() * sizeof(PbRed), hipMemcpyHostToDevice, st));
+    GS_HIP(hipEventRecord(t1, st));
+    GS_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    GS_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    g->pb_k = K;
+    g->pb_nu1 = u1.size();
+    g->pb_nu2 = u2.size();
+    g->pb_nred = red.size();
+    g->pb_nruns = nruns;
+    g->pb_ms = ms;
+    g->pb_ready = true;
+    return GS_OK;
+}
+
+// One accumulation pass: acc[v] = sum of prior over v's list, for the
+// relabelled ids 0 .. nzn-1 (acc of the others is left as it is).
+int pr_blocked(const gs_graph *g, gs_ws *w, const float *prior, float *acc) {
+    hipEvent_t kb;
+    if (g->pb_nu1) {
+        w->timer.begin(GS_K_PB_GATHER, w->stream, &kb);
+        k_pb_gather<<<(unsigned)g->pb_nu1, PB_WG, 0, w->stream>>>(g->pb_u1.ptr, g->pb_run_pm.ptr, g->pb_run_qm.ptr,
+                                                                  g->pb_src.ptr, prior, g->nzn, g->pb_val.ptr);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_PB_GATHER, w->stream, kb);
+    }
+    if (g->pb_nu2) {
+        w->timer.begin(GS_K_PB_ACCUM, w->stream, &kb);
+        k_pb_accum<<<(unsigned)g->pb_nu2, PB_WG, 0, w->stream>>>(g->pb_u2.ptr, g->pb_dst.ptr, g->pb_val.ptr, g->nzn,
+                                                                 acc, g->pb_part.ptr);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_PB_ACCUM, w->stream, kb);
+    }
+    if (g->pb_nred) {
+        w->timer.begin(GS_K_PB_REDUCE, w->stream, &kb);
+        k_pb_reduce<<<dim3(PB_B / 256, (unsigned)g->pb_nred), 256, 0, w->stream>>>(g->pb_red.ptr, g->pb_part.ptr,
+                                                                                  g->nzn, acc);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_PB_REDUCE, w->stream, kb);
+    }
+    return GS_OK;
+}
+
+}  // namespace gs

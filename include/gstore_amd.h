@@ -42,7 +42,8 @@ enum gs_pr_mode {
 enum gs_kernel {
     GS_K_PR_SCATTER = 0, GS_K_PR_FINALIZE, GS_K_BFS_EXPAND, GS_K_BFS_LEVEL_END,
     GS_K_WCC_HOOK, GS_K_WCC_COMPRESS, GS_K_WCC_CHECK, GS_K_KCORE_PEEL, GS_K_KCORE_SCAN,
-    GS_K_GEN, GS_K_CONVERT, GS_K_H2D, GS_K_PR_PULL, GS_K_INDEX, GS_K_COUNT
+    GS_K_GEN, GS_K_CONVERT, GS_K_H2D, GS_K_PR_PULL, GS_K_INDEX, GS_K_PB_GATHER, GS_K_PB_ACCUM,
+    GS_K_PB_REDUCE, GS_K_COUNT
 };
 
 typedef struct gs_info {
@@ -59,7 +60,7 @@ typedef struct gs_info {
     int32_t resident;         /* 1: whole grid in HBM */
     uint64_t index_entries;   /* neighbour-list index entries (2E once built, else 0) */
     uint64_t index_vertices;  /* vertices with at least one stored edge (index built) */
-    double index_ms;          /* device time of the index build (0 until built) */
+    double index_ms;          /* build time of the index and its blocked copy (0 until built) */
 } gs_info;
 
 typedef struct gs_stats {
@@ -215,13 +216,27 @@ void gs_server_destroy(gs_server *s);
  * and level counts are identical in both modes.  Undirected grids only. */
 int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta);
 
-/* PageRank / personalised PageRank method on undirected resident grids:
- * 1 (default) pulls over the neighbour-list index (built once from the grid,
- * as for direction-optimizing BFS): each vertex's sum is formed in registers
- * and LDS and stored, with no per-edge atomics; 0 scatters over the tile grid
- * with FP32 atomics, the reference's scheme (pr.cpp:119-154).  Directed and
+/* PageRank method on undirected resident grids:
+ * 2 (default) blocked: a copy of the neighbour-list index ordered by
+ *   destination block then source block (2^15 vertices each) turns every
+ *   iteration into two streamed passes -- stage a block of priors in LDS and
+ *   write each entry's value; add the values into an LDS array over the
+ *   destination block (falls back to 1 above 2^28 vertices with edges);
+ * 1 pulls over the neighbour-list index (built once from the grid, as for
+ *   direction-optimizing BFS): each vertex's sum is formed in registers and
+ *   LDS and stored, with no per-edge atomics;
+ * 0 scatters over the tile grid with FP32 atomics, the reference's scheme
+ *   (pr.cpp:119-154).
+ * Personalised PageRank uses 1 for both 1 and 2 (FP64 sums).  Directed and
  * semi-external grids always scatter.  Only the FP32 summation order differs. */
-int gs_set_pagerank_method(gs_graph *g, int pull);
+int gs_set_pagerank_method(gs_graph *g, int method);
+
+/* Work split of the blocked PageRank: source-major entries (<= 2^30) and
+ * runs (<= 2048) per gather workgroup, entries per accumulation workgroup
+ * (<= 2^30).  Defaults 2^20, 2048, 2^20.  Small values exercise the split
+ * paths on small graphs; results change only in FP32 summation order.  No
+ * counterpart in the reference. */
+int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries);
 
 /* PageRank / personalised PageRank on undirected grids: 1 (default) adds
  * into the (at most 2048) highest-degree vertices through a per-workgroup LDS

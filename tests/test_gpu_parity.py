@@ -311,21 +311,81 @@ def test_bfs_direction_optimizing_bit_exact(loaded, name):
 
 @pytest.mark.parametrize("name", ["kron16", "two_groups", "star", "tiles"])
 def test_pagerank_methods_same_results(loaded, name):
-    """Tile scatter (plain atomics or heavy-vertex LDS table) and the pull over
-    neighbour lists change only the FP32 summation order."""
+    """Tile scatter (plain atomics or heavy-vertex LDS table), the pull over
+    neighbour lists and the blocked two-pass method change only the FP32
+    summation order."""
     n, (start, grid, deg), g = loaded[name]
     want_c = oracle.pagerank(n, start, grid, deg, 10, 0)
     want_i = oracle.pagerank(n, start, grid, deg, 10, 1)
     src = int(np.argmax(deg))
     want_p = oracle.ppr(n, start, grid, deg, src, 20)
-    for pull, heavy in ((False, False), (False, True), (True, True)):
-        g.set_pagerank_method(pull)
+    for method, heavy in ((ga.PR_SCATTER, False), (ga.PR_SCATTER, True), (ga.PR_PULL, True),
+                          (ga.PR_BLOCKED, True)):
+        g.set_pagerank_method(method)
         g.set_pagerank_heavy(heavy)
-        _say("pr method", name, "pull", pull, "heavy", heavy)
+        _say("pr method", name, "method", method, "heavy", heavy)
         assert np.array_equal(g.pagerank(10, ga.PR_COMPAT), want_c)
         assert np.max(np.abs(g.pagerank(10, ga.PR_INTENDED) - want_i)) <= PR_TOL
         assert np.array_equal(g.pagerank(0, ga.PR_COMPAT), np.full(n, np.float32(1.0 / n)))
         p = g.ppr(src, 20)
         assert p.min() >= 0 and float(np.abs(p - want_p).sum()) <= 1e-6
-    g.set_pagerank_method(True)
+    g.set_pagerank_method(ga.PR_BLOCKED)
     g.set_pagerank_heavy(True)
+
+
+@pytest.mark.parametrize("s", [16, 18])
+def test_pagerank_blocked_split_paths(s):
+    """Blocked PageRank with tiny work limits, so that gather units split runs
+    and cut at the run limit, and destination blocks split into partial sums
+    (Kron-18 also has several 2^15-vertex blocks): same results as the oracle."""
+    n, e = graphs.kron(s)
+    start, grid, deg = oracle.convert(e, n)
+    want_c = oracle.pagerank(n, start, grid, deg, 5, 0)
+    want_i = oracle.pagerank(n, start, grid, deg, 5, 1)
+    with ga.Graph.from_arrays(n, start, grid, deg) as g:
+        for lim in ((1 << 20, 2048, 1 << 20), (97, 3, 5000), (1000, 1, 1 << 12), (4096, 2048, 1 << 30)):
+            _say("blocked limits", s, lim)
+            g.set_pagerank_blocking(*lim)
+            assert np.array_equal(g.pagerank(5, ga.PR_COMPAT), want_c)
+            assert np.max(np.abs(g.pagerank(5, ga.PR_INTENDED) - want_i)) <= PR_TOL
+        with pytest.raises(ga.GStoreError):
+            g.set_pagerank_blocking(0, 1, 1)
+        with pytest.raises(ga.GStoreError):
+            g.set_pagerank_blocking(1 << 31, 1, 1)
+        with pytest.raises(ga.GStoreError):
+            g.set_pagerank_blocking(1, 4096, 1)
+        with pytest.raises(ga.GStoreError):
+            g.set_pagerank_method(3)
+
+
+def _scan_counts(n, start, grid, depth):
+    """Per BFS level, edges and non-empty tiles in tile rows / columns holding a
+    frontier vertex (the reference's skip rule, bfs.cpp:290-312), summed."""
+    from gstore_amd import tiles
+    v0, v1 = tiles.decode(n, start, grid)
+    tr, tc = (v0 >> 16).astype(np.int64), (v1 >> 16).astype(np.int64)
+    side = int(max(tr.max(initial=0), tc.max(initial=0))) + 1
+    tile, cnt = np.unique(tr * side + tc, return_counts=True)
+    r, c = tile // side, tile % side
+    e_scan = t_scan = 0
+    for lev in range(1, int(depth.max()) + 1):
+        act = np.zeros(side, bool)
+        f = np.nonzero(depth == lev)[0] >> 16
+        act[f[f < side]] = True
+        m = act[r] | act[c]
+        e_scan += int(cnt[m].sum())
+        t_scan += int(m.sum())
+    return e_scan, t_scan
+
+
+@pytest.mark.parametrize("name", ["kron14", "tiles", "two_groups"])
+def test_bfs_scan_counts_follow_skip_rule(loaded, name):
+    """The tile BFS reports the edges and tiles its levels scan, which the bench
+    prices with the fixed byte formula: they follow the reference's skip rule."""
+    n, (start, grid, deg), g = loaded[name]
+    g.set_bfs_mode(0)
+    root = int(np.argmax(deg))
+    d = g.bfs(root).depth
+    st = g.stats()
+    assert (st["edges_scanned"], st["tiles_scanned"]) == _scan_counts(n, start, grid, d)
+    assert st["iterations"] == int(d.max())
