@@ -35,7 +35,7 @@ _STATUS = {1: "GS_ERR_ARG", 2: "GS_ERR_IO", 3: "GS_ERR_FORMAT", 4: "GS_ERR_NOMEM
            5: "GS_ERR_DEVICE", 6: "GS_ERR_UNSUPPORTED", 7: "GS_ERR_DEPTH_OVERFLOW"}
 KERNELS = ["pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end", "wcc_hook", "wcc_compress",
            "wcc_check", "kcore_peel", "kcore_scan", "kron_gen", "convert", "h2d_copy", "pr_pull", "index_build",
-           "pb_gather", "pb_accum", "pb_reduce"]
+           "pb_gather", "pb_accum", "pb_reduce", "bfs_topdown", "bfs_bottomup"]
 
 
 class GStoreError(RuntimeError):
@@ -108,6 +108,7 @@ _SIGS = {
     "gs_set_pagerank_heavy": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_pagerank_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_pagerank_blocking": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]),
+    "gs_set_wcc_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_bfs_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "gs_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_Tune)]),
     "gs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -278,6 +279,10 @@ class Graph:
                               accum_entries: int = 1 << 20) -> None:
         """Work split of the blocked PageRank (small values exercise the split paths)."""
         _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries))
+
+    def set_wcc_method(self, method: int) -> None:
+        """1 (default): sampled linking over the neighbour-list index; 0: hook passes over the tiles."""
+        _check(_lib.gs_set_wcc_method(self._h, int(method)))
 
     def set_pagerank_heavy(self, on: bool) -> None:
         _check(_lib.gs_set_pagerank_heavy(self._h, int(on)))

@@ -16,6 +16,8 @@
 // Results are mapped back to the original ids at the end of each call.  BFS
 // keeps the level structure, frontier bitmaps and depth encoding of the tile
 // BFS (bfs.cpp:56-224), so depths are bit-identical in either mode.
+#include <vector>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -84,28 +86,71 @@ __global__ void __launch_bounds__(SWEEP_WG) k_adj_scatter(SweepArgs a, const uin
 
 // ---- direction-optimizing BFS kernels ------------------------------------------
 
-__global__ void k_do_root(uint32_t root, uint8_t *depth, uint32_t *vis, uint32_t *front, uint32_t *queue) {
+__global__ void k_do_root(uint32_t root, const int32_t *__restrict__ deg, uint8_t *depth, uint32_t *vis,
+                          uint32_t *front, uint32_t *queue, int32_t *qdeg) {
     depth[root] = 1;
     vis[root >> 5] |= 1u << (root & 31);
     front[root >> 5] |= 1u << (root & 31);
     queue[0] = root;
+    qdeg[0] = deg[root];
 }
 
-// Top-down: one wave per frontier vertex from the queue, lanes over its neighbours.
-__global__ void __launch_bounds__(256) k_td_expand(const uint32_t *__restrict__ queue, uint64_t nq,
-                                                   const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
-                                                   const uint32_t *__restrict__ vis, uint32_t *next) {
-    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    for (uint64_t qi = wave; qi < nq; qi += nwaves) {
-        uint32_t u = queue[qi];
-        uint64_t b = off[u], e = off[u + 1];
-        for (uint64_t k = b + lane; k < e; k += 64) {
-            uint32_t v = nbr[k];
-            uint32_t wi = v >> 5, bit = 1u << (v & 31);
-            if ((vis[wi] & bit) == 0 && (next[wi] & bit) == 0) atomicOr(&next[wi], bit);
+// Top-down, balanced over edges: qoff is the exclusive prefix of the frontier
+// queue's degrees, and each workgroup takes TD_CHUNK consecutive frontier
+// edges.  The queue entries its range touches are staged in LDS (offset into
+// the range and list start), each lane finds its entry by a search there.  A
+// neighbour not yet visited or marked is marked with atomicOr.
+constexpr uint32_t TD_CHUNK = 4096;
+
+__device__ inline void td_visit(uint32_t v, const uint32_t *__restrict__ vis, uint32_t *next) {
+    const uint32_t wi = v >> 5, bit = 1u << (v & 31);
+    if (!(vis[wi] & bit) && !(next[wi] & bit)) atomicOr(&next[wi], bit);
+}
+
+__global__ void __launch_bounds__(256) k_td_edges(const uint32_t *__restrict__ queue, const uint64_t *__restrict__ qoff,
+                                                  uint64_t nq, uint64_t m, const uint64_t *__restrict__ off,
+                                                  const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
+                                                  uint32_t *next) {
+    __shared__ uint32_t sbeg[TD_CHUNK + 2];  // first edge of staged entry k, relative to the range (clamped)
+    __shared__ uint64_t sadj[TD_CHUNK + 1];  // list start of staged entry k
+    __shared__ uint64_t j_first;
+    __shared__ int nstaged;
+    const uint64_t c0 = blockIdx.x * (uint64_t)TD_CHUNK;
+    if (c0 >= m) return;
+    const uint32_t len = (uint32_t)min<uint64_t>(TD_CHUNK, m - c0);
+    if (threadIdx.x == 0) {
+        // largest j < nq with qoff[j] <= c0, then with qoff[j] <= c0 + len - 1
+        uint64_t lo = 0, hi = nq - 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi + 1) >> 1;
+            if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
         }
+        const uint64_t ja = lo;
+        hi = nq - 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi + 1) >> 1;
+            if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
+        }
+        j_first = ja;
+        nstaged = (int)(lo - ja + 1);
+    }
+    __syncthreads();
+    const uint64_t ja = j_first;
+    const int ns = nstaged;
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+        const uint64_t q = qoff[ja + k];
+        sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
+        sadj[k] = off[queue[ja + k]] + (q < c0 ? c0 - q : 0);  // list position of the range's first edge of entry k
+    }
+    if (threadIdx.x == 0) sbeg[ns] = len;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
+        int lo = 0, hi = ns - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        td_visit(nbr[sadj[lo] + (t - sbeg[lo])], vis, next);
     }
 }
 
@@ -123,14 +168,20 @@ __global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint64_t *_
         uint64_t v = base + lane;
         bool found = false;
         if (v < n && !((vis[v >> 5] >> (v & 31)) & 1u)) {
-            uint64_t b = off[v], e = off[v + 1];
-            for (uint64_t k = b; k < e; ++k) {
-                uint32_t u = nbr[k];
-                ++checked;
-                if ((front[u >> 5] >> (u & 31)) & 1u) {
-                    found = true;
-                    break;
-                }
+            const uint64_t b = off[v], e = off[v + 1];
+            // four neighbours per round trip: ids, then their frontier words, all loads in flight
+            for (uint64_t k = b; k < e && !found; k += 4) {
+                uint32_t u[4], fw[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) u[j] = nbr[min(k + j, e - 1)];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fw[j] = front[u[j] >> 5];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (!found && k + j < e) {
+                        ++checked;
+                        found = (fw[j] >> (u[j] & 31)) & 1u;
+                    }
             }
         }
         unsigned long long m = __ballot(found);
@@ -143,35 +194,134 @@ __global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint64_t *_
 }
 
 // Level end for the direction-optimizing BFS: as k_bfs_level_end, plus the
-// degree sum of the new frontier and its vertex queue for a top-down level.
+// degree sum of the new frontier and its vertex queue (with the degrees) for
+// a top-down level.
+// Queue slots are reserved once per workgroup and step (block scan of the
+// per-word counts), counts and degree sums added once per workgroup.
+// LEVEL_END: next becomes the frontier (bitmaps and depths updated); else the
+// current frontier is only listed.  BUILD: write the queue and its degrees.
+template <bool LEVEL_END, bool BUILD>
 __global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *front, uint32_t *next, uint8_t *depth,
-                                                      uint32_t newdepth, const int32_t *__restrict__ deg,
-                                                      uint64_t nwords, uint32_t *queue,
+                                                      uint32_t newdepth, const int32_t *__restrict__ deg, uint64_t n,
+                                                      uint64_t nwords, uint32_t *queue, int32_t *qdeg,
                                                       unsigned long long *count) {
+    __shared__ uint32_t wsum[4];
+    __shared__ unsigned long long bbase, bcnt, bdeg;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) bcnt = bdeg = 0;
     unsigned long long cnt = 0, msum = 0;
-    for (uint64_t gw = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; gw < nwords;
-         gw += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t nx = next[gw];
-        front[gw] = nx;
-        if (!nx) continue;
-        next[gw] = 0;
-        vis[gw] |= nx;
-        uint32_t c = __popc(nx);
-        unsigned long long pos = atomicAdd(&count[2], (unsigned long long)c);
-        cnt += c;
-        uint32_t m = nx;
-        while (m) {
-            uint32_t b = __ffs(m) - 1;
-            m &= m - 1;
-            uint64_t v = gw * 32 + b;
-            depth[v] = (uint8_t)newdepth;
-            msum += (uint32_t)deg[v];
-            queue[pos++] = (uint32_t)v;
+    for (uint64_t base = blockIdx.x * 256ull; base < nwords; base += (uint64_t)gridDim.x * 256) {
+        const uint64_t gw = base + threadIdx.x;
+        uint32_t nx = 0;
+        if (gw < nwords) {
+            if (LEVEL_END) {
+                nx = next[gw];
+                front[gw] = nx;
+                if (nx) {
+                    next[gw] = 0;
+                    vis[gw] |= nx;
+                }
+            } else {
+                nx = front[gw];
+            }
         }
+        const uint32_t c = __popc(nx);
+        uint32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            bbase = t ? atomicAdd(&count[2], (unsigned long long)t) : 0ull;
+        }
+        __syncthreads();
+        uint32_t before = 0;
+        for (int k = 0; k < wv; ++k) before += wsum[k];
+        unsigned long long pos = bbase + before + incl - c;
+        cnt += c;
+        const uint64_t v0 = gw * 32;
+        if (nx && v0 + 32 <= n) {
+            // the word's 32 degrees and depth bytes as 16-byte loads, updated in registers
+            int4 dq[8];
+            uint4 bq[2];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) dq[k] = reinterpret_cast<const int4 *>(deg + v0)[k];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) bq[k] = LEVEL_END ? reinterpret_cast<const uint4 *>(depth + v0)[k] : uint4{};
+#pragma unroll
+            for (int b = 0; b < 32; ++b) {
+                if (!((nx >> b) & 1u)) continue;
+                const int4 d4 = dq[b >> 2];
+                const int32_t dv = (b & 3) == 0 ? d4.x : (b & 3) == 1 ? d4.y : (b & 3) == 2 ? d4.z : d4.w;
+                if (LEVEL_END) {
+                    uint4 &bw = bq[b >> 4];
+                    uint32_t &word = ((b >> 2) & 3) == 0 ? bw.x : ((b >> 2) & 3) == 1 ? bw.y
+                                   : ((b >> 2) & 3) == 2 ? bw.z : bw.w;
+                    word = (word & ~(0xFFu << ((b & 3) * 8))) | (newdepth << ((b & 3) * 8));
+                }
+                msum += (uint32_t)dv;
+                if (BUILD) {
+                    qdeg[pos] = dv;
+                    queue[pos++] = (uint32_t)(v0 + b);
+                }
+            }
+            if (LEVEL_END) {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) reinterpret_cast<uint4 *>(depth + v0)[k] = bq[k];
+            }
+        } else {
+            uint32_t m = nx;
+            while (m) {
+                const uint32_t bit = __ffs(m) - 1;
+                m &= m - 1;
+                const uint64_t v = v0 + bit;
+                if (LEVEL_END) depth[v] = (uint8_t)newdepth;
+                const int32_t dv = deg[v];
+                msum += (uint32_t)dv;
+                if (BUILD) {
+                    qdeg[pos] = dv;
+                    queue[pos++] = (uint32_t)v;
+                }
+            }
+        }
+        __syncthreads();  // wsum and bbase are rewritten by the next step
     }
     if (cnt) {
-        atomicAdd(&count[0], cnt);
-        atomicAdd(&count[1], msum);
+        atomicAdd(&bcnt, cnt);
+        atomicAdd(&bdeg, msum);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && bcnt) {
+        atomicAdd(&count[0], bcnt);
+        atomicAdd(&count[1], bdeg);
+    }
+}
+
+// Sorts the lists of at most 64 entries ascending (one wave per list, bitonic
+// network over the lanes).  With ids relabelled by descending degree, the
+// bottom-up BFS then meets the heaviest neighbours -- the likeliest frontier
+// members -- first.
+__global__ void __launch_bounds__(256) k_sort_short_lists(const uint64_t *__restrict__ off, uint64_t v0, uint64_t v1,
+                                                          uint32_t *nbr) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t v = v0 + ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6); v < v1; v += nw) {
+        const uint64_t b = off[v], d = off[v + 1] - b;
+        if (d < 2 || d > 64) continue;  // wave-uniform
+        uint32_t x = (uint64_t)lane < d ? nbr[b + lane] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t y = __shfl_xor(x, j);
+                const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+                x = (lower == up) ? min(x, y) : max(x, y);
+            }
+        if ((uint64_t)lane < d) nbr[b + lane] = x;
     }
 }
 
@@ -276,14 +426,13 @@ int build_adjacency(gs_graph *g) {
     for (int b = NBUCKET - 1; b >= 0; --b) { unsigned long long c = h[b]; h[b] = run; run += c; }
     GS_HIP(hipMemcpyAsync(cur64.ptr, h.data(), NBUCKET * 8, hipMemcpyHostToDevice, st));
     GS_TRY(g->new_id.alloc(n));
-    DevBuf<uint32_t> old_of;
-    GS_TRY(old_of.alloc(n));
-    k_relabel<<<(unsigned)((n + 4095) / 4096), 256, 0, st>>>(g->degree.ptr, n, cur64.ptr, g->new_id.ptr, old_of.ptr);
+    GS_TRY(g->old_id.alloc(n));
+    k_relabel<<<(unsigned)((n + 4095) / 4096), 256, 0, st>>>(g->degree.ptr, n, cur64.ptr, g->new_id.ptr,
+                                                             g->old_id.ptr);
     GS_HIP(hipGetLastError());
     GS_TRY(g->deg_r.alloc(n));
-    k_gather_deg<<<vgrid(n), 256, 0, st>>>(g->degree.ptr, old_of.ptr, n, g->deg_r.ptr);
+    k_gather_deg<<<vgrid(n), 256, 0, st>>>(g->degree.ptr, g->old_id.ptr, n, g->deg_r.ptr);
     GS_HIP(hipGetLastError());
-    old_of.release();
     // 2. offsets = exclusive scan of the relabelled degrees
     const uint64_t nb = (n + 1023) / 1024;
     DevBuf<unsigned long long> bsum;
@@ -312,6 +461,10 @@ int build_adjacency(gs_graph *g) {
         k_adj_scatter<<<grid, SWEEP_WG, 0, s2>>>(a, g->new_id.ptr, g->adj_off.ptr, cur.ptr, g->adj_nbr.ptr);
     }));
     cur.release();
+    if (g->nzn) {
+        k_sort_short_lists<<<8192, 256, 0, st>>>(g->adj_off.ptr, 0, g->nzn, g->adj_nbr.ptr);
+        GS_HIP(hipGetLastError());
+    }
     // 4. pull chunk map over the vertices with edges (relabelled ids 0 .. nzn-1)
     const uint64_t total = 2 * g->E;
     g->npull = (total + PULL_CHUNK - 1) / PULL_CHUNK;
@@ -463,6 +616,9 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     if (w->depth.n != g->n) GS_TRY(w->depth.alloc(g->n));
     if (w->depth_r.n != g->n) GS_TRY(w->depth_r.alloc(g->n));
     if (w->queue.n != g->n) GS_TRY(w->queue.alloc(g->n));
+    if (w->qdeg.n != g->n) GS_TRY(w->qdeg.alloc(g->n));
+    if (w->qoff.n != g->n + 1) GS_TRY(w->qoff.alloc(g->n + 1));
+    if (w->qsum.n != g->n / 1024 + 1) GS_TRY(w->qsum.alloc(g->n / 1024 + 1));
     const uint64_t words = bitmap_words(g);
     w->timer.reset();
     w->stats = gs_stats{};
@@ -477,7 +633,8 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     uint32_t root_r = 0;
     GS_HIP(hipMemcpyAsync(&root_r, g->new_id.ptr + root, 4, hipMemcpyDeviceToHost, w->stream));
     GS_HIP(hipStreamSynchronize(w->stream));
-    k_do_root<<<1, 1, 0, w->stream>>>(root_r, w->depth_r.ptr, w->bm_vis.ptr, w->bm_front.ptr, w->queue.ptr);
+    k_do_root<<<1, 1, 0, w->stream>>>(root_r, g->deg_r.ptr, w->depth_r.ptr, w->bm_vis.ptr, w->bm_front.ptr,
+                                      w->queue.ptr, w->qdeg.ptr);
     GS_HIP(hipGetLastError());
     // heuristic state (Beamer et al.): edges still to check from the unreached side
     uint64_t m_u = 2 * g->E, n_f = 1, m_f = 0;
@@ -488,28 +645,61 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
         m_f = (uint32_t)d;
         m_u -= m_f;
     }
-    bool bottom_up = false;
+    bool bottom_up = false, have_queue = true;
     int nlev = 0, status = GS_OK;
     uint64_t checked = 0;
+    std::vector<unsigned long long> hsum;
+    // qoff = exclusive prefix of the queue's degrees (blocks of 1024, block starts on the host)
+    auto queue_prefix = [&](uint64_t nq) -> int {
+        const uint64_t nb = (nq + 1023) / 1024;
+        k_deg_block_sums<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr);
+        GS_HIP(hipGetLastError());
+        hsum.resize(nb);
+        GS_HIP(hipMemcpyAsync(hsum.data(), w->qsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        unsigned long long run = 0;
+        for (uint64_t i = 0; i < nb; ++i) { const unsigned long long c = hsum[i]; hsum[i] = run; run += c; }
+        GS_HIP(hipMemcpyAsync(w->qsum.ptr, hsum.data(), nb * 8, hipMemcpyHostToDevice, w->stream));
+        k_deg_scan<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr, w->qoff.ptr);
+        GS_HIP(hipGetLastError());
+        return GS_OK;
+    };
     for (uint32_t level = 1;; ++level) {
         const uint64_t m_f_prev = m_f;  // edges a top-down level examines
         if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha) bottom_up = true;
         else if (bottom_up && n_f < g->n / (uint64_t)g->do_beta) bottom_up = false;
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 32, w->stream));
         hipEvent_t kb;
-        w->timer.begin(GS_K_BFS_EXPAND, w->stream, &kb);
-        if (bottom_up)
+        const int kclass = bottom_up ? GS_K_BFS_BOTTOMUP : GS_K_BFS_TOPDOWN;
+        w->timer.begin(kclass, w->stream, &kb);
+        if (bottom_up) {
             k_bu_expand<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, g->adj_off.ptr, g->adj_nbr.ptr, w->bm_vis.ptr,
                                                             w->bm_front.ptr, w->bm_next.ptr, w->counters.ptr + 3);
-        else if (n_f)
-            k_td_expand<<<(unsigned)std::min<uint64_t>((n_f + 3) / 4, 8192), 256, 0, w->stream>>>(
-                w->queue.ptr, n_f, g->adj_off.ptr, g->adj_nbr.ptr, w->bm_vis.ptr, w->bm_next.ptr);
+        } else if (n_f && m_f) {
+            if (!have_queue) {  // the frontier came from a bottom-up level: list it
+                GS_HIP(hipMemsetAsync(w->counters.ptr + 4, 0, 24, w->stream));
+                k_do_level_end<false, true><<<vgrid(words), 256, 0, w->stream>>>(
+                    w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, 0, g->deg_r.ptr, g->n, words,
+                    w->queue.ptr, w->qdeg.ptr, w->counters.ptr + 4);
+                GS_HIP(hipGetLastError());
+            }
+            GS_TRY(queue_prefix(n_f));
+            k_td_edges<<<(unsigned)((m_f + TD_CHUNK - 1) / TD_CHUNK), 256, 0, w->stream>>>(
+                w->queue.ptr, w->qoff.ptr, n_f, m_f, g->adj_off.ptr, g->adj_nbr.ptr, w->bm_vis.ptr, w->bm_next.ptr);
+        }
         GS_HIP(hipGetLastError());
-        w->timer.end(GS_K_BFS_EXPAND, w->stream, kb);
+        w->timer.end(kclass, w->stream, kb);
         w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
-        k_do_level_end<<<vgrid(words), 256, 0, w->stream>>>(w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
-                                                            w->depth_r.ptr, level + 1, g->deg_r.ptr, words,
-                                                            w->queue.ptr, w->counters.ptr);
+        // after a top-down level the next one is likely top-down too: list the new frontier now
+        have_queue = !bottom_up;
+        if (have_queue)
+            k_do_level_end<true, true><<<vgrid(words), 256, 0, w->stream>>>(
+                w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n, words,
+                w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
+        else
+            k_do_level_end<true, false><<<vgrid(words), 256, 0, w->stream>>>(
+                w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n, words,
+                w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
         GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 32, hipMemcpyDeviceToHost, w->stream));

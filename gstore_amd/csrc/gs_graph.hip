@@ -111,7 +111,7 @@ size_t gs_ws::device_bytes() const {
     return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + ppr_acc.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
            bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + kc_deg.bytes() + ms_vis.bytes() +
            ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_counts.bytes() +
-           queue.bytes() + depth_r.bytes() + counters.bytes();
+           queue.bytes() + qdeg.bytes() + qoff.bytes() + qsum.bytes() + depth_r.bytes() + counters.bytes();
 }
 
 gs_graph::~gs_graph() {
@@ -539,7 +539,8 @@ const char *gs_kernel_name(int k) {
     static const char *names[GS_K_COUNT] = {"pr_scatter", "pr_finalize", "bfs_expand", "bfs_level_end",
                                             "wcc_hook", "wcc_compress", "wcc_check", "kcore_peel",
                                             "kcore_scan", "kron_gen", "convert", "h2d_copy", "pr_pull",
-                                            "index_build", "pb_gather", "pb_accum", "pb_reduce"};
+                                            "index_build", "pb_gather", "pb_accum", "pb_reduce",
+                                            "bfs_topdown", "bfs_bottomup"};
     return (k >= 0 && k < GS_K_COUNT) ? names[k] : "?";
 }
 
@@ -709,7 +710,7 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->segment_edges = g->ss ? g->ss->seg_chunks * g->chunk : 0;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
                          g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() +
-                         g->new_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() + gs::pb_bytes(g) +
+                         g->new_id.bytes() + g->old_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() + gs::pb_bytes(g) +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
     const bool built = g->adj_nbr.ptr && g->adj_off.n == g->n + 1;
     info->index_entries = built ? 2 * g->E : 0;
@@ -818,11 +819,19 @@ int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gath
         set_error("blocking limits out of range");
         return GS_ERR_ARG;
     }
+    std::lock_guard<std::mutex> lp(g->pb_mu);  // not while a blocked PageRank runs
     std::lock_guard<std::mutex> lk(g->build_mu);
     g->pb_unit1 = gather_entries;
     g->pb_runs = gather_runs;
     g->pb_unit2 = accum_entries;
     g->pb_ready = false;  // tables rebuilt at the next PageRank
+    return GS_OK;
+}
+
+int gs_set_wcc_method(gs_graph *g, int method) {
+    if (!g) { set_error("null graph"); return GS_ERR_ARG; }
+    if (method < 0 || method > 1) { set_error("WCC method %d: expected 0 or 1", method); return GS_ERR_ARG; }
+    g->wcc_method = method;
     return GS_OK;
 }
 

@@ -133,6 +133,9 @@ struct gs_ws {
     gs::DevBuf<uint64_t> ms_vis, ms_front, ms_next, ms_roots;  // batched BFS lane masks
     gs::DevBuf<uint8_t> ms_depth;                             // lane-major depth, lanes x N
     gs::DevBuf<uint32_t> queue;                               // frontier vertices (direction-optimizing BFS)
+    gs::DevBuf<int32_t> qdeg;                                 // their degrees
+    gs::DevBuf<uint64_t> qoff;                                // exclusive prefix of qdeg (top-down levels)
+    gs::DevBuf<unsigned long long> qsum;                      // per-1024 block sums of qdeg
     gs::DevBuf<uint8_t> depth_r;                              // depth by relabelled id
     gs::DevBuf<unsigned long long> ms_counts;
     gs::DevBuf<unsigned long long> counters;
@@ -187,6 +190,7 @@ struct gs_graph {
     int do_alpha = 14, do_beta = 24;         // top-down <-> bottom-up switch thresholds
     // neighbour-list index (built on demand), in relabelled ids (descending degree)
     gs::DevBuf<uint32_t> new_id;             // relabelled id of each vertex, N
+    gs::DevBuf<uint32_t> old_id;             // original id of each relabelled id, N
     gs::DevBuf<int32_t> deg_r;               // degree by relabelled id, N
     gs::DevBuf<uint64_t> adj_off;            // list offsets by relabelled id, N + 1
     gs::DevBuf<uint32_t> adj_nbr;            // neighbour ids (relabelled), 2E
@@ -194,6 +198,7 @@ struct gs_graph {
     double index_ms = 0;                     // device time of the index build
     gs::DevBuf<uint32_t> chunkz;             // vertex whose list holds entry c*PULL_CHUNK, npull + 1
     int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy
+    int wcc_method = 1;                      // 0 tile hooking passes, 1 sampled linking over the lists
     // source-blocked copy of the lists (built on demand), gs_pb.hip
     bool pb_ready = false;
     uint32_t pb_k = 0;                       // blocks; 0 = not available (PageRank pulls)
@@ -209,6 +214,7 @@ struct gs_graph {
     gs::DevBuf<gs::PbUnit2> pb_u2;
     gs::DevBuf<gs::PbRed> pb_red;
     gs::DevBuf<float> pb_part;               // partial sums of split blocks, PB_B per slot
+    std::mutex pb_mu;                        // one blocked PageRank at a time (pb_val, pb_part are shared)
     std::mutex build_mu;                     // lazy builds (index, heavy set) from concurrent workspaces
     ~gs_graph();
 };

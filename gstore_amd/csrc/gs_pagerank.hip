@@ -258,6 +258,10 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
         GS_TRY(w->pr_acc.alloc(g->n));
         GS_TRY(w->pr_fac.alloc(g->n));
     }
+    // the blocked method's value and partial buffers belong to the graph: calls
+    // from several workspaces take turns (held until the call's stream is drained)
+    std::unique_lock<std::mutex> pb_lock(const_cast<gs_graph *>(g)->pb_mu, std::defer_lock);
+    if (use_pull(g) && g->pr_method == 2) pb_lock.lock();
     GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream, true));
     w->timer.reset();
     w->stats = gs_stats{};

@@ -11,6 +11,12 @@
 // only ever hooked under a smaller id, each root is its component's minimum
 // vertex id, which is the canonical label returned.  A check pass counts
 // edges whose endpoints still differ and repeats the hook pass until none do.
+// On undirected resident grids the default method links over the
+// neighbour-list index instead (sampled linking, below): the same forest rule,
+// so the same labels, with far fewer random accesses.
+#include <algorithm>
+#include <vector>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -34,6 +40,19 @@ __device__ inline uint32_t wcc_find(uint32_t *par, uint32_t x) {
     }
 }
 
+// Joins the trees of u and v: the larger root is hooked under the smaller one
+// with atomicMin; if it was hooked elsewhere meanwhile, join that tree instead.
+__device__ inline void wcc_link(uint32_t *par, uint32_t u, uint32_t v) {
+    uint32_t ru = wcc_find(par, u), rv = wcc_find(par, v);
+    while (ru != rv) {
+        uint32_t hi = max(ru, rv), lo = min(ru, rv);
+        uint32_t old = atomicMin(&par[hi], lo);
+        if (old == hi) break;  // hi was a root and now hangs under lo
+        ru = wcc_find(par, old);
+        rv = wcc_find(par, lo);
+    }
+}
+
 __global__ void k_wcc_init(uint32_t *par, uint64_t n) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
         par[v] = (uint32_t)v;
@@ -44,17 +63,7 @@ __global__ void __launch_bounds__(SWEEP_WG) k_wcc_hook(SweepArgs a, uint32_t *pa
     uint64_t chunk;
     if (!launch_chunk(a, chunk)) return;
     uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
-    auto op = [&](uint32_t w, uint32_t rc) {
-        uint32_t ru = wcc_find(par, gv0(w, rc)), rv = wcc_find(par, gv1(w, rc));
-        while (ru != rv) {
-            uint32_t hi = max(ru, rv), lo = min(ru, rv);
-            uint32_t old = atomicMin(&par[hi], lo);
-            if (old == hi) break;  // hi was a root and now hangs under lo
-            // hi had been hooked to old meanwhile: join old's tree with lo's
-            ru = wcc_find(par, old);
-            rv = wcc_find(par, lo);
-        }
-    };
+    auto op = [&](uint32_t w, uint32_t rc) { wcc_link(par, gv0(w, rc), gv1(w, rc)); };
     sweep_chunk<false>(a, chunk, s, len, op);
 }
 
@@ -93,6 +102,54 @@ __global__ void k_wcc_label(uint32_t *par, const int32_t *__restrict__ deg, cons
 
 static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
 
+// ---- sampled linking over the neighbour-list index (Sutton, Ben-Nun and Barak,
+// "Optimizing parallel graph connectivity computation via subgraph sampling",
+// IPDPS 2018).  The forest is kept in original ids, so each root is still its
+// component's minimum id; the index supplies the neighbours (relabelled ids,
+// mapped back through old_id).
+constexpr uint32_t AFF_ROUNDS = 2;     // neighbours linked for every vertex
+constexpr uint32_t AFF_SAMPLES = 1024; // vertices sampled to find the largest component
+
+__global__ void k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                            const uint32_t *__restrict__ old_id, uint64_t nzn, uint32_t r,
+                            unsigned long long *linked) {
+    uint32_t c = 0;
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = off[x];
+        if (off[x + 1] - b > r) {
+            wcc_link(par, old_id[x], old_id[nbr[b + r]]);
+            ++c;
+        }
+    }
+    if (c) atomicAdd(linked, (unsigned long long)c);
+}
+
+__global__ void k_aff_sample(uint32_t *par, uint64_t n, uint32_t *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= AFF_SAMPLES) return;
+    uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;  // splitmix64 finaliser
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    out[i] = wcc_find(par, (uint32_t)(z % n));
+}
+
+// Remaining neighbours of every vertex outside the component rooted at skip.
+__global__ void k_aff_final(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                            const uint32_t *__restrict__ old_id, uint64_t nzn, uint32_t skip,
+                            unsigned long long *linked) {
+    uint32_t c = 0;
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = off[x] + AFF_ROUNDS, e = off[x + 1];
+        if (b >= e) continue;
+        const uint32_t u = old_id[x];
+        if (wcc_find(par, u) == skip) continue;
+        for (uint64_t k = b; k < e; ++k) wcc_link(par, u, old_id[nbr[k]]);
+        c += (uint32_t)(e - b);
+    }
+    if (c) atomicAdd(linked, (unsigned long long)c);
+}
+
 }  // namespace gs
 
 using namespace gs;
@@ -100,6 +157,8 @@ using namespace gs;
 int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp) {
     if (!g) { set_error("gs_wcc: null graph"); return GS_ERR_ARG; }
     if (w->wcc_parent.n != g->n) GS_TRY(w->wcc_parent.alloc(g->n));
+    const bool by_index = g->wcc_method == 1 && !g->directed && !g->ss;
+    if (by_index) GS_TRY(build_adjacency(const_cast<gs_graph *>(g)));
     w->timer.reset();
     w->stats = gs_stats{};
     hipEvent_t t0, t1;
@@ -108,7 +167,53 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     k_wcc_init<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
     GS_HIP(hipGetLastError());
     uint64_t passes = 0;
-    for (;;) {
+    if (by_index) {
+        // link AFF_ROUNDS neighbours of every vertex, compressing after each round
+        GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
+        hipEvent_t kb;
+        for (uint32_t r = 0; r < AFF_ROUNDS && g->nzn; ++r) {
+            w->timer.begin(GS_K_WCC_HOOK, w->stream, &kb);
+            k_aff_round<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                              g->old_id.ptr, g->nzn, r, w->counters.ptr);
+            GS_HIP(hipGetLastError());
+            w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
+            w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
+            k_wcc_compress<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
+            GS_HIP(hipGetLastError());
+            w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
+        }
+        // the most frequent root among sampled vertices: its members skip their other neighbours
+        uint32_t skip = 0xFFFFFFFFu;
+        if (g->nzn) {
+            DevBuf<uint32_t> samp;
+            GS_TRY(samp.alloc(AFF_SAMPLES));
+            k_aff_sample<<<(AFF_SAMPLES + 255) / 256, 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n, samp.ptr);
+            GS_HIP(hipGetLastError());
+            std::vector<uint32_t> h(AFF_SAMPLES);
+            GS_HIP(hipMemcpyAsync(h.data(), samp.ptr, AFF_SAMPLES * 4, hipMemcpyDeviceToHost, w->stream));
+            GS_HIP(hipStreamSynchronize(w->stream));
+            std::sort(h.begin(), h.end());
+            uint32_t best = 0;
+            for (uint32_t i = 0, j; i < AFF_SAMPLES; i = j) {
+                for (j = i; j < AFF_SAMPLES && h[j] == h[i]; ++j) {}
+                if (j - i > best) { best = j - i; skip = h[i]; }
+            }
+            w->timer.begin(GS_K_WCC_HOOK, w->stream, &kb);
+            k_aff_final<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                              g->old_id.ptr, g->nzn, skip, w->counters.ptr);
+            GS_HIP(hipGetLastError());
+            w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
+        }
+        w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
+        k_wcc_compress<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
+        GS_HIP(hipMemcpyAsync(w->h_counters + 1, w->counters.ptr, 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        w->stats.edges_scanned = w->h_counters[1];
+        passes = 1;
+    }
+    for (; !by_index;) {
         if (passes >= 64) {
             set_error("gs_wcc: no convergence after %llu hook passes", (unsigned long long)passes);
             return GS_ERR_DEVICE;
@@ -145,7 +250,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = passes;
-    w->stats.edges_scanned = g->E * passes * 2;
+    if (!by_index) w->stats.edges_scanned = g->E * passes * 2;
     if (ncomp) *ncomp = w->h_counters[0];
     if (label_out) {
         GS_HIP(hipMemcpyAsync(label_out, w->wcc_parent.ptr, g->n * 4, hipMemcpyDeviceToHost, w->stream));

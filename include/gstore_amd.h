@@ -43,7 +43,7 @@ enum gs_kernel {
     GS_K_PR_SCATTER = 0, GS_K_PR_FINALIZE, GS_K_BFS_EXPAND, GS_K_BFS_LEVEL_END,
     GS_K_WCC_HOOK, GS_K_WCC_COMPRESS, GS_K_WCC_CHECK, GS_K_KCORE_PEEL, GS_K_KCORE_SCAN,
     GS_K_GEN, GS_K_CONVERT, GS_K_H2D, GS_K_PR_PULL, GS_K_INDEX, GS_K_PB_GATHER, GS_K_PB_ACCUM,
-    GS_K_PB_REDUCE, GS_K_COUNT
+    GS_K_PB_REDUCE, GS_K_BFS_TOPDOWN, GS_K_BFS_BOTTOMUP, GS_K_COUNT
 };
 
 typedef struct gs_info {
@@ -230,6 +230,14 @@ int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta);
  * Personalised PageRank uses 1 for both 1 and 2 (FP64 sums).  Directed and
  * semi-external grids always scatter.  Only the FP32 summation order differs. */
 int gs_set_pagerank_method(gs_graph *g, int method);
+
+/* WCC method on undirected resident grids: 1 (default) links over the
+ * neighbour-list index -- two neighbours of every vertex, then, outside the
+ * largest component found by sampling, the remaining neighbours (the Afforest
+ * scheme); 0 repeats hook passes over the tile grid until no stored edge joins
+ * two trees.  Labels and count are identical (minimum vertex id per component,
+ * wcc.cpp:324-337 count).  Directed and semi-external grids use 0. */
+int gs_set_wcc_method(gs_graph *g, int method);
 
 /* Work split of the blocked PageRank: source-major entries (<= 2^30) and
  * runs (<= 2048) per gather workgroup, entries per accumulation workgroup

@@ -389,3 +389,19 @@ def test_bfs_scan_counts_follow_skip_rule(loaded, name):
     st = g.stats()
     assert (st["edges_scanned"], st["tiles_scanned"]) == _scan_counts(n, start, grid, d)
     assert st["iterations"] == int(d.max())
+
+
+@pytest.mark.parametrize("name", IDS)
+def test_wcc_methods_identical(loaded, name):
+    """Sampled linking over the neighbour-list index and hook passes over the
+    tiles give the same canonical labels and count as the oracle."""
+    n, (start, grid, deg), g = loaded[name]
+    want_l, want_c = oracle.wcc(n, start, grid)
+    for method in (0, 1):
+        g.set_wcc_method(method)
+        lab, nc = g.wcc()
+        _say("wcc method", name, method, nc)
+        assert nc == want_c and np.array_equal(lab, want_l)
+    g.set_wcc_method(1)
+    with pytest.raises(ga.GStoreError):
+        g.set_wcc_method(2)
