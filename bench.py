@@ -326,6 +326,15 @@ def main():
         m_reach[r] = int(deg64[d > 0].sum() // 2)
         ref_scan[r] = (st["edges_scanned"], st["tiles_scanned"], st["iterations"], int(np.count_nonzero(d)))
     log("BFS M_reach and reference scan counts per root computed")
+    ref_kc = None
+    if "kcore" in set(args.algos.split(",")):
+        # the reference's tile scans of k-core (same rounds and degrees as the default method)
+        g.set_kcore_method(0)
+        g.kcore(args.k, copy_out=False)
+        st = g.stats()
+        ref_kc = (st["edges_scanned"], st["tiles_scanned"], st["iterations"])
+        g.set_kcore_method(1)
+        log(f"k-core reference scans: {ref_kc}")
 
     algos = set(args.algos.split(","))
     tune = None
@@ -396,7 +405,7 @@ def main():
             absorb(st)
             per["kcore"][0] += st["ms"]
             per["kcore"][1] += 2 * E * st["iterations"]
-            per["kcore"][2] += alg_bytes_kcore(n, st["edges_scanned"], st["tiles_scanned"], st["iterations"])
+            per["kcore"][2] += alg_bytes_kcore(n, *ref_kc)
 
     for i in range(args.warmup):
         step(False)

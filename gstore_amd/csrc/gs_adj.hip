@@ -579,6 +579,38 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
     }
 }
 
+// w->qoff = exclusive prefix of w->qdeg[0 .. nq) (blocks of 1024, block starts
+// on the host); *total (if given) = the sum.
+int queue_prefix(gs_ws *w, uint64_t nq, std::vector<unsigned long long> &hsum, uint64_t *total) {
+    const uint64_t nb = (nq + 1023) / 1024;
+    k_deg_block_sums<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr);
+    GS_HIP(hipGetLastError());
+    hsum.resize(nb);
+    GS_HIP(hipMemcpyAsync(hsum.data(), w->qsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
+    GS_HIP(hipStreamSynchronize(w->stream));
+    unsigned long long run = 0;
+    for (uint64_t i = 0; i < nb; ++i) { const unsigned long long c = hsum[i]; hsum[i] = run; run += c; }
+    GS_HIP(hipMemcpyAsync(w->qsum.ptr, hsum.data(), nb * 8, hipMemcpyHostToDevice, w->stream));
+    k_deg_scan<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr, w->qoff.ptr);
+    GS_HIP(hipGetLastError());
+    if (total) *total = run;
+    return GS_OK;
+}
+
+int ensure_queue(const gs_graph *g, gs_ws *w) {
+    if (w->queue.n != g->n) GS_TRY(w->queue.alloc(g->n));
+    if (w->qdeg.n != g->n) GS_TRY(w->qdeg.alloc(g->n));
+    if (w->qoff.n != g->n + 1) GS_TRY(w->qoff.alloc(g->n + 1));
+    if (w->qsum.n != g->n / 1024 + 1) GS_TRY(w->qsum.alloc(g->n / 1024 + 1));
+    return GS_OK;
+}
+
+int unpermute_i32(const gs_graph *g, hipStream_t st, const int32_t *x_r, int32_t *out) {
+    k_unpermute<int32_t><<<vgrid(g->n), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, out);
+    GS_HIP(hipGetLastError());
+    return GS_OK;
+}
+
 int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out) {
     k_unpermute<float><<<vgrid(g->n), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, out);
     GS_HIP(hipGetLastError());
@@ -615,10 +647,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     GS_TRY(ensure_bitmaps(g, w));
     if (w->depth.n != g->n) GS_TRY(w->depth.alloc(g->n));
     if (w->depth_r.n != g->n) GS_TRY(w->depth_r.alloc(g->n));
-    if (w->queue.n != g->n) GS_TRY(w->queue.alloc(g->n));
-    if (w->qdeg.n != g->n) GS_TRY(w->qdeg.alloc(g->n));
-    if (w->qoff.n != g->n + 1) GS_TRY(w->qoff.alloc(g->n + 1));
-    if (w->qsum.n != g->n / 1024 + 1) GS_TRY(w->qsum.alloc(g->n / 1024 + 1));
+    GS_TRY(ensure_queue(g, w));
     const uint64_t words = bitmap_words(g);
     w->timer.reset();
     w->stats = gs_stats{};
@@ -649,21 +678,6 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     int nlev = 0, status = GS_OK;
     uint64_t checked = 0;
     std::vector<unsigned long long> hsum;
-    // qoff = exclusive prefix of the queue's degrees (blocks of 1024, block starts on the host)
-    auto queue_prefix = [&](uint64_t nq) -> int {
-        const uint64_t nb = (nq + 1023) / 1024;
-        k_deg_block_sums<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr);
-        GS_HIP(hipGetLastError());
-        hsum.resize(nb);
-        GS_HIP(hipMemcpyAsync(hsum.data(), w->qsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
-        GS_HIP(hipStreamSynchronize(w->stream));
-        unsigned long long run = 0;
-        for (uint64_t i = 0; i < nb; ++i) { const unsigned long long c = hsum[i]; hsum[i] = run; run += c; }
-        GS_HIP(hipMemcpyAsync(w->qsum.ptr, hsum.data(), nb * 8, hipMemcpyHostToDevice, w->stream));
-        k_deg_scan<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr, w->qoff.ptr);
-        GS_HIP(hipGetLastError());
-        return GS_OK;
-    };
     for (uint32_t level = 1;; ++level) {
         const uint64_t m_f_prev = m_f;  // edges a top-down level examines
         if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha) bottom_up = true;
@@ -683,7 +697,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
                     w->queue.ptr, w->qdeg.ptr, w->counters.ptr + 4);
                 GS_HIP(hipGetLastError());
             }
-            GS_TRY(queue_prefix(n_f));
+            GS_TRY(queue_prefix(w, n_f, hsum, nullptr));
             k_td_edges<<<(unsigned)((m_f + TD_CHUNK - 1) / TD_CHUNK), 256, 0, w->stream>>>(
                 w->queue.ptr, w->qoff.ptr, n_f, m_f, g->adj_off.ptr, g->adj_nbr.ptr, w->bm_vis.ptr, w->bm_next.ptr);
         }

@@ -828,6 +828,13 @@ int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gath
     return GS_OK;
 }
 
+int gs_set_kcore_method(gs_graph *g, int method) {
+    if (!g) { set_error("null graph"); return GS_ERR_ARG; }
+    if (method < 0 || method > 1) { set_error("k-core method %d: expected 0 or 1", method); return GS_ERR_ARG; }
+    g->kcore_method = method;
+    return GS_OK;
+}
+
 int gs_set_wcc_method(gs_graph *g, int method) {
     if (!g) { set_error("null graph"); return GS_ERR_ARG; }
     if (method < 0 || method > 1) { set_error("WCC method %d: expected 0 or 1", method); return GS_ERR_ARG; }

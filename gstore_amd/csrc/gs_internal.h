@@ -38,7 +38,7 @@ constexpr uint32_t PB_KMAX = 8192;           // blocks at most (vertices with ed
 constexpr uint32_t PB_RUNS = 2048;           // runs staged per gather workgroup at most
 struct PbSlice { uint64_t e0, e1; uint32_t q, pad; };               // build: list entries of block q
 struct PbUnit1 { uint64_t pm0, pm1; uint32_t r0, r1, p, pad; };     // gather: source-major entries of block p
-struct PbUnit2 { uint64_t e0, e1; uint32_t q, slot; };              // accum: slot ~0u = the block's only unit
+struct PbUnit2 { uint64_t e0, e1; uint32_t q, slot, dense, pad; };  // accum: slot ~0u = the block's only unit
 struct PbRed { uint32_t q, slot0, nslot, pad; };                    // split block: its partial slots
 
 void set_error(const char *fmt, ...);
@@ -199,6 +199,7 @@ struct gs_graph {
     gs::DevBuf<uint32_t> chunkz;             // vertex whose list holds entry c*PULL_CHUNK, npull + 1
     int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy
     int wcc_method = 1;                      // 0 tile hooking passes, 1 sampled linking over the lists
+    int kcore_method = 1;                    // 0 tile scans, 1 peeled lists pushed over the index
     // source-blocked copy of the lists (built on demand), gs_pb.hip
     bool pb_ready = false;
     uint32_t pb_k = 0;                       // blocks; 0 = not available (PageRank pulls)
@@ -244,6 +245,9 @@ int build_blocked(gs_graph *g);  // the blocked copy (after the lists); pb_k = 0
 int pr_blocked(const gs_graph *g, gs_ws *w, const float *prior, float *acc);
 size_t pb_bytes(const gs_graph *g);
 int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out);
+int unpermute_i32(const gs_graph *g, hipStream_t st, const int32_t *x_r, int32_t *out);
+int ensure_queue(const gs_graph *g, gs_ws *w);  // queue, qdeg, qoff, qsum sized for N
+int queue_prefix(gs_ws *w, uint64_t nq, std::vector<unsigned long long> &scratch, uint64_t *total);
 int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
                   uint64_t *level_counts, int max_levels, int *levels);
 int run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp);

@@ -11,6 +11,9 @@
 // Decrements are integer atomics, so the final degree array is independent
 // of order and bit-exact against the CPU restatement.  The active set is an
 // N-bit bitmap built with one wave ballot per 64 vertices.
+#include <algorithm>
+#include <vector>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -83,9 +86,153 @@ __global__ void k_kc_count(const int32_t *__restrict__ deg, uint64_t n, unsigned
     if (pos) atomicAdd(&out[1], (unsigned long long)pos);
 }
 
+// ---- peeling over the neighbour-list index (relabelled ids) -----------------
+// The same rule; the removed vertices are listed (with their list lengths)
+// instead of marked, and their lists are pushed: one decrement per list entry,
+// which is one per stored edge with an active endpoint, as the tile scan does.
+
+// Peel of relabelled ids 0 .. nzn-1 (the others have no edges), appending the
+// removed ones to queue / qlen; count[0] = how many.
+__global__ void __launch_bounds__(256) k_kc_peel_list(int32_t *deg, uint64_t nzn, int k,
+                                                      const uint64_t *__restrict__ off, uint32_t *queue,
+                                                      int32_t *qlen, unsigned long long *count) {
+    __shared__ uint32_t wsum[4];
+    __shared__ unsigned long long bbase;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint64_t base = blockIdx.x * 256ull; base < nzn; base += (uint64_t)gridDim.x * 256) {
+        const uint64_t x = base + threadIdx.x;
+        bool pred = false;
+        if (x < nzn) {
+            const int16_t m = (int16_t)deg[x];
+            if (m > 0 && m < k) {
+                deg[x] = 0;
+                pred = true;
+            }
+        }
+        const unsigned long long mask = __ballot(pred);
+        if (lane == 0) wsum[wv] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            bbase = t ? atomicAdd(&count[0], (unsigned long long)t) : 0ull;
+        }
+        __syncthreads();
+        if (pred) {
+            uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+            for (int j = 0; j < wv; ++j) before += wsum[j];
+            const unsigned long long pos = bbase + before;
+            queue[pos] = (uint32_t)x;
+            qlen[pos] = (int32_t)(off[x + 1] - off[x]);
+        }
+        __syncthreads();
+    }
+}
+
+// One decrement per list entry of the listed vertices, balanced over entries
+// (qoff = prefix of the list lengths; KC_CHUNK entries per workgroup, located
+// through staged queue entries as in the top-down BFS).  The KC_HUB heaviest
+// vertices (relabelled ids below KC_HUB) collect their decrements in LDS
+// first: one global atomic per workgroup and hub instead of one per entry.
+constexpr uint32_t KC_CHUNK = 2048;
+constexpr uint32_t KC_HUB = 4096;
+
+__global__ void __launch_bounds__(256) k_kc_push(const uint32_t *__restrict__ queue, const uint64_t *__restrict__ qoff,
+                                                 uint64_t nq, uint64_t m, const uint64_t *__restrict__ off,
+                                                 const uint32_t *__restrict__ nbr, int32_t *deg) {
+    __shared__ uint32_t sbeg[KC_CHUNK + 2];
+    __shared__ uint64_t sadj[KC_CHUNK + 1];
+    __shared__ uint32_t hub[KC_HUB];
+    __shared__ uint64_t j_first;
+    __shared__ int nstaged;
+    const uint64_t c0 = blockIdx.x * (uint64_t)KC_CHUNK;
+    if (c0 >= m) return;
+    const uint32_t len = (uint32_t)min<uint64_t>(KC_CHUNK, m - c0);
+    for (uint32_t i = threadIdx.x; i < KC_HUB; i += blockDim.x) hub[i] = 0;
+    if (threadIdx.x == 0) {
+        uint64_t lo = 0, hi = nq - 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi + 1) >> 1;
+            if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+        }
+        const uint64_t ja = lo;
+        hi = nq - 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi + 1) >> 1;
+            if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
+        }
+        j_first = ja;
+        nstaged = (int)(lo - ja + 1);
+    }
+    __syncthreads();
+    const uint64_t ja = j_first;
+    const int ns = nstaged;
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+        const uint64_t q = qoff[ja + k];
+        sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
+        sadj[k] = off[queue[ja + k]] + (q < c0 ? c0 - q : 0);
+    }
+    if (threadIdx.x == 0) sbeg[ns] = len;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
+        int lo = 0, hi = ns - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t y = nbr[sadj[lo] + (t - sbeg[lo])];
+        if (y < KC_HUB) atomicAdd(&hub[y], 1u);
+        else atomicSub(&deg[y], 1);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < KC_HUB; i += blockDim.x)
+        if (hub[i]) atomicSub(&deg[i], (int32_t)hub[i]);
+}
+
+static unsigned kc_grid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+
 }  // namespace gs
 
 using namespace gs;
+
+// k-core over the index: degrees in relabelled ids, unpermuted at the end.
+static int run_kcore_index(const gs_graph *gc, gs_ws *w, int k, int *rounds_out) {
+    gs_graph *g = const_cast<gs_graph *>(gc);
+    GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, g->deg_r.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+    std::vector<unsigned long long> scratch;
+    uint64_t pushed = 0;
+    int nr = 0;
+    for (;;) {
+        GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
+        hipEvent_t kb;
+        w->timer.begin(GS_K_KCORE_PEEL, w->stream, &kb);
+        if (g->nzn)
+            k_kc_peel_list<<<kc_grid(g->nzn), 256, 0, w->stream>>>(w->kc_deg.ptr, g->nzn, k, g->adj_off.ptr,
+                                                                   w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_KCORE_PEEL, w->stream, kb);
+        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        ++nr;
+        const uint64_t nq = w->h_counters[0];
+        if (nq == 0) break;
+        if (nr >= 1000000) {
+            set_error("gs_kcore: still removing vertices after %d rounds", nr);
+            return GS_ERR_DEVICE;
+        }
+        uint64_t m = 0;
+        w->timer.begin(GS_K_KCORE_SCAN, w->stream, &kb);
+        GS_TRY(queue_prefix(w, nq, scratch, &m));
+        if (m)
+            k_kc_push<<<(unsigned)((m + KC_CHUNK - 1) / KC_CHUNK), 256, 0, w->stream>>>(
+                w->queue.ptr, w->qoff.ptr, nq, m, g->adj_off.ptr, g->adj_nbr.ptr, w->kc_deg.ptr);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_KCORE_SCAN, w->stream, kb);
+        pushed += m;
+    }
+    w->stats.edges_scanned = pushed;
+    *rounds_out = nr;
+    return GS_OK;
+}
 
 int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t *count_ne0,
                    uint64_t *count_pos, int *rounds) {
@@ -96,17 +243,26 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     }
     GS_TRY(ensure_bitmaps(g, w));
     if (w->kc_deg.n != g->n) GS_TRY(w->kc_deg.alloc(g->n));
+    if (g->kcore_method == 1 && !g->ss) {
+        GS_TRY(build_adjacency(const_cast<gs_graph *>(g)));
+        GS_TRY(ensure_queue(g, w));
+    }
     w->timer.reset();
     w->stats = gs_stats{};
     hipEvent_t t0, t1;
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t0, w->stream));
-    GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, g->degree.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
-    GS_HIP(hipMemsetAsync(w->rowact.ptr, 0, w->rowact.bytes(), w->stream));
-    GS_HIP(hipMemsetAsync(w->counters.ptr + 2, 0, 16, w->stream));
+    const bool by_index = g->kcore_method == 1 && !g->ss;
     int nr = 0;
-    for (;;) {
+    if (by_index) {
+        GS_TRY(run_kcore_index(g, w, k, &nr));
+    } else {
+        GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, g->degree.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+        GS_HIP(hipMemsetAsync(w->rowact.ptr, 0, w->rowact.bytes(), w->stream));
+        GS_HIP(hipMemsetAsync(w->counters.ptr + 2, 0, 16, w->stream));
+    }
+    for (; !by_index;) {
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
         hipEvent_t kb;
         w->timer.begin(GS_K_KCORE_PEEL, w->stream, &kb);
@@ -130,6 +286,10 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     k_kc_count<<<(unsigned)std::min<uint64_t>((g->n + 255) / 256, 8192), 256, 0, w->stream>>>(w->kc_deg.ptr, g->n,
                                                                                             w->counters.ptr);
     GS_HIP(hipGetLastError());
+    if (by_index) {  // relabelled degrees back to vertex order (through qdeg, N entries)
+        GS_TRY(unpermute_i32(g, w->stream, w->kc_deg.ptr, w->qdeg.ptr));
+        GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, w->qdeg.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+    }
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 32, hipMemcpyDeviceToHost, w->stream));
     GS_TRY(sync_and_collect(w));
@@ -139,8 +299,10 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nr;
-    w->stats.edges_scanned = w->h_counters[2];
-    w->stats.tiles_scanned = w->h_counters[3];
+    if (!by_index) {
+        w->stats.edges_scanned = w->h_counters[2];
+        w->stats.tiles_scanned = w->h_counters[3];
+    }
     if (count_ne0) *count_ne0 = w->h_counters[0];
     if (count_pos) *count_pos = w->h_counters[1];
     if (rounds) *rounds = nr;

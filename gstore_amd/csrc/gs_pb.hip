@@ -29,6 +29,7 @@ namespace {
 
 constexpr int PB_WG = 1024;            // threads per workgroup of the three per-iteration kernels
 constexpr int PB_PASS = 4096;          // build: list entries staged per pass
+constexpr uint64_t PB_DENSE = 64;      // accum: entries per vertex from which a block uses LDS float atomics
 constexpr uint64_t PB_SLICE = 65536;   // build: list entries per workgroup
 
 unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
@@ -209,6 +210,20 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
 // of equal destination offsets in registers and each run into LDS once.
 typedef unsigned int pb_u32x4 __attribute__((ext_vector_type(4)));
 
+// FP32 add into LDS by a 32-bit compare-and-swap loop.  On this GPU the LDS
+// float atomic costs about four times a streamed entry when the addresses of a
+// wave are spread (blocks of low-degree vertices), while a CAS that rarely
+// retries costs about as much as an integer atomic; where a wave's lanes share
+// addresses (blocks of heavy vertices) the hardware float atomic is kept.
+__device__ inline void lds_add_cas(float *p, float x) {
+    unsigned *a = reinterpret_cast<unsigned *>(p);
+    unsigned old = *a, assumed;
+    do {
+        assumed = old;
+        old = atomicCAS(a, assumed, __float_as_uint(__uint_as_float(assumed) + x));
+    } while (old != assumed);
+}
+
 __global__ void __launch_bounds__(PB_WG) k_pb_accum(const PbUnit2 *__restrict__ units,
                                                     const uint16_t *__restrict__ dst,
                                                     const float *__restrict__ val, uint64_t nzn,
@@ -227,6 +242,11 @@ __global__ void __launch_bounds__(PB_WG) k_pb_accum(const PbUnit2 *__restrict__ 
         for (int k = 0; k < 4; ++k) vq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(val + e0) + k);
         uint32_t cur = 0xFFFFFFFFu;
         float run = 0.0f;
+        auto flush = [&]() {
+            if (cur == 0xFFFFFFFFu) return;
+            if (u.dense) atomicAdd(&sum[cur], run);
+            else lds_add_cas(&sum[cur], run);
+        };
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const uint64_t e = e0 + i;
@@ -234,14 +254,14 @@ __global__ void __launch_bounds__(PB_WG) k_pb_accum(const PbUnit2 *__restrict__ 
             const uint32_t d = (dq[i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
             const float x = __uint_as_float(vq[i >> 2][i & 3]);
             if (d != cur) {
-                if (cur != 0xFFFFFFFFu) atomicAdd(&sum[cur], run);
+                flush();
                 cur = d;
                 run = x;
             } else {
                 run += x;
             }
         }
-        if (cur != 0xFFFFFFFFu) atomicAdd(&sum[cur], run);
+        flush();
     }
     __syncthreads();
     const uint64_t qbase = (uint64_t)u.q << PB_SHIFT;
@@ -426,13 +446,16 @@ int build_blocked(gs_graph *g) {
         const uint64_t e0 = qm[(uint64_t)q * K], e1 = q + 1 < K ? qm[(uint64_t)(q + 1) * K] : run;
         if (e1 == e0) continue;
         const uint64_t parts = (e1 - e0 + unit2 - 1) / unit2;
+        // entries per vertex of the block: above PB_DENSE, lanes of a wave mostly share destinations
+        const uint64_t qn = std::min<uint64_t>(PB_B, nzn - ((uint64_t)q << PB_SHIFT));
+        const uint32_t dense = (e1 - e0) >= PB_DENSE * qn;
         if (parts == 1) {
-            u2.push_back(PbUnit2{e0, e1, q, 0xFFFFFFFFu});
+            u2.push_back(PbUnit2{e0, e1, q, 0xFFFFFFFFu, dense, 0});
             continue;
         }
         red.push_back(PbRed{q, slots, (uint32_t)parts, 0});
         for (uint64_t k = 0; k < parts; ++k)
-            u2.push_back(PbUnit2{e0 + (e1 - e0) * k / parts, e0 + (e1 - e0) * (k + 1) / parts, q, slots++});
+            u2.push_back(PbUnit2{e0 + (e1 - e0) * k / parts, e0 + (e1 - e0) * (k + 1) / parts, q, slots++, dense, 0});
     }
     GS_TRY(check_units(u1, u2, red, rpm, run, runs_max, slots));
     GS_TRY(g->pb_src.alloc(2 * g->E));

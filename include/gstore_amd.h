@@ -239,6 +239,14 @@ int gs_set_pagerank_method(gs_graph *g, int method);
  * wcc.cpp:324-337 count).  Directed and semi-external grids use 0. */
 int gs_set_wcc_method(gs_graph *g, int method);
 
+/* k-core method on undirected resident grids: 1 (default) lists the vertices
+ * each peel removes and pushes one decrement per entry of their neighbour
+ * lists (the index), the heaviest vertices' decrements gathered per
+ * workgroup; 0 rescans the tiles whose row or column holds a removed vertex
+ * (kcore.cpp:69-103 with the skip of gstore.cpp:2768-2781).  Final degrees,
+ * counts and rounds are identical.  Semi-external grids use 0. */
+int gs_set_kcore_method(gs_graph *g, int method);
+
 /* Work split of the blocked PageRank: source-major entries (<= 2^30) and
  * runs (<= 2048) per gather workgroup, entries per accumulation workgroup
  * (<= 2^30).  Defaults 2^20, 2048, 2^20.  Small values exercise the split

@@ -405,3 +405,20 @@ def test_wcc_methods_identical(loaded, name):
     g.set_wcc_method(1)
     with pytest.raises(ga.GStoreError):
         g.set_wcc_method(2)
+
+
+@pytest.mark.parametrize("name", [c for c in IDS if c != "two_groups"] + ["two_groups"])
+@pytest.mark.parametrize("k", [2, 8, 40])
+def test_kcore_methods_identical(loaded, name, k):
+    """Peeled lists pushed over the index and tile rescans give the oracle's
+    final degrees, counts and rounds."""
+    n, (start, grid, deg), g = loaded[name]
+    want = oracle.kcore(n, start, grid, deg, k)
+    for method in (0, 1):
+        g.set_kcore_method(method)
+        got = g.kcore(k)
+        _say("kcore method", name, k, method, got[1:])
+        assert np.array_equal(got[0], want[0]) and tuple(got[1:]) == tuple(want[1:])
+    g.set_kcore_method(1)
+    with pytest.raises(ga.GStoreError):
+        g.set_kcore_method(2)
