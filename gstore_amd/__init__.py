@@ -110,6 +110,7 @@ _SIGS = {
     "gs_set_pagerank_blocking": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]),
     "gs_set_wcc_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_kcore_method": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_set_batch_bfs_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_bfs_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "gs_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_Tune)]),
     "gs_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -284,6 +285,10 @@ class Graph:
     def set_wcc_method(self, method: int) -> None:
         """1 (default): sampled linking over the neighbour-list index; 0: hook passes over the tiles."""
         _check(_lib.gs_set_wcc_method(self._h, int(method)))
+
+    def set_batch_bfs_method(self, method: int) -> None:
+        """1 (default): lanes over the neighbour-list index; 0: lanes over the tile sweep."""
+        _check(_lib.gs_set_batch_bfs_method(self._h, int(method)))
 
     def set_kcore_method(self, method: int) -> None:
         """1 (default): push the removed vertices' lists over the index; 0: rescan active tiles."""

@@ -41,15 +41,22 @@ __global__ void __launch_bounds__(SWEEP_WG) k_bfs_expand(SweepArgs a, const uint
         uint32_t wi = v >> 5, b = 1u << (v & 31);
         if ((vis[wi] & b) == 0 && (next[wi] & b) == 0) atomicOr(&next[wi], b);
     };
-    auto op = [&](uint32_t w, uint32_t rc) {
-        uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
-        if ((front[v0 >> 5] >> (v0 & 31)) & 1u) {
+    // frontier words of both endpoints of the quad's four edges load together
+    auto pre = [&](uint32_t w, uint32_t rc) {
+        uint2 f;
+        f.x = front[gv0(w, rc) >> 5];
+        f.y = DIRECTED ? 0u : front[gv1(w, rc) >> 5];
+        return f;
+    };
+    auto op = [&](uint32_t w, uint32_t rc, uint2 f) {
+        const uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
+        if ((f.x >> (v0 & 31)) & 1u) {
             reach(v1);
-        } else if (!DIRECTED && ((front[v1 >> 5] >> (v1 & 31)) & 1u)) {
+        } else if (!DIRECTED && ((f.y >> (v1 & 31)) & 1u)) {
             reach(v0);
         }
     };
-    sweep_chunk<true>(a, chunk, s, len, op);
+    sweep_chunk_pre<true>(a, chunk, s, len, pre, op);
 }
 
 // One workgroup per tile row (2048 bitmap words).

@@ -8,6 +8,9 @@
 // every lane's depth array equals the single-root result bit for bit.  The
 // level-end pass writes lane i's depth (u8, 0 = unreached, root = 1) into
 // depth[i * N + v] and counts each lane's new frontier exactly.
+#include <algorithm>
+#include <vector>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -42,16 +45,18 @@ __global__ void __launch_bounds__(SWEEP_WG) k_msbfs_expand(SweepArgs a, const un
         unsigned long long m = f & ~vis[v];
         if (m && (m & ~next[v])) atomicOr(&next[v], m);
     };
-    auto op = [&](uint32_t w, uint32_t rc) {
-        uint32_t v0 = gv0(w, rc), v1 = gv1(w, rc);
-        unsigned long long f0 = front[v0];
-        if (f0) push(f0, v1);
-        if (!DIRECTED) {
-            unsigned long long f1 = front[v1];
-            if (f1) push(f1, v0);
-        }
+    // frontier masks of both endpoints of the quad's four edges load together
+    auto pre = [&](uint32_t w, uint32_t rc) {
+        ulonglong2 f;
+        f.x = front[gv0(w, rc)];
+        f.y = DIRECTED ? 0ull : front[gv1(w, rc)];
+        return f;
     };
-    sweep_chunk<true>(a, chunk, s, len, op);
+    auto op = [&](uint32_t w, uint32_t rc, ulonglong2 f) {
+        if (f.x) push(f.x, gv1(w, rc));
+        if (!DIRECTED && f.y) push(f.y, gv0(w, rc));
+    };
+    sweep_chunk_pre<true>(a, chunk, s, len, pre, op);
 }
 
 // One workgroup per tile row; lane counts accumulate in LDS.
@@ -100,12 +105,344 @@ __global__ void __launch_bounds__(256) k_msbfs_level_end(unsigned long long *vis
     }
 }
 
+// ---- multi-source BFS over the neighbour-list index (relabelled ids) -----------
+// The same per-lane rule as the tile sweep above, with Beamer's direction
+// switch on the union frontier: top-down levels push each frontier vertex's
+// lane mask along its list (balanced over edges, as the single-root top-down);
+// bottom-up levels give every vertex that still misses lanes a thread that ORs
+// its neighbours' frontier masks, heaviest neighbours first, and stops once all
+// missing lanes are found.  Depths, when kept, are written vertex-major (one
+// 64-byte row per vertex) and mapped back to vertex order at the end.
+constexpr uint32_t MS_CHUNK = 2048;
+
+__global__ void k_ms_roots(const uint64_t *__restrict__ roots, int nr, const uint32_t *__restrict__ new_id,
+                           const int32_t *__restrict__ deg, unsigned long long *vis, unsigned long long *front,
+                           uint32_t *queue, int32_t *qdeg, uint8_t *depth_vm) {
+    const int i = threadIdx.x;
+    if (i >= nr) return;
+    const uint32_t r = new_id[roots[i]];
+    atomicOr(&vis[r], 1ull << i);
+    atomicOr(&front[r], 1ull << i);
+    queue[i] = r;
+    qdeg[i] = deg[r];
+    if (depth_vm) depth_vm[(uint64_t)r * 64 + i] = 1;
+}
+
+__global__ void __launch_bounds__(256) k_ms_td(const uint32_t *__restrict__ queue, const uint64_t *__restrict__ qoff,
+                                               uint64_t nq, uint64_t m, const uint64_t *__restrict__ off,
+                                               const uint32_t *__restrict__ nbr, const unsigned long long *__restrict__ front,
+                                               const unsigned long long *__restrict__ vis, unsigned long long *next) {
+    __shared__ uint32_t sbeg[MS_CHUNK + 2];
+    __shared__ uint64_t sadj[MS_CHUNK + 1];
+    __shared__ unsigned long long sfr[MS_CHUNK + 1];
+    __shared__ uint64_t j_first;
+    __shared__ int nstaged;
+    const uint64_t c0 = blockIdx.x * (uint64_t)MS_CHUNK;
+    if (c0 >= m) return;
+    const uint32_t len = (uint32_t)min<uint64_t>(MS_CHUNK, m - c0);
+    if (threadIdx.x == 0) {
+        uint64_t lo = 0, hi = nq - 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi + 1) >> 1;
+            if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+        }
+        const uint64_t ja = lo;
+        hi = nq - 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi + 1) >> 1;
+            if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
+        }
+        j_first = ja;
+        nstaged = (int)(lo - ja + 1);
+    }
+    __syncthreads();
+    const uint64_t ja = j_first;
+    const int ns = nstaged;
+    for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+        const uint64_t q = qoff[ja + k];
+        const uint32_t v = queue[ja + k];
+        sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
+        sadj[k] = off[v] + (q < c0 ? c0 - q : 0);
+        sfr[k] = front[v];
+    }
+    if (threadIdx.x == 0) sbeg[ns] = len;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
+        int lo = 0, hi = ns - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t u = nbr[sadj[lo] + (t - sbeg[lo])];
+        const unsigned long long mm = sfr[lo] & ~vis[u];
+        if (mm && (mm & ~next[u])) atomicOr(&next[u], mm);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ms_bu(uint64_t nzn, unsigned long long all, const uint64_t *__restrict__ off,
+                                               const uint32_t *__restrict__ nbr,
+                                               const unsigned long long *__restrict__ vis,
+                                               const unsigned long long *__restrict__ front, unsigned long long *next,
+                                               unsigned long long *checked) {
+    uint32_t ck = 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nzn; v += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long need = all & ~vis[v];
+        if (!need) continue;
+        const uint64_t b = off[v], e = off[v + 1];
+        unsigned long long acc = 0;
+        for (uint64_t k = b; k < e && acc != need; k += 4) {
+            uint32_t u[4];
+            unsigned long long f[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) u[j] = nbr[min(k + j, e - 1)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = front[u[j]];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k + j < e) acc |= f[j] & need;
+            ck += (uint32_t)min<uint64_t>(4, e - k);
+        }
+        if (acc) next[v] = acc;
+    }
+    if (ck) atomicAdd(checked, (unsigned long long)ck);
+}
+
+// Level end over relabelled ids 0 .. nzn-1: next becomes the frontier; per-lane
+// counts of newly reached vertices; frontier size and degree sum (count[0],
+// count[1]) and the degree sum of vertices still missing lanes (count[3]).
+// BUILD lists the new frontier (queue, qdeg; cursor count[2]); DEPTH writes
+// depth rows.
+template <bool BUILD, bool DEPTH>
+__global__ void __launch_bounds__(256) k_ms_level_end(uint64_t nzn, unsigned long long all, unsigned long long *vis,
+                                                      unsigned long long *front, unsigned long long *next,
+                                                      const int32_t *__restrict__ deg, uint8_t *depth_vm,
+                                                      uint32_t newdepth, uint32_t *queue, int32_t *qdeg,
+                                                      unsigned long long *count, unsigned long long *lane_counts) {
+    __shared__ uint32_t cnt[64];
+    __shared__ uint32_t wsum[4];
+    __shared__ unsigned long long bbase, bsum[3];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 3) bsum[threadIdx.x] = 0;
+    uint32_t my = 0;
+    unsigned long long nf = 0, mf = 0, mu = 0;
+    for (uint64_t base = blockIdx.x * 256ull; base < nzn; base += (uint64_t)gridDim.x * 256) {
+        const uint64_t v = base + threadIdx.x;
+        unsigned long long nx = 0;
+        int32_t dv = 0;
+        if (v < nzn) {
+            nx = next[v];
+            front[v] = nx;
+            const unsigned long long vs = vis[v];
+            dv = deg[v];
+            if (nx) {
+                next[v] = 0;
+                vis[v] = vs | nx;
+                nf += 1;
+                mf += (uint32_t)dv;
+                if (DEPTH) {
+                    unsigned long long mm = nx;
+                    while (mm) {
+                        const int l = __ffsll((long long)mm) - 1;
+                        mm &= mm - 1;
+                        depth_vm[v * 64 + l] = (uint8_t)newdepth;
+                    }
+                }
+            }
+            if (((vs | nx) & all) != all) mu += (uint32_t)dv;
+        }
+        if (__ballot(nx != 0)) {
+            for (int l = 0; l < 64; ++l) {
+                const uint32_t c = (uint32_t)__popcll(__ballot((nx >> l) & 1ull));
+                if (lane == l) my += c;
+            }
+        }
+        if (BUILD) {
+            const unsigned long long mask = __ballot(nx != 0);
+            if (lane == 0) wsum[wv] = (uint32_t)__popcll(mask);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+                bbase = t ? atomicAdd(&count[2], (unsigned long long)t) : 0ull;
+            }
+            __syncthreads();
+            if (nx) {
+                uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+                for (int j = 0; j < wv; ++j) before += wsum[j];
+                queue[bbase + before] = (uint32_t)v;
+                qdeg[bbase + before] = dv;
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (my) atomicAdd(&cnt[lane], my);
+    if (nf) atomicAdd(&bsum[0], nf);
+    if (mf) atomicAdd(&bsum[1], mf);
+    if (mu) atomicAdd(&bsum[2], mu);
+    __syncthreads();
+    if (threadIdx.x < 64 && cnt[threadIdx.x]) atomicAdd(&lane_counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        if (bsum[0]) atomicAdd(&count[0], bsum[0]);
+        if (bsum[1]) atomicAdd(&count[1], bsum[1]);
+        if (bsum[2]) atomicAdd(&count[3], bsum[2]);
+    }
+}
+
+// Lists the current frontier (vertices with a non-zero mask) for a top-down level.
+__global__ void __launch_bounds__(256) k_ms_list(uint64_t nzn, const unsigned long long *__restrict__ front,
+                                                 const int32_t *__restrict__ deg, uint32_t *queue, int32_t *qdeg,
+                                                 unsigned long long *count) {
+    __shared__ uint32_t wsum[4];
+    __shared__ unsigned long long bbase;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint64_t base = blockIdx.x * 256ull; base < nzn; base += (uint64_t)gridDim.x * 256) {
+        const uint64_t v = base + threadIdx.x;
+        const bool in = v < nzn && front[v] != 0;
+        const unsigned long long mask = __ballot(in);
+        if (lane == 0) wsum[wv] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            bbase = t ? atomicAdd(&count[0], (unsigned long long)t) : 0ull;
+        }
+        __syncthreads();
+        if (in) {
+            uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+            for (int j = 0; j < wv; ++j) before += wsum[j];
+            queue[bbase + before] = (uint32_t)v;
+            qdeg[bbase + before] = deg[v];
+        }
+        __syncthreads();
+    }
+}
+
+// depth (lane-major, vertex order) from the vertex-major rows of relabelled ids.
+__global__ void k_ms_unpermute(const uint8_t *__restrict__ depth_vm, const uint32_t *__restrict__ new_id, uint64_t n,
+                               int nr, uint8_t *depth) {
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n; o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 *row = reinterpret_cast<const uint4 *>(depth_vm + (uint64_t)new_id[o] * 64);
+        uint4 q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = row[k];
+        const uint8_t *b = reinterpret_cast<const uint8_t *>(q);
+        for (int l = 0; l < nr; ++l) depth[(uint64_t)l * n + o] = b[l];
+    }
+}
+
+static unsigned ms_grid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+
 }  // namespace gs
 
 using namespace gs;
 
+// Multi-source BFS over the index (undirected resident grids).  keep_depth: write
+// the per-lane depths into w->ms_depth (lane-major, vertex order).
+static int run_bfs_batch_index(const gs_graph *gc, gs_ws *w, const uint64_t *roots, int nr, uint64_t *level_counts,
+                               int max_levels, int *levels, bool keep_depth) {
+    gs_graph *g = const_cast<gs_graph *>(gc);
+    const uint64_t n = g->n, nzn = g->nzn;
+    const unsigned long long all = nr == 64 ? ~0ull : ((1ull << nr) - 1);
+    unsigned long long *vis = (unsigned long long *)w->ms_vis.ptr, *front = (unsigned long long *)w->ms_front.ptr,
+                       *next = (unsigned long long *)w->ms_next.ptr;
+    uint8_t *dvm = keep_depth ? w->ms_depth_vm.ptr : nullptr;
+    GS_HIP(hipMemcpyAsync(w->ms_roots.ptr, roots, nr * 8, hipMemcpyHostToDevice, w->stream));
+    GS_HIP(hipMemsetAsync(vis, 0, n * 8, w->stream));
+    GS_HIP(hipMemsetAsync(front, 0, n * 8, w->stream));
+    GS_HIP(hipMemsetAsync(next, 0, n * 8, w->stream));
+    if (dvm) GS_HIP(hipMemsetAsync(dvm, 0, n * 64, w->stream));
+    k_ms_roots<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->new_id.ptr, g->deg_r.ptr, vis, front, w->queue.ptr,
+                                        w->qdeg.ptr, dvm);
+    GS_HIP(hipGetLastError());
+    std::vector<unsigned long long> lc(64), scratch;
+    uint64_t n_f = (uint64_t)nr, m_u = 2 * g->E, checked = 0;
+    bool bottom_up = false, have_queue = true;
+    int nlev = 0, status = GS_OK;
+    uint64_t m_f = 0;
+    {   // the roots' degree sum
+        std::vector<int32_t> d(nr);
+        GS_HIP(hipMemcpyAsync(d.data(), w->qdeg.ptr, nr * 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        for (int i = 0; i < nr; ++i) m_f += (uint32_t)d[i];
+    }
+    // lanes whose frontier is not empty: a finished lane can reach nothing more,
+    // so bottom-up stops looking for it
+    unsigned long long active = all;
+    for (uint32_t level = 1;; ++level) {
+        if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha) bottom_up = true;
+        else if (bottom_up && n_f < nzn / (uint64_t)g->do_beta) bottom_up = false;
+        GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 5 * 8, w->stream));
+        GS_HIP(hipMemsetAsync(w->ms_counts.ptr, 0, 64 * 8, w->stream));
+        hipEvent_t kb;
+        const int kclass = bottom_up ? GS_K_BFS_BOTTOMUP : GS_K_BFS_TOPDOWN;
+        w->timer.begin(kclass, w->stream, &kb);
+        if (bottom_up) {
+            if (nzn)
+                k_ms_bu<<<ms_grid(nzn), 256, 0, w->stream>>>(nzn, active, g->adj_off.ptr, g->adj_nbr.ptr, vis,
+                                                             front, next, w->counters.ptr + 4);
+        } else if (n_f && m_f) {
+            uint64_t nq = n_f;
+            if (!have_queue) {
+                GS_HIP(hipMemsetAsync(w->counters.ptr + 5, 0, 8, w->stream));
+                k_ms_list<<<ms_grid(nzn), 256, 0, w->stream>>>(nzn, front, g->deg_r.ptr, w->queue.ptr, w->qdeg.ptr,
+                                                               w->counters.ptr + 5);
+                GS_HIP(hipGetLastError());
+            }
+            uint64_t m = 0;
+            GS_TRY(queue_prefix(w, nq, scratch, &m));
+            if (m)
+                k_ms_td<<<(unsigned)((m + MS_CHUNK - 1) / MS_CHUNK), 256, 0, w->stream>>>(
+                    w->queue.ptr, w->qoff.ptr, nq, m, g->adj_off.ptr, g->adj_nbr.ptr, front, vis, next);
+            checked += m;
+        }
+        GS_HIP(hipGetLastError());
+        w->timer.end(kclass, w->stream, kb);
+        have_queue = !bottom_up;
+        w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
+        if (nzn) {
+#define MS_LE(B, D) k_ms_level_end<B, D><<<ms_grid(nzn), 256, 0, w->stream>>>(nzn, active, vis, front, next, g->deg_r.ptr, \
+                        dvm, level + 1, w->queue.ptr, w->qdeg.ptr, w->counters.ptr, w->ms_counts.ptr)
+            if (have_queue && dvm) MS_LE(true, true);
+            else if (have_queue) MS_LE(true, false);
+            else if (dvm) MS_LE(false, true);
+            else MS_LE(false, false);
+#undef MS_LE
+        }
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
+        GS_HIP(hipMemcpyAsync(lc.data(), w->ms_counts.ptr, 64 * 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 5 * 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        n_f = w->h_counters[0];
+        m_f = w->h_counters[1];
+        m_u = w->h_counters[3];
+        if (bottom_up) checked += w->h_counters[4];
+        unsigned long long tot = 0;
+        active = 0;
+        for (int i = 0; i < nr; ++i) {
+            if (level_counts && nlev < max_levels) level_counts[(size_t)i * max_levels + nlev] = lc[i];
+            tot += lc[i];
+            if (lc[i]) active |= 1ull << i;
+        }
+        ++nlev;
+        if (tot == 0) break;
+        if (level + 1 > 255) {
+            set_error("batched BFS needs depth > 255, beyond the u8 depth encoding");
+            status = GS_ERR_DEPTH_OVERFLOW;
+            break;
+        }
+    }
+    if (status == GS_OK && dvm) {
+        k_ms_unpermute<<<ms_grid(n), 256, 0, w->stream>>>(dvm, g->new_id.ptr, n, nr, w->ms_depth.ptr);
+        GS_HIP(hipGetLastError());
+    }
+    w->stats.edges_scanned = checked;
+    if (levels) *levels = nlev;
+    return status;
+}
+
 int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
-                       uint64_t *level_counts, int max_levels, int *levels) {
+                       uint64_t *level_counts, int max_levels, int *levels, bool keep_depth) {
     if (!g || !roots || nr < 1 || nr > 64 || max_levels < 0) {
         set_error("gs_bfs_batch: bad arguments (need 1..64 roots)");
         return GS_ERR_ARG;
@@ -115,13 +452,22 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
             set_error("gs_bfs_batch: root %llu >= N", (unsigned long long)roots[i]);
             return GS_ERR_ARG;
         }
-    GS_TRY(ensure_rowact(g, w));
+    const bool by_index = g->batch_method == 1 && !g->directed && !g->ss;
+    keep_depth = keep_depth || depth_out;
+    if (by_index) {
+        GS_TRY(build_adjacency(const_cast<gs_graph *>(g)));
+        GS_TRY(ensure_queue(g, w));
+        if (keep_depth && w->ms_depth_vm.n != g->n * 64) GS_TRY(w->ms_depth_vm.alloc(g->n * 64));
+    } else {
+        GS_TRY(ensure_rowact(g, w));
+    }
     if (w->ms_vis.n != g->n) {
         GS_TRY(w->ms_vis.alloc(g->n));
         GS_TRY(w->ms_front.alloc(g->n));
         GS_TRY(w->ms_next.alloc(g->n));
     }
-    if (w->ms_depth.n < (size_t)nr * g->n) GS_TRY(w->ms_depth.alloc((size_t)nr * g->n));
+    if (keep_depth || !by_index)
+        if (w->ms_depth.n < (size_t)nr * g->n) GS_TRY(w->ms_depth.alloc((size_t)nr * g->n));
     if (w->ms_roots.n < 64) GS_TRY(w->ms_roots.alloc(64));
     if (w->ms_counts.n < 64) GS_TRY(w->ms_counts.alloc(64));
     w->timer.reset();
@@ -130,6 +476,25 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     GS_HIP(hipEventCreate(&t0));
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t0, w->stream));
+    if (by_index) {
+        int nlev = 0;
+        const int status = run_bfs_batch_index(g, w, roots, nr, level_counts, max_levels, &nlev, keep_depth);
+        GS_HIP(hipEventRecord(t1, w->stream));
+        GS_TRY(sync_and_collect(w));
+        float ms = 0.f;
+        GS_HIP(hipEventElapsedTime(&ms, t0, t1));
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        w->stats.ms = ms;
+        w->stats.iterations = (uint64_t)nlev;
+        if (levels) *levels = nlev;
+        if (status != GS_OK) return status;
+        if (depth_out) {
+            GS_HIP(hipMemcpyAsync(depth_out, w->ms_depth.ptr, (size_t)nr * g->n, hipMemcpyDeviceToHost, w->stream));
+            GS_HIP(hipStreamSynchronize(w->stream));
+        }
+        return GS_OK;
+    }
     GS_HIP(hipMemcpyAsync(w->ms_roots.ptr, roots, nr * 8, hipMemcpyHostToDevice, w->stream));
     GS_HIP(hipMemsetAsync(w->ms_vis.ptr, 0, g->n * 8, w->stream));
     GS_HIP(hipMemsetAsync(w->ms_front.ptr, 0, g->n * 8, w->stream));
@@ -199,5 +564,5 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
 extern "C" int gs_bfs_batch(gs_graph *g, const uint64_t *roots, int nr, uint8_t *depth_out, uint64_t *level_counts,
                             int max_levels, int *levels) {
     if (!g) { set_error("gs_bfs_batch: null graph"); return GS_ERR_ARG; }
-    return run_bfs_batch(g, g->ws, roots, nr, depth_out, level_counts, max_levels, levels);
+    return run_bfs_batch(g, g->ws, roots, nr, depth_out, level_counts, max_levels, levels, depth_out != nullptr);
 }

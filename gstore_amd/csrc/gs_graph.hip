@@ -110,7 +110,7 @@ gs_ws::~gs_ws() {
 size_t gs_ws::device_bytes() const {
     return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + ppr_acc.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
            bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + kc_deg.bytes() + ms_vis.bytes() +
-           ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_counts.bytes() +
+           ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_depth_vm.bytes() + ms_counts.bytes() +
            queue.bytes() + qdeg.bytes() + qoff.bytes() + qsum.bytes() + depth_r.bytes() + counters.bytes();
 }
 
@@ -825,6 +825,13 @@ int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gath
     g->pb_runs = gather_runs;
     g->pb_unit2 = accum_entries;
     g->pb_ready = false;  // tables rebuilt at the next PageRank
+    return GS_OK;
+}
+
+int gs_set_batch_bfs_method(gs_graph *g, int method) {
+    if (!g) { set_error("null graph"); return GS_ERR_ARG; }
+    if (method < 0 || method > 1) { set_error("batched BFS method %d: expected 0 or 1", method); return GS_ERR_ARG; }
+    g->batch_method = method;
     return GS_OK;
 }
 

@@ -132,6 +132,7 @@ struct gs_ws {
     gs::DevBuf<int32_t> kc_deg;
     gs::DevBuf<uint64_t> ms_vis, ms_front, ms_next, ms_roots;  // batched BFS lane masks
     gs::DevBuf<uint8_t> ms_depth;                             // lane-major depth, lanes x N
+    gs::DevBuf<uint8_t> ms_depth_vm;                          // vertex-major depth rows (index method), N x 64
     gs::DevBuf<uint32_t> queue;                               // frontier vertices (direction-optimizing BFS)
     gs::DevBuf<int32_t> qdeg;                                 // their degrees
     gs::DevBuf<uint64_t> qoff;                                // exclusive prefix of qdeg (top-down levels)
@@ -200,6 +201,7 @@ struct gs_graph {
     int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy
     int wcc_method = 1;                      // 0 tile hooking passes, 1 sampled linking over the lists
     int kcore_method = 1;                    // 0 tile scans, 1 peeled lists pushed over the index
+    int batch_method = 1;                    // batched BFS: 0 tile sweep, 1 direction-optimizing over the index
     // source-blocked copy of the lists (built on demand), gs_pb.hip
     bool pb_ready = false;
     uint32_t pb_k = 0;                       // blocks; 0 = not available (PageRank pulls)
@@ -249,7 +251,8 @@ int unpermute_i32(const gs_graph *g, hipStream_t st, const int32_t *x_r, int32_t
 int ensure_queue(const gs_graph *g, gs_ws *w);  // queue, qdeg, qoff, qsum sized for N
 int queue_prefix(gs_ws *w, uint64_t nq, std::vector<unsigned long long> &scratch, uint64_t *total);
 int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
-                  uint64_t *level_counts, int max_levels, int *levels);
+                  uint64_t *level_counts, int max_levels, int *levels,
+                  bool keep_depth);
 int run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp);
 int run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
               int *rounds);
@@ -400,6 +403,51 @@ __device__ inline void sweep_chunk(const SweepArgs &a, uint64_t chunk, const Chu
             if (SKIP_INACTIVE && !s.act[k]) continue;
             op(w[i], s.rc[k]);
         }
+    }
+}
+
+// As sweep_chunk, in two phases per quad: pre(word, rc) for all four edges
+// first -- its loads (e.g. the frontier state of both endpoints) issue
+// together -- then op(word, rc, pre_result) for the edges to visit.  pre also
+// runs on skipped or out-of-range edges with word 0 and a valid tile, so it
+// must only read.
+template <bool SKIP_INACTIVE, class Pre, class Op>
+__device__ inline void sweep_chunk_pre(const SweepArgs &a, uint64_t chunk, const ChunkTiles &s, uint32_t len,
+                                       Pre &pre, Op &op) {
+    const uint32_t *g = a.grid + (chunk - a.grid_chunk0) * (uint64_t)a.chunk;
+    const int quads = (int)(a.chunk / (4 * SWEEP_WG));
+#pragma unroll 1
+    for (int q = 0; q < quads; ++q) {
+        uint32_t off = (uint32_t)(q * SWEEP_WG + threadIdx.x) * 4u;
+        if (off >= len) break;
+        int k = find_tile(s, off);
+        if (SKIP_INACTIVE) {
+            if (!s.act[k] && s.beg[k + 1] >= off + 4) continue;
+        }
+        uint32_t w[4];
+        if (off + 4 <= len) {
+            uint4 v = *reinterpret_cast<const uint4 *>(g + off);
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+            for (int i = 0; i < 4; ++i) w[i] = (off + i < len) ? g[off + i] : 0u;
+        }
+        uint32_t rc[4];
+        bool ok[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t e = off + i;
+            ok[i] = e < len;
+            while (ok[i] && s.beg[k + 1] <= e) ++k;
+            rc[i] = s.rc[k];
+            if (SKIP_INACTIVE && !s.act[k]) ok[i] = false;
+            if (!ok[i]) w[i] = 0u;
+        }
+        decltype(pre(0u, 0u)) st[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) st[i] = pre(w[i], rc[i]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (ok[i]) op(w[i], rc[i], st[i]);
     }
 }
 

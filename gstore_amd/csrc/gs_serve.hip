@@ -67,11 +67,21 @@ struct Worker {
 
 // Workspace bytes a query of `kind` needs on graph g: exactly what run_*
 // allocates (plus the 128-byte counter block every workspace has).
-static uint64_t need_bytes(const gs_graph *g, int kind, int lanes) {
+// BFS on an undirected resident grid runs over the neighbour-list index (lane
+// masks, frontier queue and its prefix; depths only when a query asks for
+// them); otherwise over the tiles (lane masks, lane-major depths, row bits).
+// The index itself belongs to the graph and is not counted here.
+static bool bfs_by_index(const gs_graph *g) { return g->batch_method == 1 && !g->directed && !g->ss; }
+
+static uint64_t need_bytes(const gs_graph *g, int kind, int lanes, bool keep_depth) {
     const uint64_t n = g->n, rowact = (g->p_s + 31) / 32 * 4, base = 16 * 8;
     switch (kind) {
-        case GS_Q_BFS: return base + 24 * n + (uint64_t)lanes * n + rowact + 64 * 8 * 2;
-        case GS_Q_PPR: return base + 16 * n;
+        case GS_Q_BFS:
+            if (bfs_by_index(g))
+                return base + 24 * n + 64 * 8 * 2 + 8 * n + 8 * (n + 1) + 8 * (n / 1024 + 1) +
+                       (keep_depth ? (uint64_t)lanes * n + 64 * n : 0);
+            return base + 24 * n + (uint64_t)lanes * n + rowact + 64 * 8 * 2;
+        case GS_Q_PPR: return base + 24 * n;  // prior, acc, factor, rank (f32) and the FP64 sums
         case GS_Q_PAGERANK: return base + 12 * n;
         case GS_Q_WCC: return base + 4 * n;
         default: return base;
@@ -86,8 +96,11 @@ static uint64_t kind_bytes(const gs_ws *w, int kind) {
     switch (kind) {
         case GS_Q_BFS:
             return b + w->ms_vis.bytes() + w->ms_front.bytes() + w->ms_next.bytes() + w->ms_depth.bytes() +
-                   w->rowact.bytes() + w->ms_roots.bytes() + w->ms_counts.bytes();
-        case GS_Q_PPR: return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes() + w->ppr_rank.bytes();
+                   w->ms_depth_vm.bytes() + w->rowact.bytes() + w->ms_roots.bytes() + w->ms_counts.bytes() +
+                   w->queue.bytes() + w->qdeg.bytes() + w->qoff.bytes() + w->qsum.bytes();
+        case GS_Q_PPR:
+            return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes() + w->ppr_rank.bytes() +
+                   w->ppr_acc.bytes();
         case GS_Q_PAGERANK: return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes();
         case GS_Q_WCC: return b + w->wcc_parent.bytes();
         default: return b;
@@ -98,12 +111,16 @@ static uint64_t kind_bytes(const gs_ws *w, int kind) {
 static void trim(gs_ws *w, int kind) {
     if (kind != GS_Q_BFS) {
         w->ms_vis.release(); w->ms_front.release(); w->ms_next.release(); w->ms_depth.release();
-        w->rowact.release(); w->ms_roots.release(); w->ms_counts.release();
+        w->ms_depth_vm.release(); w->rowact.release(); w->ms_roots.release(); w->ms_counts.release();
+        w->queue.release(); w->qdeg.release(); w->qoff.release(); w->qsum.release();
     }
     if (kind != GS_Q_PPR && kind != GS_Q_PAGERANK) {
         w->pr_prior.release(); w->pr_acc.release(); w->pr_fac.release();
     }
-    if (kind != GS_Q_PPR) w->ppr_rank.release();
+    if (kind != GS_Q_PPR) {
+        w->ppr_rank.release();
+        w->ppr_acc.release();
+    }
     if (kind != GS_Q_WCC) w->wcc_parent.release();
     w->depth.release(); w->bm_vis.release(); w->bm_front.release(); w->bm_next.release(); w->kc_deg.release();
 }
@@ -146,17 +163,28 @@ static void worker_main(Worker *wk) {
                     const Query &q = s->pending.front();
                     gs_graph *g = s->graphs[q.graph];
                     int lanes = 1;
+                    bool keep = false;  // some query of the batch wants its depths
                     if (q.kind == GS_Q_BFS) {
                         lanes = 0;
                         for (auto &p : s->pending)
-                            if (p.kind == GS_Q_BFS && p.graph == q.graph && lanes < s->max_batch) ++lanes;
-                        while (lanes > 1 && need_bytes(g, GS_Q_BFS, lanes) > s->budget) --lanes;
+                            if (p.kind == GS_Q_BFS && p.graph == q.graph && lanes < s->max_batch) {
+                                ++lanes;
+                                keep = keep || p.out;
+                            }
+                        while (lanes > 1 && need_bytes(g, GS_Q_BFS, lanes, keep) > s->budget) --lanes;
+                        keep = false;
+                        int k = 0;
+                        for (auto &p : s->pending)
+                            if (p.kind == GS_Q_BFS && p.graph == q.graph && k < lanes) {
+                                ++k;
+                                keep = keep || p.out;
+                            }
                     }
                     auto it = wk->ws.find(q.graph);
                     const gs_ws *cur = it != wk->ws.end() ? it->second.get() : nullptr;
                     have_all = cur ? ws_bytes(cur) : 0;
                     uint64_t have = cur ? kind_bytes(cur, q.kind) : 0;
-                    uint64_t want = need_bytes(g, q.kind, lanes);
+                    uint64_t want = need_bytes(g, q.kind, lanes, keep);
                     need = want > have ? want - have : 0;
                     if (s->held + need > s->budget) make_room(s, wk, need);
                     if (s->held + need <= s->budget) {
@@ -214,7 +242,9 @@ static void worker_main(Worker *wk) {
                     for (int i = 0; i < nr; ++i) roots[i] = batch[i].arg;
                     counts.assign((size_t)nr * 256, 0);
                     int levels = 0;
-                    status = run_bfs_batch(g, w, roots.data(), nr, nullptr, counts.data(), 256, &levels);
+                    bool keep = false;
+                    for (int i = 0; i < nr; ++i) keep = keep || batch[i].out;
+                    status = run_bfs_batch(g, w, roots.data(), nr, nullptr, counts.data(), 256, &levels, keep);
                     for (int i = 0; i < nr && status == GS_OK; ++i) {
                         uint64_t reached = 1;
                         for (int l = 0; l < levels && l < 256; ++l) reached += counts[(size_t)i * 256 + l];
@@ -322,7 +352,7 @@ int gs_server_submit(gs_server *s, int graph, int kind, uint64_t arg, void *out,
         set_error("query source %llu >= N", (unsigned long long)arg);
         return GS_ERR_ARG;
     }
-    if (need_bytes(g, kind, 1) > s->budget) {
+    if (need_bytes(g, kind, 1, out != nullptr) > s->budget) {
         set_error("query needs more workspace memory than the server budget");
         return GS_ERR_NOMEM;
     }

@@ -239,6 +239,13 @@ int gs_set_pagerank_method(gs_graph *g, int method);
  * wcc.cpp:324-337 count).  Directed and semi-external grids use 0. */
 int gs_set_wcc_method(gs_graph *g, int method);
 
+/* Batched BFS method on undirected resident grids: 1 (default) runs the roots
+ * as bit lanes over the neighbour-list index, direction-optimizing on the union
+ * frontier (top-down pushes of lane masks, bottom-up ORs of the neighbours'
+ * masks, heaviest neighbours first); 0 sweeps the tiles with lane masks.
+ * Depths and level counts are identical.  Directed and semi-external grids use 0. */
+int gs_set_batch_bfs_method(gs_graph *g, int method);
+
 /* k-core method on undirected resident grids: 1 (default) lists the vertices
  * each peel removes and pushes one decrement per entry of their neighbour
  * lists (the index), the heaviest vertices' decrements gathered per

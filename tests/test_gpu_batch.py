@@ -82,3 +82,33 @@ def test_batch_errors(kron14):
         g.bfs_batch(list(range(65)))
     with pytest.raises(ga.GStoreError):
         g.bfs_batch([n])
+
+
+@pytest.mark.parametrize("nr", [1, 64])
+def test_batch_methods_identical(kron14, nr):
+    """Lanes over the neighbour-list index and over the tile sweep give the same
+    depths and level counts."""
+    n, (start, grid, deg), g = kron14
+    rng = np.random.default_rng(100 + nr)
+    roots = rng.choice(np.flatnonzero(deg), nr, replace=False)
+    got = []
+    for method in (0, 1):
+        g.set_batch_bfs_method(method)
+        got.append(g.bfs_batch(roots))
+    g.set_batch_bfs_method(1)
+    assert np.array_equal(got[0][0], got[1][0]) and got[0][1] == got[1][1]
+    with pytest.raises(ga.GStoreError):
+        g.set_batch_bfs_method(2)
+
+
+def test_batch_index_scale18():
+    """A graph with several index blocks and both traversal directions."""
+    n, e = graphs.kron(18)
+    start, grid, deg = oracle.convert(e, n)
+    with ga.Graph.from_arrays(n, start, grid, deg) as g:
+        rng = np.random.default_rng(18)
+        roots = rng.choice(np.flatnonzero(deg), 64, replace=False)
+        depth, counts = g.bfs_batch(roots)
+        for i in (0, 17, 63):
+            d, lc = oracle.bfs(n, start, grid, int(roots[i]))
+            assert np.array_equal(depth[i], d) and counts[i][:len(lc)] == lc

@@ -88,7 +88,9 @@ def test_mix_equals_alone(fleet):
 
 def test_budget_admission(fleet):
     name, n, arrays, g = fleet[1]
-    need_bfs_one = 24 * n + n + (1 << 20)
+    # one BFS with its depths over the index: lane masks 24n, queue + degrees 8n,
+    # prefix 8(n+1) + block sums, depth rows 64n, lane-major depth n, small blocks
+    need_bfs_one = 24 * n + 8 * n + 8 * (n + 1) + 8 * (n // 1024 + 1) + 64 * n + n + (1 << 20)
     budget = 2 * need_bfs_one         # room for about two small BFS workspaces at a time
     with ga.Server(hbm_budget=budget, workers=4) as srv:
         gid = srv.add_graph(g)
