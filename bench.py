@@ -122,31 +122,50 @@ def serve_mode(args):
            "higher_is_better": True, "data": "synthetic (Graph500 Kronecker, seeds 1,2)",
            "config": {"workload": "serving", "scales": scales, "workers": args.workers}}
 
-    # 1. BFS saturation: the same K roots, one root per launch vs 64-lane batches
-    big = len(graphs) - 1
+    # 1. BFS saturation per graph: the same K roots through the server, one root
+    #    per launch (max_batch 1) vs 64-lane batches; beside them the plain
+    #    single-root tile BFS (the reference's method, one root per launch) and the
+    #    direction-optimizing single-root BFS, timed directly on the device.
     K = args.sat_queries
-    roots = rng.integers(0, graphs[big].n, K)
-    sat = {}
-    for label, max_batch, workers in (("one_root_per_launch", 1, 1), ("batched_64", 64, args.workers)):
-        with ga.Server(workers=workers) as srv:
-            gid = srv.add_graph(graphs[big])
-            srv.configure(max_batch=max_batch)
-            srv.submit(gid, ga.Q_BFS, int(roots[0]))     # warm the workspace
-            srv.drain()
-            srv.poll()
-            t = time.perf_counter()
-            for r in roots:
-                srv.submit(gid, ga.Q_BFS, int(r))
-            srv.drain()
-            dt = time.perf_counter() - t
-            res = srv.poll(K + 10)
-            lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
-            sat[label] = {"queries": K, "seconds": round(dt, 3), "qps": round(K / dt, 2),
-                          "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
-                          "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 2)}
-            log(f"serve: BFS saturation {label}: {K / dt:.1f} q/s")
-    sat["speedup"] = round(sat["batched_64"]["qps"] / sat["one_root_per_launch"]["qps"], 2)
-    out["bfs_saturation"] = {"graph_scale": scales[big], **sat}
+    sat_all = []
+    for gi, g in enumerate(graphs):
+        roots = rng.choice(nz[gi], K, replace=True)
+        sat = {"graph_scale": scales[gi], "queries": K}
+        for label, max_batch, workers in (("server_one_root_per_launch", 1, 1), ("server_batched_64", 64, args.workers)):
+            with ga.Server(workers=workers) as srv:
+                gid = srv.add_graph(g)
+                srv.configure(max_batch=max_batch)
+                srv.submit(gid, ga.Q_BFS, int(roots[0]))     # warm the workspace
+                srv.drain()
+                srv.poll()
+                t = time.perf_counter()
+                for r in roots:
+                    srv.submit(gid, ga.Q_BFS, int(r))
+                srv.drain()
+                dt = time.perf_counter() - t
+                res = srv.poll(K + 10)
+                lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
+                sat[label] = {"qps": round(K / dt, 2), "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                              "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 2)}
+        nd = min(K, 16)
+        for label, mode in (("tile_single_root", 0), ("do_single_root", 1)):
+            g.set_bfs_mode(mode)
+            g.bfs(int(roots[0]), copy_out=False)
+            ms = 0.0
+            for r in roots[:nd]:
+                g.bfs(int(r), copy_out=False)
+                ms += g.stats()["ms"]
+            sat[label] = {"qps": round(nd / (ms / 1e3), 2), "ms_per_root": round(ms / nd, 3)}
+        g.set_bfs_mode(0)
+        b = sat["server_batched_64"]["qps"]
+        sat["batched_vs_tile_single"] = round(b / sat["tile_single_root"]["qps"], 2)
+        sat["batched_vs_do_single"] = round(b / sat["do_single_root"]["qps"], 2)
+        sat["batched_vs_server_single"] = round(b / sat["server_one_root_per_launch"]["qps"], 2)
+        sat_all.append(sat)
+        log(f"serve: Kron-{scales[gi]} batched {b:.1f} q/s; x{sat['batched_vs_tile_single']} vs tile single root, "
+            f"x{sat['batched_vs_do_single']} vs direction-optimizing single root")
+    out["bfs_saturation"] = sat_all
+    big = sat_all[-1]
 
     # 2. Poisson mix
     rates = [float(x) for x in args.serve_rates.split(",")]
@@ -186,7 +205,8 @@ def serve_mode(args):
                     "bfs_mean_batch": round(float(np.mean([r["batch_size"] for r in res if r["kind"] == 0])), 2)})
         log(f"serve: rate {lam}/s -> {n_q / dt:.1f} q/s, p99 {mix[-1]['p99_ms']} ms")
     out["poisson_mix"] = mix
-    out["value"] = sat["batched_64"]["qps"]
+    out["value"] = big["server_batched_64"]["qps"]
+    out["vs_one_root_per_launch"] = big["batched_vs_tile_single"]
     print(json.dumps(out), flush=True)
     for g in graphs:
         g.close()
@@ -262,9 +282,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--mode", choices=["main", "serve", "stream", "copy"], default="main")
     ap.add_argument("--serve-scales", default="18,20,22,24,26,28")
-    ap.add_argument("--serve-rates", default="20,80,320")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--sat-queries", type=int, default=512)
+    ap.add_argument("--serve-rates", default="5,20,80")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--sat-queries", type=int, default=256)
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--budget-gb", type=int, default=16)
     ap.add_argument("--segment-mb", type=int, default=512)

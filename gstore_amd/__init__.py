@@ -61,6 +61,12 @@ class _Stats(ctypes.Structure):
                 ("kern_calls", ctypes.c_uint64 * len(KERNELS))]
 
 
+class _LevelStat(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_uint32), ("direction", ctypes.c_int32), ("frontier", ctypes.c_uint64),
+                ("edges", ctypes.c_uint64), ("tiles", ctypes.c_uint64), ("reached", ctypes.c_uint64),
+                ("ms", ctypes.c_double)]
+
+
 class _Tune(ctypes.Structure):
     _fields_ = [("chunk", ctypes.c_uint32), ("xcd_order", ctypes.c_int32), ("from_cache", ctypes.c_int32),
                 ("ncand", ctypes.c_int32), ("best_ms", ctypes.c_double), ("cand_chunk", ctypes.c_uint32 * 6),
@@ -102,6 +108,8 @@ _SIGS = {
     "gs_kcore": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_uint64),
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
     "gs_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(_Stats)]),
+    "gs_get_level_stats": (ctypes.c_int, [_P, ctypes.POINTER(_LevelStat), ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int)]),
     "gs_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_sweep_order": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_chunk": (ctypes.c_int, [_P, ctypes.c_uint32]),
@@ -285,6 +293,17 @@ class Graph:
     def set_wcc_method(self, method: int) -> None:
         """1 (default): sampled linking over the neighbour-list index; 0: hook passes over the tiles."""
         _check(_lib.gs_set_wcc_method(self._h, int(method)))
+
+    def level_stats(self) -> list:
+        """Per-level record of the last BFS: direction (tile / top-down / bottom-up),
+        frontier, edges, tiles, reached, ms."""
+        buf = (_LevelStat * 256)()
+        n = ctypes.c_int(0)
+        _check(_lib.gs_get_level_stats(self._h, buf, 256, ctypes.byref(n)))
+        names = {0: "tile", 1: "top-down", 2: "bottom-up"}
+        return [{"level": int(b.level), "direction": names.get(int(b.direction), "?"), "frontier": int(b.frontier),
+                 "edges": int(b.edges), "tiles": int(b.tiles), "reached": int(b.reached), "ms": float(b.ms)}
+                for b in buf[:min(n.value, 256)]]
 
     def set_batch_bfs_method(self, method: int) -> None:
         """1 (default): lanes over the neighbour-list index; 0: lanes over the tile sweep."""

@@ -678,8 +678,14 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     int nlev = 0, status = GS_OK;
     uint64_t checked = 0;
     std::vector<unsigned long long> hsum;
+    w->levels.clear();
+    hipEvent_t la, lb;
+    GS_HIP(hipEventCreate(&la));
+    GS_HIP(hipEventCreate(&lb));
     for (uint32_t level = 1;; ++level) {
         const uint64_t m_f_prev = m_f;  // edges a top-down level examines
+        const uint64_t n_f_prev = n_f;
+        GS_HIP(hipEventRecord(la, w->stream));
         if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha) bottom_up = true;
         else if (bottom_up && n_f < g->n / (uint64_t)g->do_beta) bottom_up = false;
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 32, w->stream));
@@ -716,11 +722,18 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
                 w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
+        GS_HIP(hipEventRecord(lb, w->stream));
         GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 32, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
         n_f = w->h_counters[0];
         m_f = w->h_counters[1];
         checked += bottom_up ? w->h_counters[3] : m_f_prev;
+        {
+            float lms = 0.f;
+            GS_HIP(hipEventElapsedTime(&lms, la, lb));
+            w->levels.push_back(gs_level_stat{level, bottom_up ? 2 : 1, n_f_prev,
+                                              bottom_up ? w->h_counters[3] : m_f_prev, 0, n_f, lms});
+        }
         m_u -= std::min(m_u, m_f);
         if (level_counts && nlev < max_levels) level_counts[nlev] = n_f;
         ++nlev;
@@ -731,6 +744,8 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
             break;
         }
     }
+    (void)hipEventDestroy(la);
+    (void)hipEventDestroy(lb);
     k_unpermute<uint8_t><<<vgrid(g->n), 256, 0, w->stream>>>(w->depth_r.ptr, g->new_id.ptr, g->n, w->depth.ptr);
     GS_HIP(hipGetLastError());
     GS_HIP(hipEventRecord(t1, w->stream));

@@ -125,7 +125,13 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     k_bfs_root<<<1, 1, 0, w->stream>>>((uint32_t)root, w->depth.ptr, w->bm_vis.ptr, w->bm_front.ptr, w->rowact.ptr);
     GS_HIP(hipGetLastError());
     int nlev = 0, status = GS_OK;
+    w->levels.clear();
+    hipEvent_t la, lb;
+    GS_HIP(hipEventCreate(&la));
+    GS_HIP(hipEventCreate(&lb));
+    uint64_t prev_e = 0, prev_t = 0, front_n = 1;
     for (uint32_t level = 1;; ++level) {
+        GS_HIP(hipEventRecord(la, w->stream));
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
         hipEvent_t kb;
         GS_TRY(sweep(g, w, GS_K_BFS_EXPAND, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
@@ -142,9 +148,19 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
                                                                    w->counters.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
+        GS_HIP(hipEventRecord(lb, w->stream));
         GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 24, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
         uint64_t cnt = w->h_counters[0];
+        {
+            float lms = 0.f;
+            GS_HIP(hipEventElapsedTime(&lms, la, lb));
+            w->levels.push_back(gs_level_stat{level, 0, front_n, w->h_counters[1] - prev_e, w->h_counters[2] - prev_t,
+                                              cnt, lms});
+            prev_e = w->h_counters[1];
+            prev_t = w->h_counters[2];
+            front_n = cnt;
+        }
         if (level_counts && nlev < max_levels) level_counts[nlev] = cnt;
         ++nlev;
         if (cnt == 0) break;
@@ -154,6 +170,8 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
             break;
         }
     }
+    (void)hipEventDestroy(la);
+    (void)hipEventDestroy(lb);
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;

@@ -775,6 +775,15 @@ int gs_get_stats(gs_graph *g, gs_stats *st) {
     return GS_OK;
 }
 
+int gs_get_level_stats(gs_graph *g, gs_level_stat *out, int max, int *n) {
+    if (!g || (max > 0 && !out) || max < 0) { set_error("bad arguments"); return GS_ERR_ARG; }
+    const auto &lv = g->ws->levels;
+    const int cnt = (int)lv.size();
+    for (int i = 0; i < cnt && i < max; ++i) out[i] = lv[i];
+    if (n) *n = cnt;
+    return GS_OK;
+}
+
 int gs_set_chunk(gs_graph *g, uint32_t edges) {
     if (!g || edges < 4 * SWEEP_WG || edges > (uint32_t)CHUNK || edges % (4 * SWEEP_WG)) {
         set_error("chunk must be a multiple of %d in [%d, %d]", 4 * SWEEP_WG, 4 * SWEEP_WG, CHUNK);

@@ -143,6 +143,7 @@ struct gs_ws {
     unsigned long long *h_counters = nullptr;  // pinned mirror of counters
     gs::KernelTimer timer;
     gs_stats stats{};
+    std::vector<gs_level_stat> levels;                        // per-level record of the last BFS
     ~gs_ws();
     size_t device_bytes() const;
 };

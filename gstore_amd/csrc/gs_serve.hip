@@ -324,6 +324,9 @@ int gs_server_create(uint64_t budget, int workers, gs_server **out) {
 int gs_server_add_graph(gs_server *s, gs_graph *g, int *graph_id) {
     if (!s || !g || !graph_id) { set_error("null argument"); return GS_ERR_ARG; }
     if (g->ss) { set_error("the server keeps graphs resident; this one is in semi-external mode"); return GS_ERR_UNSUPPORTED; }
+    // the neighbour-list index serves BFS, PPR and WCC queries on undirected
+    // graphs: build it now rather than in the first query
+    if (!g->directed) GS_TRY(build_adjacency(g));
     std::lock_guard<std::mutex> lk(s->mu);
     s->graphs.push_back(g);
     *graph_id = (int)s->graphs.size() - 1;

@@ -194,7 +194,8 @@ typedef struct gs_query_result {
 typedef struct gs_server gs_server;
 
 int gs_server_create(uint64_t hbm_budget_bytes, int workers, gs_server **out);
-/* The server does not own the graph; keep it open until gs_server_destroy. */
+/* The server does not own the graph; keep it open until gs_server_destroy.
+ * Builds the graph's neighbour-list index (undirected grids) before returning. */
 int gs_server_add_graph(gs_server *s, gs_graph *g, int *graph_id);
 /* max_batch 1..64 BFS lanes; iterations of PPR and PageRank queries (default 20). */
 int gs_server_set(gs_server *s, int max_batch, int ppr_iters, int pr_iters);
@@ -299,6 +300,22 @@ int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out);
 
 /* Stats of the last algorithm call on this handle. */
 int gs_get_stats(gs_graph *g, gs_stats *st);
+
+/* Per-level record of the last BFS call on the graph's default workspace. */
+typedef struct gs_level_stat {
+    uint32_t level;       /* 1 = expansion of the root's level */
+    int32_t direction;    /* 0 tile sweep, 1 top-down, 2 bottom-up */
+    uint64_t frontier;    /* vertices expanded at this level */
+    uint64_t edges;       /* tile sweep: edges in scanned tiles; top-down: frontier list entries;
+                             bottom-up: list entries checked */
+    uint64_t tiles;       /* tile sweep: non-empty tiles scanned (0 otherwise) */
+    uint64_t reached;     /* vertices newly reached */
+    double ms;            /* device time of the level */
+} gs_level_stat;
+
+/* Copies up to max records of the last BFS into out; *n = levels recorded.
+ * No counterpart in the reference (bfs.cpp prints only the per-level count). */
+int gs_get_level_stats(gs_graph *g, gs_level_stat *out, int max, int *n);
 
 /* 1: record per-kernel-class device time with HIP events (adds a little
  * launch overhead); 0: off (default). */

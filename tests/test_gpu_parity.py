@@ -422,3 +422,29 @@ def test_kcore_methods_identical(loaded, name, k):
     g.set_kcore_method(1)
     with pytest.raises(ga.GStoreError):
         g.set_kcore_method(2)
+
+
+@pytest.mark.parametrize("name", ["kron14", "tiles"])
+def test_bfs_level_stats(loaded, name):
+    """Per-level records: one per level, reached = the level counts, the first
+    frontier is the root; tile levels add up to the scan totals."""
+    n, (start, grid, deg), g = loaded[name]
+    root = int(np.argmax(deg))
+    want = oracle.bfs(n, start, grid, root)[1]
+    for mode in (0, 1):
+        g.set_bfs_mode(mode)
+        res = g.bfs(root)
+        st = g.stats()
+        lv = g.level_stats()
+        _say("levels", name, mode, [(x["direction"], x["frontier"], x["edges"]) for x in lv])
+        assert len(lv) == st["iterations"] == len(want)
+        assert [x["reached"] for x in lv] == want == res.level_counts
+        assert lv[0]["frontier"] == 1 and all(x["ms"] >= 0 for x in lv)
+        assert [x["frontier"] for x in lv[1:]] == want[:-1]
+        if mode == 0:
+            assert sum(x["edges"] for x in lv) == st["edges_scanned"]
+            assert sum(x["tiles"] for x in lv) == st["tiles_scanned"]
+            assert all(x["direction"] == "tile" for x in lv)
+        else:
+            assert all(x["direction"] in ("top-down", "bottom-up") for x in lv)
+    g.set_bfs_mode(0)
