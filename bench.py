@@ -340,11 +340,14 @@ def main():
     m_reach, ref_scan = {}, {}
     deg64 = deg.astype(np.int64)
     g.set_bfs_mode(0)
+    ref_levels = None
     for r in roots:
         d = g.bfs(r).depth
         st = g.stats()
         m_reach[r] = int(deg64[d > 0].sum() // 2)
         ref_scan[r] = (st["edges_scanned"], st["tiles_scanned"], st["iterations"], int(np.count_nonzero(d)))
+        if ref_levels is None:
+            ref_levels = g.level_stats()
     log("BFS M_reach and reference scan counts per root computed")
     ref_kc = None
     if "kcore" in set(args.algos.split(",")):
@@ -381,6 +384,7 @@ def main():
     # per algorithm: device ms, traversed edges (GTEPS convention), algorithmic bytes
     per = {k: [0.0, 0.0, 0.0] for k in ("pagerank", "bfs", "wcc", "kcore")}
     bfs_scanned = [0.0]
+    first_root_levels = []
 
     def absorb(st):
         for k, v in st["kernels"].items():
@@ -400,6 +404,8 @@ def main():
         for r in (roots if "bfs" in algos else []):
             g.bfs(r, copy_out=False)
             st = g.stats()
+            if record and r == roots[0] and not first_root_levels:
+                first_root_levels.extend(g.level_stats())
             if record:
                 absorb(st)
                 e_scan, t_scan, levels, reached = ref_scan[r]
@@ -456,6 +462,17 @@ def main():
             d.update(ms_per_root=ms / args.steps / len(roots), roots=len(roots),
                      mode=["tile sweep", "direction-optimizing"][max(bfs_mode, 0)],
                      gteps_scanned=rate(bfs_scanned[0], ms))
+            # per level of the first root: this run's device time against the fixed
+            # per-level bytes (scans of the reference's tile-skip rule, from the pre-pass)
+            lv = []
+            for x, ref in zip(first_root_levels, ref_levels or []):
+                b = 4 * ref["edges"] + 8 * ref["tiles"] + 3 * n / 8 + ref["reached"]
+                gbs = b / (x["ms"] / 1e3) / 1e9 if x["ms"] > 0 else None
+                lv.append({"level": x["level"], "direction": x["direction"], "frontier": x["frontier"],
+                           "edges_examined": x["edges"], "reached": x["reached"], "ms": round(x["ms"], 4),
+                           "alg_bytes": int(b), "alg_gbs": round(gbs, 1) if gbs else None,
+                           "hbm_frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None})
+            d["levels_first_root"] = lv
         if algo == "kcore":
             d.update(k=args.k)
         per_algo[algo] = {k2: round(v, 4) if isinstance(v, float) else v for k2, v in d.items()}
