@@ -649,6 +649,8 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     if (w->depth_r.n != g->n) GS_TRY(w->depth_r.alloc(g->n));
     GS_TRY(ensure_queue(g, w));
     const uint64_t words = bitmap_words(g);
+    // bitmap words that hold vertices with edges: only they change after the root
+    const uint64_t act_words = (g->nzn + 31) / 32;
     w->timer.reset();
     w->stats = gs_stats{};
     hipEvent_t t0, t1;
@@ -692,14 +694,16 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
         hipEvent_t kb;
         const int kclass = bottom_up ? GS_K_BFS_BOTTOMUP : GS_K_BFS_TOPDOWN;
         w->timer.begin(kclass, w->stream, &kb);
-        if (bottom_up) {
-            k_bu_expand<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, g->adj_off.ptr, g->adj_nbr.ptr, w->bm_vis.ptr,
-                                                            w->bm_front.ptr, w->bm_next.ptr, w->counters.ptr + 3);
+        if (bottom_up) {  // vertices without edges (relabelled ids from nzn on) can never be reached
+            if (g->nzn)
+                k_bu_expand<<<vgrid(g->nzn), 256, 0, w->stream>>>(g->nzn, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                                  w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
+                                                                  w->counters.ptr + 3);
         } else if (n_f && m_f) {
             if (!have_queue) {  // the frontier came from a bottom-up level: list it
                 GS_HIP(hipMemsetAsync(w->counters.ptr + 4, 0, 24, w->stream));
-                k_do_level_end<false, true><<<vgrid(words), 256, 0, w->stream>>>(
-                    w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, 0, g->deg_r.ptr, g->n, words,
+                k_do_level_end<false, true><<<vgrid(act_words), 256, 0, w->stream>>>(
+                    w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, 0, g->deg_r.ptr, g->n, act_words,
                     w->queue.ptr, w->qdeg.ptr, w->counters.ptr + 4);
                 GS_HIP(hipGetLastError());
             }
@@ -712,14 +716,14 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
         w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
         // after a top-down level the next one is likely top-down too: list the new frontier now
         have_queue = !bottom_up;
-        if (have_queue)
-            k_do_level_end<true, true><<<vgrid(words), 256, 0, w->stream>>>(
-                w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n, words,
-                w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
-        else
-            k_do_level_end<true, false><<<vgrid(words), 256, 0, w->stream>>>(
-                w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n, words,
-                w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
+        if (act_words && have_queue)
+            k_do_level_end<true, true><<<vgrid(act_words), 256, 0, w->stream>>>(
+                w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n,
+                act_words, w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
+        else if (act_words)
+            k_do_level_end<true, false><<<vgrid(act_words), 256, 0, w->stream>>>(
+                w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n,
+                act_words, w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
         GS_HIP(hipEventRecord(lb, w->stream));

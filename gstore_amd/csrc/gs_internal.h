@@ -245,7 +245,7 @@ int run_bfs_do(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, u
 int build_adjacency(gs_graph *g);
 int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double *acc64);
 int build_blocked(gs_graph *g);  // the blocked copy (after the lists); pb_k = 0 when too many blocks
-int pr_blocked(const gs_graph *g, gs_ws *w, const float *prior, float *acc);
+int pr_blocked(const gs_graph *g, gs_ws *w, float *prior, const float *fac);
 size_t pb_bytes(const gs_graph *g);
 int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out);
 int unpermute_i32(const gs_graph *g, hipStream_t st, const int32_t *x_r, int32_t *out);
@@ -450,6 +450,14 @@ __device__ inline void sweep_chunk_pre(const SweepArgs &a, uint64_t chunk, const
         for (int i = 0; i < 4; ++i)
             if (ok[i]) op(w[i], rc[i], st[i]);
     }
+}
+
+// PageRank: new prior of a vertex from its sum (pr.cpp:276-290): r = 0.15 +
+// 0.85 * acc in FP64 with round-to-nearest steps (no contraction), times the
+// degree factor, rounded to FP32 once.
+__device__ inline float pr_new_prior(float acc, float fac) {
+    const double r = __dadd_rn(0.15, __dmul_rn(0.85, (double)acc));
+    return __double2float_rn(__dmul_rn(r, (double)fac));
 }
 
 // Global ids of an edge word in tile rc: row << 16 | v0_local, col << 16 | v1_local.

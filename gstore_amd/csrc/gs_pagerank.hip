@@ -132,8 +132,7 @@ __global__ void k_pr_finalize(uint64_t n, float *__restrict__ prior, float *__re
                               const float *__restrict__ fac, int last) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         float f = fac[v];
-        double r = __dadd_rn(0.15, __dmul_rn(0.85, (double)acc[v]));
-        float x = __double2float_rn(__dmul_rn(r, (double)unmark(f)));
+        float x = pr_new_prior(acc[v], unmark(f));
         prior[v] = last ? x : mark(x, is_heavy(f));
         acc[v] = 0.0f;
     }
@@ -236,10 +235,7 @@ static int scatter_t(const gs_graph *g, gs_ws *w, A *acc) {
 
 // acc64 non-null: accumulate in FP64 (personalised PageRank).
 static int scatter(const gs_graph *g, gs_ws *w, double *acc64 = nullptr) {
-    if (use_pull(g)) {
-        if (!acc64 && g->pr_method == 2 && g->pb_k) return pr_blocked(g, w, w->pr_prior.ptr, w->pr_acc.ptr);
-        return pr_pull(g, w, w->pr_prior.ptr, w->pr_acc.ptr, acc64);
-    }
+    if (use_pull(g)) return pr_pull(g, w, w->pr_prior.ptr, w->pr_acc.ptr, acc64);
     if (acc64) return scatter_t<double>(g, w, acc64);
     return scatter_t<float>(g, w, w->pr_acc.ptr);
 }
@@ -275,12 +271,26 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
                                                   (float)(1.0 / (double)g->n), iters ? g->heavy_dthr : 0xFFFFFFFFu,
                                                   w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
     GS_HIP(hipGetLastError());
+    // the blocked passes write the new priors of the vertices with edges themselves
+    const bool blocked = pull && g->pr_method == 2 && g->pb_k;
     for (int it = 0; it < iters; ++it) {
+        if (blocked) {
+            GS_TRY(pr_blocked(g, w, w->pr_prior.ptr, w->pr_fac.ptr));
+            continue;
+        }
         hipEvent_t kb;
         GS_TRY(scatter(g, w));
         w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
         k_pr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr,
                                                           it + 1 == iters);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
+    }
+    if (blocked && iters > 0 && g->nzn < g->n) {  // vertices without edges: sum 0
+        hipEvent_t kb;
+        w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
+        k_pr_finalize<<<vgrid(g->n - g->nzn), 256, 0, w->stream>>>(g->n - g->nzn, w->pr_prior.ptr + g->nzn,
+                                                                    w->pr_acc.ptr + g->nzn, w->pr_fac.ptr + g->nzn, 1);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
     }

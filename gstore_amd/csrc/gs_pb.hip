@@ -227,7 +227,8 @@ __device__ inline void lds_add_cas(float *p, float x) {
 __global__ void __launch_bounds__(PB_WG) k_pb_accum(const PbUnit2 *__restrict__ units,
                                                     const uint16_t *__restrict__ dst,
                                                     const float *__restrict__ val, uint64_t nzn,
-                                                    float *__restrict__ acc, float *__restrict__ part) {
+                                                    const float *__restrict__ fac, float *__restrict__ prior,
+                                                    float *__restrict__ part) {
     __shared__ float sum[PB_B];
     const PbUnit2 u = units[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < PB_B; i += PB_WG) sum[i] = 0.0f;
@@ -266,17 +267,18 @@ __global__ void __launch_bounds__(PB_WG) k_pb_accum(const PbUnit2 *__restrict__ 
     __syncthreads();
     const uint64_t qbase = (uint64_t)u.q << PB_SHIFT;
     const uint32_t qn = (uint32_t)std::min<uint64_t>(PB_B, nzn - qbase);
-    if (u.slot == 0xFFFFFFFFu) {
-        for (uint32_t i = threadIdx.x; i < qn; i += PB_WG) acc[qbase + i] = sum[i];
+    if (u.slot == 0xFFFFFFFFu) {  // the block's only unit: its sums are final
+        for (uint32_t i = threadIdx.x; i < qn; i += PB_WG) prior[qbase + i] = pr_new_prior(sum[i], fac[qbase + i]);
     } else {
         float *pp = part + (uint64_t)u.slot * PB_B;
         for (uint32_t i = threadIdx.x; i < qn; i += PB_WG) pp[i] = sum[i];
     }
 }
 
-// acc over a split destination block = sum of its units' partial sums.
+// New priors of a split destination block from the sum of its units' partials.
 __global__ void __launch_bounds__(256) k_pb_reduce(const PbRed *__restrict__ red, const float *__restrict__ part,
-                                                   uint64_t nzn, float *__restrict__ acc) {
+                                                   uint64_t nzn, const float *__restrict__ fac,
+                                                   float *__restrict__ prior) {
     const PbRed r = red[blockIdx.y];
     const uint64_t qbase = (uint64_t)r.q << PB_SHIFT;
     const uint32_t qn = (uint32_t)std::min<uint64_t>(PB_B, nzn - qbase);
@@ -285,7 +287,7 @@ __global__ void __launch_bounds__(256) k_pb_reduce(const PbRed *__restrict__ red
     const float *pp = part + (uint64_t)r.slot0 * PB_B + i;
     float s = 0.0f;
     for (uint32_t k = 0; k < r.nslot; ++k) s += pp[(uint64_t)k * PB_B];
-    acc[qbase + i] = s;
+    prior[qbase + i] = pr_new_prior(s, fac[qbase + i]);
 }
 
 }  // namespace
@@ -498,9 +500,10 @@ int build_blocked(gs_graph *g) {
     return GS_OK;
 }
 
-// One accumulation pass: acc[v] = sum of prior over v's list, for the
-// relabelled ids 0 .. nzn-1 (acc of the others is left as it is).
-int pr_blocked(const gs_graph *g, gs_ws *w, const float *prior, float *acc) {
+// One PageRank iteration for the relabelled ids 0 .. nzn-1: the sums over
+// the lists and, fused into the accumulation, the new priors (the gather has
+// read every prior it needs before the accumulation starts).
+int pr_blocked(const gs_graph *g, gs_ws *w, float *prior, const float *fac) {
     hipEvent_t kb;
     if (g->pb_nu1) {
         w->timer.begin(GS_K_PB_GATHER, w->stream, &kb);
@@ -512,14 +515,14 @@ int pr_blocked(const gs_graph *g, gs_ws *w, const float *prior, float *acc) {
     if (g->pb_nu2) {
         w->timer.begin(GS_K_PB_ACCUM, w->stream, &kb);
         k_pb_accum<<<(unsigned)g->pb_nu2, PB_WG, 0, w->stream>>>(g->pb_u2.ptr, g->pb_dst.ptr, g->pb_val.ptr, g->nzn,
-                                                                 acc, g->pb_part.ptr);
+                                                                 fac, prior, g->pb_part.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PB_ACCUM, w->stream, kb);
     }
     if (g->pb_nred) {
         w->timer.begin(GS_K_PB_REDUCE, w->stream, &kb);
         k_pb_reduce<<<dim3(PB_B / 256, (unsigned)g->pb_nred), 256, 0, w->stream>>>(g->pb_red.ptr, g->pb_part.ptr,
-                                                                                  g->nzn, acc);
+                                                                                  g->nzn, fac, prior);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PB_REDUCE, w->stream, kb);
     }
