@@ -363,7 +363,8 @@ def main():
     tune = None
     if args.tune:
         tune = g.tune()
-        log(f"autotune: chunk {tune['chunk']} xcd_order {tune['xcd_order']} ({tune['best_ms']:.2f} ms per WCC)")
+        log(f"autotune: chunk {tune['chunk']} xcd_order {tune['xcd_order']} ({tune['best_ms']:.2f} ms of tile sweeps), "
+            f"BFS alpha {tune['do_alpha']} beta {tune['do_beta']} ({tune['do_ms']:.2f} ms for two roots)")
     bfs_mode = args.bfs_mode
     if bfs_mode < 0 and "bfs" in algos:
         best = None
@@ -517,7 +518,8 @@ def main():
                    "gen_convert_s": round(gen_s, 2), "parallelism": "single GPU",
                    "index": {"entries": idx["entries"], "build_ms": round(idx["build_ms"], 1),
                              "device_gb": round(idx["device_bytes"] / 1e9, 1)},
-                   "tuned": {"chunk": chunk, "xcd_order": tune["xcd_order"] if tune else 1}},
+                   "tuned": {"chunk": chunk, "xcd_order": tune["xcd_order"] if tune else 1,
+                             "bfs_alpha": tune["do_alpha"] if tune else 14, "bfs_beta": tune["do_beta"] if tune else 24}},
         "per_algo": per_algo, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

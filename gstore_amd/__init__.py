@@ -70,7 +70,8 @@ class _LevelStat(ctypes.Structure):
 class _Tune(ctypes.Structure):
     _fields_ = [("chunk", ctypes.c_uint32), ("xcd_order", ctypes.c_int32), ("from_cache", ctypes.c_int32),
                 ("ncand", ctypes.c_int32), ("best_ms", ctypes.c_double), ("cand_chunk", ctypes.c_uint32 * 6),
-                ("cand_order", ctypes.c_int32 * 6), ("cand_ms", ctypes.c_double * 6)]
+                ("cand_order", ctypes.c_int32 * 6), ("cand_ms", ctypes.c_double * 6),
+                ("do_alpha", ctypes.c_int32), ("do_beta", ctypes.c_int32), ("do_ms", ctypes.c_double)]
 
 
 class QueryResult(ctypes.Structure):
@@ -115,7 +116,8 @@ _SIGS = {
     "gs_set_chunk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "gs_set_pagerank_heavy": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_pagerank_method": (ctypes.c_int, [_P, ctypes.c_int]),
-    "gs_set_pagerank_blocking": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]),
+    "gs_set_pagerank_blocking": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                                                ctypes.c_uint32]),
     "gs_set_wcc_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_kcore_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_batch_bfs_method": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -271,12 +273,13 @@ class Graph:
         r = _Tune()
         _check(_lib.gs_tune(self._h, cache_path.encode() if cache_path else None, ctypes.byref(r)))
         return {"chunk": r.chunk, "xcd_order": r.xcd_order, "from_cache": bool(r.from_cache),
-                "best_ms": r.best_ms,
+                "best_ms": r.best_ms, "do_alpha": r.do_alpha, "do_beta": r.do_beta, "do_ms": r.do_ms,
                 "candidates": [{"chunk": r.cand_chunk[i], "xcd_order": r.cand_order[i], "ms": r.cand_ms[i]}
                                for i in range(r.ncand)]}
 
-    def set_bfs_mode(self, mode: int, alpha: int = 14, beta: int = 24) -> None:
-        """0: tile sweep; 1: direction-optimizing over a neighbour-list index."""
+    def set_bfs_mode(self, mode: int, alpha: int = 0, beta: int = 0) -> None:
+        """0: tile sweep; 1: direction-optimizing over a neighbour-list index
+        (alpha/beta switch thresholds; 0 keeps the current ones)."""
         _check(_lib.gs_set_bfs_mode(self._h, mode, alpha, beta))
 
     def set_pagerank_method(self, method) -> None:
@@ -286,9 +289,9 @@ class Graph:
         _check(_lib.gs_set_pagerank_method(self._h, int(method)))
 
     def set_pagerank_blocking(self, gather_entries: int = 1 << 20, gather_runs: int = 2048,
-                              accum_entries: int = 1 << 20) -> None:
+                              accum_entries: int = 1 << 20, dense_entries: int = 1 << 30) -> None:
         """Work split of the blocked PageRank (small values exercise the split paths)."""
-        _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries))
+        _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries, dense_entries))
 
     def set_wcc_method(self, method: int) -> None:
         """1 (default): sampled linking over the neighbour-list index; 0: hook passes over the tiles."""

@@ -798,11 +798,11 @@ int gs_set_chunk(gs_graph *g, uint32_t edges) {
 }
 
 int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta) {
-    if (!g || mode < 0 || mode > 1 || alpha < 1 || beta < 1) { set_error("gs_set_bfs_mode: bad arguments"); return GS_ERR_ARG; }
+    if (!g || mode < 0 || mode > 1 || alpha < 0 || beta < 0) { set_error("gs_set_bfs_mode: bad arguments"); return GS_ERR_ARG; }
     if (mode == 1 && g->directed) { set_error("direction-optimizing BFS needs an undirected grid"); return GS_ERR_UNSUPPORTED; }
     g->bfs_mode = mode;
-    g->do_alpha = alpha;
-    g->do_beta = beta;
+    if (alpha) g->do_alpha = alpha;  // 0 keeps the current (possibly tuned) value
+    if (beta) g->do_beta = beta;
     return mode == 1 ? build_adjacency(g) : GS_OK;
 }
 
@@ -821,7 +821,8 @@ int gs_set_pagerank_method(gs_graph *g, int method) {
     return GS_OK;
 }
 
-int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries) {
+int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries,
+                             uint32_t dense_entries) {
     if (!g) { set_error("null graph"); return GS_ERR_ARG; }
     if (gather_entries < 1 || gather_entries > (1ull << 30) || gather_runs < 1 || gather_runs > gs::PB_RUNS ||
         accum_entries < 1 || accum_entries > (1ull << 30)) {
@@ -833,6 +834,7 @@ int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gath
     g->pb_unit1 = gather_entries;
     g->pb_runs = gather_runs;
     g->pb_unit2 = accum_entries;
+    g->pb_dense = dense_entries;
     g->pb_ready = false;  // tables rebuilt at the next PageRank
     return GS_OK;
 }

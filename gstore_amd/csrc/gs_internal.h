@@ -211,6 +211,7 @@ struct gs_graph {
     uint64_t pb_unit1 = 1u << 20;            // gather: entries per workgroup at most
     uint32_t pb_runs = gs::PB_RUNS;          // gather: runs per workgroup at most
     uint64_t pb_unit2 = 1u << 20;            // accum: entries per workgroup at most
+    uint32_t pb_dense = 1u << 30;            // accum: entries per vertex from which a block uses LDS float atomics
     gs::DevBuf<uint16_t> pb_src, pb_dst;     // 2E: offsets of u in its block and v in its block
     gs::DevBuf<float> pb_val;                // 2E: prior[u] per entry, rewritten every iteration
     gs::DevBuf<uint64_t> pb_run_pm, pb_run_qm;  // non-empty runs, source-major: prefix, q-major start

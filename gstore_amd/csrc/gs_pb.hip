@@ -29,7 +29,6 @@ namespace {
 
 constexpr int PB_WG = 1024;            // threads per workgroup of the three per-iteration kernels
 constexpr int PB_PASS = 4096;          // build: list entries staged per pass
-constexpr uint64_t PB_DENSE = 64;      // accum: entries per vertex from which a block uses LDS float atomics
 constexpr uint64_t PB_SLICE = 65536;   // build: list entries per workgroup
 
 unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
@@ -450,7 +449,7 @@ int build_blocked(gs_graph *g) {
         const uint64_t parts = (e1 - e0 + unit2 - 1) / unit2;
         // entries per vertex of the block: above PB_DENSE, lanes of a wave mostly share destinations
         const uint64_t qn = std::min<uint64_t>(PB_B, nzn - ((uint64_t)q << PB_SHIFT));
-        const uint32_t dense = (e1 - e0) >= PB_DENSE * qn;
+        const uint32_t dense = (e1 - e0) >= (uint64_t)g->pb_dense * qn;
         if (parts == 1) {
             u2.push_back(PbUnit2{e0, e1, q, 0xFFFFFFFFu, dense, 0});
             continue;

@@ -1,11 +1,12 @@
 // gs_tune.hip -- autotuner for the sweep knobs of one graph.
 //
 // Knobs: edges per workgroup (the tile chunk each workgroup stages, 1024 /
-// 2048 / 4096) and the chunk-to-XCD order (contiguous eighths vs round-robin).
-// Each candidate is timed on one WCC call (its hook and check passes are full
-// sweeps of the grid) plus, for undirected graphs, one k-core call with k = 8
-// (sweeps that skip inactive tiles, whose cost depends on the knobs
-// differently); the smallest sum is applied.  With a cache path, results are kept per
+// 2048 / 4096) and the chunk-to-XCD order (contiguous eighths vs round-robin),
+// timed on the tile methods of WCC (hook and check passes are full sweeps) and,
+// for undirected graphs, k-core with k = 8 (sweeps that skip inactive tiles);
+// the smallest sum is applied.  On undirected resident grids, the
+// direction-optimizing BFS switch thresholds (alpha, beta) are then timed on
+// two fixed mid-degree roots.  With a cache path, results are kept per
 // graph signature (N, E, non-empty tiles, directed) so a later open of the
 // same graph applies them without timing.
 #include <stdio.h>
@@ -27,7 +28,7 @@ std::string signature(const gs_graph *g) {
     return buf;
 }
 
-bool cache_lookup(const char *path, const std::string &sig, uint32_t *chunk, int *order) {
+bool cache_lookup(const char *path, const std::string &sig, uint32_t *chunk, int *order, int *alpha, int *beta) {
     FILE *f = fopen(path, "r");
     if (!f) return false;
     char line[512];
@@ -39,10 +40,13 @@ bool cache_lookup(const char *path, const std::string &sig, uint32_t *chunk, int
         std::string key(line, bar - line);
         while (!key.empty() && key.back() == ' ') key.pop_back();
         unsigned c;
-        int o;
-        if (key == sig && sscanf(bar + 1, " chunk=%u order=%d", &c, &o) == 2) {
+        int o, al = 0, be = 0;
+        const int got = sscanf(bar + 1, " chunk=%u order=%d alpha=%d beta=%d", &c, &o, &al, &be);
+        if (key == sig && got >= 2) {
             *chunk = c;
             *order = o;
+            *alpha = got == 4 ? al : 0;
+            *beta = got == 4 ? be : 0;
             hit = true;  // last entry wins
         }
     }
@@ -57,16 +61,26 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
     gs_tune_result r{};
     const std::string sig = signature(g);
     uint32_t c;
-    int o;
-    if (cache_path && cache_lookup(cache_path, sig, &c, &o)) {
+    int o, al, be;
+    if (cache_path && cache_lookup(cache_path, sig, &c, &o, &al, &be)) {
         if (!g->ss) GS_TRY(gs_set_chunk(g, c));
         g->xcd_order = o;
+        if (al > 0 && be > 0) {
+            g->do_alpha = al;
+            g->do_beta = be;
+        }
         r.chunk = g->chunk;
         r.xcd_order = g->xcd_order;
+        r.do_alpha = al;
+        r.do_beta = be;
         r.from_cache = 1;
         if (out) *out = r;
         return GS_OK;
     }
+    // the sweep knobs act on the tile methods: time those
+    const int wm = g->wcc_method, km = g->kcore_method;
+    g->wcc_method = 0;
+    g->kcore_method = 0;
     const uint32_t chunks[3] = {1024, 2048, 4096};
     double best = 1e30;
     uint32_t best_c = g->chunk;
@@ -97,17 +111,53 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
             }
         }
     }
-    g->ws->timer.on = timing;
+    g->wcc_method = wm;
+    g->kcore_method = km;
     if (!g->ss) GS_TRY(gs_set_chunk(g, best_c));
     g->xcd_order = best_o;
     r.chunk = g->chunk;
     r.xcd_order = g->xcd_order;
     r.ncand = k < 6 ? k : 6;
     r.best_ms = best;
+    // direction-optimizing BFS thresholds, on two mid-degree roots
+    if (!g->directed && !g->ss) {
+        GS_TRY(build_adjacency(g));
+        if (g->nzn >= 3) {
+            uint32_t roots[2];
+            GS_HIP(hipMemcpy(&roots[0], g->old_id.ptr + g->nzn / 3, 4, hipMemcpyDeviceToHost));
+            GS_HIP(hipMemcpy(&roots[1], g->old_id.ptr + 2 * g->nzn / 3, 4, hipMemcpyDeviceToHost));
+            const int alphas[4] = {8, 14, 30, 60}, betas[2] = {24, 64};
+            double best_do = 1e30;
+            int ba = g->do_alpha, bb = g->do_beta;
+            GS_TRY(run_bfs_do(g, g->ws, roots[0], nullptr, nullptr, 0, nullptr));  // warm
+            for (int ai = 0; ai < 4; ++ai)
+                for (int bi = 0; bi < 2; ++bi) {
+                    g->do_alpha = alphas[ai];
+                    g->do_beta = betas[bi];
+                    double ms = 0;
+                    for (int ri = 0; ri < 2; ++ri) {
+                        GS_TRY(run_bfs_do(g, g->ws, roots[ri], nullptr, nullptr, 0, nullptr));
+                        ms += g->ws->stats.ms;
+                    }
+                    if (ms < best_do) {
+                        best_do = ms;
+                        ba = alphas[ai];
+                        bb = betas[bi];
+                    }
+                }
+            g->do_alpha = ba;
+            g->do_beta = bb;
+            r.do_alpha = ba;
+            r.do_beta = bb;
+            r.do_ms = best_do;
+        }
+    }
+    g->ws->timer.on = timing;
     if (cache_path) {
         FILE *f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%s | chunk=%u order=%d sweep_ms=%.3f\n", sig.c_str(), g->chunk, g->xcd_order, best);
+            fprintf(f, "%s | chunk=%u order=%d alpha=%d beta=%d sweep_ms=%.3f\n", sig.c_str(), g->chunk,
+                    g->xcd_order, r.do_alpha, r.do_beta, best);
             fclose(f);
         }
     }

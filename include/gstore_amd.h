@@ -213,8 +213,9 @@ void gs_server_destroy(gs_server *s);
  * each level top-down over the frontier's lists or bottom-up (each unreached
  * vertex stops at its first frontier neighbour), switching to bottom-up when
  * the frontier's edges exceed the unreached side's edges / alpha and back when
- * the frontier holds fewer than N / beta vertices (defaults 14, 24).  Depths
- * and level counts are identical in both modes.  Undirected grids only. */
+ * the frontier holds fewer than N / beta vertices (defaults 14, 24; 0 keeps
+ * the current values, e.g. those gs_tune chose).  Depths and level counts are
+ * identical in both modes.  Undirected grids only. */
 int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta);
 
 /* PageRank method on undirected resident grids:
@@ -257,10 +258,12 @@ int gs_set_kcore_method(gs_graph *g, int method);
 
 /* Work split of the blocked PageRank: source-major entries (<= 2^30) and
  * runs (<= 2048) per gather workgroup, entries per accumulation workgroup
- * (<= 2^30).  Defaults 2^20, 2048, 2^20.  Small values exercise the split
- * paths on small graphs; results change only in FP32 summation order.  No
- * counterpart in the reference. */
-int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries);
+ * (<= 2^30), and the entries per vertex from which a destination block adds
+ * with LDS float atomics rather than compare-and-swap.  Defaults 2^20, 2048,
+ * 2^20, 2^30 (compare-and-swap throughout: measured faster on Kron-28).  Small values exercise the split paths on small graphs; results
+ * change only in FP32 summation order.  No counterpart in the reference. */
+int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries,
+                             uint32_t dense_entries);
 
 /* PageRank / personalised PageRank on undirected grids: 1 (default) adds
  * into the (at most 2048) highest-degree vertices through a per-workgroup LDS
@@ -280,20 +283,24 @@ int gs_set_chunk(gs_graph *g, uint32_t edges);
  * it; gs_tune picks the faster one. */
 int gs_set_sweep_order(gs_graph *g, int xcd_contiguous);
 
-/* Autotuner: times one WCC call (two full grid sweeps) for each
- * edges-per-workgroup value (1024, 2048, 4096) and both sweep orders,
- * applies the fastest, and
- * appends it to cache_path (optional) keyed by N, E, non-empty tiles and
- * directedness; a cached entry is applied without timing. */
+/* Autotuner: times the tile methods of WCC and k-core (full and
+ * frontier-skipping sweeps) for each edges-per-workgroup value (1024, 2048,
+ * 4096) and both sweep orders, and, on undirected resident grids, the
+ * direction-optimizing BFS from two fixed mid-degree roots for each
+ * top-down/bottom-up switch pair (alpha, beta) in {8, 14, 30, 60} x {24, 64};
+ * applies the fastest, and appends it to cache_path (optional) keyed by N, E,
+ * non-empty tiles and directedness; a cached entry is applied without timing. */
 typedef struct gs_tune_result {
     uint32_t chunk;
     int32_t xcd_order;
     int32_t from_cache;
     int32_t ncand;
-    double best_ms;           /* ms of the timed WCC call for the chosen setting */
+    double best_ms;           /* ms of the timed sweeps for the chosen setting */
     uint32_t cand_chunk[6];
     int32_t cand_order[6];
     double cand_ms[6];
+    int32_t do_alpha, do_beta;  /* chosen switch thresholds (0 when not tuned) */
+    double do_ms;               /* ms of the two timed BFS with them */
 } gs_tune_result;
 
 int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out);

@@ -306,7 +306,7 @@ def test_bfs_direction_optimizing_bit_exact(loaded, name):
             d, lc = oracle.bfs(n, start, grid, root)
             assert np.array_equal(res.depth, d), (name, root, alpha, beta)
             assert res.level_counts == lc
-    g.set_bfs_mode(0)
+    g.set_bfs_mode(0, 14, 24)
 
 
 @pytest.mark.parametrize("name", ["kron16", "two_groups", "star", "tiles"])
@@ -343,17 +343,18 @@ def test_pagerank_blocked_split_paths(s):
     want_c = oracle.pagerank(n, start, grid, deg, 5, 0)
     want_i = oracle.pagerank(n, start, grid, deg, 5, 1)
     with ga.Graph.from_arrays(n, start, grid, deg) as g:
-        for lim in ((1 << 20, 2048, 1 << 20), (97, 3, 5000), (1000, 1, 1 << 12), (4096, 2048, 1 << 30)):
+        for lim in ((1 << 20, 2048, 1 << 20, 64), (97, 3, 5000, 0), (1000, 1, 1 << 12, 1 << 30),
+                    (4096, 2048, 1 << 30, 8)):
             _say("blocked limits", s, lim)
             g.set_pagerank_blocking(*lim)
             assert np.array_equal(g.pagerank(5, ga.PR_COMPAT), want_c)
             assert np.max(np.abs(g.pagerank(5, ga.PR_INTENDED) - want_i)) <= PR_TOL
         with pytest.raises(ga.GStoreError):
-            g.set_pagerank_blocking(0, 1, 1)
+            g.set_pagerank_blocking(0, 1, 1, 64)
         with pytest.raises(ga.GStoreError):
-            g.set_pagerank_blocking(1 << 31, 1, 1)
+            g.set_pagerank_blocking(1 << 31, 1, 1, 64)
         with pytest.raises(ga.GStoreError):
-            g.set_pagerank_blocking(1, 4096, 1)
+            g.set_pagerank_blocking(1, 4096, 1, 64)
         with pytest.raises(ga.GStoreError):
             g.set_pagerank_method(3)
 
