@@ -219,6 +219,7 @@ struct gs_graph {
     gs::DevBuf<gs::PbUnit2> pb_u2;
     gs::DevBuf<gs::PbRed> pb_red;
     gs::DevBuf<float> pb_part;               // partial sums of split blocks, PB_B per slot
+    gs::DevBuf<double> pb_part64;            // the same in FP64 (personalised PageRank), allocated on use
     std::mutex pb_mu;                        // one blocked PageRank at a time (pb_val, pb_part are shared)
     std::mutex build_mu;                     // lazy builds (index, heavy set) from concurrent workspaces
     ~gs_graph();
@@ -247,6 +248,7 @@ int build_adjacency(gs_graph *g);
 int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double *acc64);
 int build_blocked(gs_graph *g);  // the blocked copy (after the lists); pb_k = 0 when too many blocks
 int pr_blocked(const gs_graph *g, gs_ws *w, float *prior, const float *fac);
+int pr_blocked64(const gs_graph *g, gs_ws *w, const float *prior, double *acc);
 size_t pb_bytes(const gs_graph *g);
 int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out);
 int unpermute_i32(const gs_graph *g, hipStream_t st, const int32_t *x_r, int32_t *out);

@@ -327,7 +327,10 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
     }
     if (w->ppr_rank.n != g->n) GS_TRY(w->ppr_rank.alloc(g->n));
     if (w->ppr_acc.n != g->n) GS_TRY(w->ppr_acc.alloc(g->n));
-    GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream, false));
+    // the blocked method sums in FP64 too; its value buffer belongs to the graph (calls take turns)
+    std::unique_lock<std::mutex> pb_lock(const_cast<gs_graph *>(g)->pb_mu, std::defer_lock);
+    if (use_pull(g) && g->pr_method == 2) pb_lock.lock();
+    GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream, true));
     w->timer.reset();
     w->stats = gs_stats{};
     hipEvent_t t0, t1;
@@ -345,9 +348,11 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
     k_ppr_init<<<vgrid(g->n), 256, 0, w->stream>>>(pull ? g->deg_r.ptr : g->degree.ptr, g->n, s_id, g->heavy_dthr,
                                                    w->pr_prior.ptr, w->ppr_acc.ptr, w->pr_fac.ptr);
     GS_HIP(hipGetLastError());
+    const bool blocked = pull && g->pr_method == 2 && g->pb_k;
     for (int it = 0; it < iters; ++it) {
         hipEvent_t kb;
-        GS_TRY(scatter(g, w, w->ppr_acc.ptr));
+        if (blocked) GS_TRY(pr_blocked64(g, w, w->pr_prior.ptr, w->ppr_acc.ptr));
+        else GS_TRY(scatter(g, w, w->ppr_acc.ptr));
         w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
         k_ppr_finalize<<<vgrid(g->n), 256, 0, w->stream>>>(g->n, s_id, w->pr_prior.ptr, w->ppr_acc.ptr,
                                                            w->pr_fac.ptr, it + 1 == iters ? w->ppr_rank.ptr : nullptr);

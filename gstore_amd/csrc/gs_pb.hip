@@ -274,6 +274,83 @@ __global__ void __launch_bounds__(PB_WG) k_pb_accum(const PbUnit2 *__restrict__ 
     }
 }
 
+// FP64 variant for personalised PageRank: the same entries, summed in FP64
+// into an LDS array over half a destination block (2^14 doubles) per pass, so
+// each unit reads its entries twice.
+__device__ inline void lds_add_cas64(double *p, double x) {
+    unsigned long long *a = reinterpret_cast<unsigned long long *>(p);
+    unsigned long long old = *a, assumed;
+    do {
+        assumed = old;
+        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(__longlong_as_double((long long)assumed) + x));
+    } while (old != assumed);
+}
+
+__global__ void __launch_bounds__(PB_WG) k_pb_accum64(const PbUnit2 *__restrict__ units,
+                                                      const uint16_t *__restrict__ dst,
+                                                      const float *__restrict__ val, uint64_t nzn,
+                                                      double *__restrict__ acc, double *__restrict__ part) {
+    constexpr uint32_t H = PB_B / 2;
+    __shared__ double sum[H];
+    const PbUnit2 u = units[blockIdx.x];
+    const uint64_t qbase = (uint64_t)u.q << PB_SHIFT;
+    const uint32_t qn = (uint32_t)std::min<uint64_t>(PB_B, nzn - qbase);
+    for (uint32_t lo = 0; lo < qn; lo += H) {
+        for (uint32_t i = threadIdx.x; i < H; i += PB_WG) sum[i] = 0.0;
+        __syncthreads();
+        for (uint64_t sb = u.e0 & ~15ull; sb < u.e1; sb += 16 * PB_WG) {
+            const uint64_t e0 = sb + threadIdx.x * 16;
+            if (e0 >= u.e1) break;
+            pb_u32x4 dq[2], vq[4];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                dq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(dst + e0) + k);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                vq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(val + e0) + k);
+            uint32_t cur = 0xFFFFFFFFu;
+            double run = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const uint64_t e = e0 + i;
+                const uint32_t d = (dq[i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
+                if (e < u.e0 || e >= u.e1 || d < lo || d >= lo + H) continue;
+                const double x = (double)__uint_as_float(vq[i >> 2][i & 3]);
+                if (d != cur) {
+                    if (cur != 0xFFFFFFFFu) lds_add_cas64(&sum[cur - lo], run);
+                    cur = d;
+                    run = x;
+                } else {
+                    run += x;
+                }
+            }
+            if (cur != 0xFFFFFFFFu) lds_add_cas64(&sum[cur - lo], run);
+        }
+        __syncthreads();
+        const uint32_t hn = std::min(H, qn - lo);
+        if (u.slot == 0xFFFFFFFFu) {
+            for (uint32_t i = threadIdx.x; i < hn; i += PB_WG) acc[qbase + lo + i] = sum[i];
+        } else {
+            double *pp = part + (uint64_t)u.slot * PB_B + lo;
+            for (uint32_t i = threadIdx.x; i < hn; i += PB_WG) pp[i] = sum[i];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_pb_reduce64(const PbRed *__restrict__ red, const double *__restrict__ part,
+                                                     uint64_t nzn, double *__restrict__ acc) {
+    const PbRed r = red[blockIdx.y];
+    const uint64_t qbase = (uint64_t)r.q << PB_SHIFT;
+    const uint32_t qn = (uint32_t)std::min<uint64_t>(PB_B, nzn - qbase);
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= qn) return;
+    const double *pp = part + (uint64_t)r.slot0 * PB_B + i;
+    double s = 0.0;
+    for (uint32_t k = 0; k < r.nslot; ++k) s += pp[(uint64_t)k * PB_B];
+    acc[qbase + i] = s;
+}
+
 // New priors of a split destination block from the sum of its units' partials.
 __global__ void __launch_bounds__(256) k_pb_reduce(const PbRed *__restrict__ red, const float *__restrict__ part,
                                                    uint64_t nzn, const float *__restrict__ fac,
@@ -322,7 +399,8 @@ static int check_units(const std::vector<PbUnit1> &u1, const std::vector<PbUnit2
 
 size_t pb_bytes(const gs_graph *g) {
     return g->pb_src.bytes() + g->pb_dst.bytes() + g->pb_val.bytes() + g->pb_run_pm.bytes() +
-           g->pb_run_qm.bytes() + g->pb_u1.bytes() + g->pb_u2.bytes() + g->pb_red.bytes() + g->pb_part.bytes();
+           g->pb_run_qm.bytes() + g->pb_u1.bytes() + g->pb_u2.bytes() + g->pb_red.bytes() + g->pb_part.bytes() +
+           g->pb_part64.bytes();
 }
 
 // Builds the blocked copy from the neighbour-list index (built first if
@@ -499,18 +577,47 @@ int build_blocked(gs_graph *g) {
     return GS_OK;
 }
 
+static int pb_gather(const gs_graph *g, gs_ws *w, const float *prior) {
+    if (!g->pb_nu1) return GS_OK;
+    hipEvent_t kb;
+    w->timer.begin(GS_K_PB_GATHER, w->stream, &kb);
+    k_pb_gather<<<(unsigned)g->pb_nu1, PB_WG, 0, w->stream>>>(g->pb_u1.ptr, g->pb_run_pm.ptr, g->pb_run_qm.ptr,
+                                                              g->pb_src.ptr, prior, g->nzn, g->pb_val.ptr);
+    GS_HIP(hipGetLastError());
+    w->timer.end(GS_K_PB_GATHER, w->stream, kb);
+    return GS_OK;
+}
+
+// One FP64 accumulation (personalised PageRank): acc[v] = sum of prior over v's
+// list for the relabelled ids 0 .. nzn-1, stored (not added).
+int pr_blocked64(const gs_graph *gc, gs_ws *w, const float *prior, double *acc) {
+    gs_graph *g = const_cast<gs_graph *>(gc);
+    if (g->pb_nred && g->pb_part64.n != g->pb_part.n) GS_TRY(g->pb_part64.alloc(g->pb_part.n));
+    GS_TRY(pb_gather(g, w, prior));
+    hipEvent_t kb;
+    if (g->pb_nu2) {
+        w->timer.begin(GS_K_PB_ACCUM, w->stream, &kb);
+        k_pb_accum64<<<(unsigned)g->pb_nu2, PB_WG, 0, w->stream>>>(g->pb_u2.ptr, g->pb_dst.ptr, g->pb_val.ptr, g->nzn,
+                                                                   acc, g->pb_part64.ptr);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_PB_ACCUM, w->stream, kb);
+    }
+    if (g->pb_nred) {
+        w->timer.begin(GS_K_PB_REDUCE, w->stream, &kb);
+        k_pb_reduce64<<<dim3(PB_B / 256, (unsigned)g->pb_nred), 256, 0, w->stream>>>(g->pb_red.ptr, g->pb_part64.ptr,
+                                                                                    g->nzn, acc);
+        GS_HIP(hipGetLastError());
+        w->timer.end(GS_K_PB_REDUCE, w->stream, kb);
+    }
+    return GS_OK;
+}
+
 // One PageRank iteration for the relabelled ids 0 .. nzn-1: the sums over
 // the lists and, fused into the accumulation, the new priors (the gather has
 // read every prior it needs before the accumulation starts).
 int pr_blocked(const gs_graph *g, gs_ws *w, float *prior, const float *fac) {
+    GS_TRY(pb_gather(g, w, prior));
     hipEvent_t kb;
-    if (g->pb_nu1) {
-        w->timer.begin(GS_K_PB_GATHER, w->stream, &kb);
-        k_pb_gather<<<(unsigned)g->pb_nu1, PB_WG, 0, w->stream>>>(g->pb_u1.ptr, g->pb_run_pm.ptr, g->pb_run_qm.ptr,
-                                                                  g->pb_src.ptr, prior, g->nzn, g->pb_val.ptr);
-        GS_HIP(hipGetLastError());
-        w->timer.end(GS_K_PB_GATHER, w->stream, kb);
-    }
     if (g->pb_nu2) {
         w->timer.begin(GS_K_PB_ACCUM, w->stream, &kb);
         k_pb_accum<<<(unsigned)g->pb_nu2, PB_WG, 0, w->stream>>>(g->pb_u2.ptr, g->pb_dst.ptr, g->pb_val.ptr, g->nzn,

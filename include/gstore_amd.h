@@ -229,7 +229,8 @@ int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta);
  *   LDS and stored, with no per-edge atomics;
  * 0 scatters over the tile grid with FP32 atomics, the reference's scheme
  *   (pr.cpp:119-154).
- * Personalised PageRank uses 1 for both 1 and 2 (FP64 sums).  Directed and
+ * Personalised PageRank sums in FP64 with either (the blocked accumulation
+ * then covers half a destination block per pass).  Directed and
  * semi-external grids always scatter.  Only the FP32 summation order differs. */
 int gs_set_pagerank_method(gs_graph *g, int method);
 

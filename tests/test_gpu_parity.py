@@ -342,6 +342,8 @@ def test_pagerank_blocked_split_paths(s):
     start, grid, deg = oracle.convert(e, n)
     want_c = oracle.pagerank(n, start, grid, deg, 5, 0)
     want_i = oracle.pagerank(n, start, grid, deg, 5, 1)
+    src = int(np.argmax(deg))
+    want_p = oracle.ppr(n, start, grid, deg, src, 20)
     with ga.Graph.from_arrays(n, start, grid, deg) as g:
         for lim in ((1 << 20, 2048, 1 << 20, 64), (97, 3, 5000, 0), (1000, 1, 1 << 12, 1 << 30),
                     (4096, 2048, 1 << 30, 8)):
@@ -349,6 +351,8 @@ def test_pagerank_blocked_split_paths(s):
             g.set_pagerank_blocking(*lim)
             assert np.array_equal(g.pagerank(5, ga.PR_COMPAT), want_c)
             assert np.max(np.abs(g.pagerank(5, ga.PR_INTENDED) - want_i)) <= PR_TOL
+            p = g.ppr(src, 20)
+            assert p.min() >= 0 and float(np.abs(p - want_p).sum()) <= 1e-6
         with pytest.raises(ga.GStoreError):
             g.set_pagerank_blocking(0, 1, 1, 64)
         with pytest.raises(ga.GStoreError):
