@@ -289,7 +289,7 @@ class Graph:
         _check(_lib.gs_set_pagerank_method(self._h, int(method)))
 
     def set_pagerank_blocking(self, gather_entries: int = 1 << 20, gather_runs: int = 2048,
-                              accum_entries: int = 1 << 20, dense_entries: int = 1 << 30) -> None:
+                              accum_entries: int = 1 << 20, dense_entries: int = 8192) -> None:
         """Work split of the blocked PageRank (small values exercise the split paths)."""
         _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries, dense_entries))
 

@@ -36,8 +36,9 @@ constexpr int PB_SHIFT = 15;                 // vertices per block: 2^15, FP32 v
 constexpr uint32_t PB_B = 1u << PB_SHIFT;
 constexpr uint32_t PB_KMAX = 8192;           // blocks at most (vertices with edges <= 2^28)
 constexpr uint32_t PB_RUNS = 2048;           // runs staged per gather workgroup at most
+constexpr uint32_t PB_WIN = 16;              // entries per window: every run starts on a window and fills whole ones
 struct PbSlice { uint64_t e0, e1; uint32_t q, pad; };               // build: list entries of block q
-struct PbUnit1 { uint64_t pm0, pm1; uint32_t r0, r1, p, pad; };     // gather: source-major entries of block p
+struct PbUnit1 { uint64_t w0, w1; uint32_t r0, r1, p, pad; };       // gather: source-major windows of block p
 struct PbUnit2 { uint64_t e0, e1; uint32_t q, slot, dense, pad; };  // accum: slot ~0u = the block's only unit
 struct PbRed { uint32_t q, slot0, nslot, pad; };                    // split block: its partial slots
 
@@ -211,10 +212,11 @@ struct gs_graph {
     uint64_t pb_unit1 = 1u << 20;            // gather: entries per workgroup at most
     uint32_t pb_runs = gs::PB_RUNS;          // gather: runs per workgroup at most
     uint64_t pb_unit2 = 1u << 20;            // accum: entries per workgroup at most
-    uint32_t pb_dense = 1u << 30;            // accum: entries per vertex from which a block uses LDS float atomics
-    gs::DevBuf<uint16_t> pb_src, pb_dst;     // 2E: offsets of u in its block and v in its block
-    gs::DevBuf<float> pb_val;                // 2E: prior[u] per entry, rewritten every iteration
-    gs::DevBuf<uint64_t> pb_run_pm, pb_run_qm;  // non-empty runs, source-major: prefix, q-major start
+    uint32_t pb_dense = 8192;                // accum: entries per vertex from which a block uses LDS float atomics
+    uint64_t pb_entries = 0;                 // 2E plus the padding that fills each run's last window
+    gs::DevBuf<uint16_t> pb_src, pb_dst;     // pb_entries: offsets of u in its block and v in its block
+    gs::DevBuf<float> pb_val;                // pb_entries: prior[u] per entry, rewritten every iteration
+    gs::DevBuf<uint64_t> pb_run_pw, pb_run_qw;  // non-empty runs, source-major: window prefix, q-major window
     gs::DevBuf<gs::PbUnit1> pb_u1;
     gs::DevBuf<gs::PbUnit2> pb_u2;
     gs::DevBuf<gs::PbRed> pb_red;

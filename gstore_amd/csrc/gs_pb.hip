@@ -8,10 +8,14 @@
 // (0 .. nzn-1) are cut into blocks of PB_B = 2^15 vertices, whose FP32 values
 // fill 128 KiB of LDS.  Every list entry (v <- u) is copied once into an array
 // ordered by destination block q = v / PB_B, then source block p = u / PB_B
-// (the run (p, q)), keeping only the two 15-bit offsets inside the blocks:
-//   gather, per source block p: stage prior[p*PB_B ..] in LDS and write
-//           val[e] = prior[u] at every entry of the runs (p, *) -- 2 bytes read
-//           and 4 bytes written per entry, contiguous within each run;
+// (the run (p, q)), keeping only the two 15-bit offsets inside the blocks.
+// Every run starts on a window of PB_WIN = 16 entries and fills whole windows
+// (on a Kronecker graph of 2^28 vertices the padding adds about 1.2% of
+// entries: runs average ~900 entries); padding entries read a zero and add it
+// to the run's last destination.
+//   gather, per source block p: stage prior[p*PB_B ..] in LDS and, one lane
+//           per window of the runs (p, *), write val[e] = prior[u] -- 2 bytes
+//           read and 4 bytes written per entry in 16-byte accesses;
 //   accum,  per destination block q: add val[e] into an LDS array over the
 //           block at the entry's destination offset (one LDS atomic per run of
 //           equal offsets in a wave) and store the block's sums.
@@ -22,6 +26,8 @@
 #include <vector>
 
 #include "gs_internal.h"
+
+#include <hipcub/block/block_radix_sort.hpp>
 
 namespace gs {
 
@@ -55,26 +61,37 @@ __global__ void __launch_bounds__(PB_WG) k_pb_count(const PbSlice *__restrict__ 
 
 // Copy one slice of the lists of block q into the runs (p, q): each workgroup
 // reserves a contiguous stretch of every run it feeds (one global atomic per
-// run), then places entries with LDS cursors.  cur[q*K + p] starts at the
-// run's first position.
+// run), then places its entries in list order, so within a stretch a run's
+// entries are sorted by destination and a destination's entries from this
+// slice are adjacent (the accumulation then adds them in registers).  Per pass
+// of PB_PASS entries: a stable block radix sort by source block gives each
+// entry its rank among the pass's entries of the same run.  cur[q*K + p]
+// starts at the run's first position.
+typedef hipcub::BlockRadixSort<uint32_t, PB_WG, PB_PASS / PB_WG, uint32_t> PbPassSort;
+
 __global__ void __launch_bounds__(PB_WG) k_pb_fill(const PbSlice *__restrict__ sl, const uint64_t *__restrict__ off,
                                                    const uint32_t *__restrict__ nbr, uint64_t nzn, uint32_t K,
                                                    unsigned long long *cur, uint16_t *__restrict__ src_out,
                                                    uint16_t *__restrict__ dst_out) {
-    __shared__ uint32_t hist[PB_KMAX];
+    constexpr int IPT = PB_PASS / PB_WG;   // entries per thread per pass, consecutive (blocked arrangement)
+    constexpr uint32_t NONE = PB_KMAX;     // sort key of the slots past the slice's end
+    __shared__ uint32_t head[PB_KMAX];     // slice counts, then per pass: first sorted slot of each run
     __shared__ unsigned long long base[PB_KMAX];
-    __shared__ uint64_t voff[PB_PASS + 2];
+    __shared__ union {
+        uint64_t voff[PB_PASS + 2];
+        typename PbPassSort::TempStorage sort;
+        uint16_t key[PB_PASS];
+    } sh;
     __shared__ uint64_t v_first;
     __shared__ int nvs;
     const PbSlice s = sl[blockIdx.x];
-    for (uint32_t k = threadIdx.x; k < K; k += PB_WG) hist[k] = 0;
+    for (uint32_t k = threadIdx.x; k < K; k += PB_WG) head[k] = 0;
     __syncthreads();
-    for (uint64_t e = s.e0 + threadIdx.x; e < s.e1; e += PB_WG) atomicAdd(&hist[nbr[e] >> PB_SHIFT], 1u);
+    for (uint64_t e = s.e0 + threadIdx.x; e < s.e1; e += PB_WG) atomicAdd(&head[nbr[e] >> PB_SHIFT], 1u);
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < K; k += PB_WG) {
-        uint32_t c = hist[k];
+        uint32_t c = head[k];
         base[k] = c ? atomicAdd(&cur[(uint64_t)s.q * K + k], (unsigned long long)c) : 0ull;
-        hist[k] = 0;
     }
     const uint64_t vlo = (uint64_t)s.q << PB_SHIFT, vhi = std::min<uint64_t>(vlo + PB_B, nzn);
     for (uint64_t a = s.e0; a < s.e1; a += PB_PASS) {
@@ -99,28 +116,73 @@ __global__ void __launch_bounds__(PB_WG) k_pb_fill(const PbSlice *__restrict__ s
         __syncthreads();
         const int nv = nvs;
         const uint64_t v0 = v_first;
-        for (int k = threadIdx.x; k <= nv; k += PB_WG) voff[k] = off[v0 + k];
+        for (int k = threadIdx.x; k <= nv; k += PB_WG) sh.voff[k] = off[v0 + k];
         __syncthreads();
-        for (uint64_t e = a + threadIdx.x; e < b; e += PB_WG) {
-            int lo = 0, hi = nv - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (voff[mid] <= e) lo = mid; else hi = mid - 1;
+        uint32_t key[IPT], val[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const uint64_t e = a + threadIdx.x * IPT + j;
+            key[j] = NONE;
+            val[j] = 0;
+            if (e < b) {
+                int lo = 0, hi = nv - 1;
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (sh.voff[mid] <= e) lo = mid; else hi = mid - 1;
+                }
+                const uint32_t u = nbr[e];
+                key[j] = u >> PB_SHIFT;
+                val[j] = (u & (PB_B - 1)) | (uint32_t)((v0 + lo) & (PB_B - 1)) << 16;
             }
-            const uint32_t u = nbr[e];
-            const uint32_t p = u >> PB_SHIFT;
-            const unsigned long long pos = base[p] + atomicAdd(&hist[p], 1u);
-            src_out[pos] = (uint16_t)(u & (PB_B - 1));
-            dst_out[pos] = (uint16_t)((v0 + lo) & (PB_B - 1));
+        }
+        __syncthreads();   // voff is dead; its storage becomes the sort's
+        PbPassSort(sh.sort).Sort(key, val, 0, 14);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) sh.key[threadIdx.x * IPT + j] = (uint16_t)key[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const uint32_t i = threadIdx.x * IPT + j;
+            if (key[j] != NONE && (i == 0 || sh.key[i - 1] != key[j])) head[key[j]] = i;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (key[j] == NONE) continue;
+            const uint32_t i = threadIdx.x * IPT + j;
+            const unsigned long long pos = base[key[j]] + (i - head[key[j]]);
+            src_out[pos] = (uint16_t)(val[j] & 0xFFFFu);
+            dst_out[pos] = (uint16_t)(val[j] >> 16);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const uint32_t i = threadIdx.x * IPT + j;
+            if (key[j] != NONE && (i + 1 == PB_PASS || sh.key[i + 1] != key[j])) base[key[j]] += i - head[key[j]] + 1;
+        }
+    }
+}
+
+// After the fill, cur[i] is the end of run i's entries; the rest of its last
+// window becomes padding: source offset PB_B (a zero in the gather's LDS) and
+// the destination offset of the run's last entry.
+__global__ void k_pb_pad(const unsigned long long *__restrict__ cur, uint64_t nrun, uint16_t *__restrict__ src,
+                         uint16_t *__restrict__ dst) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nrun; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t end = cur[i];
+        if ((end & (PB_WIN - 1)) == 0) continue;
+        const uint16_t d = dst[end - 1];
+        for (uint64_t j = end; j & (PB_WIN - 1); ++j) {
+            src[j] = (uint16_t)PB_B;
+            dst[j] = d;
         }
     }
 }
 
 // ---- per iteration -----------------------------------------------------------
 
-constexpr int PB_PER = 16;                         // entries per lane per step
-constexpr uint32_t PB_WAVE_SPAN = 64 * PB_PER;     // a wave's entries per step, lane-interleaved
-constexpr uint32_t PB_STEP = PB_WAVE_SPAN * (PB_WG / 64);
+constexpr int PB_WPL = 4;   // gather: windows per lane per step, all loads issued first
 
 // prior[pbase .. pbase + pn) into LDS with 16-byte loads, all issued before
 // the stores (pbase is a multiple of 2^15, so the loads are aligned).
@@ -142,78 +204,75 @@ __device__ inline void stage_block(const float *__restrict__ prior, uint64_t pba
     for (uint32_t i = (n4 << 2) + threadIdx.x; i < pn; i += PB_WG) pri[i] = prior[pbase + i];
 }
 
-// val[e] = prior[p*PB_B + src[e]] for the entries [pm0, pm1) of source block p,
-// counted in source-major order; the unit's runs are staged in LDS.  A wave
-// takes 1024 consecutive entries per step, lane-interleaved, so each lane's
-// run index only moves forward (at most 64 runs per 64 entries).
+typedef unsigned int pb_u32x4 __attribute__((ext_vector_type(4)));
+typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
+
+// val[e] = prior[p*PB_B + src[e]] for the windows [w0, w1) of source block p,
+// counted in source-major order; the unit's runs are staged in LDS.  Lane i of
+// the workgroup takes windows w0 + i, w0 + i + PB_WG, ... (adjacent lanes
+// mostly share a run, so a wave reads and writes 2 KiB and 4 KiB runs); a
+// window's run comes from an 11-step search over the staged run starts.
 __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__ units,
-                                                     const uint64_t *__restrict__ run_pm,
-                                                     const uint64_t *__restrict__ run_qm,
+                                                     const uint64_t *__restrict__ run_pw,
+                                                     const uint64_t *__restrict__ run_qw,
                                                      const uint16_t *__restrict__ src,
                                                      const float *__restrict__ prior, uint64_t nzn,
                                                      float *__restrict__ val) {
-    __shared__ float4 pri4[PB_B / 4];
+    __shared__ float4 pri4[PB_B / 4 + 1];   // + the zero read by padding entries (offset PB_B)
     __shared__ uint32_t rs[PB_RUNS + 1];
     __shared__ uint64_t rq[PB_RUNS];
     const PbUnit1 u = units[blockIdx.x];
     const uint64_t pbase = (uint64_t)u.p << PB_SHIFT;
     const uint32_t pn = (uint32_t)std::min<uint64_t>(PB_B, nzn - pbase);
     stage_block(prior, pbase, pn, pri4);
+    if (threadIdx.x == 0) pri4[PB_B / 4] = make_float4(0.f, 0.f, 0.f, 0.f);
     const float *pri = reinterpret_cast<const float *>(pri4);
     const int nr = (int)(u.r1 - u.r0);
     for (int k = threadIdx.x; k < nr; k += PB_WG) {
-        const uint64_t r0 = run_pm[u.r0 + k];
-        const uint64_t s = r0 > u.pm0 ? r0 : u.pm0;
-        rs[k] = (uint32_t)(s - u.pm0);
-        rq[k] = run_qm[u.r0 + k] + (s - r0);
+        const uint64_t r0 = run_pw[u.r0 + k];
+        const uint64_t s = r0 > u.w0 ? r0 : u.w0;
+        rs[k] = (uint32_t)(s - u.w0);
+        rq[k] = run_qw[u.r0 + k] + (s - r0);
     }
-    const uint32_t len = (uint32_t)(u.pm1 - u.pm0);
+    const uint32_t len = (uint32_t)(u.w1 - u.w0);
     if (threadIdx.x == 0) rs[nr] = len;
     __syncthreads();
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t sb = 0; sb < len; sb += PB_STEP) {
-        const uint32_t wb = sb + wave * PB_WAVE_SPAN;
-        if (wb >= len) break;  // wave-uniform, no barrier follows
-        // largest k < nr with rs[k] <= e, by fixed-step searches (rs[0] = 0)
-        int k = 0;
-        {
-            const uint32_t e = wb + lane;
+    for (uint32_t wb = threadIdx.x; wb < len; wb += PB_WPL * PB_WG) {
+        uint64_t qw[PB_WPL];
+        pb_u32x4 sq[PB_WPL][2];
+#pragma unroll
+        for (int j = 0; j < PB_WPL; ++j) {
+            // past the unit, a lane reloads its first window and stores nothing
+            const uint32_t w = wb + j * PB_WG < len ? wb + j * PB_WG : wb;
+            int k = 0;
 #pragma unroll
             for (int step = PB_RUNS / 2; step > 0; step >>= 1)
-                if (k + step < nr && rs[k + step] <= e) k += step;
+                if (k + step < nr && rs[k + step] <= w) k += step;
+            qw[j] = rq[k] + (w - rs[k]);
+            const pb_u32x4 *s4 = reinterpret_cast<const pb_u32x4 *>(src + qw[j] * PB_WIN);
+            sq[j][0] = __builtin_nontemporal_load(s4);
+            sq[j][1] = __builtin_nontemporal_load(s4 + 1);
         }
-        uint64_t pos[PB_PER];
 #pragma unroll
-        for (int j = 0; j < PB_PER; ++j) {
-            const uint32_t e = wb + j * 64 + lane;
-            if (e < len && rs[k + 1] <= e) {
-                // 64 entries on, at most 64 runs on
+        for (int j = 0; j < PB_WPL; ++j) {
+            if (wb + j * PB_WG >= len) break;
+            pb_f32x4 *v4 = reinterpret_cast<pb_f32x4 *>(val + qw[j] * PB_WIN);
 #pragma unroll
-                for (int step = 64; step > 0; step >>= 1)
-                    if (k + step < nr && rs[k + step] <= e) k += step;
+            for (int h = 0; h < 4; ++h) {
+                const uint32_t a = sq[j][h >> 1][(h & 1) * 2], b = sq[j][h >> 1][(h & 1) * 2 + 1];
+                // plain stores: nontemporal ones measured four times slower here
+                v4[h] = pb_f32x4{pri[a & 0xFFFFu], pri[a >> 16], pri[b & 0xFFFFu], pri[b >> 16]};
             }
-            pos[j] = e < len ? rq[k] + (e - rs[k]) : ~0ull;
         }
-        uint32_t sv[PB_PER];
-#pragma unroll
-        for (int j = 0; j < PB_PER; ++j) sv[j] = pos[j] != ~0ull ? src[pos[j]] : 0u;
-#pragma unroll
-        for (int j = 0; j < PB_PER; ++j)
-            if (pos[j] != ~0ull) val[pos[j]] = pri[sv[j]];
     }
 }
-
-// Sums of val over the entries [e0, e1) of destination block q, in LDS.  Each
-// lane takes 16 consecutive entries (16-entry aligned, so the loads are two
-// and four 16-byte loads; both arrays carry 16 entries of padding), adds runs
-// of equal destination offsets in registers and each run into LDS once.
-typedef unsigned int pb_u32x4 __attribute__((ext_vector_type(4)));
 
 // FP32 add into LDS by a 32-bit compare-and-swap loop.  On this GPU the LDS
 // float atomic costs about four times a streamed entry when the addresses of a
 // wave are spread (blocks of low-degree vertices), while a CAS that rarely
 // retries costs about as much as an integer atomic; where a wave's lanes share
-// addresses (blocks of heavy vertices) the hardware float atomic is kept.
+// addresses (blocks of the heaviest vertices, whose runs are sorted by
+// destination) the CAS retries and the hardware float atomic is used.
 __device__ inline void lds_add_cas(float *p, float x) {
     unsigned *a = reinterpret_cast<unsigned *>(p);
     unsigned old = *a, assumed;
@@ -317,14 +376,20 @@ __global__ void __launch_bounds__(PB_WG) k_pb_accum64(const PbUnit2 *__restrict_
                 if (e < u.e0 || e >= u.e1 || d < lo || d >= lo + H) continue;
                 const double x = (double)__uint_as_float(vq[i >> 2][i & 3]);
                 if (d != cur) {
-                    if (cur != 0xFFFFFFFFu) lds_add_cas64(&sum[cur - lo], run);
+                    if (cur != 0xFFFFFFFFu) {
+                        if (u.dense) atomicAdd(&sum[cur - lo], run);
+                        else lds_add_cas64(&sum[cur - lo], run);
+                    }
                     cur = d;
                     run = x;
                 } else {
                     run += x;
                 }
             }
-            if (cur != 0xFFFFFFFFu) lds_add_cas64(&sum[cur - lo], run);
+            if (cur != 0xFFFFFFFFu) {
+                if (u.dense) atomicAdd(&sum[cur - lo], run);
+                else lds_add_cas64(&sum[cur - lo], run);
+            }
         }
         __syncthreads();
         const uint32_t hn = std::min(H, qn - lo);
@@ -371,21 +436,25 @@ __global__ void __launch_bounds__(256) k_pb_reduce(const PbRed *__restrict__ red
 // The unit tables must tile the entry ranges exactly and stay within the
 // LDS tables of the kernels: checked on the host before the first launch.
 static int check_units(const std::vector<PbUnit1> &u1, const std::vector<PbUnit2> &u2,
-                       const std::vector<PbRed> &red, const std::vector<uint64_t> &rpm, uint64_t total,
+                       const std::vector<PbRed> &red, const std::vector<uint64_t> &rpw, uint64_t total,
                        uint32_t runs_max, uint32_t slots) {
-    uint64_t pm = 0;
-    const uint64_t nr = rpm.size() - 1;
+    uint64_t pw = 0;
+    const uint64_t nr = rpw.size() - 1;
     for (const PbUnit1 &x : u1) {
-        const bool ok = x.pm0 == pm && x.pm1 > x.pm0 && x.r0 < x.r1 && x.r1 <= nr && x.r1 - x.r0 <= runs_max &&
-                        x.r1 - x.r0 <= PB_RUNS && rpm[x.r0] <= x.pm0 && x.pm0 < rpm[x.r0 + 1] &&
-                        rpm[x.r1 - 1] < x.pm1 && x.pm1 <= rpm[x.r1] && x.pm1 - x.pm0 < (1ull << 32);
-        if (!ok) { set_error("blocked index: bad gather unit at entry %llu", (unsigned long long)pm); return GS_ERR_FORMAT; }
-        pm = x.pm1;
+        const bool ok = x.w0 == pw && x.w1 > x.w0 && x.r0 < x.r1 && x.r1 <= nr && x.r1 - x.r0 <= runs_max &&
+                        x.r1 - x.r0 <= PB_RUNS && rpw[x.r0] <= x.w0 && x.w0 < rpw[x.r0 + 1] &&
+                        rpw[x.r1 - 1] < x.w1 && x.w1 <= rpw[x.r1] && x.w1 - x.w0 < (1ull << 31);
+        if (!ok) { set_error("blocked index: bad gather unit at window %llu", (unsigned long long)pw); return GS_ERR_FORMAT; }
+        pw = x.w1;
     }
-    if (pm != total) { set_error("blocked index: gather units end at %llu", (unsigned long long)pm); return GS_ERR_FORMAT; }
+    if (pw * PB_WIN != total) {
+        set_error("blocked index: gather units end at window %llu", (unsigned long long)pw);
+        return GS_ERR_FORMAT;
+    }
     uint64_t e = 0;
     for (const PbUnit2 &x : u2) {
-        if (x.e0 != e || x.e1 <= x.e0 || (x.slot != 0xFFFFFFFFu && x.slot >= slots)) {
+        if (x.e0 != e || x.e1 <= x.e0 || (x.e0 & (PB_WIN - 1)) || (x.e1 & (PB_WIN - 1)) ||
+            (x.slot != 0xFFFFFFFFu && x.slot >= slots)) {
             set_error("blocked index: bad accum unit at entry %llu", (unsigned long long)e);
             return GS_ERR_FORMAT;
         }
@@ -398,8 +467,8 @@ static int check_units(const std::vector<PbUnit1> &u1, const std::vector<PbUnit2
 }
 
 size_t pb_bytes(const gs_graph *g) {
-    return g->pb_src.bytes() + g->pb_dst.bytes() + g->pb_val.bytes() + g->pb_run_pm.bytes() +
-           g->pb_run_qm.bytes() + g->pb_u1.bytes() + g->pb_u2.bytes() + g->pb_red.bytes() + g->pb_part.bytes() +
+    return g->pb_src.bytes() + g->pb_dst.bytes() + g->pb_val.bytes() + g->pb_run_pw.bytes() +
+           g->pb_run_qw.bytes() + g->pb_u1.bytes() + g->pb_u2.bytes() + g->pb_red.bytes() + g->pb_part.bytes() +
            g->pb_part64.bytes();
 }
 
@@ -453,15 +522,20 @@ int build_blocked(gs_graph *g) {
     std::vector<unsigned long long> c(KK);
     GS_HIP(hipMemcpyAsync(c.data(), cnt.ptr, KK * 8, hipMemcpyDeviceToHost, st));
     GS_HIP(hipStreamSynchronize(st));
-    // 3. run starts: q-major positions, and the non-empty runs in source-major order
+    // 3. run starts: q-major positions (whole windows), and the non-empty runs
+    //    in source-major order with their window prefix and q-major window
     std::vector<unsigned long long> qm(KK);
-    unsigned long long run = 0;
-    for (uint64_t i = 0; i < KK; ++i) { qm[i] = run; run += c[i]; }
-    if (run != 2 * g->E) {
-        set_error("blocked index: %llu entries, expected %llu", run, (unsigned long long)(2 * g->E));
+    unsigned long long run = 0, real = 0;
+    for (uint64_t i = 0; i < KK; ++i) {
+        qm[i] = run;
+        run += (c[i] + PB_WIN - 1) & ~(unsigned long long)(PB_WIN - 1);
+        real += c[i];
+    }
+    if (real != 2 * g->E) {
+        set_error("blocked index: %llu entries, expected %llu", real, (unsigned long long)(2 * g->E));
         return GS_ERR_FORMAT;
     }
-    std::vector<uint64_t> rpm, rqm;
+    std::vector<uint64_t> rpw, rqw;
     std::vector<uint32_t> prun(K + 1);
     {
         // source-major walk of a q-major table, in 64 x 64 tiles for the host caches
@@ -474,57 +548,58 @@ int build_blocked(gs_graph *g) {
                         tmp_c[(uint64_t)p * K + q] = c[(uint64_t)q * K + p];
                         tmp_q[(uint64_t)p * K + q] = qm[(uint64_t)q * K + p];
                     }
-        uint64_t pm = 0;
+        uint64_t pw = 0;
         for (uint32_t p = 0; p < K; ++p) {
-            prun[p] = (uint32_t)rpm.size();
+            prun[p] = (uint32_t)rpw.size();
             for (uint32_t q = 0; q < K; ++q) {
                 const uint64_t n_ = tmp_c[(uint64_t)p * K + q];
                 if (!n_) continue;
-                rpm.push_back(pm);
-                rqm.push_back(tmp_q[(uint64_t)p * K + q]);
-                pm += n_;
+                rpw.push_back(pw);
+                rqw.push_back(tmp_q[(uint64_t)p * K + q] / PB_WIN);
+                pw += (n_ + PB_WIN - 1) / PB_WIN;
             }
         }
-        prun[K] = (uint32_t)rpm.size();
-        rpm.push_back(pm);
+        prun[K] = (uint32_t)rpw.size();
+        rpw.push_back(pw);
     }
-    const uint64_t nruns = rqm.size();
-    const uint64_t unit1 = g->pb_unit1, unit2 = g->pb_unit2;
+    const uint64_t nruns = rqw.size();
+    const uint64_t unit1 = std::max<uint64_t>(g->pb_unit1 / PB_WIN, 1), unit2 = g->pb_unit2;
     const uint32_t runs_max = g->pb_runs;
-    // 4. gather units: source-major entry ranges within one source block, at
+    // 4. gather units: source-major window ranges within one source block, at
     //    most pb_unit1 entries and pb_runs runs each
     std::vector<PbUnit1> u1;
     for (uint32_t p = 0; p < K; ++p) {
         uint32_t r = prun[p];
         const uint32_t rend = prun[p + 1];
-        uint64_t pm = rpm[r];
-        const uint64_t pend = rpm[rend];
-        while (pm < pend) {
+        uint64_t pw = rpw[r];
+        const uint64_t pend = rpw[rend];
+        while (pw < pend) {
             PbUnit1 x{};
             x.p = p;
-            x.pm0 = pm;
+            x.w0 = pw;
             x.r0 = r;
-            uint64_t lim = std::min(pm + unit1, pend);
-            if (r + runs_max < rend && rpm[r + runs_max] < lim) lim = rpm[r + runs_max];
-            x.pm1 = lim;
-            // runs overlapping [pm, lim): r .. the run holding lim - 1
+            uint64_t lim = std::min(pw + unit1, pend);
+            if (r + runs_max < rend && rpw[r + runs_max] < lim) lim = rpw[r + runs_max];
+            x.w1 = lim;
+            // runs overlapping [pw, lim): r .. the run holding lim - 1
             uint32_t rl = r;
-            while (rl + 1 < rend && rpm[rl + 1] <= lim - 1) ++rl;
+            while (rl + 1 < rend && rpw[rl + 1] <= lim - 1) ++rl;
             x.r1 = rl + 1;
             u1.push_back(x);
-            pm = lim;
-            r = (rl + 1 < rend && rpm[rl + 1] <= pm) ? rl + 1 : rl;
+            pw = lim;
+            r = (rl + 1 < rend && rpw[rl + 1] <= pw) ? rl + 1 : rl;
         }
     }
     // 5. accum units: each destination block in equal parts of at most
-    //    pb_unit2 entries; split blocks get partial slots
+    //    pb_unit2 entries, cut on window boundaries; split blocks get partial slots
     std::vector<PbUnit2> u2;
     std::vector<PbRed> red;
     uint32_t slots = 0;
     for (uint32_t q = 0; q < K; ++q) {
         const uint64_t e0 = qm[(uint64_t)q * K], e1 = q + 1 < K ? qm[(uint64_t)(q + 1) * K] : run;
         if (e1 == e0) continue;
-        const uint64_t parts = (e1 - e0 + unit2 - 1) / unit2;
+        const uint64_t nw = (e1 - e0) / PB_WIN;
+        const uint64_t parts = std::min<uint64_t>((e1 - e0 + unit2 - 1) / unit2, nw);
         // entries per vertex of the block: above PB_DENSE, lanes of a wave mostly share destinations
         const uint64_t qn = std::min<uint64_t>(PB_B, nzn - ((uint64_t)q << PB_SHIFT));
         const uint32_t dense = (e1 - e0) >= (uint64_t)g->pb_dense * qn;
@@ -534,27 +609,32 @@ int build_blocked(gs_graph *g) {
         }
         red.push_back(PbRed{q, slots, (uint32_t)parts, 0});
         for (uint64_t k = 0; k < parts; ++k)
-            u2.push_back(PbUnit2{e0 + (e1 - e0) * k / parts, e0 + (e1 - e0) * (k + 1) / parts, q, slots++, dense, 0});
+            u2.push_back(PbUnit2{e0 + nw * k / parts * PB_WIN, e0 + nw * (k + 1) / parts * PB_WIN, q, slots++, dense, 0});
     }
-    GS_TRY(check_units(u1, u2, red, rpm, run, runs_max, slots));
-    GS_TRY(g->pb_src.alloc(2 * g->E));
-    GS_TRY(g->pb_dst.alloc(2 * g->E + 16));  // padding: the accumulation reads whole 16-entry groups
-    GS_TRY(g->pb_val.alloc(2 * g->E + 16));
-    GS_TRY(g->pb_run_pm.alloc(rpm.size()));
-    GS_TRY(g->pb_run_qm.alloc(std::max<uint64_t>(nruns, 1)));
+    GS_TRY(check_units(u1, u2, red, rpw, run, runs_max, slots));
+    // padding: the accumulation reads whole 16-entry groups from any unit start
+    GS_TRY(g->pb_src.alloc(run + PB_WIN));
+    GS_TRY(g->pb_dst.alloc(run + PB_WIN));
+    GS_TRY(g->pb_val.alloc(run + PB_WIN));
+    GS_TRY(g->pb_run_pw.alloc(rpw.size()));
+    GS_TRY(g->pb_run_qw.alloc(std::max<uint64_t>(nruns, 1)));
     GS_TRY(g->pb_u1.alloc(u1.size()));
     GS_TRY(g->pb_u2.alloc(u2.size()));
     GS_TRY(g->pb_red.alloc(red.size()));
     GS_TRY(g->pb_part.alloc((uint64_t)slots * PB_B));
-    // 6. the copy, and the unit tables
+    // 6. the copy, the padding of each run's last window, and the unit tables
     GS_HIP(hipMemcpyAsync(cnt.ptr, qm.data(), KK * 8, hipMemcpyHostToDevice, st));
     if (!slices.empty()) {
         k_pb_fill<<<(unsigned)slices.size(), PB_WG, 0, st>>>(d_sl.ptr, g->adj_off.ptr, g->adj_nbr.ptr, nzn, K,
                                                              cnt.ptr, g->pb_src.ptr, g->pb_dst.ptr);
         GS_HIP(hipGetLastError());
+        k_pb_pad<<<vgrid(KK), 256, 0, st>>>(cnt.ptr, KK, g->pb_src.ptr, g->pb_dst.ptr);
+        GS_HIP(hipGetLastError());
     }
-    GS_HIP(hipMemcpyAsync(g->pb_run_pm.ptr, rpm.data(), rpm.size() * 8, hipMemcpyHostToDevice, st));
-    if (nruns) GS_HIP(hipMemcpyAsync(g->pb_run_qm.ptr, rqm.data(), nruns * 8, hipMemcpyHostToDevice, st));
+    GS_HIP(hipMemsetAsync(g->pb_src.ptr + run, 0, PB_WIN * 2, st));
+    GS_HIP(hipMemsetAsync(g->pb_dst.ptr + run, 0, PB_WIN * 2, st));
+    GS_HIP(hipMemcpyAsync(g->pb_run_pw.ptr, rpw.data(), rpw.size() * 8, hipMemcpyHostToDevice, st));
+    if (nruns) GS_HIP(hipMemcpyAsync(g->pb_run_qw.ptr, rqw.data(), nruns * 8, hipMemcpyHostToDevice, st));
     if (!u1.empty())
         GS_HIP(hipMemcpyAsync(g->pb_u1.ptr, u1.data(), u1.size() * sizeof(PbUnit1), hipMemcpyHostToDevice, st));
     if (!u2.empty())
@@ -572,6 +652,7 @@ int build_blocked(gs_graph *g) {
     g->pb_nu2 = u2.size();
     g->pb_nred = red.size();
     g->pb_nruns = nruns;
+    g->pb_entries = run;
     g->pb_ms = ms;
     g->pb_ready = true;
     return GS_OK;
@@ -581,7 +662,7 @@ static int pb_gather(const gs_graph *g, gs_ws *w, const float *prior) {
     if (!g->pb_nu1) return GS_OK;
     hipEvent_t kb;
     w->timer.begin(GS_K_PB_GATHER, w->stream, &kb);
-    k_pb_gather<<<(unsigned)g->pb_nu1, PB_WG, 0, w->stream>>>(g->pb_u1.ptr, g->pb_run_pm.ptr, g->pb_run_qm.ptr,
+    k_pb_gather<<<(unsigned)g->pb_nu1, PB_WG, 0, w->stream>>>(g->pb_u1.ptr, g->pb_run_pw.ptr, g->pb_run_qw.ptr,
                                                               g->pb_src.ptr, prior, g->nzn, g->pb_val.ptr);
     GS_HIP(hipGetLastError());
     w->timer.end(GS_K_PB_GATHER, w->stream, kb);

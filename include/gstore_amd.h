@@ -261,8 +261,11 @@ int gs_set_kcore_method(gs_graph *g, int method);
  * runs (<= 2048) per gather workgroup, entries per accumulation workgroup
  * (<= 2^30), and the entries per vertex from which a destination block adds
  * with LDS float atomics rather than compare-and-swap.  Defaults 2^20, 2048,
- * 2^20, 2^30 (compare-and-swap throughout: measured faster on Kron-28).  Small values exercise the split paths on small graphs; results
- * change only in FP32 summation order.  No counterpart in the reference. */
+ * 2^20, 8192 (on Kron-28 the hardware atomic wins only on the blocks of the
+ * heaviest vertices, whose sorted runs put one destination in many lanes of
+ * a wave).  Small values exercise the split paths on small graphs; results
+ * change only in FP32 (FP64 for personalised PageRank) summation order.  No
+ * counterpart in the reference. */
 int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries,
                              uint32_t dense_entries);
 
