@@ -115,17 +115,30 @@ __global__ void __launch_bounds__(256) k_msbfs_level_end(unsigned long long *vis
 // 64-byte row per vertex) and mapped back to vertex order at the end.
 constexpr uint32_t MS_CHUNK = 2048;
 
-__global__ void k_ms_roots(const uint64_t *__restrict__ roots, int nr, const uint32_t *__restrict__ new_id,
-                           const int32_t *__restrict__ deg, unsigned long long *vis, unsigned long long *front,
-                           uint32_t *queue, int32_t *qdeg, uint8_t *depth_vm) {
+// One wave.  Roots without edges go to the end of the first queue: zero-width
+// entries there are never inside a top-down chunk's range, so a chunk stages
+// at most one entry per edge position (plus one).
+__global__ void __launch_bounds__(64) k_ms_roots(const uint64_t *__restrict__ roots, int nr,
+                                                 const uint32_t *__restrict__ new_id, const int32_t *__restrict__ deg,
+                                                 unsigned long long *vis, unsigned long long *front, uint32_t *queue,
+                                                 int32_t *qdeg, uint8_t *depth_vm) {
     const int i = threadIdx.x;
-    if (i >= nr) return;
-    const uint32_t r = new_id[roots[i]];
-    atomicOr(&vis[r], 1ull << i);
-    atomicOr(&front[r], 1ull << i);
-    queue[i] = r;
-    qdeg[i] = deg[r];
-    if (depth_vm) depth_vm[(uint64_t)r * 64 + i] = 1;
+    const bool in = i < nr;
+    uint32_t r = 0;
+    int32_t d = 0;
+    if (in) {
+        r = new_id[roots[i]];
+        d = deg[r];
+        atomicOr(&vis[r], 1ull << i);
+        atomicOr(&front[r], 1ull << i);
+        if (depth_vm) depth_vm[(uint64_t)r * 64 + i] = 1;
+    }
+    const unsigned long long with_edges = __ballot(in && d > 0), without = __ballot(in && d <= 0);
+    if (!in) return;
+    const unsigned long long below = (1ull << i) - 1;
+    const int pos = d > 0 ? __popcll(with_edges & below) : __popcll(with_edges) + __popcll(without & below);
+    queue[pos] = r;
+    qdeg[pos] = d;
 }
 
 __global__ void __launch_bounds__(256) k_ms_td(const uint32_t *__restrict__ queue, const uint64_t *__restrict__ qoff,
@@ -213,11 +226,10 @@ __global__ void __launch_bounds__(256) k_ms_bu(uint64_t nzn, unsigned long long 
 // BUILD lists the new frontier (queue, qdeg; cursor count[2]); DEPTH writes
 // depth rows.
 template <bool BUILD, bool DEPTH>
-__global__ void __launch_bounds__(256) k_ms_level_end(uint64_t nzn, unsigned long long all, unsigned long long *vis,
-                                                      unsigned long long *front, unsigned long long *next,
-                                                      const int32_t *__restrict__ deg, uint8_t *depth_vm,
-                                                      uint32_t newdepth, uint32_t *queue, int32_t *qdeg,
-                                                      unsigned long long *count, unsigned long long *lane_counts) {
+__device__ inline void ms_level_end(uint64_t nzn, unsigned long long all, unsigned long long *vis,
+                                    unsigned long long *front, unsigned long long *next, const int32_t *__restrict__ deg,
+                                    uint8_t *depth_vm, uint32_t newdepth, uint32_t *queue, int32_t *qdeg,
+                                    unsigned long long *count, unsigned long long *lane_counts) {
     __shared__ uint32_t cnt[64];
     __shared__ uint32_t wsum[4];
     __shared__ unsigned long long bbase, bsum[3];
@@ -289,6 +301,15 @@ __global__ void __launch_bounds__(256) k_ms_level_end(uint64_t nzn, unsigned lon
     }
 }
 
+template <bool BUILD, bool DEPTH>
+__global__ void __launch_bounds__(256) k_ms_level_end(uint64_t nzn, unsigned long long all, unsigned long long *vis,
+                                                      unsigned long long *front, unsigned long long *next,
+                                                      const int32_t *__restrict__ deg, uint8_t *depth_vm,
+                                                      uint32_t newdepth, uint32_t *queue, int32_t *qdeg,
+                                                      unsigned long long *count, unsigned long long *lane_counts) {
+    ms_level_end<BUILD, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, newdepth, queue, qdeg, count, lane_counts);
+}
+
 // Lists the current frontier (vertices with a non-zero mask) for a top-down level.
 __global__ void __launch_bounds__(256) k_ms_list(uint64_t nzn, const unsigned long long *__restrict__ front,
                                                  const int32_t *__restrict__ deg, uint32_t *queue, int32_t *qdeg,
@@ -331,6 +352,295 @@ __global__ void k_ms_unpermute(const uint8_t *__restrict__ depth_vm, const uint3
 }
 
 static unsigned ms_grid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+
+// ---- the same levels, driven from the device (one hipGraph per workspace) -------
+// The host loop above reads three counters after every level to choose the
+// direction and to stop.  Here those decisions are kernels: per level, a
+// one-thread kernel picks the direction from the previous level's counters
+// and raises flags; the list, queue-prefix, top-down, bottom-up and level-end
+// kernels return at once when their flag is down (or the search is done); a
+// 64-thread kernel closes the level.  MSG_LEVELS levels are captured once into
+// a hipGraph, which the host launches until the done flag is up, so a batch
+// costs one host round trip per MSG_LEVELS levels instead of one per level.
+// Control words, u64, in w->ms_ctl (the first four are the level-end counters
+// in the order k_ms_level_end adds them):
+enum : int {
+    C_NF = 0,      // frontier vertices after the last level end
+    C_MF = 1,      // their degree sum
+    C_BUILD = 2,   // queue cursor of the level end
+    C_MU = 3,      // degree sum of vertices still missing lanes
+    C_CHECKED = 4, // entries examined, all levels
+    C_M = 5,       // top-down: the queue's degree sum (edges this level)
+    C_LEVEL = 6,   // current level (1 = roots)
+    C_FLAGS = 7,
+    C_ACTIVE = 8,  // lanes whose frontier is not empty
+    C_ALL = 9,     // lanes in use
+    C_NQ = 10,     // queue length this level
+    C_ALPHA = 11,
+    C_BETA = 12,
+    C_NLEV = 13,   // levels done
+    C_LIST = 14,   // queue cursor of the frontier listing
+    C_LANES = 16,  // per-level lane counts, 64 words per level
+    C_WORDS = C_LANES + 64 * 256
+};
+enum : unsigned long long { F_BU = 1, F_HAVEQ = 2, F_DONE = 4, F_TD = 8, F_LIST = 16, F_OVERFLOW = 32 };
+constexpr int MSG_LEVELS = 4;
+constexpr unsigned MSG_TD_GRID = 2048;
+
+__global__ void k_msg_init(unsigned long long *ctl, int nr, const int32_t *__restrict__ qdeg, uint64_t m_u,
+                           uint32_t alpha, uint32_t beta) {
+    if (threadIdx.x != 0) return;
+    unsigned long long mf = 0;
+    for (int i = 0; i < nr; ++i) mf += (uint32_t)qdeg[i];
+    const unsigned long long all = nr == 64 ? ~0ull : ((1ull << nr) - 1);
+    ctl[C_NF] = (unsigned long long)nr;
+    ctl[C_MF] = mf;
+    ctl[C_MU] = m_u;
+    ctl[C_LEVEL] = 1;
+    ctl[C_FLAGS] = F_HAVEQ;
+    ctl[C_ACTIVE] = all;
+    ctl[C_ALL] = all;
+    ctl[C_ALPHA] = alpha;
+    ctl[C_BETA] = beta;
+}
+
+__global__ void k_msg_decide(unsigned long long *ctl, uint64_t nzn) {
+    const unsigned long long f = ctl[C_FLAGS];
+    if (f & F_DONE) return;
+    const unsigned long long nf = ctl[C_NF], mf = ctl[C_MF], mu = ctl[C_MU];
+    bool bu = f & F_BU;
+    if (!bu && mf > mu / ctl[C_ALPHA]) bu = true;
+    else if (bu && nf < nzn / ctl[C_BETA]) bu = false;
+    const bool td = !bu && nf && mf;
+    const bool list = td && !(f & F_HAVEQ);
+    ctl[C_NQ] = nf;
+    ctl[C_NF] = ctl[C_MF] = ctl[C_MU] = 0;
+    ctl[C_BUILD] = ctl[C_LIST] = ctl[C_M] = 0;
+    // the level end lists the new frontier (the next level's queue) unless bottom-up
+    ctl[C_FLAGS] = (bu ? F_BU : F_HAVEQ) | (td ? F_TD : 0) | (list ? F_LIST : 0);
+}
+
+__global__ void __launch_bounds__(256) k_msg_list(const unsigned long long *ctl, uint64_t nzn,
+                                                  const unsigned long long *__restrict__ front,
+                                                  const int32_t *__restrict__ deg, uint32_t *queue, int32_t *qdeg,
+                                                  unsigned long long *cursor) {
+    if ((ctl[C_FLAGS] & (F_LIST | F_DONE)) != F_LIST) return;
+    __shared__ uint32_t wsum[4];
+    __shared__ unsigned long long bbase;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint64_t base = blockIdx.x * 256ull; base < nzn; base += (uint64_t)gridDim.x * 256) {
+        const uint64_t v = base + threadIdx.x;
+        const bool in = v < nzn && front[v] != 0;
+        const unsigned long long mask = __ballot(in);
+        if (lane == 0) wsum[wv] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            bbase = t ? atomicAdd(cursor, (unsigned long long)t) : 0ull;
+        }
+        __syncthreads();
+        if (in) {
+            uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+            for (int j = 0; j < wv; ++j) before += wsum[j];
+            queue[bbase + before] = (uint32_t)v;
+            qdeg[bbase + before] = deg[v];
+        }
+        __syncthreads();
+    }
+}
+
+// qsum[b] = degree sum of queue entries [1024 b, 1024 b + 1024)
+__global__ void __launch_bounds__(256) k_msg_block_sums(const unsigned long long *ctl, const int32_t *__restrict__ qdeg,
+                                                        unsigned long long *qsum) {
+    if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
+    const uint64_t nq = ctl[C_NQ], base = (uint64_t)blockIdx.x * 1024;
+    if (base >= nq) return;
+    __shared__ unsigned long long s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    unsigned long long c = 0;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x)
+        if (base + i < nq) c += (uint32_t)qdeg[base + i];
+    atomicAdd(&s, c);
+    __syncthreads();
+    if (threadIdx.x == 0) qsum[blockIdx.x] = s;
+}
+
+// Exclusive prefix of qsum[0 .. nb) in place (one workgroup); the total is the
+// level's top-down edge count.
+__global__ void __launch_bounds__(1024) k_msg_scan(unsigned long long *ctl, unsigned long long *qsum) {
+    if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
+    __shared__ unsigned long long part[1024];
+    const uint64_t nb = (ctl[C_NQ] + 1023) / 1024;
+    const uint64_t per = (nb + 1023) / 1024, b0 = threadIdx.x * per, b1 = std::min<uint64_t>(b0 + per, nb);
+    unsigned long long s = 0;
+    for (uint64_t b = b0; b < b1; ++b) s += qsum[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int step = 1; step < 1024; step <<= 1) {
+        const unsigned long long v = threadIdx.x >= (unsigned)step ? part[threadIdx.x - step] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = part[threadIdx.x] - s;   // exclusive
+    for (uint64_t b = b0; b < b1; ++b) {
+        const unsigned long long c = qsum[b];
+        qsum[b] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) ctl[C_M] = part[1023];
+}
+
+// qoff[j] = exclusive prefix of qdeg over the queue (blocks of 1024, as k_deg_scan)
+__global__ void __launch_bounds__(256) k_msg_deg_scan(const unsigned long long *ctl, const int32_t *__restrict__ qdeg,
+                                                      const unsigned long long *__restrict__ bstart, uint64_t *qoff) {
+    if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
+    const uint64_t nq = ctl[C_NQ], base = (uint64_t)blockIdx.x * 1024;
+    if (base >= nq) return;
+    __shared__ uint32_t d[1024];
+    __shared__ unsigned long long part[256];
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) d[i] = base + i < nq ? (uint32_t)qdeg[base + i] : 0;
+    __syncthreads();
+    unsigned long long loc[4], run = 0;
+    for (int j = 0; j < 4; ++j) {
+        loc[j] = run;
+        run += d[threadIdx.x * 4 + j];
+    }
+    part[threadIdx.x] = run;
+    __syncthreads();
+    for (int step = 1; step < 256; step <<= 1) {
+        const unsigned long long v = threadIdx.x >= (unsigned)step ? part[threadIdx.x - step] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const unsigned long long before = (threadIdx.x ? part[threadIdx.x - 1] : 0) + bstart[blockIdx.x];
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t v = base + threadIdx.x * 4 + j;
+        if (v < nq) qoff[v] = before + loc[j];
+    }
+}
+
+// Top-down over the queue's edges, MS_CHUNK per workgroup step (grid-stride:
+// the grid is fixed when the level sequence is captured).
+__global__ void __launch_bounds__(256) k_msg_td(const unsigned long long *ctl, const uint32_t *__restrict__ queue,
+                                                const uint64_t *__restrict__ qoff, const uint64_t *__restrict__ off,
+                                                const uint32_t *__restrict__ nbr,
+                                                const unsigned long long *__restrict__ front,
+                                                const unsigned long long *__restrict__ vis, unsigned long long *next) {
+    if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
+    __shared__ uint32_t sbeg[MS_CHUNK + 2];
+    __shared__ uint64_t sadj[MS_CHUNK + 1];
+    __shared__ unsigned long long sfr[MS_CHUNK + 1];
+    __shared__ uint64_t j_first;
+    __shared__ int nstaged;
+    const uint64_t nq = ctl[C_NQ], m = ctl[C_M];
+    for (uint64_t c0 = blockIdx.x * (uint64_t)MS_CHUNK; c0 < m; c0 += (uint64_t)gridDim.x * MS_CHUNK) {
+        const uint32_t len = (uint32_t)min<uint64_t>(MS_CHUNK, m - c0);
+        if (threadIdx.x == 0) {
+            uint64_t lo = 0, hi = nq - 1;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+            }
+            const uint64_t ja = lo;
+            hi = nq - 1;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
+            }
+            j_first = ja;
+            nstaged = (int)(lo - ja + 1);
+        }
+        __syncthreads();
+        const uint64_t ja = j_first;
+        const int ns = nstaged;
+        for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+            const uint64_t q = qoff[ja + k];
+            const uint32_t v = queue[ja + k];
+            sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
+            sadj[k] = off[v] + (q < c0 ? c0 - q : 0);
+            sfr[k] = front[v];
+        }
+        if (threadIdx.x == 0) sbeg[ns] = len;
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
+            int lo = 0, hi = ns - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t u = nbr[sadj[lo] + (t - sbeg[lo])];
+            const unsigned long long mm = sfr[lo] & ~vis[u];
+            if (mm && (mm & ~next[u])) atomicOr(&next[u], mm);
+        }
+        __syncthreads();   // the staged tables are rewritten by the next step
+    }
+}
+
+__global__ void __launch_bounds__(256) k_msg_bu(unsigned long long *ctl, uint64_t nzn, const uint64_t *__restrict__ off,
+                                                const uint32_t *__restrict__ nbr,
+                                                const unsigned long long *__restrict__ vis,
+                                                const unsigned long long *__restrict__ front,
+                                                unsigned long long *next) {
+    if ((ctl[C_FLAGS] & (F_BU | F_DONE)) != F_BU) return;
+    const unsigned long long all = ctl[C_ACTIVE];
+    uint32_t ck = 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nzn; v += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long need = all & ~vis[v];
+        if (!need) continue;
+        const uint64_t b = off[v], e = off[v + 1];
+        unsigned long long acc = 0;
+        for (uint64_t k = b; k < e && acc != need; k += 4) {
+            uint32_t u[4];
+            unsigned long long f[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) u[j] = nbr[min(k + j, e - 1)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = front[u[j]];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k + j < e) acc |= f[j] & need;
+            ck += (uint32_t)min<uint64_t>(4, e - k);
+        }
+        if (acc) next[v] = acc;
+    }
+    if (ck) atomicAdd(&ctl[C_CHECKED], (unsigned long long)ck);
+}
+
+template <bool DEPTH>
+__global__ void __launch_bounds__(256) k_msg_level_end(unsigned long long *ctl, uint64_t nzn, unsigned long long *vis,
+                                                       unsigned long long *front, unsigned long long *next,
+                                                       const int32_t *__restrict__ deg, uint8_t *depth_vm,
+                                                       uint32_t *queue, int32_t *qdeg) {
+    const unsigned long long f = ctl[C_FLAGS];
+    if (f & F_DONE) return;
+    const uint32_t level = (uint32_t)ctl[C_LEVEL];
+    const unsigned long long all = ctl[C_ALL];
+    unsigned long long *lanes = ctl + C_LANES + 64ull * (level - 1);
+    if (f & F_HAVEQ)
+        ms_level_end<true, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
+    else
+        ms_level_end<false, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
+}
+
+// Closes a level: lanes still active, entries examined, and the stop test
+// (no lane reached a new vertex, or the next depth would not fit in u8).
+__global__ void __launch_bounds__(64) k_msg_finish(unsigned long long *ctl) {
+    const unsigned long long f = ctl[C_FLAGS];
+    if (f & F_DONE) return;
+    const uint32_t level = (uint32_t)ctl[C_LEVEL];
+    const unsigned long long c = ctl[C_LANES + 64ull * (level - 1) + threadIdx.x];
+    const unsigned long long active = __ballot(c != 0);
+    if (threadIdx.x != 0) return;
+    ctl[C_ACTIVE] = active;
+    if (f & F_TD) ctl[C_CHECKED] += ctl[C_M];
+    ctl[C_NLEV] += 1;
+    if (!active) ctl[C_FLAGS] = f | F_DONE;
+    else if (level + 2 > 256) ctl[C_FLAGS] = f | F_DONE | F_OVERFLOW;
+    else ctl[C_LEVEL] = level + 1;
+}
 
 }  // namespace gs
 
@@ -441,6 +751,110 @@ static int run_bfs_batch_index(const gs_graph *gc, gs_ws *w, const uint64_t *roo
     return status;
 }
 
+// The index method with its level loop on the device: the MSG_LEVELS-level
+// sequence is captured once per workspace (again when a buffer it names has
+// moved) and launched until the done flag is up.
+static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roots, int nr, uint64_t *level_counts,
+                               int max_levels, int *levels, bool keep_depth) {
+    gs_graph *g = const_cast<gs_graph *>(gc);
+    const uint64_t n = g->n, nzn = g->nzn;
+    unsigned long long *vis = (unsigned long long *)w->ms_vis.ptr, *front = (unsigned long long *)w->ms_front.ptr,
+                       *next = (unsigned long long *)w->ms_next.ptr;
+    uint8_t *dvm = keep_depth ? w->ms_depth_vm.ptr : nullptr;
+    if (w->ms_ctl.n != C_WORDS) GS_TRY(w->ms_ctl.alloc(C_WORDS));
+    unsigned long long *ctl = w->ms_ctl.ptr;
+    GS_HIP(hipMemcpyAsync(w->ms_roots.ptr, roots, nr * 8, hipMemcpyHostToDevice, w->stream));
+    GS_HIP(hipMemsetAsync(vis, 0, n * 8, w->stream));
+    GS_HIP(hipMemsetAsync(front, 0, n * 8, w->stream));
+    GS_HIP(hipMemsetAsync(next, 0, n * 8, w->stream));
+    if (dvm) GS_HIP(hipMemsetAsync(dvm, 0, n * 64, w->stream));
+    GS_HIP(hipMemsetAsync(ctl, 0, C_WORDS * 8, w->stream));
+    k_ms_roots<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->new_id.ptr, g->deg_r.ptr, vis, front, w->queue.ptr,
+                                        w->qdeg.ptr, dvm);
+    GS_HIP(hipGetLastError());
+    k_msg_init<<<1, 64, 0, w->stream>>>(ctl, nr, w->qdeg.ptr, 2 * g->E, (uint32_t)g->do_alpha, (uint32_t)g->do_beta);
+    GS_HIP(hipGetLastError());
+    // what the captured kernels were given (the thresholds travel in ctl)
+    const std::vector<const void *> key = {g, (const void *)(uintptr_t)nzn, vis, front, next, dvm, ctl,
+                                           w->queue.ptr, w->qdeg.ptr, w->qoff.ptr, w->qsum.ptr, g->adj_off.ptr,
+                                           g->adj_nbr.ptr, g->deg_r.ptr};
+    if (!w->ms_graph || w->ms_graph_key != key) {
+        if (w->ms_graph) {
+            GS_HIP(hipGraphExecDestroy(w->ms_graph));
+            w->ms_graph = nullptr;
+        }
+        GS_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+        const unsigned gv = nzn ? ms_grid(nzn) : 1, gb = (unsigned)std::max<uint64_t>((nzn + 1023) / 1024, 1);
+        for (int l = 0; l < MSG_LEVELS; ++l) {
+            k_msg_decide<<<1, 1, 0, w->stream>>>(ctl, nzn);
+            k_msg_list<<<gv, 256, 0, w->stream>>>(ctl, nzn, front, g->deg_r.ptr, w->queue.ptr, w->qdeg.ptr,
+                                                  ctl + C_LIST);
+            k_msg_block_sums<<<gb, 256, 0, w->stream>>>(ctl, w->qdeg.ptr, w->qsum.ptr);
+            k_msg_scan<<<1, 1024, 0, w->stream>>>(ctl, w->qsum.ptr);
+            k_msg_deg_scan<<<gb, 256, 0, w->stream>>>(ctl, w->qdeg.ptr, w->qsum.ptr, w->qoff.ptr);
+            k_msg_td<<<MSG_TD_GRID, 256, 0, w->stream>>>(ctl, w->queue.ptr, w->qoff.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                         front, vis, next);
+            k_msg_bu<<<gv, 256, 0, w->stream>>>(ctl, nzn, g->adj_off.ptr, g->adj_nbr.ptr, vis, front, next);
+            if (dvm)
+                k_msg_level_end<true><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr, dvm,
+                                                                 w->queue.ptr, w->qdeg.ptr);
+            else
+                k_msg_level_end<false><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr, dvm,
+                                                                  w->queue.ptr, w->qdeg.ptr);
+            k_msg_finish<<<1, 64, 0, w->stream>>>(ctl);
+        }
+        const hipError_t launch = hipGetLastError();
+        hipGraph_t graph = nullptr;
+        const hipError_t end = hipStreamEndCapture(w->stream, &graph);
+        if (launch != hipSuccess || end != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            set_error("batched BFS: capture of the level sequence failed: %s",
+                      hipGetErrorString(launch != hipSuccess ? launch : end));
+            return GS_ERR_DEVICE;
+        }
+        const hipError_t inst = hipGraphInstantiate(&w->ms_graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (inst != hipSuccess) {
+            w->ms_graph = nullptr;
+            set_error("batched BFS: hipGraphInstantiate: %s", hipGetErrorString(inst));
+            return GS_ERR_DEVICE;
+        }
+        w->ms_graph_key = key;
+    }
+    unsigned long long *h = w->h_counters;   // pinned: flags, levels done, entries examined
+    for (;;) {
+        GS_HIP(hipGraphLaunch(w->ms_graph, w->stream));
+        GS_HIP(hipMemcpyAsync(&h[0], ctl + C_FLAGS, 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipMemcpyAsync(&h[1], ctl + C_NLEV, 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipMemcpyAsync(&h[2], ctl + C_CHECKED, 8, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipStreamSynchronize(w->stream));
+        if (h[0] & F_DONE) break;
+    }
+    const unsigned long long flags = h[0], checked = h[2];
+    const int nlev = (int)h[1];
+    if (level_counts && max_levels > 0) {
+        const int rows = std::min(nlev, max_levels);
+        std::vector<unsigned long long> lc((size_t)rows * 64);
+        if (rows) {
+            GS_HIP(hipMemcpyAsync(lc.data(), ctl + C_LANES, lc.size() * 8, hipMemcpyDeviceToHost, w->stream));
+            GS_HIP(hipStreamSynchronize(w->stream));
+        }
+        for (int l = 0; l < rows; ++l)
+            for (int i = 0; i < nr; ++i) level_counts[(size_t)i * max_levels + l] = lc[(size_t)l * 64 + i];
+    }
+    w->stats.edges_scanned = checked;
+    if (levels) *levels = nlev;
+    if (flags & F_OVERFLOW) {
+        set_error("batched BFS needs depth > 255, beyond the u8 depth encoding");
+        return GS_ERR_DEPTH_OVERFLOW;
+    }
+    if (dvm) {
+        k_ms_unpermute<<<ms_grid(n), 256, 0, w->stream>>>(dvm, g->new_id.ptr, n, nr, w->ms_depth.ptr);
+        GS_HIP(hipGetLastError());
+    }
+    return GS_OK;
+}
+
 int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, uint8_t *depth_out,
                        uint64_t *level_counts, int max_levels, int *levels, bool keep_depth) {
     if (!g || !roots || nr < 1 || nr > 64 || max_levels < 0) {
@@ -452,7 +866,7 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
             set_error("gs_bfs_batch: root %llu >= N", (unsigned long long)roots[i]);
             return GS_ERR_ARG;
         }
-    const bool by_index = g->batch_method == 1 && !g->directed && !g->ss;
+    const bool by_index = g->batch_method >= 1 && !g->directed && !g->ss;
     keep_depth = keep_depth || depth_out;
     if (by_index) {
         GS_TRY(build_adjacency(const_cast<gs_graph *>(g)));
@@ -478,7 +892,9 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     GS_HIP(hipEventRecord(t0, w->stream));
     if (by_index) {
         int nlev = 0;
-        const int status = run_bfs_batch_index(g, w, roots, nr, level_counts, max_levels, &nlev, keep_depth);
+        const int status = g->batch_method == 2
+                               ? run_bfs_batch_graph(g, w, roots, nr, level_counts, max_levels, &nlev, keep_depth)
+                               : run_bfs_batch_index(g, w, roots, nr, level_counts, max_levels, &nlev, keep_depth);
         GS_HIP(hipEventRecord(t1, w->stream));
         GS_TRY(sync_and_collect(w));
         float ms = 0.f;

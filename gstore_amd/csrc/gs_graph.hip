@@ -103,6 +103,7 @@ int graph_create(gs_graph **out) {
 gs_ws::~gs_ws() {
     if (stream) (void)hipStreamSynchronize(stream);
     timer.collect();
+    if (ms_graph) (void)hipGraphExecDestroy(ms_graph);
     if (h_counters) (void)hipHostFree(h_counters);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -111,7 +112,7 @@ size_t gs_ws::device_bytes() const {
     return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + ppr_acc.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
            bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + kc_deg.bytes() + ms_vis.bytes() +
            ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_depth_vm.bytes() + ms_counts.bytes() +
-           queue.bytes() + qdeg.bytes() + qoff.bytes() + qsum.bytes() + depth_r.bytes() + counters.bytes();
+           queue.bytes() + qdeg.bytes() + qoff.bytes() + qsum.bytes() + depth_r.bytes() + counters.bytes() + ms_ctl.bytes();
 }
 
 gs_graph::~gs_graph() {
@@ -841,7 +842,7 @@ int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gath
 
 int gs_set_batch_bfs_method(gs_graph *g, int method) {
     if (!g) { set_error("null graph"); return GS_ERR_ARG; }
-    if (method < 0 || method > 1) { set_error("batched BFS method %d: expected 0 or 1", method); return GS_ERR_ARG; }
+    if (method < 0 || method > 2) { set_error("batched BFS method %d: expected 0, 1 or 2", method); return GS_ERR_ARG; }
     g->batch_method = method;
     return GS_OK;
 }

@@ -140,6 +140,9 @@ struct gs_ws {
     gs::DevBuf<unsigned long long> qsum;                      // per-1024 block sums of qdeg
     gs::DevBuf<uint8_t> depth_r;                              // depth by relabelled id
     gs::DevBuf<unsigned long long> ms_counts;
+    gs::DevBuf<unsigned long long> ms_ctl;                    // device-driven batched BFS: control words, level counts
+    hipGraphExec_t ms_graph = nullptr;                        // its captured level sequence
+    std::vector<const void *> ms_graph_key;                   // what the capture baked in
     gs::DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;  // pinned mirror of counters
     gs::KernelTimer timer;
@@ -203,7 +206,7 @@ struct gs_graph {
     int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy
     int wcc_method = 1;                      // 0 tile hooking passes, 1 sampled linking over the lists
     int kcore_method = 1;                    // 0 tile scans, 1 peeled lists pushed over the index
-    int batch_method = 1;                    // batched BFS: 0 tile sweep, 1 direction-optimizing over the index
+    int batch_method = 1;                    // batched BFS: 0 tile sweep, 1 over the index, 2 the same in a hipGraph
     // source-blocked copy of the lists (built on demand), gs_pb.hip
     bool pb_ready = false;
     uint32_t pb_k = 0;                       // blocks; 0 = not available (PageRank pulls)

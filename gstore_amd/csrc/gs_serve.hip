@@ -71,7 +71,7 @@ struct Worker {
 // masks, frontier queue and its prefix; depths only when a query asks for
 // them); otherwise over the tiles (lane masks, lane-major depths, row bits).
 // The index itself belongs to the graph and is not counted here.
-static bool bfs_by_index(const gs_graph *g) { return g->batch_method == 1 && !g->directed && !g->ss; }
+static bool bfs_by_index(const gs_graph *g) { return g->batch_method >= 1 && !g->directed && !g->ss; }
 
 static uint64_t need_bytes(const gs_graph *g, int kind, int lanes, bool keep_depth) {
     const uint64_t n = g->n, rowact = (g->p_s + 31) / 32 * 4, base = 16 * 8;
@@ -79,7 +79,8 @@ static uint64_t need_bytes(const gs_graph *g, int kind, int lanes, bool keep_dep
         case GS_Q_BFS:
             if (bfs_by_index(g))
                 return base + 24 * n + 64 * 8 * 2 + 8 * n + 8 * (n + 1) + 8 * (n / 1024 + 1) +
-                       (keep_depth ? (uint64_t)lanes * n + 64 * n : 0);
+                       (keep_depth ? (uint64_t)lanes * n + 64 * n : 0) +
+                       (g->batch_method == 2 ? 8 * (16 + 64 * 256) : 0);   // device-driven levels: control words
             return base + 24 * n + (uint64_t)lanes * n + rowact + 64 * 8 * 2;
         case GS_Q_PPR: return base + 24 * n;  // prior, acc, factor, rank (f32) and the FP64 sums
         case GS_Q_PAGERANK: return base + 12 * n;
@@ -97,7 +98,7 @@ static uint64_t kind_bytes(const gs_ws *w, int kind) {
         case GS_Q_BFS:
             return b + w->ms_vis.bytes() + w->ms_front.bytes() + w->ms_next.bytes() + w->ms_depth.bytes() +
                    w->ms_depth_vm.bytes() + w->rowact.bytes() + w->ms_roots.bytes() + w->ms_counts.bytes() +
-                   w->queue.bytes() + w->qdeg.bytes() + w->qoff.bytes() + w->qsum.bytes();
+                   w->queue.bytes() + w->qdeg.bytes() + w->qoff.bytes() + w->qsum.bytes() + w->ms_ctl.bytes();
         case GS_Q_PPR:
             return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes() + w->ppr_rank.bytes() +
                    w->ppr_acc.bytes();

@@ -6,7 +6,9 @@
 // for undirected graphs, k-core with k = 8 (sweeps that skip inactive tiles);
 // the smallest sum is applied.  On undirected resident grids, the
 // direction-optimizing BFS switch thresholds (alpha, beta) are then timed on
-// two fixed mid-degree roots.  With a cache path, results are kept per
+// eight roots at evenly spaced degree ranks (what uniformly drawn roots with
+// edges look like); a setting replaces the current one only when at least 2%
+// faster, so timing noise does not move it.  With a cache path, results are kept per
 // graph signature (N, E, non-empty tiles, directed) so a later open of the
 // same graph applies them without timing.
 #include <stdio.h>
@@ -119,27 +121,35 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
     r.xcd_order = g->xcd_order;
     r.ncand = k < 6 ? k : 6;
     r.best_ms = best;
-    // direction-optimizing BFS thresholds, on two mid-degree roots
+    // direction-optimizing BFS thresholds, on roots at evenly spaced degree ranks
     if (!g->directed && !g->ss) {
         GS_TRY(build_adjacency(g));
-        if (g->nzn >= 3) {
-            uint32_t roots[2];
-            GS_HIP(hipMemcpy(&roots[0], g->old_id.ptr + g->nzn / 3, 4, hipMemcpyDeviceToHost));
-            GS_HIP(hipMemcpy(&roots[1], g->old_id.ptr + 2 * g->nzn / 3, 4, hipMemcpyDeviceToHost));
+        if (g->nzn >= 16) {
+            constexpr int NR = 8;
+            uint32_t roots[NR];
+            for (int i = 0; i < NR; ++i)
+                GS_HIP(hipMemcpy(&roots[i], g->old_id.ptr + g->nzn * (2 * i + 1) / (2 * NR), 4, hipMemcpyDeviceToHost));
             const int alphas[4] = {8, 14, 30, 60}, betas[2] = {24, 64};
-            double best_do = 1e30;
+            auto time_roots = [&](double *ms) -> int {
+                *ms = 0;
+                for (int ri = 0; ri < NR; ++ri) {
+                    GS_TRY(run_bfs_do(g, g->ws, roots[ri], nullptr, nullptr, 0, nullptr));
+                    *ms += g->ws->stats.ms;
+                }
+                return GS_OK;
+            };
             int ba = g->do_alpha, bb = g->do_beta;
             GS_TRY(run_bfs_do(g, g->ws, roots[0], nullptr, nullptr, 0, nullptr));  // warm
+            double best_do = 0;
+            GS_TRY(time_roots(&best_do));   // the current setting is the one to beat
             for (int ai = 0; ai < 4; ++ai)
                 for (int bi = 0; bi < 2; ++bi) {
+                    if (alphas[ai] == ba && betas[bi] == bb) continue;
                     g->do_alpha = alphas[ai];
                     g->do_beta = betas[bi];
                     double ms = 0;
-                    for (int ri = 0; ri < 2; ++ri) {
-                        GS_TRY(run_bfs_do(g, g->ws, roots[ri], nullptr, nullptr, 0, nullptr));
-                        ms += g->ws->stats.ms;
-                    }
-                    if (ms < best_do) {
+                    GS_TRY(time_roots(&ms));
+                    if (ms < 0.98 * best_do) {
                         best_do = ms;
                         ba = alphas[ai];
                         bb = betas[bi];

@@ -86,19 +86,48 @@ def test_batch_errors(kron14):
 
 @pytest.mark.parametrize("nr", [1, 64])
 def test_batch_methods_identical(kron14, nr):
-    """Lanes over the neighbour-list index and over the tile sweep give the same
-    depths and level counts."""
+    """Lanes over the neighbour-list index (host-driven and device-driven
+    levels) and over the tile sweep give the same depths and level counts."""
     n, (start, grid, deg), g = kron14
     rng = np.random.default_rng(100 + nr)
     roots = rng.choice(np.flatnonzero(deg), nr, replace=False)
     got = []
-    for method in (0, 1):
+    for method in (0, 1, 2):
         g.set_batch_bfs_method(method)
         got.append(g.bfs_batch(roots))
     g.set_batch_bfs_method(1)
-    assert np.array_equal(got[0][0], got[1][0]) and got[0][1] == got[1][1]
+    for k in (1, 2):
+        assert np.array_equal(got[0][0], got[k][0]) and got[0][1] == got[k][1]
     with pytest.raises(ga.GStoreError):
-        g.set_batch_bfs_method(2)
+        g.set_batch_bfs_method(3)
+
+
+@pytest.mark.parametrize("s", [14, 18])
+def test_batch_graph_replays(s):
+    """The captured level sequence launched for one batch after another, with
+    other work on the graph between them: every launch matches the host-driven
+    levels and the oracle, including a root without edges, one root, 64 roots
+    and batches deeper than one launch's four levels."""
+    n, e = graphs.kron(s)
+    start, grid, deg = oracle.convert(e, n)
+    rng = np.random.default_rng(1000 + s)
+    nz = np.flatnonzero(deg)
+    iso = np.flatnonzero(deg == 0)
+    batches = [rng.choice(nz, 64, replace=False), nz[-1:], rng.choice(nz, 7, replace=False),
+               np.concatenate([iso[:1], rng.choice(nz, 20, replace=False)]) if iso.size else nz[:21],
+               rng.choice(nz, 64, replace=False)]
+    with ga.Graph.from_arrays(n, start, grid, deg) as g:
+        for i, roots in enumerate(batches):
+            _say("graph replay", s, i, len(roots))
+            g.set_batch_bfs_method(1)
+            want = g.bfs_batch(roots)
+            g.bfs(int(roots[0]))          # eager work between the launches
+            g.set_batch_bfs_method(2)
+            got = g.bfs_batch(roots)
+            assert np.array_equal(got[0], want[0]) and got[1] == want[1]
+            d, lc = oracle.bfs(n, start, grid, int(roots[-1]))
+            assert np.array_equal(got[0][-1], d) and got[1][-1][:len(lc)] == lc
+        assert max(len(c) for c in got[1]) > 4 or s == 14
 
 
 def test_batch_index_scale18():
