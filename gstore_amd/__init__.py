@@ -309,9 +309,9 @@ class Graph:
                 for b in buf[:min(n.value, 256)]]
 
     def set_batch_bfs_method(self, method: int) -> None:
-        """1 (default): lanes over the neighbour-list index, a host round trip per level;
-        2: the same levels driven from the device, four levels per hipGraph launch;
-        0: lanes over the tile sweep."""
+        """2 (default): lanes over the neighbour-list index, levels driven from the
+        device, four levels per hipGraph launch; 1: the same with a host round trip
+        per level; 0: lanes over the tile sweep."""
         _check(_lib.gs_set_batch_bfs_method(self._h, int(method)))
 
     def set_kcore_method(self, method: int) -> None:

@@ -422,6 +422,8 @@ int build_adjacency(gs_graph *g) {
     GS_HIP(hipMemcpyAsync(h.data(), cur64.ptr, NBUCKET * 8, hipMemcpyDeviceToHost, st));
     GS_HIP(hipStreamSynchronize(st));
     g->nzn = n - h[0];
+    g->ms_nhub = 0;   // degree >= 2^MS_HEAVY_LOG: buckets from 1 + 8 * MS_HEAVY_LOG up (deg_bucket)
+    for (int b = 1 + 8 * (int)MS_HEAVY_LOG; b < NBUCKET; ++b) g->ms_nhub += h[b];
     unsigned long long run = 0;
     for (int b = NBUCKET - 1; b >= 0; --b) { unsigned long long c = h[b]; h[b] = run; run += c; }
     GS_HIP(hipMemcpyAsync(cur64.ptr, h.data(), NBUCKET * 8, hipMemcpyHostToDevice, st));

@@ -192,13 +192,15 @@ __global__ void __launch_bounds__(256) k_ms_td(const uint32_t *__restrict__ queu
     }
 }
 
-__global__ void __launch_bounds__(256) k_ms_bu(uint64_t nzn, unsigned long long all, const uint64_t *__restrict__ off,
-                                               const uint32_t *__restrict__ nbr,
-                                               const unsigned long long *__restrict__ vis,
-                                               const unsigned long long *__restrict__ front, unsigned long long *next,
-                                               unsigned long long *checked) {
+// Bottom-up, one thread per vertex of ids [v0, v1) (degree < MS_HEAVY_DEG):
+// OR the neighbours' frontier masks, four at a time, until every lane the
+// vertex misses is found.
+__device__ inline void ms_bu_light(uint64_t v0, uint64_t v1, unsigned long long all, const uint64_t *__restrict__ off,
+                                   const uint32_t *__restrict__ nbr, const unsigned long long *__restrict__ vis,
+                                   const unsigned long long *__restrict__ front, unsigned long long *next,
+                                   unsigned long long *checked) {
     uint32_t ck = 0;
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nzn; v += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t v = v0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < v1; v += (uint64_t)gridDim.x * blockDim.x) {
         const unsigned long long need = all & ~vis[v];
         if (!need) continue;
         const uint64_t b = off[v], e = off[v + 1];
@@ -218,6 +220,59 @@ __global__ void __launch_bounds__(256) k_ms_bu(uint64_t nzn, unsigned long long 
         if (acc) next[v] = acc;
     }
     if (ck) atomicAdd(checked, (unsigned long long)ck);
+}
+
+// Bottom-up for the heavy vertices (ids [0, nheavy), degree >= MS_HEAVY_DEG):
+// one wave per vertex, lanes stride its list 256 entries per step, and the
+// wave ORs its lanes' finds after each step, stopping once every missing lane
+// is found.  A thread per vertex here would walk a hub's whole list alone
+// while the rest of the grid waits.
+__device__ inline void ms_bu_heavy(uint64_t nheavy, unsigned long long all, const uint64_t *__restrict__ off,
+                                   const uint32_t *__restrict__ nbr, const unsigned long long *__restrict__ vis,
+                                   const unsigned long long *__restrict__ front, unsigned long long *next,
+                                   unsigned long long *checked) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long ck = 0;
+    for (uint64_t v = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; v < nheavy; v += nw) {
+        const unsigned long long need = all & ~vis[v];
+        if (!need) continue;
+        const uint64_t b = off[v], e = off[v + 1];
+        unsigned long long acc = 0;
+        for (uint64_t k0 = b; k0 < e; k0 += 256) {
+            uint32_t u[4];
+            unsigned long long f[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) u[j] = nbr[min(k0 + j * 64 + lane, e - 1)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[j] = front[u[j]];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k0 + j * 64 + lane < e) acc |= f[j] & need;
+            ck += min<uint64_t>(256, e - k0);
+#pragma unroll
+            for (int s = 32; s > 0; s >>= 1) acc |= __shfl_xor(acc, s);
+            if (acc == need) break;
+        }
+        if (lane == 0 && acc) next[v] = acc;
+    }
+    if (lane == 0 && ck) atomicAdd(checked, ck);
+}
+
+__global__ void __launch_bounds__(256) k_ms_bu(uint64_t v0, uint64_t nzn, unsigned long long all,
+                                               const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                                               const unsigned long long *__restrict__ vis,
+                                               const unsigned long long *__restrict__ front, unsigned long long *next,
+                                               unsigned long long *checked) {
+    ms_bu_light(v0, nzn, all, off, nbr, vis, front, next, checked);
+}
+
+__global__ void __launch_bounds__(256) k_ms_bu_heavy(uint64_t nheavy, unsigned long long all,
+                                                     const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                                                     const unsigned long long *__restrict__ vis,
+                                                     const unsigned long long *__restrict__ front,
+                                                     unsigned long long *next, unsigned long long *checked) {
+    ms_bu_heavy(nheavy, all, off, nbr, vis, front, next, checked);
 }
 
 // Level end over relabelled ids 0 .. nzn-1: next becomes the frontier; per-lane
@@ -352,6 +407,8 @@ __global__ void k_ms_unpermute(const uint8_t *__restrict__ depth_vm, const uint3
 }
 
 static unsigned ms_grid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
+// four waves per workgroup, one heavy vertex per wave at a time
+static unsigned ms_heavy_grid(uint64_t nh) { return (unsigned)std::min<uint64_t>((nh + 3) / 4, 8192); }
 
 // ---- the same levels, driven from the device (one hipGraph per workspace) -------
 // The host loop above reads three counters after every level to choose the
@@ -579,34 +636,15 @@ __global__ void __launch_bounds__(256) k_msg_td(const unsigned long long *ctl, c
     }
 }
 
-__global__ void __launch_bounds__(256) k_msg_bu(unsigned long long *ctl, uint64_t nzn, const uint64_t *__restrict__ off,
-                                                const uint32_t *__restrict__ nbr,
+template <bool HEAVY>
+__global__ void __launch_bounds__(256) k_msg_bu(unsigned long long *ctl, uint64_t v0, uint64_t v1,
+                                                const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                                                 const unsigned long long *__restrict__ vis,
                                                 const unsigned long long *__restrict__ front,
                                                 unsigned long long *next) {
     if ((ctl[C_FLAGS] & (F_BU | F_DONE)) != F_BU) return;
-    const unsigned long long all = ctl[C_ACTIVE];
-    uint32_t ck = 0;
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nzn; v += (uint64_t)gridDim.x * blockDim.x) {
-        const unsigned long long need = all & ~vis[v];
-        if (!need) continue;
-        const uint64_t b = off[v], e = off[v + 1];
-        unsigned long long acc = 0;
-        for (uint64_t k = b; k < e && acc != need; k += 4) {
-            uint32_t u[4];
-            unsigned long long f[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) u[j] = nbr[min(k + j, e - 1)];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) f[j] = front[u[j]];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (k + j < e) acc |= f[j] & need;
-            ck += (uint32_t)min<uint64_t>(4, e - k);
-        }
-        if (acc) next[v] = acc;
-    }
-    if (ck) atomicAdd(&ctl[C_CHECKED], (unsigned long long)ck);
+    if (HEAVY) ms_bu_heavy(v1, ctl[C_ACTIVE], off, nbr, vis, front, next, ctl + C_CHECKED);
+    else ms_bu_light(v0, v1, ctl[C_ACTIVE], off, nbr, vis, front, next, ctl + C_CHECKED);
 }
 
 template <bool DEPTH>
@@ -617,7 +655,7 @@ __global__ void __launch_bounds__(256) k_msg_level_end(unsigned long long *ctl, 
     const unsigned long long f = ctl[C_FLAGS];
     if (f & F_DONE) return;
     const uint32_t level = (uint32_t)ctl[C_LEVEL];
-    const unsigned long long all = ctl[C_ALL];
+    const unsigned long long all = ctl[C_ACTIVE];   // as the host loop: the lanes still searching
     unsigned long long *lanes = ctl + C_LANES + 64ull * (level - 1);
     if (f & F_HAVEQ)
         ms_level_end<true, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
@@ -687,9 +725,13 @@ static int run_bfs_batch_index(const gs_graph *gc, gs_ws *w, const uint64_t *roo
         const int kclass = bottom_up ? GS_K_BFS_BOTTOMUP : GS_K_BFS_TOPDOWN;
         w->timer.begin(kclass, w->stream, &kb);
         if (bottom_up) {
-            if (nzn)
-                k_ms_bu<<<ms_grid(nzn), 256, 0, w->stream>>>(nzn, active, g->adj_off.ptr, g->adj_nbr.ptr, vis,
-                                                             front, next, w->counters.ptr + 4);
+            const uint64_t nh = g->ms_nhub;
+            if (nh)
+                k_ms_bu_heavy<<<ms_heavy_grid(nh), 256, 0, w->stream>>>(nh, active, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                                        vis, front, next, w->counters.ptr + 4);
+            if (nzn > nh)
+                k_ms_bu<<<ms_grid(nzn - nh), 256, 0, w->stream>>>(nh, nzn, active, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                                  vis, front, next, w->counters.ptr + 4);
         } else if (n_f && m_f) {
             uint64_t nq = n_f;
             if (!have_queue) {
@@ -775,7 +817,8 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
     k_msg_init<<<1, 64, 0, w->stream>>>(ctl, nr, w->qdeg.ptr, 2 * g->E, (uint32_t)g->do_alpha, (uint32_t)g->do_beta);
     GS_HIP(hipGetLastError());
     // what the captured kernels were given (the thresholds travel in ctl)
-    const std::vector<const void *> key = {g, (const void *)(uintptr_t)nzn, vis, front, next, dvm, ctl,
+    const std::vector<const void *> key = {g, (const void *)(uintptr_t)nzn, (const void *)(uintptr_t)g->ms_nhub,
+                                           vis, front, next, dvm, ctl,
                                            w->queue.ptr, w->qdeg.ptr, w->qoff.ptr, w->qsum.ptr, g->adj_off.ptr,
                                            g->adj_nbr.ptr, g->deg_r.ptr};
     if (!w->ms_graph || w->ms_graph_key != key) {
@@ -785,6 +828,8 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
         }
         GS_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
         const unsigned gv = nzn ? ms_grid(nzn) : 1, gb = (unsigned)std::max<uint64_t>((nzn + 1023) / 1024, 1);
+        const uint64_t nh = g->ms_nhub;
+        const unsigned hgrid = ms_heavy_grid(nh);
         for (int l = 0; l < MSG_LEVELS; ++l) {
             k_msg_decide<<<1, 1, 0, w->stream>>>(ctl, nzn);
             k_msg_list<<<gv, 256, 0, w->stream>>>(ctl, nzn, front, g->deg_r.ptr, w->queue.ptr, w->qdeg.ptr,
@@ -794,7 +839,11 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
             k_msg_deg_scan<<<gb, 256, 0, w->stream>>>(ctl, w->qdeg.ptr, w->qsum.ptr, w->qoff.ptr);
             k_msg_td<<<MSG_TD_GRID, 256, 0, w->stream>>>(ctl, w->queue.ptr, w->qoff.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
                                                          front, vis, next);
-            k_msg_bu<<<gv, 256, 0, w->stream>>>(ctl, nzn, g->adj_off.ptr, g->adj_nbr.ptr, vis, front, next);
+            if (nh)
+                k_msg_bu<true><<<hgrid, 256, 0, w->stream>>>(ctl, 0, nh, g->adj_off.ptr, g->adj_nbr.ptr, vis, front, next);
+            if (nzn > nh)
+                k_msg_bu<false><<<ms_grid(nzn - nh), 256, 0, w->stream>>>(ctl, nh, nzn, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                                          vis, front, next);
             if (dvm)
                 k_msg_level_end<true><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr, dvm,
                                                                  w->queue.ptr, w->qdeg.ptr);

@@ -34,6 +34,8 @@ constexpr uint32_t ROW_WORDS = 65536 / 32;   // bitmap words per tile row
 // Source-blocked copy of the neighbour-list index (PageRank), gs_pb.hip.
 constexpr int PB_SHIFT = 15;                 // vertices per block: 2^15, FP32 values fill 128 KiB of LDS
 constexpr uint32_t PB_B = 1u << PB_SHIFT;
+constexpr uint32_t MS_HEAVY_LOG = 7;         // batched bottom-up: a wave per vertex of degree >= 2^7
+constexpr uint32_t MS_HEAVY_DEG = 1u << MS_HEAVY_LOG;
 constexpr uint32_t PB_KMAX = 8192;           // blocks at most (vertices with edges <= 2^28)
 constexpr uint32_t PB_RUNS = 2048;           // runs staged per gather workgroup at most
 constexpr uint32_t PB_WIN = 16;              // entries per window: every run starts on a window and fills whole ones
@@ -201,12 +203,13 @@ struct gs_graph {
     gs::DevBuf<uint64_t> adj_off;            // list offsets by relabelled id, N + 1
     gs::DevBuf<uint32_t> adj_nbr;            // neighbour ids (relabelled), 2E
     uint64_t nzn = 0, npull = 0;             // vertices with edges (ids 0 .. nzn-1); pull chunks
+    uint64_t ms_nhub = 0;                    // ids 0 .. ms_nhub-1 have degree >= MS_HEAVY_DEG
     double index_ms = 0;                     // device time of the index build
     gs::DevBuf<uint32_t> chunkz;             // vertex whose list holds entry c*PULL_CHUNK, npull + 1
     int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy
     int wcc_method = 1;                      // 0 tile hooking passes, 1 sampled linking over the lists
     int kcore_method = 1;                    // 0 tile scans, 1 peeled lists pushed over the index
-    int batch_method = 1;                    // batched BFS: 0 tile sweep, 1 over the index, 2 the same in a hipGraph
+    int batch_method = 2;                    // batched BFS: 0 tile sweep, 1 over the index, 2 the same in a hipGraph
     // source-blocked copy of the lists (built on demand), gs_pb.hip
     bool pb_ready = false;
     uint32_t pb_k = 0;                       // blocks; 0 = not available (PageRank pulls)

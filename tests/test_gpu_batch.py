@@ -95,7 +95,7 @@ def test_batch_methods_identical(kron14, nr):
     for method in (0, 1, 2):
         g.set_batch_bfs_method(method)
         got.append(g.bfs_batch(roots))
-    g.set_batch_bfs_method(1)
+    g.set_batch_bfs_method(2)
     for k in (1, 2):
         assert np.array_equal(got[0][0], got[k][0]) and got[0][1] == got[k][1]
     with pytest.raises(ga.GStoreError):
