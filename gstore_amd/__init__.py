@@ -117,7 +117,7 @@ _SIGS = {
     "gs_set_pagerank_heavy": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_pagerank_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_pagerank_blocking": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
-                                                ctypes.c_uint32]),
+                                                ctypes.c_uint32, ctypes.c_uint32]),
     "gs_set_wcc_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_kcore_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_batch_bfs_method": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -288,10 +288,12 @@ class Graph:
         tile scatter with atomics.  True / False mean PR_PULL / PR_SCATTER."""
         _check(_lib.gs_set_pagerank_method(self._h, int(method)))
 
-    def set_pagerank_blocking(self, gather_entries: int = 1 << 20, gather_runs: int = 2048,
-                              accum_entries: int = 1 << 20, dense_entries: int = 8192) -> None:
-        """Work split of the blocked PageRank (small values exercise the split paths)."""
-        _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries, dense_entries))
+    def set_pagerank_blocking(self, gather_entries: int = 1 << 20, gather_runs: int = 1024,
+                              accum_entries: int = 1 << 20, dense_entries: int = 8192,
+                              compact_permille: int = 500) -> None:
+        """Work split and layout of the blocked PageRank (small values exercise the split paths)."""
+        _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries, dense_entries,
+                                             compact_permille))
 
     def set_wcc_method(self, method: int) -> None:
         """1 (default): sampled linking over the neighbour-list index; 0: hook passes over the tiles."""

@@ -599,6 +599,30 @@ int queue_prefix(gs_ws *w, uint64_t nq, std::vector<unsigned long long> &hsum, u
     return GS_OK;
 }
 
+int scan_counts(hipStream_t st, const int32_t *cnt, uint64_t n, uint64_t *off, uint64_t *total) {
+    if (!n) {
+        GS_HIP(hipMemsetAsync(off, 0, 8, st));
+        if (total) *total = 0;
+        return GS_OK;
+    }
+    const uint64_t nb = (n + 1023) / 1024;
+    DevBuf<unsigned long long> bsum;
+    GS_TRY(bsum.alloc(nb));
+    k_deg_block_sums<<<(unsigned)nb, 256, 0, st>>>(cnt, n, bsum.ptr);
+    GS_HIP(hipGetLastError());
+    std::vector<unsigned long long> hb(nb);
+    GS_HIP(hipMemcpyAsync(hb.data(), bsum.ptr, nb * 8, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    unsigned long long run = 0;
+    for (uint64_t i = 0; i < nb; ++i) { const unsigned long long c = hb[i]; hb[i] = run; run += c; }
+    GS_HIP(hipMemcpyAsync(bsum.ptr, hb.data(), nb * 8, hipMemcpyHostToDevice, st));
+    k_deg_scan<<<(unsigned)nb, 256, 0, st>>>(cnt, n, bsum.ptr, off);
+    GS_HIP(hipGetLastError());
+    GS_HIP(hipStreamSynchronize(st));   // bsum and hb die here
+    if (total) *total = run;
+    return GS_OK;
+}
+
 int ensure_queue(const gs_graph *g, gs_ws *w) {
     if (w->queue.n != g->n) GS_TRY(w->queue.alloc(g->n));
     if (w->qdeg.n != g->n) GS_TRY(w->qdeg.alloc(g->n));

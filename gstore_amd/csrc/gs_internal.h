@@ -37,7 +37,8 @@ constexpr uint32_t PB_B = 1u << PB_SHIFT;
 constexpr uint32_t MS_HEAVY_LOG = 7;         // batched bottom-up: a wave per vertex of degree >= 2^7
 constexpr uint32_t MS_HEAVY_DEG = 1u << MS_HEAVY_LOG;
 constexpr uint32_t PB_KMAX = 8192;           // blocks at most (vertices with edges <= 2^28)
-constexpr uint32_t PB_RUNS = 2048;           // runs staged per gather workgroup at most
+constexpr uint32_t PB_RUNS = 1024;           // runs staged per gather workgroup at most
+constexpr unsigned long long PB_COMPACT = 1ull << 63;   // run stream base: the run's block is compacted
 constexpr uint32_t PB_WIN = 16;              // entries per window: every run starts on a window and fills whole ones
 struct PbSlice { uint64_t e0, e1; uint32_t q, pad; };               // build: list entries of block q
 struct PbUnit1 { uint64_t w0, w1; uint32_t r0, r1, p, pad; };       // gather: source-major windows of block p
@@ -220,9 +221,15 @@ struct gs_graph {
     uint64_t pb_unit2 = 1u << 20;            // accum: entries per workgroup at most
     uint32_t pb_dense = 8192;                // accum: entries per vertex from which a block uses LDS float atomics
     uint64_t pb_entries = 0;                 // 2E plus the padding that fills each run's last window
-    gs::DevBuf<uint16_t> pb_src, pb_dst;     // pb_entries: offsets of u in its block and v in its block
-    gs::DevBuf<float> pb_val;                // pb_entries: prior[u] per entry, rewritten every iteration
+    uint64_t pb_stream = 0;                  // accumulation stream length (with block alignment)
+    uint32_t pb_compact = 500;               // a block is compacted when its segments <= this / 1000 of its entries
+    gs::DevBuf<uint16_t> pb_src;             // pb_entries, q-major: offset of u in its block (compacted
+                                             // blocks: bit 15 marks a segment's last entry)
+    gs::DevBuf<uint32_t> pb_wcoff;           // per q-major window: its first segment, relative to its run
+    gs::DevBuf<uint16_t> pb_dst;             // pb_stream: destination offset per stream value
+    gs::DevBuf<float> pb_val;                // pb_stream: the stream values, rewritten every iteration
     gs::DevBuf<uint64_t> pb_run_pw, pb_run_qw;  // non-empty runs, source-major: window prefix, q-major window
+    gs::DevBuf<uint64_t> pb_run_st;          // their stream base (PB_COMPACT: the block is compacted)
     gs::DevBuf<gs::PbUnit1> pb_u1;
     gs::DevBuf<gs::PbUnit2> pb_u2;
     gs::DevBuf<gs::PbRed> pb_red;
@@ -254,7 +261,9 @@ int run_bfs_do(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, u
                int max_levels, int *levels);
 int build_adjacency(gs_graph *g);
 int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double *acc64);
-int build_blocked(gs_graph *g);  // the blocked copy (after the lists); pb_k = 0 when too many blocks
+int build_blocked(gs_graph *g);
+// off[0 .. n] = exclusive prefix of cnt[0 .. n) on stream st (off[n] = the total).
+int scan_counts(hipStream_t st, const int32_t *cnt, uint64_t n, uint64_t *off, uint64_t *total);  // the blocked copy (after the lists); pb_k = 0 when too many blocks
 int pr_blocked(const gs_graph *g, gs_ws *w, float *prior, const float *fac);
 int pr_blocked64(const gs_graph *g, gs_ws *w, const float *prior, double *acc);
 size_t pb_bytes(const gs_graph *g);

@@ -180,6 +180,96 @@ __global__ void k_pb_pad(const unsigned long long *__restrict__ cur, uint64_t nr
     }
 }
 
+// Per q-major window: a bit for the last entry of each segment of equal
+// destinations (padding, source offset PB_B, never ends one; a segment never
+// crosses a window) and the segment count.
+__global__ void k_pb_wmask(const uint16_t *__restrict__ src, const uint16_t *__restrict__ dst, uint64_t nwin,
+                           uint16_t *__restrict__ wmask, int32_t *__restrict__ wcnt) {
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwin; w += (uint64_t)gridDim.x * blockDim.x) {
+        uint16_t s[PB_WIN], d[PB_WIN];
+#pragma unroll
+        for (int i = 0; i < PB_WIN; ++i) {
+            s[i] = src[w * PB_WIN + i];
+            d[i] = dst[w * PB_WIN + i];
+        }
+        uint32_t m = 0;
+#pragma unroll
+        for (int i = 0; i < PB_WIN; ++i) {
+            const bool real = s[i] != (uint16_t)PB_B;
+            const bool last = i + 1 == PB_WIN || s[i + 1] == (uint16_t)PB_B || d[i + 1] != d[i];
+            if (real && last) m |= 1u << i;
+        }
+        wmask[w] = (uint16_t)m;
+        wcnt[w] = __popc(m);
+    }
+}
+
+// out[i] = a[idx[i]]
+__global__ void k_pb_pick(const uint64_t *__restrict__ a, const uint64_t *__restrict__ idx, uint64_t n,
+                          uint64_t *__restrict__ out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = a[idx[i]];
+}
+
+// wcoff[w] = segments before window w within its run (a wave per run).
+__global__ void k_pb_wcoff(const uint64_t *__restrict__ run_pw, const uint64_t *__restrict__ run_qw, uint64_t nruns,
+                           const uint64_t *__restrict__ cpos, uint32_t *__restrict__ wcoff) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t r = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; r < nruns; r += nw) {
+        const uint64_t q0 = run_qw[r], n = run_pw[r + 1] - run_pw[r], base = cpos[q0];
+        for (uint64_t j = lane; j < n; j += 64) wcoff[q0 + j] = (uint32_t)(cpos[q0 + j] - base);
+    }
+}
+
+// The stream's destination offsets, per q-major window: the window's 16
+// offsets for a plain block, one offset per segment for a compacted block.
+// blk_w[q] = first window of block q (K + 1 entries), blk_sb[q] = its stream
+// base (PB_COMPACT set when compacted), blk_c0[q] = segments before it.
+__global__ void k_pb_stream_dst(const uint16_t *__restrict__ dst, const uint16_t *__restrict__ wmask, uint64_t nwin,
+                                const uint64_t *__restrict__ cpos, const uint64_t *__restrict__ blk_w,
+                                const uint64_t *__restrict__ blk_sb, const uint64_t *__restrict__ blk_c0, uint32_t K,
+                                uint16_t *__restrict__ out) {
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwin; w += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = K - 1;   // largest q with blk_w[q] <= w
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (blk_w[mid] <= w) lo = mid; else hi = mid - 1;
+        }
+        const uint64_t sb = blk_sb[lo];
+        if (sb & PB_COMPACT) {
+            uint16_t *o = out + (sb & ~PB_COMPACT) + (cpos[w] - blk_c0[lo]);
+            const uint32_t m = wmask[w];
+            uint32_t k = 0;
+            for (int i = 0; i < PB_WIN; ++i)
+                if ((m >> i) & 1u) o[k++] = dst[w * PB_WIN + i];
+        } else {
+            uint16_t *o = out + sb + (w - blk_w[lo]) * PB_WIN;
+            for (int i = 0; i < PB_WIN; ++i) o[i] = dst[w * PB_WIN + i];
+        }
+    }
+}
+
+// In the windows of compacted blocks, bit 15 of a source offset marks the last
+// entry of a segment and padding becomes offset 0 without the mark (it follows
+// the window's last mark, so the gather never stores it).
+__global__ void k_pb_mark(uint16_t *__restrict__ src, const uint16_t *__restrict__ wmask, uint64_t nwin,
+                          const uint64_t *__restrict__ blk_w, const uint64_t *__restrict__ blk_sb, uint32_t K) {
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwin; w += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = K - 1;   // largest q with blk_w[q] <= w
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (blk_w[mid] <= w) lo = mid; else hi = mid - 1;
+        }
+        if (!(blk_sb[lo] & PB_COMPACT)) continue;
+        const uint32_t m = wmask[w];
+        for (int i = 0; i < PB_WIN; ++i) {
+            const uint16_t s = src[w * PB_WIN + i];
+            src[w * PB_WIN + i] = s == (uint16_t)PB_B ? (uint16_t)0 : (uint16_t)(s | (((m >> i) & 1u) << 15));
+        }
+    }
+}
+
 // ---- per iteration -----------------------------------------------------------
 
 constexpr int PB_WPL = 4;   // gather: windows per lane per step, all loads issued first
@@ -207,20 +297,26 @@ __device__ inline void stage_block(const float *__restrict__ prior, uint64_t pba
 typedef unsigned int pb_u32x4 __attribute__((ext_vector_type(4)));
 typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
 
-// val[e] = prior[p*PB_B + src[e]] for the windows [w0, w1) of source block p,
-// counted in source-major order; the unit's runs are staged in LDS.  Lane i of
-// the workgroup takes windows w0 + i, w0 + i + PB_WG, ... (adjacent lanes
-// mostly share a run, so a wave reads and writes 2 KiB and 4 KiB runs); a
-// window's run comes from an 11-step search over the staged run starts.
+// Values of the windows [w0, w1) of source block p (counted in source-major
+// order) into the accumulation stream; the unit's runs are staged in LDS.
+// Lane i of the workgroup takes windows w0 + i, w0 + i + PB_WG, ... (adjacent
+// lanes mostly share a run); a window's run comes from a 10-step search over
+// the staged run starts.  A run of a plain destination block writes
+// val[e] = prior[p*PB_B + src[e]] for each of its 16 entries (four 16-byte
+// stores); a run of a compacted block writes one sum per segment of equal
+// destinations (bit 15 of a source offset marks each segment's last entry,
+// padding follows the last mark) at the window's place in the compacted stream.
 __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__ units,
                                                      const uint64_t *__restrict__ run_pw,
                                                      const uint64_t *__restrict__ run_qw,
+                                                     const uint64_t *__restrict__ run_st,
                                                      const uint16_t *__restrict__ src,
+                                                     const uint32_t *__restrict__ wcoff,
                                                      const float *__restrict__ prior, uint64_t nzn,
                                                      float *__restrict__ val) {
     __shared__ float4 pri4[PB_B / 4 + 1];   // + the zero read by padding entries (offset PB_B)
     __shared__ uint32_t rs[PB_RUNS + 1];
-    __shared__ uint64_t rq[PB_RUNS];
+    __shared__ uint64_t rq[PB_RUNS], rt[PB_RUNS];
     const PbUnit1 u = units[blockIdx.x];
     const uint64_t pbase = (uint64_t)u.p << PB_SHIFT;
     const uint32_t pn = (uint32_t)std::min<uint64_t>(PB_B, nzn - pbase);
@@ -231,15 +327,19 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
     for (int k = threadIdx.x; k < nr; k += PB_WG) {
         const uint64_t r0 = run_pw[u.r0 + k];
         const uint64_t s = r0 > u.w0 ? r0 : u.w0;
+        const uint64_t t = run_st[u.r0 + k];
         rs[k] = (uint32_t)(s - u.w0);
         rq[k] = run_qw[u.r0 + k] + (s - r0);
+        // plain runs: the stream follows the windows; compacted: wcoff is relative to the run
+        rt[k] = (t & PB_COMPACT) ? t : t + (s - r0) * PB_WIN;
     }
     const uint32_t len = (uint32_t)(u.w1 - u.w0);
     if (threadIdx.x == 0) rs[nr] = len;
     __syncthreads();
     for (uint32_t wb = threadIdx.x; wb < len; wb += PB_WPL * PB_WG) {
-        uint64_t qw[PB_WPL];
+        uint64_t qw[PB_WPL], st[PB_WPL];
         pb_u32x4 sq[PB_WPL][2];
+        uint32_t co[PB_WPL];
 #pragma unroll
         for (int j = 0; j < PB_WPL; ++j) {
             // past the unit, a lane reloads its first window and stores nothing
@@ -249,19 +349,40 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
             for (int step = PB_RUNS / 2; step > 0; step >>= 1)
                 if (k + step < nr && rs[k + step] <= w) k += step;
             qw[j] = rq[k] + (w - rs[k]);
+            st[j] = rt[k] & PB_COMPACT ? rt[k] : rt[k] + (uint64_t)(w - rs[k]) * PB_WIN;
             const pb_u32x4 *s4 = reinterpret_cast<const pb_u32x4 *>(src + qw[j] * PB_WIN);
             sq[j][0] = __builtin_nontemporal_load(s4);
             sq[j][1] = __builtin_nontemporal_load(s4 + 1);
+            co[j] = wcoff[qw[j]];
         }
 #pragma unroll
         for (int j = 0; j < PB_WPL; ++j) {
             if (wb + j * PB_WG >= len) break;
-            pb_f32x4 *v4 = reinterpret_cast<pb_f32x4 *>(val + qw[j] * PB_WIN);
+            if (st[j] & PB_COMPACT) {
+                float *out = val + (st[j] & ~PB_COMPACT) + co[j];
+                float acc = 0.0f;
+                uint32_t seg = 0;
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                const uint32_t a = sq[j][h >> 1][(h & 1) * 2], b = sq[j][h >> 1][(h & 1) * 2 + 1];
+                for (int i = 0; i < PB_WIN; ++i) {
+                    const uint32_t s = (sq[j][i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
+                    acc += pri[s & 0x7FFFu];
+                    if (s >> 15) {
+                        out[seg++] = acc;
+                        acc = 0.0f;
+                    }
+                }
+            } else {
+                float x[PB_WIN];
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    const uint32_t a = sq[j][h >> 2][h & 3];
+                    x[2 * h] = pri[a & 0xFFFFu];
+                    x[2 * h + 1] = pri[a >> 16];
+                }
                 // plain stores: nontemporal ones measured four times slower here
-                v4[h] = pb_f32x4{pri[a & 0xFFFFu], pri[a >> 16], pri[b & 0xFFFFu], pri[b >> 16]};
+                pb_f32x4 *v4 = reinterpret_cast<pb_f32x4 *>(val + st[j]);
+#pragma unroll
+                for (int h = 0; h < 4; ++h) v4[h] = pb_f32x4{x[4 * h], x[4 * h + 1], x[4 * h + 2], x[4 * h + 3]};
             }
         }
     }
@@ -437,7 +558,8 @@ __global__ void __launch_bounds__(256) k_pb_reduce(const PbRed *__restrict__ red
 // LDS tables of the kernels: checked on the host before the first launch.
 static int check_units(const std::vector<PbUnit1> &u1, const std::vector<PbUnit2> &u2,
                        const std::vector<PbRed> &red, const std::vector<uint64_t> &rpw, uint64_t total,
-                       uint32_t runs_max, uint32_t slots) {
+                       uint32_t runs_max, uint32_t slots, const std::vector<uint64_t> &sb,
+                       const std::vector<uint64_t> &slen) {
     uint64_t pw = 0;
     const uint64_t nr = rpw.size() - 1;
     for (const PbUnit1 &x : u1) {
@@ -451,24 +573,30 @@ static int check_units(const std::vector<PbUnit1> &u1, const std::vector<PbUnit2
         set_error("blocked index: gather units end at window %llu", (unsigned long long)pw);
         return GS_ERR_FORMAT;
     }
-    uint64_t e = 0;
-    for (const PbUnit2 &x : u2) {
-        if (x.e0 != e || x.e1 <= x.e0 || (x.e0 & (PB_WIN - 1)) || (x.e1 & (PB_WIN - 1)) ||
-            (x.slot != 0xFFFFFFFFu && x.slot >= slots)) {
-            set_error("blocked index: bad accum unit at entry %llu", (unsigned long long)e);
-            return GS_ERR_FORMAT;
+    // accumulation units: in block order, tiling each block's stream range
+    size_t i = 0;
+    for (uint32_t q = 0; q < (uint32_t)sb.size(); ++q) {
+        if (!slen[q]) continue;
+        uint64_t e = sb[q] & ~PB_COMPACT;
+        const uint64_t end = e + slen[q];
+        while (e < end) {
+            if (i >= u2.size() || u2[i].q != q || u2[i].e0 != e || u2[i].e1 <= e || u2[i].e1 > end ||
+                (u2[i].slot != 0xFFFFFFFFu && u2[i].slot >= slots)) {
+                set_error("blocked index: bad accum unit for block %u at %llu", q, (unsigned long long)e);
+                return GS_ERR_FORMAT;
+            }
+            e = u2[i++].e1;
         }
-        e = x.e1;
     }
-    if (e != total) { set_error("blocked index: accum units end at %llu", (unsigned long long)e); return GS_ERR_FORMAT; }
+    if (i != u2.size()) { set_error("blocked index: %zu accum units left over", u2.size() - i); return GS_ERR_FORMAT; }
     for (const PbRed &r : red)
         if (r.nslot < 2 || r.slot0 + r.nslot > slots) { set_error("blocked index: bad partial slots"); return GS_ERR_FORMAT; }
     return GS_OK;
 }
 
 size_t pb_bytes(const gs_graph *g) {
-    return g->pb_src.bytes() + g->pb_dst.bytes() + g->pb_val.bytes() + g->pb_run_pw.bytes() +
-           g->pb_run_qw.bytes() + g->pb_u1.bytes() + g->pb_u2.bytes() + g->pb_red.bytes() + g->pb_part.bytes() +
+    return g->pb_src.bytes() + g->pb_wcoff.bytes() + g->pb_dst.bytes() + g->pb_val.bytes() +
+           g->pb_run_pw.bytes() + g->pb_run_qw.bytes() + g->pb_run_st.bytes() + g->pb_u1.bytes() + g->pb_u2.bytes() + g->pb_red.bytes() + g->pb_part.bytes() +
            g->pb_part64.bytes();
 }
 
@@ -536,7 +664,7 @@ int build_blocked(gs_graph *g) {
         return GS_ERR_FORMAT;
     }
     std::vector<uint64_t> rpw, rqw;
-    std::vector<uint32_t> prun(K + 1);
+    std::vector<uint32_t> prun(K + 1), rblk;   // rblk: destination block of each run
     {
         // source-major walk of a q-major table, in 64 x 64 tiles for the host caches
         std::vector<uint64_t> tmp_c(KK), tmp_q(KK);
@@ -554,8 +682,13 @@ int build_blocked(gs_graph *g) {
             for (uint32_t q = 0; q < K; ++q) {
                 const uint64_t n_ = tmp_c[(uint64_t)p * K + q];
                 if (!n_) continue;
+                if (n_ >= (1ull << 32)) {
+                    set_error("blocked index: a run of %llu entries", (unsigned long long)n_);
+                    return GS_ERR_FORMAT;
+                }
                 rpw.push_back(pw);
                 rqw.push_back(tmp_q[(uint64_t)p * K + q] / PB_WIN);
+                rblk.push_back(q);
                 pw += (n_ + PB_WIN - 1) / PB_WIN;
             }
         }
@@ -590,57 +723,142 @@ int build_blocked(gs_graph *g) {
             r = (rl + 1 < rend && rpw[rl + 1] <= pw) ? rl + 1 : rl;
         }
     }
-    // 5. accum units: each destination block in equal parts of at most
-    //    pb_unit2 entries, cut on window boundaries; split blocks get partial slots
+    // 5. the copy in q-major order (runs sorted by destination), the padding of
+    //    each run's last window, and the segment masks of every window
+    const uint64_t nwin = run / PB_WIN;
+    GS_TRY(g->pb_src.alloc(run + PB_WIN));
+    DevBuf<uint16_t> qdst;   // q-major destination offsets, until the stream is laid out
+    GS_TRY(qdst.alloc(run + PB_WIN));
+    GS_HIP(hipMemcpyAsync(cnt.ptr, qm.data(), KK * 8, hipMemcpyHostToDevice, st));
+    if (!slices.empty()) {
+        k_pb_fill<<<(unsigned)slices.size(), PB_WG, 0, st>>>(d_sl.ptr, g->adj_off.ptr, g->adj_nbr.ptr, nzn, K,
+                                                             cnt.ptr, g->pb_src.ptr, qdst.ptr);
+        GS_HIP(hipGetLastError());
+        k_pb_pad<<<vgrid(KK), 256, 0, st>>>(cnt.ptr, KK, g->pb_src.ptr, qdst.ptr);
+        GS_HIP(hipGetLastError());
+    }
+    GS_HIP(hipMemsetAsync(g->pb_src.ptr + run, 0, PB_WIN * 2, st));
+    DevBuf<uint16_t> wmask;   // per q-major window: the last entry of each segment
+    GS_TRY(wmask.alloc(std::max<uint64_t>(nwin, 1)));
+    GS_TRY(g->pb_wcoff.alloc(std::max<uint64_t>(nwin, 1)));
+    DevBuf<uint64_t> cpos;   // segments before each q-major window (nwin + 1)
+    GS_TRY(cpos.alloc(nwin + 1));
+    uint64_t nseg = 0;
+    {
+        DevBuf<int32_t> wcnt;
+        GS_TRY(wcnt.alloc(std::max<uint64_t>(nwin, 1)));
+        if (nwin) {
+            k_pb_wmask<<<vgrid(nwin), 256, 0, st>>>(g->pb_src.ptr, qdst.ptr, nwin, wmask.ptr, wcnt.ptr);
+            GS_HIP(hipGetLastError());
+        }
+        GS_TRY(scan_counts(st, wcnt.ptr, nwin, cpos.ptr, &nseg));
+    }
+    // 6. per block: plain or compacted, and its place in the stream (starts
+    //    16-aligned, so plain blocks keep 16-byte stores)
+    std::vector<uint64_t> bw(K + 1), bc(K + 1);
+    for (uint32_t q = 0; q < K; ++q) bw[q] = qm[(uint64_t)q * K] / PB_WIN;
+    bw[K] = nwin;
+    {
+        DevBuf<uint64_t> idx, out;
+        GS_TRY(idx.alloc(K + 1));
+        GS_TRY(out.alloc(K + 1));
+        GS_HIP(hipMemcpyAsync(idx.ptr, bw.data(), (K + 1) * 8, hipMemcpyHostToDevice, st));
+        k_pb_pick<<<vgrid(K + 1), 256, 0, st>>>(cpos.ptr, idx.ptr, K + 1, out.ptr);
+        GS_HIP(hipGetLastError());
+        GS_HIP(hipMemcpyAsync(bc.data(), out.ptr, (K + 1) * 8, hipMemcpyDeviceToHost, st));
+        GS_HIP(hipStreamSynchronize(st));
+    }
+    std::vector<uint64_t> sb(K), slen(K);
+    uint64_t stream = 0;
+    for (uint32_t q = 0; q < K; ++q) {
+        const uint64_t ent = (bw[q + 1] - bw[q]) * PB_WIN, seg = bc[q + 1] - bc[q];
+        const bool comp = ent && seg * 1000 <= (uint64_t)g->pb_compact * ent;
+        sb[q] = stream | (comp ? PB_COMPACT : 0);
+        slen[q] = comp ? seg : ent;
+        stream = (stream + slen[q] + PB_WIN - 1) & ~(uint64_t)(PB_WIN - 1);
+    }
+    // 7. stream base of every run: its block's base plus, for a compacted
+    //    block, the segments before the run, else its entries before it
+    std::vector<uint64_t> rst(nruns);
+    if (nruns) {
+        DevBuf<uint64_t> idx, out;
+        GS_TRY(idx.alloc(nruns));
+        GS_TRY(out.alloc(nruns));
+        GS_HIP(hipMemcpyAsync(idx.ptr, rqw.data(), nruns * 8, hipMemcpyHostToDevice, st));
+        k_pb_pick<<<vgrid(nruns), 256, 0, st>>>(cpos.ptr, idx.ptr, nruns, out.ptr);
+        GS_HIP(hipGetLastError());
+        GS_HIP(hipMemcpyAsync(rst.data(), out.ptr, nruns * 8, hipMemcpyDeviceToHost, st));
+        GS_HIP(hipStreamSynchronize(st));
+        for (uint64_t r = 0; r < nruns; ++r) {
+            const uint32_t q = rblk[r];
+            rst[r] = (sb[q] & PB_COMPACT) ? sb[q] + (rst[r] - bc[q]) : sb[q] + (rqw[r] - bw[q]) * PB_WIN;
+        }
+    }
+    GS_TRY(g->pb_run_pw.alloc(rpw.size()));
+    GS_TRY(g->pb_run_qw.alloc(std::max<uint64_t>(nruns, 1)));
+    GS_TRY(g->pb_run_st.alloc(std::max<uint64_t>(nruns, 1)));
+    GS_HIP(hipMemcpyAsync(g->pb_run_pw.ptr, rpw.data(), rpw.size() * 8, hipMemcpyHostToDevice, st));
+    if (nruns) {
+        GS_HIP(hipMemcpyAsync(g->pb_run_qw.ptr, rqw.data(), nruns * 8, hipMemcpyHostToDevice, st));
+        GS_HIP(hipMemcpyAsync(g->pb_run_st.ptr, rst.data(), nruns * 8, hipMemcpyHostToDevice, st));
+        k_pb_wcoff<<<(unsigned)std::min<uint64_t>((nruns + 3) / 4, 8192), 256, 0, st>>>(
+            g->pb_run_pw.ptr, g->pb_run_qw.ptr, nruns, cpos.ptr, g->pb_wcoff.ptr);
+        GS_HIP(hipGetLastError());
+    }
+    // 8. the stream's destination offsets; the q-major copy is then dropped
+    GS_TRY(g->pb_dst.alloc(stream + PB_WIN));
+    GS_TRY(g->pb_val.alloc(stream + PB_WIN));
+    GS_HIP(hipMemsetAsync(g->pb_dst.ptr, 0, (stream + PB_WIN) * 2, st));
+    if (nwin) {
+        DevBuf<uint64_t> d_bw, d_sb, d_bc;
+        GS_TRY(d_bw.alloc(K + 1));
+        GS_TRY(d_sb.alloc(K));
+        GS_TRY(d_bc.alloc(K + 1));
+        GS_HIP(hipMemcpyAsync(d_bw.ptr, bw.data(), (K + 1) * 8, hipMemcpyHostToDevice, st));
+        GS_HIP(hipMemcpyAsync(d_sb.ptr, sb.data(), K * 8, hipMemcpyHostToDevice, st));
+        GS_HIP(hipMemcpyAsync(d_bc.ptr, bc.data(), (K + 1) * 8, hipMemcpyHostToDevice, st));
+        k_pb_stream_dst<<<vgrid(nwin), 256, 0, st>>>(qdst.ptr, wmask.ptr, nwin, cpos.ptr, d_bw.ptr, d_sb.ptr,
+                                                     d_bc.ptr, K, g->pb_dst.ptr);
+        GS_HIP(hipGetLastError());
+        k_pb_mark<<<vgrid(nwin), 256, 0, st>>>(g->pb_src.ptr, wmask.ptr, nwin, d_bw.ptr, d_sb.ptr, K);
+        GS_HIP(hipGetLastError());
+        GS_HIP(hipStreamSynchronize(st));
+    }
+    qdst.release();
+    cpos.release();
+    wmask.release();
+    // 9. accum units: each block's stream range in equal parts of at most
+    //    pb_unit2 values; split blocks get partial slots
     std::vector<PbUnit2> u2;
     std::vector<PbRed> red;
     uint32_t slots = 0;
     for (uint32_t q = 0; q < K; ++q) {
-        const uint64_t e0 = qm[(uint64_t)q * K], e1 = q + 1 < K ? qm[(uint64_t)(q + 1) * K] : run;
-        if (e1 == e0) continue;
-        const uint64_t nw = (e1 - e0) / PB_WIN;
-        const uint64_t parts = std::min<uint64_t>((e1 - e0 + unit2 - 1) / unit2, nw);
-        // entries per vertex of the block: above PB_DENSE, lanes of a wave mostly share destinations
+        const uint64_t e0 = sb[q] & ~PB_COMPACT, n_ = slen[q], e1 = e0 + n_;
+        if (!n_) continue;
+        const uint64_t parts = (n_ + unit2 - 1) / unit2;
+        // stream values per vertex of the block: above pb_dense, lanes of a wave mostly share destinations
         const uint64_t qn = std::min<uint64_t>(PB_B, nzn - ((uint64_t)q << PB_SHIFT));
-        const uint32_t dense = (e1 - e0) >= (uint64_t)g->pb_dense * qn;
+        const uint32_t dense = n_ >= (uint64_t)g->pb_dense * qn;
         if (parts == 1) {
             u2.push_back(PbUnit2{e0, e1, q, 0xFFFFFFFFu, dense, 0});
             continue;
         }
         red.push_back(PbRed{q, slots, (uint32_t)parts, 0});
         for (uint64_t k = 0; k < parts; ++k)
-            u2.push_back(PbUnit2{e0 + nw * k / parts * PB_WIN, e0 + nw * (k + 1) / parts * PB_WIN, q, slots++, dense, 0});
+            u2.push_back(PbUnit2{e0 + n_ * k / parts, e0 + n_ * (k + 1) / parts, q, slots++, dense, 0});
     }
-    GS_TRY(check_units(u1, u2, red, rpw, run, runs_max, slots));
-    // padding: the accumulation reads whole 16-entry groups from any unit start
-    GS_TRY(g->pb_src.alloc(run + PB_WIN));
-    GS_TRY(g->pb_dst.alloc(run + PB_WIN));
-    GS_TRY(g->pb_val.alloc(run + PB_WIN));
-    GS_TRY(g->pb_run_pw.alloc(rpw.size()));
-    GS_TRY(g->pb_run_qw.alloc(std::max<uint64_t>(nruns, 1)));
+    GS_TRY(check_units(u1, u2, red, rpw, run, runs_max, slots, sb, slen));
     GS_TRY(g->pb_u1.alloc(u1.size()));
     GS_TRY(g->pb_u2.alloc(u2.size()));
     GS_TRY(g->pb_red.alloc(red.size()));
     GS_TRY(g->pb_part.alloc((uint64_t)slots * PB_B));
-    // 6. the copy, the padding of each run's last window, and the unit tables
-    GS_HIP(hipMemcpyAsync(cnt.ptr, qm.data(), KK * 8, hipMemcpyHostToDevice, st));
-    if (!slices.empty()) {
-        k_pb_fill<<<(unsigned)slices.size(), PB_WG, 0, st>>>(d_sl.ptr, g->adj_off.ptr, g->adj_nbr.ptr, nzn, K,
-                                                             cnt.ptr, g->pb_src.ptr, g->pb_dst.ptr);
-        GS_HIP(hipGetLastError());
-        k_pb_pad<<<vgrid(KK), 256, 0, st>>>(cnt.ptr, KK, g->pb_src.ptr, g->pb_dst.ptr);
-        GS_HIP(hipGetLastError());
-    }
-    GS_HIP(hipMemsetAsync(g->pb_src.ptr + run, 0, PB_WIN * 2, st));
-    GS_HIP(hipMemsetAsync(g->pb_dst.ptr + run, 0, PB_WIN * 2, st));
-    GS_HIP(hipMemcpyAsync(g->pb_run_pw.ptr, rpw.data(), rpw.size() * 8, hipMemcpyHostToDevice, st));
-    if (nruns) GS_HIP(hipMemcpyAsync(g->pb_run_qw.ptr, rqw.data(), nruns * 8, hipMemcpyHostToDevice, st));
     if (!u1.empty())
         GS_HIP(hipMemcpyAsync(g->pb_u1.ptr, u1.data(), u1.size() * sizeof(PbUnit1), hipMemcpyHostToDevice, st));
     if (!u2.empty())
         GS_HIP(hipMemcpyAsync(g->pb_u2.ptr, u2.data(), u2.size() * sizeof(PbUnit2), hipMemcpyHostToDevice, st));
     if (!red.empty())
         GS_HIP(hipMemcpyAsync(g->pb_red.ptr, red.data(), red.size() * sizeof(PbRed), hipMemcpyHostToDevice, st));
+    g->pb_stream = stream;
     GS_HIP(hipEventRecord(t1, st));
     GS_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
@@ -663,7 +881,8 @@ static int pb_gather(const gs_graph *g, gs_ws *w, const float *prior) {
     hipEvent_t kb;
     w->timer.begin(GS_K_PB_GATHER, w->stream, &kb);
     k_pb_gather<<<(unsigned)g->pb_nu1, PB_WG, 0, w->stream>>>(g->pb_u1.ptr, g->pb_run_pw.ptr, g->pb_run_qw.ptr,
-                                                              g->pb_src.ptr, prior, g->nzn, g->pb_val.ptr);
+                                                              g->pb_run_st.ptr, g->pb_src.ptr, g->pb_wcoff.ptr, prior,
+                                                              g->nzn, g->pb_val.ptr);
     GS_HIP(hipGetLastError());
     w->timer.end(GS_K_PB_GATHER, w->stream, kb);
     return GS_OK;

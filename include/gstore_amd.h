@@ -261,16 +261,18 @@ int gs_set_batch_bfs_method(gs_graph *g, int method);
 int gs_set_kcore_method(gs_graph *g, int method);
 
 /* Work split of the blocked PageRank: source-major entries (<= 2^30) and
- * runs (<= 2048) per gather workgroup, entries per accumulation workgroup
- * (<= 2^30), and the entries per vertex from which a destination block adds
- * with LDS float atomics rather than compare-and-swap.  Defaults 2^20, 2048,
- * 2^20, 8192 (on Kron-28 the hardware atomic wins only on the blocks of the
- * heaviest vertices, whose sorted runs put one destination in many lanes of
- * a wave).  Small values exercise the split paths on small graphs; results
- * change only in FP32 (FP64 for personalised PageRank) summation order.  No
- * counterpart in the reference. */
+ * runs (<= 1024) per gather workgroup, stream values per accumulation
+ * workgroup (<= 2^30), the stream values per vertex from which a destination
+ * block adds with LDS float atomics rather than compare-and-swap, and the
+ * compaction threshold in permille (0..1000): a destination block whose
+ * segments of equal destinations (per 16-entry window) number at most that
+ * share of its entries gets one stream value per segment, summed by the
+ * gather, instead of one per entry (0: never, 1000: always).  Defaults 2^20,
+ * 1024, 2^20, 8192, 500.  Small values exercise the split paths on small
+ * graphs; results change only in FP32 (FP64 for personalised PageRank)
+ * summation order.  No counterpart in the reference. */
 int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries,
-                             uint32_t dense_entries);
+                             uint32_t dense_entries, uint32_t compact_permille);
 
 /* PageRank / personalised PageRank on undirected grids: 1 (default) adds
  * into the (at most 2048) highest-degree vertices through a per-workgroup LDS
