@@ -290,7 +290,7 @@ class Graph:
 
     def set_pagerank_blocking(self, gather_entries: int = 1 << 20, gather_runs: int = 1024,
                               accum_entries: int = 1 << 20, dense_entries: int = 8192,
-                              compact_permille: int = 500) -> None:
+                              compact_permille: int = 650) -> None:
         """Work split and layout of the blocked PageRank (small values exercise the split paths)."""
         _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries, dense_entries,
                                              compact_permille))

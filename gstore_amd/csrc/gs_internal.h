@@ -222,7 +222,7 @@ struct gs_graph {
     uint32_t pb_dense = 8192;                // accum: entries per vertex from which a block uses LDS float atomics
     uint64_t pb_entries = 0;                 // 2E plus the padding that fills each run's last window
     uint64_t pb_stream = 0;                  // accumulation stream length (with block alignment)
-    uint32_t pb_compact = 500;               // a block is compacted when its segments <= this / 1000 of its entries
+    uint32_t pb_compact = 650;               // a block is compacted when its segments <= this / 1000 of its entries
     gs::DevBuf<uint16_t> pb_src;             // pb_entries, q-major: offset of u in its block (compacted
                                              // blocks: bit 15 marks a segment's last entry)
     gs::DevBuf<uint32_t> pb_wcoff;           // per q-major window: its first segment, relative to its run

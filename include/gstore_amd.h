@@ -268,7 +268,7 @@ int gs_set_kcore_method(gs_graph *g, int method);
  * segments of equal destinations (per 16-entry window) number at most that
  * share of its entries gets one stream value per segment, summed by the
  * gather, instead of one per entry (0: never, 1000: always).  Defaults 2^20,
- * 1024, 2^20, 8192, 500.  Small values exercise the split paths on small
+ * 1024, 2^20, 8192, 650.  Small values exercise the split paths on small
  * graphs; results change only in FP32 (FP64 for personalised PageRank)
  * summation order.  No counterpart in the reference. */
 int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries,
