@@ -306,9 +306,9 @@ typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
 // stores); a run of a compacted block writes one sum per segment of equal
 // destinations (bit 15 of a source offset marks each segment's last entry,
 // padding follows the last mark) at the window's place in the compacted stream.
-// The first eight sums of a window collect in registers and leave in at most
-// four vector stores (16, 16, 8 and 4 bytes, 4-byte aligned): one scalar store
-// per segment cost about twice as much at Kron-28.
+// A window's sums collect in registers (selects, no indexing) and leave in
+// 16-, 8- and 4-byte stores at 4-byte alignment: one scalar store per segment
+// made the gather 30% slower at Kron-28.
 __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__ units,
                                                      const uint64_t *__restrict__ run_pw,
                                                      const uint64_t *__restrict__ run_qw,
@@ -363,9 +363,9 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
             if (wb + j * PB_WG >= len) break;
             if (st[j] & PB_COMPACT) {
                 float *out = val + (st[j] & ~PB_COMPACT) + co[j];
-                float acc = 0.0f, sl[8];
+                float acc = 0.0f, sl[PB_WIN];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) sl[k] = 0.0f;
+                for (int k = 0; k < PB_WIN; ++k) sl[k] = 0.0f;
                 uint32_t seg = 0;
 #pragma unroll
                 for (int i = 0; i < PB_WIN; ++i) {
@@ -373,22 +373,22 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
                     acc += pri[s & 0x7FFFu];
                     const bool e = s >> 15;
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) sl[k] = (e && seg == (uint32_t)k) ? acc : sl[k];
-                    if (e && seg >= 8) out[seg] = acc;
+                    for (int k = 0; k <= i; ++k) sl[k] = (e && seg == (uint32_t)k) ? acc : sl[k];
                     seg += e;
                     acc = e ? 0.0f : acc;
                 }
-                const uint32_t n = seg < 8 ? seg : 8;
-                if (n >= 4) {
-                    const pb_f32x4 a = {sl[0], sl[1], sl[2], sl[3]};
-                    __builtin_memcpy(out, &a, 16);
-                }
-                if (n >= 8) {
-                    const pb_f32x4 a = {sl[4], sl[5], sl[6], sl[7]};
-                    __builtin_memcpy(out + 4, &a, 16);
-                }
-                const uint32_t b = n & 4, r = n & 3;
-                const float t0 = b ? sl[4] : sl[0], t1 = b ? sl[5] : sl[1], t2 = b ? sl[6] : sl[2];
+                const uint32_t n = seg;
+#pragma unroll
+                for (int g4 = 0; g4 < 4; ++g4)
+                    if (n >= 4u * (g4 + 1)) {
+                        const pb_f32x4 a = {sl[4 * g4], sl[4 * g4 + 1], sl[4 * g4 + 2], sl[4 * g4 + 3]};
+                        __builtin_memcpy(out + 4 * g4, &a, 16);
+                    }
+                const uint32_t b = n & ~3u, r = n & 3;
+                float t0 = sl[0], t1 = sl[1], t2 = sl[2];
+#pragma unroll
+                for (int g4 = 1; g4 < 4; ++g4)
+                    if (b == 4u * g4) { t0 = sl[4 * g4]; t1 = sl[4 * g4 + 1]; t2 = sl[4 * g4 + 2]; }
                 if (r >= 2) {
                     const float2 a = {t0, t1};
                     __builtin_memcpy(out + b, &a, 8);
