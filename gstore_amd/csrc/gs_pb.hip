@@ -297,6 +297,49 @@ __device__ inline void stage_block(const float *__restrict__ prior, uint64_t pba
 typedef unsigned int pb_u32x4 __attribute__((ext_vector_type(4)));
 typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
 
+// Segment sums of one compacted window (at most S segments) into out[0 ..
+// segments): the running sum of each segment lands in register slot `seg` by
+// selects (no indexing), then leaves in 16-, 8- and 4-byte stores at 4-byte
+// alignment.
+template <int S>
+__device__ inline void pb_segments(const pb_u32x4 (&sq)[2], const float *pri, float *out) {
+    float acc = 0.0f, sl[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) sl[k] = 0.0f;
+    uint32_t seg = 0;
+#pragma unroll
+    for (int i = 0; i < PB_WIN; ++i) {
+        const uint32_t s = (sq[i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
+        acc += pri[s & 0x7FFFu];
+        const bool e = s >> 15;
+#pragma unroll
+        for (int k = 0; k < S && k <= i; ++k) sl[k] = (e && seg == (uint32_t)k) ? acc : sl[k];
+        seg += e;
+        acc = e ? 0.0f : acc;
+    }
+    const uint32_t n = seg;
+#pragma unroll
+    for (int g4 = 0; g4 < S / 4; ++g4)
+        if (n >= 4u * (g4 + 1)) {
+            const pb_f32x4 a = {sl[4 * g4], sl[4 * g4 + 1], sl[4 * g4 + 2], sl[4 * g4 + 3]};
+            __builtin_memcpy(out + 4 * g4, &a, 16);
+        }
+    const uint32_t b = n & ~3u, r = n & 3;
+    float t0 = sl[0], t1 = sl[1], t2 = S > 2 ? sl[S > 2 ? 2 : 0] : 0.0f;
+#pragma unroll
+    for (int g4 = 1; g4 < S / 4; ++g4)
+        if (b == 4u * g4) {
+            t0 = sl[4 * g4];
+            t1 = sl[4 * g4 + 1];
+            t2 = sl[4 * g4 + 2];
+        }
+    if (r >= 2) {
+        const float2 a = {t0, t1};
+        __builtin_memcpy(out + b, &a, 8);
+    }
+    if (r & 1) out[b + (r & 2)] = (r & 2) ? t2 : t0;
+}
+
 // Values of the windows [w0, w1) of source block p (counted in source-major
 // order) into the accumulation stream; the unit's runs are staged in LDS.
 // Lane i of the workgroup takes windows w0 + i, w0 + i + PB_WG, ... (adjacent
@@ -306,8 +349,9 @@ typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
 // stores); a run of a compacted block writes one sum per segment of equal
 // destinations (bit 15 of a source offset marks each segment's last entry,
 // padding follows the last mark) at the window's place in the compacted stream.
-// A window's sums collect in registers (selects, no indexing) and leave in
-// 16-, 8- and 4-byte stores at 4-byte alignment: one scalar store per segment
+// A window's sums collect in registers (pb_segments; the slot count follows
+// the wave's largest segment count, so windows of the heaviest blocks pay for
+// two or four slots) and leave in vector stores: one scalar store per segment
 // made the gather 30% slower at Kron-28.
 __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__ units,
                                                      const uint64_t *__restrict__ run_pw,
@@ -363,37 +407,17 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
             if (wb + j * PB_WG >= len) break;
             if (st[j] & PB_COMPACT) {
                 float *out = val + (st[j] & ~PB_COMPACT) + co[j];
-                float acc = 0.0f, sl[PB_WIN];
+                uint32_t n = 0;
 #pragma unroll
-                for (int k = 0; k < PB_WIN; ++k) sl[k] = 0.0f;
-                uint32_t seg = 0;
-#pragma unroll
-                for (int i = 0; i < PB_WIN; ++i) {
-                    const uint32_t s = (sq[j][i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
-                    acc += pri[s & 0x7FFFu];
-                    const bool e = s >> 15;
-#pragma unroll
-                    for (int k = 0; k <= i; ++k) sl[k] = (e && seg == (uint32_t)k) ? acc : sl[k];
-                    seg += e;
-                    acc = e ? 0.0f : acc;
+                for (int h = 0; h < 8; ++h) {
+                    const uint32_t a = sq[j][h >> 2][h & 3];
+                    n += ((a >> 15) & 1u) + (a >> 31);
                 }
-                const uint32_t n = seg;
-#pragma unroll
-                for (int g4 = 0; g4 < 4; ++g4)
-                    if (n >= 4u * (g4 + 1)) {
-                        const pb_f32x4 a = {sl[4 * g4], sl[4 * g4 + 1], sl[4 * g4 + 2], sl[4 * g4 + 3]};
-                        __builtin_memcpy(out + 4 * g4, &a, 16);
-                    }
-                const uint32_t b = n & ~3u, r = n & 3;
-                float t0 = sl[0], t1 = sl[1], t2 = sl[2];
-#pragma unroll
-                for (int g4 = 1; g4 < 4; ++g4)
-                    if (b == 4u * g4) { t0 = sl[4 * g4]; t1 = sl[4 * g4 + 1]; t2 = sl[4 * g4 + 2]; }
-                if (r >= 2) {
-                    const float2 a = {t0, t1};
-                    __builtin_memcpy(out + b, &a, 8);
-                }
-                if (r & 1) out[b + (r & 2)] = (r & 2) ? t2 : t0;
+                // register slots for the wave's largest segment count (wave-uniform branches)
+                if (__ballot(n > 8)) pb_segments<16>(sq[j], pri, out);
+                else if (__ballot(n > 4)) pb_segments<8>(sq[j], pri, out);
+                else if (__ballot(n > 2)) pb_segments<4>(sq[j], pri, out);
+                else pb_segments<2>(sq[j], pri, out);
             } else {
                 float x[PB_WIN];
 #pragma unroll
