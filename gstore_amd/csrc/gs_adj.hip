@@ -154,40 +154,61 @@ __global__ void __launch_bounds__(256) k_td_edges(const uint32_t *__restrict__ q
     }
 }
 
-// Bottom-up: one lane per vertex; an unreached vertex with neighbours scans
-// them until it meets a frontier vertex.  Each wave owns 64 consecutive
-// vertices, so it writes its two next-bitmap words without atomics.
+// Bottom-up: two vertices per lane (v and v + 64 of the wave's 128), so each
+// lane keeps both vertices' loads in flight; an unreached vertex with
+// neighbours scans them until it meets a frontier vertex.  Each wave owns 128
+// consecutive vertices, so it writes its four next-bitmap words without
+// atomics.  Lanes that are done load nbr[0] (in bounds) rather than branch, so
+// the loads of the other vertex still issue together.
 __global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint64_t *__restrict__ off,
                                                    const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
                                                    const uint32_t *__restrict__ front, uint32_t *next,
                                                    unsigned long long *edges_checked) {
     const int lane = threadIdx.x & 63;
     uint32_t checked = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull; base < n; base += stride) {
-        uint64_t v = base + lane;
-        bool found = false;
-        if (v < n && !((vis[v >> 5] >> (v & 31)) & 1u)) {
-            const uint64_t b = off[v], e = off[v + 1];
-            // four neighbours per round trip: ids, then their frontier words, all loads in flight
-            for (uint64_t k = b; k < e && !found; k += 4) {
-                uint32_t u[4], fw[4];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2;
+    for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull) * 2; base < n; base += stride) {
+        uint64_t v[2] = {base + lane, base + 64 + lane}, k[2] = {0, 0}, e[2] = {0, 0};
+        bool found[2] = {false, false};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) u[j] = nbr[min(k + j, e - 1)];
+        for (int h = 0; h < 2; ++h)
+            if (v[h] < n && !((vis[v[h] >> 5] >> (v[h] & 31)) & 1u)) {
+                k[h] = off[v[h]];
+                e[h] = off[v[h] + 1];
+            }
+        // four neighbours of each vertex per round trip: ids, then their frontier words
+        while ((k[0] < e[0] && !found[0]) || (k[1] < e[1] && !found[1])) {
+            bool live[2];
+            uint32_t u[2][4], fw[2][4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) fw[j] = front[u[j] >> 5];
+            for (int h = 0; h < 2; ++h) {
+                live[h] = k[h] < e[h] && !found[h];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) u[h][j] = nbr[live[h] ? min(k[h] + j, e[h] - 1) : 0];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fw[h][j] = front[u[h][j] >> 5];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (!live[h]) continue;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (!found && k + j < e) {
+                    if (!found[h] && k[h] + j < e[h]) {
                         ++checked;
-                        found = (fw[j] >> (u[j] & 31)) & 1u;
+                        found[h] = (fw[h][j] >> (u[h][j] & 31)) & 1u;
                     }
+                k[h] += 4;
             }
         }
-        unsigned long long m = __ballot(found);
-        if (lane == 0 && m) {
-            next[base >> 5] = (uint32_t)m;
-            next[(base >> 5) + 1] = (uint32_t)(m >> 32);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const unsigned long long m = __ballot(found[h]);
+            if (lane == 0 && m) {
+                next[((base + 64 * h) >> 5)] = (uint32_t)m;
+                next[((base + 64 * h) >> 5) + 1] = (uint32_t)(m >> 32);
+            }
         }
     }
     if (checked) atomicAdd(edges_checked, (unsigned long long)checked);
@@ -196,10 +217,66 @@ __global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint64_t *_
 // Level end for the direction-optimizing BFS: as k_bfs_level_end, plus the
 // degree sum of the new frontier and its vertex queue (with the degrees) for
 // a top-down level.
-// Queue slots are reserved once per workgroup and step (block scan of the
-// per-word counts), counts and degree sums added once per workgroup.
 // LEVEL_END: next becomes the frontier (bitmaps and depths updated); else the
 // current frontier is only listed.  BUILD: write the queue and its degrees.
+// Each thread takes LE_WPT consecutive words per step.  With BUILD, queue
+// slots are reserved once per workgroup and step (block scan of the thread
+// counts): one global atomic per 32K vertices, where one per 8K serialised on
+// the cursor.  Without BUILD there is no scan and no reservation.  Counts and
+// degree sums are added once per workgroup.
+constexpr int LE_WPT = 4;
+
+template <bool LEVEL_END, bool BUILD>
+__device__ inline void do_level_word(uint32_t nx, uint64_t gw, uint64_t n, uint8_t *depth, uint32_t newdepth,
+                                     const int32_t *__restrict__ deg, uint32_t *queue, int32_t *qdeg,
+                                     unsigned long long &pos, unsigned long long &msum) {
+    const uint64_t v0 = gw * 32;
+    if (nx && v0 + 32 <= n) {
+        // the word's 32 degrees and depth bytes as 16-byte loads, updated in registers
+        int4 dq[8];
+        uint4 bq[2];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dq[k] = reinterpret_cast<const int4 *>(deg + v0)[k];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) bq[k] = LEVEL_END ? reinterpret_cast<const uint4 *>(depth + v0)[k] : uint4{};
+#pragma unroll
+        for (int b = 0; b < 32; ++b) {
+            if (!((nx >> b) & 1u)) continue;
+            const int4 d4 = dq[b >> 2];
+            const int32_t dv = (b & 3) == 0 ? d4.x : (b & 3) == 1 ? d4.y : (b & 3) == 2 ? d4.z : d4.w;
+            if (LEVEL_END) {
+                uint4 &bw = bq[b >> 4];
+                uint32_t &word = ((b >> 2) & 3) == 0 ? bw.x : ((b >> 2) & 3) == 1 ? bw.y
+                               : ((b >> 2) & 3) == 2 ? bw.z : bw.w;
+                word = (word & ~(0xFFu << ((b & 3) * 8))) | (newdepth << ((b & 3) * 8));
+            }
+            msum += (uint32_t)dv;
+            if (BUILD) {
+                qdeg[pos] = dv;
+                queue[pos++] = (uint32_t)(v0 + b);
+            }
+        }
+        if (LEVEL_END) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) reinterpret_cast<uint4 *>(depth + v0)[k] = bq[k];
+        }
+    } else {
+        uint32_t m = nx;
+        while (m) {
+            const uint32_t bit = __ffs(m) - 1;
+            m &= m - 1;
+            const uint64_t v = v0 + bit;
+            if (LEVEL_END) depth[v] = (uint8_t)newdepth;
+            const int32_t dv = deg[v];
+            msum += (uint32_t)dv;
+            if (BUILD) {
+                qdeg[pos] = dv;
+                queue[pos++] = (uint32_t)v;
+            }
+        }
+    }
+}
+
 template <bool LEVEL_END, bool BUILD>
 __global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *front, uint32_t *next, uint8_t *depth,
                                                       uint32_t newdepth, const int32_t *__restrict__ deg, uint64_t n,
@@ -210,85 +287,71 @@ __global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *f
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (threadIdx.x == 0) bcnt = bdeg = 0;
     unsigned long long cnt = 0, msum = 0;
-    for (uint64_t base = blockIdx.x * 256ull; base < nwords; base += (uint64_t)gridDim.x * 256) {
-        const uint64_t gw = base + threadIdx.x;
-        uint32_t nx = 0;
-        if (gw < nwords) {
+    for (uint64_t base = blockIdx.x * (256ull * LE_WPT); base < nwords; base += (uint64_t)gridDim.x * 256 * LE_WPT) {
+        const uint64_t g0 = base + threadIdx.x * LE_WPT;
+        uint32_t nx[LE_WPT];
+#pragma unroll
+        for (int k = 0; k < LE_WPT; ++k) nx[k] = 0;
+        if (g0 + LE_WPT <= nwords) {
             if (LEVEL_END) {
-                nx = next[gw];
-                front[gw] = nx;
-                if (nx) {
-                    next[gw] = 0;
-                    vis[gw] |= nx;
+                const uint4 q = *reinterpret_cast<const uint4 *>(next + g0);
+                nx[0] = q.x; nx[1] = q.y; nx[2] = q.z; nx[3] = q.w;
+                *reinterpret_cast<uint4 *>(front + g0) = q;
+                if (q.x | q.y | q.z | q.w) {
+                    *reinterpret_cast<uint4 *>(next + g0) = uint4{0, 0, 0, 0};
+                    uint4 vv = *reinterpret_cast<const uint4 *>(vis + g0);
+                    vv.x |= q.x; vv.y |= q.y; vv.z |= q.z; vv.w |= q.w;
+                    *reinterpret_cast<uint4 *>(vis + g0) = vv;
                 }
             } else {
-                nx = front[gw];
-            }
-        }
-        const uint32_t c = __popc(nx);
-        uint32_t incl = c;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o);
-            if (lane >= o) incl += y;
-        }
-        if (lane == 63) wsum[wv] = incl;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-            bbase = t ? atomicAdd(&count[2], (unsigned long long)t) : 0ull;
-        }
-        __syncthreads();
-        uint32_t before = 0;
-        for (int k = 0; k < wv; ++k) before += wsum[k];
-        unsigned long long pos = bbase + before + incl - c;
-        cnt += c;
-        const uint64_t v0 = gw * 32;
-        if (nx && v0 + 32 <= n) {
-            // the word's 32 degrees and depth bytes as 16-byte loads, updated in registers
-            int4 dq[8];
-            uint4 bq[2];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) dq[k] = reinterpret_cast<const int4 *>(deg + v0)[k];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) bq[k] = LEVEL_END ? reinterpret_cast<const uint4 *>(depth + v0)[k] : uint4{};
-#pragma unroll
-            for (int b = 0; b < 32; ++b) {
-                if (!((nx >> b) & 1u)) continue;
-                const int4 d4 = dq[b >> 2];
-                const int32_t dv = (b & 3) == 0 ? d4.x : (b & 3) == 1 ? d4.y : (b & 3) == 2 ? d4.z : d4.w;
-                if (LEVEL_END) {
-                    uint4 &bw = bq[b >> 4];
-                    uint32_t &word = ((b >> 2) & 3) == 0 ? bw.x : ((b >> 2) & 3) == 1 ? bw.y
-                                   : ((b >> 2) & 3) == 2 ? bw.z : bw.w;
-                    word = (word & ~(0xFFu << ((b & 3) * 8))) | (newdepth << ((b & 3) * 8));
-                }
-                msum += (uint32_t)dv;
-                if (BUILD) {
-                    qdeg[pos] = dv;
-                    queue[pos++] = (uint32_t)(v0 + b);
-                }
-            }
-            if (LEVEL_END) {
-#pragma unroll
-                for (int k = 0; k < 2; ++k) reinterpret_cast<uint4 *>(depth + v0)[k] = bq[k];
+                const uint4 q = *reinterpret_cast<const uint4 *>(front + g0);
+                nx[0] = q.x; nx[1] = q.y; nx[2] = q.z; nx[3] = q.w;
             }
         } else {
-            uint32_t m = nx;
-            while (m) {
-                const uint32_t bit = __ffs(m) - 1;
-                m &= m - 1;
-                const uint64_t v = v0 + bit;
-                if (LEVEL_END) depth[v] = (uint8_t)newdepth;
-                const int32_t dv = deg[v];
-                msum += (uint32_t)dv;
-                if (BUILD) {
-                    qdeg[pos] = dv;
-                    queue[pos++] = (uint32_t)v;
+#pragma unroll
+            for (int k = 0; k < LE_WPT; ++k) {
+                const uint64_t gw = g0 + k;
+                if (gw >= nwords) break;
+                if (LEVEL_END) {
+                    nx[k] = next[gw];
+                    front[gw] = nx[k];
+                    if (nx[k]) {
+                        next[gw] = 0;
+                        vis[gw] |= nx[k];
+                    }
+                } else {
+                    nx[k] = front[gw];
                 }
             }
         }
-        __syncthreads();  // wsum and bbase are rewritten by the next step
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < LE_WPT; ++k) c += __popc(nx[k]);
+        cnt += c;
+        unsigned long long pos = 0;
+        if (BUILD) {
+            uint32_t incl = c;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            if (lane == 63) wsum[wv] = incl;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+                bbase = t ? atomicAdd(&count[2], (unsigned long long)t) : 0ull;
+            }
+            __syncthreads();
+            uint32_t before = 0;
+            for (int k = 0; k < wv; ++k) before += wsum[k];
+            pos = bbase + before + incl - c;
+        }
+#pragma unroll
+        for (int k = 0; k < LE_WPT; ++k)
+            if (nx[k])
+                do_level_word<LEVEL_END, BUILD>(nx[k], g0 + k, n, depth, newdepth, deg, queue, qdeg, pos, msum);
+        if (BUILD) __syncthreads();  // wsum and bbase are rewritten by the next step
     }
     if (cnt) {
         atomicAdd(&bcnt, cnt);
@@ -704,7 +767,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     }
     bool bottom_up = false, have_queue = true;
     int nlev = 0, status = GS_OK;
-    uint64_t checked = 0;
+    uint64_t checked = 0, n_f_before = 0;
     std::vector<unsigned long long> hsum;
     w->levels.clear();
     hipEvent_t la, lb;
@@ -714,21 +777,25 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
         const uint64_t m_f_prev = m_f;  // edges a top-down level examines
         const uint64_t n_f_prev = n_f;
         GS_HIP(hipEventRecord(la, w->stream));
-        if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha) bottom_up = true;
+        // top-down to bottom-up only while the frontier grows: a shrinking tail
+        // with few unchecked edges left stays top-down, where a bottom-up pass
+        // would still walk every vertex's visited bit
+        if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha && n_f > n_f_before) bottom_up = true;
         else if (bottom_up && n_f < g->n / (uint64_t)g->do_beta) bottom_up = false;
+        n_f_before = n_f;
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 32, w->stream));
         hipEvent_t kb;
         const int kclass = bottom_up ? GS_K_BFS_BOTTOMUP : GS_K_BFS_TOPDOWN;
         w->timer.begin(kclass, w->stream, &kb);
         if (bottom_up) {  // vertices without edges (relabelled ids from nzn on) can never be reached
             if (g->nzn)
-                k_bu_expand<<<vgrid(g->nzn), 256, 0, w->stream>>>(g->nzn, g->adj_off.ptr, g->adj_nbr.ptr,
+                k_bu_expand<<<vgrid((g->nzn + 1) / 2), 256, 0, w->stream>>>(g->nzn, g->adj_off.ptr, g->adj_nbr.ptr,
                                                                   w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
                                                                   w->counters.ptr + 3);
         } else if (n_f && m_f) {
             if (!have_queue) {  // the frontier came from a bottom-up level: list it
                 GS_HIP(hipMemsetAsync(w->counters.ptr + 4, 0, 24, w->stream));
-                k_do_level_end<false, true><<<vgrid(act_words), 256, 0, w->stream>>>(
+                k_do_level_end<false, true><<<vgrid((act_words + LE_WPT - 1) / LE_WPT), 256, 0, w->stream>>>(
                     w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, 0, g->deg_r.ptr, g->n, act_words,
                     w->queue.ptr, w->qdeg.ptr, w->counters.ptr + 4);
                 GS_HIP(hipGetLastError());
@@ -743,11 +810,11 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
         // after a top-down level the next one is likely top-down too: list the new frontier now
         have_queue = !bottom_up;
         if (act_words && have_queue)
-            k_do_level_end<true, true><<<vgrid(act_words), 256, 0, w->stream>>>(
+            k_do_level_end<true, true><<<vgrid((act_words + LE_WPT - 1) / LE_WPT), 256, 0, w->stream>>>(
                 w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n,
                 act_words, w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
         else if (act_words)
-            k_do_level_end<true, false><<<vgrid(act_words), 256, 0, w->stream>>>(
+            k_do_level_end<true, false><<<vgrid((act_words + LE_WPT - 1) / LE_WPT), 256, 0, w->stream>>>(
                 w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n,
                 act_words, w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
         GS_HIP(hipGetLastError());
