@@ -154,56 +154,67 @@ __global__ void __launch_bounds__(256) k_td_edges(const uint32_t *__restrict__ q
     }
 }
 
-// Bottom-up: two vertices per lane (v and v + 64 of the wave's 128), so each
-// lane keeps both vertices' loads in flight; an unreached vertex with
-// neighbours scans them until it meets a frontier vertex.  Each wave owns 128
-// consecutive vertices, so it writes its four next-bitmap words without
+// Bottom-up: BU_VPL vertices per lane (v, v + 64, ... of the wave's 64 x BU_VPL),
+// so each lane keeps all its vertices' loads in flight; an unreached vertex
+// with neighbours scans them until it meets a frontier vertex.  Each wave owns
+// 64 x BU_VPL consecutive vertices, so it writes its next-bitmap words without
 // atomics.  Lanes that are done load nbr[0] (in bounds) rather than branch, so
-// the loads of the other vertex still issue together.
+// the loads of the other vertices still issue together.
+constexpr int BU_VPL = 4;
+
 __global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint64_t *__restrict__ off,
                                                    const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
                                                    const uint32_t *__restrict__ front, uint32_t *next,
                                                    unsigned long long *edges_checked) {
     const int lane = threadIdx.x & 63;
     uint32_t checked = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2;
-    for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull) * 2; base < n; base += stride) {
-        uint64_t v[2] = {base + lane, base + 64 + lane}, k[2] = {0, 0}, e[2] = {0, 0};
-        bool found[2] = {false, false};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * BU_VPL;
+    for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull) * BU_VPL; base < n;
+         base += stride) {
+        uint64_t v[BU_VPL], k[BU_VPL], e[BU_VPL];
+        bool found[BU_VPL], any = false;
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < BU_VPL; ++h) {
+            v[h] = base + 64 * h + lane;
+            k[h] = e[h] = 0;
+            found[h] = false;
             if (v[h] < n && !((vis[v[h] >> 5] >> (v[h] & 31)) & 1u)) {
                 k[h] = off[v[h]];
                 e[h] = off[v[h] + 1];
             }
+            any = any || k[h] < e[h];
+        }
         // four neighbours of each vertex per round trip: ids, then their frontier words
-        while ((k[0] < e[0] && !found[0]) || (k[1] < e[1] && !found[1])) {
-            bool live[2];
-            uint32_t u[2][4], fw[2][4];
+        while (any) {
+            bool live[BU_VPL];
+            uint32_t u[BU_VPL][4], fw[BU_VPL][4];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < BU_VPL; ++h) {
                 live[h] = k[h] < e[h] && !found[h];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) u[h][j] = nbr[live[h] ? min(k[h] + j, e[h] - 1) : 0];
             }
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < BU_VPL; ++h)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) fw[h][j] = front[u[h][j] >> 5];
+            any = false;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                if (!live[h]) continue;
+            for (int h = 0; h < BU_VPL; ++h) {
+                if (live[h]) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (!found[h] && k[h] + j < e[h]) {
-                        ++checked;
-                        found[h] = (fw[h][j] >> (u[h][j] & 31)) & 1u;
-                    }
-                k[h] += 4;
+                    for (int j = 0; j < 4; ++j)
+                        if (!found[h] && k[h] + j < e[h]) {
+                            ++checked;
+                            found[h] = (fw[h][j] >> (u[h][j] & 31)) & 1u;
+                        }
+                    k[h] += 4;
+                }
+                any = any || (k[h] < e[h] && !found[h]);
             }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < BU_VPL; ++h) {
             const unsigned long long m = __ballot(found[h]);
             if (lane == 0 && m) {
                 next[((base + 64 * h) >> 5)] = (uint32_t)m;
@@ -789,7 +800,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
         w->timer.begin(kclass, w->stream, &kb);
         if (bottom_up) {  // vertices without edges (relabelled ids from nzn on) can never be reached
             if (g->nzn)
-                k_bu_expand<<<vgrid((g->nzn + 1) / 2), 256, 0, w->stream>>>(g->nzn, g->adj_off.ptr, g->adj_nbr.ptr,
+                k_bu_expand<<<vgrid((g->nzn + BU_VPL - 1) / BU_VPL), 256, 0, w->stream>>>(g->nzn, g->adj_off.ptr, g->adj_nbr.ptr,
                                                                   w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
                                                                   w->counters.ptr + 3);
         } else if (n_f && m_f) {
