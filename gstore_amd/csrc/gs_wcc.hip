@@ -12,8 +12,9 @@
 // vertex id, which is the canonical label returned.  A check pass counts
 // edges whose endpoints still differ and repeats the hook pass until none do.
 // On undirected resident grids the default method links over the
-// neighbour-list index instead (sampled linking, below): the same forest rule,
-// so the same labels, with far fewer random accesses.
+// neighbour-list index instead (sampled linking, below), in the index's
+// relabelled ids with the forest ordered by original id: the same roots, so
+// the same labels, with far fewer random accesses.
 #include <algorithm>
 #include <vector>
 
@@ -28,7 +29,9 @@ __device__ inline void st_par(uint32_t *p, uint32_t x, uint32_t v) {
     __hip_atomic_store(p + x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Root of x with path halving; parent[y] <= y holds throughout.
+// Root of x with path halving (halving keeps every vertex under the same root,
+// whatever order the linking rule keeps: id order here, original-id order in
+// the sampled linking below).
 __device__ inline uint32_t wcc_find(uint32_t *par, uint32_t x) {
     for (;;) {
         uint32_t p = ld_par(par, x);
@@ -104,11 +107,28 @@ static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255
 
 // ---- sampled linking over the neighbour-list index (Sutton, Ben-Nun and Barak,
 // "Optimizing parallel graph connectivity computation via subgraph sampling",
-// IPDPS 2018).  The forest is kept in original ids, so each root is still its
-// component's minimum id; the index supplies the neighbours (relabelled ids,
-// mapped back through old_id).
+// IPDPS 2018).  The forest is kept in relabelled ids (the index's own), so a
+// vertex's parent entry sits next to its neighbours' in the order the lists
+// are walked and the heaviest vertices' entries stay cached; it is ordered by
+// original id (a parent's original id is at most its child's), so each root is
+// its component's minimum original id, the label the reference reports.
 constexpr uint32_t AFF_ROUNDS = 2;     // neighbours linked for every vertex
 constexpr uint32_t AFF_SAMPLES = 1024; // vertices sampled to find the largest component
+
+// Joins the trees of u and v (relabelled ids): the root with the larger
+// original id is hooked under the other by compare-and-swap; if it was hooked
+// elsewhere meanwhile, join that tree instead.
+__device__ inline void aff_link(uint32_t *par, const uint32_t *__restrict__ key, uint32_t u, uint32_t v) {
+    uint32_t ru = wcc_find(par, u), rv = wcc_find(par, v);
+    while (ru != rv) {
+        const bool u_hi = key[ru] > key[rv];
+        const uint32_t hi = u_hi ? ru : rv, lo = u_hi ? rv : ru;
+        const uint32_t old = atomicCAS(&par[hi], hi, lo);
+        if (old == hi) break;  // hi was a root and now hangs under lo
+        ru = wcc_find(par, old);
+        rv = wcc_find(par, lo);
+    }
+}
 
 __global__ void k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                             const uint32_t *__restrict__ old_id, uint64_t nzn, uint32_t r,
@@ -117,7 +137,7 @@ __global__ void k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, con
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = off[x];
         if (off[x + 1] - b > r) {
-            wcc_link(par, old_id[x], old_id[nbr[b + r]]);
+            aff_link(par, old_id, (uint32_t)x, nbr[b + r]);
             ++c;
         }
     }
@@ -142,12 +162,32 @@ __global__ void k_aff_final(uint32_t *par, const uint64_t *__restrict__ off, con
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = off[x] + AFF_ROUNDS, e = off[x + 1];
         if (b >= e) continue;
-        const uint32_t u = old_id[x];
+        const uint32_t u = (uint32_t)x;
         if (wcc_find(par, u) == skip) continue;
-        for (uint64_t k = b; k < e; ++k) wcc_link(par, u, old_id[nbr[k]]);
+        for (uint64_t k = b; k < e; ++k) aff_link(par, old_id, u, nbr[k]);
         c += (uint32_t)(e - b);
     }
     if (c) atomicAdd(linked, (unsigned long long)c);
+}
+
+// Labels by original id from the compressed forest over relabelled ids: the
+// root's original id for vertices with edges (relabelled ids below nzn),
+// 0xFFFFFFFF for the others; counts the roots.
+__global__ void k_aff_label(const uint32_t *__restrict__ par, const uint32_t *__restrict__ old_id,
+                            const uint32_t *__restrict__ new_id, uint64_t nzn, uint64_t n, uint32_t *label,
+                            unsigned long long *ncomp) {
+    uint32_t c = 0;
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n; o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = new_id[o];
+        if (x < nzn) {
+            const uint32_t r = par[x];
+            label[o] = old_id[r];
+            c += r == x;
+        } else {
+            label[o] = 0xFFFFFFFFu;
+        }
+    }
+    if (c) atomicAdd(ncomp, (unsigned long long)c);
 }
 
 }  // namespace gs
@@ -158,7 +198,10 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     if (!g) { set_error("gs_wcc: null graph"); return GS_ERR_ARG; }
     if (w->wcc_parent.n != g->n) GS_TRY(w->wcc_parent.alloc(g->n));
     const bool by_index = g->wcc_method == 1 && !g->directed && !g->ss;
-    if (by_index) GS_TRY(build_adjacency(const_cast<gs_graph *>(g)));
+    if (by_index) {
+        GS_TRY(build_adjacency(const_cast<gs_graph *>(g)));
+        if (w->wcc_label.n != g->n) GS_TRY(w->wcc_label.alloc(g->n));
+    }
     w->timer.reset();
     w->stats = gs_stats{};
     hipEvent_t t0, t1;
@@ -178,7 +221,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
             w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
-            k_wcc_compress<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
+            k_wcc_compress<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn);
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
         }
@@ -187,7 +230,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
         if (g->nzn) {
             DevBuf<uint32_t> samp;
             GS_TRY(samp.alloc(AFF_SAMPLES));
-            k_aff_sample<<<(AFF_SAMPLES + 255) / 256, 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n, samp.ptr);
+            k_aff_sample<<<(AFF_SAMPLES + 255) / 256, 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn, samp.ptr);
             GS_HIP(hipGetLastError());
             std::vector<uint32_t> h(AFF_SAMPLES);
             GS_HIP(hipMemcpyAsync(h.data(), samp.ptr, AFF_SAMPLES * 4, hipMemcpyDeviceToHost, w->stream));
@@ -204,10 +247,12 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
         }
-        w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
-        k_wcc_compress<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
-        GS_HIP(hipGetLastError());
-        w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
+        if (g->nzn) {   // relabelled ids from nzn on have no edges: their own roots
+            w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
+            k_wcc_compress<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn);
+            GS_HIP(hipGetLastError());
+            w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
+        }
         GS_HIP(hipMemcpyAsync(w->h_counters + 1, w->counters.ptr, 8, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
         w->stats.edges_scanned = w->h_counters[1];
@@ -237,8 +282,13 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
         if (w->h_counters[0] == 0) break;
     }
     GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
-    k_wcc_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->degree.ptr,
-                                                    g->directed ? g->in_touch.ptr : nullptr, g->n, w->counters.ptr);
+    if (by_index) {
+        k_aff_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->old_id.ptr, g->new_id.ptr, g->nzn, g->n,
+                                                        w->wcc_label.ptr, w->counters.ptr);
+    } else {
+        k_wcc_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->degree.ptr,
+                                                        g->directed ? g->in_touch.ptr : nullptr, g->n, w->counters.ptr);
+    }
     GS_HIP(hipGetLastError());
     GS_HIP(hipEventCreate(&t1));
     GS_HIP(hipEventRecord(t1, w->stream));
@@ -253,7 +303,8 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     if (!by_index) w->stats.edges_scanned = g->E * passes * 2;
     if (ncomp) *ncomp = w->h_counters[0];
     if (label_out) {
-        GS_HIP(hipMemcpyAsync(label_out, w->wcc_parent.ptr, g->n * 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipMemcpyAsync(label_out, by_index ? w->wcc_label.ptr : w->wcc_parent.ptr, g->n * 4,
+                              hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
     }
     return GS_OK;
