@@ -92,39 +92,74 @@ __global__ void k_kc_count(const int32_t *__restrict__ deg, uint64_t n, unsigned
 // which is one per stored edge with an active endpoint, as the tile scan does.
 
 // Peel of relabelled ids 0 .. nzn-1 (the others have no edges), appending the
-// removed ones to queue / qlen; count[0] = how many.
+// removed ones to queue / qlen; count[0] = how many.  Each thread takes
+// KC_VPT consecutive vertices (two 16-byte degree loads), so a workgroup
+// reserves queue slots once per 2048 vertices: one reservation per 256
+// serialised on the cursor when most vertices peel in the first round.
+constexpr int KC_VPT = 8;
+
 __global__ void __launch_bounds__(256) k_kc_peel_list(int32_t *deg, uint64_t nzn, int k,
                                                       const uint64_t *__restrict__ off, uint32_t *queue,
                                                       int32_t *qlen, unsigned long long *count) {
     __shared__ uint32_t wsum[4];
     __shared__ unsigned long long bbase;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (uint64_t base = blockIdx.x * 256ull; base < nzn; base += (uint64_t)gridDim.x * 256) {
-        const uint64_t x = base + threadIdx.x;
-        bool pred = false;
-        if (x < nzn) {
-            const int16_t m = (int16_t)deg[x];
+    for (uint64_t base = blockIdx.x * (256ull * KC_VPT); base < nzn; base += (uint64_t)gridDim.x * 256 * KC_VPT) {
+        const uint64_t x0 = base + threadIdx.x * KC_VPT;
+        int32_t d[KC_VPT];
+        const bool whole = x0 + KC_VPT <= nzn;
+        if (whole) {
+            const int4 a = reinterpret_cast<const int4 *>(deg + x0)[0], b = reinterpret_cast<const int4 *>(deg + x0)[1];
+            d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+        } else {
+#pragma unroll
+            for (int i = 0; i < KC_VPT; ++i) d[i] = x0 + i < nzn ? deg[x0 + i] : 0;
+        }
+        uint32_t pm = 0;
+#pragma unroll
+        for (int i = 0; i < KC_VPT; ++i) {
+            const int16_t m = (int16_t)d[i];
             if (m > 0 && m < k) {
-                deg[x] = 0;
-                pred = true;
+                pm |= 1u << i;
+                d[i] = 0;
             }
         }
-        const unsigned long long mask = __ballot(pred);
-        if (lane == 0) wsum[wv] = (uint32_t)__popcll(mask);
+        if (pm) {
+            if (whole) {
+                reinterpret_cast<int4 *>(deg + x0)[0] = int4{d[0], d[1], d[2], d[3]};
+                reinterpret_cast<int4 *>(deg + x0)[1] = int4{d[4], d[5], d[6], d[7]};
+            } else {
+#pragma unroll
+                for (int i = 0; i < KC_VPT; ++i)
+                    if ((pm >> i) & 1u) deg[x0 + i] = 0;
+            }
+        }
+        const uint32_t c = __popc(pm);
+        uint32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wsum[wv] = incl;
         __syncthreads();
         if (threadIdx.x == 0) {
             const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
             bbase = t ? atomicAdd(&count[0], (unsigned long long)t) : 0ull;
         }
         __syncthreads();
-        if (pred) {
-            uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+        if (pm) {
+            uint32_t before = incl - c;
             for (int j = 0; j < wv; ++j) before += wsum[j];
-            const unsigned long long pos = bbase + before;
-            queue[pos] = (uint32_t)x;
-            qlen[pos] = (int32_t)(off[x + 1] - off[x]);
+            unsigned long long pos = bbase + before;
+#pragma unroll
+            for (int i = 0; i < KC_VPT; ++i)
+                if ((pm >> i) & 1u) {
+                    queue[pos] = (uint32_t)(x0 + i);
+                    qlen[pos++] = (int32_t)(off[x0 + i + 1] - off[x0 + i]);
+                }
         }
-        __syncthreads();
+        __syncthreads();   // wsum and bbase are rewritten by the next step
     }
 }
 
@@ -206,7 +241,7 @@ static int run_kcore_index(const gs_graph *gc, gs_ws *w, int k, int *rounds_out)
         hipEvent_t kb;
         w->timer.begin(GS_K_KCORE_PEEL, w->stream, &kb);
         if (g->nzn)
-            k_kc_peel_list<<<kc_grid(g->nzn), 256, 0, w->stream>>>(w->kc_deg.ptr, g->nzn, k, g->adj_off.ptr,
+            k_kc_peel_list<<<kc_grid((g->nzn + KC_VPT - 1) / KC_VPT), 256, 0, w->stream>>>(w->kc_deg.ptr, g->nzn, k, g->adj_off.ptr,
                                                                    w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_KCORE_PEEL, w->stream, kb);
