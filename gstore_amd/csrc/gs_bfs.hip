@@ -93,6 +93,7 @@ __global__ void __launch_bounds__(256) k_bfs_level_end(uint32_t *vis, uint32_t *
         if (any) atomicOr(&rowact[r >> 5], 1u << (r & 31));
         else atomicAnd(&rowact[r >> 5], ~(1u << (r & 31)));
         if (tot) atomicAdd(count, tot);
+        if (any) atomicAdd(count + 3, 1ull);  // tile rows holding the new frontier
     }
 }
 
@@ -121,7 +122,7 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     GS_HIP(hipMemsetAsync(w->bm_front.ptr, 0, words * 4, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_next.ptr, 0, words * 4, w->stream));
     GS_HIP(hipMemsetAsync(w->rowact.ptr, 0, w->rowact.bytes(), w->stream));
-    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 24, w->stream));
+    GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 32, w->stream));
     k_bfs_root<<<1, 1, 0, w->stream>>>((uint32_t)root, w->depth.ptr, w->bm_vis.ptr, w->bm_front.ptr, w->rowact.ptr);
     GS_HIP(hipGetLastError());
     int nlev = 0, status = GS_OK;
@@ -129,11 +130,16 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     hipEvent_t la, lb;
     GS_HIP(hipEventCreate(&la));
     GS_HIP(hipEventCreate(&lb));
-    uint64_t prev_e = 0, prev_t = 0, front_n = 1;
+    uint64_t prev_e = 0, prev_t = 0, front_n = 1, rows_on = 1;
     for (uint32_t level = 1;; ++level) {
         GS_HIP(hipEventRecord(la, w->stream));
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
+        GS_HIP(hipMemsetAsync(w->counters.ptr + 3, 0, 8, w->stream));
         hipEvent_t kb;
+        // semi-external: with the frontier in at most 1/16 of the tile rows the
+        // sweep reads about 1/8 of the tiles, which it reads from host memory in
+        // place rather than streaming the whole grid through the segments
+        const bool direct = g->ss && rows_on * 16 <= g->rows;
         GS_TRY(sweep(g, w, GS_K_BFS_EXPAND, [&](const SweepArgs &a, unsigned grid, hipStream_t st) {
             if (g->directed)
                 k_bfs_expand<true><<<grid, SWEEP_WG, 0, st>>>(a, w->bm_front.ptr, w->bm_vis.ptr, w->bm_next.ptr,
@@ -141,7 +147,7 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
             else
                 k_bfs_expand<false><<<grid, SWEEP_WG, 0, st>>>(a, w->bm_front.ptr, w->bm_vis.ptr, w->bm_next.ptr,
                                                               w->rowact.ptr, w->counters.ptr + 1);
-        }));
+        }, direct));
         w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
         k_bfs_level_end<<<(unsigned)g->rows, 256, 0, w->stream>>>(w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
                                                                    w->rowact.ptr, w->depth.ptr, level + 1,
@@ -149,9 +155,10 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
         GS_HIP(hipEventRecord(lb, w->stream));
-        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 24, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 32, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
         uint64_t cnt = w->h_counters[0];
+        rows_on = w->h_counters[3];
         {
             float lms = 0.f;
             GS_HIP(hipEventElapsedTime(&lms, la, lb));

@@ -163,6 +163,7 @@ namespace gs {
 // segment is being swept.
 struct StreamState {
     uint32_t *h_grid = nullptr;  // pinned host copy of the whole grid
+    const uint32_t *h_grid_dev = nullptr;  // the same memory as the device addresses it (direct sweeps)
     uint64_t res_chunks = 0;     // chunks [0, res_chunks) resident in gs_graph::grid
     uint64_t seg_chunks = 0;     // chunks per segment buffer
     DevBuf<uint32_t> seg[2];
@@ -490,7 +491,7 @@ __device__ inline uint32_t gv1(uint32_t w, uint32_t rc) { return (rc << 16) | (w
 // host-to-device copy, double-buffered so copy k+1 overlaps sweep k.
 // launch(SweepArgs, grid_size, stream) issues the kernel.
 template <class F>
-int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch) {
+int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch, bool direct = false) {
     if (!g->nchunk) return GS_OK;
     auto run = [&](const uint32_t *grid, uint64_t gc0, uint64_t c0, uint64_t cn) -> int {
         hipEvent_t kb;
@@ -503,6 +504,12 @@ int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch) {
     StreamState *ss = g->ss;
     if (!ss) return run(g->grid.ptr, 0, 0, g->nchunk);
     if (ss->res_chunks) GS_TRY(run(g->grid.ptr, 0, 0, ss->res_chunks));
+    // direct: a sweep that reads few tiles (a sparse frontier) reads them from
+    // the pinned host grid in place, instead of copying every segment over
+    if (direct && ss->h_grid_dev) {
+        if (ss->res_chunks < g->nchunk) GS_TRY(run(ss->h_grid_dev, 0, ss->res_chunks, g->nchunk - ss->res_chunks));
+        return GS_OK;
+    }
     int k = 0;
     for (uint64_t c0 = ss->res_chunks; c0 < g->nchunk; c0 += ss->seg_chunks, k ^= 1) {
         uint64_t cn = std::min(ss->seg_chunks, g->nchunk - c0);

@@ -53,6 +53,9 @@ int stream_setup(gs_graph *g, uint32_t *h_grid, uint64_t budget, uint64_t seg_by
         GS_HIP(hipEventRecord(ss->ev_done[i], ss->copy));  // first waits complete at once
     }
     GS_HIP(hipStreamSynchronize(ss->copy));
+    void *dev = nullptr;
+    GS_HIP(hipHostGetDevicePointer(&dev, h_grid, 0));
+    ss->h_grid_dev = static_cast<const uint32_t *>(dev);
     ss->h_grid = h_grid;
     g->ss = ss.release();
     return GS_OK;

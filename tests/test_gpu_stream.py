@@ -42,6 +42,27 @@ def test_streaming_equals_resident(res_chunks):
         assert np.array_equal(s2, start) and np.array_equal(g2, grid)
 
 
+def test_sparse_levels_read_host_grid_in_place():
+    """Semi-external BFS levels whose frontier sits in at most 1/16 of the tile
+    rows sweep the pinned host grid in place instead of copying segments: the
+    same depths, level counts and scan counts as resident, and fewer bytes
+    copied than one full stream per level."""
+    n, e = graphs.kron(20)   # 16 tile rows: a level with one frontier row reads in place
+    start, grid, deg = oracle.convert(e, n)
+    seg = 64 * CH
+    with ga.Graph.from_arrays(n, start, grid, deg) as ref, ga.Graph.from_arrays(n, start, grid, deg) as g:
+        g.set_device_budget(2 * seg + CH, seg)
+        assert not g.resident
+        per_stream = 4 * (g.E - g.resident_edges)
+        for root in (int(np.argmax(deg)), int(np.flatnonzero(deg)[-1]), 1):
+            _say("direct sweeps, root", root)
+            got, want = g.bfs(root), ref.bfs(root)
+            assert np.array_equal(got.depth, want.depth) and got.level_counts == want.level_counts
+            st, rs = g.stats(), ref.stats()
+            assert st["edges_scanned"] == rs["edges_scanned"] and st["tiles_scanned"] == rs["tiles_scanned"]
+            assert st["h2d_bytes"] < len(got.level_counts) * per_stream
+
+
 def test_open_with_budget(tmp_path):
     n, e = graphs.kron(14)
     pre = str(tmp_path / "k14")
