@@ -8,18 +8,22 @@
 // (0 .. nzn-1) are cut into blocks of PB_B = 2^15 vertices, whose FP32 values
 // fill 128 KiB of LDS.  Every list entry (v <- u) is copied once into an array
 // ordered by destination block q = v / PB_B, then source block p = u / PB_B
-// (the run (p, q)), keeping only the two 15-bit offsets inside the blocks.
-// Every run starts on a window of PB_WIN = 16 entries and fills whole windows
-// (on a Kronecker graph of 2^28 vertices the padding adds about 1.2% of
-// entries: runs average ~900 entries); padding entries read a zero and add it
-// to the run's last destination.
+// (the run (p, q)), keeping the 15-bit source offset.  Every run starts on a
+// window of PB_WIN = 16 entries and fills whole windows (on a Kronecker graph
+// of 2^28 vertices the padding adds about 1.2% of entries: runs average ~900
+// entries), and a run's entries are sorted by destination, so a window splits
+// into segments of equal destination.  The accumulation reads a stream laid
+// out per destination block: one value per entry (a plain block), or one per
+// segment (a compacted block, chosen when its segments are few; bit 15 of a
+// source offset marks a segment's last entry).
 //   gather, per source block p: stage prior[p*PB_B ..] in LDS and, one lane
-//           per window of the runs (p, *), write val[e] = prior[u] -- 2 bytes
-//           read and 4 bytes written per entry in 16-byte accesses;
-//   accum,  per destination block q: add val[e] into an LDS array over the
-//           block at the entry's destination offset (one LDS atomic per run of
-//           equal offsets in a wave) and store the block's sums.
-// A destination block with more entries than one workgroup takes is split
+//           per window of the runs (p, *), write prior[u] per entry or the
+//           sum per segment into the stream -- 2 bytes read per entry, 16-byte
+//           loads, vector stores;
+//   accum,  per destination block q: add the stream values into an LDS array
+//           over the block at their destination offsets (one LDS atomic per
+//           run of equal offsets in a lane) and write the new priors.
+// A destination block with more values than one workgroup takes is split
 // into units whose sums go to a partial buffer, added by k_pb_reduce.  The
 // order of the FP32 additions is the only difference from the other methods.
 #include <algorithm>
