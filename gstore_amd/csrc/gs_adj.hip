@@ -156,30 +156,66 @@ __global__ void __launch_bounds__(256) k_td_edges(const uint32_t *__restrict__ q
 
 // Bottom-up: BU_VPL vertices per lane (v, v + 64, ... of the wave's 64 x BU_VPL),
 // so each lane keeps all its vertices' loads in flight; an unreached vertex
-// with neighbours scans them until it meets a frontier vertex.  Each wave owns
-// 64 x BU_VPL consecutive vertices, so it writes its next-bitmap words without
-// atomics.  Lanes that are done load nbr[0] (in bounds) rather than branch, so
-// the loads of the other vertices still issue together.
+// with neighbours scans them until it meets a frontier vertex.  The first
+// list entry comes from first[] (one coalesced load, no offsets): short lists
+// are sorted, so it is the heaviest neighbour and on the big levels nearly
+// always in the frontier; only the vertices it leaves unresolved load their
+// offsets and scan on from the second entry.  Without LIST each wave owns
+// 64 x BU_VPL consecutive vertices 0 .. n-1 and skips the reached ones, and
+// writes its next-bitmap words without atomics.  With LIST the vertices come
+// from a list of the unreached ones (*list_n entries, k_unreached_list), so a
+// late level touches only them; found vertices are or-ed into next.  Lanes
+// that are done load nbr[0] (in bounds) rather than branch, so the loads of
+// the other vertices still issue together.
 constexpr int BU_VPL = 4;
+constexpr int BU_LIST_DIV = 32;  // bottom-up from the unreached list once at most nzn / 32 are left
 
-__global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint64_t *__restrict__ off,
-                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
+template <bool LIST>
+__global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint32_t *__restrict__ list,
+                                                   const unsigned long long *__restrict__ list_n,
+                                                   const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                                                   const uint32_t *__restrict__ first, const uint32_t *__restrict__ vis,
                                                    const uint32_t *__restrict__ front, uint32_t *next,
                                                    unsigned long long *edges_checked) {
     const int lane = threadIdx.x & 63;
     uint32_t checked = 0;
+    const uint64_t cnt = LIST ? *list_n : n;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * BU_VPL;
-    for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull) * BU_VPL; base < n;
+    for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull) * BU_VPL; base < cnt;
          base += stride) {
         uint64_t v[BU_VPL], k[BU_VPL], e[BU_VPL];
-        bool found[BU_VPL], any = false;
+        bool found[BU_VPL], open[BU_VPL], any = false;
+        uint32_t f0[BU_VPL];
 #pragma unroll
         for (int h = 0; h < BU_VPL; ++h) {
-            v[h] = base + 64 * h + lane;
+            const uint64_t i = base + 64 * h + lane;
             k[h] = e[h] = 0;
             found[h] = false;
-            if (v[h] < n && !((vis[v[h] >> 5] >> (v[h] & 31)) & 1u)) {
-                k[h] = off[v[h]];
+            if (LIST) {
+                open[h] = i < cnt;
+                v[h] = open[h] ? list[i] : 0;
+            } else {
+                v[h] = i;
+                open[h] = i < n && !((vis[i >> 5] >> (i & 31)) & 1u);
+            }
+            any = any || open[h];
+        }
+        if (!__ballot(any)) continue;  // the wave's vertices are all reached (late levels: nearly every wave)
+        any = false;
+#pragma unroll
+        for (int h = 0; h < BU_VPL; ++h) f0[h] = first[open[h] ? v[h] : 0];
+#pragma unroll
+        for (int h = 0; h < BU_VPL; ++h) {
+            const uint32_t fw = front[f0[h] >> 5];
+            if (open[h]) {
+                ++checked;
+                found[h] = (fw >> (f0[h] & 31)) & 1u;
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < BU_VPL; ++h) {
+            if (open[h] && !found[h]) {
+                k[h] = off[v[h]] + 1;
                 e[h] = off[v[h] + 1];
             }
             any = any || k[h] < e[h];
@@ -213,16 +249,72 @@ __global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint64_t *_
                 any = any || (k[h] < e[h] && !found[h]);
             }
         }
+        if (LIST) {
 #pragma unroll
-        for (int h = 0; h < BU_VPL; ++h) {
-            const unsigned long long m = __ballot(found[h]);
-            if (lane == 0 && m) {
-                next[((base + 64 * h) >> 5)] = (uint32_t)m;
-                next[((base + 64 * h) >> 5) + 1] = (uint32_t)(m >> 32);
+            for (int h = 0; h < BU_VPL; ++h)
+                if (found[h]) atomicOr(&next[v[h] >> 5], 1u << (v[h] & 31));
+        } else {
+#pragma unroll
+            for (int h = 0; h < BU_VPL; ++h) {
+                const unsigned long long m = __ballot(found[h]);
+                if (lane == 0 && m) {
+                    next[((base + 64 * h) >> 5)] = (uint32_t)m;
+                    next[((base + 64 * h) >> 5) + 1] = (uint32_t)(m >> 32);
+                }
             }
         }
     }
     if (checked) atomicAdd(edges_checked, (unsigned long long)checked);
+}
+
+// The unreached vertices with edges (clear bits of vis below nzn), listed in
+// list[0 .. *count): four words per thread, slots reserved once per workgroup
+// and step (wave scan of the thread counts, one global atomic).
+__global__ void __launch_bounds__(256) k_unreached_list(const uint32_t *__restrict__ vis, uint64_t nzn, uint32_t *list,
+                                                        unsigned long long *count) {
+    __shared__ uint32_t wsum[4];
+    __shared__ unsigned long long bbase;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t nwords = (nzn + 31) / 32;
+    for (uint64_t base = blockIdx.x * 1024ull; base < nwords; base += (uint64_t)gridDim.x * 1024) {
+        const uint64_t g0 = base + threadIdx.x * 4;
+        uint32_t m[4], c = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t gw = g0 + k;
+            m[k] = 0;
+            if (gw < nwords) {
+                m[k] = ~vis[gw];
+                if (gw * 32 + 32 > nzn) m[k] &= (1u << (uint32_t)(nzn - gw * 32)) - 1u;
+            }
+            c += __popc(m[k]);
+        }
+        uint32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            bbase = t ? atomicAdd(count, (unsigned long long)t) : 0ull;
+        }
+        __syncthreads();
+        unsigned long long pos = bbase + incl - c;
+        for (int w = 0; w < wv; ++w) pos += wsum[w];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t x = m[k];
+            while (x) {
+                const uint32_t b = __ffs(x) - 1;
+                x &= x - 1;
+                list[pos++] = (uint32_t)((g0 + k) * 32 + b);
+            }
+        }
+        __syncthreads();  // wsum and bbase are rewritten by the next step
+    }
 }
 
 // Level end for the direction-optimizing BFS: as k_bfs_level_end, plus the
@@ -373,6 +465,13 @@ __global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *f
         atomicAdd(&count[0], bcnt);
         atomicAdd(&count[1], bdeg);
     }
+}
+
+// first[v] = the first entry of v's list (every vertex below nzn has one).
+__global__ void k_first_entry(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, uint64_t nzn,
+                              uint32_t *first) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < nzn; v += (uint64_t)gridDim.x * blockDim.x)
+        first[v] = nbr[off[v]];
 }
 
 // Sorts the lists of at most 64 entries ascending (one wave per list, bitonic
@@ -540,6 +639,9 @@ int build_adjacency(gs_graph *g) {
     if (g->nzn) {
         k_sort_short_lists<<<8192, 256, 0, st>>>(g->adj_off.ptr, 0, g->nzn, g->adj_nbr.ptr);
         GS_HIP(hipGetLastError());
+        GS_TRY(g->adj_first.alloc(g->nzn));
+        k_first_entry<<<vgrid(g->nzn), 256, 0, st>>>(g->adj_off.ptr, g->adj_nbr.ptr, g->nzn, g->adj_first.ptr);
+        GS_HIP(hipGetLastError());
     }
     // 4. pull chunk map over the vertices with edges (relabelled ids 0 .. nzn-1)
     const uint64_t total = 2 * g->E;
@@ -657,10 +759,45 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
 
 // w->qoff = exclusive prefix of w->qdeg[0 .. nq) (blocks of 1024, block starts
 // on the host); *total (if given) = the sum.
+// Exclusive scan of nb block sums in place by one workgroup: each thread adds
+// a contiguous slice, the slice sums are scanned in LDS, and each thread
+// rewrites its slice.
+__global__ void __launch_bounds__(1024) k_scan_sums(unsigned long long *s, uint64_t nb) {
+    __shared__ unsigned long long part[1024];
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t b = std::min<uint64_t>(threadIdx.x * per, nb), e = std::min<uint64_t>(b + per, nb);
+    unsigned long long t = 0;
+    for (uint64_t i = b; i < e; ++i) t += s[i];
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const unsigned long long y = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += y;
+        __syncthreads();
+    }
+    unsigned long long run = part[threadIdx.x] - t;
+    for (uint64_t i = b; i < e; ++i) {
+        const unsigned long long c = s[i];
+        s[i] = run;
+        run += c;
+    }
+}
+
+// qoff = exclusive prefix of qdeg[0 .. nq).  Without total the block sums are
+// scanned on the device (no host round trip); with total they go through the
+// host, which returns the sum.
 int queue_prefix(gs_ws *w, uint64_t nq, std::vector<unsigned long long> &hsum, uint64_t *total) {
     const uint64_t nb = (nq + 1023) / 1024;
     k_deg_block_sums<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr);
     GS_HIP(hipGetLastError());
+    if (!total) {
+        k_scan_sums<<<1, 1024, 0, w->stream>>>(w->qsum.ptr, nb);
+        GS_HIP(hipGetLastError());
+        k_deg_scan<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr, w->qoff.ptr);
+        GS_HIP(hipGetLastError());
+        return GS_OK;
+    }
     hsum.resize(nb);
     GS_HIP(hipMemcpyAsync(hsum.data(), w->qsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
     GS_HIP(hipStreamSynchronize(w->stream));
@@ -779,6 +916,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     bool bottom_up = false, have_queue = true;
     int nlev = 0, status = GS_OK;
     uint64_t checked = 0, n_f_before = 0;
+    uint64_t reached = root_r < g->nzn ? 1 : 0;  // reached vertices with edges
     std::vector<unsigned long long> hsum;
     w->levels.clear();
     hipEvent_t la, lb;
@@ -790,19 +928,32 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
         GS_HIP(hipEventRecord(la, w->stream));
         // top-down to bottom-up only while the frontier grows: a shrinking tail
         // with few unchecked edges left stays top-down, where a bottom-up pass
-        // would still walk every vertex's visited bit
+        // would still walk every vertex's visited bit.  Back to top-down once
+        // the frontier is small, unless its edges outnumber all the unreached
+        // vertices' edges (a bottom-up level checks at most those).
         if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha && n_f > n_f_before) bottom_up = true;
-        else if (bottom_up && n_f < g->n / (uint64_t)g->do_beta) bottom_up = false;
+        else if (bottom_up && n_f < g->n / (uint64_t)g->do_beta && m_f <= m_u) bottom_up = false;
         n_f_before = n_f;
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 32, w->stream));
         hipEvent_t kb;
         const int kclass = bottom_up ? GS_K_BFS_BOTTOMUP : GS_K_BFS_TOPDOWN;
         w->timer.begin(kclass, w->stream, &kb);
         if (bottom_up) {  // vertices without edges (relabelled ids from nzn on) can never be reached
-            if (g->nzn)
-                k_bu_expand<<<vgrid((g->nzn + BU_VPL - 1) / BU_VPL), 256, 0, w->stream>>>(g->nzn, g->adj_off.ptr, g->adj_nbr.ptr,
-                                                                  w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr,
-                                                                  w->counters.ptr + 3);
+            const uint64_t unreached = g->nzn - std::min<uint64_t>(g->nzn, reached);
+            if (unreached && unreached <= g->nzn / BU_LIST_DIV) {
+                // few left: list them (the queue is free during bottom-up levels) and visit only those
+                GS_HIP(hipMemsetAsync(w->counters.ptr + 8, 0, 8, w->stream));
+                k_unreached_list<<<vgrid((act_words + 3) / 4), 256, 0, w->stream>>>(w->bm_vis.ptr, g->nzn, w->queue.ptr,
+                                                                                    w->counters.ptr + 8);
+                GS_HIP(hipGetLastError());
+                k_bu_expand<true><<<vgrid((unreached + BU_VPL - 1) / BU_VPL), 256, 0, w->stream>>>(
+                    g->nzn, w->queue.ptr, w->counters.ptr + 8, g->adj_off.ptr, g->adj_nbr.ptr, g->adj_first.ptr,
+                    w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->counters.ptr + 3);
+            } else if (unreached) {
+                k_bu_expand<false><<<vgrid((g->nzn + BU_VPL - 1) / BU_VPL), 256, 0, w->stream>>>(
+                    g->nzn, nullptr, nullptr, g->adj_off.ptr, g->adj_nbr.ptr, g->adj_first.ptr, w->bm_vis.ptr,
+                    w->bm_front.ptr, w->bm_next.ptr, w->counters.ptr + 3);
+            }
         } else if (n_f && m_f) {
             if (!have_queue) {  // the frontier came from a bottom-up level: list it
                 GS_HIP(hipMemsetAsync(w->counters.ptr + 4, 0, 24, w->stream));
@@ -843,6 +994,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
                                               bottom_up ? w->h_counters[3] : m_f_prev, 0, n_f, lms});
         }
         m_u -= std::min(m_u, m_f);
+        reached += n_f;
         if (level_counts && nlev < max_levels) level_counts[nlev] = n_f;
         ++nlev;
         if (n_f == 0) break;

@@ -710,7 +710,7 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->resident_edges = g->ss ? std::min<uint64_t>(g->E, g->ss->res_chunks * g->chunk) : g->E;
     info->segment_edges = g->ss ? g->ss->seg_chunks * g->chunk : 0;
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
-                         g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() +
+                         g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() + g->adj_first.bytes() +
                          g->new_id.bytes() + g->old_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() + gs::pb_bytes(g) +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
     const bool built = g->adj_nbr.ptr && g->adj_off.n == g->n + 1;

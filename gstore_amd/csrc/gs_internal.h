@@ -205,6 +205,7 @@ struct gs_graph {
     gs::DevBuf<int32_t> deg_r;               // degree by relabelled id, N
     gs::DevBuf<uint64_t> adj_off;            // list offsets by relabelled id, N + 1
     gs::DevBuf<uint32_t> adj_nbr;            // neighbour ids (relabelled), 2E
+    gs::DevBuf<uint32_t> adj_first;          // first list entry of each vertex with edges, nzn
     uint64_t nzn = 0, npull = 0;             // vertices with edges (ids 0 .. nzn-1); pull chunks
     uint64_t ms_nhub = 0;                    // ids 0 .. ms_nhub-1 have degree >= MS_HEAVY_DEG
     double index_ms = 0;                     // device time of the index build
