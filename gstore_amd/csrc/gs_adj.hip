@@ -564,6 +564,42 @@ __global__ void k_unpermute(const T *__restrict__ x_r, const uint32_t *__restric
         out[v] = x_r[new_id[v]];
 }
 
+// BFS depths by original id: 16 vertices per thread (four 16-byte loads of
+// new_id, one 16-byte store).  Vertices without edges (relabelled ids from
+// nzn on) are unreached unless one is the root, so only the others gather.
+__global__ void __launch_bounds__(256) k_unpermute_depth(const uint8_t *__restrict__ d_r,
+                                                         const uint32_t *__restrict__ new_id, uint64_t n, uint64_t nzn,
+                                                         uint32_t root_r, uint8_t *out) {
+    for (uint64_t v0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) * 16; v0 < n;
+         v0 += (uint64_t)gridDim.x * blockDim.x * 16) {
+        if (v0 + 16 <= n) {
+            uint32_t id[16];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint4 q = reinterpret_cast<const uint4 *>(new_id + v0)[k];
+                id[4 * k] = q.x; id[4 * k + 1] = q.y; id[4 * k + 2] = q.z; id[4 * k + 3] = q.w;
+            }
+            uint32_t wd[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                wd[k] = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t r = id[4 * k + j];
+                    const uint32_t d = r < nzn ? d_r[r] : (r == root_r ? 1u : 0u);
+                    wd[k] |= d << (8 * j);
+                }
+            }
+            *reinterpret_cast<uint4 *>(out + v0) = uint4{wd[0], wd[1], wd[2], wd[3]};
+        } else {
+            for (uint64_t v = v0; v < n; ++v) {
+                const uint32_t r = new_id[v];
+                out[v] = r < nzn ? d_r[r] : (uint8_t)(r == root_r ? 1 : 0);
+            }
+        }
+    }
+}
+
 // chunkz[c] = index (in nzv) of the vertex whose neighbour list holds entry c*PULL_CHUNK.
 __global__ void k_pull_chunk_map(const uint64_t *__restrict__ offz, uint64_t nzn, uint32_t *chunkz) {
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nzn; k += (uint64_t)gridDim.x * blockDim.x) {
@@ -1006,7 +1042,8 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     }
     (void)hipEventDestroy(la);
     (void)hipEventDestroy(lb);
-    k_unpermute<uint8_t><<<vgrid(g->n), 256, 0, w->stream>>>(w->depth_r.ptr, g->new_id.ptr, g->n, w->depth.ptr);
+    k_unpermute_depth<<<vgrid((g->n + 15) / 16), 256, 0, w->stream>>>(w->depth_r.ptr, g->new_id.ptr, g->n, g->nzn,
+                                                                       root_r, w->depth.ptr);
     GS_HIP(hipGetLastError());
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_TRY(sync_and_collect(w));
