@@ -86,18 +86,58 @@ __global__ void __launch_bounds__(SWEEP_WG) k_adj_scatter(SweepArgs a, const uin
 
 // ---- direction-optimizing BFS kernels ------------------------------------------
 
-__global__ void k_do_root(uint32_t root, const int32_t *__restrict__ deg, uint8_t *depth, uint32_t *vis,
-                          uint32_t *front, uint32_t *queue, int32_t *qdeg) {
-    depth[root] = 1;
-    vis[root >> 5] |= 1u << (root & 31);
-    front[root >> 5] |= 1u << (root & 31);
-    queue[0] = root;
-    qdeg[0] = deg[root];
+// The levels run from the device: per level, k_do_begin picks the direction
+// from the previous level's counters and raises flags; the listing, queue
+// prefix, top-down, bottom-up and level-end kernels return at once unless
+// their flag is up (and the search is not done); k_do_end closes the level.
+// DO_LEVELS levels are captured once per workspace into a hipGraph, which the
+// host launches until the done flag is up: one host round trip per DO_LEVELS
+// levels.  Control words, u64, in w->do_ctl; the first four are the counters
+// the level end and bottom-up kernels add into, in this order.
+enum : int {
+    DO_NF = 0,        // level end: vertices of the new frontier
+    DO_MF = 1,        //            their degree sum
+    DO_CUR = 2,       //            queue cursor
+    DO_CHK = 3,       // bottom-up: list entries checked
+    DO_LIST = 4,      // frontier listing: count, degree sum, cursor (4 .. 6)
+    DO_ULN = 8,       // unreached-list length
+    DO_NQ = 9,        // frontier at the level's start (queue length)
+    DO_MQ = 10,       // its degree sum (the top-down level's edges)
+    DO_MU = 11,       // degree sum of the vertices in no frontier yet
+    DO_NFB = 12,      // frontier one level earlier
+    DO_REACHED = 13,  // vertices with edges reached
+    DO_FLAGS = 14,
+    DO_LEVEL = 15,    // current level (1 = the root's)
+    DO_NLEV = 16,     // levels done
+    DO_CHECKED = 17,  // entries examined, all levels
+    DO_ALPHA = 18,
+    DO_BETA = 19,
+    DO_REC = 32,      // per level: direction, frontier, entries, reached, start and end clock
+    DO_RECW = 6,
+    DO_WORDS = DO_REC + DO_RECW * 256
+};
+enum : unsigned long long {
+    DF_BU = 1,        // bottom-up level
+    DF_HAVEQ = 2,     // the level end lists the new frontier (a top-down level)
+    DF_DONE = 4,
+    DF_TD = 8,        // top-down kernels run
+    DF_LIST = 16,     // the frontier is listed first (it came from a bottom-up level)
+    DF_OVERFLOW = 32, // the next depth would not fit in u8
+    DF_ULIST = 64,    // bottom-up over the unreached list
+    DF_BUFULL = 128   // bottom-up over all vertices with edges
+};
+constexpr int DO_LEVELS = 4;
+constexpr unsigned DO_TD_GRID = 2048;
+constexpr unsigned DO_SCAN_GRID = 1024;
+
+// true when the kernel's flag is up and the search is not done
+__device__ inline bool do_gate(const unsigned long long *ctl, unsigned long long flag) {
+    return (ctl[DO_FLAGS] & (flag | DF_DONE)) == flag;
 }
 
 // Top-down, balanced over edges: qoff is the exclusive prefix of the frontier
-// queue's degrees, and each workgroup takes TD_CHUNK consecutive frontier
-// edges.  The queue entries its range touches are staged in LDS (offset into
+// queue's degrees, and each workgroup step takes TD_CHUNK consecutive frontier
+// edges (grid-stride: the grid is fixed when the levels are captured).  The queue entries its range touches are staged in LDS (offset into
 // the range and list start), each lane finds its entry by a search there.  A
 // neighbour not yet visited or marked is marked with atomicOr.
 constexpr uint32_t TD_CHUNK = 4096;
@@ -107,50 +147,53 @@ __device__ inline void td_visit(uint32_t v, const uint32_t *__restrict__ vis, ui
     if (!(vis[wi] & bit) && !(next[wi] & bit)) atomicOr(&next[wi], bit);
 }
 
-__global__ void __launch_bounds__(256) k_td_edges(const uint32_t *__restrict__ queue, const uint64_t *__restrict__ qoff,
-                                                  uint64_t nq, uint64_t m, const uint64_t *__restrict__ off,
+__global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl, const uint32_t *__restrict__ queue,
+                                                  const uint64_t *__restrict__ qoff, const uint64_t *__restrict__ off,
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
                                                   uint32_t *next) {
+    if (!do_gate(ctl, DF_TD)) return;
     __shared__ uint32_t sbeg[TD_CHUNK + 2];  // first edge of staged entry k, relative to the range (clamped)
     __shared__ uint64_t sadj[TD_CHUNK + 1];  // list start of staged entry k
     __shared__ uint64_t j_first;
     __shared__ int nstaged;
-    const uint64_t c0 = blockIdx.x * (uint64_t)TD_CHUNK;
-    if (c0 >= m) return;
-    const uint32_t len = (uint32_t)min<uint64_t>(TD_CHUNK, m - c0);
-    if (threadIdx.x == 0) {
-        // largest j < nq with qoff[j] <= c0, then with qoff[j] <= c0 + len - 1
-        uint64_t lo = 0, hi = nq - 1;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+    const uint64_t nq = ctl[DO_NQ], m = ctl[DO_MQ];
+    for (uint64_t c0 = blockIdx.x * (uint64_t)TD_CHUNK; c0 < m; c0 += (uint64_t)gridDim.x * TD_CHUNK) {
+        const uint32_t len = (uint32_t)min<uint64_t>(TD_CHUNK, m - c0);
+        if (threadIdx.x == 0) {
+            // largest j < nq with qoff[j] <= c0, then with qoff[j] <= c0 + len - 1
+            uint64_t lo = 0, hi = nq - 1;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+            }
+            const uint64_t ja = lo;
+            hi = nq - 1;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
+            }
+            j_first = ja;
+            nstaged = (int)(lo - ja + 1);
         }
-        const uint64_t ja = lo;
-        hi = nq - 1;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
+        __syncthreads();
+        const uint64_t ja = j_first;
+        const int ns = nstaged;
+        for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+            const uint64_t q = qoff[ja + k];
+            sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
+            sadj[k] = off[queue[ja + k]] + (q < c0 ? c0 - q : 0);  // list position of the range's first edge of entry k
         }
-        j_first = ja;
-        nstaged = (int)(lo - ja + 1);
-    }
-    __syncthreads();
-    const uint64_t ja = j_first;
-    const int ns = nstaged;
-    for (int k = threadIdx.x; k < ns; k += blockDim.x) {
-        const uint64_t q = qoff[ja + k];
-        sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
-        sadj[k] = off[queue[ja + k]] + (q < c0 ? c0 - q : 0);  // list position of the range's first edge of entry k
-    }
-    if (threadIdx.x == 0) sbeg[ns] = len;
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
-        int lo = 0, hi = ns - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+        if (threadIdx.x == 0) sbeg[ns] = len;
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
+            int lo = 0, hi = ns - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+            }
+            td_visit(nbr[sadj[lo] + (t - sbeg[lo])], vis, next);
         }
-        td_visit(nbr[sadj[lo] + (t - sbeg[lo])], vis, next);
+        __syncthreads();  // the staged tables are rewritten by the next step
     }
 }
 
@@ -163,7 +206,7 @@ __global__ void __launch_bounds__(256) k_td_edges(const uint32_t *__restrict__ q
 // offsets and scan on from the second entry.  Without LIST each wave owns
 // 64 x BU_VPL consecutive vertices 0 .. n-1 and skips the reached ones, and
 // writes its next-bitmap words without atomics.  With LIST the vertices come
-// from a list of the unreached ones (*list_n entries, k_unreached_list), so a
+// from a list of the unreached ones (k_unreached_list), so a
 // late level touches only them; found vertices are or-ed into next.  Lanes
 // that are done load nbr[0] (in bounds) rather than branch, so the loads of
 // the other vertices still issue together.
@@ -171,15 +214,14 @@ constexpr int BU_VPL = 4;
 constexpr int BU_LIST_DIV = 32;  // bottom-up from the unreached list once at most nzn / 32 are left
 
 template <bool LIST>
-__global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint32_t *__restrict__ list,
-                                                   const unsigned long long *__restrict__ list_n,
+__global__ void __launch_bounds__(256) k_bu_expand(unsigned long long *ctl, uint64_t n, const uint32_t *__restrict__ list,
                                                    const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                                                    const uint32_t *__restrict__ first, const uint32_t *__restrict__ vis,
-                                                   const uint32_t *__restrict__ front, uint32_t *next,
-                                                   unsigned long long *edges_checked) {
+                                                   const uint32_t *__restrict__ front, uint32_t *next) {
+    if (!do_gate(ctl, LIST ? DF_ULIST : DF_BUFULL)) return;
     const int lane = threadIdx.x & 63;
     uint32_t checked = 0;
-    const uint64_t cnt = LIST ? *list_n : n;
+    const uint64_t cnt = LIST ? ctl[DO_ULN] : n;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * BU_VPL;
     for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull) * BU_VPL; base < cnt;
          base += stride) {
@@ -264,14 +306,16 @@ __global__ void __launch_bounds__(256) k_bu_expand(uint64_t n, const uint32_t *_
             }
         }
     }
-    if (checked) atomicAdd(edges_checked, (unsigned long long)checked);
+    if (checked) atomicAdd(ctl + DO_CHK, (unsigned long long)checked);
 }
 
 // The unreached vertices with edges (clear bits of vis below nzn), listed in
-// list[0 .. *count): four words per thread, slots reserved once per workgroup
+// list[0 .. ctl[DO_ULN]): four words per thread, slots reserved once per workgroup
 // and step (wave scan of the thread counts, one global atomic).
-__global__ void __launch_bounds__(256) k_unreached_list(const uint32_t *__restrict__ vis, uint64_t nzn, uint32_t *list,
-                                                        unsigned long long *count) {
+__global__ void __launch_bounds__(256) k_unreached_list(unsigned long long *ctl, const uint32_t *__restrict__ vis,
+                                                        uint64_t nzn, uint32_t *list) {
+    if (!do_gate(ctl, DF_ULIST)) return;
+    unsigned long long *count = ctl + DO_ULN;
     __shared__ uint32_t wsum[4];
     __shared__ unsigned long long bbase;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -381,10 +425,12 @@ __device__ inline void do_level_word(uint32_t nx, uint64_t gw, uint64_t n, uint8
 }
 
 template <bool LEVEL_END, bool BUILD>
-__global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *front, uint32_t *next, uint8_t *depth,
-                                                      uint32_t newdepth, const int32_t *__restrict__ deg, uint64_t n,
-                                                      uint64_t nwords, uint32_t *queue, int32_t *qdeg,
-                                                      unsigned long long *count) {
+__global__ void __launch_bounds__(256) k_do_level_end(const unsigned long long *ctl, unsigned long long flag,
+                                                      uint32_t *vis, uint32_t *front, uint32_t *next, uint8_t *depth,
+                                                      const int32_t *__restrict__ deg, uint64_t n, uint64_t nwords,
+                                                      uint32_t *queue, int32_t *qdeg, unsigned long long *count) {
+    if (!do_gate(ctl, flag)) return;
+    const uint32_t newdepth = (uint32_t)ctl[DO_LEVEL] + 1;
     __shared__ uint32_t wsum[4];
     __shared__ unsigned long long bbase, bcnt, bdeg;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -465,6 +511,164 @@ __global__ void __launch_bounds__(256) k_do_level_end(uint32_t *vis, uint32_t *f
         atomicAdd(&count[0], bcnt);
         atomicAdd(&count[1], bdeg);
     }
+}
+
+// The root's level: depth 1, visited, the frontier queue holds it, and ctl
+// starts the heuristic (Beamer et al.): m_u = degree sum of the vertices in no
+// frontier yet.
+__global__ void k_do_root(uint64_t root, const uint32_t *__restrict__ new_id, const int32_t *__restrict__ deg,
+                          uint64_t nzn, uint64_t m_total, uint32_t alpha, uint32_t beta, unsigned long long *ctl,
+                          uint8_t *depth, uint32_t *vis, uint32_t *front, uint32_t *queue, int32_t *qdeg) {
+    const uint32_t r = new_id[root];
+    const int32_t d = deg[r];
+    depth[r] = 1;
+    vis[r >> 5] |= 1u << (r & 31);
+    front[r >> 5] |= 1u << (r & 31);
+    queue[0] = r;
+    qdeg[0] = d;
+    ctl[DO_NF] = 1;
+    ctl[DO_MF] = (uint32_t)d;
+    ctl[DO_MU] = m_total - (uint32_t)d;
+    ctl[DO_REACHED] = r < nzn ? 1 : 0;
+    ctl[DO_FLAGS] = DF_HAVEQ;
+    ctl[DO_LEVEL] = 1;
+    ctl[DO_ALPHA] = alpha;
+    ctl[DO_BETA] = beta;
+}
+
+// Picks the level's direction (Beamer et al.) and kernels from the previous
+// level's counters, then clears the counters.  Top-down to bottom-up only
+// while the frontier grows: a shrinking tail with few unchecked edges left
+// stays top-down, where a bottom-up pass would still walk every vertex's
+// visited bit.  Back to top-down once the frontier is small, unless its edges
+// outnumber all the unreached vertices' edges (a bottom-up level checks at
+// most those).
+__global__ void k_do_begin(unsigned long long *ctl, uint64_t n, uint64_t nzn) {
+    const unsigned long long f = ctl[DO_FLAGS];
+    if (f & DF_DONE) return;
+    const unsigned long long nf = ctl[DO_NF], mf = ctl[DO_MF], mu = ctl[DO_MU];
+    bool bu = f & DF_BU;
+    if (!bu && mf > mu / ctl[DO_ALPHA] && nf > ctl[DO_NFB]) bu = true;
+    else if (bu && nf < n / ctl[DO_BETA] && mf <= mu) bu = false;
+    const unsigned long long reached = ctl[DO_REACHED];
+    const unsigned long long unreached = nzn - (reached < nzn ? reached : nzn);
+    unsigned long long nfl = bu ? DF_BU : DF_HAVEQ;
+    if (bu) {
+        if (unreached && unreached <= nzn / BU_LIST_DIV) nfl |= DF_ULIST;
+        else if (unreached) nfl |= DF_BUFULL;
+    } else if (nf && mf) {
+        nfl |= DF_TD | (f & DF_HAVEQ ? 0 : DF_LIST);
+    }
+    ctl[DO_NFB] = nf;
+    ctl[DO_NQ] = nf;
+    ctl[DO_MQ] = mf;
+    for (int i = 0; i <= DO_ULN; ++i) ctl[i] = 0;
+    unsigned long long *rec = ctl + DO_REC + DO_RECW * (ctl[DO_LEVEL] - 1);
+    rec[0] = bu ? 2 : 1;
+    rec[1] = nf;
+    rec[4] = (unsigned long long)wall_clock64();
+    ctl[DO_FLAGS] = nfl;
+}
+
+// qsum[b] = degree sum of queue entries [1024 b, 1024 b + 1024) (grid-stride)
+__global__ void __launch_bounds__(256) k_do_block_sums(const unsigned long long *ctl, const int32_t *__restrict__ qdeg,
+                                                       unsigned long long *qsum) {
+    if (!do_gate(ctl, DF_TD)) return;
+    __shared__ unsigned long long s;
+    const uint64_t nq = ctl[DO_NQ];
+    for (uint64_t base = (uint64_t)blockIdx.x * 1024; base < nq; base += (uint64_t)gridDim.x * 1024) {
+        if (threadIdx.x == 0) s = 0;
+        __syncthreads();
+        unsigned long long c = 0;
+        for (int i = threadIdx.x; i < 1024; i += blockDim.x)
+            if (base + i < nq) c += (uint32_t)qdeg[base + i];
+        atomicAdd(&s, c);
+        __syncthreads();
+        if (threadIdx.x == 0) qsum[base / 1024] = s;
+        __syncthreads();
+    }
+}
+
+// Exclusive prefix of the block sums in place (one workgroup: each thread adds
+// a contiguous slice, the slice sums are scanned in LDS, each thread rewrites
+// its slice).
+__global__ void __launch_bounds__(1024) k_do_scan(const unsigned long long *ctl, unsigned long long *qsum) {
+    if (!do_gate(ctl, DF_TD)) return;
+    __shared__ unsigned long long part[1024];
+    const uint64_t nb = (ctl[DO_NQ] + 1023) / 1024;
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t b0 = std::min<uint64_t>(threadIdx.x * per, nb), b1 = std::min<uint64_t>(b0 + per, nb);
+    unsigned long long t = 0;
+    for (uint64_t b = b0; b < b1; ++b) t += qsum[b];
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const unsigned long long y = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += y;
+        __syncthreads();
+    }
+    unsigned long long run = part[threadIdx.x] - t;
+    for (uint64_t b = b0; b < b1; ++b) {
+        const unsigned long long c = qsum[b];
+        qsum[b] = run;
+        run += c;
+    }
+}
+
+// qoff[j] = exclusive prefix of qdeg over the queue (blocks of 1024 as
+// k_deg_scan, grid-stride)
+__global__ void __launch_bounds__(256) k_do_deg_scan(const unsigned long long *ctl, const int32_t *__restrict__ qdeg,
+                                                     const unsigned long long *__restrict__ bstart, uint64_t *qoff) {
+    if (!do_gate(ctl, DF_TD)) return;
+    __shared__ uint32_t d[1024];
+    __shared__ unsigned long long part[256];
+    const uint64_t nq = ctl[DO_NQ];
+    for (uint64_t base = (uint64_t)blockIdx.x * 1024; base < nq; base += (uint64_t)gridDim.x * 1024) {
+        for (int i = threadIdx.x; i < 1024; i += blockDim.x) d[i] = base + i < nq ? (uint32_t)qdeg[base + i] : 0;
+        __syncthreads();
+        unsigned long long loc[4], run = 0;
+        for (int j = 0; j < 4; ++j) {
+            loc[j] = run;
+            run += d[threadIdx.x * 4 + j];
+        }
+        part[threadIdx.x] = run;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const unsigned long long y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0ull;
+            __syncthreads();
+            part[threadIdx.x] += y;
+            __syncthreads();
+        }
+        const unsigned long long before = (threadIdx.x ? part[threadIdx.x - 1] : 0ull) + bstart[base / 1024];
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t v = base + threadIdx.x * 4 + j;
+            if (v < nq) qoff[v] = before + loc[j];
+        }
+        __syncthreads();  // d and part are rewritten by the next step
+    }
+}
+
+// Closes the level: entries examined, the level's record, the heuristic's
+// state, and the stop test (nothing new reached, or the next depth would not
+// fit in u8).
+__global__ void k_do_end(unsigned long long *ctl) {
+    const unsigned long long f = ctl[DO_FLAGS];
+    if (f & DF_DONE) return;
+    const unsigned long long nf = ctl[DO_NF], mf = ctl[DO_MF], level = ctl[DO_LEVEL];
+    const unsigned long long edges = (f & DF_BU) ? ctl[DO_CHK] : ctl[DO_MQ];
+    unsigned long long *rec = ctl + DO_REC + DO_RECW * (level - 1);
+    rec[2] = edges;
+    rec[3] = nf;
+    rec[5] = (unsigned long long)wall_clock64();
+    ctl[DO_CHECKED] += edges;
+    const unsigned long long mu = ctl[DO_MU];
+    ctl[DO_MU] = mu - (mf < mu ? mf : mu);
+    ctl[DO_REACHED] += nf;
+    ctl[DO_NLEV] = level;
+    if (nf == 0) ctl[DO_FLAGS] = f | DF_DONE;
+    else if (level + 1 > 255) ctl[DO_FLAGS] = f | DF_DONE | DF_OVERFLOW;
+    else ctl[DO_LEVEL] = level + 1;
 }
 
 // first[v] = the first entry of v's list (every vertex below nzn has one).
@@ -569,7 +773,7 @@ __global__ void k_unpermute(const T *__restrict__ x_r, const uint32_t *__restric
 // nzn on) are unreached unless one is the root, so only the others gather.
 __global__ void __launch_bounds__(256) k_unpermute_depth(const uint8_t *__restrict__ d_r,
                                                          const uint32_t *__restrict__ new_id, uint64_t n, uint64_t nzn,
-                                                         uint32_t root_r, uint8_t *out) {
+                                                         uint64_t root, uint8_t *out) {
     for (uint64_t v0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) * 16; v0 < n;
          v0 += (uint64_t)gridDim.x * blockDim.x * 16) {
         if (v0 + 16 <= n) {
@@ -586,7 +790,7 @@ __global__ void __launch_bounds__(256) k_unpermute_depth(const uint8_t *__restri
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const uint32_t r = id[4 * k + j];
-                    const uint32_t d = r < nzn ? d_r[r] : (r == root_r ? 1u : 0u);
+                    const uint32_t d = r < nzn ? d_r[r] : (v0 + 4 * k + j == root ? 1u : 0u);
                     wd[k] |= d << (8 * j);
                 }
             }
@@ -594,7 +798,7 @@ __global__ void __launch_bounds__(256) k_unpermute_depth(const uint8_t *__restri
         } else {
             for (uint64_t v = v0; v < n; ++v) {
                 const uint32_t r = new_id[v];
-                out[v] = r < nzn ? d_r[r] : (uint8_t)(r == root_r ? 1 : 0);
+                out[v] = r < nzn ? d_r[r] : (uint8_t)(v == root ? 1 : 0);
             }
         }
     }
@@ -795,45 +999,10 @@ __global__ void __launch_bounds__(256) k_pr_pull(const uint32_t *__restrict__ nb
 
 // w->qoff = exclusive prefix of w->qdeg[0 .. nq) (blocks of 1024, block starts
 // on the host); *total (if given) = the sum.
-// Exclusive scan of nb block sums in place by one workgroup: each thread adds
-// a contiguous slice, the slice sums are scanned in LDS, and each thread
-// rewrites its slice.
-__global__ void __launch_bounds__(1024) k_scan_sums(unsigned long long *s, uint64_t nb) {
-    __shared__ unsigned long long part[1024];
-    const uint64_t per = (nb + 1023) / 1024;
-    const uint64_t b = std::min<uint64_t>(threadIdx.x * per, nb), e = std::min<uint64_t>(b + per, nb);
-    unsigned long long t = 0;
-    for (uint64_t i = b; i < e; ++i) t += s[i];
-    part[threadIdx.x] = t;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const unsigned long long y = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
-        __syncthreads();
-        part[threadIdx.x] += y;
-        __syncthreads();
-    }
-    unsigned long long run = part[threadIdx.x] - t;
-    for (uint64_t i = b; i < e; ++i) {
-        const unsigned long long c = s[i];
-        s[i] = run;
-        run += c;
-    }
-}
-
-// qoff = exclusive prefix of qdeg[0 .. nq).  Without total the block sums are
-// scanned on the device (no host round trip); with total they go through the
-// host, which returns the sum.
 int queue_prefix(gs_ws *w, uint64_t nq, std::vector<unsigned long long> &hsum, uint64_t *total) {
     const uint64_t nb = (nq + 1023) / 1024;
     k_deg_block_sums<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr);
     GS_HIP(hipGetLastError());
-    if (!total) {
-        k_scan_sums<<<1, 1024, 0, w->stream>>>(w->qsum.ptr, nb);
-        GS_HIP(hipGetLastError());
-        k_deg_scan<<<(unsigned)nb, 256, 0, w->stream>>>(w->qdeg.ptr, nq, w->qsum.ptr, w->qoff.ptr);
-        GS_HIP(hipGetLastError());
-        return GS_OK;
-    }
     hsum.resize(nb);
     GS_HIP(hipMemcpyAsync(hsum.data(), w->qsum.ptr, nb * 8, hipMemcpyDeviceToHost, w->stream));
     GS_HIP(hipStreamSynchronize(w->stream));
@@ -909,6 +1078,20 @@ int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double 
 
 using namespace gs;
 
+// Rate of the device's constant clock (wall_clock64), kHz.
+static double wall_clock_khz() {
+    static double khz = 0.0;
+    if (khz == 0.0) {
+        int dev = 0, rate = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev) == hipSuccess && rate > 0)
+            khz = rate;
+        else
+            khz = 100000.0;   // the documented 100 MHz of gfx9 parts
+    }
+    return khz;
+}
+
 int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
                    int max_levels, int *levels) {
     gs_graph *g = const_cast<gs_graph *>(gc);
@@ -934,123 +1117,108 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     GS_HIP(hipMemsetAsync(w->bm_vis.ptr, 0, words * 4, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_front.ptr, 0, words * 4, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_next.ptr, 0, words * 4, w->stream));
-    uint32_t root_r = 0;
-    GS_HIP(hipMemcpyAsync(&root_r, g->new_id.ptr + root, 4, hipMemcpyDeviceToHost, w->stream));
-    GS_HIP(hipStreamSynchronize(w->stream));
-    k_do_root<<<1, 1, 0, w->stream>>>(root_r, g->deg_r.ptr, w->depth_r.ptr, w->bm_vis.ptr, w->bm_front.ptr,
+    if (w->do_ctl.n != DO_WORDS) GS_TRY(w->do_ctl.alloc(DO_WORDS));
+    unsigned long long *ctl = w->do_ctl.ptr;
+    GS_HIP(hipMemsetAsync(ctl, 0, DO_WORDS * 8, w->stream));
+    k_do_root<<<1, 1, 0, w->stream>>>(root, g->new_id.ptr, g->deg_r.ptr, g->nzn, 2 * g->E, (uint32_t)g->do_alpha,
+                                      (uint32_t)g->do_beta, ctl, w->depth_r.ptr, w->bm_vis.ptr, w->bm_front.ptr,
                                       w->queue.ptr, w->qdeg.ptr);
     GS_HIP(hipGetLastError());
-    // heuristic state (Beamer et al.): edges still to check from the unreached side
-    uint64_t m_u = 2 * g->E, n_f = 1, m_f = 0;
-    {
-        int32_t d = 0;
-        GS_HIP(hipMemcpyAsync(&d, g->deg_r.ptr + root_r, 4, hipMemcpyDeviceToHost, w->stream));
-        GS_HIP(hipStreamSynchronize(w->stream));
-        m_f = (uint32_t)d;
-        m_u -= m_f;
+    // what the captured kernels were given (the thresholds travel in ctl)
+    const std::vector<const void *> key = {g, (const void *)(uintptr_t)g->n, (const void *)(uintptr_t)g->nzn, ctl,
+                                           w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, w->queue.ptr,
+                                           w->qdeg.ptr, w->qoff.ptr, w->qsum.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
+                                           g->adj_first.ptr, g->deg_r.ptr};
+    if (!w->do_graph || w->do_graph_key != key) {
+        if (w->do_graph) {
+            GS_HIP(hipGraphExecDestroy(w->do_graph));
+            w->do_graph = nullptr;
+        }
+        const uint64_t nzn = g->nzn;
+        const unsigned gle = std::max(1u, vgrid((act_words + LE_WPT - 1) / LE_WPT));
+        const unsigned gul = std::max(1u, vgrid((act_words + 3) / 4));
+        const unsigned gfull = std::max(1u, vgrid((nzn + BU_VPL - 1) / BU_VPL));
+        const unsigned glist = std::max(1u, vgrid(nzn / BU_LIST_DIV / BU_VPL + 1));
+        const unsigned gsc = (unsigned)std::min<uint64_t>(DO_SCAN_GRID, std::max<uint64_t>((nzn + 1023) / 1024, 1));
+        uint32_t *vis = w->bm_vis.ptr, *front = w->bm_front.ptr, *next = w->bm_next.ptr;
+        GS_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+        for (int l = 0; l < DO_LEVELS; ++l) {
+            k_do_begin<<<1, 1, 0, w->stream>>>(ctl, g->n, nzn);
+            // top-down: list the frontier if a bottom-up level made it, prefix of its degrees, edges
+            k_do_level_end<false, true><<<gle, 256, 0, w->stream>>>(ctl, DF_LIST, vis, front, next, w->depth_r.ptr,
+                                                                   g->deg_r.ptr, g->n, act_words, w->queue.ptr,
+                                                                   w->qdeg.ptr, ctl + DO_LIST);
+            k_do_block_sums<<<gsc, 256, 0, w->stream>>>(ctl, w->qdeg.ptr, w->qsum.ptr);
+            k_do_scan<<<1, 1024, 0, w->stream>>>(ctl, w->qsum.ptr);
+            k_do_deg_scan<<<gsc, 256, 0, w->stream>>>(ctl, w->qdeg.ptr, w->qsum.ptr, w->qoff.ptr);
+            k_td_edges<<<DO_TD_GRID, 256, 0, w->stream>>>(ctl, w->queue.ptr, w->qoff.ptr, g->adj_off.ptr,
+                                                         g->adj_nbr.ptr, vis, next);
+            // bottom-up: over the unreached list (few left; the queue is free then) or over all
+            k_unreached_list<<<gul, 256, 0, w->stream>>>(ctl, vis, nzn, w->queue.ptr);
+            k_bu_expand<true><<<glist, 256, 0, w->stream>>>(ctl, nzn, w->queue.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                           g->adj_first.ptr, vis, front, next);
+            k_bu_expand<false><<<gfull, 256, 0, w->stream>>>(ctl, nzn, nullptr, g->adj_off.ptr, g->adj_nbr.ptr,
+                                                            g->adj_first.ptr, vis, front, next);
+            // level end: after a top-down level the next one is likely top-down too, so list the new frontier
+            k_do_level_end<true, true><<<gle, 256, 0, w->stream>>>(ctl, DF_HAVEQ, vis, front, next, w->depth_r.ptr,
+                                                                  g->deg_r.ptr, g->n, act_words, w->queue.ptr,
+                                                                  w->qdeg.ptr, ctl);
+            k_do_level_end<true, false><<<gle, 256, 0, w->stream>>>(ctl, DF_BU, vis, front, next, w->depth_r.ptr,
+                                                                   g->deg_r.ptr, g->n, act_words, w->queue.ptr,
+                                                                   w->qdeg.ptr, ctl);
+            k_do_end<<<1, 1, 0, w->stream>>>(ctl);
+        }
+        const hipError_t launch = hipGetLastError();
+        hipGraph_t graph = nullptr;
+        const hipError_t end = hipStreamEndCapture(w->stream, &graph);
+        if (launch != hipSuccess || end != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            set_error("BFS: capture of the level sequence failed: %s",
+                      hipGetErrorString(launch != hipSuccess ? launch : end));
+            return GS_ERR_DEVICE;
+        }
+        const hipError_t inst = hipGraphInstantiate(&w->do_graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (inst != hipSuccess) {
+            w->do_graph = nullptr;
+            set_error("BFS: hipGraphInstantiate: %s", hipGetErrorString(inst));
+            return GS_ERR_DEVICE;
+        }
+        w->do_graph_key = key;
     }
-    bool bottom_up = false, have_queue = true;
-    int nlev = 0, status = GS_OK;
-    uint64_t checked = 0, n_f_before = 0;
-    uint64_t reached = root_r < g->nzn ? 1 : 0;  // reached vertices with edges
-    std::vector<unsigned long long> hsum;
-    w->levels.clear();
-    hipEvent_t la, lb;
-    GS_HIP(hipEventCreate(&la));
-    GS_HIP(hipEventCreate(&lb));
-    for (uint32_t level = 1;; ++level) {
-        const uint64_t m_f_prev = m_f;  // edges a top-down level examines
-        const uint64_t n_f_prev = n_f;
-        GS_HIP(hipEventRecord(la, w->stream));
-        // top-down to bottom-up only while the frontier grows: a shrinking tail
-        // with few unchecked edges left stays top-down, where a bottom-up pass
-        // would still walk every vertex's visited bit.  Back to top-down once
-        // the frontier is small, unless its edges outnumber all the unreached
-        // vertices' edges (a bottom-up level checks at most those).
-        if (!bottom_up && m_f > m_u / (uint64_t)g->do_alpha && n_f > n_f_before) bottom_up = true;
-        else if (bottom_up && n_f < g->n / (uint64_t)g->do_beta && m_f <= m_u) bottom_up = false;
-        n_f_before = n_f;
-        GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 32, w->stream));
-        hipEvent_t kb;
-        const int kclass = bottom_up ? GS_K_BFS_BOTTOMUP : GS_K_BFS_TOPDOWN;
-        w->timer.begin(kclass, w->stream, &kb);
-        if (bottom_up) {  // vertices without edges (relabelled ids from nzn on) can never be reached
-            const uint64_t unreached = g->nzn - std::min<uint64_t>(g->nzn, reached);
-            if (unreached && unreached <= g->nzn / BU_LIST_DIV) {
-                // few left: list them (the queue is free during bottom-up levels) and visit only those
-                GS_HIP(hipMemsetAsync(w->counters.ptr + 8, 0, 8, w->stream));
-                k_unreached_list<<<vgrid((act_words + 3) / 4), 256, 0, w->stream>>>(w->bm_vis.ptr, g->nzn, w->queue.ptr,
-                                                                                    w->counters.ptr + 8);
-                GS_HIP(hipGetLastError());
-                k_bu_expand<true><<<vgrid((unreached + BU_VPL - 1) / BU_VPL), 256, 0, w->stream>>>(
-                    g->nzn, w->queue.ptr, w->counters.ptr + 8, g->adj_off.ptr, g->adj_nbr.ptr, g->adj_first.ptr,
-                    w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->counters.ptr + 3);
-            } else if (unreached) {
-                k_bu_expand<false><<<vgrid((g->nzn + BU_VPL - 1) / BU_VPL), 256, 0, w->stream>>>(
-                    g->nzn, nullptr, nullptr, g->adj_off.ptr, g->adj_nbr.ptr, g->adj_first.ptr, w->bm_vis.ptr,
-                    w->bm_front.ptr, w->bm_next.ptr, w->counters.ptr + 3);
-            }
-        } else if (n_f && m_f) {
-            if (!have_queue) {  // the frontier came from a bottom-up level: list it
-                GS_HIP(hipMemsetAsync(w->counters.ptr + 4, 0, 24, w->stream));
-                k_do_level_end<false, true><<<vgrid((act_words + LE_WPT - 1) / LE_WPT), 256, 0, w->stream>>>(
-                    w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, 0, g->deg_r.ptr, g->n, act_words,
-                    w->queue.ptr, w->qdeg.ptr, w->counters.ptr + 4);
-                GS_HIP(hipGetLastError());
-            }
-            GS_TRY(queue_prefix(w, n_f, hsum, nullptr));
-            k_td_edges<<<(unsigned)((m_f + TD_CHUNK - 1) / TD_CHUNK), 256, 0, w->stream>>>(
-                w->queue.ptr, w->qoff.ptr, n_f, m_f, g->adj_off.ptr, g->adj_nbr.ptr, w->bm_vis.ptr, w->bm_next.ptr);
-        }
-        GS_HIP(hipGetLastError());
-        w->timer.end(kclass, w->stream, kb);
-        w->timer.begin(GS_K_BFS_LEVEL_END, w->stream, &kb);
-        // after a top-down level the next one is likely top-down too: list the new frontier now
-        have_queue = !bottom_up;
-        if (act_words && have_queue)
-            k_do_level_end<true, true><<<vgrid((act_words + LE_WPT - 1) / LE_WPT), 256, 0, w->stream>>>(
-                w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n,
-                act_words, w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
-        else if (act_words)
-            k_do_level_end<true, false><<<vgrid((act_words + LE_WPT - 1) / LE_WPT), 256, 0, w->stream>>>(
-                w->bm_vis.ptr, w->bm_front.ptr, w->bm_next.ptr, w->depth_r.ptr, level + 1, g->deg_r.ptr, g->n,
-                act_words, w->queue.ptr, w->qdeg.ptr, w->counters.ptr);
-        GS_HIP(hipGetLastError());
-        w->timer.end(GS_K_BFS_LEVEL_END, w->stream, kb);
-        GS_HIP(hipEventRecord(lb, w->stream));
-        GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 32, hipMemcpyDeviceToHost, w->stream));
+    unsigned long long *h = w->h_counters;   // pinned
+    for (;;) {
+        GS_HIP(hipGraphLaunch(w->do_graph, w->stream));
+        GS_HIP(hipMemcpyAsync(&h[0], ctl + DO_FLAGS, 8, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
-        n_f = w->h_counters[0];
-        m_f = w->h_counters[1];
-        checked += bottom_up ? w->h_counters[3] : m_f_prev;
-        {
-            float lms = 0.f;
-            GS_HIP(hipEventElapsedTime(&lms, la, lb));
-            w->levels.push_back(gs_level_stat{level, bottom_up ? 2 : 1, n_f_prev,
-                                              bottom_up ? w->h_counters[3] : m_f_prev, 0, n_f, lms});
-        }
-        m_u -= std::min(m_u, m_f);
-        reached += n_f;
-        if (level_counts && nlev < max_levels) level_counts[nlev] = n_f;
-        ++nlev;
-        if (n_f == 0) break;
-        if (level + 1 > 255) {
-            set_error("BFS from %llu needs depth > 255, beyond the u8 depth encoding", (unsigned long long)root);
-            status = GS_ERR_DEPTH_OVERFLOW;
-            break;
-        }
+        if (h[0] & DF_DONE) break;
     }
-    (void)hipEventDestroy(la);
-    (void)hipEventDestroy(lb);
-    k_unpermute_depth<<<vgrid((g->n + 15) / 16), 256, 0, w->stream>>>(w->depth_r.ptr, g->new_id.ptr, g->n, g->nzn,
-                                                                       root_r, w->depth.ptr);
+    k_unpermute_depth<<<vgrid((g->n + 15) / 16), 256, 0, w->stream>>>(w->depth_r.ptr, g->new_id.ptr, g->n, g->nzn, root,
+                                                                       w->depth.ptr);
     GS_HIP(hipGetLastError());
     GS_HIP(hipEventRecord(t1, w->stream));
+    std::vector<unsigned long long> hc(DO_WORDS);
+    GS_HIP(hipMemcpyAsync(hc.data(), ctl, DO_WORDS * 8, hipMemcpyDeviceToHost, w->stream));
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    const int nlev = (int)hc[DO_NLEV];
+    const unsigned long long checked = hc[DO_CHECKED];
+    const double khz = wall_clock_khz();
+    w->levels.clear();
+    for (int l = 0; l < nlev; ++l) {
+        const unsigned long long *rec = hc.data() + DO_REC + DO_RECW * l;
+        w->levels.push_back(gs_level_stat{(uint32_t)(l + 1), (int32_t)rec[0], rec[1], rec[2], 0, rec[3],
+                                          (double)(rec[5] - rec[4]) / khz});
+        if (level_counts && l < max_levels) level_counts[l] = rec[3];
+    }
+    int status = GS_OK;
+    if (hc[DO_FLAGS] & DF_OVERFLOW) {
+        set_error("BFS from %llu needs depth > 255, beyond the u8 depth encoding", (unsigned long long)root);
+        status = GS_ERR_DEPTH_OVERFLOW;
+    }
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nlev;
     w->stats.edges_scanned = checked;

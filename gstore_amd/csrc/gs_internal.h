@@ -147,6 +147,9 @@ struct gs_ws {
     gs::DevBuf<unsigned long long> ms_ctl;                    // device-driven batched BFS: control words, level counts
     hipGraphExec_t ms_graph = nullptr;                        // its captured level sequence
     std::vector<const void *> ms_graph_key;                   // what the capture baked in
+    gs::DevBuf<unsigned long long> do_ctl;                    // direction-optimizing BFS: control words, level records
+    hipGraphExec_t do_graph = nullptr;                        // its captured level sequence
+    std::vector<const void *> do_graph_key;
     gs::DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;  // pinned mirror of counters
     gs::KernelTimer timer;
