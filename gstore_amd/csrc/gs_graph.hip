@@ -826,7 +826,7 @@ int gs_set_pagerank_method(gs_graph *g, int method) {
 int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries,
                              uint32_t dense_entries, uint32_t compact_permille) {
     if (!g) { set_error("null graph"); return GS_ERR_ARG; }
-    if (gather_entries < 1 || gather_entries > (1ull << 30) || gather_runs < 1 || gather_runs > gs::PB_RUNS ||
+    if (gather_entries < 1 || gather_entries > (1ull << 20) || gather_runs < 1 || gather_runs > gs::PB_RUNS ||
         accum_entries < 1 || accum_entries > (1ull << 30) || compact_permille > 1000) {
         set_error("blocking limits out of range");
         return GS_ERR_ARG;

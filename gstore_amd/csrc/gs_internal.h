@@ -37,7 +37,7 @@ constexpr uint32_t PB_B = 1u << PB_SHIFT;
 constexpr uint32_t MS_HEAVY_LOG = 7;         // batched bottom-up: a wave per vertex of degree >= 2^7
 constexpr uint32_t MS_HEAVY_DEG = 1u << MS_HEAVY_LOG;
 constexpr uint32_t PB_KMAX = 8192;           // blocks at most (vertices with edges <= 2^28)
-constexpr uint32_t PB_RUNS = 1024;           // runs staged per gather workgroup at most
+constexpr uint32_t PB_RUNS = 2048;           // runs staged per gather workgroup at most
 constexpr unsigned long long PB_COMPACT = 1ull << 63;   // run stream base: the run's block is compacted
 constexpr uint32_t PB_WIN = 16;              // entries per window: every run starts on a window and fills whole ones
 struct PbSlice { uint64_t e0, e1; uint32_t q, pad; };               // build: list entries of block q

@@ -344,6 +344,46 @@ __device__ inline void pb_segments(const pb_u32x4 (&sq)[2], const float *pri, fl
     if (r & 1) out[b + (r & 2)] = (r & 2) ? t2 : t0;
 }
 
+// The same for windows with up to 16 segments, a quarter (4 entries) at a
+// time: a quarter's segments land in 4 slots (40 selects per window, where 16
+// slots over the whole window take 136) and leave at the quarter's place,
+// after the segments of the quarters before it.  The running sum carries over
+// quarters, so a segment may span them.
+__device__ inline void pb_segments_q(const pb_u32x4 (&sq)[2], const float *pri, float *out) {
+    float acc = 0.0f;
+    uint32_t base = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        uint32_t seg = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = 4 * q + j;
+            const uint32_t s = (sq[i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
+            acc += pri[s & 0x7FFFu];
+            const bool e = s >> 15;
+            s0 = (e && seg == 0) ? acc : s0;
+            if (j >= 1) s1 = (e && seg == 1) ? acc : s1;
+            if (j >= 2) s2 = (e && seg == 2) ? acc : s2;
+            if (j >= 3) s3 = (e && seg == 3) ? acc : s3;
+            seg += e;
+            acc = e ? 0.0f : acc;
+        }
+        float *o = out + base;
+        if (seg == 4) {
+            const pb_f32x4 a = {s0, s1, s2, s3};
+            __builtin_memcpy(o, &a, 16);
+        } else {
+            if (seg & 2) {
+                const float2 a = {s0, s1};
+                __builtin_memcpy(o, &a, 8);
+            }
+            if (seg & 1) o[seg & 2] = (seg & 2) ? s2 : s0;
+        }
+        base += seg;
+    }
+}
+
 // Values of the windows [w0, w1) of source block p (counted in source-major
 // order) into the accumulation stream; the unit's runs are staged in LDS.
 // Lane i of the workgroup takes windows w0 + i, w0 + i + PB_WG, ... (adjacent
@@ -353,10 +393,10 @@ __device__ inline void pb_segments(const pb_u32x4 (&sq)[2], const float *pri, fl
 // stores); a run of a compacted block writes one sum per segment of equal
 // destinations (bit 15 of a source offset marks each segment's last entry,
 // padding follows the last mark) at the window's place in the compacted stream.
-// A window's sums collect in registers (pb_segments; the slot count follows
-// the wave's largest segment count, so windows of the heaviest blocks pay for
-// two or four slots) and leave in vector stores: one scalar store per segment
-// made the gather 30% slower at Kron-28.
+// A window's sums collect in registers and leave in vector stores (one scalar
+// store per segment made the gather 30% slower at Kron-28): pb_segments with
+// two or four slots when no window of the wave has more segments (the
+// heaviest blocks), else pb_segments_q a quarter window at a time.
 __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__ units,
                                                      const uint64_t *__restrict__ run_pw,
                                                      const uint64_t *__restrict__ run_qw,
@@ -366,8 +406,11 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
                                                      const float *__restrict__ prior, uint64_t nzn,
                                                      float *__restrict__ val) {
     __shared__ float4 pri4[PB_B / 4 + 1];   // + the zero read by padding entries (offset PB_B)
-    __shared__ uint32_t rs[PB_RUNS + 1];
-    __shared__ uint64_t rq[PB_RUNS], rt[PB_RUNS];
+    // per staged run: first window in the unit (< 2^16: a unit has at most 2^20
+    // entries), first q-major window (< 2^32, checked at build) and stream offset
+    __shared__ uint16_t rs[PB_RUNS];
+    __shared__ uint32_t rq[PB_RUNS];
+    __shared__ uint64_t rt[PB_RUNS];
     const PbUnit1 u = units[blockIdx.x];
     const uint64_t pbase = (uint64_t)u.p << PB_SHIFT;
     const uint32_t pn = (uint32_t)std::min<uint64_t>(PB_B, nzn - pbase);
@@ -379,13 +422,12 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
         const uint64_t r0 = run_pw[u.r0 + k];
         const uint64_t s = r0 > u.w0 ? r0 : u.w0;
         const uint64_t t = run_st[u.r0 + k];
-        rs[k] = (uint32_t)(s - u.w0);
-        rq[k] = run_qw[u.r0 + k] + (s - r0);
+        rs[k] = (uint16_t)(s - u.w0);
+        rq[k] = (uint32_t)(run_qw[u.r0 + k] + (s - r0));
         // plain runs: the stream follows the windows; compacted: wcoff is relative to the run
         rt[k] = (t & PB_COMPACT) ? t : t + (s - r0) * PB_WIN;
     }
     const uint32_t len = (uint32_t)(u.w1 - u.w0);
-    if (threadIdx.x == 0) rs[nr] = len;
     __syncthreads();
     for (uint32_t wb = threadIdx.x; wb < len; wb += PB_WPL * PB_WG) {
         uint64_t qw[PB_WPL], st[PB_WPL];
@@ -399,7 +441,7 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
 #pragma unroll
             for (int step = PB_RUNS / 2; step > 0; step >>= 1)
                 if (k + step < nr && rs[k + step] <= w) k += step;
-            qw[j] = rq[k] + (w - rs[k]);
+            qw[j] = (uint64_t)rq[k] + (w - rs[k]);
             st[j] = rt[k] & PB_COMPACT ? rt[k] : rt[k] + (uint64_t)(w - rs[k]) * PB_WIN;
             const pb_u32x4 *s4 = reinterpret_cast<const pb_u32x4 *>(src + qw[j] * PB_WIN);
             sq[j][0] = __builtin_nontemporal_load(s4);
@@ -418,8 +460,7 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
                     n += ((a >> 15) & 1u) + (a >> 31);
                 }
                 // register slots for the wave's largest segment count (wave-uniform branches)
-                if (__ballot(n > 8)) pb_segments<16>(sq[j], pri, out);
-                else if (__ballot(n > 4)) pb_segments<8>(sq[j], pri, out);
+                if (__ballot(n > 4)) pb_segments_q(sq[j], pri, out);
                 else if (__ballot(n > 2)) pb_segments<4>(sq[j], pri, out);
                 else pb_segments<2>(sq[j], pri, out);
             } else {
@@ -616,7 +657,7 @@ static int check_units(const std::vector<PbUnit1> &u1, const std::vector<PbUnit2
     for (const PbUnit1 &x : u1) {
         const bool ok = x.w0 == pw && x.w1 > x.w0 && x.r0 < x.r1 && x.r1 <= nr && x.r1 - x.r0 <= runs_max &&
                         x.r1 - x.r0 <= PB_RUNS && rpw[x.r0] <= x.w0 && x.w0 < rpw[x.r0 + 1] &&
-                        rpw[x.r1 - 1] < x.w1 && x.w1 <= rpw[x.r1] && x.w1 - x.w0 < (1ull << 31);
+                        rpw[x.r1 - 1] < x.w1 && x.w1 <= rpw[x.r1] && x.w1 - x.w0 <= (1ull << 16);
         if (!ok) { set_error("blocked index: bad gather unit at window %llu", (unsigned long long)pw); return GS_ERR_FORMAT; }
         pw = x.w1;
     }
@@ -777,6 +818,10 @@ int build_blocked(gs_graph *g) {
     // 5. the copy in q-major order (runs sorted by destination), the padding of
     //    each run's last window, and the segment masks of every window
     const uint64_t nwin = run / PB_WIN;
+    if (nwin >= (1ull << 32)) {  // the gather stages q-major window indices as u32
+        set_error("blocked index: %llu windows exceed 2^32", (unsigned long long)nwin);
+        return GS_ERR_UNSUPPORTED;
+    }
     GS_TRY(g->pb_src.alloc(run + PB_WIN));
     DevBuf<uint16_t> qdst;   // q-major destination offsets, until the stream is laid out
     GS_TRY(qdst.alloc(run + PB_WIN));

@@ -260,15 +260,15 @@ int gs_set_batch_bfs_method(gs_graph *g, int method);
  * counts and rounds are identical.  Semi-external grids use 0. */
 int gs_set_kcore_method(gs_graph *g, int method);
 
-/* Work split of the blocked PageRank: source-major entries (<= 2^30) and
- * runs (<= 1024) per gather workgroup, stream values per accumulation
+/* Work split of the blocked PageRank: source-major entries (<= 2^20) and
+ * runs (<= 2048) per gather workgroup, stream values per accumulation
  * workgroup (<= 2^30), the stream values per vertex from which a destination
  * block adds with LDS float atomics rather than compare-and-swap, and the
  * compaction threshold in permille (0..1000): a destination block whose
  * segments of equal destinations (per 16-entry window) number at most that
  * share of its entries gets one stream value per segment, summed by the
  * gather, instead of one per entry (0: never, 1000: always).  Defaults 2^19,
- * 1024, 2^20, 2^30 (compare-and-swap throughout), 800.  Small values
+ * 2048, 2^20, 2^30 (compare-and-swap throughout), 800.  Small values
  * exercise the split paths on small graphs; results change only in FP32
  * (FP64 for personalised PageRank) summation order.  No counterpart in the
  * reference. */

@@ -345,7 +345,7 @@ def test_pagerank_blocked_split_paths(s):
     src = int(np.argmax(deg))
     want_p = oracle.ppr(n, start, grid, deg, src, 20)
     with ga.Graph.from_arrays(n, start, grid, deg) as g:
-        for lim in ((1 << 20, 1024, 1 << 20, 64, 500), (97, 3, 5000, 0, 1000), (1000, 1, 1 << 12, 1 << 30, 0),
+        for lim in ((1 << 20, 2048, 1 << 20, 64, 500), (97, 3, 5000, 0, 1000), (1000, 1, 1 << 12, 1 << 30, 0),
                     (4096, 1024, 1 << 30, 8, 1000), (160, 2, 3000, 1 << 30, 1000), (1 << 20, 1024, 1 << 20, 8192, 500)):
             _say("blocked limits", s, lim)
             g.set_pagerank_blocking(*lim)
@@ -356,9 +356,9 @@ def test_pagerank_blocked_split_paths(s):
         with pytest.raises(ga.GStoreError):
             g.set_pagerank_blocking(0, 1, 1, 64)
         with pytest.raises(ga.GStoreError):
-            g.set_pagerank_blocking(1 << 31, 1, 1, 64)
+            g.set_pagerank_blocking((1 << 20) + 1, 1, 1, 64)
         with pytest.raises(ga.GStoreError):
-            g.set_pagerank_blocking(1, 2048, 1, 64)
+            g.set_pagerank_blocking(1, 2049, 1, 64)
         with pytest.raises(ga.GStoreError):
             g.set_pagerank_blocking(1, 1, 1, 64, 1001)
         with pytest.raises(ga.GStoreError):
