@@ -443,9 +443,10 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
                 if (k + step < nr && rs[k + step] <= w) k += step;
             qw[j] = (uint64_t)rq[k] + (w - rs[k]);
             st[j] = rt[k] & PB_COMPACT ? rt[k] : rt[k] + (uint64_t)(w - rs[k]) * PB_WIN;
+            // plain loads: nontemporal ones made the gather 11% slower at Kron-28
             const pb_u32x4 *s4 = reinterpret_cast<const pb_u32x4 *>(src + qw[j] * PB_WIN);
-            sq[j][0] = __builtin_nontemporal_load(s4);
-            sq[j][1] = __builtin_nontemporal_load(s4 + 1);
+            sq[j][0] = s4[0];
+            sq[j][1] = s4[1];
             co[j] = wcoff[qw[j]];
         }
 #pragma unroll
@@ -504,37 +505,41 @@ __global__ void __launch_bounds__(PB_WG) k_pb_accum(const PbUnit2 *__restrict__ 
     const PbUnit2 u = units[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < PB_B; i += PB_WG) sum[i] = 0.0f;
     __syncthreads();
+    uint32_t cur = 0xFFFFFFFFu;
+    float run = 0.0f;
+    auto flush = [&]() {
+        if (cur == 0xFFFFFFFFu) return;
+        if (u.dense) atomicAdd(&sum[cur], run);
+        else lds_add_cas(&sum[cur], run);
+    };
+    auto add = [&](uint32_t d, float x) {
+        if (d != cur) {
+            flush();
+            cur = d;
+            run = x;
+        } else {
+            run += x;
+        }
+    };
+    // 16 consecutive values per lane: runs of one destination (the
+    // heaviest vertices' segments continue over windows) add once.  Plain
+    // loads: nontemporal ones made this kernel half as slow again at Kron-28.
     for (uint64_t sb = u.e0 & ~15ull; sb < u.e1; sb += 16 * PB_WG) {
         const uint64_t e0 = sb + threadIdx.x * 16;
         if (e0 >= u.e1) break;
         pb_u32x4 dq[2], vq[4];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) dq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(dst + e0) + k);
+        for (int k = 0; k < 2; ++k) dq[k] = *(reinterpret_cast<const pb_u32x4 *>(dst + e0) + k);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) vq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(val + e0) + k);
-        uint32_t cur = 0xFFFFFFFFu;
-        float run = 0.0f;
-        auto flush = [&]() {
-            if (cur == 0xFFFFFFFFu) return;
-            if (u.dense) atomicAdd(&sum[cur], run);
-            else lds_add_cas(&sum[cur], run);
-        };
+        for (int k = 0; k < 4; ++k) vq[k] = *(reinterpret_cast<const pb_u32x4 *>(val + e0) + k);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const uint64_t e = e0 + i;
             if (e < u.e0 || e >= u.e1) continue;
-            const uint32_t d = (dq[i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
-            const float x = __uint_as_float(vq[i >> 2][i & 3]);
-            if (d != cur) {
-                flush();
-                cur = d;
-                run = x;
-            } else {
-                run += x;
-            }
+            add((dq[i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu, __uint_as_float(vq[i >> 2][i & 3]));
         }
-        flush();
     }
+    flush();
     __syncthreads();
     const uint64_t qbase = (uint64_t)u.q << PB_SHIFT;
     const uint32_t qn = (uint32_t)std::min<uint64_t>(PB_B, nzn - qbase);
@@ -576,10 +581,10 @@ __global__ void __launch_bounds__(PB_WG) k_pb_accum64(const PbUnit2 *__restrict_
             pb_u32x4 dq[2], vq[4];
 #pragma unroll
             for (int k = 0; k < 2; ++k)
-                dq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(dst + e0) + k);
+                dq[k] = *(reinterpret_cast<const pb_u32x4 *>(dst + e0) + k);
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                vq[k] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(val + e0) + k);
+                vq[k] = *(reinterpret_cast<const pb_u32x4 *>(val + e0) + k);
             uint32_t cur = 0xFFFFFFFFu;
             double run = 0.0;
 #pragma unroll
