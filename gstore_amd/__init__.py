@@ -288,8 +288,8 @@ class Graph:
         tile scatter with atomics.  True / False mean PR_PULL / PR_SCATTER."""
         _check(_lib.gs_set_pagerank_method(self._h, int(method)))
 
-    def set_pagerank_blocking(self, gather_entries: int = 1 << 19, gather_runs: int = 2048,
-                              accum_entries: int = 1 << 20, dense_entries: int = 1 << 30,
+    def set_pagerank_blocking(self, gather_entries: int = 1 << 16, gather_runs: int = 2048,
+                              accum_entries: int = 1 << 21, dense_entries: int = 1 << 30,
                               compact_permille: int = 800) -> None:
         """Work split and layout of the blocked PageRank (small values exercise the split paths)."""
         _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries, dense_entries,

@@ -222,9 +222,9 @@ struct gs_graph {
     uint32_t pb_k = 0;                       // blocks; 0 = not available (PageRank pulls)
     uint64_t pb_nu1 = 0, pb_nu2 = 0, pb_nred = 0, pb_nruns = 0;
     double pb_ms = 0;                        // build time
-    uint64_t pb_unit1 = 1u << 19;            // gather: entries per workgroup at most
+    uint64_t pb_unit1 = 1u << 16;            // gather: entries per workgroup at most
     uint32_t pb_runs = gs::PB_RUNS;          // gather: runs per workgroup at most
-    uint64_t pb_unit2 = 1u << 20;            // accum: entries per workgroup at most
+    uint64_t pb_unit2 = 1u << 21;            // accum: entries per workgroup at most
     uint32_t pb_dense = 1u << 30;            // accum: stream values per vertex from which a block uses LDS float atomics
     uint64_t pb_entries = 0;                 // 2E plus the padding that fills each run's last window
     uint64_t pb_stream = 0;                  // accumulation stream length (with block alignment)

@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
             const pb_u32x4 *s4 = reinterpret_cast<const pb_u32x4 *>(src + qw[j] * PB_WIN);
             sq[j][0] = s4[0];
             sq[j][1] = s4[1];
-            co[j] = wcoff[qw[j]];
+            co[j] = st[j] & PB_COMPACT ? wcoff[qw[j]] : 0u;   // plain windows need no offset
         }
 #pragma unroll
         for (int j = 0; j < PB_WPL; ++j) {

@@ -267,8 +267,8 @@ int gs_set_kcore_method(gs_graph *g, int method);
  * compaction threshold in permille (0..1000): a destination block whose
  * segments of equal destinations (per 16-entry window) number at most that
  * share of its entries gets one stream value per segment, summed by the
- * gather, instead of one per entry (0: never, 1000: always).  Defaults 2^19,
- * 2048, 2^20, 2^30 (compare-and-swap throughout), 800.  Small values
+ * gather, instead of one per entry (0: never, 1000: always).  Defaults 2^16,
+ * 2048, 2^21, 2^30 (compare-and-swap throughout), 800.  Small values
  * exercise the split paths on small graphs; results change only in FP32
  * (FP64 for personalised PageRank) summation order.  No counterpart in the
  * reference. */
