@@ -209,6 +209,14 @@ def test_errors(tmp_path):
         assert g.bfs(150).depth.max() == 151
         with pytest.raises(ga.GStoreError):
             g.bfs(n)                         # root out of range
+        g.set_bfs_mode(1, 14, 24)            # the device-driven levels stop at the same place
+        with pytest.raises(ga.GStoreError) as ei:
+            g.bfs(0)
+        assert ei.value.status == 7
+        res = g.bfs(150)
+        d, lc = oracle.bfs(n, start, grid, 150)
+        assert np.array_equal(res.depth, d) and res.level_counts == lc
+        g.set_bfs_mode(0, 14, 24)
     bad = deg.copy()
     bad[5] += 1
     with pytest.raises(ga.GStoreError) as ei:
