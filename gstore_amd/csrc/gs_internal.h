@@ -134,6 +134,7 @@ struct gs_ws {
     gs::DevBuf<uint32_t> bm_vis, bm_front, bm_next, rowact;  // bitmaps, words
     gs::DevBuf<uint32_t> wcc_parent;
     gs::DevBuf<uint32_t> wcc_label;                           // labels by original id (index method)
+    gs::DevBuf<uint32_t> wcc_cmin;                            // smallest original id per root (index method)
     gs::DevBuf<int32_t> kc_deg;
     gs::DevBuf<uint64_t> ms_vis, ms_front, ms_next, ms_roots;  // batched BFS lane masks
     gs::DevBuf<uint8_t> ms_depth;                             // lane-major depth, lanes x N

@@ -387,7 +387,7 @@ __device__ inline void pb_segments_q(const pb_u32x4 (&sq)[2], const float *pri, 
 // Values of the windows [w0, w1) of source block p (counted in source-major
 // order) into the accumulation stream; the unit's runs are staged in LDS.
 // Lane i of the workgroup takes windows w0 + i, w0 + i + PB_WG, ... (adjacent
-// lanes mostly share a run); a window's run comes from a 10-step search over
+// lanes mostly share a run); a window's run comes from an 11-step search over
 // the staged run starts.  A run of a plain destination block writes
 // val[e] = prior[p*PB_B + src[e]] for each of its 16 entries (four 16-byte
 // stores); a run of a compacted block writes one sum per segment of equal

@@ -84,7 +84,7 @@ static uint64_t need_bytes(const gs_graph *g, int kind, int lanes, bool keep_dep
             return base + 24 * n + (uint64_t)lanes * n + rowact + 64 * 8 * 2;
         case GS_Q_PPR: return base + 24 * n;  // prior, acc, factor, rank (f32) and the FP64 sums
         case GS_Q_PAGERANK: return base + 12 * n;
-        case GS_Q_WCC: return base + (g->wcc_method == 1 && !g->directed && !g->ss ? 8 : 4) * n;
+        case GS_Q_WCC: return base + (g->wcc_method == 1 && !g->directed && !g->ss ? 12 : 4) * n;
         default: return base;
     }
 }
@@ -103,7 +103,7 @@ static uint64_t kind_bytes(const gs_ws *w, int kind) {
             return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes() + w->ppr_rank.bytes() +
                    w->ppr_acc.bytes();
         case GS_Q_PAGERANK: return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes();
-        case GS_Q_WCC: return b + w->wcc_parent.bytes() + w->wcc_label.bytes();
+        case GS_Q_WCC: return b + w->wcc_parent.bytes() + w->wcc_label.bytes() + w->wcc_cmin.bytes();
         default: return b;
     }
 }
@@ -125,6 +125,7 @@ static void trim(gs_ws *w, int kind) {
     if (kind != GS_Q_WCC) {
         w->wcc_parent.release();
         w->wcc_label.release();
+        w->wcc_cmin.release();
     }
     w->depth.release(); w->bm_vis.release(); w->bm_front.release(); w->bm_next.release(); w->kc_deg.release();
 }

@@ -29,9 +29,9 @@ __device__ inline void st_par(uint32_t *p, uint32_t x, uint32_t v) {
     __hip_atomic_store(p + x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Root of x with path halving (halving keeps every vertex under the same root,
-// whatever order the linking rule keeps: id order here, original-id order in
-// the sampled linking below).
+// Root of x with path halving (halving keeps every vertex under the same root;
+// the links keep id order: original ids over the tiles, relabelled ids over
+// the index).
 __device__ inline uint32_t wcc_find(uint32_t *par, uint32_t x) {
     for (;;) {
         uint32_t p = ld_par(par, x);
@@ -109,35 +109,23 @@ static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255
 // "Optimizing parallel graph connectivity computation via subgraph sampling",
 // IPDPS 2018).  The forest is kept in relabelled ids (the index's own), so a
 // vertex's parent entry sits next to its neighbours' in the order the lists
-// are walked and the heaviest vertices' entries stay cached; it is ordered by
-// original id (a parent's original id is at most its child's), so each root is
-// its component's minimum original id, the label the reference reports.
+// are walked and the heaviest vertices' entries stay cached.  It is ordered by
+// relabelled id (wcc_link: a root hangs under a smaller one), so the heaviest
+// vertex of a component becomes its root: a vertex linked to its heaviest
+// neighbour hangs itself under it, one compare on its own entry, where an
+// order unrelated to degree would make every neighbour of a hub race to hang
+// the hub under itself.  The reference's label, the component's smallest
+// original id, is found afterwards (k_aff_cmin) and written by k_aff_label.
 constexpr uint32_t AFF_ROUNDS = 2;     // neighbours linked for every vertex
 constexpr uint32_t AFF_SAMPLES = 1024; // vertices sampled to find the largest component
 
-// Joins the trees of u and v (relabelled ids): the root with the larger
-// original id is hooked under the other by compare-and-swap; if it was hooked
-// elsewhere meanwhile, join that tree instead.
-__device__ inline void aff_link(uint32_t *par, const uint32_t *__restrict__ key, uint32_t u, uint32_t v) {
-    uint32_t ru = wcc_find(par, u), rv = wcc_find(par, v);
-    while (ru != rv) {
-        const bool u_hi = key[ru] > key[rv];
-        const uint32_t hi = u_hi ? ru : rv, lo = u_hi ? rv : ru;
-        const uint32_t old = atomicCAS(&par[hi], hi, lo);
-        if (old == hi) break;  // hi was a root and now hangs under lo
-        ru = wcc_find(par, old);
-        rv = wcc_find(par, lo);
-    }
-}
-
 __global__ void k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
-                            const uint32_t *__restrict__ old_id, uint64_t nzn, uint32_t r,
-                            unsigned long long *linked) {
+                            uint64_t nzn, uint32_t r, unsigned long long *linked) {
     uint32_t c = 0;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = off[x];
         if (off[x + 1] - b > r) {
-            aff_link(par, old_id, (uint32_t)x, nbr[b + r]);
+            wcc_link(par, (uint32_t)x, nbr[b + r]);
             ++c;
         }
     }
@@ -156,24 +144,58 @@ __global__ void k_aff_sample(uint32_t *par, uint64_t n, uint32_t *out) {
 
 // Remaining neighbours of every vertex outside the component rooted at skip.
 __global__ void k_aff_final(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
-                            const uint32_t *__restrict__ old_id, uint64_t nzn, uint32_t skip,
-                            unsigned long long *linked) {
+                            uint64_t nzn, uint32_t skip, unsigned long long *linked) {
     uint32_t c = 0;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = off[x] + AFF_ROUNDS, e = off[x + 1];
         if (b >= e) continue;
         const uint32_t u = (uint32_t)x;
         if (wcc_find(par, u) == skip) continue;
-        for (uint64_t k = b; k < e; ++k) aff_link(par, old_id, u, nbr[k]);
+        for (uint64_t k = b; k < e; ++k) wcc_link(par, u, nbr[k]);
         c += (uint32_t)(e - b);
     }
     if (c) atomicAdd(linked, (unsigned long long)c);
 }
 
+// cmin[r] = the root's own original id, for every root r of the compressed
+// forest (par[x] is x's root).
+__global__ void k_aff_cmin_init(const uint32_t *__restrict__ par, const uint32_t *__restrict__ old_id, uint64_t nzn,
+                                uint32_t *cmin) {
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x)
+        if (par[x] == (uint32_t)x) cmin[x] = old_id[x];
+}
+
+// cmin[r] = the smallest original id of r's component.  The largest component
+// (the one holding the sampled vertex `skip`, nearly every vertex at scale)
+// is reduced in registers and once per workgroup; the others take one
+// atomicMin per vertex on their root.
+__global__ void __launch_bounds__(256) k_aff_cmin(const uint32_t *__restrict__ par, const uint32_t *__restrict__ old_id,
+                                                  uint64_t nzn, uint32_t skip, uint32_t *cmin) {
+    __shared__ uint32_t wmin[4];
+    const uint32_t big = skip < nzn ? par[skip] : 0xFFFFFFFFu;   // its root, after the last links
+    uint32_t m = 0xFFFFFFFFu;
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = par[x];
+        if (r == (uint32_t)x) continue;   // the root's own id is already in cmin
+        const uint32_t o = old_id[x];
+        if (r == big) m = min(m, o);
+        else atomicMin(&cmin[r], o);
+    }
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) m = min(m, (uint32_t)__shfl_xor(m, s));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t b = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+        if (b != 0xFFFFFFFFu) atomicMin(&cmin[big], b);   // b is set only when big is a root below nzn
+    }
+}
+
 // Labels by original id from the compressed forest over relabelled ids: the
-// root's original id for vertices with edges (relabelled ids below nzn),
-// 0xFFFFFFFF for the others; counts the roots.
-__global__ void k_aff_label(const uint32_t *__restrict__ par, const uint32_t *__restrict__ old_id,
+// component's smallest original id (cmin of the root) for vertices with
+// edges (relabelled ids below nzn), 0xFFFFFFFF for the others; counts the
+// roots.
+__global__ void k_aff_label(const uint32_t *__restrict__ par, const uint32_t *__restrict__ cmin,
                             const uint32_t *__restrict__ new_id, uint64_t nzn, uint64_t n, uint32_t *label,
                             unsigned long long *ncomp) {
     uint32_t c = 0;
@@ -181,7 +203,7 @@ __global__ void k_aff_label(const uint32_t *__restrict__ par, const uint32_t *__
         const uint32_t x = new_id[o];
         if (x < nzn) {
             const uint32_t r = par[x];
-            label[o] = old_id[r];
+            label[o] = cmin[r];
             c += r == x;
         } else {
             label[o] = 0xFFFFFFFFu;
@@ -201,6 +223,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     if (by_index) {
         GS_TRY(build_adjacency(const_cast<gs_graph *>(g)));
         if (w->wcc_label.n != g->n) GS_TRY(w->wcc_label.alloc(g->n));
+        if (w->wcc_cmin.n != std::max<uint64_t>(g->nzn, 1)) GS_TRY(w->wcc_cmin.alloc(std::max<uint64_t>(g->nzn, 1)));
     }
     w->timer.reset();
     w->stats = gs_stats{};
@@ -217,7 +240,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
         for (uint32_t r = 0; r < AFF_ROUNDS && g->nzn; ++r) {
             w->timer.begin(GS_K_WCC_HOOK, w->stream, &kb);
             k_aff_round<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
-                                                              g->old_id.ptr, g->nzn, r, w->counters.ptr);
+                                                              g->nzn, r, w->counters.ptr);
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
             w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
@@ -243,13 +266,20 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
             }
             w->timer.begin(GS_K_WCC_HOOK, w->stream, &kb);
             k_aff_final<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
-                                                              g->old_id.ptr, g->nzn, skip, w->counters.ptr);
+                                                              g->nzn, skip, w->counters.ptr);
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
         }
         if (g->nzn) {   // relabelled ids from nzn on have no edges: their own roots
             w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
             k_wcc_compress<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn);
+            GS_HIP(hipGetLastError());
+            // each component's smallest original id, by root
+            k_aff_cmin_init<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->old_id.ptr, g->nzn,
+                                                                  w->wcc_cmin.ptr);
+            GS_HIP(hipGetLastError());
+            k_aff_cmin<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->old_id.ptr, g->nzn, skip,
+                                                             w->wcc_cmin.ptr);
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
         }
@@ -283,7 +313,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     }
     GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
     if (by_index) {
-        k_aff_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->old_id.ptr, g->new_id.ptr, g->nzn, g->n,
+        k_aff_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, w->wcc_cmin.ptr, g->new_id.ptr, g->nzn, g->n,
                                                         w->wcc_label.ptr, w->counters.ptr);
     } else {
         k_wcc_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->degree.ptr,
