@@ -118,13 +118,14 @@ enum : int {
 };
 enum : unsigned long long {
     DF_BU = 1,        // bottom-up level
-    DF_HAVEQ = 2,     // the level end lists the new frontier (a top-down level)
+    DF_HAVEQ = 2,     // the queue holds the frontier (the root's level)
     DF_DONE = 4,
     DF_TD = 8,        // top-down kernels run
-    DF_LIST = 16,     // the frontier is listed first (it came from a bottom-up level)
+    DF_LIST = 16,     // the frontier is listed first (every top-down level after the root's)
     DF_OVERFLOW = 32, // the next depth would not fit in u8
     DF_ULIST = 64,    // bottom-up over the unreached list
-    DF_BUFULL = 128   // bottom-up over all vertices with edges
+    DF_BUFULL = 128,  // bottom-up over all vertices with edges
+    DF_LEVEL = 256    // a level runs (its level end)
 };
 constexpr int DO_LEVELS = 4;
 constexpr unsigned DO_TD_GRID = 2048;
@@ -552,7 +553,10 @@ __global__ void k_do_begin(unsigned long long *ctl, uint64_t n, uint64_t nzn) {
     else if (bu && nf < n / ctl[DO_BETA] && mf <= mu) bu = false;
     const unsigned long long reached = ctl[DO_REACHED];
     const unsigned long long unreached = nzn - (reached < nzn ? reached : nzn);
-    unsigned long long nfl = bu ? DF_BU : DF_HAVEQ;
+    // the level end never lists the new frontier: the next level is as likely
+    // bottom-up (s28 level 3 -> 4), and listing 4.8M vertices there cost more
+    // than the listing passes the small top-down levels now make
+    unsigned long long nfl = DF_LEVEL | (bu ? DF_BU : 0);
     if (bu) {
         if (unreached && unreached <= nzn / BU_LIST_DIV) nfl |= DF_ULIST;
         else if (unreached) nfl |= DF_BUFULL;
@@ -1159,11 +1163,7 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
                                                            g->adj_first.ptr, vis, front, next);
             k_bu_expand<false><<<gfull, 256, 0, w->stream>>>(ctl, nzn, nullptr, g->adj_off.ptr, g->adj_nbr.ptr,
                                                             g->adj_first.ptr, vis, front, next);
-            // level end: after a top-down level the next one is likely top-down too, so list the new frontier
-            k_do_level_end<true, true><<<gle, 256, 0, w->stream>>>(ctl, DF_HAVEQ, vis, front, next, w->depth_r.ptr,
-                                                                  g->deg_r.ptr, g->n, act_words, w->queue.ptr,
-                                                                  w->qdeg.ptr, ctl);
-            k_do_level_end<true, false><<<gle, 256, 0, w->stream>>>(ctl, DF_BU, vis, front, next, w->depth_r.ptr,
+            k_do_level_end<true, false><<<gle, 256, 0, w->stream>>>(ctl, DF_LEVEL, vis, front, next, w->depth_r.ptr,
                                                                    g->deg_r.ptr, g->n, act_words, w->queue.ptr,
                                                                    w->qdeg.ptr, ctl);
             k_do_end<<<1, 1, 0, w->stream>>>(ctl);
