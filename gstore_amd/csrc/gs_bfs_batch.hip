@@ -193,8 +193,9 @@ __global__ void __launch_bounds__(256) k_ms_td(const uint32_t *__restrict__ queu
 }
 
 // Bottom-up, one thread per vertex of ids [v0, v1) (degree < MS_HEAVY_DEG):
-// OR the neighbours' frontier masks, four at a time, until every lane the
-// vertex misses is found.
+// OR the neighbours' frontier masks, four at a time, until every lane of `all`
+// the vertex misses is found.  The finds are or-ed into next: the top-down
+// kernel of the same level, run before, owns the other lanes' bits.
 __device__ inline void ms_bu_light(uint64_t v0, uint64_t v1, unsigned long long all, const uint64_t *__restrict__ off,
                                    const uint32_t *__restrict__ nbr, const unsigned long long *__restrict__ vis,
                                    const unsigned long long *__restrict__ front, unsigned long long *next,
@@ -217,7 +218,7 @@ __device__ inline void ms_bu_light(uint64_t v0, uint64_t v1, unsigned long long 
                 if (k + j < e) acc |= f[j] & need;
             ck += (uint32_t)min<uint64_t>(4, e - k);
         }
-        if (acc) next[v] = acc;
+        if (acc) next[v] |= acc;   // a top-down kernel of the same level may have set other lanes' bits
     }
     if (ck) atomicAdd(checked, (unsigned long long)ck);
 }
@@ -254,7 +255,7 @@ __device__ inline void ms_bu_heavy(uint64_t nheavy, unsigned long long all, cons
             for (int s = 32; s > 0; s >>= 1) acc |= __shfl_xor(acc, s);
             if (acc == need) break;
         }
-        if (lane == 0 && acc) next[v] = acc;
+        if (lane == 0 && acc) next[v] |= acc;
     }
     if (lane == 0 && ck) atomicAdd(checked, ck);
 }
@@ -411,12 +412,12 @@ static unsigned ms_grid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 2
 static unsigned ms_heavy_grid(uint64_t nh) { return (unsigned)std::min<uint64_t>((nh + 3) / 4, 8192); }
 
 // ---- the same levels, driven from the device (one hipGraph per workspace) -------
-// The host loop above reads three counters after every level to choose the
-// direction and to stop.  Here those decisions are kernels: per level, a
-// one-thread kernel picks the direction from the previous level's counters
-// and raises flags; the list, queue-prefix, top-down, bottom-up and level-end
-// kernels return at once when their flag is down (or the search is done); a
-// 64-thread kernel closes the level.  MSG_LEVELS levels are captured once into
+// The host loop above reads three counters after every level to choose one
+// direction for the batch and to stop.  Here those decisions are kernels: per
+// level, a one-wave kernel picks each lane's direction from its frontier count
+// (k_msg_decide) and raises flags; the list, queue-prefix, top-down, bottom-up
+// and level-end kernels return at once when their flag is down (or the search
+// is done); a 64-thread kernel closes the level.  MSG_LEVELS levels are captured once into
 // a hipGraph, which the host launches until the done flag is up, so a batch
 // costs one host round trip per MSG_LEVELS levels instead of one per level.
 // Control words, u64, in w->ms_ctl (the first four are the level-end counters
@@ -433,16 +434,19 @@ enum : int {
     C_ACTIVE = 8,  // lanes whose frontier is not empty
     C_ALL = 9,     // lanes in use
     C_NQ = 10,     // queue length this level
-    C_ALPHA = 11,
-    C_BETA = 12,
+    C_BUL = 11,    // lanes searched bottom-up this level
+    C_TDL = 12,    // lanes pushed top-down this level
     C_NLEV = 13,   // levels done
     C_LIST = 14,   // queue cursor of the frontier listing
-    C_LANES = 16,  // per-level lane counts, 64 words per level
+    C_ALPHA = 15,
+    C_BETA = 16,
+    C_LANES = 18,  // per-level lane counts, 64 words per level
     C_WORDS = C_LANES + 64 * 256
 };
 enum : unsigned long long { F_BU = 1, F_HAVEQ = 2, F_DONE = 4, F_TD = 8, F_LIST = 16, F_OVERFLOW = 32 };
 constexpr int MSG_LEVELS = 4;
 constexpr unsigned MSG_TD_GRID = 2048;
+constexpr uint64_t MS_TD_DIV = 4096;   // in a bottom-up level, lanes with frontiers below nzn / 4096 stay top-down
 
 __global__ void k_msg_init(unsigned long long *ctl, int nr, const int32_t *__restrict__ qdeg, uint64_t m_u,
                            uint32_t alpha, uint32_t beta) {
@@ -461,20 +465,40 @@ __global__ void k_msg_init(unsigned long long *ctl, int nr, const int32_t *__res
     ctl[C_BETA] = beta;
 }
 
-__global__ void k_msg_decide(unsigned long long *ctl, uint64_t nzn) {
+// The level's direction.  For the batch as a whole Beamer's rule on the
+// summed frontier (as the host loop).  When that says bottom-up, the lanes
+// whose own frontiers are still small (below nzn / MS_TD_DIV: a lane's first
+// levels or its tail) are pushed top-down in the same level instead: searched
+// bottom-up, every vertex they miss would scan its whole list for them, where
+// the other lanes' finds stop the scan at the first neighbours.  One wave, a
+// lane per BFS lane.
+__global__ void __launch_bounds__(64) k_msg_decide(unsigned long long *ctl, uint64_t nzn) {
     const unsigned long long f = ctl[C_FLAGS];
     if (f & F_DONE) return;
-    const unsigned long long nf = ctl[C_NF], mf = ctl[C_MF], mu = ctl[C_MU];
+    const uint64_t level = ctl[C_LEVEL];
+    const unsigned long long act = ctl[C_ACTIVE], mf = ctl[C_MF], mu = ctl[C_MU];
     bool bu = f & F_BU;
     if (!bu && mf > mu / ctl[C_ALPHA]) bu = true;
-    else if (bu && nf < nzn / ctl[C_BETA]) bu = false;
-    const bool td = !bu && nf && mf;
-    const bool list = td && !(f & F_HAVEQ);
-    ctl[C_NQ] = nf;
+    else if (bu && ctl[C_NF] < nzn / ctl[C_BETA]) bu = false;
+    const int l = threadIdx.x;
+    // the frontier of lane l: its count of vertices reached by the previous level (the root before)
+    const unsigned long long nfl = level > 1 ? ctl[C_LANES + 64ull * (level - 2) + l] : 1ull;
+    const unsigned long long bul = __ballot(bu && ((act >> l) & 1ull) && nfl * MS_TD_DIV >= nzn);
+    if (l != 0) return;
+    const unsigned long long tdl = act & ~bul, nf = ctl[C_NF];
+    const bool td = tdl != 0 && nf != 0;
+    ctl[C_BUL] = bul;
+    ctl[C_TDL] = tdl;
+    ctl[C_NQ] = nf;   // the roots' queue; later levels list their top-down frontier (C_LIST)
     ctl[C_NF] = ctl[C_MF] = ctl[C_MU] = 0;
     ctl[C_BUILD] = ctl[C_LIST] = ctl[C_M] = 0;
-    // the level end lists the new frontier (the next level's queue) unless bottom-up
-    ctl[C_FLAGS] = (bu ? F_BU : F_HAVEQ) | (td ? F_TD : 0) | (list ? F_LIST : 0);
+    // F_BU also when no lane is left for it: the batch stays in its bottom-up phase
+    ctl[C_FLAGS] = (bu ? F_BU : 0) | (td ? F_TD : 0) | (td && !(f & F_HAVEQ) ? F_LIST : 0);
+}
+
+// Queue length of a top-down level: the roots, or the listed frontier.
+__device__ inline uint64_t msg_nq(const unsigned long long *ctl) {
+    return (ctl[C_FLAGS] & F_LIST) ? ctl[C_LIST] : ctl[C_NQ];
 }
 
 __global__ void __launch_bounds__(256) k_msg_list(const unsigned long long *ctl, uint64_t nzn,
@@ -485,9 +509,10 @@ __global__ void __launch_bounds__(256) k_msg_list(const unsigned long long *ctl,
     __shared__ uint32_t wsum[4];
     __shared__ unsigned long long bbase;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long tdl = ctl[C_TDL];
     for (uint64_t base = blockIdx.x * 256ull; base < nzn; base += (uint64_t)gridDim.x * 256) {
         const uint64_t v = base + threadIdx.x;
-        const bool in = v < nzn && front[v] != 0;
+        const bool in = v < nzn && (front[v] & tdl) != 0;
         const unsigned long long mask = __ballot(in);
         if (lane == 0) wsum[wv] = (uint32_t)__popcll(mask);
         __syncthreads();
@@ -510,7 +535,7 @@ __global__ void __launch_bounds__(256) k_msg_list(const unsigned long long *ctl,
 __global__ void __launch_bounds__(256) k_msg_block_sums(const unsigned long long *ctl, const int32_t *__restrict__ qdeg,
                                                         unsigned long long *qsum) {
     if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
-    const uint64_t nq = ctl[C_NQ], base = (uint64_t)blockIdx.x * 1024;
+    const uint64_t nq = msg_nq(ctl), base = (uint64_t)blockIdx.x * 1024;
     if (base >= nq) return;
     __shared__ unsigned long long s;
     if (threadIdx.x == 0) s = 0;
@@ -528,7 +553,7 @@ __global__ void __launch_bounds__(256) k_msg_block_sums(const unsigned long long
 __global__ void __launch_bounds__(1024) k_msg_scan(unsigned long long *ctl, unsigned long long *qsum) {
     if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
     __shared__ unsigned long long part[1024];
-    const uint64_t nb = (ctl[C_NQ] + 1023) / 1024;
+    const uint64_t nb = (msg_nq(ctl) + 1023) / 1024;
     const uint64_t per = (nb + 1023) / 1024, b0 = threadIdx.x * per, b1 = std::min<uint64_t>(b0 + per, nb);
     unsigned long long s = 0;
     for (uint64_t b = b0; b < b1; ++b) s += qsum[b];
@@ -553,7 +578,7 @@ __global__ void __launch_bounds__(1024) k_msg_scan(unsigned long long *ctl, unsi
 __global__ void __launch_bounds__(256) k_msg_deg_scan(const unsigned long long *ctl, const int32_t *__restrict__ qdeg,
                                                       const unsigned long long *__restrict__ bstart, uint64_t *qoff) {
     if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
-    const uint64_t nq = ctl[C_NQ], base = (uint64_t)blockIdx.x * 1024;
+    const uint64_t nq = msg_nq(ctl), base = (uint64_t)blockIdx.x * 1024;
     if (base >= nq) return;
     __shared__ uint32_t d[1024];
     __shared__ unsigned long long part[256];
@@ -592,7 +617,8 @@ __global__ void __launch_bounds__(256) k_msg_td(const unsigned long long *ctl, c
     __shared__ unsigned long long sfr[MS_CHUNK + 1];
     __shared__ uint64_t j_first;
     __shared__ int nstaged;
-    const uint64_t nq = ctl[C_NQ], m = ctl[C_M];
+    const uint64_t nq = msg_nq(ctl), m = ctl[C_M];
+    const unsigned long long tdl = ctl[C_TDL];
     for (uint64_t c0 = blockIdx.x * (uint64_t)MS_CHUNK; c0 < m; c0 += (uint64_t)gridDim.x * MS_CHUNK) {
         const uint32_t len = (uint32_t)min<uint64_t>(MS_CHUNK, m - c0);
         if (threadIdx.x == 0) {
@@ -618,7 +644,7 @@ __global__ void __launch_bounds__(256) k_msg_td(const unsigned long long *ctl, c
             const uint32_t v = queue[ja + k];
             sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
             sadj[k] = off[v] + (q < c0 ? c0 - q : 0);
-            sfr[k] = front[v];
+            sfr[k] = front[v] & tdl;   // the top-down lanes only
         }
         if (threadIdx.x == 0) sbeg[ns] = len;
         __syncthreads();
@@ -642,9 +668,9 @@ __global__ void __launch_bounds__(256) k_msg_bu(unsigned long long *ctl, uint64_
                                                 const unsigned long long *__restrict__ vis,
                                                 const unsigned long long *__restrict__ front,
                                                 unsigned long long *next) {
-    if ((ctl[C_FLAGS] & (F_BU | F_DONE)) != F_BU) return;
-    if (HEAVY) ms_bu_heavy(v1, ctl[C_ACTIVE], off, nbr, vis, front, next, ctl + C_CHECKED);
-    else ms_bu_light(v0, v1, ctl[C_ACTIVE], off, nbr, vis, front, next, ctl + C_CHECKED);
+    if ((ctl[C_FLAGS] & (F_BU | F_DONE)) != F_BU || !ctl[C_BUL]) return;
+    if (HEAVY) ms_bu_heavy(v1, ctl[C_BUL], off, nbr, vis, front, next, ctl + C_CHECKED);
+    else ms_bu_light(v0, v1, ctl[C_BUL], off, nbr, vis, front, next, ctl + C_CHECKED);
 }
 
 template <bool DEPTH>
@@ -657,10 +683,8 @@ __global__ void __launch_bounds__(256) k_msg_level_end(unsigned long long *ctl, 
     const uint32_t level = (uint32_t)ctl[C_LEVEL];
     const unsigned long long all = ctl[C_ACTIVE];   // as the host loop: the lanes still searching
     unsigned long long *lanes = ctl + C_LANES + 64ull * (level - 1);
-    if (f & F_HAVEQ)
-        ms_level_end<true, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
-    else
-        ms_level_end<false, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
+    // no queue: a top-down level lists the frontier of its own top-down lanes
+    ms_level_end<false, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
 }
 
 // Closes a level: lanes still active, entries examined, and the stop test
@@ -831,7 +855,7 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
         const uint64_t nh = g->ms_nhub;
         const unsigned hgrid = ms_heavy_grid(nh);
         for (int l = 0; l < MSG_LEVELS; ++l) {
-            k_msg_decide<<<1, 1, 0, w->stream>>>(ctl, nzn);
+            k_msg_decide<<<1, 64, 0, w->stream>>>(ctl, nzn);
             k_msg_list<<<gv, 256, 0, w->stream>>>(ctl, nzn, front, g->deg_r.ptr, w->queue.ptr, w->qdeg.ptr,
                                                   ctl + C_LIST);
             k_msg_block_sums<<<gb, 256, 0, w->stream>>>(ctl, w->qdeg.ptr, w->qsum.ptr);

@@ -80,7 +80,7 @@ static uint64_t need_bytes(const gs_graph *g, int kind, int lanes, bool keep_dep
             if (bfs_by_index(g))
                 return base + 24 * n + 64 * 8 * 2 + 8 * n + 8 * (n + 1) + 8 * (n / 1024 + 1) +
                        (keep_depth ? (uint64_t)lanes * n + 64 * n : 0) +
-                       (g->batch_method == 2 ? 8 * (16 + 64 * 256) : 0);   // device-driven levels: control words
+                       (g->batch_method == 2 ? 8 * (18 + 64 * 256) : 0);   // device-driven levels: control words
             return base + 24 * n + (uint64_t)lanes * n + rowact + 64 * 8 * 2;
         case GS_Q_PPR: return base + 24 * n;  // prior, acc, factor, rank (f32) and the FP64 sums
         case GS_Q_PAGERANK: return base + 12 * n;
