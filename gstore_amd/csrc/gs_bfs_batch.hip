@@ -446,7 +446,8 @@ enum : int {
 enum : unsigned long long { F_BU = 1, F_HAVEQ = 2, F_DONE = 4, F_TD = 8, F_LIST = 16, F_OVERFLOW = 32 };
 constexpr int MSG_LEVELS = 4;
 constexpr unsigned MSG_TD_GRID = 2048;
-constexpr uint64_t MS_TD_DIV = 4096;   // in a bottom-up level, lanes with frontiers below nzn / 4096 stay top-down
+constexpr uint64_t MS_TD_DIV = 16384;       // in a bottom-up level, lanes with frontiers below nzn / 16384 stay top-down
+constexpr uint64_t MS_SPLIT_NZN = 1ull << 24; // ... on graphs with at least 2^24 vertices with edges
 
 __global__ void k_msg_init(unsigned long long *ctl, int nr, const int32_t *__restrict__ qdeg, uint64_t m_u,
                            uint32_t alpha, uint32_t beta) {
@@ -470,8 +471,12 @@ __global__ void k_msg_init(unsigned long long *ctl, int nr, const int32_t *__res
 // whose own frontiers are still small (below nzn / MS_TD_DIV: a lane's first
 // levels or its tail) are pushed top-down in the same level instead: searched
 // bottom-up, every vertex they miss would scan its whole list for them, where
-// the other lanes' finds stop the scan at the first neighbours.  One wave, a
-// lane per BFS lane.
+// the other lanes' finds stop the scan at the first neighbours.  A 64-root
+// batch at Kron-28 went from 140 to 81 ms and at Kron-26 the served rate rose
+// by a third; up to Kron-24 the split lost 7-20% (a vertex missing one
+// top-down lane usually misses bottom-up ones too, so its scan barely
+// shortens, while the pushed hub lists cost atomics), so it applies from
+// MS_SPLIT_NZN vertices with edges.  One wave, a lane per BFS lane.
 __global__ void __launch_bounds__(64) k_msg_decide(unsigned long long *ctl, uint64_t nzn) {
     const unsigned long long f = ctl[C_FLAGS];
     if (f & F_DONE) return;
@@ -483,17 +488,22 @@ __global__ void __launch_bounds__(64) k_msg_decide(unsigned long long *ctl, uint
     const int l = threadIdx.x;
     // the frontier of lane l: its count of vertices reached by the previous level (the root before)
     const unsigned long long nfl = level > 1 ? ctl[C_LANES + 64ull * (level - 2) + l] : 1ull;
-    const unsigned long long bul = __ballot(bu && ((act >> l) & 1ull) && nfl * MS_TD_DIV >= nzn);
+    const bool split = nzn >= MS_SPLIT_NZN;
+    const unsigned long long bul = __ballot(bu && ((act >> l) & 1ull) && (!split || nfl * MS_TD_DIV >= nzn));
     if (l != 0) return;
     const unsigned long long tdl = act & ~bul, nf = ctl[C_NF];
     const bool td = tdl != 0 && nf != 0;
     ctl[C_BUL] = bul;
     ctl[C_TDL] = tdl;
-    ctl[C_NQ] = nf;   // the roots' queue; later levels list their top-down frontier (C_LIST)
+    ctl[C_NQ] = nf;   // the queue the last level end wrote (all frontier vertices), if it wrote one
     ctl[C_NF] = ctl[C_MF] = ctl[C_MU] = 0;
     ctl[C_BUILD] = ctl[C_LIST] = ctl[C_M] = 0;
-    // F_BU also when no lane is left for it: the batch stays in its bottom-up phase
-    ctl[C_FLAGS] = (bu ? F_BU : 0) | (td ? F_TD : 0) | (td && !(f & F_HAVEQ) ? F_LIST : 0);
+    // F_BU also when no lane is left for it: the batch stays in its bottom-up phase.  After a
+    // top-down level the level end lists the new frontier (F_HAVEQ), which serves a top-down
+    // level of every lane; after a bottom-up level, or when some lanes go bottom-up (their
+    // frontiers are the big ones), the top-down lanes' frontier is listed first (F_LIST).
+    const bool list = td && (!(f & F_HAVEQ) || bul);
+    ctl[C_FLAGS] = (bu ? F_BU : F_HAVEQ) | (td ? F_TD : 0) | (list ? F_LIST : 0);
 }
 
 // Queue length of a top-down level: the roots, or the listed frontier.
@@ -683,8 +693,10 @@ __global__ void __launch_bounds__(256) k_msg_level_end(unsigned long long *ctl, 
     const uint32_t level = (uint32_t)ctl[C_LEVEL];
     const unsigned long long all = ctl[C_ACTIVE];   // as the host loop: the lanes still searching
     unsigned long long *lanes = ctl + C_LANES + 64ull * (level - 1);
-    // no queue: a top-down level lists the frontier of its own top-down lanes
-    ms_level_end<false, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
+    if (f & F_HAVEQ)
+        ms_level_end<true, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
+    else
+        ms_level_end<false, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
 }
 
 // Closes a level: lanes still active, entries examined, and the stop test
