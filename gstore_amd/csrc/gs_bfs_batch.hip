@@ -446,8 +446,7 @@ enum : int {
 enum : unsigned long long { F_BU = 1, F_HAVEQ = 2, F_DONE = 4, F_TD = 8, F_LIST = 16, F_OVERFLOW = 32 };
 constexpr int MSG_LEVELS = 4;
 constexpr unsigned MSG_TD_GRID = 2048;
-constexpr uint64_t MS_TD_DIV = 16384;       // in a bottom-up level, lanes with frontiers below nzn / 16384 stay top-down
-constexpr uint64_t MS_SPLIT_NZN = 1ull << 24; // ... on graphs with at least 2^24 vertices with edges
+constexpr uint64_t MS_TD_DIV = 16384;   // in a bottom-up level, lanes with frontiers below nzn / 16384 stay top-down
 
 __global__ void k_msg_init(unsigned long long *ctl, int nr, const int32_t *__restrict__ qdeg, uint64_t m_u,
                            uint32_t alpha, uint32_t beta) {
@@ -472,11 +471,9 @@ __global__ void k_msg_init(unsigned long long *ctl, int nr, const int32_t *__res
 // levels or its tail) are pushed top-down in the same level instead: searched
 // bottom-up, every vertex they miss would scan its whole list for them, where
 // the other lanes' finds stop the scan at the first neighbours.  A 64-root
-// batch at Kron-28 went from 140 to 81 ms and at Kron-26 the served rate rose
-// by a third; up to Kron-24 the split lost 7-20% (a vertex missing one
-// top-down lane usually misses bottom-up ones too, so its scan barely
-// shortens, while the pushed hub lists cost atomics), so it applies from
-// MS_SPLIT_NZN vertices with edges.  One wave, a lane per BFS lane.
+// batch took 140 -> 80 ms at Kron-28, 8.9 -> 7.7 ms at Kron-24 and 2.9 -> 2.5
+// ms at Kron-20 (MS_TD_DIV from a sweep of 2^10 .. 2^18 at Kron-26 and -28).
+// One wave, a lane per BFS lane.
 __global__ void __launch_bounds__(64) k_msg_decide(unsigned long long *ctl, uint64_t nzn) {
     const unsigned long long f = ctl[C_FLAGS];
     if (f & F_DONE) return;
@@ -488,8 +485,7 @@ __global__ void __launch_bounds__(64) k_msg_decide(unsigned long long *ctl, uint
     const int l = threadIdx.x;
     // the frontier of lane l: its count of vertices reached by the previous level (the root before)
     const unsigned long long nfl = level > 1 ? ctl[C_LANES + 64ull * (level - 2) + l] : 1ull;
-    const bool split = nzn >= MS_SPLIT_NZN;
-    const unsigned long long bul = __ballot(bu && ((act >> l) & 1ull) && (!split || nfl * MS_TD_DIV >= nzn));
+    const unsigned long long bul = __ballot(bu && ((act >> l) & 1ull) && nfl * MS_TD_DIV >= nzn);
     if (l != 0) return;
     const unsigned long long tdl = act & ~bul, nf = ctl[C_NF];
     const bool td = tdl != 0 && nf != 0;
