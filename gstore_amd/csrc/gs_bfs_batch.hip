@@ -193,19 +193,27 @@ __global__ void __launch_bounds__(256) k_ms_td(const uint32_t *__restrict__ queu
 }
 
 // Bottom-up, one thread per vertex of ids [v0, v1) (degree < MS_HEAVY_DEG):
-// OR the neighbours' frontier masks, four at a time, until every lane of `all`
-// the vertex misses is found.  The finds are or-ed into next: the top-down
+// OR the neighbours' frontier masks, the first list entry from first[] (no
+// offset loads), then four at a time, until every lane of `all` the vertex
+// misses is found.  The finds are or-ed into next: the top-down
 // kernel of the same level, run before, owns the other lanes' bits.
 __device__ inline void ms_bu_light(uint64_t v0, uint64_t v1, unsigned long long all, const uint64_t *__restrict__ off,
-                                   const uint32_t *__restrict__ nbr, const unsigned long long *__restrict__ vis,
+                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ first,
+                                   const unsigned long long *__restrict__ vis,
                                    const unsigned long long *__restrict__ front, unsigned long long *next,
                                    unsigned long long *checked) {
     uint32_t ck = 0;
     for (uint64_t v = v0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < v1; v += (uint64_t)gridDim.x * blockDim.x) {
         const unsigned long long need = all & ~vis[v];
         if (!need) continue;
-        const uint64_t b = off[v], e = off[v + 1];
-        unsigned long long acc = 0;
+        // the first list entry (the heaviest neighbour) without the offsets; the rest if lanes remain
+        unsigned long long acc = front[first[v]] & need;
+        ++ck;
+        if (acc == need) {
+            next[v] |= acc;
+            continue;
+        }
+        const uint64_t b = off[v] + 1, e = off[v + 1];
         for (uint64_t k = b; k < e && acc != need; k += 4) {
             uint32_t u[4];
             unsigned long long f[4];
@@ -262,10 +270,11 @@ __device__ inline void ms_bu_heavy(uint64_t nheavy, unsigned long long all, cons
 
 __global__ void __launch_bounds__(256) k_ms_bu(uint64_t v0, uint64_t nzn, unsigned long long all,
                                                const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                                               const uint32_t *__restrict__ first,
                                                const unsigned long long *__restrict__ vis,
                                                const unsigned long long *__restrict__ front, unsigned long long *next,
                                                unsigned long long *checked) {
-    ms_bu_light(v0, nzn, all, off, nbr, vis, front, next, checked);
+    ms_bu_light(v0, nzn, all, off, nbr, first, vis, front, next, checked);
 }
 
 __global__ void __launch_bounds__(256) k_ms_bu_heavy(uint64_t nheavy, unsigned long long all,
@@ -671,12 +680,13 @@ __global__ void __launch_bounds__(256) k_msg_td(const unsigned long long *ctl, c
 template <bool HEAVY>
 __global__ void __launch_bounds__(256) k_msg_bu(unsigned long long *ctl, uint64_t v0, uint64_t v1,
                                                 const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                                                const uint32_t *__restrict__ first,
                                                 const unsigned long long *__restrict__ vis,
                                                 const unsigned long long *__restrict__ front,
                                                 unsigned long long *next) {
     if ((ctl[C_FLAGS] & (F_BU | F_DONE)) != F_BU || !ctl[C_BUL]) return;
     if (HEAVY) ms_bu_heavy(v1, ctl[C_BUL], off, nbr, vis, front, next, ctl + C_CHECKED);
-    else ms_bu_light(v0, v1, ctl[C_BUL], off, nbr, vis, front, next, ctl + C_CHECKED);
+    else ms_bu_light(v0, v1, ctl[C_BUL], off, nbr, first, vis, front, next, ctl + C_CHECKED);
 }
 
 template <bool DEPTH>
@@ -763,7 +773,8 @@ static int run_bfs_batch_index(const gs_graph *gc, gs_ws *w, const uint64_t *roo
                                                                         vis, front, next, w->counters.ptr + 4);
             if (nzn > nh)
                 k_ms_bu<<<ms_grid(nzn - nh), 256, 0, w->stream>>>(nh, nzn, active, g->adj_off.ptr, g->adj_nbr.ptr,
-                                                                  vis, front, next, w->counters.ptr + 4);
+                                                                  g->adj_first.ptr, vis, front, next,
+                                                                  w->counters.ptr + 4);
         } else if (n_f && m_f) {
             uint64_t nq = n_f;
             if (!have_queue) {
@@ -852,7 +863,7 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
     const std::vector<const void *> key = {g, (const void *)(uintptr_t)nzn, (const void *)(uintptr_t)g->ms_nhub,
                                            vis, front, next, dvm, ctl,
                                            w->queue.ptr, w->qdeg.ptr, w->qoff.ptr, w->qsum.ptr, g->adj_off.ptr,
-                                           g->adj_nbr.ptr, g->deg_r.ptr};
+                                           g->adj_nbr.ptr, g->adj_first.ptr, g->deg_r.ptr};
     if (!w->ms_graph || w->ms_graph_key != key) {
         if (w->ms_graph) {
             GS_HIP(hipGraphExecDestroy(w->ms_graph));
@@ -872,10 +883,11 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
             k_msg_td<<<MSG_TD_GRID, 256, 0, w->stream>>>(ctl, w->queue.ptr, w->qoff.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
                                                          front, vis, next);
             if (nh)
-                k_msg_bu<true><<<hgrid, 256, 0, w->stream>>>(ctl, 0, nh, g->adj_off.ptr, g->adj_nbr.ptr, vis, front, next);
+                k_msg_bu<true><<<hgrid, 256, 0, w->stream>>>(ctl, 0, nh, g->adj_off.ptr, g->adj_nbr.ptr, nullptr, vis,
+                                                             front, next);
             if (nzn > nh)
                 k_msg_bu<false><<<ms_grid(nzn - nh), 256, 0, w->stream>>>(ctl, nh, nzn, g->adj_off.ptr, g->adj_nbr.ptr,
-                                                                          vis, front, next);
+                                                                          g->adj_first.ptr, vis, front, next);
             if (dvm)
                 k_msg_level_end<true><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr, dvm,
                                                                  w->queue.ptr, w->qdeg.ptr);
