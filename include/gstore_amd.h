@@ -246,10 +246,12 @@ int gs_set_wcc_method(gs_graph *g, int method);
  * lanes over the neighbour-list index, direction-optimizing on the union
  * frontier (top-down pushes of lane masks, bottom-up ORs of the neighbours'
  * masks with a wave per vertex of degree >= 128), with a host round trip per
- * level; 2 (default) runs the same levels with the direction choice and the
- * stop test in kernels, four levels per launch of a hipGraph captured once per
- * workspace; 0 sweeps the tiles with lane masks.  Depths and level counts are
- * identical.  Directed and semi-external grids use 0. */
+ * level; 2 (default) runs the levels with the direction choice and the stop
+ * test in kernels, four levels per launch of a hipGraph captured once per
+ * workspace, and in bottom-up levels pushes the lanes whose own frontiers are
+ * still small top-down in the same level; 0 sweeps the tiles with lane masks.
+ * Depths and level counts are identical.  Directed and semi-external grids
+ * use 0. */
 int gs_set_batch_bfs_method(gs_graph *g, int method);
 
 /* k-core method on undirected resident grids: 1 (default) lists the vertices
