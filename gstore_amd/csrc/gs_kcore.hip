@@ -164,12 +164,15 @@ __global__ void __launch_bounds__(256) k_kc_peel_list(int32_t *deg, uint64_t nzn
 }
 
 // One decrement per list entry of the listed vertices, balanced over entries
-// (qoff = prefix of the list lengths; KC_CHUNK entries per workgroup, located
-// through staged queue entries as in the top-down BFS).  The KC_HUB heaviest
-// vertices (relabelled ids below KC_HUB) collect their decrements in LDS
-// first: one global atomic per workgroup and hub instead of one per entry.
+// (qoff = prefix of the list lengths; KC_CHUNK entries per workgroup step,
+// located through staged queue entries as in the top-down BFS).  The KC_HUB
+// heaviest vertices (relabelled ids below KC_HUB) collect their decrements in
+// LDS first.  Workgroups stride over the chunks (KC_GRID of them), so the LDS
+// counts are cleared and flushed once per workgroup, not once per chunk: a
+// hub met in many chunks costs one global atomic per workgroup.
 constexpr uint32_t KC_CHUNK = 2048;
 constexpr uint32_t KC_HUB = 4096;
+constexpr uint32_t KC_GRID = 2048;
 
 __global__ void __launch_bounds__(256) k_kc_push(const uint32_t *__restrict__ queue, const uint64_t *__restrict__ qoff,
                                                  uint64_t nq, uint64_t m, const uint64_t *__restrict__ off,
@@ -179,46 +182,46 @@ __global__ void __launch_bounds__(256) k_kc_push(const uint32_t *__restrict__ qu
     __shared__ uint32_t hub[KC_HUB];
     __shared__ uint64_t j_first;
     __shared__ int nstaged;
-    const uint64_t c0 = blockIdx.x * (uint64_t)KC_CHUNK;
-    if (c0 >= m) return;
-    const uint32_t len = (uint32_t)min<uint64_t>(KC_CHUNK, m - c0);
     for (uint32_t i = threadIdx.x; i < KC_HUB; i += blockDim.x) hub[i] = 0;
-    if (threadIdx.x == 0) {
-        uint64_t lo = 0, hi = nq - 1;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+    for (uint64_t c0 = blockIdx.x * (uint64_t)KC_CHUNK; c0 < m; c0 += (uint64_t)gridDim.x * KC_CHUNK) {
+        const uint32_t len = (uint32_t)min<uint64_t>(KC_CHUNK, m - c0);
+        if (threadIdx.x == 0) {
+            uint64_t lo = 0, hi = nq - 1;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+            }
+            const uint64_t ja = lo;
+            hi = nq - 1;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
+            }
+            j_first = ja;
+            nstaged = (int)(lo - ja + 1);
         }
-        const uint64_t ja = lo;
-        hi = nq - 1;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
+        __syncthreads();
+        const uint64_t ja = j_first;
+        const int ns = nstaged;
+        for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+            const uint64_t q = qoff[ja + k];
+            sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
+            sadj[k] = off[queue[ja + k]] + (q < c0 ? c0 - q : 0);
         }
-        j_first = ja;
-        nstaged = (int)(lo - ja + 1);
-    }
-    __syncthreads();
-    const uint64_t ja = j_first;
-    const int ns = nstaged;
-    for (int k = threadIdx.x; k < ns; k += blockDim.x) {
-        const uint64_t q = qoff[ja + k];
-        sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
-        sadj[k] = off[queue[ja + k]] + (q < c0 ? c0 - q : 0);
-    }
-    if (threadIdx.x == 0) sbeg[ns] = len;
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
-        int lo = 0, hi = ns - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+        if (threadIdx.x == 0) sbeg[ns] = len;
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
+            int lo = 0, hi = ns - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t y = nbr[sadj[lo] + (t - sbeg[lo])];
+            if (y < KC_HUB) atomicAdd(&hub[y], 1u);
+            else atomicSub(&deg[y], 1);
         }
-        const uint32_t y = nbr[sadj[lo] + (t - sbeg[lo])];
-        if (y < KC_HUB) atomicAdd(&hub[y], 1u);
-        else atomicSub(&deg[y], 1);
+        __syncthreads();   // the staged tables are rewritten by the next step
     }
-    __syncthreads();
     for (uint32_t i = threadIdx.x; i < KC_HUB; i += blockDim.x)
         if (hub[i]) atomicSub(&deg[i], (int32_t)hub[i]);
 }
@@ -258,7 +261,7 @@ static int run_kcore_index(const gs_graph *gc, gs_ws *w, int k, int *rounds_out)
         w->timer.begin(GS_K_KCORE_SCAN, w->stream, &kb);
         GS_TRY(queue_prefix(w, nq, scratch, &m));
         if (m)
-            k_kc_push<<<(unsigned)((m + KC_CHUNK - 1) / KC_CHUNK), 256, 0, w->stream>>>(
+            k_kc_push<<<(unsigned)std::min<uint64_t>((m + KC_CHUNK - 1) / KC_CHUNK, KC_GRID), 256, 0, w->stream>>>(
                 w->queue.ptr, w->qoff.ptr, nq, m, g->adj_off.ptr, g->adj_nbr.ptr, w->kc_deg.ptr);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_KCORE_SCAN, w->stream, kb);
