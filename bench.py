@@ -85,7 +85,7 @@ def cpu_baseline(scale, threads):
     start, grid, degree = oracle.convert(edges, n)
     del edges
     log(f"cpu baseline graph s{scale} ready in {time.time() - t0:.1f}s, E={grid.size}")
-    iters = 5
+    iters = 10   # about 10 s of PageRank on the host at the default scale
     t = time.perf_counter()
     oracle.pagerank(n, start, grid, degree, iters, mode=1, nthreads=threads)
     dt = time.perf_counter() - t
@@ -297,7 +297,7 @@ def main():
     ap.add_argument("--pr-iters", type=int, default=20)
     ap.add_argument("--bfs-roots", type=int, default=16)
     ap.add_argument("--k", type=int, default=8)
-    ap.add_argument("--cpu-scale", type=int, default=22)
+    ap.add_argument("--cpu-scale", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--algos", default="pr,bfs,wcc,kcore",
                     help="subset of the step's jobs (profiling runs); the default is the full step")
