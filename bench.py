@@ -309,8 +309,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine has no multi-GPU path "
+              "(tiles and index blocks would shard over a 2-D checkerboard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.mode == "serve":
