@@ -203,16 +203,25 @@ __global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl,
 // with neighbours scans them until it meets a frontier vertex.  The first
 // list entry comes from first[] (one coalesced load, no offsets): short lists
 // are sorted, so it is the heaviest neighbour and on the big levels nearly
-// always in the frontier; only the vertices it leaves unresolved load their
-// offsets and scan on from the second entry.  Without LIST each wave owns
-// 64 x BU_VPL consecutive vertices 0 .. n-1 and skips the reached ones, and
-// writes its next-bitmap words without atomics.  With LIST the vertices come
-// from a list of the unreached ones (k_unreached_list), so a
-// late level touches only them; found vertices are or-ed into next.  Lanes
-// that are done load nbr[0] (in bounds) rather than branch, so the loads of
-// the other vertices still issue together.
+// always in the frontier.  The vertices it leaves unresolved (about a tenth on
+// the big levels) are packed into LDS per wave and scanned one per lane from
+// the second entry: four loads of ids and four of frontier words per round for
+// the wave, where scanning each lane's four vertices side by side issued 32
+// loads per round whatever few lanes still had work (the texture address unit
+// was 90% busy at s28 level 4).  Without LIST each wave owns 64 x BU_VPL
+// consecutive vertices 0 .. n-1, skips the reached ones, gathers its finds in
+// LDS words and writes its next-bitmap words without atomics.  With LIST the
+// vertices come from a list of the unreached ones (k_unreached_list), so a
+// late level touches only them; found vertices are or-ed into next.
 constexpr int BU_VPL = 4;
 constexpr int BU_LIST_DIV = 32;  // bottom-up from the unreached list once at most nzn / 32 are left
+
+// Orders a wave's LDS writes before its lanes read each other's entries.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <bool LIST>
 __global__ void __launch_bounds__(256) k_bu_expand(unsigned long long *ctl, uint64_t n, const uint32_t *__restrict__ list,
@@ -220,19 +229,20 @@ __global__ void __launch_bounds__(256) k_bu_expand(unsigned long long *ctl, uint
                                                    const uint32_t *__restrict__ first, const uint32_t *__restrict__ vis,
                                                    const uint32_t *__restrict__ front, uint32_t *next) {
     if (!do_gate(ctl, LIST ? DF_ULIST : DF_BUFULL)) return;
-    const int lane = threadIdx.x & 63;
+    __shared__ uint32_t s_u[4][64 * BU_VPL];   // per wave: its unresolved vertices
+    __shared__ uint32_t s_fw[4][2 * BU_VPL];   // per wave: next-bitmap words of its vertices (full mode)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t checked = 0;
     const uint64_t cnt = LIST ? ctl[DO_ULN] : n;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * BU_VPL;
     for (uint64_t base = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull) * BU_VPL; base < cnt;
          base += stride) {
-        uint64_t v[BU_VPL], k[BU_VPL], e[BU_VPL];
+        uint64_t v[BU_VPL];
         bool found[BU_VPL], open[BU_VPL], any = false;
         uint32_t f0[BU_VPL];
 #pragma unroll
         for (int h = 0; h < BU_VPL; ++h) {
             const uint64_t i = base + 64 * h + lane;
-            k[h] = e[h] = 0;
             found[h] = false;
             if (LIST) {
                 open[h] = i < cnt;
@@ -244,7 +254,6 @@ __global__ void __launch_bounds__(256) k_bu_expand(unsigned long long *ctl, uint
             any = any || open[h];
         }
         if (!__ballot(any)) continue;  // the wave's vertices are all reached (late levels: nearly every wave)
-        any = false;
 #pragma unroll
         for (int h = 0; h < BU_VPL; ++h) f0[h] = first[open[h] ? v[h] : 0];
 #pragma unroll
@@ -255,56 +264,59 @@ __global__ void __launch_bounds__(256) k_bu_expand(unsigned long long *ctl, uint
                 found[h] = (fw >> (f0[h] & 31)) & 1u;
             }
         }
+        // the vertices the first entry left unresolved, packed per wave (the
+        // wave's first-entry finds go to LDS words first in the full mode)
+        uint32_t nu = 0;
 #pragma unroll
         for (int h = 0; h < BU_VPL; ++h) {
-            if (open[h] && !found[h]) {
-                k[h] = off[v[h]] + 1;
-                e[h] = off[v[h] + 1];
-            }
-            any = any || k[h] < e[h];
-        }
-        // four neighbours of each vertex per round trip: ids, then their frontier words
-        while (any) {
-            bool live[BU_VPL];
-            uint32_t u[BU_VPL][4], fw[BU_VPL][4];
-#pragma unroll
-            for (int h = 0; h < BU_VPL; ++h) {
-                live[h] = k[h] < e[h] && !found[h];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) u[h][j] = nbr[live[h] ? min(k[h] + j, e[h] - 1) : 0];
-            }
-#pragma unroll
-            for (int h = 0; h < BU_VPL; ++h)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) fw[h][j] = front[u[h][j] >> 5];
-            any = false;
-#pragma unroll
-            for (int h = 0; h < BU_VPL; ++h) {
-                if (live[h]) {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        if (!found[h] && k[h] + j < e[h]) {
-                            ++checked;
-                            found[h] = (fw[h][j] >> (u[h][j] & 31)) & 1u;
-                        }
-                    k[h] += 4;
-                }
-                any = any || (k[h] < e[h] && !found[h]);
-            }
-        }
-        if (LIST) {
-#pragma unroll
-            for (int h = 0; h < BU_VPL; ++h)
-                if (found[h]) atomicOr(&next[v[h] >> 5], 1u << (v[h] & 31));
-        } else {
-#pragma unroll
-            for (int h = 0; h < BU_VPL; ++h) {
+            if (!LIST) {
                 const unsigned long long m = __ballot(found[h]);
-                if (lane == 0 && m) {
-                    next[((base + 64 * h) >> 5)] = (uint32_t)m;
-                    next[((base + 64 * h) >> 5) + 1] = (uint32_t)(m >> 32);
+                if (lane == 0) {
+                    s_fw[wv][2 * h] = (uint32_t)m;
+                    s_fw[wv][2 * h + 1] = (uint32_t)(m >> 32);
                 }
+            } else if (found[h]) {
+                atomicOr(&next[v[h] >> 5], 1u << (v[h] & 31));
             }
+            const bool un = open[h] && !found[h];
+            const unsigned long long um = __ballot(un);
+            if (un) s_u[wv][nu + __popcll(um & ((1ull << lane) - 1))] = (uint32_t)v[h];
+            nu += (uint32_t)__popcll(um);
+        }
+        wave_sync();
+        // one unresolved vertex per lane, four neighbours per round trip: ids, then their
+        // frontier words (8 loads per round for the wave, where four vertices per lane took 32)
+        for (uint32_t t = lane; t < nu; t += 64) {
+            const uint32_t x = s_u[wv][t];
+            uint64_t kk = off[x] + 1;
+            const uint64_t ee = off[x + 1];
+            bool fx = false;
+            while (kk < ee && !fx) {
+                uint32_t u[4], fw[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) u[j] = nbr[min(kk + j, ee - 1)];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fw[j] = front[u[j] >> 5];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (!fx && kk + j < ee) {
+                        ++checked;
+                        fx = (fw[j] >> (u[j] & 31)) & 1u;
+                    }
+                kk += 4;
+            }
+            if (fx) {
+                if (LIST) atomicOr(&next[x >> 5], 1u << (x & 31));
+                else atomicOr(&s_fw[wv][(uint32_t)(x - base) >> 5], 1u << ((x - base) & 31));
+            }
+        }
+        if (!LIST) {
+            wave_sync();
+            // the wave owns the 8 next-bitmap words of its 256 vertices
+            if (lane < 8 && s_fw[wv][lane]) next[(base >> 5) + lane] = s_fw[wv][lane];
+            wave_sync();   // s_fw and s_u are rewritten by the next step
+        } else {
+            wave_sync();   // s_u is rewritten by the next step
         }
     }
     if (checked) atomicAdd(ctl + DO_CHK, (unsigned long long)checked);
