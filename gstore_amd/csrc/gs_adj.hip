@@ -130,6 +130,7 @@ enum : unsigned long long {
 constexpr int DO_LEVELS = 4;
 constexpr unsigned DO_TD_GRID = 2048;
 constexpr unsigned DO_SCAN_GRID = 1024;
+constexpr unsigned DO_GRID = 2048;
 
 // true when the kernel's flag is up and the search is not done
 __device__ inline bool do_gate(const unsigned long long *ctl, unsigned long long flag) {
@@ -1151,10 +1152,13 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
             w->do_graph = nullptr;
         }
         const uint64_t nzn = g->nzn;
-        const unsigned gle = std::max(1u, vgrid((act_words + LE_WPT - 1) / LE_WPT));
-        const unsigned gul = std::max(1u, vgrid((act_words + 3) / 4));
-        const unsigned gfull = std::max(1u, vgrid((nzn + BU_VPL - 1) / BU_VPL));
-        const unsigned glist = std::max(1u, vgrid(nzn / BU_LIST_DIV / BU_VPL + 1));
+        // grid-stride kernels, at most DO_GRID workgroups: enough to fill the GPU, and a
+        // kernel whose flag is down for the level costs the launch of fewer empty workgroups
+        auto cap = [](unsigned x) { return std::max(1u, std::min(x, DO_GRID)); };
+        const unsigned gle = cap(vgrid((act_words + LE_WPT - 1) / LE_WPT));
+        const unsigned gul = cap(vgrid((act_words + 3) / 4));
+        const unsigned gfull = cap(vgrid((nzn + BU_VPL - 1) / BU_VPL));
+        const unsigned glist = cap(vgrid(nzn / BU_LIST_DIV / BU_VPL + 1));
         const unsigned gsc = (unsigned)std::min<uint64_t>(DO_SCAN_GRID, std::max<uint64_t>((nzn + 1023) / 1024, 1));
         uint32_t *vis = w->bm_vis.ptr, *front = w->bm_front.ptr, *next = w->bm_next.ptr;
         GS_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
