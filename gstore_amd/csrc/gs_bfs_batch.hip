@@ -455,6 +455,7 @@ enum : int {
 enum : unsigned long long { F_BU = 1, F_HAVEQ = 2, F_DONE = 4, F_TD = 8, F_LIST = 16, F_OVERFLOW = 32 };
 constexpr int MSG_LEVELS = 4;
 constexpr unsigned MSG_TD_GRID = 2048;
+constexpr unsigned MSG_GRID = 2048;
 constexpr uint64_t MS_TD_DIV = 16384;   // in a bottom-up level, lanes with frontiers below nzn / 16384 stay top-down
 
 __global__ void k_msg_init(unsigned long long *ctl, int nr, const int32_t *__restrict__ qdeg, uint64_t m_u,
@@ -550,17 +551,19 @@ __global__ void __launch_bounds__(256) k_msg_list(const unsigned long long *ctl,
 __global__ void __launch_bounds__(256) k_msg_block_sums(const unsigned long long *ctl, const int32_t *__restrict__ qdeg,
                                                         unsigned long long *qsum) {
     if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
-    const uint64_t nq = msg_nq(ctl), base = (uint64_t)blockIdx.x * 1024;
-    if (base >= nq) return;
+    const uint64_t nq = msg_nq(ctl);
     __shared__ unsigned long long s;
-    if (threadIdx.x == 0) s = 0;
-    __syncthreads();
-    unsigned long long c = 0;
-    for (int i = threadIdx.x; i < 1024; i += blockDim.x)
-        if (base + i < nq) c += (uint32_t)qdeg[base + i];
-    atomicAdd(&s, c);
-    __syncthreads();
-    if (threadIdx.x == 0) qsum[blockIdx.x] = s;
+    for (uint64_t base = (uint64_t)blockIdx.x * 1024; base < nq; base += (uint64_t)gridDim.x * 1024) {
+        if (threadIdx.x == 0) s = 0;
+        __syncthreads();
+        unsigned long long c = 0;
+        for (int i = threadIdx.x; i < 1024; i += blockDim.x)
+            if (base + i < nq) c += (uint32_t)qdeg[base + i];
+        atomicAdd(&s, c);
+        __syncthreads();
+        if (threadIdx.x == 0) qsum[base / 1024] = s;
+        __syncthreads();
+    }
 }
 
 // Exclusive prefix of qsum[0 .. nb) in place (one workgroup); the total is the
@@ -593,29 +596,31 @@ __global__ void __launch_bounds__(1024) k_msg_scan(unsigned long long *ctl, unsi
 __global__ void __launch_bounds__(256) k_msg_deg_scan(const unsigned long long *ctl, const int32_t *__restrict__ qdeg,
                                                       const unsigned long long *__restrict__ bstart, uint64_t *qoff) {
     if ((ctl[C_FLAGS] & (F_TD | F_DONE)) != F_TD) return;
-    const uint64_t nq = msg_nq(ctl), base = (uint64_t)blockIdx.x * 1024;
-    if (base >= nq) return;
+    const uint64_t nq = msg_nq(ctl);
     __shared__ uint32_t d[1024];
     __shared__ unsigned long long part[256];
-    for (int i = threadIdx.x; i < 1024; i += blockDim.x) d[i] = base + i < nq ? (uint32_t)qdeg[base + i] : 0;
-    __syncthreads();
-    unsigned long long loc[4], run = 0;
-    for (int j = 0; j < 4; ++j) {
-        loc[j] = run;
-        run += d[threadIdx.x * 4 + j];
-    }
-    part[threadIdx.x] = run;
-    __syncthreads();
-    for (int step = 1; step < 256; step <<= 1) {
-        const unsigned long long v = threadIdx.x >= (unsigned)step ? part[threadIdx.x - step] : 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * 1024; base < nq; base += (uint64_t)gridDim.x * 1024) {
+        for (int i = threadIdx.x; i < 1024; i += blockDim.x) d[i] = base + i < nq ? (uint32_t)qdeg[base + i] : 0;
         __syncthreads();
-        part[threadIdx.x] += v;
+        unsigned long long loc[4], run = 0;
+        for (int j = 0; j < 4; ++j) {
+            loc[j] = run;
+            run += d[threadIdx.x * 4 + j];
+        }
+        part[threadIdx.x] = run;
         __syncthreads();
-    }
-    const unsigned long long before = (threadIdx.x ? part[threadIdx.x - 1] : 0) + bstart[blockIdx.x];
-    for (int j = 0; j < 4; ++j) {
-        const uint64_t v = base + threadIdx.x * 4 + j;
-        if (v < nq) qoff[v] = before + loc[j];
+        for (int step = 1; step < 256; step <<= 1) {
+            const unsigned long long v = threadIdx.x >= (unsigned)step ? part[threadIdx.x - step] : 0;
+            __syncthreads();
+            part[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const unsigned long long before = (threadIdx.x ? part[threadIdx.x - 1] : 0) + bstart[base / 1024];
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t v = base + threadIdx.x * 4 + j;
+            if (v < nq) qoff[v] = before + loc[j];
+        }
+        __syncthreads();   // d and part are rewritten by the next step
     }
 }
 
@@ -870,9 +875,12 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
             w->ms_graph = nullptr;
         }
         GS_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-        const unsigned gv = nzn ? ms_grid(nzn) : 1, gb = (unsigned)std::max<uint64_t>((nzn + 1023) / 1024, 1);
+        // grid-stride kernels, at most MSG_GRID workgroups: enough to fill the GPU, and a kernel
+        // whose flag is down for the level costs the launch of fewer empty workgroups
+        auto cap = [](unsigned x) { return std::max(1u, std::min(x, MSG_GRID)); };
+        const unsigned gv = cap(ms_grid(nzn)), gb = cap((unsigned)std::min<uint64_t>((nzn + 1023) / 1024, MSG_GRID));
         const uint64_t nh = g->ms_nhub;
-        const unsigned hgrid = ms_heavy_grid(nh);
+        const unsigned hgrid = cap(ms_heavy_grid(nh)), glight = cap(ms_grid(nzn - nh));
         for (int l = 0; l < MSG_LEVELS; ++l) {
             k_msg_decide<<<1, 64, 0, w->stream>>>(ctl, nzn);
             k_msg_list<<<gv, 256, 0, w->stream>>>(ctl, nzn, front, g->deg_r.ptr, w->queue.ptr, w->qdeg.ptr,
@@ -886,7 +894,7 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
                 k_msg_bu<true><<<hgrid, 256, 0, w->stream>>>(ctl, 0, nh, g->adj_off.ptr, g->adj_nbr.ptr, nullptr, vis,
                                                              front, next);
             if (nzn > nh)
-                k_msg_bu<false><<<ms_grid(nzn - nh), 256, 0, w->stream>>>(ctl, nh, nzn, g->adj_off.ptr, g->adj_nbr.ptr,
+                k_msg_bu<false><<<glight, 256, 0, w->stream>>>(ctl, nh, nzn, g->adj_off.ptr, g->adj_nbr.ptr,
                                                                           g->adj_first.ptr, vis, front, next);
             if (dvm)
                 k_msg_level_end<true><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr, dvm,
