@@ -139,9 +139,10 @@ __device__ inline bool do_gate(const unsigned long long *ctl, unsigned long long
 
 // Top-down, balanced over edges: qoff is the exclusive prefix of the frontier
 // queue's degrees, and each workgroup step takes TD_CHUNK consecutive frontier
-// edges (grid-stride: the grid is fixed when the levels are captured).  The queue entries its range touches are staged in LDS (offset into
-// the range and list start), each lane finds its entry by a search there.  A
-// neighbour not yet visited or marked is marked with atomicOr.
+// edges (grid-stride: the grid is fixed when the levels are captured).  The
+// queue entries its range touches are staged in LDS (offset into the range
+// and list start), each lane finds its entry by a search there.  A neighbour
+// not yet visited or marked is marked with atomicOr.
 constexpr uint32_t TD_CHUNK = 4096;
 
 __device__ inline void td_visit(uint32_t v, const uint32_t *__restrict__ vis, uint32_t *next) {
