@@ -119,10 +119,18 @@ static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255
 constexpr uint32_t AFF_ROUNDS = 2;     // neighbours linked for every vertex
 constexpr uint32_t AFF_SAMPLES = 1024; // vertices sampled to find the largest component
 
+// Round r links every vertex with its r-th list entry; round 0 takes it from
+// first[] (every vertex below nzn has one), without the offsets.
 __global__ void k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
-                            uint64_t nzn, uint32_t r, unsigned long long *linked) {
+                            const uint32_t *__restrict__ first, uint64_t nzn, uint32_t r,
+                            unsigned long long *linked) {
     uint32_t c = 0;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
+        if (r == 0) {
+            wcc_link(par, (uint32_t)x, first[x]);
+            ++c;
+            continue;
+        }
         const uint64_t b = off[x];
         if (off[x + 1] - b > r) {
             wcc_link(par, (uint32_t)x, nbr[b + r]);
@@ -199,14 +207,32 @@ __global__ void k_aff_label(const uint32_t *__restrict__ par, const uint32_t *__
                             const uint32_t *__restrict__ new_id, uint64_t nzn, uint64_t n, uint32_t *label,
                             unsigned long long *ncomp) {
     uint32_t c = 0;
-    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n; o += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t x = new_id[o];
-        if (x < nzn) {
-            const uint32_t r = par[x];
-            label[o] = cmin[r];
-            c += r == x;
+    // four vertices per thread: a 16-byte load of new_id and a 16-byte label store
+    for (uint64_t o0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) * 4; o0 < n;
+         o0 += (uint64_t)gridDim.x * blockDim.x * 4) {
+        if (o0 + 4 <= n) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(new_id + o0);
+            const uint32_t x[4] = {q.x, q.y, q.z, q.w};
+            uint32_t r[4], l[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = x[k] < nzn ? par[x[k]] : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                l[k] = x[k] < nzn ? cmin[r[k]] : 0xFFFFFFFFu;
+                c += x[k] < nzn && r[k] == x[k];
+            }
+            *reinterpret_cast<uint4 *>(label + o0) = uint4{l[0], l[1], l[2], l[3]};
         } else {
-            label[o] = 0xFFFFFFFFu;
+            for (uint64_t o = o0; o < n; ++o) {
+                const uint32_t x = new_id[o];
+                if (x < nzn) {
+                    const uint32_t r = par[x];
+                    label[o] = cmin[r];
+                    c += r == x;
+                } else {
+                    label[o] = 0xFFFFFFFFu;
+                }
+            }
         }
     }
     if (c) atomicAdd(ncomp, (unsigned long long)c);
@@ -240,7 +266,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
         for (uint32_t r = 0; r < AFF_ROUNDS && g->nzn; ++r) {
             w->timer.begin(GS_K_WCC_HOOK, w->stream, &kb);
             k_aff_round<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
-                                                              g->nzn, r, w->counters.ptr);
+                                                              g->adj_first.ptr, g->nzn, r, w->counters.ptr);
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
             w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
@@ -313,7 +339,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     }
     GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
     if (by_index) {
-        k_aff_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, w->wcc_cmin.ptr, g->new_id.ptr, g->nzn, g->n,
+        k_aff_label<<<vgrid((g->n + 3) / 4), 256, 0, w->stream>>>(w->wcc_parent.ptr, w->wcc_cmin.ptr, g->new_id.ptr, g->nzn, g->n,
                                                         w->wcc_label.ptr, w->counters.ptr);
     } else {
         k_wcc_label<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->degree.ptr,
