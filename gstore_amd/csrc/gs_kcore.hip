@@ -163,65 +163,58 @@ __global__ void __launch_bounds__(256) k_kc_peel_list(int32_t *deg, uint64_t nzn
     }
 }
 
-// One decrement per list entry of the listed vertices, balanced over entries
-// (qoff = prefix of the list lengths; KC_CHUNK entries per workgroup step,
-// located through staged queue entries as in the top-down BFS).  The KC_HUB
-// heaviest vertices (relabelled ids below KC_HUB) collect their decrements in
-// LDS first.  Workgroups stride over the chunks (KC_GRID of them), so the LDS
-// counts are cleared and flushed once per workgroup, not once per chunk: a
-// hub met in many chunks costs one global atomic per workgroup.
-constexpr uint32_t KC_CHUNK = 2048;
+// One decrement per list entry of the listed vertices.  The KC_HUB heaviest
+// vertices (relabelled ids below KC_HUB) collect their decrements in LDS
+// first, and the workgroups (at most KC_GRID) stride over the queue, so a
+// hub's count is flushed with one global atomic per workgroup.
 constexpr uint32_t KC_HUB = 4096;
 constexpr uint32_t KC_GRID = 2048;
 
-__global__ void __launch_bounds__(256) k_kc_push(const uint32_t *__restrict__ queue, const uint64_t *__restrict__ qoff,
-                                                 uint64_t nq, uint64_t m, const uint64_t *__restrict__ off,
-                                                 const uint32_t *__restrict__ nbr, int32_t *deg) {
-    __shared__ uint32_t sbeg[KC_CHUNK + 2];
-    __shared__ uint64_t sadj[KC_CHUNK + 1];
+// A lane per listed vertex pushes a list of at most KC_LANE_MAX entries
+// alone; longer lists (a vertex whose degree fell below k keeps its whole
+// list, and the (int16) test can list a hub) are pushed by the whole wave, 64
+// entries per step.  No prefix of the list lengths and no search per entry
+// (the edge-balanced push with both took 7.6 ms per run at s28, this one 6.8);
+// *pushed counts the entries.
+constexpr uint32_t KC_LANE_MAX = 64;
+
+__global__ void __launch_bounds__(256) k_kc_push(const uint32_t *__restrict__ queue, const int32_t *__restrict__ qlen,
+                                                 uint64_t nq, const uint64_t *__restrict__ off,
+                                                 const uint32_t *__restrict__ nbr, int32_t *deg,
+                                                 unsigned long long *pushed) {
     __shared__ uint32_t hub[KC_HUB];
-    __shared__ uint64_t j_first;
-    __shared__ int nstaged;
     for (uint32_t i = threadIdx.x; i < KC_HUB; i += blockDim.x) hub[i] = 0;
-    for (uint64_t c0 = blockIdx.x * (uint64_t)KC_CHUNK; c0 < m; c0 += (uint64_t)gridDim.x * KC_CHUNK) {
-        const uint32_t len = (uint32_t)min<uint64_t>(KC_CHUNK, m - c0);
-        if (threadIdx.x == 0) {
-            uint64_t lo = 0, hi = nq - 1;
-            while (lo < hi) {
-                uint64_t mid = (lo + hi + 1) >> 1;
-                if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
-            }
-            const uint64_t ja = lo;
-            hi = nq - 1;
-            while (lo < hi) {
-                uint64_t mid = (lo + hi + 1) >> 1;
-                if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
-            }
-            j_first = ja;
-            nstaged = (int)(lo - ja + 1);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    auto push = [&](uint32_t y) {
+        if (y < KC_HUB) atomicAdd(&hub[y], 1u);
+        else atomicSub(&deg[y], 1);
+    };
+    unsigned long long cnt = 0;
+    for (uint64_t w0 = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6) * 64; w0 < nq; w0 += nw * 64) {
+        const uint64_t i = w0 + lane;
+        uint32_t v = 0, len = 0;
+        if (i < nq) {
+            v = queue[i];
+            len = (uint32_t)qlen[i];
         }
-        __syncthreads();
-        const uint64_t ja = j_first;
-        const int ns = nstaged;
-        for (int k = threadIdx.x; k < ns; k += blockDim.x) {
-            const uint64_t q = qoff[ja + k];
-            sbeg[k] = q <= c0 ? 0u : (uint32_t)(q - c0);
-            sadj[k] = off[queue[ja + k]] + (q < c0 ? c0 - q : 0);
+        if (len && len <= KC_LANE_MAX) {
+            const uint64_t b = off[v];
+            for (uint32_t k = 0; k < len; ++k) push(nbr[b + k]);
         }
-        if (threadIdx.x == 0) sbeg[ns] = len;
-        __syncthreads();
-        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
-            int lo = 0, hi = ns - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
-            }
-            const uint32_t y = nbr[sadj[lo] + (t - sbeg[lo])];
-            if (y < KC_HUB) atomicAdd(&hub[y], 1u);
-            else atomicSub(&deg[y], 1);
+        cnt += len;
+        unsigned long long heavy = __ballot(len > KC_LANE_MAX);
+        while (heavy) {   // wave-uniform
+            const int l = __ffsll((long long)heavy) - 1;
+            heavy &= heavy - 1;
+            const uint32_t hv = __shfl(v, l), hl = __shfl(len, l);
+            const uint64_t b = off[hv];
+            for (uint32_t k = lane; k < hl; k += 64) push(nbr[b + k]);
         }
-        __syncthreads();   // the staged tables are rewritten by the next step
     }
+    if (cnt) atomicAdd(pushed, cnt);
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < KC_HUB; i += blockDim.x)
         if (hub[i]) atomicSub(&deg[i], (int32_t)hub[i]);
 }
@@ -236,8 +229,7 @@ using namespace gs;
 static int run_kcore_index(const gs_graph *gc, gs_ws *w, int k, int *rounds_out) {
     gs_graph *g = const_cast<gs_graph *>(gc);
     GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, g->deg_r.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
-    std::vector<unsigned long long> scratch;
-    uint64_t pushed = 0;
+    GS_HIP(hipMemsetAsync(w->counters.ptr + 1, 0, 8, w->stream));   // entries pushed, all rounds
     int nr = 0;
     for (;;) {
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
@@ -257,17 +249,15 @@ static int run_kcore_index(const gs_graph *gc, gs_ws *w, int k, int *rounds_out)
             set_error("gs_kcore: still removing vertices after %d rounds", nr);
             return GS_ERR_DEVICE;
         }
-        uint64_t m = 0;
         w->timer.begin(GS_K_KCORE_SCAN, w->stream, &kb);
-        GS_TRY(queue_prefix(w, nq, scratch, &m));
-        if (m)
-            k_kc_push<<<(unsigned)std::min<uint64_t>((m + KC_CHUNK - 1) / KC_CHUNK, KC_GRID), 256, 0, w->stream>>>(
-                w->queue.ptr, w->qoff.ptr, nq, m, g->adj_off.ptr, g->adj_nbr.ptr, w->kc_deg.ptr);
+        k_kc_push<<<(unsigned)std::min<uint64_t>((nq + 255) / 256, KC_GRID), 256, 0, w->stream>>>(
+            w->queue.ptr, w->qdeg.ptr, nq, g->adj_off.ptr, g->adj_nbr.ptr, w->kc_deg.ptr, w->counters.ptr + 1);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_KCORE_SCAN, w->stream, kb);
-        pushed += m;
     }
-    w->stats.edges_scanned = pushed;
+    GS_HIP(hipMemcpyAsync(w->h_counters + 1, w->counters.ptr + 1, 8, hipMemcpyDeviceToHost, w->stream));
+    GS_HIP(hipStreamSynchronize(w->stream));
+    w->stats.edges_scanned = w->h_counters[1];
     *rounds_out = nr;
     return GS_OK;
 }
