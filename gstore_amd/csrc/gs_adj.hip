@@ -720,13 +720,49 @@ __global__ void __launch_bounds__(256) k_sort_short_lists(const uint64_t *__rest
     }
 }
 
+// Sorts the lists of 65 to MID_LIST entries ascending: one workgroup per list,
+// a bitonic network over the list padded to a power of two in LDS.  Vertices
+// v0 .. v1-1 are those whose degree bucket may hold such lists; the others
+// are skipped (workgroup-uniform).  First entries then are the heaviest
+// neighbours for these lists too, which the bottom-up first-entry check tests.
+constexpr uint32_t MID_LIST = 4096;
+__global__ void __launch_bounds__(256) k_sort_mid_lists(const uint64_t *__restrict__ off, uint64_t v0, uint64_t v1,
+                                                        uint32_t *nbr) {
+    __shared__ uint32_t s[MID_LIST];
+    for (uint64_t v = v0 + blockIdx.x; v < v1; v += gridDim.x) {
+        const uint64_t b = off[v];
+        const uint32_t d = (uint32_t)(off[v + 1] - b);
+        if (d <= 64 || d > MID_LIST) continue;
+        uint32_t p = 128;
+        while (p < d) p <<= 1;
+        for (uint32_t i = threadIdx.x; i < p; i += 256) s[i] = i < d ? nbr[b + i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= p; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < p; i += 256) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint32_t x = s[i], y = s[l];
+                        if ((x > y) == ((i & k) == 0)) {
+                            s[i] = y;
+                            s[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t i = threadIdx.x; i < d; i += 256) nbr[b + i] = s[i];
+        __syncthreads();   // s is refilled for the next list
+    }
+}
+
 // ---- relabelling by descending degree ------------------------------------------------
 
 // Bucket of a degree: 8 sub-buckets per power of two; 0 for no edges.  Higher
 // bucket = earlier position.
-__device__ inline uint32_t deg_bucket(uint32_t d) {
+__host__ __device__ inline uint32_t deg_bucket(uint32_t d) {
     if (!d) return 0;
-    uint32_t lg = 31 - __clz(d);
+    uint32_t lg = 31 - __builtin_clz(d);
     uint32_t sub = lg >= 3 ? (d >> (lg - 3)) & 7 : 0;
     return 1 + lg * 8 + sub;  // 1 .. 256
 }
@@ -871,6 +907,8 @@ int build_adjacency(gs_graph *g) {
     for (int b = 1 + 8 * (int)MS_HEAVY_LOG; b < NBUCKET; ++b) g->ms_nhub += h[b];
     unsigned long long run = 0;
     for (int b = NBUCKET - 1; b >= 0; --b) { unsigned long long c = h[b]; h[b] = run; run += c; }
+    // ids of the buckets that hold degrees 65 .. MID_LIST (h[b] = first id of bucket b)
+    const uint64_t mid_lo = h[deg_bucket(MID_LIST)], mid_hi = h[deg_bucket(65) - 1];
     GS_HIP(hipMemcpyAsync(cur64.ptr, h.data(), NBUCKET * 8, hipMemcpyHostToDevice, st));
     GS_TRY(g->new_id.alloc(n));
     GS_TRY(g->old_id.alloc(n));
@@ -911,6 +949,11 @@ int build_adjacency(gs_graph *g) {
     if (g->nzn) {
         k_sort_short_lists<<<8192, 256, 0, st>>>(g->adj_off.ptr, 0, g->nzn, g->adj_nbr.ptr);
         GS_HIP(hipGetLastError());
+        if (mid_hi > mid_lo) {
+            k_sort_mid_lists<<<(unsigned)std::min<uint64_t>(mid_hi - mid_lo, 8192), 256, 0, st>>>(
+                    g->adj_off.ptr, mid_lo, mid_hi, g->adj_nbr.ptr);
+            GS_HIP(hipGetLastError());
+        }
         GS_TRY(g->adj_first.alloc(g->nzn));
         k_first_entry<<<vgrid(g->nzn), 256, 0, st>>>(g->adj_off.ptr, g->adj_nbr.ptr, g->nzn, g->adj_first.ptr);
         GS_HIP(hipGetLastError());
