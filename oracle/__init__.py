@@ -40,7 +40,13 @@ def lib() -> ctypes.CDLL:
         L.or_convert_count.restype = i64
         L.or_convert_scatter.argtypes = [p, u64, u64, i, p, p]
         L.or_pagerank.argtypes = [u64, i, p, p, p, i, i, i, p]
-        L.or_ppr.argtypes = [u64, i, p, p, p, u64, i, p]
+        L.or_ppr.argtypes = [u64, i, p, p, p, u64, i, i, p]
+        L.or_pagerank64.argtypes = [u64, i, p, p, p, i, i, p]
+        L.or_kron_convert_count.argtypes = [i, i, u64, u64, i, u64, p, p]
+        L.or_kron_convert_count.restype = i64
+        L.or_kron_convert_scatter.argtypes = [i, i, u64, u64, i, u64, p, p]
+        L.or_grid_checksum.argtypes = [u64, p, p]
+        L.or_grid_checksum.restype = u64
         L.or_bfs.argtypes = [u64, i, p, p, u64, p, p, i]
         L.or_wcc.argtypes = [u64, i, p, p, p, p]
         L.or_kcore.argtypes = [u64, p, p, p, i, p, p, p]
@@ -87,12 +93,43 @@ def pagerank(n, start, grid, degree, iters=5, mode=0, nthreads=1, directed=False
     return out
 
 
-def ppr(n, start, grid, degree, source, iters=20, directed=False) -> np.ndarray:
+def pagerank64(n, start, grid, degree, iters=5, nthreads=1, directed=False) -> np.ndarray:
+    """FP64 PageRank of the intended form (exact 1/deg factor)."""
+    out = np.empty(n, np.float64)
+    if lib().or_pagerank64(n, int(directed), _p(start), _p(grid), _p(degree), iters, nthreads, _p(out)):
+        raise RuntimeError("or_pagerank64 failed")
+    return out
+
+
+def ppr(n, start, grid, degree, source, iters=20, directed=False, nthreads=1) -> np.ndarray:
     """FP64 personalised PageRank."""
     out = np.empty(n, np.float64)
-    if lib().or_ppr(n, int(directed), _p(start), _p(grid), _p(degree), source, iters, _p(out)):
+    if lib().or_ppr(n, int(directed), _p(start), _p(grid), _p(degree), source, iters, nthreads, _p(out)):
         raise RuntimeError("or_ppr failed")
     return out
+
+
+def kron_convert(scale: int, edge_factor: int = 16, seed1: int = 1, seed2: int = 2, directed: bool = False,
+                 chunk: int = 1 << 26):
+    """Generate and convert in chunks (the tuple list is never held whole)
+    -> (start, grid, degree), equal to convert(kron(scale, 0, M))."""
+    L = lib()
+    n = 1 << scale
+    T = L.or_total_tiles(256 * L.or_groups(n), int(directed))
+    start = np.empty(T + 1, np.uint64)
+    degree = np.empty(n, np.int32)
+    E = L.or_kron_convert_count(scale, edge_factor, seed1, seed2, int(directed), chunk, _p(start), _p(degree))
+    if E < 0:
+        raise RuntimeError("or_kron_convert_count failed")
+    grid = np.empty(E, np.uint32)
+    if L.or_kron_convert_scatter(scale, edge_factor, seed1, seed2, int(directed), chunk, _p(start), _p(grid)):
+        raise RuntimeError("or_kron_convert_scatter failed")
+    return start, grid, degree
+
+
+def grid_checksum(start, grid) -> int:
+    """Order-independent checksum of the (tile, edge word) multiset."""
+    return int(lib().or_grid_checksum(start.size - 1, _p(start), _p(grid)))
 
 
 def bfs(n, start, grid, root, directed=False):

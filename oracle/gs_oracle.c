@@ -19,6 +19,12 @@
  *                         degree = endpoint count, start monotone, sentinel)
  *   or_pagerank compat -- closed form (only degree-1 vertices are non-zero)
  *   or_pagerank intd.  -- float64 scipy recurrence in tests (tolerance stated)
+ *   or_pagerank64      -- the same float64 scipy recurrence (1e-12), and the
+ *                         FP32 or_pagerank within FP32 rounding of it
+ *   or_ppr             -- float64 scipy recurrence from two sources (1e-12)
+ *   or_kron_convert_*  -- equal to or_convert(or_kron_range) over the whole list
+ *   or_grid_checksum   -- invariant under in-tile permutation, changed by one
+ *                         moved or altered edge
  *   or_bfs             -- scipy.sparse.csgraph breadth-first distances
  *   or_wcc             -- scipy.sparse.csgraph.connected_components
  *   or_kcore           -- independent numpy peeling in tests + brute force
@@ -300,6 +306,90 @@ void or_convert_scatter(const uint32_t *edges, uint64_t m, uint64_t n, int direc
     free(cur);
 }
 
+/* Generate-and-convert in chunks, for scales whose tuple list does not fit in
+ * host memory beside the grid (M = ef * 2^lg tuples; at scale 28 the list is
+ * 34 GB and the grid 17 GB).  The tuples are generated twice, `chunk` at a
+ * time: or_kron_convert_count fills start[0..T] and degree[0..N) exactly as
+ * or_convert_count does for the whole list, or_kron_convert_scatter fills the
+ * grid as or_convert_scatter does (input order within a tile).  The two calls
+ * together give the same arrays as or_convert(or_kron_range(0, M)). */
+int64_t or_kron_convert_count(int lg, int ef, uint64_t s1, uint64_t s2, int directed, uint64_t chunk,
+                              uint64_t *start, int32_t *degree)
+{
+    uint64_t n = 1ull << lg, m = (uint64_t)ef << lg;
+    uint64_t p = or_groups(n), T = or_total_tiles(256 * p, directed);
+    uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * 2 * chunk);
+    if (!buf) return -1;
+    memset(start, 0, sizeof(uint64_t) * (T + 1));
+    memset(degree, 0, sizeof(int32_t) * n);
+    for (uint64_t e0 = 0; e0 < m; e0 += chunk) {
+        uint64_t c = m - e0 < chunk ? m - e0 : chunk;
+        or_kron_range(lg, s1, s2, (int64_t)e0, (int64_t)(e0 + c), buf);
+        for (uint64_t k = 0; k < c; ++k) {
+            uint64_t a = buf[2 * k], b = buf[2 * k + 1];
+            if (a == b) continue;
+            uint64_t v0 = a, v1 = b;
+            if (!directed && v0 > v1) { v0 = b; v1 = a; }
+            start[or_tile_slot(v0 >> 16, v1 >> 16, p, directed)]++;
+            degree[a]++;
+            if (!directed) degree[b]++;
+        }
+    }
+    free(buf);
+    uint64_t s = 0;
+    for (uint64_t t = 0; t <= T; ++t) { uint64_t c = start[t]; start[t] = s; s += c; }
+    return (int64_t)s;
+}
+
+int or_kron_convert_scatter(int lg, int ef, uint64_t s1, uint64_t s2, int directed, uint64_t chunk,
+                            const uint64_t *start, uint32_t *grid)
+{
+    uint64_t n = 1ull << lg, m = (uint64_t)ef << lg;
+    uint64_t p = or_groups(n), T = or_total_tiles(256 * p, directed);
+    uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * 2 * chunk);
+    uint64_t *cur = (uint64_t *)malloc(sizeof(uint64_t) * T);
+    if (!buf || !cur) { free(buf); free(cur); return -1; }
+    memcpy(cur, start, sizeof(uint64_t) * T);
+    for (uint64_t e0 = 0; e0 < m; e0 += chunk) {
+        uint64_t c = m - e0 < chunk ? m - e0 : chunk;
+        or_kron_range(lg, s1, s2, (int64_t)e0, (int64_t)(e0 + c), buf);
+        for (uint64_t k = 0; k < c; ++k) {
+            uint64_t a = buf[2 * k], b = buf[2 * k + 1];
+            if (a == b) continue;
+            uint64_t v0 = a, v1 = b;
+            if (!directed && v0 > v1) { v0 = b; v1 = a; }
+            uint64_t t = or_tile_slot(v0 >> 16, v1 >> 16, p, directed);
+            grid[cur[t]++] = (uint32_t)(v0 & 0xFFFF) | ((uint32_t)(v1 & 0xFFFF) << 16);
+        }
+    }
+    free(buf);
+    free(cur);
+    return 0;
+}
+
+/* Order-independent checksum of a grid: the sum (mod 2^64) over stored edges
+ * of a 64-bit mix of (tile slot << 32 | edge word).  Equal for two grids that
+ * hold the same multiset of edges per tile, whatever their in-tile order
+ * (which the reference's converter leaves to thread timing,
+ * gstore.cpp:1442-1446). */
+static inline uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27; x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+uint64_t or_grid_checksum(uint64_t T, const uint64_t *start, const uint32_t *grid)
+{
+    uint64_t s = 0;
+    #pragma omp parallel for reduction(+:s) schedule(dynamic, 4096)
+    for (uint64_t t = 0; t < T; ++t)
+        for (uint64_t k = start[t]; k < start[t + 1]; ++k)
+            s += mix64((t << 32) | grid[k]);
+    return s;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Common tile walk: every tile slot in .start order with its coordinates.   */
 /* ------------------------------------------------------------------------ */
@@ -408,26 +498,88 @@ int or_pagerank(uint64_t n, int directed, const uint64_t *start, const uint32_t 
     return 0;
 }
 
-/* Personalised PageRank in FP64 (no reference counterpart; uses the tile
- * scatter of pr.cpp:141-152): r_0 = e_src, prior = r / deg,
- * r_{t+1}[v] = 0.15 [v == src] + 0.85 * sum of prior over in-edges. */
-int or_ppr(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, const int32_t *degree,
-           uint64_t src, int iters, double *out)
+static inline double atomic_add_f64(double *a, double v)
 {
-    walk_t w;
-    if (src >= n) return -1;
-    if (walk_init(&w, n, directed, start, grid)) return -1;
-    double *prior = (double *)calloc(n, sizeof(double)), *acc = (double *)calloc(n, sizeof(double));
-    prior[src] = degree[src] ? 1.0 / degree[src] : 0.0;
-    for (int it = 0; it < iters; ++it) {
-        for (uint64_t t = 0; t < w.T; ++t) {
-            uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
-            for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+    union { double f; uint64_t u; } o, nw;
+    o.u = __atomic_load_n((uint64_t *)a, __ATOMIC_RELAXED);
+    do { nw.f = o.f + v; } while (!__atomic_compare_exchange_n((uint64_t *)a, &o.u, nw.u, 1,
+                                                               __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+    return o.f;
+}
+
+/* One FP64 scatter over every stored edge (pr.cpp:141-152): acc[v1] +=
+ * prior[v0], and acc[v0] += prior[v1] on undirected grids.  nthreads > 1
+ * sweeps tiles in parallel with CAS adds (FP64 summation order then varies,
+ * far below the FP32 tolerances this oracle is used for). */
+static void scatter64(const walk_t *w, const double *prior, double *acc, int nthreads)
+{
+    const uint32_t *grid = w->grid;
+    int directed = w->directed;
+    if (nthreads <= 1) {
+        for (uint64_t t = 0; t < w->T; ++t) {
+            uint64_t o0 = (uint64_t)w->row[t] << 16, o1 = (uint64_t)w->col[t] << 16;
+            for (uint64_t k = w->start[t]; k < w->start[t + 1]; ++k) {
                 uint64_t v0 = o0 + (grid[k] & 0xFFFF), v1 = o1 + (grid[k] >> 16);
                 acc[v1] += prior[v0];
                 if (!directed) acc[v0] += prior[v1];
             }
         }
+        return;
+    }
+    #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+    for (uint64_t t = 0; t < w->T; ++t) {
+        uint64_t o0 = (uint64_t)w->row[t] << 16, o1 = (uint64_t)w->col[t] << 16;
+        for (uint64_t k = w->start[t]; k < w->start[t + 1]; ++k) {
+            uint64_t v0 = o0 + (grid[k] & 0xFFFF), v1 = o1 + (grid[k] >> 16);
+            atomic_add_f64(&acc[v1], prior[v0]);
+            if (!directed) atomic_add_f64(&acc[v0], prior[v1]);
+        }
+    }
+}
+
+/* PageRank of the intended form in FP64 (the float degree factor of
+ * GS_PR_INTENDED taken exactly): prior0 = 1/N, acc = scatter of prior,
+ * prior = (0.15 + 0.85 acc) / deg (0 for deg 0), as pr.cpp:82-117, 286-290
+ * with the int32 truncation of pr.cpp:112-114 left out.  Output is prior after
+ * `iters` iterations (the post-multiplied form of pr.cpp, Q2).  This is the
+ * FP64 reference that FP32 PageRank is held to. */
+int or_pagerank64(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid,
+                  const int32_t *degree, int iters, int nthreads, double *out)
+{
+    walk_t w;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    double *prior = out, *acc = (double *)calloc(n, sizeof(double));
+    if (!acc) { walk_free(&w); return -1; }
+    for (uint64_t v = 0; v < n; ++v) prior[v] = 1.0 / (double)n;
+    for (int it = 0; it < iters; ++it) {
+        scatter64(&w, prior, acc, nthreads);
+        #pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+        for (uint64_t v = 0; v < n; ++v) {
+            double r = 0.15 + 0.85 * acc[v];
+            prior[v] = degree[v] ? r / (double)degree[v] : 0.0;
+            acc[v] = 0.0;
+        }
+    }
+    free(acc);
+    walk_free(&w);
+    return 0;
+}
+
+/* Personalised PageRank in FP64 (no reference counterpart; uses the tile
+ * scatter of pr.cpp:141-152): r_0 = e_src, prior = r / deg,
+ * r_{t+1}[v] = 0.15 [v == src] + 0.85 * sum of prior over in-edges. */
+int or_ppr(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, const int32_t *degree,
+           uint64_t src, int iters, int nthreads, double *out)
+{
+    walk_t w;
+    if (src >= n) return -1;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    double *prior = (double *)calloc(n, sizeof(double)), *acc = (double *)calloc(n, sizeof(double));
+    if (!prior || !acc) { free(prior); free(acc); walk_free(&w); return -1; }
+    prior[src] = degree[src] ? 1.0 / degree[src] : 0.0;
+    for (int it = 0; it < iters; ++it) {
+        scatter64(&w, prior, acc, nthreads);
+        #pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
         for (uint64_t v = 0; v < n; ++v) {
             double r = (v == src ? 0.15 : 0.0) + 0.85 * acc[v];
             out[v] = r;

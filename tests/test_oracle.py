@@ -189,6 +189,66 @@ def test_ppr_vs_scipy(name, n, e):
         assert got.sum() <= 1.0 + 1e-9
 
 
+@pytest.mark.parametrize("name,n,e", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("directed", [False, True])
+def test_pagerank64_vs_float64(name, n, e, directed):
+    start, grid, degree = oracle.convert(e, n, directed)
+    A = _adj(n, start, grid, directed)
+    want = _pr_f64(A.T.tocsr(), n, degree, 10)
+    for nt in (1, 4):
+        got = oracle.pagerank64(n, start, grid, degree, 10, nthreads=nt, directed=directed)
+        assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+    # the FP32 oracle in intended mode sits within FP32 rounding of it
+    f32 = oracle.pagerank(n, start, grid, degree, 10, mode=1, directed=directed)
+    assert np.max(np.abs(f32 - want)) <= 2e-6 * max(1.0, np.max(np.abs(want)))
+
+
+def test_ppr_threaded_vs_scipy():
+    n, e = graphs.kron(12)
+    start, grid, degree = oracle.convert(e, n)
+    A = _adj(n, start, grid)
+    inv = np.where(degree > 0, 1.0 / np.maximum(degree, 1), 0.0)
+    src = int(np.argmax(degree))
+    r = np.zeros(n)
+    r[src] = 1.0
+    for _ in range(20):
+        r = 0.85 * (A.T @ (r * inv))
+        r[src] += 0.15
+    got = oracle.ppr(n, start, grid, degree, src, 20, nthreads=4)
+    assert np.max(np.abs(got - r)) <= 1e-12
+
+
+def test_kron_convert_chunked_equals_whole_list():
+    for s, chunk in ((12, 1000), (17, 1 << 20)):
+        want = oracle.convert(oracle.kron(s, 0, 16 << s), 1 << s)
+        got = oracle.kron_convert(s, chunk=chunk)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b)
+
+
+def test_grid_checksum_is_a_multiset_hash():
+    n, e = graphs.tile_crossing()
+    start, grid, degree = oracle.convert(e, n)
+    c = oracle.grid_checksum(start, grid)
+    # in-tile permutation leaves it unchanged
+    rng = np.random.default_rng(1)
+    g2 = grid.copy()
+    full = np.flatnonzero(np.diff(start.astype(np.int64)) > 3)
+    assert full.size >= 4
+    for t in full:
+        a, b = int(start[t]), int(start[t + 1])
+        g2[a:b] = rng.permutation(g2[a:b])
+    assert oracle.grid_checksum(start, g2) == c
+    # an altered edge word, or an edge moved to the neighbouring tile, changes it
+    g3 = grid.copy()
+    g3[7] ^= 1
+    assert oracle.grid_checksum(start, g3) != c
+    t = int(np.flatnonzero(np.diff(start.astype(np.int64)) > 0)[3])
+    s4 = start.copy()
+    s4[t + 1] -= 1      # last edge of tile t now belongs to tile t+1
+    assert oracle.grid_checksum(s4, grid) != c
+
+
 def test_pagerank_parallel_matches_sequential_within_fp32():
     n, e = graphs.kron(12)
     start, grid, degree = oracle.convert(e, n)
