@@ -112,6 +112,7 @@ _SIGS = {
     "gs_get_level_stats": (ctypes.c_int, [_P, ctypes.POINTER(_LevelStat), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int)]),
     "gs_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_debug_fail_alloc": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_sweep_order": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_chunk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "gs_set_pagerank_heavy": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -136,6 +137,12 @@ _SIGS = {
     "gs_server_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64)]),
     "gs_server_destroy": (None, [_P]),
+    "gs_server_memory": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "gs_server_bfs_policy": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]),
+    "gs_server_set_batch_min": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    "gs_server_last_error": (ctypes.c_size_t, [_P, ctypes.c_char_p, ctypes.c_size_t]),
     "gs_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 
@@ -329,6 +336,10 @@ class Graph:
     def set_sweep_order(self, xcd_contiguous: bool) -> None:
         _check(_lib.gs_set_sweep_order(self._h, int(xcd_contiguous)))
 
+    def debug_fail_alloc(self, nth: int) -> None:
+        """Test hook: the nth device allocation of the next index build fails."""
+        _check(_lib.gs_debug_fail_alloc(self._h, nth))
+
     def set_kernel_timing(self, on: bool) -> None:
         _check(_lib.gs_set_kernel_timing(self._h, int(on)))
 
@@ -428,6 +439,28 @@ class Server:
         h, p, i = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
         _check(_lib.gs_server_info(self._h, ctypes.byref(h), ctypes.byref(p), ctypes.byref(i)))
         return {"held_bytes": h.value, "pending": p.value, "inflight": i.value}
+
+    def memory(self) -> dict:
+        """Workspace bytes held by queries (bounded by the HBM budget) and the graphs' own bytes."""
+        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(_lib.gs_server_memory(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return {"workspace_bytes": a.value, "graph_bytes": b.value}
+
+    def bfs_policy(self, graph: int) -> dict:
+        """Batching rule measured when the graph was added."""
+        k = ctypes.c_int(0)
+        a, b, c = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+        _check(_lib.gs_server_bfs_policy(self._h, graph, ctypes.byref(k), ctypes.byref(a), ctypes.byref(b),
+                                         ctypes.byref(c)))
+        return {"batch_min": k.value, "single_ms": a.value, "batch1_ms": b.value, "batch64_ms": c.value}
+
+    def set_batch_min(self, graph: int, batch_min: int) -> None:
+        _check(_lib.gs_server_set_batch_min(self._h, graph, batch_min))
+
+    def last_error(self) -> str:
+        buf = ctypes.create_string_buffer(1024)
+        _lib.gs_server_last_error(self._h, buf, 1024)
+        return buf.value.decode(errors="replace")
 
     def close(self) -> None:
         if self._h:

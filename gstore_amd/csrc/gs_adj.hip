@@ -882,20 +882,69 @@ __global__ void k_pull_chunk_map(const uint64_t *__restrict__ offz, uint64_t nzn
 
 static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
 
+// Frees every array of the index; a failed build leaves none of it behind.
+static void release_adjacency(gs_graph *g) {
+    g->adj_ready = false;
+    g->new_id.release();
+    g->old_id.release();
+    g->deg_r.release();
+    g->adj_off.release();
+    g->adj_nbr.release();
+    g->adj_first.release();
+    g->chunkz.release();
+    g->nzn = g->npull = g->ms_nhub = 0;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+static int build_adjacency_locked(gs_graph *g);
+
+// Every device allocation of the index build goes through here, so a test can
+// make the k-th one fail (gs_debug_fail_alloc) and check the clean-up.
+template <class T>
+static int index_alloc(gs_graph *g, DevBuf<T> &b, size_t count) {
+    if (g->fail_alloc_at > 0 && --g->fail_alloc_at == 0) {
+        set_error("device allocation %zu bytes failed (injected by gs_debug_fail_alloc)", count * sizeof(T));
+        return GS_ERR_NOMEM;
+    }
+    return b.alloc(count);
+}
+
+// The index counts as built only once every step has finished on the device
+// (adj_ready); any failure on the way (an allocation, the degree-sum check, a
+// launch) releases what was allocated, so the next call builds again.
 int build_adjacency(gs_graph *g) {
     std::lock_guard<std::mutex> lk(g->build_mu);
-    if (g->adj_nbr.ptr && g->adj_off.n == g->n + 1) return GS_OK;
+    if (g->adj_ready) return GS_OK;
     if (g->directed) { set_error("the adjacency index is built for undirected grids"); return GS_ERR_UNSUPPORTED; }
+    const int rc = build_adjacency_locked(g);
+    if (rc != GS_OK) {
+        (void)hipStreamSynchronize(g->ws->stream);   // no kernel may still write the arrays being freed
+        release_adjacency(g);
+        return rc;
+    }
+    g->adj_ready = true;
+    return GS_OK;
+}
+
+static int build_adjacency_locked(gs_graph *g) {
     gs_ws *w = g->ws;
     hipStream_t st = w->stream;
     const uint64_t n = g->n;
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev;
+    GS_HIP(hipEventCreate(&ev.a));
+    GS_HIP(hipEventCreate(&ev.b));
+    hipEvent_t t0 = ev.a, t1 = ev.b;
     GS_HIP(hipEventRecord(t0, st));
     // 1. relabel: histogram of degree buckets, descending bucket order
     DevBuf<unsigned long long> cur64;
-    GS_TRY(cur64.alloc(NBUCKET));
+    GS_TRY(index_alloc(g, cur64, NBUCKET));
     GS_HIP(hipMemsetAsync(cur64.ptr, 0, NBUCKET * 8, st));
     k_bucket_hist<<<vgrid(n), 256, 0, st>>>(g->degree.ptr, n, cur64.ptr);
     GS_HIP(hipGetLastError());
@@ -910,20 +959,20 @@ int build_adjacency(gs_graph *g) {
     // ids of the buckets that hold degrees 65 .. MID_LIST (h[b] = first id of bucket b)
     const uint64_t mid_lo = h[deg_bucket(MID_LIST)], mid_hi = h[deg_bucket(65) - 1];
     GS_HIP(hipMemcpyAsync(cur64.ptr, h.data(), NBUCKET * 8, hipMemcpyHostToDevice, st));
-    GS_TRY(g->new_id.alloc(n));
-    GS_TRY(g->old_id.alloc(n));
+    GS_TRY(index_alloc(g, g->new_id, n));
+    GS_TRY(index_alloc(g, g->old_id, n));
     k_relabel<<<(unsigned)((n + 4095) / 4096), 256, 0, st>>>(g->degree.ptr, n, cur64.ptr, g->new_id.ptr,
                                                              g->old_id.ptr);
     GS_HIP(hipGetLastError());
-    GS_TRY(g->deg_r.alloc(n));
+    GS_TRY(index_alloc(g, g->deg_r, n));
     k_gather_deg<<<vgrid(n), 256, 0, st>>>(g->degree.ptr, g->old_id.ptr, n, g->deg_r.ptr);
     GS_HIP(hipGetLastError());
     // 2. offsets = exclusive scan of the relabelled degrees
     const uint64_t nb = (n + 1023) / 1024;
     DevBuf<unsigned long long> bsum;
-    GS_TRY(bsum.alloc(nb));
-    GS_TRY(g->adj_off.alloc(n + 1));
-    GS_TRY(g->adj_nbr.alloc(std::max<uint64_t>(2 * g->E, 1)));
+    GS_TRY(index_alloc(g, bsum, nb));
+    GS_TRY(index_alloc(g, g->adj_off, n + 1));
+    GS_TRY(index_alloc(g, g->adj_nbr, std::max<uint64_t>(2 * g->E, 1)));
     k_deg_block_sums<<<(unsigned)nb, 256, 0, st>>>(g->deg_r.ptr, n, bsum.ptr);
     GS_HIP(hipGetLastError());
     std::vector<unsigned long long> hb(nb);
@@ -940,7 +989,7 @@ int build_adjacency(gs_graph *g) {
     GS_HIP(hipGetLastError());
     // 3. neighbour lists, relabelled, by one sweep over the grid
     DevBuf<uint32_t> cur;
-    GS_TRY(cur.alloc(n));
+    GS_TRY(index_alloc(g, cur, n));
     GS_HIP(hipMemsetAsync(cur.ptr, 0, n * 4, st));
     GS_TRY(sweep(g, w, GS_K_CONVERT, [&](const SweepArgs &a, unsigned grid, hipStream_t s2) {
         k_adj_scatter<<<grid, SWEEP_WG, 0, s2>>>(a, g->new_id.ptr, g->adj_off.ptr, cur.ptr, g->adj_nbr.ptr);
@@ -954,14 +1003,14 @@ int build_adjacency(gs_graph *g) {
                     g->adj_off.ptr, mid_lo, mid_hi, g->adj_nbr.ptr);
             GS_HIP(hipGetLastError());
         }
-        GS_TRY(g->adj_first.alloc(g->nzn));
+        GS_TRY(index_alloc(g, g->adj_first, g->nzn));
         k_first_entry<<<vgrid(g->nzn), 256, 0, st>>>(g->adj_off.ptr, g->adj_nbr.ptr, g->nzn, g->adj_first.ptr);
         GS_HIP(hipGetLastError());
     }
     // 4. pull chunk map over the vertices with edges (relabelled ids 0 .. nzn-1)
     const uint64_t total = 2 * g->E;
     g->npull = (total + PULL_CHUNK - 1) / PULL_CHUNK;
-    GS_TRY(g->chunkz.alloc(g->npull + 1));
+    GS_TRY(index_alloc(g, g->chunkz, g->npull + 1));
     if (g->nzn) {
         k_pull_chunk_map<<<vgrid(g->nzn), 256, 0, st>>>(g->adj_off.ptr, g->nzn, g->chunkz.ptr);
         GS_HIP(hipGetLastError());
@@ -972,8 +1021,6 @@ int build_adjacency(gs_graph *g) {
     GS_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     g->index_ms = ms;
     w->timer.collect();
     w->timer.reset();
@@ -1165,6 +1212,14 @@ static double wall_clock_khz() {
             khz = 100000.0;   // the documented 100 MHz of gfx9 parts
     }
     return khz;
+}
+
+uint64_t gs::do_bfs_bytes(const gs_graph *g) {
+    const uint64_t n = g->n;
+    return 3 * 4 * bitmap_words(g) + 4 * ((g->p_s + 31) / 32) +  // bitmaps and tile-row bits (ensure_bitmaps)
+           2 * n +                                               // depth, depth_r
+           4 * n + 4 * n + 8 * (n + 1) + 8 * (n / 1024 + 1) +   // queue, qdeg, qoff, qsum (ensure_queue)
+           8 * (uint64_t)DO_WORDS;                               // control block
 }
 
 int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,

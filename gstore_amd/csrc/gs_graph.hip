@@ -23,6 +23,8 @@ void set_error(const char *fmt, ...) {
     g_err = buf;
 }
 
+const char *error_text() { return g_err.c_str(); }
+
 // ---- kernel timer ------------------------------------------------------
 void KernelTimer::begin(int k, hipStream_t s, hipEvent_t *out) {
     *out = nullptr;
@@ -714,7 +716,7 @@ int gs_get_info(gs_graph *g, gs_info *info) {
                          g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() + g->adj_first.bytes() +
                          g->new_id.bytes() + g->old_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() + gs::pb_bytes(g) +
                          (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
-    const bool built = g->adj_nbr.ptr && g->adj_off.n == g->n + 1;
+    const bool built = g->adj_ready;
     info->index_entries = built ? 2 * g->E : 0;
     info->index_vertices = built ? g->nzn : 0;
     info->index_ms = built ? g->index_ms + g->pb_ms : 0.0;
@@ -839,6 +841,13 @@ int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gath
     g->pb_dense = dense_entries;
     g->pb_compact = compact_permille;
     g->pb_ready = false;  // tables rebuilt at the next PageRank
+    return GS_OK;
+}
+
+int gs_debug_fail_alloc(gs_graph *g, int nth) {
+    if (!g || nth < 0) { set_error("bad argument"); return GS_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(g->build_mu);
+    g->fail_alloc_at = nth;
     return GS_OK;
 }
 

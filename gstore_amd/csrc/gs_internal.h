@@ -46,6 +46,7 @@ struct PbUnit2 { uint64_t e0, e1; uint32_t q, slot, dense, pad; };  // accum: sl
 struct PbRed { uint32_t q, slot0, nslot, pad; };                    // split block: its partial slots
 
 void set_error(const char *fmt, ...);
+const char *error_text();  // message of the last failure on this thread
 
 #define GS_HIP(call)                                                              \
     do {                                                                          \
@@ -214,6 +215,8 @@ struct gs_graph {
     uint64_t ms_nhub = 0;                    // ids 0 .. ms_nhub-1 have degree >= MS_HEAVY_DEG
     double index_ms = 0;                     // device time of the index build
     gs::DevBuf<uint32_t> chunkz;             // vertex whose list holds entry c*PULL_CHUNK, npull + 1
+    bool adj_ready = false;                  // every index array above built and complete
+    int fail_alloc_at = 0;                   // test hook: the k-th allocation of the next index build fails
     int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy
     int wcc_method = 1;                      // 0 tile hooking passes, 1 sampled linking over the lists
     int kcore_method = 1;                    // 0 tile scans, 1 peeled lists pushed over the index
@@ -266,6 +269,7 @@ int run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint
             int max_levels, int *levels);
 int run_bfs_do(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
                int max_levels, int *levels);
+uint64_t do_bfs_bytes(const gs_graph *g);  // workspace bytes run_bfs_do allocates
 int build_adjacency(gs_graph *g);
 int pr_pull(const gs_graph *g, gs_ws *w, const float *prior, float *acc, double *acc64);
 int build_blocked(gs_graph *g);

@@ -175,7 +175,9 @@ int gs_kcore(gs_graph *g, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t
  * query; a BFS takes every pending BFS of its graph along (up to max_batch,
  * default 64) into one gs_bfs_batch sweep.  A query starts only when the
  * workspace memory it needs fits hbm_budget_bytes (0 = unlimited) beside the
- * memory other workspaces hold; idle workspaces are freed to make room.
+ * memory other workspaces hold; idle workspaces are freed to make room.  A BFS
+ * that finds fewer waiting BFS of its graph than the graph's batch_min runs
+ * alone on the single-root direction-optimizing path instead.
  * Results: `out` (optional, caller-owned, must stay valid until the query
  * completes) receives the BFS depth (N bytes), the PPR / PageRank ranks or the
  * WCC labels (4N bytes); the result record carries latency stamps. */
@@ -195,7 +197,9 @@ typedef struct gs_server gs_server;
 
 int gs_server_create(uint64_t hbm_budget_bytes, int workers, gs_server **out);
 /* The server does not own the graph; keep it open until gs_server_destroy.
- * Builds the graph's neighbour-list index (undirected grids) before returning. */
+ * On undirected grids it builds the graph's neighbour-list index and blocked
+ * PageRank copy, and measures the BFS batching rule (gs_server_bfs_policy),
+ * before returning. */
 int gs_server_add_graph(gs_server *s, gs_graph *g, int *graph_id);
 /* max_batch 1..64 BFS lanes; iterations of PPR and PageRank queries (default 20). */
 int gs_server_set(gs_server *s, int max_batch, int ppr_iters, int pr_iters);
@@ -205,6 +209,20 @@ int gs_server_poll(gs_server *s, gs_query_result *out, int max, int wait_ms, int
 /* Block until nothing is pending or running. */
 int gs_server_drain(gs_server *s);
 int gs_server_info(gs_server *s, uint64_t *held_bytes, uint64_t *pending, uint64_t *inflight);
+/* Device memory: workspace bytes the queries hold (what hbm_budget_bytes
+ * bounds) and the graphs' own bytes (grid, index, blocked copy). */
+int gs_server_memory(gs_server *s, uint64_t *workspace_bytes, uint64_t *graph_bytes);
+/* The BFS batching rule of one graph, measured when it was added: a batch
+ * forms once batch_min BFS of the graph are waiting (65: never); fewer run one
+ * at a time on the single-root direction-optimizing path.  Also returned: the
+ * measured ms of one single-root BFS and of batches of 1 and 64 lanes. */
+int gs_server_bfs_policy(gs_server *s, int graph, int *batch_min, double *single_ms, double *batch1_ms,
+                         double *batch64_ms);
+/* Override batch_min (1 = always batch, 65 = never). */
+int gs_server_set_batch_min(gs_server *s, int graph, int batch_min);
+/* Message of the last query that failed inside a worker (status in its
+ * result record); returns its length. */
+size_t gs_server_last_error(gs_server *s, char *buf, size_t len);
 void gs_server_destroy(gs_server *s);
 
 /* BFS method for gs_bfs: 0 (default) sweeps the active tiles of the grid each
@@ -339,6 +357,11 @@ int gs_get_level_stats(gs_graph *g, gs_level_stat *out, int max, int *n);
 /* 1: record per-kernel-class device time with HIP events (adds a little
  * launch overhead); 0: off (default). */
 int gs_set_kernel_timing(gs_graph *g, int on);
+
+/* Test hook (no reference counterpart): the nth device allocation of the
+ * next neighbour-list index build fails with GS_ERR_NOMEM (0 = off).  A failed
+ * build frees every index array it allocated; the next call builds again. */
+int gs_debug_fail_alloc(gs_graph *g, int nth);
 
 /* Name of a gs_kernel class. */
 const char *gs_kernel_name(int k);

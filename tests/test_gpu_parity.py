@@ -127,6 +127,17 @@ def test_pagerank_intended_within_tolerance(loaded, name):
     err = float(np.max(np.abs(got.astype(np.float64) - want)))
     _say("  max-abs vs oracle", err)
     assert err <= PR_TOL
+    # against the FP64 recurrence: absolute (the spec), relative on every
+    # non-zero value (ranks start at 1/N, so a 1e-6 absolute bound alone would
+    # let a small rank be off by tens of percent) and L1, reported
+    w64 = oracle.pagerank64(n, start, grid, deg, 20)
+    d = np.abs(got.astype(np.float64) - w64)
+    nz = w64 > 0
+    rel = float(np.max(d[nz] / w64[nz])) if nz.any() else 0.0
+    _say(f"  vs FP64: max-abs {d.max():.3g}, max-rel {rel:.3g}, L1 {d.sum():.3g}, L1/N {d.sum() / n:.3g}")
+    assert d.max() <= PR_TOL
+    assert rel <= 1e-4
+    assert np.all(got[~nz] == 0)
 
 
 @pytest.mark.parametrize("name", IDS)
@@ -197,6 +208,29 @@ def test_directed_parity(name):
 
 
 # ---- error paths -------------------------------------------------------------------
+
+@pytest.mark.parametrize("nth", [1, 6, 8, 10])
+def test_index_build_failure_releases_and_rebuilds(nth):
+    """An allocation failure part way through the index build (the 1st, 6th,
+    8th or last of its 10 allocations) returns GS_ERR_NOMEM, leaves no index
+    behind, and the next call builds it again and answers correctly."""
+    n, e = graphs.kron(14)
+    start, grid, deg = oracle.convert(e, n)
+    with ga.Graph.from_arrays(n, start, grid, deg) as g:
+        base = g.index_info()["device_bytes"]
+        g.debug_fail_alloc(nth)
+        with pytest.raises(ga.GStoreError) as ei:
+            g.set_bfs_mode(1)          # builds the index
+        assert ei.value.status == 4
+        info = g.index_info()
+        assert info["entries"] == 0 and info["device_bytes"] <= base + 64 * n
+        want, lc = oracle.bfs(n, start, grid, 0)
+        got = g.bfs(0)
+        assert np.array_equal(got.depth, want) and got.level_counts == lc
+        assert g.index_info()["entries"] == 2 * g.E
+        lab, nc = g.wcc()
+        assert np.array_equal(lab, oracle.wcc(n, start, grid)[0])
+
 
 def test_errors(tmp_path):
     _say("errors")
