@@ -12,6 +12,8 @@
 // with one atomicOr per (vertex word) and their depth bytes are written once
 // by the level-end pass, which also rebuilds the tile-row activity bits and
 // counts the new frontier exactly.
+#include <chrono>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -194,6 +196,131 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     if (depth_out) {
         GS_HIP(hipMemcpyAsync(depth_out, w->depth.ptr, g->n, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
+    }
+    return GS_OK;
+}
+
+// ---- BFS that allocates nothing ------------------------------------------------
+// The same tile-sweep levels with every work array -- level counters, tile-row
+// activity bits, the three bitmaps and the depth bytes -- carved out of one
+// file-scope device array instead of the workspace.  Each GPU holds its own copy
+// of the array for the life of the process; its device address is looked up
+// once per device with hipGetSymbolAddress (the array's name in host code is
+// only a stand-in and must never reach a kernel).  Calls on one GPU share the
+// array, so calls that overlap on one GPU race; each call clears what it uses.
+namespace gs {
+
+constexpr uint64_t NOALLOC_BYTES = 32ull << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex scratch_mu;
+static unsigned char *scratch_dev[64];   // per device id: scratch's device address
+
+static int scratch_base(unsigned char **out) {
+    int dev = 0;
+    GS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) { set_error("device id %d out of range", dev); return GS_ERR_DEVICE; }
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+            return GS_ERR_DEVICE;
+        }
+        scratch_dev[dev] = (unsigned char *)p;
+    }
+    *out = scratch_dev[dev];
+    return GS_OK;
+}
+
+__global__ void k_zero16(uint4 *p, uint64_t n16) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ void k_level_reset(unsigned long long *c) {
+    c[0] = 0;  // vertices reached this level
+    c[3] = 0;  // tile rows holding them
+}
+
+static uint64_t up256(uint64_t b) { return (b + 255) & ~255ull; }
+
+// Scratch bytes the no-allocation BFS needs on graph g.
+static uint64_t noalloc_bytes(const gs_graph *g) {
+    return 256 + up256(4 * ((g->p_s + 31) / 32)) + 3 * up256(4 * bitmap_words(g)) + up256(g->n);
+}
+
+}  // namespace gs
+
+extern "C" int gs_bfs_noalloc(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
+                              int max_levels, int *levels) {
+    if (!g || root >= g->n || max_levels < 0) {
+        set_error("gs_bfs_noalloc: bad arguments");
+        return GS_ERR_ARG;
+    }
+    if (g->ss) {
+        set_error("gs_bfs_noalloc: the grid is in semi-external mode, whose sweeps stream through segments");
+        return GS_ERR_UNSUPPORTED;
+    }
+    const uint64_t need = noalloc_bytes(g);
+    if (need > NOALLOC_BYTES) {
+        set_error("gs_bfs_noalloc: N = %llu needs %llu bytes of work space, beyond the %llu-byte static array "
+                  "(about 24 M vertices at most)", (unsigned long long)g->n, (unsigned long long)need,
+                  (unsigned long long)NOALLOC_BYTES);
+        return GS_ERR_ARG;
+    }
+    unsigned char *base = nullptr;
+    GS_TRY(scratch_base(&base));
+    gs_ws *w = g->ws;
+    hipStream_t st = w->stream;
+    const uint64_t words = bitmap_words(g), raw = (g->p_s + 31) / 32;
+    unsigned long long *counters = (unsigned long long *)base;
+    uint32_t *rowact = (uint32_t *)(base + 256);
+    uint32_t *vis = (uint32_t *)((unsigned char *)rowact + up256(4 * raw));
+    uint32_t *front = (uint32_t *)((unsigned char *)vis + up256(4 * words));
+    uint32_t *next = (uint32_t *)((unsigned char *)front + up256(4 * words));
+    uint8_t *depth = (uint8_t *)next + up256(4 * words);
+    const auto t0 = std::chrono::steady_clock::now();
+    k_zero16<<<(unsigned)std::min<uint64_t>((need / 16 + 255) / 256, 4096), 256, 0, st>>>((uint4 *)base, need / 16);
+    GS_HIP(hipGetLastError());
+    k_bfs_root<<<1, 1, 0, st>>>((uint32_t)root, depth, vis, front, rowact);
+    GS_HIP(hipGetLastError());
+    int nlev = 0, status = GS_OK;
+    const unsigned grid = (unsigned)(g->xcd_order ? xcd_grid(g->nchunk) : g->nchunk);
+    for (uint32_t level = 1;; ++level) {
+        k_level_reset<<<1, 1, 0, st>>>(counters);
+        if (g->nchunk) {
+            const SweepArgs a = sweep_args(g, g->grid.ptr, 0, 0, g->nchunk);
+            if (g->directed)
+                k_bfs_expand<true><<<grid, SWEEP_WG, 0, st>>>(a, front, vis, next, rowact, counters + 1);
+            else
+                k_bfs_expand<false><<<grid, SWEEP_WG, 0, st>>>(a, front, vis, next, rowact, counters + 1);
+        }
+        k_bfs_level_end<<<(unsigned)g->rows, 256, 0, st>>>(vis, front, next, rowact, depth, level + 1, counters);
+        GS_HIP(hipGetLastError());
+        GS_HIP(hipMemcpyAsync(w->h_counters, counters, 32, hipMemcpyDeviceToHost, st));
+        GS_HIP(hipStreamSynchronize(st));
+        const uint64_t cnt = w->h_counters[0];
+        if (level_counts && nlev < max_levels) level_counts[nlev] = cnt;
+        ++nlev;
+        if (cnt == 0) break;
+        if (level + 1 > 255) {
+            set_error("BFS from %llu needs depth > 255, beyond the u8 depth encoding", (unsigned long long)root);
+            status = GS_ERR_DEPTH_OVERFLOW;
+            break;
+        }
+    }
+    w->stats = gs_stats{};
+    w->stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    w->stats.iterations = (uint64_t)nlev;
+    w->stats.edges_scanned = w->h_counters[1];
+    w->stats.tiles_scanned = w->h_counters[2];
+    if (levels) *levels = nlev;
+    if (status != GS_OK) return status;
+    if (depth_out) {
+        GS_HIP(hipMemcpyAsync(depth_out, depth, g->n, hipMemcpyDeviceToHost, st));
+        GS_HIP(hipStreamSynchronize(st));
     }
     return GS_OK;
 }

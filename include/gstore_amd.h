@@ -146,6 +146,19 @@ int gs_ppr(gs_graph *g, uint64_t source, int iters, float *rank_out);
 int gs_bfs(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
            int max_levels, int *levels);
 
+/* BFS that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture
+ * setup, tight serving loops, harnesses that forbid hipMalloc).  Same results
+ * and encodings as gs_bfs with the tile-sweep method; its work arrays (level
+ * counters, tile-row bits, three N-bit bitmaps, N depth bytes: about 1.375 N
+ * bytes) live in one static 32 MiB device array per GPU, so it serves graphs
+ * of up to about 24 M vertices and returns GS_ERR_ARG beyond that, and
+ * GS_ERR_UNSUPPORTED on semi-external graphs.  It calls no hipMalloc/hipFree.
+ * Every call on one GPU shares that array: calls that overlap on one GPU race
+ * (each call clears what it uses before it starts). */
+int gs_bfs_noalloc(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *level_counts,
+                   int max_levels, int *levels);
+
 /* Batched BFS: up to 64 roots run as bit lanes of one multi-source sweep
  * (no reference counterpart; each lane applies the rule of bfs.cpp:158-177).
  * depth_out: n_roots x N bytes, lane i at depth_out + i*N, same encoding as
