@@ -9,12 +9,22 @@ Workload (one "step" = one job of each algorithm on a resident graph):
     BFS           from 16 seeded random roots with degree > 0 (-j 20)
     WCC           (-j 23)
     k-core        k = 8 (-j 22 -a 8)
-value = aggregate GTEPS over the step, using per algorithm: PageRank 2E per
-iteration; BFS Graph500 M_reach per root (stored edges in the reached
-component); WCC and k-core 2E per pass / round.  Each algorithm also reports
-its fraction of HBM peak from fixed algorithmic-byte formulas (independent of
-the GPU method; BFS and k-core scans counted with the reference's tile-skip
-rule), and the roofline entry is one PageRank iteration.
+value = aggregate GTEPS over the step.  Traversed edges per job, independent
+of the GPU method: PageRank 2E per iteration; BFS Graph500 M_reach per root
+(stored edges with a reached endpoint); WCC 2E per run (every edge once);
+k-core 2 x the stored edges the reference's tile sweeps scan (tiles whose row
+or column holds a removed vertex, counted by the tile method in an untimed
+pre-pass).
+
+Per algorithm the line also gives: ref_formula_gbs, the fixed algorithmic
+bytes of the reference's tile sweeps per unit of time (a rate, not a fraction:
+methods that skip work the tile sweep does exceed the HBM peak on it); and,
+where a rocprofv3 --pmc record for this workload is kept in
+profiles/traffic.json and its kernel time is within 30% of this run's,
+traffic_bytes and traffic_frac = measured HBM bytes / time / 8 TB/s.  The
+roofline entry is one PageRank iteration.  A `serving` entry measures batched
+BFS serving on the same graph, and `cpu_baseline` times the oracle's threaded
+forms of all four jobs on the host.
 
 Prints one JSON line (stdout).  Progress goes to stderr.
 """
@@ -57,43 +67,79 @@ def alg_bytes_kcore(n, e_scan, t_scan, rounds):
     return 4 * e_scan + 8 * t_scan + rounds * (4 * n + n / 8)
 
 
-def measured_traffic(workload, method, iter_ms):
-    """HBM-side bytes of one PageRank iteration measured with rocprofv3 --pmc
-    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration), kept in
-    profiles/traffic.json with the iteration time of that run.  Used only when
-    this run's iteration time is within 30% of it (same kernels)."""
+def measured_traffic(workload, key, live_ms):
+    """HBM-side bytes of one unit of work (a PageRank iteration, one BFS root,
+    one WCC or k-core run) measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by the copy calibration) and kept in
+    profiles/traffic.json with the kernel time of that unit in an unprofiled
+    run.  Returned only when this run's time is within 30% of it (the same
+    kernels doing the same work)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f).get(workload, {}).get(method)
+            rec = json.load(f).get(workload, {}).get(key)
     except (OSError, ValueError):
         return None
-    if not rec or abs(iter_ms - rec["iteration_ms"]) > 0.3 * rec["iteration_ms"]:
+    ref_ms = rec and rec.get("ms", rec.get("iteration_ms"))
+    if not rec or not ref_ms or abs(live_ms - ref_ms) > 0.3 * ref_ms:
         return None
-    return {"bytes": int(rec["fetch_bytes"] + rec["write_bytes"]), "source": rec["source"]}
+    return {"bytes": int(rec["fetch_bytes"] + rec["write_bytes"]), "source": rec["source"], "rec": rec}
 
 
 # kernel classes of one PageRank iteration, per method
 PR_ITER_KERNELS = ("pb_gather", "pb_accum", "pb_reduce", "pr_pull", "pr_scatter", "pr_finalize")
 
 
-def cpu_baseline(scale, threads):
+def host_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when set (the GPU boxes give each GPU a share of
+    the host and set it)."""
+    visible = len(os.sched_getaffinity(0))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(visible, cap) if cap > 0 else visible), visible
+
+
+def cpu_baseline(scale, threads, visible):
+    """The oracle's threaded forms (tile sweeps with atomics, the reference's
+    OpenMP scheme) of the four jobs on Kron-<scale>, with the GPU line's
+    traversed-edge conventions: PageRank 5 iterations, BFS from 2 of the
+    bench's roots, WCC, k-core k = 8."""
     import oracle
     t0 = time.time()
     n = 1 << scale
     edges = oracle.kron(scale, 0, 16 << scale)
     start, grid, degree = oracle.convert(edges, n)
     del edges
-    log(f"cpu baseline graph s{scale} ready in {time.time() - t0:.1f}s, E={grid.size}")
-    iters = 10   # about 10 s of PageRank on the host at the default scale
+    E = grid.size
+    log(f"cpu baseline graph s{scale} ready in {time.time() - t0:.1f}s, E={E}")
+    roots = [int(r) for r in np.random.default_rng(20240601).choice(np.flatnonzero(degree), 2, replace=False)]
+    deg64 = degree.astype(np.int64)
+    per = {}
+    iters = 5
     t = time.perf_counter()
     oracle.pagerank(n, start, grid, degree, iters, mode=1, nthreads=threads)
-    dt = time.perf_counter() - t
-    return {"value": round(2 * grid.size * iters / dt / 1e9, 4), "unit": "GTEPS", "cores": threads,
-            "kind": "port",
-            "sample": f"PageRank {iters} iterations (float degree factor) on Kron-{scale} "
-                      f"(E={grid.size}), C/OpenMP restatement with CAS float atomics like pr.cpp:60-80, "
-                      f"{dt:.2f} s"}
+    per["pagerank"] = (time.perf_counter() - t, 2 * E * iters)
+    t = time.perf_counter()
+    m = 0
+    for r in roots:
+        d, _ = oracle.bfs(n, start, grid, r, nthreads=threads)
+        m += int(deg64[d > 0].sum() // 2)
+    per["bfs"] = (time.perf_counter() - t, m)
+    t = time.perf_counter()
+    oracle.wcc(n, start, grid, nthreads=threads)
+    per["wcc"] = (time.perf_counter() - t, 2 * E)
+    scanned = []
+    t = time.perf_counter()
+    oracle.kcore(n, start, grid, degree, 8, nthreads=threads, scanned=scanned)
+    per["kcore"] = (time.perf_counter() - t, 2 * scanned[0])
+    tot_s = sum(v[0] for v in per.values())
+    tot_e = sum(v[1] for v in per.values())
+    return {"value": round(tot_e / tot_s / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "host_cpus_visible": visible,
+            "per_algo": {k: {"s": round(v[0], 2), "gteps": round(v[1] / v[0] / 1e9, 4)} for k, v in per.items()},
+            "sample": f"Kron-{scale} (E={E}): PageRank {iters} iterations (float degree factor), BFS from "
+                      f"{len(roots)} roots, WCC, k-core k=8; the oracle's C/OpenMP threaded forms (tile sweeps "
+                      f"with atomics like pr.cpp:60-80, kcore.cpp:69-103), {tot_s:.1f} s on {threads} threads"}
 
 
 def pct(xs, q):
@@ -103,11 +149,89 @@ def pct(xs, q):
     return round(xs[min(len(xs) - 1, int(q * len(xs)))], 3)
 
 
+def bfs_saturation(ga, g, roots, workers, tile_ms_per_root=None, do_ms_per_root=None):
+    """BFS throughput through the server on one graph: the same roots
+    (submitted together) one root at a time on the single-root path
+    (max_batch 1, one worker), in forced 64-lane batches, and under the
+    server's own measured batching rule; beside them the direct single-root
+    BFS per launch, tile sweep (the reference's method) and
+    direction-optimizing (device time per root; measured here unless given)."""
+    K = len(roots)
+    sat = {"queries": K}
+    for label, max_batch, nw, force in (("server_single_root", 1, 1, False),
+                                        ("server_batched_64", 64, workers, True),
+                                        ("server_auto", 64, workers, False)):
+        with ga.Server(workers=nw) as srv:
+            gid = srv.add_graph(g)
+            srv.configure(max_batch=max_batch)
+            if label == "server_auto":
+                sat["policy"] = {k: round(v, 3) if isinstance(v, float) else v for k, v in srv.bfs_policy(gid).items()}
+            if force:
+                srv.set_batch_min(gid, 1)
+            srv.submit(gid, ga.Q_BFS, int(roots[0]))     # warm the workspace
+            srv.drain()
+            srv.poll()
+            t = time.perf_counter()
+            for r in roots:
+                srv.submit(gid, ga.Q_BFS, int(r))
+            srv.drain()
+            dt = time.perf_counter() - t
+            res = srv.poll(K + 10)
+        lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
+        sat[label] = {"qps": round(K / dt, 2), "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                      "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 2)}
+    nd = min(K, 16)
+    for label, mode, given in (("tile_single_root", 0, tile_ms_per_root), ("do_single_root", 1, do_ms_per_root)):
+        if given is None:
+            g.set_bfs_mode(mode)
+            g.bfs(int(roots[0]), copy_out=False)
+            ms = 0.0
+            for r in roots[:nd]:
+                g.bfs(int(r), copy_out=False)
+                ms += g.stats()["ms"]
+            given = ms / nd
+        sat[label] = {"qps": round(1e3 / given, 2), "ms_per_root": round(given, 3)}
+    b = sat["server_batched_64"]["qps"]
+    best_single = max(sat["do_single_root"]["qps"], sat["server_single_root"]["qps"])
+    sat["batched_vs_best_single_root"] = round(b / best_single, 2)
+    sat["batched_vs_tile_single_root"] = round(b / sat["tile_single_root"]["qps"], 2)
+    sat["server_single_vs_do_single"] = round(sat["server_single_root"]["qps"] / sat["do_single_root"]["qps"], 3)
+    return sat
+
+
+def bfs_poisson(ga, g, pool, rate, seconds, workers, rng):
+    """BFS queries (uniform roots from `pool`) arriving as a Poisson process at
+    `rate` per second for `seconds`, served with the server's own batching
+    rule: achieved q/s and p50/p99 latency (submit to done)."""
+    n_q = max(10, int(rate * seconds))
+    gaps = rng.exponential(1.0 / rate, n_q)
+    roots = rng.choice(pool, n_q, replace=True)
+    with ga.Server(workers=workers) as srv:
+        gid = srv.add_graph(g)
+        srv.submit(gid, ga.Q_BFS, int(roots[0]))
+        srv.drain()
+        srv.poll()
+        t0 = time.perf_counter()
+        due = t0
+        for r, gap in zip(roots, gaps):
+            due += gap
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(gid, ga.Q_BFS, int(r))
+        srv.drain()
+        dt = time.perf_counter() - t0
+        res = srv.poll(n_q + 10)
+    lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
+    return {"offered_qps": rate, "queries": n_q, "achieved_qps": round(n_q / dt, 2), "p50_ms": pct(lat, 0.5),
+            "p99_ms": pct(lat, 0.99), "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 2)}
+
+
 def serve_mode(args):
     """Batched serving (Kronecker scales --serve-scales resident together): BFS
-    saturation throughput, batched vs one root per launch, then Poisson arrivals
-    of a 70/20/10 BFS/PPR/WCC mix at swept rates, p50/p99 latency per kind."""
-    import threading
+    saturation throughput per graph (batched, single root through the server,
+    direct single roots), then Poisson arrivals of a 70/20/10 BFS/PPR/WCC mix
+    at swept rates, p50/p99 latency per kind."""
     import gstore_amd as ga
     scales = [int(x) for x in args.serve_scales.split(",")]
     graphs = []
@@ -121,49 +245,15 @@ def serve_mode(args):
     out = {"metric": "batched BFS queries/s, p50/p99 latency (serving)", "unit": "queries/s", "n_gpus": 1,
            "higher_is_better": True, "data": "synthetic (Graph500 Kronecker, seeds 1,2)",
            "config": {"workload": "serving", "scales": scales, "workers": args.workers}}
-
-    # 1. BFS saturation per graph: the same K roots through the server, one root
-    #    per launch (max_batch 1) vs 64-lane batches; beside them the plain
-    #    single-root tile BFS (the reference's method, one root per launch) and the
-    #    direction-optimizing single-root BFS, timed directly on the device.
-    K = args.sat_queries
     sat_all = []
     for gi, g in enumerate(graphs):
-        roots = rng.choice(nz[gi], K, replace=True)
-        sat = {"graph_scale": scales[gi], "queries": K}
-        for label, max_batch, workers in (("server_one_root_per_launch", 1, 1), ("server_batched_64", 64, args.workers)):
-            with ga.Server(workers=workers) as srv:
-                gid = srv.add_graph(g)
-                srv.configure(max_batch=max_batch)
-                srv.submit(gid, ga.Q_BFS, int(roots[0]))     # warm the workspace
-                srv.drain()
-                srv.poll()
-                t = time.perf_counter()
-                for r in roots:
-                    srv.submit(gid, ga.Q_BFS, int(r))
-                srv.drain()
-                dt = time.perf_counter() - t
-                res = srv.poll(K + 10)
-                lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
-                sat[label] = {"qps": round(K / dt, 2), "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
-                              "mean_batch": round(float(np.mean([r["batch_size"] for r in res])), 2)}
-        nd = min(K, 16)
-        for label, mode in (("tile_single_root", 0), ("do_single_root", 1)):
-            g.set_bfs_mode(mode)
-            g.bfs(int(roots[0]), copy_out=False)
-            ms = 0.0
-            for r in roots[:nd]:
-                g.bfs(int(r), copy_out=False)
-                ms += g.stats()["ms"]
-            sat[label] = {"qps": round(nd / (ms / 1e3), 2), "ms_per_root": round(ms / nd, 3)}
+        sat = {"graph_scale": scales[gi], **bfs_saturation(ga, g, rng.choice(nz[gi], args.sat_queries, replace=True),
+                                                           args.workers)}
         g.set_bfs_mode(0)
-        b = sat["server_batched_64"]["qps"]
-        sat["batched_vs_tile_single"] = round(b / sat["tile_single_root"]["qps"], 2)
-        sat["batched_vs_do_single"] = round(b / sat["do_single_root"]["qps"], 2)
-        sat["batched_vs_server_single"] = round(b / sat["server_one_root_per_launch"]["qps"], 2)
         sat_all.append(sat)
-        log(f"serve: Kron-{scales[gi]} batched {b:.1f} q/s; x{sat['batched_vs_tile_single']} vs tile single root, "
-            f"x{sat['batched_vs_do_single']} vs direction-optimizing single root")
+        log(f"serve: Kron-{scales[gi]} batched {sat['server_batched_64']['qps']:.1f} q/s; "
+            f"x{sat['batched_vs_best_single_root']} vs the best single root, "
+            f"x{sat['batched_vs_tile_single_root']} vs the tile single root; policy {sat['policy']}")
     out["bfs_saturation"] = sat_all
     big = sat_all[-1]
 
@@ -206,7 +296,8 @@ def serve_mode(args):
         log(f"serve: rate {lam}/s -> {n_q / dt:.1f} q/s, p99 {mix[-1]['p99_ms']} ms")
     out["poisson_mix"] = mix
     out["value"] = big["server_batched_64"]["qps"]
-    out["vs_one_root_per_launch"] = big["batched_vs_tile_single"]
+    out["batched_vs_best_single_root"] = big["batched_vs_best_single_root"]
+    out["batched_vs_tile_single_root"] = big["batched_vs_tile_single_root"]
     print(json.dumps(out), flush=True)
     for g in graphs:
         g.close()
@@ -260,6 +351,96 @@ def stream_mode(args):
     g.close()
 
 
+def _pmc_dispatches(path):
+    """rocprofv3 --pmc counter CSV -> [(kernel name, value)] in dispatch order,
+    the counter's values summed per dispatch (one counter per run)."""
+    import csv
+    per = {}
+    with open(path, newline="") as f:
+        rd = csv.DictReader(f)
+        cols = {c.lower(): c for c in rd.fieldnames}
+        kd, kn, kv = cols["dispatch_id"], cols["kernel_name"], cols["counter_value"]
+        for row in rd:
+            d = int(row[kd])
+            name, v = per.get(d, (row[kn], 0.0))
+            per[d] = (name, v + float(row[kv]))
+    return [per[d] for d in sorted(per)]
+
+
+def _short(name):
+    base = name.split("(")[0].split("<")[0].split("::")[-1].strip()
+    return base.replace("void ", "")
+
+
+def traffic_mode(args):
+    """HBM traffic per algorithm from two rocprofv3 --pmc runs (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch) of one bench invocation, scaled by the
+    copy calibration (FETCH_SIZE reads half of a known copy's bytes: factor 2;
+    WRITE_SIZE: 1).  The bench's step order splits the dispatches: PageRank
+    iterations (the blocked kernels, per iteration), then BFS from the first
+    k_do_root (direction-optimizing; one level per k_do_begin), then WCC from
+    its first k_aff_/k_wcc_init kernel, then k-core from its first peel."""
+    fetch, write = _pmc_dispatches(args.pmc_fetch), _pmc_dispatches(args.pmc_write)
+    if [n for n, _ in fetch] != [n for n, _ in write]:
+        sys.exit("the two runs dispatched different kernel sequences")
+    ff, wf = args.fetch_factor, args.write_factor
+    rows = [(_short(n), fv * 1024 * ff, wv * 1024 * wf) for (n, fv), (_, wv) in zip(fetch, write)]
+    names = [r[0] for r in rows]
+    out = {"workload": f"kron{args.scale}", "fetch_factor": ff, "write_factor": wf, "dispatches": len(rows)}
+
+    def total(sel):
+        f = sum(rows[i][1] for i in sel)
+        w = sum(rows[i][2] for i in sel)
+        by = {}
+        for i in sel:
+            b = by.setdefault(rows[i][0], [0, 0.0, 0.0])
+            b[0] += 1
+            b[1] += rows[i][1]
+            b[2] += rows[i][2]
+        top = sorted(by.items(), key=lambda kv: -(kv[1][1] + kv[1][2]))[:8]
+        return {"fetch_bytes": int(f), "write_bytes": int(w),
+                "kernels": {k: {"calls": v[0], "fetch_bytes": int(v[1]), "write_bytes": int(v[2])} for k, v in top}}
+
+    pb = [i for i, n in enumerate(names) if n in ("k_pb_gather", "k_pb_accum", "k_pb_reduce")]
+    iters = names.count("k_pb_gather")
+    if iters:
+        t = total(pb)
+        out["pagerank"] = {"iterations": iters, "fetch_bytes": t["fetch_bytes"] // iters,
+                           "write_bytes": t["write_bytes"] // iters, "kernels": t["kernels"]}
+
+    def first(pred, lo=0):
+        return next((i for i in range(lo, len(names)) if pred(names[i])), None)
+
+    b0 = first(lambda n: n == "k_do_root")
+    w0 = first(lambda n: n.startswith("k_aff_") or n == "k_wcc_init", (b0 or 0))
+    k0 = first(lambda n: n in ("k_kc_peel_list", "k_kc_peel"), (w0 or b0 or 0))
+    if b0 is not None:
+        b1 = w0 if w0 is not None else (k0 if k0 is not None else len(rows))
+        t = total(range(b0, b1))
+        levels, cur = [], None
+        for i in range(b0, b1):
+            if names[i] == "k_do_begin":
+                cur = [i]
+                levels.append(cur)
+            elif cur is not None:
+                cur.append(i)
+                if names[i] == "k_do_end":   # a level closes with k_do_end; what follows the last
+                    cur = None               # one (unpermuting the depths) belongs to no level
+        lv = []
+        for li, sel in enumerate(levels):
+            tl = total(sel)
+            lv.append({"level": li + 1, "fetch_bytes": tl["fetch_bytes"], "write_bytes": tl["write_bytes"],
+                       "top_kernel": next(iter(tl["kernels"]), None)})
+        out["bfs"] = {"fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"], "kernels": t["kernels"],
+                      "levels": lv}
+    if w0 is not None:
+        t = total(range(w0, k0 if k0 is not None else len(rows)))
+        out["wcc"] = t
+    if k0 is not None:
+        out["kcore"] = total(range(k0, len(rows)))
+    print(json.dumps(out, indent=1), flush=True)
+
+
 def copy_mode(args):
     """Reference copy for counter calibration: 5 device-to-device copies of 1 GiB.
     Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE each dispatch should read and
@@ -278,9 +459,21 @@ def copy_mode(args):
                       "ms": round(dt * 1e3, 3), "gbs": round(2 * (1 << 30) / dt / 1e9, 1)}), flush=True)
 
 
+def level_method_bytes(n_act_words, lv):
+    """Bytes one direction-optimizing BFS level touches by the engine's own
+    counters: 4 per list entry examined (gs_level_stat.edges), 16 per frontier
+    vertex on a top-down level (queue entry, degree, two offsets), the three
+    bitmaps over the vertices with edges read and written once (12 bytes per
+    word) and one depth byte per vertex reached.  An estimate of the
+    algorithm's own bytes (random 4-byte reads priced at 4 bytes), beside the
+    counter-measured traffic."""
+    top = lv["direction"] == "top-down"
+    return 4 * lv["edges"] + (16 * lv["frontier"] if top else 0) + 12 * n_act_words + lv["reached"]
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["main", "serve", "stream", "copy"], default="main")
+    ap.add_argument("--mode", choices=["main", "serve", "stream", "copy", "traffic"], default="main")
     ap.add_argument("--serve-scales", default="18,20,22,24,26,28")
     ap.add_argument("--serve-rates", default="5,20,80")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
@@ -296,9 +489,13 @@ def main():
     ap.add_argument("--scale", type=int, default=28)
     ap.add_argument("--pr-iters", type=int, default=20)
     ap.add_argument("--bfs-roots", type=int, default=16)
+    ap.add_argument("--bfs-run-roots", type=int, default=0,
+                    help="run the step's BFS from the first K of the drawn roots only (profiling runs); 0 = all")
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--cpu-scale", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=300.0, help="offered BFS queries/s of the serving entry")
     ap.add_argument("--algos", default="pr,bfs,wcc,kcore",
                     help="subset of the step's jobs (profiling runs); the default is the full step")
     ap.add_argument("--bfs-mode", type=int, default=-1, help="-1: autotune; 0: tile sweep; 1: direction-optimizing")
@@ -306,6 +503,10 @@ def main():
     ap.add_argument("--pr-method", choices=["blocked", "pull", "scatter"], default="blocked",
                     help="PageRank: two streamed passes over the blocked index copy, pull over the "
                          "neighbour-list index, or the tile scatter with atomics")
+    ap.add_argument("--pmc-fetch", help="traffic mode: counter CSV of the FETCH_SIZE run")
+    ap.add_argument("--pmc-write", help="traffic mode: counter CSV of the WRITE_SIZE run")
+    ap.add_argument("--fetch-factor", type=float, default=2.0)
+    ap.add_argument("--write-factor", type=float, default=1.0)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -319,6 +520,8 @@ def main():
         return stream_mode(args)
     if args.mode == "copy":
         return copy_mode(args)
+    if args.mode == "traffic":
+        return traffic_mode(args)
 
     import torch
     import gstore_amd as ga
@@ -332,25 +535,30 @@ def main():
     deg = g.degree()
     rng = np.random.default_rng(20240601)
     roots = [int(r) for r in rng.choice(np.flatnonzero(deg), args.bfs_roots, replace=False)]
+    run_roots = roots[:args.bfs_run_roots] if args.bfs_run_roots > 0 else roots
+    algos = set(args.algos.split(","))
+    workload = f"kron{args.scale}"
 
     # Untimed pre-pass with the tile BFS (its scans follow the reference's
     # tile-skip rule): Graph500 M_reach and the scanned edges / tiles of each
-    # root, for the fixed algorithmic-bytes formula.  BFS output is
-    # deterministic, so these hold for every BFS mode.
+    # root, for the fixed formula.  BFS output is deterministic, so these hold
+    # for every BFS mode.  Its device time is the reference method's time per root.
     m_reach, ref_scan = {}, {}
     deg64 = deg.astype(np.int64)
     g.set_bfs_mode(0)
     ref_levels = None
-    for r in roots:
+    tile_ms = []
+    for r in run_roots:
         d = g.bfs(r).depth
         st = g.stats()
+        tile_ms.append(st["ms"])
         m_reach[r] = int(deg64[d > 0].sum() // 2)
         ref_scan[r] = (st["edges_scanned"], st["tiles_scanned"], st["iterations"], int(np.count_nonzero(d)))
         if ref_levels is None:
             ref_levels = g.level_stats()
     log("BFS M_reach and reference scan counts per root computed")
     ref_kc = None
-    if "kcore" in set(args.algos.split(",")):
+    if "kcore" in algos:
         # the reference's tile scans of k-core (same rounds and degrees as the default method)
         g.set_kcore_method(0)
         g.kcore(args.k, copy_out=False)
@@ -359,7 +567,6 @@ def main():
         g.set_kcore_method(1)
         log(f"k-core reference scans: {ref_kc}")
 
-    algos = set(args.algos.split(","))
     tune = None
     if args.tune:
         tune = g.tune()
@@ -382,7 +589,7 @@ def main():
     g.set_kernel_timing(True)
     kern = {}
     T = g.T
-    # per algorithm: device ms, traversed edges (GTEPS convention), algorithmic bytes
+    # per algorithm: device ms, traversed edges (GTEPS convention), fixed reference-formula bytes
     per = {k: [0.0, 0.0, 0.0] for k in ("pagerank", "bfs", "wcc", "kcore")}
     bfs_scanned = [0.0]
     first_root_levels = []
@@ -402,7 +609,7 @@ def main():
                 per["pagerank"][0] += st["ms"]
                 per["pagerank"][1] += 2 * E * args.pr_iters
                 per["pagerank"][2] += args.pr_iters * alg_bytes_pr_iter(n, E, T)
-        for r in (roots if "bfs" in algos else []):
+        for r in (run_roots if "bfs" in algos else []):
             g.bfs(r, copy_out=False)
             st = g.stats()
             if record and r == roots[0] and not first_root_levels:
@@ -422,7 +629,7 @@ def main():
                 passes = st["iterations"]
                 jumps = st["kernels"].get("wcc_compress", {}).get("calls", 0)
                 per["wcc"][0] += st["ms"]
-                per["wcc"][1] += 2 * E * passes
+                per["wcc"][1] += 2 * E
                 per["wcc"][2] += alg_bytes_wcc(n, E, T, passes, jumps)
         if "kcore" not in algos:
             return
@@ -431,7 +638,7 @@ def main():
         if record:
             absorb(st)
             per["kcore"][0] += st["ms"]
-            per["kcore"][1] += 2 * E * st["iterations"]
+            per["kcore"][1] += 2 * ref_kc[0]
             per["kcore"][2] += alg_bytes_kcore(n, *ref_kc)
 
     for i in range(args.warmup):
@@ -450,36 +657,59 @@ def main():
     def rate(x, ms):
         return x / (ms / 1e3) / 1e9 if ms > 0 else None
 
+    idx = g.index_info()
+    act_words = (idx["vertices"] + 31) // 32
     names = {"pagerank": "pr", "bfs": "bfs", "wcc": "wcc", "kcore": "kcore"}
+    trf_keys = {"pagerank": args.pr_method, "bfs": "bfs_do" if bfs_mode == 1 else "bfs_tile",
+                "wcc": "wcc_index", "kcore": "kcore_index"}
     per_algo = {}
     for algo, (ms, teps, ab) in per.items():
         if names[algo] not in algos:
             continue
-        d = {"ms": ms / args.steps, "gteps": rate(teps, ms), "alg_gbs": rate(ab, ms),
-             "hbm_frac": rate(ab, ms) / HBM_PEAK_GBS if ms > 0 else None}
+        unit_ms = ms / args.steps
+        d = {"ms": unit_ms, "gteps": rate(teps, ms), "ref_formula_gbs": rate(ab, ms)}
         if algo == "pagerank":
-            d.update(iterations=args.pr_iters, method=args.pr_method)
+            unit_ms /= args.pr_iters
+            d.update(iterations=args.pr_iters, method=args.pr_method,
+                     gteps_convention="2E per iteration")
         if algo == "bfs":
-            d.update(ms_per_root=ms / args.steps / len(roots), roots=len(roots),
+            d.update(ms_per_root=ms / args.steps / len(run_roots), roots=len(run_roots),
                      mode=["tile sweep", "direction-optimizing"][max(bfs_mode, 0)],
-                     gteps_scanned=rate(bfs_scanned[0], ms))
-            # per level of the first root: this run's device time against the fixed
-            # per-level bytes (scans of the reference's tile-skip rule, from the pre-pass)
+                     gteps_convention="Graph500 M_reach per root", gteps_ref_scanned=rate(bfs_scanned[0], ms))
+            unit_ms = sum(x["ms"] for x in first_root_levels)   # the profiled unit is the first root
+            trf = measured_traffic(workload, trf_keys[algo], unit_ms)
             lv = []
-            for x, ref in zip(first_root_levels, ref_levels or []):
-                b = 4 * ref["edges"] + 8 * ref["tiles"] + 3 * n / 8 + ref["reached"]
-                gbs = b / (x["ms"] / 1e3) / 1e9 if x["ms"] > 0 else None
-                lv.append({"level": x["level"], "direction": x["direction"], "frontier": x["frontier"],
-                           "edges_examined": x["edges"], "reached": x["reached"], "ms": round(x["ms"], 4),
-                           "alg_bytes": int(b), "alg_gbs": round(gbs, 1) if gbs else None,
-                           "hbm_frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None})
+            rec_lv = trf["rec"].get("levels", []) if trf else []
+            for i, x in enumerate(first_root_levels):
+                ref = (ref_levels or [])[i] if i < len(ref_levels or []) else None
+                e = {"level": x["level"], "direction": x["direction"], "frontier": x["frontier"],
+                     "edges_examined": x["edges"], "reached": x["reached"], "ms": round(x["ms"], 4),
+                     "method_bytes_est": int(level_method_bytes(act_words, x))}
+                if ref:
+                    rb = 4 * ref["edges"] + 8 * ref["tiles"] + 3 * n / 8 + ref["reached"]
+                    e["ref_formula_gbs"] = round(rb / (x["ms"] / 1e3) / 1e9, 1) if x["ms"] > 0 else None
+                if i < len(rec_lv) and x["ms"] > 0:
+                    lb = rec_lv[i]["fetch_bytes"] + rec_lv[i]["write_bytes"]
+                    e["traffic_bytes"] = int(lb)
+                    e["traffic_frac"] = round(lb / (x["ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                lv.append(e)
             d["levels_first_root"] = lv
+            d["first_root_ms"] = round(unit_ms, 4)
+            d["first_root_method_bytes_est"] = int(sum(e["method_bytes_est"] for e in lv))
+        else:
+            trf = measured_traffic(workload, trf_keys[algo], unit_ms)
+        if algo == "wcc":
+            d["gteps_convention"] = "2E per run"
         if algo == "kcore":
-            d.update(k=args.k)
+            d.update(k=args.k, gteps_convention="2 x stored edges scanned by the reference's tile sweeps")
+        if trf:
+            d["traffic_bytes"] = trf["bytes"]
+            d["traffic_frac"] = trf["bytes"] / (unit_ms / 1e3) / 1e9 / HBM_PEAK_GBS
+            d["traffic_unit"] = {"pagerank": "iteration", "bfs": "first root", "wcc": "run", "kcore": "run"}[algo]
+            d["traffic_source"] = trf["source"]
         per_algo[algo] = {k2: round(v, 4) if isinstance(v, float) else v for k2, v in d.items()}
 
     chunk = tune["chunk"] if tune else 4096
-    idx = g.index_info()
     kernels = {name: {"total_ms": round(ms / args.steps, 3), "launches_per_step": calls / args.steps,
                       "avg_ms": round(ms / max(calls, 1), 4)} for name, (ms, calls) in kern.items()}
     # roofline: one PageRank iteration (all its kernels) against the fixed formula
@@ -495,17 +725,30 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                     "bytes_per_launch": int(b), "avg_launch_ms": round(it_ms, 4),
                     "kernel_ms": {k: round(v, 4) for k, v in it_kern.items()}}
-        trf = measured_traffic(f"kron{args.scale}", args.pr_method, it_ms)
+        trf = measured_traffic(workload, args.pr_method, it_ms)
         if trf:
             roofline["traffic"] = trf["bytes"]
             roofline["traffic_frac"] = round(trf["bytes"] / (it_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
             roofline["traffic_source"] = trf["source"]
 
+    serving = None
+    if not args.no_serving and "bfs" in algos:
+        g.set_kernel_timing(False)
+        srng = np.random.default_rng(777)
+        pool = np.flatnonzero(deg)
+        log("serving: BFS saturation and Poisson arrivals on the same graph")
+        serving = bfs_saturation(ga, g, srng.choice(pool, 256, replace=True), args.workers,
+                                 tile_ms_per_root=float(np.mean(tile_ms)),
+                                 do_ms_per_root=per_algo["bfs"]["ms_per_root"] if bfs_mode == 1 else None)
+        serving["poisson"] = bfs_poisson(ga, g, pool, args.serve_rate, 3.0, args.workers, srng)
+        serving["graph_scale"] = args.scale
+        log(f"serving: {serving}")
+
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(args.cpu_scale, threads)
+        threads, visible = host_threads()
+        log(f"cpu baseline on {threads} threads ({visible} CPUs visible)")
+        cpu = cpu_baseline(args.cpu_scale, threads, visible)
 
     out = {
         "metric": "GTEPS + % HBM peak per algo at Kron-28; batched BFS queries/s, p50/p99 lat",
@@ -513,14 +756,14 @@ def main():
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+u32", "data": "synthetic (Graph500 Kronecker, seeds 1,2, generated on device)",
         "config": {"workload": f"Kron-{args.scale} ef16 undirected, resident; PR {args.pr_iters} it + "
-                               f"BFS {len(roots)} roots + WCC + k-core k={args.k}",
+                               f"BFS {len(run_roots)} roots + WCC + k-core k={args.k}",
                    "scale": args.scale, "vertices": n, "stored_edges": E, "nonempty_tiles": g.nonempty_tiles,
                    "gen_convert_s": round(gen_s, 2), "parallelism": "single GPU",
                    "index": {"entries": idx["entries"], "build_ms": round(idx["build_ms"], 1),
                              "device_gb": round(idx["device_bytes"] / 1e9, 1)},
                    "tuned": {"chunk": chunk, "xcd_order": tune["xcd_order"] if tune else 1,
                              "bfs_alpha": tune["do_alpha"] if tune else 14, "bfs_beta": tune["do_beta"] if tune else 24}},
-        "per_algo": per_algo, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+        "per_algo": per_algo, "kernels": kernels, "roofline": roofline, "serving": serving, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     g.close()

@@ -46,6 +46,9 @@ def lib() -> ctypes.CDLL:
         L.or_kron_convert_count.restype = i64
         L.or_kron_convert_scatter.argtypes = [i, i, u64, u64, i, u64, p, p]
         L.or_grid_checksum.argtypes = [u64, p, p]
+        L.or_bfs_mt.argtypes = [u64, i, p, p, u64, p, p, i, i]
+        L.or_wcc_mt.argtypes = [u64, i, p, p, p, p, i]
+        L.or_kcore_mt.argtypes = [u64, p, p, p, i, p, p, p, i, p]
         L.or_grid_checksum.restype = u64
         L.or_bfs.argtypes = [u64, i, p, p, u64, p, p, i]
         L.or_wcc.argtypes = [u64, i, p, p, p, p]
@@ -132,28 +135,43 @@ def grid_checksum(start, grid) -> int:
     return int(lib().or_grid_checksum(start.size - 1, _p(start), _p(grid)))
 
 
-def bfs(n, start, grid, root, directed=False):
+def bfs(n, start, grid, root, directed=False, nthreads=1):
     depth = np.empty(n, np.uint8)
     counts = np.zeros(256, np.uint64)
-    nl = lib().or_bfs(n, int(directed), _p(start), _p(grid), root, _p(depth), _p(counts), 256)
+    if nthreads > 1:
+        nl = lib().or_bfs_mt(n, int(directed), _p(start), _p(grid), root, _p(depth), _p(counts), 256, nthreads)
+    else:
+        nl = lib().or_bfs(n, int(directed), _p(start), _p(grid), root, _p(depth), _p(counts), 256)
     if nl < 0:
         raise RuntimeError(f"or_bfs failed ({nl})")
     return depth, [int(c) for c in counts[:nl]]
 
 
-def wcc(n, start, grid, directed=False):
+def wcc(n, start, grid, directed=False, nthreads=1):
     label = np.empty(n, np.uint32)
     nc = np.zeros(1, np.uint64)
-    if lib().or_wcc(n, int(directed), _p(start), _p(grid), _p(label), _p(nc)):
+    if nthreads > 1:
+        rc = lib().or_wcc_mt(n, int(directed), _p(start), _p(grid), _p(label), _p(nc), nthreads)
+    else:
+        rc = lib().or_wcc(n, int(directed), _p(start), _p(grid), _p(label), _p(nc))
+    if rc:
         raise RuntimeError("or_wcc failed")
     return label, int(nc[0])
 
 
-def kcore(n, start, grid, degree, k):
+def kcore(n, start, grid, degree, k, nthreads=1, scanned=None):
+    """-> (deg, count_ne0, count_pos, rounds); with nthreads > 1 and a list
+    `scanned`, appends the stored edges its edge passes visited (tile skip rule)."""
     deg = np.empty(n, np.int32)
-    c = np.zeros(2, np.uint64)
-    rounds = lib().or_kcore(n, _p(start), _p(grid), _p(degree), k, _p(deg), _p(c[:1]),
-                            ctypes.c_void_p(c.ctypes.data + 8))
+    c = np.zeros(3, np.uint64)
+    if nthreads > 1:
+        rounds = lib().or_kcore_mt(n, _p(start), _p(grid), _p(degree), k, _p(deg), _p(c[:1]),
+                                   ctypes.c_void_p(c.ctypes.data + 8), nthreads, ctypes.c_void_p(c.ctypes.data + 16))
+        if scanned is not None:
+            scanned.append(int(c[2]))
+    else:
+        rounds = lib().or_kcore(n, _p(start), _p(grid), _p(degree), k, _p(deg), _p(c[:1]),
+                                ctypes.c_void_p(c.ctypes.data + 8))
     if rounds < 0:
         raise RuntimeError("or_kcore failed")
     return deg, int(c[0]), int(c[1]), rounds

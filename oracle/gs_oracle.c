@@ -25,6 +25,7 @@
  *   or_kron_convert_*  -- equal to or_convert(or_kron_range) over the whole list
  *   or_grid_checksum   -- invariant under in-tile permutation, changed by one
  *                         moved or altered edge
+ *   or_*_mt            -- equal to the serial forms (tests), which are pinned
  *   or_bfs             -- scipy.sparse.csgraph breadth-first distances
  *   or_wcc             -- scipy.sparse.csgraph.connected_components
  *   or_kcore           -- independent numpy peeling in tests + brute force
@@ -736,6 +737,167 @@ int or_kcore(uint64_t n, const uint64_t *start, const uint32_t *grid, const int3
     *count_ne0 = ne0;
     *count_pos = pos;
     free(act);
+    walk_free(&w);
+    return rounds;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Threaded forms (the CPU baseline's timed runs).  They parallelise the     */
+/* tile sweeps over tiles with atomics, as the reference's OpenMP jobs do    */
+/* (bfs.cpp:158-177 sets depths with plain stores; kcore.cpp:69-103 and      */
+/* pr.cpp:60-80 use atomic updates), and give the serial functions' results. */
+/* ------------------------------------------------------------------------ */
+
+/* or_bfs with the level sweep split over tiles.  A vertex reached by two
+ * threads in one level gets the same depth from both; the level count is
+ * taken afterwards from the depth array, so it is exact. */
+int or_bfs_mt(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, uint64_t root,
+              uint8_t *depth, uint64_t *level_counts, int max_levels, int nthreads)
+{
+    walk_t w;
+    if (root >= n) return -1;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    memset(depth, 0, n);
+    depth[root] = 1;
+    int level = 1, nlev = 0;
+    for (;;) {
+        int any = 0;
+        #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64) reduction(|:any)
+        for (uint64_t t = 0; t < w.T; ++t) {
+            uint8_t *d0 = depth + ((uint64_t)w.row[t] << 16);
+            uint8_t *d1 = depth + ((uint64_t)w.col[t] << 16);
+            for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                uint32_t a = grid[k] & 0xFFFF, b = grid[k] >> 16;
+                uint8_t da = __atomic_load_n(&d0[a], __ATOMIC_RELAXED), db = __atomic_load_n(&d1[b], __ATOMIC_RELAXED);
+                if (da == level && db == 0) { __atomic_store_n(&d1[b], (uint8_t)(level + 1), __ATOMIC_RELAXED); any = 1; }
+                else if (!directed && db == level && da == 0) {
+                    __atomic_store_n(&d0[a], (uint8_t)(level + 1), __ATOMIC_RELAXED);
+                    any = 1;
+                }
+            }
+        }
+        uint64_t cnt = 0;
+        if (any) {
+            #pragma omp parallel for num_threads(nthreads) reduction(+:cnt) schedule(static)
+            for (uint64_t v = 0; v < n; ++v) cnt += depth[v] == level + 1;
+        }
+        if (nlev < max_levels && level_counts) level_counts[nlev] = cnt;
+        ++nlev;
+        if (cnt == 0) break;
+        if (level + 1 > 255) { walk_free(&w); return -2; }
+        ++level;
+    }
+    walk_free(&w);
+    return nlev;
+}
+
+/* WCC by parallel hooking over the tiles: every stored edge hangs the larger
+ * of its endpoints' roots under the smaller (compare-and-swap on the root's
+ * parent entry), then every parent pointer jumps to its root; repeated until
+ * a pass hooks nothing.  A root only ever hangs under a smaller id, so each
+ * final root is its component's smallest vertex: the same labels and count
+ * as or_wcc. */
+static uint32_t mt_root(uint32_t *par, uint32_t x)
+{
+    uint32_t p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
+    while (p != x) { x = p; p = __atomic_load_n(&par[x], __ATOMIC_RELAXED); }
+    return x;
+}
+
+int or_wcc_mt(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, uint32_t *label,
+              uint64_t *ncomp, int nthreads)
+{
+    walk_t w;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    uint32_t *par = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint8_t *touched = (uint8_t *)calloc(n, 1);
+    if (!par || !touched) { free(par); free(touched); walk_free(&w); return -1; }
+    #pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (uint64_t v = 0; v < n; ++v) par[v] = (uint32_t)v;
+    for (int changed = 1; changed;) {
+        changed = 0;
+        #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64) reduction(|:changed)
+        for (uint64_t t = 0; t < w.T; ++t) {
+            uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+            for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                uint32_t a = (uint32_t)(o0 + (grid[k] & 0xFFFF)), b = (uint32_t)(o1 + (grid[k] >> 16));
+                touched[a] = touched[b] = 1;
+                for (;;) {
+                    uint32_t ra = mt_root(par, a), rb = mt_root(par, b);
+                    if (ra == rb) break;
+                    uint32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra, exp = hi;
+                    if (__atomic_compare_exchange_n(&par[hi], &exp, lo, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                        changed = 1;
+                        break;
+                    }
+                }
+            }
+        }
+        #pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (uint64_t v = 0; v < n; ++v) par[v] = mt_root(par, (uint32_t)v);
+    }
+    uint64_t c = 0;
+    #pragma omp parallel for num_threads(nthreads) reduction(+:c) schedule(static)
+    for (uint64_t v = 0; v < n; ++v) {
+        if (!touched[v]) { label[v] = 0xFFFFFFFFu; continue; }
+        label[v] = par[v];
+        c += (par[v] == v);
+    }
+    *ncomp = c;
+    free(par);
+    free(touched);
+    walk_free(&w);
+    return 0;
+}
+
+/* or_kcore with the peel and the edge pass split over threads; degree
+ * decrements are atomic (kcore.cpp:69-103 uses atomic subtraction).  The
+ * edge pass skips tiles whose tile row and column hold no removed vertex, as
+ * the reference does (gstore.cpp:2768-2781). */
+int or_kcore_mt(uint64_t n, const uint64_t *start, const uint32_t *grid, const int32_t *degree,
+                int k, int32_t *deg, uint64_t *count_ne0, uint64_t *count_pos, int nthreads,
+                uint64_t *scanned)
+{
+    uint64_t scan = 0;
+    walk_t w;
+    if (walk_init(&w, n, 0, start, grid)) return -1;
+    uint8_t *act = (uint8_t *)calloc(n, 1);
+    uint64_t rows = (n + 65535) >> 16;
+    uint8_t *rowact = (uint8_t *)calloc(rows, 1);
+    if (!act || !rowact) { free(act); free(rowact); walk_free(&w); return -1; }
+    memcpy(deg, degree, sizeof(int32_t) * n);
+    int rounds = 0;
+    for (;;) {
+        uint64_t f = 0;
+        memset(rowact, 0, rows);
+        #pragma omp parallel for num_threads(nthreads) reduction(+:f) schedule(static, 65536)
+        for (uint64_t v = 0; v < n; ++v) {
+            int16_t m = (int16_t)deg[v];
+            act[v] = 0;
+            if (m < k && m > 0) { deg[v] = 0; act[v] = 1; ++f; rowact[v >> 16] = 1; }
+        }
+        ++rounds;
+        if (f == 0) break;
+        #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64) reduction(+:scan)
+        for (uint64_t t = 0; t < w.T; ++t) {
+            if (!rowact[w.row[t]] && !rowact[w.col[t]]) continue;
+            scan += w.start[t + 1] - w.start[t];
+            uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+            for (uint64_t e = w.start[t]; e < w.start[t + 1]; ++e) {
+                uint64_t a = o0 + (grid[e] & 0xFFFF), b = o1 + (grid[e] >> 16);
+                if (act[a]) __atomic_fetch_sub(&deg[b], 1, __ATOMIC_RELAXED);
+                if (act[b]) __atomic_fetch_sub(&deg[a], 1, __ATOMIC_RELAXED);
+            }
+        }
+    }
+    uint64_t ne0 = 0, pos = 0;
+    #pragma omp parallel for num_threads(nthreads) reduction(+:ne0,pos) schedule(static)
+    for (uint64_t v = 0; v < n; ++v) { ne0 += (deg[v] != 0); pos += (deg[v] > 0); }
+    *count_ne0 = ne0;
+    *count_pos = pos;
+    if (scanned) *scanned = scan;   /* stored edges in the tiles the edge passes visited */
+    free(act);
+    free(rowact);
     walk_free(&w);
     return rounds;
 }

@@ -249,6 +249,21 @@ def test_grid_checksum_is_a_multiset_hash():
     assert oracle.grid_checksum(s4, grid) != c
 
 
+@pytest.mark.parametrize("name,n,e", CASES, ids=[c[0] for c in CASES])
+def test_threaded_forms_equal_serial(name, n, e):
+    for directed in (False, True):
+        start, grid, degree = oracle.convert(e, n, directed)
+        for root in (0, int(np.argmax(degree)), n - 1):
+            a, b = oracle.bfs(n, start, grid, root, directed), oracle.bfs(n, start, grid, root, directed, nthreads=4)
+            assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+        a, b = oracle.wcc(n, start, grid, directed), oracle.wcc(n, start, grid, directed, nthreads=4)
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    start, grid, degree = oracle.convert(e, n)
+    for k in (2, 4, 8):
+        a, b = oracle.kcore(n, start, grid, degree, k), oracle.kcore(n, start, grid, degree, k, nthreads=4)
+        assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
+
+
 def test_pagerank_parallel_matches_sequential_within_fp32():
     n, e = graphs.kron(12)
     start, grid, degree = oracle.convert(e, n)
