@@ -120,6 +120,7 @@ _SIGS = {
     "gs_set_pagerank_blocking": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
                                                 ctypes.c_uint32, ctypes.c_uint32]),
     "gs_set_wcc_method": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gs_set_pagerank_gather": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "gs_set_kcore_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_batch_bfs_method": (ctypes.c_int, [_P, ctypes.c_int]),
     "gs_set_bfs_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -301,6 +302,10 @@ class Graph:
         """Work split and layout of the blocked PageRank (small values exercise the split paths)."""
         _check(_lib.gs_set_pagerank_blocking(self._h, gather_entries, gather_runs, accum_entries, dense_entries,
                                              compact_permille))
+
+    def set_pagerank_gather(self, units_per_grab: int) -> None:
+        """Blocked PageRank gather: 0 = a workgroup per unit; n = resident workgroups taking n units per grab."""
+        _check(_lib.gs_set_pagerank_gather(self._h, units_per_grab))
 
     def set_wcc_method(self, method: int) -> None:
         """1 (default): sampled linking over the neighbour-list index; 0: hook passes over the tiles."""

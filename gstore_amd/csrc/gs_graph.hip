@@ -844,6 +844,13 @@ int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gath
     return GS_OK;
 }
 
+int gs_set_pagerank_gather(gs_graph *g, uint32_t units_per_grab) {
+    if (!g || units_per_grab > 4096) { set_error("gs_set_pagerank_gather: 0..4096 units per grab"); return GS_ERR_ARG; }
+    std::lock_guard<std::mutex> lp(g->pb_mu);
+    g->pb_grab = units_per_grab;
+    return GS_OK;
+}
+
 int gs_debug_fail_alloc(gs_graph *g, int nth) {
     if (!g || nth < 0) { set_error("bad argument"); return GS_ERR_ARG; }
     std::lock_guard<std::mutex> lk(g->build_mu);

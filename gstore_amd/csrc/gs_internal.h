@@ -233,6 +233,9 @@ struct gs_graph {
     uint64_t pb_entries = 0;                 // 2E plus the padding that fills each run's last window
     uint64_t pb_stream = 0;                  // accumulation stream length (with block alignment)
     uint32_t pb_compact = 800;               // a block is compacted when its segments <= this / 1000 of its entries
+    uint32_t pb_grab = 8;                    // gather: units per grab of a resident workgroup (0: a workgroup per unit)
+    gs::DevBuf<unsigned long long> pb_ctr;   // gather: next grab
+    uint32_t num_cus = 0;                    // compute units of the device (queried once)
     gs::DevBuf<uint16_t> pb_src;             // pb_entries, q-major: offset of u in its block (compacted
                                              // blocks: bit 15 marks a segment's last entry)
     gs::DevBuf<uint32_t> pb_wcoff;           // per q-major window: its first segment, relative to its run

@@ -278,21 +278,27 @@ __global__ void k_pb_mark(uint16_t *__restrict__ src, const uint16_t *__restrict
 
 constexpr int PB_WPL = 4;   // gather: windows per lane per step, all loads issued first
 
-// prior[pbase .. pbase + pn) into LDS with 16-byte loads, all issued before
-// the stores (pbase is a multiple of 2^15, so the loads are aligned).
+// prior[pbase .. pbase + pn) into LDS with 16-byte loads, two per thread in
+// flight at a time (pbase is a multiple of 2^15, so the loads are aligned).
+// Staging happens once per source block a workgroup meets, so it keeps few
+// registers live beside the window loop rather than all its loads in flight.
 __device__ inline void stage_block(const float *__restrict__ prior, uint64_t pbase, uint32_t pn, float4 *pri4) {
     const float4 *p4 = reinterpret_cast<const float4 *>(prior + pbase);
     const uint32_t n4 = pn >> 2;
-    float4 v[PB_B / 4 / PB_WG];
+    constexpr uint32_t PER = PB_B / 4 / PB_WG, IN = 2;
 #pragma unroll
-    for (uint32_t j = 0; j < PB_B / 4 / PB_WG; ++j) {
-        const uint32_t i = threadIdx.x + j * PB_WG;
-        v[j] = i < n4 ? p4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    for (uint32_t j0 = 0; j0 < PER; j0 += IN) {
+        float4 v[IN];
 #pragma unroll
-    for (uint32_t j = 0; j < PB_B / 4 / PB_WG; ++j) {
-        const uint32_t i = threadIdx.x + j * PB_WG;
-        if (i < n4) pri4[i] = v[j];
+        for (uint32_t j = 0; j < IN; ++j) {
+            const uint32_t i = threadIdx.x + (j0 + j) * PB_WG;
+            v[j] = i < n4 ? p4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < IN; ++j) {
+            const uint32_t i = threadIdx.x + (j0 + j) * PB_WG;
+            if (i < n4) pri4[i] = v[j];
+        }
     }
     float *pri = reinterpret_cast<float *>(pri4);
     for (uint32_t i = (n4 << 2) + threadIdx.x; i < pn; i += PB_WG) pri[i] = prior[pbase + i];
@@ -397,7 +403,15 @@ __device__ inline void pb_segments_q(const pb_u32x4 (&sq)[2], const float *pri, 
 // store per segment made the gather 30% slower at Kron-28): pb_segments with
 // two or four slots when no window of the wave has more segments (the
 // heaviest blocks), else pb_segments_q a quarter window at a time.
-__global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__ units,
+//
+// Work split: with grab == 0 each workgroup takes one unit; otherwise one
+// workgroup per CU stays resident and takes `grab` consecutive units at a time
+// from a counter (ctr, zero at launch) until none are left, staging a prior
+// block only when its source block changes -- consecutive units mostly share
+// one (a heavy source block spans many units), so most units skip the 128 KiB
+// staging and the barrier behind it.
+__global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__ units, uint32_t nunits,
+                                                     uint32_t grab, unsigned long long *ctr,
                                                      const uint64_t *__restrict__ run_pw,
                                                      const uint64_t *__restrict__ run_qw,
                                                      const uint64_t *__restrict__ run_st,
@@ -411,71 +425,91 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
     __shared__ uint16_t rs[PB_RUNS];
     __shared__ uint32_t rq[PB_RUNS];
     __shared__ uint64_t rt[PB_RUNS];
-    const PbUnit1 u = units[blockIdx.x];
-    const uint64_t pbase = (uint64_t)u.p << PB_SHIFT;
-    const uint32_t pn = (uint32_t)std::min<uint64_t>(PB_B, nzn - pbase);
-    stage_block(prior, pbase, pn, pri4);
+    __shared__ uint32_t s_grab;
     if (threadIdx.x == 0) pri4[PB_B / 4] = make_float4(0.f, 0.f, 0.f, 0.f);
     const float *pri = reinterpret_cast<const float *>(pri4);
-    const int nr = (int)(u.r1 - u.r0);
-    for (int k = threadIdx.x; k < nr; k += PB_WG) {
-        const uint64_t r0 = run_pw[u.r0 + k];
-        const uint64_t s = r0 > u.w0 ? r0 : u.w0;
-        const uint64_t t = run_st[u.r0 + k];
-        rs[k] = (uint16_t)(s - u.w0);
-        rq[k] = (uint32_t)(run_qw[u.r0 + k] + (s - r0));
-        // plain runs: the stream follows the windows; compacted: wcoff is relative to the run
-        rt[k] = (t & PB_COMPACT) ? t : t + (s - r0) * PB_WIN;
-    }
-    const uint32_t len = (uint32_t)(u.w1 - u.w0);
-    __syncthreads();
-    for (uint32_t wb = threadIdx.x; wb < len; wb += PB_WPL * PB_WG) {
-        uint64_t qw[PB_WPL], st[PB_WPL];
-        pb_u32x4 sq[PB_WPL][2];
-        uint32_t co[PB_WPL];
-#pragma unroll
-        for (int j = 0; j < PB_WPL; ++j) {
-            // past the unit, a lane reloads its first window and stores nothing
-            const uint32_t w = wb + j * PB_WG < len ? wb + j * PB_WG : wb;
-            int k = 0;
-#pragma unroll
-            for (int step = PB_RUNS / 2; step > 0; step >>= 1)
-                if (k + step < nr && rs[k + step] <= w) k += step;
-            qw[j] = (uint64_t)rq[k] + (w - rs[k]);
-            st[j] = rt[k] & PB_COMPACT ? rt[k] : rt[k] + (uint64_t)(w - rs[k]) * PB_WIN;
-            // plain loads: nontemporal ones made the gather 11% slower at Kron-28
-            const pb_u32x4 *s4 = reinterpret_cast<const pb_u32x4 *>(src + qw[j] * PB_WIN);
-            sq[j][0] = s4[0];
-            sq[j][1] = s4[1];
-            co[j] = st[j] & PB_COMPACT ? wcoff[qw[j]] : 0u;   // plain windows need no offset
+    uint32_t ui = grab ? 0 : blockIdx.x, uend = grab ? 0 : blockIdx.x + 1;
+    uint32_t staged = 0xFFFFFFFFu;
+    for (;;) {
+        if (ui >= uend) {
+            if (!grab) break;
+            __syncthreads();   // every lane has read the previous grab
+            if (threadIdx.x == 0) s_grab = (uint32_t)atomicAdd(ctr, 1ull);
+            __syncthreads();
+            // the same in every lane: read it into a scalar register, so the unit
+            // records below load through the scalar cache and hold no vector registers
+            const uint64_t first = (uint64_t)__builtin_amdgcn_readfirstlane(s_grab) * grab;
+            if (first >= nunits) break;   // the same value in every lane: the whole workgroup leaves
+            ui = (uint32_t)first;
+            uend = (uint32_t)std::min<uint64_t>(nunits, first + grab);
         }
+        const PbUnit1 u = units[ui++];
+        __syncthreads();   // the previous unit's reads of the prior block and run tables are done
+        if (u.p != staged) {
+            const uint64_t pbase = (uint64_t)u.p << PB_SHIFT;
+            stage_block(prior, pbase, (uint32_t)std::min<uint64_t>(PB_B, nzn - pbase), pri4);
+            staged = u.p;
+        }
+        const int nr = (int)(u.r1 - u.r0);
+        for (int k = threadIdx.x; k < nr; k += PB_WG) {
+            const uint64_t r0 = run_pw[u.r0 + k];
+            const uint64_t s = r0 > u.w0 ? r0 : u.w0;
+            const uint64_t t = run_st[u.r0 + k];
+            rs[k] = (uint16_t)(s - u.w0);
+            rq[k] = (uint32_t)(run_qw[u.r0 + k] + (s - r0));
+            // plain runs: the stream follows the windows; compacted: wcoff is relative to the run
+            rt[k] = (t & PB_COMPACT) ? t : t + (s - r0) * PB_WIN;
+        }
+        const uint32_t len = (uint32_t)(u.w1 - u.w0);
+        __syncthreads();
+        for (uint32_t wb = threadIdx.x; wb < len; wb += PB_WPL * PB_WG) {
+            uint64_t qw[PB_WPL], st[PB_WPL];
+            pb_u32x4 sq[PB_WPL][2];
+            uint32_t co[PB_WPL];
 #pragma unroll
-        for (int j = 0; j < PB_WPL; ++j) {
-            if (wb + j * PB_WG >= len) break;
-            if (st[j] & PB_COMPACT) {
-                float *out = val + (st[j] & ~PB_COMPACT) + co[j];
-                uint32_t n = 0;
+            for (int j = 0; j < PB_WPL; ++j) {
+                // past the unit, a lane reloads its first window and stores nothing
+                const uint32_t w = wb + j * PB_WG < len ? wb + j * PB_WG : wb;
+                int k = 0;
 #pragma unroll
-                for (int h = 0; h < 8; ++h) {
-                    const uint32_t a = sq[j][h >> 2][h & 3];
-                    n += ((a >> 15) & 1u) + (a >> 31);
+                for (int step = PB_RUNS / 2; step > 0; step >>= 1)
+                    if (k + step < nr && rs[k + step] <= w) k += step;
+                qw[j] = (uint64_t)rq[k] + (w - rs[k]);
+                st[j] = rt[k] & PB_COMPACT ? rt[k] : rt[k] + (uint64_t)(w - rs[k]) * PB_WIN;
+                // plain loads: nontemporal ones made the gather 11% slower at Kron-28
+                const pb_u32x4 *s4 = reinterpret_cast<const pb_u32x4 *>(src + qw[j] * PB_WIN);
+                sq[j][0] = s4[0];
+                sq[j][1] = s4[1];
+                co[j] = st[j] & PB_COMPACT ? wcoff[qw[j]] : 0u;   // plain windows need no offset
+            }
+#pragma unroll
+            for (int j = 0; j < PB_WPL; ++j) {
+                if (wb + j * PB_WG >= len) break;
+                if (st[j] & PB_COMPACT) {
+                    float *out = val + (st[j] & ~PB_COMPACT) + co[j];
+                    uint32_t n = 0;
+#pragma unroll
+                    for (int h = 0; h < 8; ++h) {
+                        const uint32_t a = sq[j][h >> 2][h & 3];
+                        n += ((a >> 15) & 1u) + (a >> 31);
+                    }
+                    // register slots for the wave's largest segment count (wave-uniform branches)
+                    if (__ballot(n > 4)) pb_segments_q(sq[j], pri, out);
+                    else if (__ballot(n > 2)) pb_segments<4>(sq[j], pri, out);
+                    else pb_segments<2>(sq[j], pri, out);
+                } else {
+                    float x[PB_WIN];
+#pragma unroll
+                    for (int h = 0; h < 8; ++h) {
+                        const uint32_t a = sq[j][h >> 2][h & 3];
+                        x[2 * h] = pri[a & 0xFFFFu];
+                        x[2 * h + 1] = pri[a >> 16];
+                    }
+                    // plain stores: nontemporal ones measured four times slower here
+                    pb_f32x4 *v4 = reinterpret_cast<pb_f32x4 *>(val + st[j]);
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) v4[h] = pb_f32x4{x[4 * h], x[4 * h + 1], x[4 * h + 2], x[4 * h + 3]};
                 }
-                // register slots for the wave's largest segment count (wave-uniform branches)
-                if (__ballot(n > 4)) pb_segments_q(sq[j], pri, out);
-                else if (__ballot(n > 2)) pb_segments<4>(sq[j], pri, out);
-                else pb_segments<2>(sq[j], pri, out);
-            } else {
-                float x[PB_WIN];
-#pragma unroll
-                for (int h = 0; h < 8; ++h) {
-                    const uint32_t a = sq[j][h >> 2][h & 3];
-                    x[2 * h] = pri[a & 0xFFFFu];
-                    x[2 * h + 1] = pri[a >> 16];
-                }
-                // plain stores: nontemporal ones measured four times slower here
-                pb_f32x4 *v4 = reinterpret_cast<pb_f32x4 *>(val + st[j]);
-#pragma unroll
-                for (int h = 0; h < 4; ++h) v4[h] = pb_f32x4{x[4 * h], x[4 * h + 1], x[4 * h + 2], x[4 * h + 3]};
             }
         }
     }
@@ -977,13 +1011,31 @@ int build_blocked(gs_graph *g) {
     return GS_OK;
 }
 
-static int pb_gather(const gs_graph *g, gs_ws *w, const float *prior) {
+__global__ void k_pb_ctr_zero(unsigned long long *ctr) { *ctr = 0; }
+
+static int pb_gather(const gs_graph *gc, gs_ws *w, const float *prior) {
+    gs_graph *g = const_cast<gs_graph *>(gc);
     if (!g->pb_nu1) return GS_OK;
+    const uint32_t grab = g->pb_grab;
+    unsigned grid = (unsigned)g->pb_nu1;
+    if (grab) {
+        if (!g->pb_ctr.ptr) GS_TRY(g->pb_ctr.alloc(1));
+        if (!g->num_cus) {
+            int dev = 0, cus = 0;
+            GS_HIP(hipGetDevice(&dev));
+            GS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            g->num_cus = (uint32_t)std::max(cus, 1);
+        }
+        // one resident workgroup per CU (the kernel's LDS admits one)
+        grid = (unsigned)std::min<uint64_t>(g->num_cus, (g->pb_nu1 + grab - 1) / grab);
+        k_pb_ctr_zero<<<1, 1, 0, w->stream>>>(g->pb_ctr.ptr);
+        GS_HIP(hipGetLastError());
+    }
     hipEvent_t kb;
     w->timer.begin(GS_K_PB_GATHER, w->stream, &kb);
-    k_pb_gather<<<(unsigned)g->pb_nu1, PB_WG, 0, w->stream>>>(g->pb_u1.ptr, g->pb_run_pw.ptr, g->pb_run_qw.ptr,
-                                                              g->pb_run_st.ptr, g->pb_src.ptr, g->pb_wcoff.ptr, prior,
-                                                              g->nzn, g->pb_val.ptr);
+    k_pb_gather<<<grid, PB_WG, 0, w->stream>>>(g->pb_u1.ptr, (uint32_t)g->pb_nu1, grab, g->pb_ctr.ptr,
+                                               g->pb_run_pw.ptr, g->pb_run_qw.ptr, g->pb_run_st.ptr, g->pb_src.ptr,
+                                               g->pb_wcoff.ptr, prior, g->nzn, g->pb_val.ptr);
     GS_HIP(hipGetLastError());
     w->timer.end(GS_K_PB_GATHER, w->stream, kb);
     return GS_OK;

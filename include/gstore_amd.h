@@ -308,6 +308,13 @@ int gs_set_kcore_method(gs_graph *g, int method);
 int gs_set_pagerank_blocking(gs_graph *g, uint64_t gather_entries, uint32_t gather_runs, uint64_t accum_entries,
                              uint32_t dense_entries, uint32_t compact_permille);
 
+/* Work split of the blocked PageRank's gather: 0 launches one workgroup per
+ * unit (each stages its source block's priors); n > 0 (default 8) keeps one
+ * workgroup per CU resident, taking n consecutive units at a time and
+ * staging a source block only when it changes.  Results do not change.  No
+ * counterpart in the reference. */
+int gs_set_pagerank_gather(gs_graph *g, uint32_t units_per_grab);
+
 /* PageRank / personalised PageRank on undirected grids: 1 (default) adds
  * into the (at most 2048) highest-degree vertices through a per-workgroup LDS
  * table flushed once per workgroup, instead of one contended global atomic
