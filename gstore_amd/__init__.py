@@ -20,7 +20,7 @@ import numpy as np
 
 from . import tiles
 
-__all__ = ["Graph", "Server", "QueryResult", "Q_BFS", "Q_PPR", "Q_WCC", "Q_PAGERANK", "GStoreError", "kron_edges", "convert", "load_library", "tiles", "LIB_PATH",
+__all__ = ["Graph", "Server", "QueryResult", "Q_BFS", "Q_PPR", "Q_WCC", "Q_PAGERANK", "GStoreError", "kron_edges", "convert", "text_to_bin", "text_file_to_bin", "load_library", "tiles", "LIB_PATH",
            "PR_COMPAT", "PR_INTENDED"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -108,6 +108,15 @@ _SIGS = {
     "gs_wcc": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64)]),
     "gs_kcore": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_uint64),
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
+    "gs_bfs_async": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "gs_traverse": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32, _P, _P, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)]),
+    "gs_text_to_bin": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint64)]),
+    "gs_text_to_bin_mem": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, _P, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
+                                          ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]),
     "gs_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(_Stats)]),
     "gs_get_level_stats": (ctypes.c_int, [_P, ctypes.POINTER(_LevelStat), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int)]),
@@ -393,6 +402,24 @@ class Graph:
         _check(_lib.gs_wcc(self._h, _ptr(labels) if copy_out else None, ctypes.byref(nc)))
         return labels, int(nc.value)
 
+    def bfs_async(self, root: int = 1, copy_out: bool = True, max_sweeps: int = 4096):
+        """-j 30: asynchronous BFS -> (depth u8 or None, per-sweep store counts)."""
+        depth = np.empty(self.n, np.uint8) if copy_out else None
+        counts = np.zeros(max_sweeps, np.uint64)
+        ns = ctypes.c_int(0)
+        _check(_lib.gs_bfs_async(self._h, root, _ptr(depth) if copy_out else None, _ptr(counts), max_sweeps,
+                                 ctypes.byref(ns)))
+        return depth, [int(c) for c in counts[:min(ns.value, max_sweeps)]]
+
+    def traverse(self, pivot: int = 1, color: int = 0, copy_out: bool = True, max_sweeps: int = 1 << 16):
+        """-j 4: single colour propagation -> (cid u32 or None, per-sweep counts, coloured vertices)."""
+        cid = np.empty(self.n, np.uint32) if copy_out else None
+        counts = np.zeros(max_sweeps, np.uint64)
+        ns, col = ctypes.c_int(0), ctypes.c_uint64(0)
+        _check(_lib.gs_traverse(self._h, pivot, color, _ptr(cid) if copy_out else None, _ptr(counts), max_sweeps,
+                                ctypes.byref(ns), ctypes.byref(col)))
+        return cid, [int(c) for c in counts[:min(ns.value, max_sweeps)]], int(col.value)
+
     def kcore(self, k: int = 10, copy_out: bool = True):
         deg = np.empty(self.n, np.int32) if copy_out else None
         ne0, pos, rounds = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_int(0)
@@ -477,6 +504,28 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def text_to_bin(text: bytes):
+    """-c 4 on the device: text edge list -> (pairs (M, 2) u32, v_min, v_max, tokens)."""
+    load_library()
+    pairs, vmin, vmax, ntok = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+    _check(_lib.gs_text_to_bin_mem(text, len(text), None, 0, ctypes.byref(pairs), ctypes.byref(vmin),
+                                   ctypes.byref(vmax), ctypes.byref(ntok)))
+    out = np.empty((pairs.value, 2), np.uint32)
+    if pairs.value:
+        _check(_lib.gs_text_to_bin_mem(text, len(text), _ptr(out), pairs.value, ctypes.byref(pairs),
+                                       ctypes.byref(vmin), ctypes.byref(vmax), ctypes.byref(ntok)))
+    return out, vmin.value, vmax.value, ntok.value
+
+
+def text_file_to_bin(path: str, out_path: Optional[str] = None) -> dict:
+    """-c 4 on a file: writes out_path (default path + '.edge')."""
+    load_library()
+    pairs, vmin, vmax, ntok = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+    _check(_lib.gs_text_to_bin(path.encode(), out_path.encode() if out_path else None, ctypes.byref(pairs),
+                               ctypes.byref(vmin), ctypes.byref(vmax), ctypes.byref(ntok)))
+    return {"pairs": pairs.value, "vmin": vmin.value, "vmax": vmax.value, "tokens": ntok.value}
 
 
 def kron_edges(scale: int, e0: int, e1: int, seed1: int = 1, seed2: int = 2) -> np.ndarray:

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 HIP_SOURCES = ["gs_graph.hip", "gs_pagerank.hip", "gs_bfs.hip", "gs_bfs_batch.hip", "gs_wcc.hip", "gs_kcore.hip",
-               "gs_serve.hip", "gs_stream.hip", "gs_tune.hip", "gs_adj.hip", "gs_pb.hip"]
+               "gs_serve.hip", "gs_stream.hip", "gs_tune.hip", "gs_adj.hip", "gs_pb.hip", "gs_jobs.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

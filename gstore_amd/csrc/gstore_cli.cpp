@@ -8,8 +8,11 @@
 //   -m MB     device budget for the tile grid in MiB (the reference's -m
 //             sizes its streaming buffer)
 // Jobs (code order, gstore.cpp:235-333): 0/10/20 BFS (-a root), 1/11/21
-// PageRank (-a iterations), 2/12/22 k-core (-a k), 3/13/23 WCC.  The x, 1x and
-// 2x variants all run the resident HIP engine.  Defaults follow the
+// PageRank (-a iterations), 2/12/22 k-core (-a k), 3/13/23 WCC, 30
+// asynchronous BFS (-a root, default 1), 4 single colour propagation (-a
+// pivot, default 1).  The x, 1x and 2x variants all run the resident HIP
+// engine.  Conversions: -c 1 binary u32 pairs -> tile files, -c 4 text edge
+// list -> binary pairs in EDGES.edge (util.cpp:58-168).  Defaults follow the
 // reference: root 0 (-j 0; -j 10/20 use root 1, gstore.cpp:2634), 5
 // iterations, k = 10.
 // Methods: on an undirected grid resident in HBM the jobs run the engine's
@@ -86,10 +89,10 @@ int main(int argc, char **argv) {
             case 'h':
             default:
                 fprintf(stderr,
-                        "usage: %s -s S|-v N -i PREFIX -j {0,1,2,3,10..13,20..23} [-a ARG] [-m MB] [-r MODE] [-b 0|1] [-p OUT]\n"
-                        "       %s -s S|-v N -i EDGES -o PREFIX -c 1\n"
+                        "usage: %s -s S|-v N -i PREFIX -j {0,1,2,3,4,10..13,20..23,30} [-a ARG] [-m MB] [-r MODE] [-b 0|1] [-p OUT]\n"
+                        "       %s -s S|-v N -i EDGES -o PREFIX -c 1    |    %s -i TEXT -c 4\n"
                         "       %s -g S [-e EF] -o PREFIX\n",
-                        argv[0], argv[0], argv[0]);
+                        argv[0], argv[0], argv[0], argv[0]);
                 return o == 'h' ? 0 : 1;
         }
     }
@@ -107,12 +110,23 @@ int main(int argc, char **argv) {
         gs_close(g);
         return 0;
     }
-    if (n == 0) { fprintf(stderr, "give the vertex count with -s or -v\n"); return 1; }
     uint64_t p = (n >> 24) + ((n & 0xFFFFFF) != 0);
+    if (conv == 4) {   // text -> binary needs no vertex count (text_to_bin, util.cpp:58-168)
+        printf("Total Physcial Group Count (in One Direction) = %llu\n", (unsigned long long)p);
+        printf("Total Tiles Count (in One Direction) = %llu\n", (unsigned long long)(256 * p));
+        uint64_t pairs = 0, tokens = 0;
+        uint32_t vmin = 0, vmax = 0;
+        if (gs_text_to_bin(in.c_str(), out.empty() ? nullptr : out.c_str(), &pairs, &vmin, &vmax, &tokens))
+            die("gs_text_to_bin");
+        fprintf(stderr, "max vertex id: %u\nmin vertex id: %u\nedge count: %llu\nvert count: %llu\n", vmax, vmin,
+                (unsigned long long)pairs, (unsigned long long)((uint64_t)(uint32_t)(vmax - vmin) + 1));
+        return 0;
+    }
+    if (n == 0) { fprintf(stderr, "give the vertex count with -s or -v\n"); return 1; }
     printf("Total Physcial Group Count (in One Direction) = %llu\n", (unsigned long long)p);
     printf("Total Tiles Count (in One Direction) = %llu\n", (unsigned long long)(256 * p));
     if (conv) {
-        if (conv != 1) { fprintf(stderr, "-c %d: only -c 1 (in-memory conversion) is provided\n", conv); return 1; }
+        if (conv != 1) { fprintf(stderr, "-c %d: -c 1 and -c 4 are provided\n", conv); return 1; }
         struct stat st;
         if (stat(in.c_str(), &st) != 0 || st.st_size % 8) { fprintf(stderr, "bad edge file %s\n", in.c_str()); return 1; }
         std::vector<uint32_t> edges(st.st_size / 4);
@@ -130,8 +144,43 @@ int main(int argc, char **argv) {
     if (gs_get_info(g, &info)) die("gs_get_info");
     if (!directed && info.resident && bfs_method == 1 && gs_set_bfs_mode(g, 1, 0, 0)) die("gs_set_bfs_mode");
     gs_stats st;
+    if (job == 30) {   // grid::bfs2 (gstore.cpp:1900-2042), root -a, default 1
+        const uint64_t root = arg == -1 ? 1 : (uint64_t)arg;
+        printf("Root = %llu\n", (unsigned long long)root);
+        std::vector<uint8_t> depth(n);
+        std::vector<uint64_t> counts(1 << 16);
+        int sweeps = 0;
+        if (gs_bfs_async(g, root, depth.data(), counts.data(), (int)counts.size(), &sweeps)) die("gs_bfs_async");
+        gs_get_stats(g, &st);
+        for (int l = 0; l < sweeps && l < (int)counts.size(); ++l)
+            printf("level: %d frontier count: %llu\n", l + 1, (unsigned long long)counts[l]);
+        printf("Half BFS time = %f\n", st.ms / 1e3);
+        write_raw(result_path, depth.data(), n);
+        gs_close(g);
+        return 0;
+    }
+    if (job == 4) {    // grid::traverse (gstore.cpp:2169-2177), pivot -a, default 1, colour 0
+        const uint64_t pivot = arg == -1 ? 1 : (uint64_t)arg;
+        std::vector<uint32_t> cid(n);
+        std::vector<uint64_t> counts(1 << 16);
+        int sweeps = 0;
+        uint64_t colored = 0;
+        if (gs_traverse(g, pivot, 0, cid.data(), counts.data(), (int)counts.size(), &sweeps, &colored))
+            die("gs_traverse");
+        gs_get_stats(g, &st);
+        for (int l = 0; l < sweeps && l < (int)counts.size(); ++l)
+            printf("level: %d frontier count: %llu\n\n", l, (unsigned long long)counts[l]);
+        printf("Single color propagation time = %f\n", st.ms / 1e3);
+        write_raw(result_path, cid.data(), n * 4);
+        gs_close(g);
+        return 0;
+    }
     const int base = job % 10;
-    if (job != base && job != base + 10 && job != base + 20) { printf("Wrong value for -j argument\n"); return 1; }
+    if (base > 3 || (job != base && job != base + 10 && job != base + 20)) {
+        printf("Wrong value for -j argument\n");
+        gs_close(g);
+        return 1;
+    }
     switch (base) {
         case 0: {
             uint64_t root = arg == -1 ? (job == 0 ? 0 : 1) : (uint64_t)arg;

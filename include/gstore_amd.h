@@ -181,6 +181,44 @@ int gs_wcc(gs_graph *g, uint32_t *label_out, uint64_t *ncomp);
 int gs_kcore(gs_graph *g, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
              int *rounds);
 
+/* Asynchronous BFS, -j 30 (bfs2_t, bfs2.cpp:82-179; driver grid::bfs2,
+ * gstore.cpp:1900-2042; root -a, default 1).  Sweeps relax every stored edge
+ * in place -- d[v1] = d[v0] + 1 when v1 is unreached or deeper by more than
+ * one, and the reverse on undirected grids -- until a sweep changes nothing;
+ * after the first sweep that stores at most 16 p depths (4 p_s directed),
+ * only tiles whose row (or column) changed in the previous sweep are swept.
+ * depth_out: u8, identical to gs_bfs.  sweep_counts[s] = depths stored in
+ * sweep s (the order of stores, hence the sweep count and these counts, varies
+ * between runs as in the reference).  *sweeps = sweeps run, the last one
+ * storing nothing.  Works on semi-external grids too. */
+int gs_bfs_async(gs_graph *g, uint64_t root, uint8_t *depth_out, uint64_t *sweep_counts, int max_sweeps,
+                 int *sweeps);
+
+/* Single colour propagation, -j 4 (traverse_t, traverse.cpp:56-211; driver
+ * grid::traverse, gstore.cpp:2169-2177; pivot -a, default 1, colour 0).
+ * cid starts at 0xFFFFFFFF except cid[pivot] = color; each sweep sets
+ * cid[v1] = color for every stored edge (v0, v1) with cid[v0] == color and
+ * cid[v1] unset -- v0 to v1 only, so on an undirected grid colour spreads
+ * along increasing vertex ids -- until a sweep colours nothing.  cid_out: N
+ * u32; sweep_counts[s] = vertices coloured in sweep s (order-dependent like
+ * gs_bfs_async's); *colored = vertices with the colour.  color may not be
+ * 0xFFFFFFFF. */
+int gs_traverse(gs_graph *g, uint64_t pivot, uint32_t color, uint32_t *cid_out, uint64_t *sweep_counts,
+                int max_sweeps, int *sweeps, uint64_t *colored);
+
+/* Text edge list to u32 pairs, -c 4 (text_to_bin, util.cpp:58-168): leading
+ * lines starting with '%' skipped; a token at the text's start and after every
+ * run of ' ', '\n', '\t'; each token's value as atoi reads it; pairs of
+ * consecutive tokens with the minimum id subtracted, the minimum starting at
+ * 999999 as the reference's does.  Parsed on the device.  Writes out_path
+ * (NULL: text_path + ".edge"), *pairs = tokens / 2, *tokens, *vmin, *vmax. */
+int gs_text_to_bin(const char *text_path, const char *out_path, uint64_t *pairs, uint32_t *vmin, uint32_t *vmax,
+                   uint64_t *tokens);
+/* The same from a text in host memory; out (cap_pairs pairs) receives the
+ * first min(cap_pairs, *pairs) pairs. */
+int gs_text_to_bin_mem(const char *text, uint64_t size, uint32_t *out, uint64_t cap_pairs, uint64_t *pairs,
+                       uint32_t *vmin, uint32_t *vmax, uint64_t *tokens);
+
 /* ---- Serving (no reference counterpart) ------------------------------------
  * Several resident graphs, queries submitted at any time, a pool of worker
  * threads each with its own stream and per-graph workspaces so queries overlap

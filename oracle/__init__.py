@@ -46,6 +46,10 @@ def lib() -> ctypes.CDLL:
         L.or_kron_convert_count.restype = i64
         L.or_kron_convert_scatter.argtypes = [i, i, u64, u64, i, u64, p, p]
         L.or_grid_checksum.argtypes = [u64, p, p]
+        L.or_bfs2.argtypes = [u64, i, p, p, u64, p, p, i]
+        L.or_traverse.argtypes = [u64, i, p, p, u64, ctypes.c_uint32, p, p, i]
+        L.or_text_to_bin.argtypes = [ctypes.c_char_p, u64, p, u64, p, p, p]
+        L.or_text_to_bin.restype = i64
         L.or_bfs_mt.argtypes = [u64, i, p, p, u64, p, p, i, i]
         L.or_wcc_mt.argtypes = [u64, i, p, p, p, p, i]
         L.or_kcore_mt.argtypes = [u64, p, p, p, i, p, p, p, i, p]
@@ -175,6 +179,36 @@ def kcore(n, start, grid, degree, k, nthreads=1, scanned=None):
     if rounds < 0:
         raise RuntimeError("or_kcore failed")
     return deg, int(c[0]), int(c[1]), rounds
+
+
+def bfs2(n, start, grid, root, directed=False):
+    """-j 30: asynchronous BFS -> (depth u8, per-sweep update counts)."""
+    depth = np.empty(n, np.uint8)
+    counts = np.zeros(4096, np.uint64)
+    ns = lib().or_bfs2(n, int(directed), _p(start), _p(grid), root, _p(depth), _p(counts), 4096)
+    if ns < 0:
+        raise RuntimeError(f"or_bfs2 failed ({ns})")
+    return depth, [int(c) for c in counts[:ns]]
+
+
+def traverse(n, start, grid, pivot, color=0, directed=False):
+    """-j 4: single colour propagation -> (cid u32, per-sweep coloured counts)."""
+    cid = np.empty(n, np.uint32)
+    counts = np.zeros(1 << 16, np.uint64)
+    ns = lib().or_traverse(n, int(directed), _p(start), _p(grid), pivot, color, _p(cid), _p(counts), 1 << 16)
+    if ns < 0:
+        raise RuntimeError(f"or_traverse failed ({ns})")
+    return cid, [int(c) for c in counts[:ns]]
+
+
+def text_to_bin(text: bytes):
+    """-c 4: text edge list -> (pairs (M, 2) u32, v_min, v_max, tokens)."""
+    L = lib()
+    vmin, vmax, ntok = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+    m = L.or_text_to_bin(text, len(text), None, 0, ctypes.byref(vmin), ctypes.byref(vmax), ctypes.byref(ntok))
+    out = np.empty((m, 2), np.uint32)
+    L.or_text_to_bin(text, len(text), _p(out), m, ctypes.byref(vmin), ctypes.byref(vmax), ctypes.byref(ntok))
+    return out, vmin.value, vmax.value, ntok.value
 
 
 def ref_generator():

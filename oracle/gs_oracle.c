@@ -26,6 +26,11 @@
  *   or_grid_checksum   -- invariant under in-tile permutation, changed by one
  *                         moved or altered edge
  *   or_*_mt            -- equal to the serial forms (tests), which are pinned
+ *   or_bfs2            -- depths equal scipy breadth-first distances + 1
+ *   or_traverse        -- coloured set = scipy reachability from the pivot over
+ *                         the edges oriented v0 -> v1
+ *   or_text_to_bin     -- hand-written texts with known pairs, and a round trip
+ *                         of random pairs printed with several separators
  *   or_bfs             -- scipy.sparse.csgraph breadth-first distances
  *   or_wcc             -- scipy.sparse.csgraph.connected_components
  *   or_kcore           -- independent numpy peeling in tests + brute force
@@ -900,6 +905,175 @@ int or_kcore_mt(uint64_t n, const uint64_t *start, const uint32_t *grid, const i
     free(rowact);
     walk_free(&w);
     return rounds;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Remaining jobs and tools: -j 30 (bfs2), -j 4 (traverse), -c 4 (text).     */
+/* ------------------------------------------------------------------------ */
+
+/* Asynchronous BFS, -j 30 (bfs2_t, bfs2.cpp:82-179; driver grid::bfs2,
+ * gstore.cpp:1900-2042).  depth u8, 0 = unreached, root = 1.  A sweep visits
+ * the tiles in .start order and relaxes every stored edge in place:
+ * l0 = d[v0], l1 = d[v1]; if l0 != 0 and (l1 == 0 or l1 - l0 > 1) then
+ * d[v1] = l0 + 1, else (undirected) if l1 != 0 and (l0 == 0 or l0 - l1 > 1)
+ * then d[v0] = l1 + 1 (bfs2.cpp:103-118).  A forward update marks the tile
+ * column's row bit in the current and the next read set, a backward one the
+ * tile row's bit in the next set (bfs2.cpp:121-129).  Until a sweep's update
+ * count drops to 16 p (undirected; 4 p_s directed) every tile is visited
+ * (ioskip, bfs2.cpp:161-172); afterwards only tiles whose row -- or, on
+ * undirected grids, column -- bit is set (bfs2.cpp:251-275).  The run stops
+ * at the first sweep without updates.  sweep_counts[s] = updates in sweep s.
+ * The fixed point is the BFS depth of or_bfs.  Returns the number of sweeps,
+ * -2 if a depth beyond 255 would be stored (the reference's u8 wraps). */
+int or_bfs2(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, uint64_t root,
+            uint8_t *depth, uint64_t *sweep_counts, int max_sweeps)
+{
+    walk_t w;
+    if (root >= n) return -1;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    uint64_t ps = 256 * w.p;
+    uint8_t *cur = (uint8_t *)calloc(ps, 1), *nxt = (uint8_t *)calloc(ps, 1);
+    if (!cur || !nxt) { free(cur); free(nxt); walk_free(&w); return -1; }
+    memset(depth, 0, n);
+    depth[root] = 1;
+    int ioskip = 1, sweeps = 0, rc = 0;
+    const uint64_t small = directed ? 4 * ps : 16 * w.p;
+    for (;;) {
+        uint64_t total = 0;
+        for (uint64_t t = 0; t < w.T && !rc; ++t) {
+            uint32_t r = w.row[t], c = w.col[t];
+            if (!(ioskip || cur[r] || (!directed && cur[c]))) continue;
+            uint8_t *d0 = depth + ((uint64_t)r << 16), *d1 = depth + ((uint64_t)c << 16);
+            uint64_t fwd = 0, back = 0;
+            for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                uint32_t a = grid[k] & 0xFFFF, b = grid[k] >> 16;
+                int l0 = d0[a], l1 = d1[b];
+                if (l0 != 0 && (l1 == 0 || l1 - l0 > 1)) {
+                    if (l0 + 1 > 255) { rc = -2; break; }
+                    d1[b] = (uint8_t)(l0 + 1);
+                    ++fwd;
+                } else if (!directed && l1 != 0 && (l0 == 0 || l0 - l1 > 1)) {
+                    if (l1 + 1 > 255) { rc = -2; break; }
+                    d0[a] = (uint8_t)(l1 + 1);
+                    ++back;
+                }
+            }
+            if (fwd) { nxt[c] = 1; cur[c] = 1; }
+            if (back) nxt[r] = 1;
+            total += fwd + back;
+        }
+        if (rc) break;
+        if (sweeps < max_sweeps && sweep_counts) sweep_counts[sweeps] = total;
+        ++sweeps;
+        if (total == 0) break;
+        uint8_t *tmp = cur; cur = nxt; nxt = tmp;
+        memset(nxt, 0, ps);
+        if (total <= small) ioskip = 0;
+    }
+    free(cur);
+    free(nxt);
+    walk_free(&w);
+    return rc ? rc : sweeps;
+}
+
+/* Single colour propagation, -j 4 (traverse_t, traverse.cpp:56-211; driver
+ * grid::traverse, gstore.cpp:2169-2177, and do_algo_prop).  cid starts at
+ * 0xFFFFFFFF (invalid_cid; the reference's calloc path, traverse.cpp:66-69)
+ * with cid[pivot] = color; a sweep visits every tile and, for each stored
+ * edge, sets cid[v1] = color when cid[v0] == color and cid[v1] is invalid
+ * (traverse.cpp:178-186) -- one direction only, also on undirected grids,
+ * where v0 < v1.  Stops at the first sweep that colours nothing.
+ * sweep_counts[s] = vertices coloured in sweep s.  Returns the sweeps run,
+ * -1 on bad arguments (color may not be 0xFFFFFFFF). */
+int or_traverse(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, uint64_t pivot,
+                uint32_t color, uint32_t *cid, uint64_t *sweep_counts, int max_sweeps)
+{
+    walk_t w;
+    if (pivot >= n || color == 0xFFFFFFFFu) return -1;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    for (uint64_t v = 0; v < n; ++v) cid[v] = 0xFFFFFFFFu;
+    cid[pivot] = color;
+    int sweeps = 0;
+    for (;;) {
+        uint64_t total = 0;
+        for (uint64_t t = 0; t < w.T; ++t) {
+            uint32_t *c0 = cid + ((uint64_t)w.row[t] << 16), *c1 = cid + ((uint64_t)w.col[t] << 16);
+            for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                uint32_t a = grid[k] & 0xFFFF, b = grid[k] >> 16;
+                if (c0[a] == color && c1[b] == 0xFFFFFFFFu) { c1[b] = color; ++total; }
+            }
+        }
+        if (sweeps < max_sweeps && sweep_counts) sweep_counts[sweeps] = total;
+        ++sweeps;
+        if (total == 0) break;
+    }
+    walk_free(&w);
+    return sweeps;
+}
+
+/* Text edge list to u32 pairs, -c 4 (text_to_bin, util.cpp:58-168).  Lines
+ * at the very start that begin with '%' are skipped (util.cpp:71-77).  The
+ * scanner takes a token at the start of the remaining text and after every
+ * run of ' ', '\n', '\t' (util.cpp:91-103), so text that starts with one of
+ * those yields its first number twice.  Each token's value is atoi of its
+ * text: leading white space skipped, optional sign, decimal digits, the
+ * strtol result (saturated to the 64-bit range) cut to 32 bits.  v_min starts
+ * at 999999 and v_max at 0 (util.cpp:86-96, Q14); there are tokens/2 pairs
+ * (v0 - v_min, v1 - v_min) (util.cpp:130-161).  Fills out[0 .. 2*pairs) when
+ * out is given (cap_pairs pairs at most) and returns the number of pairs;
+ * *ntok = tokens. */
+static int txt_delim(char c) { return c == ' ' || c == '\n' || c == '\t'; }
+static int txt_space(char c) { return c == ' ' || (c >= '\t' && c <= '\r'); }
+
+static uint32_t txt_atoi(const char *s, uint64_t i, uint64_t size)
+{
+    while (i < size && txt_space(s[i])) ++i;
+    int neg = 0;
+    if (i < size && (s[i] == '+' || s[i] == '-')) { neg = s[i] == '-'; ++i; }
+    uint64_t v = 0;
+    int sat = 0;
+    while (i < size && s[i] >= '0' && s[i] <= '9') {
+        uint64_t d = (uint64_t)(s[i] - '0');
+        if (!sat && v > (0x7FFFFFFFFFFFFFFFull - d) / 10) sat = 1;
+        if (!sat) v = v * 10 + d;
+        ++i;
+    }
+    int64_t r;
+    if (sat) r = neg ? (int64_t)(-0x7FFFFFFFFFFFFFFFll - 1) : 0x7FFFFFFFFFFFFFFFll;
+    else r = neg ? -(int64_t)v : (int64_t)v;
+    return (uint32_t)(uint64_t)r;
+}
+
+int64_t or_text_to_bin(const char *text, uint64_t size, uint32_t *out, uint64_t cap_pairs, uint32_t *vmin,
+                       uint32_t *vmax, uint64_t *ntok)
+{
+    uint64_t h = 0;
+    while (h < size && text[h] == '%') {
+        while (h < size && text[h] != '\n') ++h;
+        if (h < size) ++h;
+    }
+    const char *ss = text + h;
+    uint64_t sz = size - h, tok = 0;
+    uint32_t mn = 999999u, mx = 0;
+    for (uint64_t i = 0; i < sz; ++i) {
+        if (i != 0 && !(txt_delim(ss[i - 1]) && !txt_delim(ss[i]))) continue;
+        uint32_t a = txt_atoi(ss, i, sz);
+        if (a > mx) mx = a;
+        if (a < mn) mn = a;
+        ++tok;
+    }
+    uint64_t pairs = tok / 2, t = 0;
+    if (out) {
+        for (uint64_t i = 0; i < sz && t < 2 * pairs; ++i) {
+            if (i != 0 && !(txt_delim(ss[i - 1]) && !txt_delim(ss[i]))) continue;
+            if (t < 2 * cap_pairs) out[t] = txt_atoi(ss, i, sz) - mn;
+            ++t;
+        }
+    }
+    *vmin = mn;
+    *vmax = mx;
+    *ntok = tok;
+    return (int64_t)pairs;
 }
 
 int or_num_threads(void)

@@ -90,3 +90,45 @@ def test_directed_tool(tmp_path):
     assert got == oracle.bfs(n, start, grid, 3, directed=True)[1]
     out = run("-v", n, "-i", pre, "-j", 2, "-a", 4, exe=GSD)
     assert "Kcore not applicable for directed graphs" in out
+
+
+def test_fast_bfs_default_and_tile_flag(converted):
+    """gstoreu runs the direction-optimizing BFS on a resident undirected grid
+    and prints the index build on its own line; -b 0 keeps the tile sweep.
+    Both print the oracle's level counts."""
+    n, e, pre, d = converted
+    start, grid, deg = oracle.convert(e, n)
+    depth, counts = oracle.bfs(n, start, grid, 7)
+    for flag in ([], ["-b", 0]):
+        out = run("-v", n, "-i", pre, "-j", 0, "-a", 7, *flag, "-p", str(d / "d.bin"))
+        assert [int(m) for m in re.findall(r"level: \d+ frontier count: (\d+)", out)] == counts
+        assert np.array_equal(np.fromfile(str(d / "d.bin"), np.uint8), depth)
+        assert ("Index build time = " in out) == (not flag)
+
+
+def test_async_bfs_and_traverse_jobs(converted):
+    n, e, pre, d = converted
+    start, grid, deg = oracle.convert(e, n)
+    out = run("-v", n, "-i", pre, "-j", 30, "-p", str(d / "d30.bin"))
+    assert "Root = 1" in out and "Half BFS time" in out
+    assert np.array_equal(np.fromfile(str(d / "d30.bin"), np.uint8), oracle.bfs(n, start, grid, 1)[0])
+    sweeps = [int(m) for m in re.findall(r"level: \d+ frontier count: (\d+)", out)]
+    assert sweeps[-1] == 0 and all(sweeps[:-1])
+    out = run("-v", n, "-i", pre, "-j", 4, "-a", 3, "-p", str(d / "c4.bin"))
+    assert "Single color propagation time" in out
+    cid, counts = oracle.traverse(n, start, grid, 3, 0)
+    assert np.array_equal(np.fromfile(str(d / "c4.bin"), np.uint32), cid)
+
+
+def test_text_to_bin_tool(tmp_path):
+    rng = np.random.default_rng(2)
+    e = rng.integers(5, 1 << 18, (20000, 2)).astype(np.uint32)
+    text = b"%%MatrixMarket comment\n" + b"".join(f"{a}\t{b}\n".encode() for a, b in e)
+    p = tmp_path / "g.txt"
+    p.write_bytes(text)
+    r = subprocess.run([GSU, "-i", str(p), "-c", "4"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    want, vmin, vmax, ntok = oracle.text_to_bin(text)
+    got = np.fromfile(str(p) + ".edge", np.uint32).reshape(-1, 2)
+    assert np.array_equal(got, want) and np.array_equal(got, e - vmin)
+    assert f"min vertex id: {vmin}" in r.stderr and f"edge count: {len(e)}" in r.stderr

@@ -414,3 +414,74 @@ def test_kcore_int16_quirk():
     deg, ne0, pos, _ = oracle.kcore(n, start, grid, degree, 4)
     assert deg[0] == deg[1] == -65538
     assert ne0 == 5 and pos == 0   # 0, 1 negative; 2, 3, 4 removed at (2 < 4) and left at -2
+
+
+# ---- -j 30 (bfs2), -j 4 (traverse), -c 4 (text to binary) ---------------------------
+
+@pytest.mark.parametrize("name,n,e", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("directed", [False, True])
+def test_bfs2_reaches_bfs_depths(name, n, e, directed):
+    """The asynchronous sweeps' fixed point is the breadth-first depth."""
+    start, grid, degree = oracle.convert(e, n, directed)
+    A = _adj(n, start, grid, directed)
+    for root in (0, 1, int(np.argmax(degree)), n - 1):
+        depth, counts = oracle.bfs2(n, start, grid, root, directed)
+        assert np.array_equal(depth, _bfs_expected(A, root)), (name, root)
+        assert counts[-1] == 0 and all(c > 0 for c in counts[:-1])
+        assert sum(counts) >= int(np.count_nonzero(depth)) - 1   # every reached vertex was set at least once
+
+
+def test_bfs2_depth_overflow_reported():
+    n, e = graphs.path(300)
+    start, grid, _ = oracle.convert(e, n)
+    with pytest.raises(RuntimeError):
+        oracle.bfs2(n, start, grid, 0)
+
+
+@pytest.mark.parametrize("name,n,e", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("directed", [False, True])
+def test_traverse_colours_the_oriented_reach(name, n, e, directed):
+    """Colour spreads along stored edges v0 -> v1 only: on an undirected grid
+    (v0 < v1) the coloured set is what increasing-id paths reach."""
+    start, grid, degree = oracle.convert(e, n, directed)
+    v0, v1 = tiles.decode(n, start, grid, directed)
+    D = sp.coo_matrix((np.ones(v0.size), (v0.astype(np.int64), v1.astype(np.int64))), shape=(n, n)).tocsr()
+    for pivot in (0, 1, int(np.argmax(degree)), n // 2):
+        cid, counts = oracle.traverse(n, start, grid, pivot, color=7, directed=directed)
+        reach = np.isfinite(csgraph.shortest_path(D, unweighted=True, indices=pivot, directed=True))
+        assert np.array_equal(cid == 7, reach), (name, pivot)
+        assert np.all(cid[~reach] == 0xFFFFFFFF)
+        assert counts[-1] == 0 and sum(counts) == int(reach.sum()) - 1
+
+
+def test_traverse_rejects_invalid_colour():
+    n, e = graphs.path(10)
+    start, grid, _ = oracle.convert(e, n)
+    with pytest.raises(RuntimeError):
+        oracle.traverse(n, start, grid, 0, color=0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("text,pairs,vmin,vmax,ntok", [
+    (b"1 2\n3 4\n", [(0, 1), (2, 3)], 1, 4, 4),                     # ids shift down by the minimum
+    (b"%%MatrixMarket\n% c\n0 5\n5 9\n", [(0, 5), (5, 9)], 0, 9, 4),  # '%' header lines skipped
+    (b"3\t4\r\n5 6\r\n", [(0, 1), (2, 3)], 3, 6, 4),                # tabs, CRLF (\r stays in the token)
+    (b" 5 6\n7\t8", [(0, 0), (1, 2)], 5, 8, 5),                    # leading blank: first number twice
+    (b"2 3\n4", [(0, 1)], 2, 4, 3),                                 # odd token count: last one dropped
+    (b"1000001 1000002\n1000003 1000004\n", [(2, 3), (4, 5)], 999999, 1000004, 4),  # v_min starts at 999999
+    (b"7 -1\n", [(0, 0xFFFFFFF8)], 7, 0xFFFFFFFF, 2),               # atoi of -1 is 0xFFFFFFFF as u32, the maximum
+    (b"", [], 999999, 0, 0),
+])
+def test_text_to_bin_known_answers(text, pairs, vmin, vmax, ntok):
+    got, mn, mx, nt = oracle.text_to_bin(text)
+    assert got.tolist() == [list(p) for p in pairs] and (mn, mx, nt) == (vmin, vmax, ntok)
+
+
+def test_text_to_bin_round_trip():
+    rng = np.random.default_rng(5)
+    e = rng.integers(0, 1 << 20, (5000, 2)).astype(np.uint32)
+    e[17] = (0, 3)                                   # the minimum id is 0: ids come back unchanged
+    seps = [b" ", b"\t", b"  "]
+    ends = [b"\n", b"\r\n", b" \n"]
+    lines = [str(a).encode() + seps[i % 3] + str(b).encode() + ends[i % 3] for i, (a, b) in enumerate(e)]
+    got, mn, mx, nt = oracle.text_to_bin(b"%header\n" + b"".join(lines))
+    assert np.array_equal(got, e) and mn == 0 and mx == int(e.max()) and nt == 2 * len(e)
