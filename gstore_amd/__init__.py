@@ -74,6 +74,12 @@ class _Tune(ctypes.Structure):
                 ("do_alpha", ctypes.c_int32), ("do_beta", ctypes.c_int32), ("do_ms", ctypes.c_double)]
 
 
+class _IoStats(ctypes.Structure):
+    _fields_ = [("disk_bytes", ctypes.c_uint64), ("direct_bytes", ctypes.c_uint64), ("read_ms", ctypes.c_double),
+                ("wait_ms", ctypes.c_double), ("io_threads", ctypes.c_int32), ("staging_buffers", ctypes.c_int32),
+                ("external", ctypes.c_int32), ("read_mode", ctypes.c_int32)]
+
+
 class QueryResult(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("arg", ctypes.c_uint64), ("value", ctypes.c_uint64),
                 ("aux", ctypes.c_uint64), ("checksum", ctypes.c_uint64), ("t_submit_ms", ctypes.c_double),
@@ -101,6 +107,9 @@ _SIGS = {
     "gs_get_info": (ctypes.c_int, [_P, ctypes.POINTER(_Info)]),
     "gs_set_device_budget": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64]),
     "gs_get_arrays": (ctypes.c_int, [_P, _P, _P, _P]),
+    "gs_open_external": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                        ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "gs_get_io_stats": (ctypes.c_int, [_P, ctypes.POINTER(_IoStats)]),
     "gs_pagerank": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P]),
     "gs_ppr": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_int, _P]),
     "gs_bfs": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
@@ -222,6 +231,23 @@ class Graph:
         h = ctypes.c_void_p()
         _check(_lib.gs_open(prefix.encode(), vert_count, int(directed), device_budget, ctypes.byref(h)))
         return cls(h)
+
+    @classmethod
+    def open_external(cls, prefix: str, vert_count: int, device_budget: int, directed: bool = False,
+                      segment_bytes: int = 0) -> "Graph":
+        """External mode: the grid stays in PREFIX.grid and streams from the file
+        every pass (beyond a resident prefix of the device budget)."""
+        load_library()
+        h = ctypes.c_void_p()
+        _check(_lib.gs_open_external(prefix.encode(), vert_count, int(directed), device_budget, segment_bytes,
+                                     ctypes.byref(h)))
+        return cls(h)
+
+    def io_stats(self) -> dict:
+        """File reads of an external-mode graph since it was opened."""
+        st = _IoStats()
+        _check(_lib.gs_get_io_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in _IoStats._fields_}
 
     @classmethod
     def from_arrays(cls, n: int, start: np.ndarray, grid: np.ndarray, degree: np.ndarray,

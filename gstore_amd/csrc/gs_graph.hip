@@ -568,7 +568,23 @@ int gs_open_arrays(uint64_t n, int directed, const uint64_t *start, const uint32
     return GS_OK;
 }
 
+static int open_files(const char *prefix, uint64_t n, int directed, uint64_t budget, bool external, uint64_t seg_bytes,
+                      gs_graph **out);
+
 int gs_open(const char *prefix, uint64_t n, int directed, uint64_t budget, gs_graph **out) {
+    return open_files(prefix, n, directed, budget, false, 0, out);
+}
+
+int gs_open_external(const char *prefix, uint64_t n, int directed, uint64_t budget, uint64_t seg_bytes,
+                     gs_graph **out) {
+    return open_files(prefix, n, directed, budget, true, seg_bytes, out);
+}
+
+// Loads .start and .degree, and the grid: into HBM when it fits the budget;
+// else into pinned host memory with a resident HBM prefix (semi-external), or,
+// external, left in its file with only the prefix read (gs_stream.hip).
+static int open_files(const char *prefix, uint64_t n, int directed, uint64_t budget, bool external, uint64_t seg_bytes,
+                      gs_graph **out) {
     if (!prefix || !out) { set_error("null argument"); return GS_ERR_ARG; }
     gs_graph *g = nullptr;
     GS_TRY(graph_create(&g));
@@ -600,7 +616,9 @@ int gs_open(const char *prefix, uint64_t n, int directed, uint64_t budget, gs_gr
     }
     const bool streaming = budget && g->E * 4 > budget;
     GS_TRY(g->degree.alloc(n));
-    if (streaming) {
+    if (streaming && external) {
+        GS_TRY(disk_setup(g, pre + ".grid", budget, seg_bytes));
+    } else if (streaming) {
         // semi-external: the grid goes to pinned host memory, a prefix to HBM
         uint32_t *h = nullptr;
         GS_HIP(hipHostMalloc((void **)&h, std::max<uint64_t>(g->E, 1) * 4, hipHostMallocDefault));
@@ -747,7 +765,8 @@ int gs_get_arrays(gs_graph *g, uint64_t *start, uint32_t *grid, int32_t *degree)
         memcpy(start, st.data(), (g->T + 1) * 8);
     }
     if (grid && g->E) {
-        if (g->ss) memcpy(grid, g->ss->h_grid, g->E * 4);
+        if (g->ss && g->ss->disk) GS_TRY(disk_read_all(g->ss, grid));
+        else if (g->ss) memcpy(grid, g->ss->h_grid, g->E * 4);
         else GS_HIP(hipMemcpy(grid, g->grid.ptr, g->E * 4, hipMemcpyDeviceToHost));
     }
     if (degree) GS_HIP(hipMemcpy(degree, g->degree.ptr, g->n * 4, hipMemcpyDeviceToHost));
@@ -758,12 +777,14 @@ int gs_save(gs_graph *g, const char *prefix) {
     if (!g || !prefix) { set_error("null argument"); return GS_ERR_ARG; }
     std::vector<uint64_t> st;
     GS_TRY(rebuild_start(g, &st));
-    std::vector<uint32_t> grid(g->ss ? 0 : g->E);
+    const bool host_copy = g->ss && !g->ss->disk;
+    std::vector<uint32_t> grid(host_copy ? 0 : g->E);
     std::vector<int32_t> deg(g->n);
     if (g->E && !g->ss) GS_HIP(hipMemcpy(grid.data(), g->grid.ptr, g->E * 4, hipMemcpyDeviceToHost));
+    if (g->E && g->ss && g->ss->disk) GS_TRY(disk_read_all(g->ss, grid.data()));
     GS_HIP(hipMemcpy(deg.data(), g->degree.ptr, g->n * 4, hipMemcpyDeviceToHost));
     std::string pre(prefix);
-    GS_TRY(write_file(pre + ".grid", g->ss ? g->ss->h_grid : grid.data(), g->E * 4));
+    GS_TRY(write_file(pre + ".grid", host_copy ? g->ss->h_grid : grid.data(), g->E * 4));
     GS_TRY(write_file(pre + ".start", st.data(), (g->T + 1) * 8));
     GS_TRY(write_file(pre + ".degree", deg.data(), g->n * 4));
     return GS_OK;

@@ -166,7 +166,10 @@ namespace gs {
 // res_chunks chunks stay resident in HBM and the rest pass through two
 // device segment buffers, copied on a side stream while the previous
 // segment is being swept.
+struct DiskStage;  // external mode: the grid read from its file every sweep (gs_stream.hip)
+
 struct StreamState {
+    DiskStage *disk = nullptr;   // set: no host copy; the non-resident chunks come from the .grid file
     uint32_t *h_grid = nullptr;  // pinned host copy of the whole grid
     const uint32_t *h_grid_dev = nullptr;  // the same memory as the device addresses it (direct sweeps)
     uint64_t res_chunks = 0;     // chunks [0, res_chunks) resident in gs_graph::grid
@@ -262,6 +265,14 @@ int build_chunk_map(gs_graph *g, const std::vector<uint64_t> &beg);  // chunk_tz
 int ensure_bitmaps(const gs_graph *g, gs_ws *w);
 int ensure_rowact(const gs_graph *g, gs_ws *w);
 int stream_setup(gs_graph *g, uint32_t *h_grid, uint64_t budget, uint64_t seg_bytes);
+// External mode (gs_stream.hip): before a streamed pass, wait for reads left
+// from an earlier pass; per segment i of the pass (edges [e0, e1)), wait for
+// its file read, then copy it to dst on the copy stream, keeping the next
+// segments' reads in flight.
+int disk_pass_begin(StreamState *ss);
+int disk_copy(const gs_graph *g, StreamState *ss, uint64_t i, uint64_t e0, uint64_t e1, uint32_t *dst);
+int disk_setup(gs_graph *g, const std::string &path, uint64_t budget, uint64_t seg_bytes);
+int disk_read_all(const StreamState *ss, uint32_t *dst);   // the whole grid from its file
 uint64_t bitmap_words(const gs_graph *g);  // rows * ROW_WORDS
 int sync_and_collect(gs_ws *w);
 
@@ -518,10 +529,11 @@ int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch, bool direct = fal
     if (ss->res_chunks) GS_TRY(run(g->grid.ptr, 0, 0, ss->res_chunks));
     // direct: a sweep that reads few tiles (a sparse frontier) reads them from
     // the pinned host grid in place, instead of copying every segment over
-    if (direct && ss->h_grid_dev) {
+    if (direct && ss->h_grid_dev) {   // never in external mode (no host copy)
         if (ss->res_chunks < g->nchunk) GS_TRY(run(ss->h_grid_dev, 0, ss->res_chunks, g->nchunk - ss->res_chunks));
         return GS_OK;
     }
+    if (ss->disk && ss->res_chunks < g->nchunk) GS_TRY(disk_pass_begin(ss));
     int k = 0;
     for (uint64_t c0 = ss->res_chunks; c0 < g->nchunk; c0 += ss->seg_chunks, k ^= 1) {
         uint64_t cn = std::min(ss->seg_chunks, g->nchunk - c0);
@@ -529,7 +541,10 @@ int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch, bool direct = fal
         GS_HIP(hipStreamWaitEvent(ss->copy, ss->ev_done[k], 0));
         hipEvent_t cb;
         w->timer.begin(GS_K_H2D, ss->copy, &cb);
-        GS_HIP(hipMemcpyAsync(ss->seg[k].ptr, ss->h_grid + e0, (e1 - e0) * 4, hipMemcpyHostToDevice, ss->copy));
+        if (ss->disk)
+            GS_TRY(disk_copy(g, ss, (c0 - ss->res_chunks) / ss->seg_chunks, e0, e1, ss->seg[k].ptr));
+        else
+            GS_HIP(hipMemcpyAsync(ss->seg[k].ptr, ss->h_grid + e0, (e1 - e0) * 4, hipMemcpyHostToDevice, ss->copy));
         w->timer.end(GS_K_H2D, ss->copy, cb);
         GS_HIP(hipEventRecord(ss->ev_copied[k], ss->copy));
         GS_HIP(hipStreamWaitEvent(w->stream, ss->ev_copied[k], 0));

@@ -125,6 +125,37 @@ int gs_get_info(gs_graph *g, gs_info *info);
  * at a time. */
 int gs_set_device_budget(gs_graph *g, uint64_t budget_bytes, uint64_t segment_bytes);
 
+/* Open with the grid left in its file (external mode; the SSD-streaming
+ * path of the reference: io_driver / read_aio_random, gstore.cpp:3623-3797,
+ * buffers read_aio_init :3044-3119, cross-iteration cache :1491-1768).  When
+ * the grid is larger than device_budget_bytes, the first budget - 2 segments
+ * of it are read once into HBM (the cache, swept first in every pass); every
+ * streamed pass reads the rest from PREFIX.grid with O_DIRECT preads
+ * (through an aligned bounce buffer where the pinned buffers cannot be
+ * O_DIRECT targets, buffered where the file system refuses O_DIRECT) by 4 IO
+ * threads into 3
+ * pinned staging buffers, and copies each staged segment to one of two device
+ * segment buffers while the previous one is swept.  No host copy of the grid
+ * is kept.  segment_bytes 0 = min(256 MiB, budget / 8).  A grid that fits the
+ * budget opens resident as with gs_open. */
+int gs_open_external(const char *prefix, uint64_t vert_count, int directed, uint64_t device_budget_bytes,
+                     uint64_t segment_bytes, gs_graph **out);
+
+/* File reads of an external-mode graph since it was opened (zeros
+ * otherwise): bytes read, of them with O_DIRECT, read time summed over IO
+ * threads, and the time sweeps waited for a segment's reads. */
+typedef struct gs_io_stats {
+    uint64_t disk_bytes;
+    uint64_t direct_bytes;
+    double read_ms;
+    double wait_ms;
+    int32_t io_threads;
+    int32_t staging_buffers;
+    int32_t external;
+    int32_t read_mode;     /* 0 O_DIRECT into pinned buffers, 1 O_DIRECT through a bounce buffer, 2 buffered */
+} gs_io_stats;
+int gs_get_io_stats(gs_graph *g, gs_io_stats *out);
+
 /* Copy the tile arrays back to host buffers (start: T+1, grid: E, degree: N). */
 int gs_get_arrays(gs_graph *g, uint64_t *start, uint32_t *grid, int32_t *degree);
 
