@@ -303,20 +303,64 @@ def serve_mode(args):
         g.close()
 
 
+def pinned_h2d_peak(gb=1):
+    """Host-to-device copy rate of a pinned buffer (hipMemcpyAsync through
+    torch), the roof of the semi-external stream: best of 5 copies of `gb` GiB."""
+    import torch
+    h = torch.empty(gb << 28, dtype=torch.float32, pin_memory=True)
+    d = torch.empty(gb << 28, dtype=torch.float32, device="cuda")
+    best = None
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        d.copy_(h, non_blocking=True)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del h, d
+    torch.cuda.empty_cache()
+    if hasattr(torch._C, "_host_emptyCache"):   # give the pinned block back before the engine pins the grid
+        torch._C._host_emptyCache()
+    return round((gb << 30) / (best / 1e3) / 1e9, 2)
+
+
 def stream_mode(args):
-    """Semi-external: Kron-S grid in pinned host memory with the device tile
-    budget capped; PageRank iterations and BFS roots streamed."""
+    """Semi-external: Kron-S grid in pinned host memory (or, --external, left
+    in a .grid file on disk) with the device tile budget capped; PageRank
+    iterations and BFS roots streamed.  Reports the achieved host-to-device
+    rate beside the measured pinned-copy peak, the compute/transfer overlap
+    and, for the external mode, the file read rate."""
+    # torch (the copy-peak probe) initialises the device before the engine does
+    peak = pinned_h2d_peak()
+    log(f"stream: pinned host-to-device copy peak {peak} GB/s")
     import gstore_amd as ga
     t0 = time.time()
     g = ga.Graph.kron(args.scale)
     log(f"stream: Kron-{args.scale} generated ({time.time() - t0:.1f}s), E={g.E}")
     t0 = time.time()
-    g.set_device_budget(args.budget_gb << 30, args.segment_mb << 20)
-    log(f"stream: moved to pinned host, resident edges {g.resident_edges} of {g.E} ({time.time() - t0:.1f}s)")
+    ext_dir = None
+    if args.external:
+        ext_dir = args.external_dir or os.path.join(os.environ.get("TMPDIR", "/tmp"), f"gstore_ext_{os.getpid()}")
+        os.makedirs(ext_dir, exist_ok=True)
+        pre = os.path.join(ext_dir, f"kron{args.scale}")
+        g.save(pre)
+        n = g.n
+        g.close()
+        log(f"stream: tile files written to {ext_dir} ({time.time() - t0:.1f}s)")
+        t0 = time.time()
+        g = ga.Graph.open_external(pre, n, args.budget_gb << 30, segment_bytes=args.segment_mb << 20)
+        log(f"stream: opened from disk, resident edges {g.resident_edges} of {g.E} ({time.time() - t0:.1f}s), "
+            f"io {g.io_stats()}")
+    else:
+        g.set_device_budget(args.budget_gb << 30, args.segment_mb << 20)
+        log(f"stream: moved to pinned host, resident edges {g.resident_edges} of {g.E} ({time.time() - t0:.1f}s)")
     g.set_kernel_timing(True)
     res = {}
+    io0 = g.io_stats()
     g.pagerank(args.stream_pr_iters, ga.PR_INTENDED, copy_out=False)
     st = g.stats()
+    io1 = g.io_stats()
     k = st["kernels"]
     kern_ms = sum(v["ms"] for n_, v in k.items() if n_ != "h2d_copy")
     h2d_ms = k.get("h2d_copy", {}).get("ms", 0.0)
@@ -330,25 +374,114 @@ def stream_mode(args):
                        "overlap_frac": round(1 - exposed / st["ms"], 4),
                        "copy_hidden_frac": round(1 - exposed / h2d_ms, 4) if h2d_ms else None,
                        "gteps": round(2 * g.E * args.stream_pr_iters / (st["ms"] / 1e3) / 1e9, 3)}
+    res["pagerank"]["h2d_frac_of_pinned_peak"] = round(res["pagerank"]["h2d_gbs_wall"] / peak, 4)
+    if args.external:
+        db = io1["disk_bytes"] - io0["disk_bytes"]
+        res["pagerank"].update(disk_gb=round(db / 1e9, 2), disk_gbs_wall=round(db / (st["ms"] / 1e3) / 1e9, 2),
+                               read_wait_ms=round(io1["wait_ms"] - io0["wait_ms"], 1),
+                               read_mode=["O_DIRECT into pinned", "O_DIRECT via bounce buffer",
+                                          "buffered"][io1["read_mode"]])
     log(f"stream: pagerank {res['pagerank']}")
     deg = g.degree()
     roots = [int(r) for r in np.random.default_rng(5).choice(np.flatnonzero(deg), args.stream_bfs_roots, replace=False)]
     bfs = []
     for r in roots:
-        d = g.bfs(r, copy_out=False)
+        io0 = g.io_stats()
+        g.bfs(r, copy_out=False)
         st = g.stats()
-        bfs.append({"root": r, "levels": st["iterations"], "ms": round(st["ms"], 1),
-                    "h2d_gb": round(st["h2d_bytes"] / 1e9, 2)})
+        io1 = g.io_stats()
+        e = {"root": r, "levels": st["iterations"], "ms": round(st["ms"], 1), "h2d_gb": round(st["h2d_bytes"] / 1e9, 2)}
+        if args.external:
+            e["disk_gb"] = round((io1["disk_bytes"] - io0["disk_bytes"]) / 1e9, 2)
+        bfs.append(e)
+        log(f"stream: bfs {e}")
     res["bfs"] = bfs
+    mode = "external (grid read from disk every pass)" if args.external else "semi-external (grid in pinned host memory)"
     out = {"metric": "semi-external PageRank GTEPS, H2D GB/s and overlap", "value": res["pagerank"]["gteps"],
            "unit": "GTEPS", "n_gpus": 1, "higher_is_better": True,
            "data": "synthetic (Graph500 Kronecker, seeds 1,2)",
-           "config": {"workload": f"Kron-{args.scale} semi-external, device tile budget {args.budget_gb} GB",
+           "config": {"workload": f"Kron-{args.scale} {mode}, device tile budget {args.budget_gb} GB",
                       "scale": args.scale, "stored_edges": g.E, "resident_edges": g.resident_edges,
                       "segment_edges": g.segment_edges},
-           **res}
+           "pinned_h2d_peak_gbs": peak, **res}
     print(json.dumps(out), flush=True)
     g.close()
+    if ext_dir and not args.external_dir:
+        for f in os.listdir(ext_dir):
+            os.remove(os.path.join(ext_dir, f))
+        os.rmdir(ext_dir)
+
+
+def jobs_mode(args):
+    """The remaining jobs on Kron-S resident: -j 30 asynchronous BFS from root
+    1, -j 4 colour propagation from pivot 1 (device time, sweeps); and -c 4
+    text to binary on the text of Kron-<text-scale>'s tuples: device parse
+    rate (text bytes per second, the host-to-device copy of the text
+    included) beside the oracle's serial parse of the same text on the host."""
+    import gstore_amd as ga
+    import oracle
+    out = {"metric": "remaining jobs: -j 30 / -j 4 device ms, -c 4 text GB/s", "n_gpus": 1,
+           "data": "synthetic (Graph500 Kronecker, seeds 1,2)", "config": {"workload": f"Kron-{args.scale} resident"}}
+    with ga.Graph.kron(args.scale) as g:
+        deg = g.degree().astype(np.int64)
+        for label, fn in (("bfs_async", lambda: g.bfs_async(1)), ("traverse", lambda: g.traverse(1, 0))):
+            fn()
+            r = fn()
+            st = g.stats()
+            rec = {"ms": round(st["ms"], 2), "sweeps": st["iterations"]}
+            if label == "bfs_async":
+                d = r[0]
+                m = int(deg[d > 0].sum() // 2)
+                rec.update(reached=int(np.count_nonzero(d)), gteps_m_reach=round(m / (st["ms"] / 1e3) / 1e9, 2))
+            else:
+                rec.update(colored=r[2])
+            out[label] = rec
+            log(f"jobs: {label} {rec}")
+    ts = args.text_scale
+    e = oracle.kron(ts, 0, 16 << ts)
+    text = b"% Kronecker tuples\n" + "\n".join(f"{a} {b}" for a, b in e).encode() + b"\n"
+    del e
+    tdir = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"gstore_txt_{os.getpid()}")
+    os.makedirs(tdir, exist_ok=True)
+    path = os.path.join(tdir, "edges.txt")
+    with open(path, "wb") as f:
+        f.write(text)
+    ga.text_to_bin(text[:1 << 20])       # warm the device
+    t = time.perf_counter()
+    r = ga.text_file_to_bin(path)
+    dev_s = time.perf_counter() - t
+    t = time.perf_counter()
+    want = oracle.text_to_bin(text)
+    cpu_s = time.perf_counter() - t
+    got = np.fromfile(path + ".edge", np.uint32).reshape(-1, 2)
+    assert np.array_equal(got, want[0]), "text conversion differs from the oracle"
+    out["text_to_bin"] = {"text_bytes": len(text), "pairs": r["pairs"],
+                          "device_s": round(dev_s, 3), "device_gbs": round(len(text) / dev_s / 1e9, 2),
+                          "host_serial_s": round(cpu_s, 3), "host_serial_gbs": round(len(text) / cpu_s / 1e9, 3),
+                          "note": "device time includes reading the file and writing the .edge file"}
+    log(f"jobs: text_to_bin {out['text_to_bin']}")
+    for f_ in os.listdir(tdir):
+        os.remove(os.path.join(tdir, f_))
+    os.rmdir(tdir)
+    print(json.dumps(out), flush=True)
+
+
+def copy_mode(args):
+    """Reference copy for counter calibration: 5 device-to-device copies of 1 GiB.
+    Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE each dispatch should read and
+    write 1 GiB; the ratio to what the counters report scales kernel traffic."""
+    import torch
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / 5
+    print(json.dumps({"metric": "device copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                      "ms": round(dt * 1e3, 3), "gbs": round(2 * (1 << 30) / dt / 1e9, 1)}), flush=True)
 
 
 def _pmc_dispatches(path):
@@ -441,24 +574,6 @@ def traffic_mode(args):
     print(json.dumps(out, indent=1), flush=True)
 
 
-def copy_mode(args):
-    """Reference copy for counter calibration: 5 device-to-device copies of 1 GiB.
-    Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE each dispatch should read and
-    write 1 GiB; the ratio to what the counters report scales kernel traffic."""
-    import torch
-    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t) / 5
-    print(json.dumps({"metric": "device copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
-                      "ms": round(dt * 1e3, 3), "gbs": round(2 * (1 << 30) / dt / 1e9, 1)}), flush=True)
-
-
 def level_method_bytes(n_act_words, lv):
     """Bytes one direction-optimizing BFS level touches by the engine's own
     counters: 4 per list entry examined (gs_level_stat.edges), 16 per frontier
@@ -473,7 +588,8 @@ def level_method_bytes(n_act_words, lv):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["main", "serve", "stream", "copy", "traffic"], default="main")
+    ap.add_argument("--mode", choices=["main", "serve", "stream", "copy", "traffic", "jobs"], default="main")
+    ap.add_argument("--text-scale", type=int, default=20, help="jobs mode: Kronecker scale of the -c 4 text")
     ap.add_argument("--serve-scales", default="18,20,22,24,26,28")
     ap.add_argument("--serve-rates", default="5,20,80")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
@@ -483,6 +599,8 @@ def main():
     ap.add_argument("--segment-mb", type=int, default=512)
     ap.add_argument("--stream-pr-iters", type=int, default=10)
     ap.add_argument("--stream-bfs-roots", type=int, default=4)
+    ap.add_argument("--external", action="store_true", help="stream mode: keep the grid in a file on disk")
+    ap.add_argument("--external-dir", default=None, help="stream mode --external: where the tile files go")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -522,6 +640,8 @@ def main():
         return copy_mode(args)
     if args.mode == "traffic":
         return traffic_mode(args)
+    if args.mode == "jobs":
+        return jobs_mode(args)
 
     import torch
     import gstore_amd as ga
