@@ -169,3 +169,37 @@ def test_streamed_matches_oracle(big):
         assert crc(deg) == rec["deg_crc32"]
         r = h.pagerank(20, ga.PR_COMPAT)
         assert crc(r) == fx["pagerank_compat"]["crc32"]
+
+
+def test_config1_kron20_against_full_oracle():
+    """The plain CPU reference configuration (Kron-20, edge factor 16): PageRank
+    5 iterations, BFS from root 1, WCC, k-core k = 4 -- every output array
+    against the oracle run on the host over the same tuples.  Compat PageRank
+    and all integer outputs bit for bit; intended PageRank within 1e-6
+    max-abs and 1e-4 relative of FP64, its L1 reported."""
+    s, threads = 20, min(16, len(os.sched_getaffinity(0)))
+    n = 1 << s
+    start, grid, deg = oracle.convert(oracle.kron(s, 0, 16 << s), n)
+    with ga.Graph.kron(s) as g:
+        gs_, gg, gd = g.arrays()
+        assert np.array_equal(gs_, start) and np.array_equal(gd, deg)
+        assert oracle.grid_checksum(gs_, gg) == oracle.grid_checksum(start, grid)
+        assert np.array_equal(g.pagerank(5, ga.PR_COMPAT), oracle.pagerank(n, start, grid, deg, 5, 0, nthreads=threads))
+        got = g.pagerank(5, ga.PR_INTENDED)
+        w64 = oracle.pagerank64(n, start, grid, deg, 5, nthreads=threads)
+        d = np.abs(got.astype(np.float64) - w64)
+        nz = w64 > 0
+        print(f"Kron-20 intended PageRank vs FP64: max-abs {d.max():.3g}, max-rel {np.max(d[nz] / w64[nz]):.3g}, "
+              f"L1 {d.sum():.3g}", flush=True)
+        assert d.max() <= 1e-6 and np.max(d[nz] / w64[nz]) <= 1e-4
+        want_d, want_c = oracle.bfs(n, start, grid, 1, nthreads=threads)
+        for mode in (0, 1):
+            g.set_bfs_mode(mode)
+            r = g.bfs(1)
+            assert np.array_equal(r.depth, want_d) and r.level_counts == want_c
+        lab, nc = g.wcc()
+        olab, onc = oracle.wcc(n, start, grid, nthreads=threads)
+        assert np.array_equal(lab, olab) and nc == onc
+        kd, ne0, pos, rounds = g.kcore(4)
+        okd, one0, opos, orounds = oracle.kcore(n, start, grid, deg, 4)
+        assert np.array_equal(kd, okd) and (ne0, pos, rounds) == (one0, opos, orounds)
