@@ -82,7 +82,7 @@ class _IoStats(ctypes.Structure):
 
 class QueryResult(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("arg", ctypes.c_uint64), ("value", ctypes.c_uint64),
-                ("aux", ctypes.c_uint64), ("checksum", ctypes.c_uint64), ("t_submit_ms", ctypes.c_double),
+                ("aux", ctypes.c_uint64), ("reserved", ctypes.c_uint64), ("t_submit_ms", ctypes.c_double),
                 ("t_start_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double), ("graph", ctypes.c_int32),
                 ("kind", ctypes.c_int32), ("status", ctypes.c_int32), ("batch_size", ctypes.c_int32)]
 

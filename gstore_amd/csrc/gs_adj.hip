@@ -450,6 +450,9 @@ __global__ void __launch_bounds__(256) k_do_level_end(const unsigned long long *
     __shared__ unsigned long long bbase, bcnt, bdeg;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (threadIdx.x == 0) bcnt = bdeg = 0;
+    // before any wave adds into them: without the listing variant's barriers a
+    // wave could finish its words and add before the clear, losing its count
+    __syncthreads();
     unsigned long long cnt = 0, msum = 0;
     for (uint64_t base = blockIdx.x * (256ull * LE_WPT); base < nwords; base += (uint64_t)gridDim.x * 256 * LE_WPT) {
         const uint64_t g0 = base + threadIdx.x * LE_WPT;

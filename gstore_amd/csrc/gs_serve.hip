@@ -178,13 +178,6 @@ static void make_room(gs_server *s, Worker *self, uint64_t want) {
     }
 }
 
-static uint64_t fnv(const void *p, size_t n) {
-    const unsigned char *c = (const unsigned char *)p;
-    uint64_t h = 1469598103934665603ull;
-    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
-    return h;
-}
-
 static void worker_main(Worker *wk) {
     gs_server *s = wk->srv;
     std::vector<uint64_t> roots;
@@ -292,7 +285,6 @@ static void worker_main(Worker *wk) {
                         for (int l = 0; l < levels && l < 256; ++l) reached += counts[l];
                         res[0].value = reached;
                         res[0].aux = (uint64_t)levels;
-                        if (batch[0].out) res[0].checksum = fnv(batch[0].out, g->n);
                     }
                     break;
                 }
@@ -310,30 +302,24 @@ static void worker_main(Worker *wk) {
                         for (int l = 0; l < levels && l < 256; ++l) reached += counts[(size_t)i * 256 + l];
                         res[i].value = reached;
                         res[i].aux = (uint64_t)levels;
-                        if (batch[i].out) {
-                            if (hipMemcpyAsync(batch[i].out, w->ms_depth.ptr + (size_t)i * g->n, g->n,
-                                               hipMemcpyDeviceToHost, w->stream) != hipSuccess ||
-                                hipStreamSynchronize(w->stream) != hipSuccess)
-                                status = GS_ERR_DEVICE;
-                            else
-                                res[i].checksum = fnv(batch[i].out, g->n);
-                        }
+                        if (batch[i].out &&
+                            (hipMemcpyAsync(batch[i].out, w->ms_depth.ptr + (size_t)i * g->n, g->n,
+                                            hipMemcpyDeviceToHost, w->stream) != hipSuccess ||
+                             hipStreamSynchronize(w->stream) != hipSuccess))
+                            status = GS_ERR_DEVICE;
                     }
                     break;
                 }
                 case GS_Q_PPR:
                     status = run_ppr(g, w, batch[0].arg, s->ppr_iters, (float *)batch[0].out);
-                    if (status == GS_OK && batch[0].out) res[0].checksum = fnv(batch[0].out, g->n * 4);
                     break;
                 case GS_Q_PAGERANK:
                     status = run_pagerank(g, w, s->pr_iters, (int)batch[0].arg, (float *)batch[0].out);
-                    if (status == GS_OK && batch[0].out) res[0].checksum = fnv(batch[0].out, g->n * 4);
                     break;
                 case GS_Q_WCC: {
                     uint64_t nc = 0;
                     status = run_wcc(g, w, (uint32_t *)batch[0].out, &nc);
                     res[0].value = nc;
-                    if (status == GS_OK && batch[0].out) res[0].checksum = fnv(batch[0].out, g->n * 4);
                     break;
                 }
                 default:

@@ -270,7 +270,7 @@ typedef struct gs_query_result {
     uint64_t arg;        /* BFS root / PPR source / PageRank mode */
     uint64_t value;      /* BFS: reached vertices; WCC: component count */
     uint64_t aux;        /* BFS: levels of its batch */
-    uint64_t checksum;   /* FNV-1a of `out` when given */
+    uint64_t reserved;   /* 0 */
     double t_submit_ms, t_start_ms, t_done_ms;  /* server clock */
     int32_t graph, kind, status, batch_size;
 } gs_query_result;
@@ -362,8 +362,8 @@ int gs_set_batch_bfs_method(gs_graph *g, int method);
  * counts and rounds are identical.  Semi-external grids use 0. */
 int gs_set_kcore_method(gs_graph *g, int method);
 
-/* Work split of the blocked PageRank: source-major entries (<= 2^20) and
- * runs (<= 2048) per gather workgroup, stream values per accumulation
+/* Work split of the blocked PageRank: source-major entries (accepted up to
+ * 2^20, used up to 2^16) and runs (<= 2048) per gather workgroup, stream values per accumulation
  * workgroup (<= 2^30), the stream values per vertex from which a destination
  * block adds with LDS float atomics rather than compare-and-swap, and the
  * compaction threshold in permille (0..1000): a destination block whose

@@ -148,7 +148,7 @@ def test_poisson_mix_over_six_graphs(fleet):
         elif kind == ga.Q_BFS:
             fx = next(b for b in rec["fx"]["bfs"] if b["root"] == arg)
             assert crc(out) == fx["depth_crc32"], (s, arg)
-            assert r["value"] == fx["reached"]
+            assert r["value"] == fx["reached"], (s, arg, r["batch_size"], r["value"], fx["reached"])
             checked["fixture"] += 1
         elif kind == ga.Q_WCC:
             assert crc(out) == rec["fx"]["wcc"]["label_crc32"] and r["value"] == rec["fx"]["wcc"]["ncomp"]
