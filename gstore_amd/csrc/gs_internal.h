@@ -40,6 +40,7 @@ constexpr uint32_t PB_KMAX = 8192;           // blocks at most (vertices with ed
 constexpr uint32_t PB_RUNS = 2048;           // runs staged per gather workgroup at most
 constexpr unsigned long long PB_COMPACT = 1ull << 63;   // run stream base: the run's block is compacted
 constexpr uint32_t PB_WIN = 16;              // entries per window: every run starts on a window and fills whole ones
+constexpr uint32_t PB_UNIT_WIN = 4096;       // gather: windows per unit at most (2^16 entries)
 struct PbSlice { uint64_t e0, e1; uint32_t q, pad; };               // build: list entries of block q
 struct PbUnit1 { uint64_t w0, w1; uint32_t r0, r1, p, pad; };       // gather: source-major windows of block p
 struct PbUnit2 { uint64_t e0, e1; uint32_t q, slot, dense, pad; };  // accum: slot ~0u = the block's only unit

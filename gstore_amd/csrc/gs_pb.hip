@@ -277,6 +277,7 @@ __global__ void k_pb_mark(uint16_t *__restrict__ src, const uint16_t *__restrict
 // ---- per iteration -----------------------------------------------------------
 
 constexpr int PB_WPL = 4;   // gather: windows per lane per step, all loads issued first
+constexpr uint32_t PB_CHUNKS = PB_UNIT_WIN / 64;   // gather: 64-window chunks of a unit
 
 // prior[pbase .. pbase + pn) into LDS with 16-byte loads, two per thread in
 // flight at a time (pbase is a multiple of 2^15, so the loads are aligned).
@@ -351,40 +352,43 @@ __device__ inline void pb_segments(const pb_u32x4 (&sq)[2], const float *pri, fl
 }
 
 // The same for windows with up to 16 segments, a quarter (4 entries) at a
-// time: a quarter's segments land in 4 slots (40 selects per window, where 16
-// slots over the whole window take 136) and leave at the quarter's place,
-// after the segments of the quarters before it.  The running sum carries over
-// quarters, so a segment may span them.
+// time.  The running sums a0..a3 (reset after each mark) are formed first;
+// the quarter's sums, in order, are then picked by the marks themselves (7
+// selects per quarter, where a slot per segment count took 40 selects and
+// compares per window) and leave at the quarter's place, after the segments
+// of the quarters before it.  The running sum carries over quarters, so a
+// segment may span them; each value is the same FP32 sum in the same order as
+// one accumulator over the window gives.
 __device__ inline void pb_segments_q(const pb_u32x4 (&sq)[2], const float *pri, float *out) {
     float acc = 0.0f;
     uint32_t base = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        uint32_t seg = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int i = 4 * q + j;
-            const uint32_t s = (sq[i >> 3][(i >> 1) & 3] >> ((i & 1) * 16)) & 0xFFFFu;
-            acc += pri[s & 0x7FFFu];
-            const bool e = s >> 15;
-            s0 = (e && seg == 0) ? acc : s0;
-            if (j >= 1) s1 = (e && seg == 1) ? acc : s1;
-            if (j >= 2) s2 = (e && seg == 2) ? acc : s2;
-            if (j >= 3) s3 = (e && seg == 3) ? acc : s3;
-            seg += e;
-            acc = e ? 0.0f : acc;
-        }
+        const uint32_t w0 = sq[q >> 1][(q & 1) * 2], w1 = sq[q >> 1][(q & 1) * 2 + 1];
+        const uint32_t s0 = w0 & 0xFFFFu, s1 = w0 >> 16, s2 = w1 & 0xFFFFu, s3 = w1 >> 16;
+        const bool e0 = (s0 >> 15) != 0, e1 = (s1 >> 15) != 0, e2 = (s2 >> 15) != 0, e3 = (s3 >> 15) != 0;
+        const float a0 = acc + pri[s0 & 0x7FFFu];
+        const float a1 = (e0 ? 0.0f : a0) + pri[s1 & 0x7FFFu];
+        const float a2 = (e1 ? 0.0f : a1) + pri[s2 & 0x7FFFu];
+        const float a3 = (e2 ? 0.0f : a2) + pri[s3 & 0x7FFFu];
+        acc = e3 ? 0.0f : a3;
+        // k-th marked sum of the quarter: o0 first, o1 second, o2 third, a3 fourth
+        const float y = e2 ? a2 : a3;
+        const float x = e1 ? a1 : y;
+        const float o0 = e0 ? a0 : x;
+        const float o1 = e0 ? x : (e1 ? y : a3);
+        const float o2 = (e0 && e1) ? y : a3;
+        const uint32_t seg = (uint32_t)e0 + (uint32_t)e1 + (uint32_t)e2 + (uint32_t)e3;
         float *o = out + base;
         if (seg == 4) {
-            const pb_f32x4 a = {s0, s1, s2, s3};
-            __builtin_memcpy(o, &a, 16);
+            const pb_f32x4 v = {o0, o1, o2, a3};
+            __builtin_memcpy(o, &v, 16);
         } else {
             if (seg & 2) {
-                const float2 a = {s0, s1};
-                __builtin_memcpy(o, &a, 8);
+                const float2 v = {o0, o1};
+                __builtin_memcpy(o, &v, 8);
             }
-            if (seg & 1) o[seg & 2] = (seg & 2) ? s2 : s0;
+            if (seg & 1) o[seg & 2] = (seg & 2) ? o2 : o0;
         }
         base += seg;
     }
@@ -393,16 +397,19 @@ __device__ inline void pb_segments_q(const pb_u32x4 (&sq)[2], const float *pri, 
 // Values of the windows [w0, w1) of source block p (counted in source-major
 // order) into the accumulation stream; the unit's runs are staged in LDS.
 // Lane i of the workgroup takes windows w0 + i, w0 + i + PB_WG, ... (adjacent
-// lanes mostly share a run); a window's run comes from an 11-step search over
-// the staged run starts.  A run of a plain destination block writes
+// lanes mostly share a run).  A wave's 64 windows form one 64-window chunk of
+// the unit: a window's run is the run holding the chunk's first window plus
+// the runs starting in the chunk at or before it, a popcount of the chunk's
+// run-start bits under the lane's mask (an 11-step search over the staged run
+// starts had cost a fifth of the kernel's VALU work).  A run of a plain destination block writes
 // val[e] = prior[p*PB_B + src[e]] for each of its 16 entries (four 16-byte
 // stores); a run of a compacted block writes one sum per segment of equal
 // destinations (bit 15 of a source offset marks each segment's last entry,
 // padding follows the last mark) at the window's place in the compacted stream.
 // A window's sums collect in registers and leave in vector stores (one scalar
 // store per segment made the gather 30% slower at Kron-28): pb_segments with
-// two or four slots when no window of the wave has more segments (the
-// heaviest blocks), else pb_segments_q a quarter window at a time.
+// two slots when no window of the wave has more segments (the heaviest
+// blocks), else pb_segments_q a quarter window at a time.
 //
 // Work split: with grab == 0 each workgroup takes one unit; otherwise one
 // workgroup per CU stays resident and takes `grab` consecutive units at a time
@@ -420,14 +427,18 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
                                                      const float *__restrict__ prior, uint64_t nzn,
                                                      float *__restrict__ val) {
     __shared__ float4 pri4[PB_B / 4 + 1];   // + the zero read by padding entries (offset PB_B)
-    // per staged run: first window in the unit (< 2^16: a unit has at most 2^20
-    // entries), first q-major window (< 2^32, checked at build) and stream offset
+    // per staged run: first window in the unit (< PB_UNIT_WIN), first q-major
+    // window (< 2^32, checked at build) and stream offset
     __shared__ uint16_t rs[PB_RUNS];
     __shared__ uint32_t rq[PB_RUNS];
     __shared__ uint64_t rt[PB_RUNS];
+    __shared__ unsigned long long cbits[PB_CHUNKS];   // runs starting inside each 64-window chunk (bit 0 unused)
+    __shared__ uint16_t ck0[PB_CHUNKS];               // the run holding each chunk's first window
     __shared__ uint32_t s_grab;
     if (threadIdx.x == 0) pri4[PB_B / 4] = make_float4(0.f, 0.f, 0.f, 0.f);
     const float *pri = reinterpret_cast<const float *>(pri4);
+    const uint32_t lane = threadIdx.x & 63;   // window w of the unit sits in lane w mod 64 of some wave
+    const unsigned long long lmask = lane == 63 ? ~0ull : ((2ull << lane) - 1);
     uint32_t ui = grab ? 0 : blockIdx.x, uend = grab ? 0 : blockIdx.x + 1;
     uint32_t staged = 0xFFFFFFFFu;
     for (;;) {
@@ -460,7 +471,15 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
             // plain runs: the stream follows the windows; compacted: wcoff is relative to the run
             rt[k] = (t & PB_COMPACT) ? t : t + (s - r0) * PB_WIN;
         }
-        const uint32_t len = (uint32_t)(u.w1 - u.w0);
+        const uint32_t len = (uint32_t)(u.w1 - u.w0);   // <= PB_CHUNKS * 64 (the unit tables are built so)
+        for (uint32_t c = threadIdx.x; c < PB_CHUNKS; c += PB_WG) cbits[c] = 0;
+        __syncthreads();
+        // runs are non-empty and adjacent, rs[0] = 0: run k holds windows [rs[k], rs[k+1])
+        for (int k = threadIdx.x; k < nr; k += PB_WG) {
+            const uint32_t s = rs[k], e = k + 1 < nr ? rs[k + 1] : len;
+            if (s & 63) atomicOr(&cbits[s >> 6], 1ull << (s & 63));
+            for (uint32_t c = (s + 63) >> 6; (c << 6) < e; ++c) ck0[c] = (uint16_t)k;
+        }
         __syncthreads();
         for (uint32_t wb = threadIdx.x; wb < len; wb += PB_WPL * PB_WG) {
             uint64_t qw[PB_WPL], st[PB_WPL];
@@ -470,10 +489,7 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
             for (int j = 0; j < PB_WPL; ++j) {
                 // past the unit, a lane reloads its first window and stores nothing
                 const uint32_t w = wb + j * PB_WG < len ? wb + j * PB_WG : wb;
-                int k = 0;
-#pragma unroll
-                for (int step = PB_RUNS / 2; step > 0; step >>= 1)
-                    if (k + step < nr && rs[k + step] <= w) k += step;
+                const int k = ck0[w >> 6] + __popcll(cbits[w >> 6] & lmask);
                 qw[j] = (uint64_t)rq[k] + (w - rs[k]);
                 st[j] = rt[k] & PB_COMPACT ? rt[k] : rt[k] + (uint64_t)(w - rs[k]) * PB_WIN;
                 // plain loads: nontemporal ones made the gather 11% slower at Kron-28
@@ -487,15 +503,11 @@ __global__ void __launch_bounds__(PB_WG) k_pb_gather(const PbUnit1 *__restrict__
                 if (wb + j * PB_WG >= len) break;
                 if (st[j] & PB_COMPACT) {
                     float *out = val + (st[j] & ~PB_COMPACT) + co[j];
-                    uint32_t n = 0;
+                    uint32_t n = 0;   // segments: the marks of the window
 #pragma unroll
-                    for (int h = 0; h < 8; ++h) {
-                        const uint32_t a = sq[j][h >> 2][h & 3];
-                        n += ((a >> 15) & 1u) + (a >> 31);
-                    }
-                    // register slots for the wave's largest segment count (wave-uniform branches)
-                    if (__ballot(n > 4)) pb_segments_q(sq[j], pri, out);
-                    else if (__ballot(n > 2)) pb_segments<4>(sq[j], pri, out);
+                    for (int h = 0; h < 8; ++h) n += __popc(sq[j][h >> 2][h & 3] & 0x80008000u);
+                    // two register slots when the wave's windows allow (wave-uniform branch)
+                    if (__ballot(n > 2)) pb_segments_q(sq[j], pri, out);
                     else pb_segments<2>(sq[j], pri, out);
                 } else {
                     float x[PB_WIN];
@@ -827,7 +839,8 @@ int build_blocked(gs_graph *g) {
         rpw.push_back(pw);
     }
     const uint64_t nruns = rqw.size();
-    const uint64_t unit1 = std::max<uint64_t>(g->pb_unit1 / PB_WIN, 1), unit2 = g->pb_unit2;
+    const uint64_t unit1 = std::min<uint64_t>(std::max<uint64_t>(g->pb_unit1 / PB_WIN, 1), PB_UNIT_WIN);
+    const uint64_t unit2 = g->pb_unit2;
     const uint32_t runs_max = g->pb_runs;
     // 4. gather units: source-major window ranges within one source block, at
     //    most pb_unit1 entries and pb_runs runs each
