@@ -328,7 +328,9 @@ class Graph:
     def set_pagerank_method(self, method) -> None:
         """PR_BLOCKED (2, default): two streamed passes over the blocked copy of the
         neighbour-list index; PR_PULL (1): pull over the lists; PR_SCATTER (0):
-        tile scatter with atomics.  True / False mean PR_PULL / PR_SCATTER."""
+        tile scatter with atomics.  On directed and semi-external grids 1 and 2
+        select the binned scatter (no global atomics), 0 the atomic one.
+        True / False mean PR_PULL / PR_SCATTER."""
         _check(_lib.gs_set_pagerank_method(self._h, int(method)))
 
     def set_pagerank_blocking(self, gather_entries: int = 1 << 16, gather_runs: int = 2048,

@@ -153,6 +153,10 @@ struct gs_ws {
     gs::DevBuf<unsigned long long> do_ctl;                    // direction-optimizing BFS: control words, level records
     hipGraphExec_t do_graph = nullptr;                        // its captured level sequence
     std::vector<const void *> do_graph_key;
+    gs::DevBuf<uint32_t> bin_cnt, bin_off, bin_cur, bin_uoff;  // binned scatter: per bin counts, offsets, cursors, units
+    gs::DevBuf<int64_t> bin_pc, bin_pr;                       // per tile of a piece: column / row bin position - first edge
+    gs::DevBuf<uint16_t> bin_o;                               // per contribution: destination offset in its bin
+    gs::DevBuf<float> bin_v;                                  // per contribution: the source's prior
     gs::DevBuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;  // pinned mirror of counters
     gs::KernelTimer timer;
@@ -193,6 +197,8 @@ struct gs_graph {
     gs::DevBuf<uint64_t> tz_beg;
     gs::DevBuf<uint32_t> tz_rc;
     gs::DevBuf<uint32_t> chunk_tz;
+    std::vector<uint32_t> h_chunk_tz;        // host copy of chunk_tz (binned scatter), for chunk size h_chunk_tz_of
+    uint32_t h_chunk_tz_of = 0;
     gs::DevBuf<int32_t> degree;
     gs::DevBuf<uint32_t> in_touch;           // directed grids: bitmap of vertices with an in-edge
     gs::DevBuf<unsigned long long> counters;
@@ -221,7 +227,8 @@ struct gs_graph {
     gs::DevBuf<uint32_t> chunkz;             // vertex whose list holds entry c*PULL_CHUNK, npull + 1
     bool adj_ready = false;                  // every index array above built and complete
     int fail_alloc_at = 0;                   // test hook: the k-th allocation of the next index build fails
-    int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy
+    int pr_method = 2;                       // 0 tile scatter, 1 pull over the lists, 2 blocked copy (binned scatter
+                                             // on directed and semi-external grids for 1 and 2)
     int wcc_method = 1;                      // 0 tile hooking passes, 1 sampled linking over the lists
     int kcore_method = 1;                    // 0 tile scans, 1 peeled lists pushed over the index
     int batch_method = 2;                    // batched BFS: 0 tile sweep, 1 over the index, 2 the same in a hipGraph
@@ -513,9 +520,12 @@ __device__ inline uint32_t gv1(uint32_t w, uint32_t rc) { return (rc << 16) | (w
 // Launch one sweep over every chunk of the grid: one launch when resident;
 // when streaming, the resident prefix first, then each segment after its
 // host-to-device copy, double-buffered so copy k+1 overlaps sweep k.
-// launch(SweepArgs, grid_size, stream) issues the kernel.
+// launch(SweepArgs, grid_size, stream) issues the kernel.  interleave (sweeps
+// whose result does not depend on the order of the chunks, such as the
+// PageRank scatter): the resident prefix is cut into segment-sized pieces
+// launched between the streamed segments, so the copies never wait for it.
 template <class F>
-int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch, bool direct = false) {
+int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch, bool direct = false, bool interleave = false) {
     if (!g->nchunk) return GS_OK;
     auto run = [&](const uint32_t *grid, uint64_t gc0, uint64_t c0, uint64_t cn) -> int {
         hipEvent_t kb;
@@ -527,7 +537,13 @@ int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch, bool direct = fal
     };
     StreamState *ss = g->ss;
     if (!ss) return run(g->grid.ptr, 0, 0, g->nchunk);
-    if (ss->res_chunks) GS_TRY(run(g->grid.ptr, 0, 0, ss->res_chunks));
+    const uint64_t nseg = (g->nchunk - ss->res_chunks + ss->seg_chunks - 1) / ss->seg_chunks;
+    interleave = interleave && !direct && nseg > 0;
+    // resident chunks launched before each streamed segment when interleaving
+    const uint64_t res_per_seg =
+        interleave ? (ss->res_chunks / ss->seg_chunks + nseg) / nseg * ss->seg_chunks : 0;
+    uint64_t res_done = interleave ? 0 : ss->res_chunks;
+    if (!interleave && ss->res_chunks) GS_TRY(run(g->grid.ptr, 0, 0, ss->res_chunks));
     // direct: a sweep that reads few tiles (a sparse frontier) reads them from
     // the pinned host grid in place, instead of copying every segment over
     if (direct && ss->h_grid_dev) {   // never in external mode (no host copy)
@@ -548,11 +564,17 @@ int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch, bool direct = fal
             GS_HIP(hipMemcpyAsync(ss->seg[k].ptr, ss->h_grid + e0, (e1 - e0) * 4, hipMemcpyHostToDevice, ss->copy));
         w->timer.end(GS_K_H2D, ss->copy, cb);
         GS_HIP(hipEventRecord(ss->ev_copied[k], ss->copy));
+        if (res_done < ss->res_chunks) {   // interleaved resident chunks, while the copy runs
+            const uint64_t rn = std::min(res_per_seg, ss->res_chunks - res_done);
+            GS_TRY(run(g->grid.ptr, 0, res_done, rn));
+            res_done += rn;
+        }
         GS_HIP(hipStreamWaitEvent(w->stream, ss->ev_copied[k], 0));
         GS_TRY(run(ss->seg[k].ptr, c0, c0, cn));
         GS_HIP(hipEventRecord(ss->ev_done[k], w->stream));
         w->stats.h2d_bytes += (e1 - e0) * 4;
     }
+    if (res_done < ss->res_chunks) GS_TRY(run(g->grid.ptr, 0, res_done, ss->res_chunks - res_done));
     return GS_OK;
 }
 

@@ -166,6 +166,260 @@ __global__ void k_ppr_finalize(uint64_t n, uint64_t src, float *__restrict__ pri
     }
 }
 
+// ---- Binned scatter (semi-external grids and directed resident grids) ----
+// The tile scatter above adds into acc with global atomics whose addresses,
+// on a grid that streams through HBM, spread over the whole accumulator (a
+// segment spans a few tile rows but every tile column), so each add misses
+// the caches.  The binned scatter takes the grid a piece at a time (one
+// streamed segment, or a segment-sized run of the resident prefix) and:
+//   1. counts the piece's contributions per destination bin -- a tile column
+//      (and, undirected, a tile row): 2^16 destinations (k_bin_count);
+//   2. scans the counts into bin offsets (k_bin_scan);
+//   3. reserves each tile's range in its column bin and row bin, one atomic
+//      per tile and bin with the lanes of a wave that share a bin combined
+//      (k_bin_pos);
+//   4. writes every edge's contributions -- (destination offset in the bin,
+//      prior of the source) -- at its tile's place (k_bin_write, a sweep);
+//   5. sums each bin in LDS, a part of 2^15 (FP32) or 2^14 (FP64)
+//      destinations per workgroup, and adds the part into acc with plain
+//      loads and stores: no other workgroup touches those destinations in
+//      this step (k_bin_acc).
+// Work memory per workspace: 6 bytes per contribution of a piece (two per
+// undirected edge), bin tables and 16 bytes per tile of a piece.
+
+// atomicAdd(&ctr[key], n) for each active lane, lanes of the wave with equal
+// keys combined into one atomic; returns each active lane's old value plus
+// the n of the lanes below it with the same key (its reserved position).
+// Every lane of the wave calls it.
+__device__ inline uint32_t wave_reserve(uint32_t *ctr, uint32_t key, uint32_t n, bool active) {
+    const int lane = threadIdx.x & 63;
+    uint32_t res = 0;
+    unsigned long long todo = __ballot(active);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const unsigned long long m = __ballot(active && key == lk) & todo;
+        const bool mine = (m >> lane) & 1ull;
+        const uint32_t x = mine ? n : 0u;
+        uint32_t incl = x;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        const uint32_t tot = __shfl(incl, 63);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lk], tot);
+        base = __shfl(base, leader);
+        if (mine) res = base + incl - x;
+        todo &= ~m;
+    }
+    return res;
+}
+
+// Edges of tile tz inside the piece [e0, e1) and the first of them.
+__device__ inline uint32_t piece_tile(const uint64_t *__restrict__ tz_beg, uint64_t tz, uint64_t e0, uint64_t e1,
+                                      uint64_t &first) {
+    const uint64_t b = max(tz_beg[tz], e0), e = min(tz_beg[tz + 1], e1);
+    first = b;
+    return e > b ? (uint32_t)(e - b) : 0u;
+}
+
+// cnt[bin] += the piece's contributions to it; tiles t0 .. t0 + nt - 1.
+__global__ void __launch_bounds__(256) k_bin_count(const uint64_t *__restrict__ tz_beg,
+                                                   const uint32_t *__restrict__ tz_rc, uint64_t t0, uint64_t nt,
+                                                   uint64_t e0, uint64_t e1, int directed, uint32_t *cnt) {
+    for (uint64_t b = blockIdx.x * 256ull; b < nt; b += (uint64_t)gridDim.x * 256) {   // wave-uniform bound
+        const uint64_t i = b + threadIdx.x;
+        uint64_t first = 0;
+        const uint32_t n = i < nt ? piece_tile(tz_beg, t0 + i, e0, e1, first) : 0u;
+        const uint32_t rc = i < nt ? tz_rc[t0 + i] : 0u;
+        (void)wave_reserve(cnt, rc & 0xFFFFu, n, n != 0);
+        if (!directed) (void)wave_reserve(cnt, rc >> 16, n, n != 0);
+    }
+}
+
+// off[0 .. nb] = exclusive prefix of cnt (off[nb] = the piece's contributions);
+// cur = off[0 .. nb) (the reservation cursors); uoff[0 .. nb] = exclusive
+// prefix of the bins' work units (a non-empty bin has ceil(cnt / BIN_SLICE)).
+// One workgroup.
+constexpr uint32_t BIN_SLICE = 1u << 18;   // contributions per work unit of k_bin_acc at most
+
+__device__ inline uint32_t bin_units(uint32_t c) { return (c + BIN_SLICE - 1) / BIN_SLICE; }
+
+__global__ void __launch_bounds__(1024) k_bin_scan(const uint32_t *__restrict__ cnt, uint32_t nb, uint32_t *off,
+                                                   uint32_t *cur, uint32_t *uoff) {
+    __shared__ uint32_t part[1024], upart[1024];
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+    uint32_t t = 0, tu = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        t += cnt[b];
+        tu += bin_units(cnt[b]);
+    }
+    part[threadIdx.x] = t;
+    upart[threadIdx.x] = tu;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t y = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+        const uint32_t yu = (int)threadIdx.x >= o ? upart[threadIdx.x - o] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += y;
+        upart[threadIdx.x] += yu;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - t, urun = upart[threadIdx.x] - tu;
+    for (uint32_t b = b0; b < b1; ++b) {
+        off[b] = run;
+        cur[b] = run;
+        uoff[b] = urun;
+        run += cnt[b];
+        urun += bin_units(cnt[b]);
+    }
+    if (threadIdx.x == 1023) {
+        off[nb] = part[1023];
+        uoff[nb] = upart[1023];
+    }
+}
+
+// Per tile of the piece: where its contributions start in the column bin
+// (pc) and the row bin (pr), as position minus the tile's first edge of the
+// piece, so an edge's position is pc + its global edge index.
+__global__ void __launch_bounds__(256) k_bin_pos(const uint64_t *__restrict__ tz_beg,
+                                                 const uint32_t *__restrict__ tz_rc, uint64_t t0, uint64_t nt,
+                                                 uint64_t e0, uint64_t e1, int directed, uint32_t *cur, int64_t *pc,
+                                                 int64_t *pr) {
+    for (uint64_t b = blockIdx.x * 256ull; b < nt; b += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = b + threadIdx.x;
+        uint64_t first = 0;
+        const uint32_t n = i < nt ? piece_tile(tz_beg, t0 + i, e0, e1, first) : 0u;
+        const uint32_t rc = i < nt ? tz_rc[t0 + i] : 0u;
+        const uint32_t c = wave_reserve(cur, rc & 0xFFFFu, n, n != 0);
+        if (n) pc[i] = (int64_t)c - (int64_t)first;
+        if (!directed) {
+            const uint32_t r = wave_reserve(cur, rc >> 16, n, n != 0);
+            if (n) pr[i] = (int64_t)r - (int64_t)first;
+        }
+    }
+}
+
+// Every edge (v0, v1) of the piece's chunks: (v1 mod 2^16, prior[v0]) into
+// v1's column bin and, undirected, (v0 mod 2^16, prior[v1]) into v0's row bin.
+template <bool DIRECTED>
+__global__ void __launch_bounds__(SWEEP_WG) k_bin_write(SweepArgs a, uint64_t t0, const int64_t *__restrict__ pc,
+                                                        const int64_t *__restrict__ pr,
+                                                        const float *__restrict__ prior, uint16_t *bo, float *bv) {
+    __shared__ ChunkTiles s;
+    uint64_t chunk;
+    if (!launch_chunk(a, chunk)) return;
+    const uint32_t len = stage_chunk(a, chunk, s, [](uint32_t) { return true; });
+    const uint64_t cbase = chunk * (uint64_t)a.chunk, tb = a.chunk_tz[chunk] - t0;
+    const uint32_t *g = a.grid + (chunk - a.grid_chunk0) * (uint64_t)a.chunk;
+    const int quads = (int)(a.chunk / (4 * SWEEP_WG));
+#pragma unroll 1
+    for (int q = 0; q < quads; ++q) {
+        const uint32_t off = (uint32_t)(q * SWEEP_WG + threadIdx.x) * 4u;
+        if (off >= len) break;
+        int k = find_tile(s, off);
+        uint32_t w[4];
+        if (off + 4 <= len) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(g + off);
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+            for (int i = 0; i < 4; ++i) w[i] = (off + i < len) ? g[off + i] : 0u;
+        }
+        int kl = -1;
+        int64_t bc = 0, br = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t e = off + i;
+            if (e >= len) break;
+            while (s.beg[k + 1] <= e) ++k;
+            if (k != kl) {
+                bc = pc[tb + k];
+                if (!DIRECTED) br = pr[tb + k];
+                kl = k;
+            }
+            const uint32_t rc = s.rc[k], v0 = gv0(w[i], rc), v1 = gv1(w[i], rc);
+            const int64_t ge = (int64_t)(cbase + e);
+            const uint64_t p1 = (uint64_t)(bc + ge);
+            bo[p1] = (uint16_t)(v1 & 0xFFFFu);
+            bv[p1] = unmark(prior[v0]);
+            if (!DIRECTED) {
+                const uint64_t p0 = (uint64_t)(br + ge);
+                bo[p0] = (uint16_t)(v0 & 0xFFFFu);
+                bv[p0] = unmark(prior[v1]);
+            }
+        }
+    }
+}
+
+__device__ inline void lds_add(float *p, float x) {
+    unsigned *a = reinterpret_cast<unsigned *>(p);
+    unsigned old = *a, assumed;
+    do {
+        assumed = old;
+        old = atomicCAS(a, assumed, __float_as_uint(__uint_as_float(assumed) + x));
+    } while (old != assumed);
+}
+__device__ inline void lds_add(double *p, double x) {
+    unsigned long long *a = reinterpret_cast<unsigned long long *>(p);
+    unsigned long long old = *a, assumed;
+    do {
+        assumed = old;
+        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(__longlong_as_double((long long)assumed) + x));
+    } while (old != assumed);
+}
+
+// Workgroup (unit, part): a unit is a slice of at most BIN_SLICE of one bin's
+// contributions; the slice's contributions to its part of the destinations
+// are summed in LDS (eight loads in flight per thread), then added into acc:
+// with plain loads and stores when the bin is one slice (no other workgroup
+// touches those destinations in this step), else with atomics for the
+// non-zero sums (the few bins of a segment's own tile rows).  Workgroups
+// past the piece's units leave at once.
+template <class A>
+__global__ void __launch_bounds__(1024) k_bin_acc(const uint32_t *__restrict__ off, const uint32_t *__restrict__ uoff,
+                                                  uint32_t nb, const uint16_t *__restrict__ bo,
+                                                  const float *__restrict__ bv, uint64_t n, A *acc) {
+    constexpr uint32_t SH = sizeof(A) == 4 ? 15 : 14, PS = 1u << SH, PARTS = 1u << (16 - SH);
+    __shared__ A sum[PS];
+    const uint32_t u = blockIdx.x / PARTS, part = blockIdx.x % PARTS;
+    if (u >= uoff[nb]) return;   // the same in every thread of the workgroup
+    uint32_t lo = 0, hi = nb - 1;   // the bin: largest b with uoff[b] <= u
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (uoff[mid] <= u) lo = mid; else hi = mid - 1;
+    }
+    const uint32_t bin = lo, slice = u - uoff[bin], whole = uoff[bin + 1] - uoff[bin] == 1;
+    const uint32_t i0 = off[bin] + slice * BIN_SLICE, i1 = min(off[bin + 1], i0 + BIN_SLICE);
+    const uint64_t vbase = ((uint64_t)bin << 16) + (uint64_t)part * PS;
+    if (vbase >= n) return;
+    for (uint32_t i = threadIdx.x; i < PS; i += 1024) sum[i] = A(0);
+    __syncthreads();
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += 8 * 1024) {
+        uint32_t o[8];
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t x = i + j * 1024;
+            o[j] = x < i1 ? (uint32_t)bo[x] : 0xFFFFFFFFu;
+            v[j] = x < i1 ? bv[x] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if ((o[j] >> SH) == part) atomicAdd(&sum[o[j] & (PS - 1)], (A)v[j]);
+    }
+    __syncthreads();
+    const uint32_t m = (uint32_t)min<uint64_t>(PS, n - vbase);
+    for (uint32_t j = threadIdx.x; j < m; j += 1024) {
+        const A x = sum[j];
+        if (x == A(0)) continue;
+        if (whole) acc[vbase + j] += x;
+        else unsafeAtomicAdd(&acc[vbase + j], x);
+    }
+}
+
 static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
 
 static bool use_pull(const gs_graph *g);
@@ -215,12 +469,107 @@ int ensure_heavy(gs_graph *g, hipStream_t st, bool blocked) {
 }
 
 static bool use_pull(const gs_graph *g) { return g->pr_method >= 1 && !g->directed && !g->ss; }
+static bool use_binned(const gs_graph *g) { return g->pr_method >= 1 && (g->directed || g->ss); }
+
+// Chunks per piece of the binned scatter: a streamed segment, else 2^25 edges.
+static uint64_t bin_piece(const gs_graph *g) {
+    return g->ss ? g->ss->seg_chunks : std::max<uint64_t>(1, (1ull << 25) / g->chunk);
+}
+
+// The binned scatter's host copy of chunk_tz and its work buffers, sized for
+// the largest piece the sweep hands it (the resident prefix and the segments
+// are cut at the same piece boundaries).
+static int ensure_bins(gs_graph *g, gs_ws *w) {
+    const uint64_t piece = bin_piece(g);
+    {
+        std::lock_guard<std::mutex> lk(g->build_mu);
+        if (g->h_chunk_tz_of != g->chunk || g->h_chunk_tz.size() != g->nchunk + 1) {
+            g->h_chunk_tz.resize(g->nchunk + 1);
+            GS_HIP(hipMemcpy(g->h_chunk_tz.data(), g->chunk_tz.ptr, (g->nchunk + 1) * 4, hipMemcpyDeviceToHost));
+            g->h_chunk_tz_of = g->chunk;
+        }
+    }
+    uint64_t tiles = 1;
+    auto span = [&](uint64_t c0, uint64_t c1) {
+        for (uint64_t c = c0; c < c1; c += piece) {
+            const uint64_t ce = std::min(c1, c + piece);
+            tiles = std::max<uint64_t>(tiles, (uint64_t)g->h_chunk_tz[ce] - g->h_chunk_tz[c] + 1);
+        }
+    };
+    const uint64_t res = g->ss ? g->ss->res_chunks : g->nchunk;
+    span(0, res);
+    span(res, g->nchunk);
+    const uint64_t contrib = (g->directed ? 1 : 2) * std::min<uint64_t>(piece * g->chunk, g->E);
+    if (contrib >= (1ull << 32)) {
+        set_error("binned scatter: %llu contributions per piece exceed 2^32", (unsigned long long)contrib);
+        return GS_ERR_UNSUPPORTED;
+    }
+    const uint64_t nb = g->p_s;
+    if (w->bin_cnt.n != nb) {
+        GS_TRY(w->bin_cnt.alloc(nb));
+        GS_TRY(w->bin_off.alloc(nb + 1));
+        GS_TRY(w->bin_cur.alloc(nb));
+        GS_TRY(w->bin_uoff.alloc(nb + 1));
+    }
+    if (w->bin_pc.n < tiles) {
+        GS_TRY(w->bin_pc.alloc(tiles));
+        if (!g->directed) GS_TRY(w->bin_pr.alloc(tiles));
+    }
+    if (w->bin_o.n < contrib) {
+        GS_TRY(w->bin_o.alloc(contrib));
+        GS_TRY(w->bin_v.alloc(contrib));
+    }
+    return GS_OK;
+}
+
+template <class A>
+static int scatter_binned(const gs_graph *gc, gs_ws *w, A *acc) {
+    gs_graph *g = const_cast<gs_graph *>(gc);
+    GS_TRY(ensure_bins(g, w));
+    const uint64_t piece = bin_piece(g), nb = g->p_s;
+    constexpr uint32_t PARTS = sizeof(A) == 4 ? 2 : 4;
+    hipError_t err = hipSuccess;
+    GS_TRY(sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned, hipStream_t st) {
+        for (uint64_t c = a.chunk0; c < a.chunk0 + a.nlaunch; c += piece) {
+            const uint64_t cn = std::min(piece, a.chunk0 + a.nlaunch - c);
+            const uint64_t e0 = c * g->chunk, e1 = std::min(g->E, (c + cn) * g->chunk);
+            const uint64_t t0 = g->h_chunk_tz[c], nt = (uint64_t)g->h_chunk_tz[c + cn] - t0 + 1;
+            const unsigned tg = (unsigned)std::min<uint64_t>((nt + 255) / 256, 8192);
+            if (err == hipSuccess) err = hipMemsetAsync(w->bin_cnt.ptr, 0, nb * 4, st);
+            k_bin_count<<<tg, 256, 0, st>>>(a.tz_beg, a.tz_rc, t0, nt, e0, e1, (int)g->directed, w->bin_cnt.ptr);
+            k_bin_scan<<<1, 1024, 0, st>>>(w->bin_cnt.ptr, (uint32_t)nb, w->bin_off.ptr, w->bin_cur.ptr,
+                                           w->bin_uoff.ptr);
+            k_bin_pos<<<tg, 256, 0, st>>>(a.tz_beg, a.tz_rc, t0, nt, e0, e1, (int)g->directed, w->bin_cur.ptr,
+                                          w->bin_pc.ptr, w->bin_pr.ptr);
+            SweepArgs pa = a;
+            pa.chunk0 = c;
+            pa.nlaunch = cn;
+            const unsigned wg = (unsigned)(g->xcd_order ? xcd_grid(cn) : cn);
+            if (g->directed)
+                k_bin_write<true><<<wg, SWEEP_WG, 0, st>>>(pa, t0, w->bin_pc.ptr, w->bin_pr.ptr, w->pr_prior.ptr,
+                                                           w->bin_o.ptr, w->bin_v.ptr);
+            else
+                k_bin_write<false><<<wg, SWEEP_WG, 0, st>>>(pa, t0, w->bin_pc.ptr, w->bin_pr.ptr, w->pr_prior.ptr,
+                                                            w->bin_o.ptr, w->bin_v.ptr);
+            // units: at most one per bin plus one per BIN_SLICE contributions
+            const uint64_t units = nb + (2 * (e1 - e0) + BIN_SLICE - 1) / BIN_SLICE;
+            k_bin_acc<A><<<(unsigned)(units * PARTS), 1024, 0, st>>>(w->bin_off.ptr, w->bin_uoff.ptr, (uint32_t)nb,
+                                                                     w->bin_o.ptr, w->bin_v.ptr, g->n, acc);
+        }
+    }, false, true));
+    if (err != hipSuccess) {
+        set_error("binned scatter: %s", hipGetErrorString(err));
+        return GS_ERR_DEVICE;
+    }
+    return GS_OK;
+}
 
 // One accumulation pass: pull over the neighbour lists when the graph has
 // them, else the tile scatter (with the heavy-vertex kernel when there is a
 // heavy set).
 template <class A>
 static int scatter_t(const gs_graph *g, gs_ws *w, A *acc) {
+    if (use_binned(g)) return scatter_binned(g, w, acc);
     if (g->nheavy && !g->directed) {
         return sweep(g, w, GS_K_PR_SCATTER, [&](const SweepArgs &a, unsigned, hipStream_t st) {
             unsigned grid = (unsigned)std::min<uint64_t>(a.nlaunch, (uint64_t)g->persistent_wgs);

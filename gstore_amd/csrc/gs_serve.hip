@@ -151,6 +151,8 @@ static void trim(gs_ws *w, int kind) {
     }
     if (kind != GS_Q_PPR && kind != GS_Q_PAGERANK) {
         w->pr_prior.release(); w->pr_acc.release(); w->pr_fac.release();
+        w->bin_cnt.release(); w->bin_off.release(); w->bin_cur.release(); w->bin_uoff.release(); w->bin_pc.release(); w->bin_pr.release();
+        w->bin_o.release(); w->bin_v.release();
     }
     if (kind != GS_Q_PPR) {
         w->ppr_rank.release();

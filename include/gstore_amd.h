@@ -331,7 +331,12 @@ int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta);
  *   (pr.cpp:119-154).
  * Personalised PageRank sums in FP64 with either (the blocked accumulation
  * then covers half a destination block per pass).  Directed and
- * semi-external grids always scatter.  Only the FP32 summation order differs. */
+ * semi-external grids scatter over the tiles: 1 and 2 select the binned
+ * scatter -- per piece of the grid (a streamed segment), each edge's
+ * contributions are written into bins of 2^16 destinations (its tile's row
+ * and column) at places reserved per tile, and each bin is summed in LDS and
+ * added to the ranks without global atomics -- and 0 the atomic scatter.
+ * Only the FP32 summation order differs. */
 int gs_set_pagerank_method(gs_graph *g, int method);
 
 /* WCC method on undirected resident grids: 1 (default) links over the
