@@ -288,13 +288,17 @@ __global__ void __launch_bounds__(256) k_ms_bu_heavy(uint64_t nheavy, unsigned l
 // Level end over relabelled ids 0 .. nzn-1: next becomes the frontier; per-lane
 // counts of newly reached vertices; frontier size and degree sum (count[0],
 // count[1]) and the degree sum of vertices still missing lanes (count[3]).
-// BUILD lists the new frontier (queue, qdeg; cursor count[2]); DEPTH writes
-// depth rows.
+// BUILD lists the new frontier (queue, qdeg; cursor count[2]), eight vertices
+// per thread per step so a workgroup reserves its queue slots with one atomic
+// per 2048 vertices (one per 256 had serialised ~470K atomics on the cursor
+// at Kron-28: 5.7 ms for the level that lists a 100M-vertex frontier); DEPTH
+// writes depth rows.
 template <bool BUILD, bool DEPTH>
 __device__ inline void ms_level_end(uint64_t nzn, unsigned long long all, unsigned long long *vis,
                                     unsigned long long *front, unsigned long long *next, const int32_t *__restrict__ deg,
                                     uint8_t *depth_vm, uint32_t newdepth, uint32_t *queue, int32_t *qdeg,
                                     unsigned long long *count, unsigned long long *lane_counts) {
+    constexpr int VPT = BUILD ? 8 : 1;
     __shared__ uint32_t cnt[64];
     __shared__ uint32_t wsum[4];
     __shared__ unsigned long long bbase, bsum[3];
@@ -303,52 +307,72 @@ __device__ inline void ms_level_end(uint64_t nzn, unsigned long long all, unsign
     if (threadIdx.x < 3) bsum[threadIdx.x] = 0;
     uint32_t my = 0;
     unsigned long long nf = 0, mf = 0, mu = 0;
-    for (uint64_t base = blockIdx.x * 256ull; base < nzn; base += (uint64_t)gridDim.x * 256) {
-        const uint64_t v = base + threadIdx.x;
-        unsigned long long nx = 0;
-        int32_t dv = 0;
-        if (v < nzn) {
-            nx = next[v];
-            front[v] = nx;
-            const unsigned long long vs = vis[v];
-            dv = deg[v];
-            if (nx) {
-                next[v] = 0;
-                vis[v] = vs | nx;
-                nf += 1;
-                mf += (uint32_t)dv;
-                if (DEPTH) {
-                    unsigned long long mm = nx;
-                    while (mm) {
-                        const int l = __ffsll((long long)mm) - 1;
-                        mm &= mm - 1;
-                        depth_vm[v * 64 + l] = (uint8_t)newdepth;
+    for (uint64_t base = blockIdx.x * (256ull * VPT); base < nzn; base += (uint64_t)gridDim.x * 256 * VPT) {
+        unsigned long long nxs[VPT], vss[VPT];
+        int32_t dvs[VPT];
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {   // all loads first: the stores below may not pass them
+            const uint64_t v = base + k * 256ull + threadIdx.x;
+            nxs[k] = v < nzn ? next[v] : 0ull;
+            vss[k] = v < nzn ? vis[v] : 0ull;
+            dvs[k] = v < nzn ? deg[v] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+            const uint64_t v = base + k * 256ull + threadIdx.x;
+            const unsigned long long nx = nxs[k];
+            const int32_t dv = dvs[k];
+            if (v < nzn) {
+                front[v] = nx;
+                const unsigned long long vs = vss[k];
+                if (nx) {
+                    next[v] = 0;
+                    vis[v] = vs | nx;
+                    nf += 1;
+                    mf += (uint32_t)dv;
+                    if (DEPTH) {
+                        unsigned long long mm = nx;
+                        while (mm) {
+                            const int l = __ffsll((long long)mm) - 1;
+                            mm &= mm - 1;
+                            depth_vm[v * 64 + l] = (uint8_t)newdepth;
+                        }
                     }
                 }
+                if (((vs | nx) & all) != all) mu += (uint32_t)dv;
             }
-            if (((vs | nx) & all) != all) mu += (uint32_t)dv;
-        }
-        if (__ballot(nx != 0)) {
-            for (int l = 0; l < 64; ++l) {
-                const uint32_t c = (uint32_t)__popcll(__ballot((nx >> l) & 1ull));
-                if (lane == l) my += c;
+            if (__ballot(nx != 0)) {
+                for (int l = 0; l < 64; ++l) {
+                    const uint32_t cl = (uint32_t)__popcll(__ballot((nx >> l) & 1ull));
+                    if (lane == l) my += cl;
+                }
             }
+            c += nx != 0;
         }
         if (BUILD) {
-            const unsigned long long mask = __ballot(nx != 0);
-            if (lane == 0) wsum[wv] = (uint32_t)__popcll(mask);
+            uint32_t incl = c;   // this thread's slots after the lanes below it
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            if (lane == 63) wsum[wv] = incl;
             __syncthreads();
             if (threadIdx.x == 0) {
                 const uint32_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
                 bbase = t ? atomicAdd(&count[2], (unsigned long long)t) : 0ull;
             }
             __syncthreads();
-            if (nx) {
-                uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-                for (int j = 0; j < wv; ++j) before += wsum[j];
-                queue[bbase + before] = (uint32_t)v;
-                qdeg[bbase + before] = dv;
-            }
+            uint64_t pos = bbase + incl - c;
+            for (int j = 0; j < wv; ++j) pos += wsum[j];
+#pragma unroll
+            for (int k = 0; k < VPT; ++k)
+                if (nxs[k]) {
+                    queue[pos] = (uint32_t)(base + k * 256ull + threadIdx.x);
+                    qdeg[pos] = dvs[k];
+                    ++pos;
+                }
             __syncthreads();
         }
     }
@@ -694,20 +718,19 @@ __global__ void __launch_bounds__(256) k_msg_bu(unsigned long long *ctl, uint64_
     else ms_bu_light(v0, v1, ctl[C_BUL], off, nbr, first, vis, front, next, ctl + C_CHECKED);
 }
 
-template <bool DEPTH>
+template <bool DEPTH, bool BUILD>
 __global__ void __launch_bounds__(256) k_msg_level_end(unsigned long long *ctl, uint64_t nzn, unsigned long long *vis,
                                                        unsigned long long *front, unsigned long long *next,
                                                        const int32_t *__restrict__ deg, uint8_t *depth_vm,
                                                        uint32_t *queue, int32_t *qdeg) {
     const unsigned long long f = ctl[C_FLAGS];
-    if (f & F_DONE) return;
+    // the listing and the plain level end are separate kernels (the listing
+    // one needs more registers); the one that does not match the level returns
+    if ((f & F_DONE) || ((f & F_HAVEQ) != 0) != BUILD) return;
     const uint32_t level = (uint32_t)ctl[C_LEVEL];
     const unsigned long long all = ctl[C_ACTIVE];   // as the host loop: the lanes still searching
     unsigned long long *lanes = ctl + C_LANES + 64ull * (level - 1);
-    if (f & F_HAVEQ)
-        ms_level_end<true, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
-    else
-        ms_level_end<false, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
+    ms_level_end<BUILD, DEPTH>(nzn, all, vis, front, next, deg, depth_vm, level + 1, queue, qdeg, ctl, lanes);
 }
 
 // Closes a level: lanes still active, entries examined, and the stop test
@@ -896,12 +919,17 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
             if (nzn > nh)
                 k_msg_bu<false><<<glight, 256, 0, w->stream>>>(ctl, nh, nzn, g->adj_off.ptr, g->adj_nbr.ptr,
                                                                           g->adj_first.ptr, vis, front, next);
-            if (dvm)
-                k_msg_level_end<true><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr, dvm,
-                                                                 w->queue.ptr, w->qdeg.ptr);
-            else
-                k_msg_level_end<false><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr, dvm,
-                                                                  w->queue.ptr, w->qdeg.ptr);
+            if (dvm) {
+                k_msg_level_end<true, true><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr, dvm,
+                                                                       w->queue.ptr, w->qdeg.ptr);
+                k_msg_level_end<true, false><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr,
+                                                                        dvm, w->queue.ptr, w->qdeg.ptr);
+            } else {
+                k_msg_level_end<false, true><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr,
+                                                                        dvm, w->queue.ptr, w->qdeg.ptr);
+                k_msg_level_end<false, false><<<gv, 256, 0, w->stream>>>(ctl, nzn, vis, front, next, g->deg_r.ptr,
+                                                                         dvm, w->queue.ptr, w->qdeg.ptr);
+            }
             k_msg_finish<<<1, 64, 0, w->stream>>>(ctl);
         }
         const hipError_t launch = hipGetLastError();
