@@ -175,9 +175,9 @@ __global__ void k_ppr_finalize(uint64_t n, uint64_t src, float *__restrict__ pri
 //   1. counts the piece's contributions per destination bin -- a tile column
 //      (and, undirected, a tile row): 2^16 destinations (k_bin_count);
 //   2. scans the counts into bin offsets (k_bin_scan);
-//   3. reserves each tile's range in its column bin and row bin, one atomic
-//      per tile and bin with the lanes of a wave that share a bin combined
-//      (k_bin_pos);
+//   3. reserves each tile's range in its column bin and row bin: an atomic
+//      per tile for the column (a wave's tiles mostly differ there), one per
+//      wave for each row it holds (k_bin_pos);
 //   4. writes every edge's contributions -- (destination offset in the bin,
 //      prior of the source) -- at its tile's place (k_bin_write, a sweep);
 //   5. sums each bin in LDS, a part of 2^15 (FP32) or 2^14 (FP64)
@@ -234,7 +234,8 @@ __global__ void __launch_bounds__(256) k_bin_count(const uint64_t *__restrict__ 
         uint64_t first = 0;
         const uint32_t n = i < nt ? piece_tile(tz_beg, t0 + i, e0, e1, first) : 0u;
         const uint32_t rc = i < nt ? tz_rc[t0 + i] : 0u;
-        (void)wave_reserve(cnt, rc & 0xFFFFu, n, n != 0);
+        // a wave's tiles span a few tile rows but many columns: rows combined, columns one atomic each
+        if (n) atomicAdd(&cnt[rc & 0xFFFFu], n);
         if (!directed) (void)wave_reserve(cnt, rc >> 16, n, n != 0);
     }
 }
@@ -294,8 +295,7 @@ __global__ void __launch_bounds__(256) k_bin_pos(const uint64_t *__restrict__ tz
         uint64_t first = 0;
         const uint32_t n = i < nt ? piece_tile(tz_beg, t0 + i, e0, e1, first) : 0u;
         const uint32_t rc = i < nt ? tz_rc[t0 + i] : 0u;
-        const uint32_t c = wave_reserve(cur, rc & 0xFFFFu, n, n != 0);
-        if (n) pc[i] = (int64_t)c - (int64_t)first;
+        if (n) pc[i] = (int64_t)atomicAdd(&cur[rc & 0xFFFFu], n) - (int64_t)first;
         if (!directed) {
             const uint32_t r = wave_reserve(cur, rc >> 16, n, n != 0);
             if (n) pr[i] = (int64_t)r - (int64_t)first;
