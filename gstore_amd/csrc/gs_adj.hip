@@ -125,7 +125,8 @@ enum : unsigned long long {
     DF_OVERFLOW = 32, // the next depth would not fit in u8
     DF_ULIST = 64,    // bottom-up over the unreached list
     DF_BUFULL = 128,  // bottom-up over all vertices with edges
-    DF_LEVEL = 256    // a level runs (its level end)
+    DF_LEVEL = 256,   // a level runs (its level end)
+    DF_TDNV = 512     // top-down without the visited test (few vertices reached yet)
 };
 constexpr int DO_LEVELS = 4;
 constexpr unsigned DO_TD_GRID = 2048;
@@ -142,12 +143,16 @@ __device__ inline bool do_gate(const unsigned long long *ctl, unsigned long long
 // edges (grid-stride: the grid is fixed when the levels are captured).  The
 // queue entries its range touches are staged in LDS (offset into the range
 // and list start), each lane finds its entry by a search there.  A neighbour
-// not yet visited or marked is marked with atomicOr.
+// not yet visited or marked is marked with atomicOr.  While few vertices are
+// reached (DF_TDNV: the levels after the root's, where a frontier of hubs
+// sends hundreds of millions of edges to unvisited vertices) the visited test
+// is left out -- one random read per edge fewer -- and the level end drops
+// the visited vertices from the marks.
 constexpr uint32_t TD_CHUNK = 4096;
 
-__device__ inline void td_visit(uint32_t v, const uint32_t *__restrict__ vis, uint32_t *next) {
+__device__ inline void td_visit(uint32_t v, const uint32_t *__restrict__ vis, uint32_t *next, bool test_vis) {
     const uint32_t wi = v >> 5, bit = 1u << (v & 31);
-    if (!(vis[wi] & bit) && !(next[wi] & bit)) atomicOr(&next[wi], bit);
+    if ((!test_vis || !(vis[wi] & bit)) && !(next[wi] & bit)) atomicOr(&next[wi], bit);
 }
 
 __global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl, const uint32_t *__restrict__ queue,
@@ -155,6 +160,7 @@ __global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl,
                                                   const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
                                                   uint32_t *next) {
     if (!do_gate(ctl, DF_TD)) return;
+    const bool test_vis = !(ctl[DO_FLAGS] & DF_TDNV);
     __shared__ uint32_t sbeg[TD_CHUNK + 2];  // first edge of staged entry k, relative to the range (clamped)
     __shared__ uint64_t sadj[TD_CHUNK + 1];  // list start of staged entry k
     __shared__ uint64_t j_first;
@@ -194,7 +200,7 @@ __global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl,
                 int mid = (lo + hi + 1) >> 1;
                 if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
             }
-            td_visit(nbr[sadj[lo] + (t - sbeg[lo])], vis, next);
+            td_visit(nbr[sadj[lo] + (t - sbeg[lo])], vis, next, test_vis);
         }
         __syncthreads();  // the staged tables are rewritten by the next step
     }
@@ -461,15 +467,18 @@ __global__ void __launch_bounds__(256) k_do_level_end(const unsigned long long *
         for (int k = 0; k < LE_WPT; ++k) nx[k] = 0;
         if (g0 + LE_WPT <= nwords) {
             if (LEVEL_END) {
-                const uint4 q = *reinterpret_cast<const uint4 *>(next + g0);
-                nx[0] = q.x; nx[1] = q.y; nx[2] = q.z; nx[3] = q.w;
-                *reinterpret_cast<uint4 *>(front + g0) = q;
+                // marks of visited vertices (left by a top-down level without the
+                // visited test) are dropped here
+                uint4 q = *reinterpret_cast<const uint4 *>(next + g0);
                 if (q.x | q.y | q.z | q.w) {
                     *reinterpret_cast<uint4 *>(next + g0) = uint4{0, 0, 0, 0};
                     uint4 vv = *reinterpret_cast<const uint4 *>(vis + g0);
+                    q.x &= ~vv.x; q.y &= ~vv.y; q.z &= ~vv.z; q.w &= ~vv.w;
                     vv.x |= q.x; vv.y |= q.y; vv.z |= q.z; vv.w |= q.w;
                     *reinterpret_cast<uint4 *>(vis + g0) = vv;
                 }
+                nx[0] = q.x; nx[1] = q.y; nx[2] = q.z; nx[3] = q.w;
+                *reinterpret_cast<uint4 *>(front + g0) = q;
             } else {
                 const uint4 q = *reinterpret_cast<const uint4 *>(front + g0);
                 nx[0] = q.x; nx[1] = q.y; nx[2] = q.z; nx[3] = q.w;
@@ -481,11 +490,12 @@ __global__ void __launch_bounds__(256) k_do_level_end(const unsigned long long *
                 if (gw >= nwords) break;
                 if (LEVEL_END) {
                     nx[k] = next[gw];
-                    front[gw] = nx[k];
                     if (nx[k]) {
                         next[gw] = 0;
+                        nx[k] &= ~vis[gw];
                         vis[gw] |= nx[k];
                     }
+                    front[gw] = nx[k];
                 } else {
                     nx[k] = front[gw];
                 }
@@ -579,6 +589,7 @@ __global__ void k_do_begin(unsigned long long *ctl, uint64_t n, uint64_t nzn) {
         else if (unreached) nfl |= DF_BUFULL;
     } else if (nf && mf) {
         nfl |= DF_TD | (f & DF_HAVEQ ? 0 : DF_LIST);
+        if (reached < nzn / 64) nfl |= DF_TDNV;
     }
     ctl[DO_NFB] = nf;
     ctl[DO_NQ] = nf;

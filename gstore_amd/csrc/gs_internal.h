@@ -213,7 +213,7 @@ struct gs_graph {
     int nheavy = 0;
     gs::DevBuf<uint32_t> heavy_ids;          // sorted heavy vertex ids
     int persistent_wgs = 1024;               // workgroups of the persistent PageRank scatter
-    int do_alpha = 14, do_beta = 24;         // top-down <-> bottom-up switch thresholds
+    int do_alpha = 20, do_beta = 24;         // top-down <-> bottom-up switch thresholds
     // neighbour-list index (built on demand), in relabelled ids (descending degree)
     gs::DevBuf<uint32_t> new_id;             // relabelled id of each vertex, N
     gs::DevBuf<uint32_t> old_id;             // original id of each relabelled id, N

@@ -129,7 +129,7 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
             uint32_t roots[NR];
             for (int i = 0; i < NR; ++i)
                 GS_HIP(hipMemcpy(&roots[i], g->old_id.ptr + g->nzn * (2 * i + 1) / (2 * NR), 4, hipMemcpyDeviceToHost));
-            const int alphas[4] = {8, 14, 30, 60}, betas[2] = {24, 64};
+            const int alphas[6] = {10, 14, 20, 30, 45, 60}, betas[2] = {24, 64};
             auto time_roots = [&](double *ms) -> int {
                 *ms = 0;
                 for (int ri = 0; ri < NR; ++ri) {
@@ -142,7 +142,7 @@ extern "C" int gs_tune(gs_graph *g, const char *cache_path, gs_tune_result *out)
             GS_TRY(run_bfs_do(g, g->ws, roots[0], nullptr, nullptr, 0, nullptr));  // warm
             double best_do = 0;
             GS_TRY(time_roots(&best_do));   // the current setting is the one to beat
-            for (int ai = 0; ai < 4; ++ai)
+            for (int ai = 0; ai < 6; ++ai)
                 for (int bi = 0; bi < 2; ++bi) {
                     if (alphas[ai] == ba && betas[bi] == bb) continue;
                     g->do_alpha = alphas[ai];

@@ -313,7 +313,7 @@ void gs_server_destroy(gs_server *s);
  * each level top-down over the frontier's lists or bottom-up (each unreached
  * vertex stops at its first frontier neighbour), switching to bottom-up when
  * the frontier's edges exceed the unreached side's edges / alpha and back when
- * the frontier holds fewer than N / beta vertices (defaults 14, 24; 0 keeps
+ * the frontier holds fewer than N / beta vertices (defaults 20, 24; 0 keeps
  * the current values, e.g. those gs_tune chose).  Depths and level counts are
  * identical in both modes.  Undirected grids only. */
 int gs_set_bfs_mode(gs_graph *g, int mode, int alpha, int beta);
@@ -410,8 +410,9 @@ int gs_set_sweep_order(gs_graph *g, int xcd_contiguous);
 /* Autotuner: times the tile methods of WCC and k-core (full and
  * frontier-skipping sweeps) for each edges-per-workgroup value (1024, 2048,
  * 4096) and both sweep orders, and, on undirected resident grids, the
- * direction-optimizing BFS from two fixed mid-degree roots for each
- * top-down/bottom-up switch pair (alpha, beta) in {8, 14, 30, 60} x {24, 64};
+ * direction-optimizing BFS from eight roots at evenly spaced degree ranks for
+ * each top-down/bottom-up switch pair (alpha, beta) in {10, 14, 20, 30, 45,
+ * 60} x {24, 64}, keeping a pair only when it is at least 2% faster;
  * applies the fastest, and appends it to cache_path (optional) keyed by N, E,
  * non-empty tiles and directedness; a cached entry is applied without timing. */
 typedef struct gs_tune_result {

@@ -5,6 +5,7 @@ engine at full size without the oracle re-running there.
 
     python tests/golden/make_scale_fixtures.py 25     # ~2 min on 8 cores
     python tests/golden/make_scale_fixtures.py 28     # ~1 h on 8 cores, 25 GB of RAM
+    python tests/golden/make_scale_fixtures.py 28 fp64  # adds the FP64 samples to it
 
 Graph: undirected Graph500 Kronecker, edge factor 16, seeds (1, 2), generated
 and converted in chunks by the oracle (or_kron_convert_*).  Results:
@@ -16,10 +17,12 @@ and converted in chunks by the oracle (or_kron_convert_*).  Results:
   * k-core k = 8 (and k = 4): counts, rounds, CRC32 of the final degrees
   * PageRank compat, 20 iterations: CRC32 of the FP32 output (bit-exact at any
     summation order: only degree-1 vertices are non-zero and each has one term)
-  * scale <= 26: FP64 intended PageRank (20 iterations) and FP64 personalised
-    PageRank (20 iterations, one seeded source) on a vertex sample (the 1024
-    highest-degree vertices and 65536 uniform ones), with the FP64 sums over
-    all vertices, in scale_kronS_samples.npz
+  * scale <= 26, or with the argument fp64 (which adds them to an existing
+    fixture and recomputes nothing else): FP64 intended PageRank (20
+    iterations) and FP64 personalised PageRank (20 iterations, one seeded
+    source) on a vertex sample (the 1024 highest-degree vertices and 65536
+    uniform ones), with the FP64 sums over all vertices, in
+    scale_kronS_samples.npz
 """
 import json
 import os
@@ -58,11 +61,36 @@ def log(*a):
     print(f"[{time.strftime('%H:%M:%S')}]", *a, flush=True)
 
 
-def main(scale: int):
+def fp64_samples(scale, n, start, grid, degree, fx, t0):
+    idx = sample_vertices(degree)
+    pr64 = oracle.pagerank64(n, start, grid, degree, 20, nthreads=THREADS)
+    log(f"s{scale}: FP64 PageRank ({time.time() - t0:.0f}s)")
+    src = int(np.random.default_rng(7).choice(np.flatnonzero(degree)))
+    ppr64 = oracle.ppr(n, start, grid, degree, src, 20, nthreads=THREADS)
+    np.savez_compressed(os.path.join(HERE, f"scale_kron{scale}_samples.npz"), idx=idx,
+                        pr64=pr64[idx], ppr64=ppr64[idx])
+    fx["pagerank_intended64"] = {"iterations": 20, "sum": float(pr64.sum()), "sample": int(idx.size)}
+    fx["ppr64"] = {"iterations": 20, "source": src, "sum": float(ppr64.sum()), "sample": int(idx.size)}
+    log(f"s{scale}: FP64 PageRank and PPR from {src} ({time.time() - t0:.0f}s)")
+
+
+def main(scale: int, only_fp64: bool = False):
     t0 = time.time()
     n = 1 << scale
     start, grid, degree = oracle.kron_convert(scale)
     log(f"s{scale}: generated and converted, E={grid.size} ({time.time() - t0:.0f}s)")
+    if only_fp64:
+        path = os.path.join(HERE, f"scale_kron{scale}.json")
+        with open(path) as f:
+            fx = json.load(f)
+        if crc(start) != fx["start_crc32"] or crc(degree) != fx["degree_crc32"]:
+            raise SystemExit(f"s{scale}: graph differs from the fixture's")
+        fp64_samples(scale, n, start, grid, degree, fx, t0)
+        fx["oracle_seconds_fp64"] = round(time.time() - t0, 1)
+        with open(path, "w") as f:
+            json.dump(fx, f, indent=1)
+        log(f"s{scale}: added the FP64 samples to scale_kron{scale}.json")
+        return
     T = start.size - 1
     fx = {"scale": scale, "edge_factor": 16, "seeds": [1, 2], "vertices": n, "stored_edges": int(grid.size),
           "tile_slots": int(T), "nonempty_tiles": int(np.count_nonzero(np.diff(start))),
@@ -92,15 +120,7 @@ def main(scale: int):
     del pr
     log(f"s{scale}: PageRank compat ({time.time() - t0:.0f}s)")
     if scale <= 26:
-        idx = sample_vertices(degree)
-        pr64 = oracle.pagerank64(n, start, grid, degree, 20, nthreads=THREADS)
-        src = int(np.random.default_rng(7).choice(np.flatnonzero(degree)))
-        ppr64 = oracle.ppr(n, start, grid, degree, src, 20, nthreads=THREADS)
-        np.savez_compressed(os.path.join(HERE, f"scale_kron{scale}_samples.npz"), idx=idx,
-                            pr64=pr64[idx], ppr64=ppr64[idx])
-        fx["pagerank_intended64"] = {"iterations": 20, "sum": float(pr64.sum()), "sample": int(idx.size)}
-        fx["ppr64"] = {"iterations": 20, "source": src, "sum": float(ppr64.sum()), "sample": int(idx.size)}
-        log(f"s{scale}: FP64 PageRank and PPR from {src} ({time.time() - t0:.0f}s)")
+        fp64_samples(scale, n, start, grid, degree, fx, t0)
     fx["oracle_seconds"] = round(time.time() - t0, 1)
     with open(os.path.join(HERE, f"scale_kron{scale}.json"), "w") as f:
         json.dump(fx, f, indent=1)
@@ -108,4 +128,4 @@ def main(scale: int):
 
 
 if __name__ == "__main__":
-    main(int(sys.argv[1]) if len(sys.argv) > 1 else 25)
+    main(int(sys.argv[1]) if len(sys.argv) > 1 else 25, len(sys.argv) > 2 and sys.argv[2] == "fp64")
