@@ -55,6 +55,9 @@ def lib() -> ctypes.CDLL:
         L.or_kcore_mt.argtypes = [u64, p, p, p, i, p, p, p, i, p]
         L.or_grid_checksum.restype = u64
         L.or_bfs.argtypes = [u64, i, p, p, u64, p, p, i]
+        L.or_bfs_check.argtypes = [u64, i, p, p, u64, p, i]
+        L.or_bfs_check.restype = i64
+        L.or_pr_step64_sample.argtypes = [u64, i, p, p, p, p, p, u64, i, p]
         L.or_wcc.argtypes = [u64, i, p, p, p, p]
         L.or_kcore.argtypes = [u64, p, p, p, i, p, p, p]
         L.or_total_tiles.argtypes = [u64, i]
@@ -149,6 +152,30 @@ def bfs(n, start, grid, root, directed=False, nthreads=1):
     if nl < 0:
         raise RuntimeError(f"or_bfs failed ({nl})")
     return depth, [int(c) for c in counts[:nl]]
+
+
+def bfs_check(n, start, grid, root, depth, directed=False, nthreads=1) -> int:
+    """Violations of the conditions that fix BFS depths (0: depth is the BFS
+    depth array from root)."""
+    d = np.ascontiguousarray(depth, np.uint8)
+    if d.size != n:
+        raise ValueError("depth array of the wrong size")
+    bad = lib().or_bfs_check(n, int(directed), _p(start), _p(grid), root, _p(d), nthreads)
+    if bad < 0:
+        raise RuntimeError("or_bfs_check failed")
+    return int(bad)
+
+
+def pr_step64_sample(n, start, grid, degree, prior, idx, directed=False, nthreads=1) -> np.ndarray:
+    """One FP64 PageRank step (intended form) from the iterate `prior`,
+    evaluated at the sorted distinct vertices idx."""
+    pr = np.ascontiguousarray(prior, np.float64)
+    ix = np.ascontiguousarray(idx, np.uint32)
+    out = np.empty(ix.size, np.float64)
+    if pr.size != n or lib().or_pr_step64_sample(n, int(directed), _p(start), _p(grid), _p(degree), _p(pr),
+                                                 _p(ix), ix.size, nthreads, _p(out)):
+        raise RuntimeError("or_pr_step64_sample failed")
+    return out
 
 
 def wcc(n, start, grid, directed=False, nthreads=1):

@@ -34,6 +34,13 @@
  *   or_bfs             -- scipy.sparse.csgraph breadth-first distances
  *   or_wcc             -- scipy.sparse.csgraph.connected_components
  *   or_kcore           -- independent numpy peeling in tests + brute force
+ *   or_pr_step64_sample -- step k -> k+1 of the scipy float64 recurrence at the
+ *                         sampled vertices (1e-12); fails on the iterate of
+ *                         another step and on a directed grid read undirected
+ *   or_bfs_check       -- accepts scipy distances + 1 on every test graph;
+ *                         rejects each planted mistake (one depth raised or
+ *                         lowered, a vertex dropped, one of another component
+ *                         marked reached, a second depth-1 vertex)
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -598,6 +605,111 @@ int or_ppr(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid
     free(acc);
     walk_free(&w);
     return 0;
+}
+
+/* One FP64 PageRank step (pr.cpp:141-152 scatter, pr.cpp:286-290 update, in
+ * the intended form of or_pagerank64) evaluated only at the sampled vertices
+ * idx[0 .. ns) (sorted, distinct): out[i] = (0.15 + 0.85 * sum of prior[u]
+ * over the stored edges (u, idx[i]) -- both directions on undirected grids)
+ * / deg(idx[i]), 0 for degree 0.  prior is any PageRank iterate in output
+ * form, so a FP32 engine's iterate k+1 can be held against this step of its
+ * own iterate k at sizes where the whole FP64 run does not fit the host.
+ * One pass over the grid; per-thread sums over the sample, no atomics. */
+static int64_t sample_pos(const uint32_t *idx, uint64_t ns, uint64_t v)
+{
+    uint64_t lo = 0, hi = ns;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (idx[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo < ns && idx[lo] == v ? (int64_t)lo : -1;
+}
+
+int or_pr_step64_sample(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid,
+                        const int32_t *degree, const double *prior, const uint32_t *idx, uint64_t ns,
+                        int nthreads, double *out)
+{
+    walk_t w;
+    if (nthreads < 1) nthreads = 1;
+    for (uint64_t i = 0; i < ns; ++i)
+        if (idx[i] >= n || (i && idx[i] <= idx[i - 1])) return -1;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    uint8_t *mark = (uint8_t *)calloc((n + 7) / 8, 1);
+    double *part = (double *)calloc((size_t)nthreads * (ns ? ns : 1), sizeof(double));
+    if (!mark || !part) { free(mark); free(part); walk_free(&w); return -1; }
+    for (uint64_t i = 0; i < ns; ++i) mark[idx[i] >> 3] |= (uint8_t)(1u << (idx[i] & 7));
+    #pragma omp parallel num_threads(nthreads)
+    {
+        double *acc = part + (size_t)omp_get_thread_num() * ns;
+        #pragma omp for schedule(dynamic, 64)
+        for (uint64_t t = 0; t < w.T; ++t) {
+            uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+            for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+                uint64_t v0 = o0 + (grid[k] & 0xFFFF), v1 = o1 + (grid[k] >> 16);
+                if ((mark[v1 >> 3] >> (v1 & 7)) & 1) acc[sample_pos(idx, ns, v1)] += prior[v0];
+                if (!directed && ((mark[v0 >> 3] >> (v0 & 7)) & 1)) acc[sample_pos(idx, ns, v0)] += prior[v1];
+            }
+        }
+    }
+    for (uint64_t i = 0; i < ns; ++i) {
+        double a = 0.0;
+        for (int th = 0; th < nthreads; ++th) a += part[(size_t)th * ns + i];
+        int32_t d = degree[idx[i]];
+        out[i] = d ? (0.15 + 0.85 * a) / (double)d : 0.0;
+    }
+    free(mark);
+    free(part);
+    walk_free(&w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BFS depth check (the conditions that fix BFS distances)                   */
+/* ------------------------------------------------------------------------ */
+
+/* Returns the number of violations of the conditions that make depth[] the
+ * BFS depth array from root in the u8 encoding of bfs.cpp (0 unreached, root
+ * 1): depth[root] == 1; over every stored edge, undirected grids: both ends
+ * reached or neither, and reached ends differ by at most 1; directed grids
+ * (v0 -> v1): v0 reached implies v1 reached with depth[v1] <= depth[v0] + 1;
+ * and every reached vertex but the root has a reached neighbour (an
+ * in-neighbour on directed grids) one level closer.  Together these hold
+ * only for the true distances + 1: the parent chain bounds a depth from
+ * below, the edge rule bounds it from above along a shortest path, and the
+ * reached set is closed under the edges.  One pass over the grid plus one
+ * over the vertices, so it serves sizes where the oracle's BFS is too slow. */
+int64_t or_bfs_check(uint64_t n, int directed, const uint64_t *start, const uint32_t *grid, uint64_t root,
+                     const uint8_t *depth, int nthreads)
+{
+    walk_t w;
+    if (root >= n) return -1;
+    if (nthreads < 1) nthreads = 1;
+    if (walk_init(&w, n, directed, start, grid)) return -1;
+    uint8_t *has_parent = (uint8_t *)calloc(n, 1);
+    if (!has_parent) { walk_free(&w); return -1; }
+    int64_t bad = depth[root] == 1 ? 0 : 1;
+    #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64) reduction(+ : bad)
+    for (uint64_t t = 0; t < w.T; ++t) {
+        uint64_t o0 = (uint64_t)w.row[t] << 16, o1 = (uint64_t)w.col[t] << 16;
+        for (uint64_t k = w.start[t]; k < w.start[t + 1]; ++k) {
+            uint64_t v0 = o0 + (grid[k] & 0xFFFF), v1 = o1 + (grid[k] >> 16);
+            int a = depth[v0], b = depth[v1];
+            if (directed) {
+                if (a && (!b || b > a + 1)) ++bad;
+                if (a && b == a + 1) __atomic_store_n(&has_parent[v1], 1, __ATOMIC_RELAXED);
+            } else {
+                if ((a == 0) != (b == 0) || (a && (a > b + 1 || b > a + 1))) ++bad;
+                if (a && b && b == a + 1) __atomic_store_n(&has_parent[v1], 1, __ATOMIC_RELAXED);
+                if (a && b && a == b + 1) __atomic_store_n(&has_parent[v0], 1, __ATOMIC_RELAXED);
+            }
+        }
+    }
+    #pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : bad)
+    for (uint64_t v = 0; v < n; ++v)
+        if (v != root && depth[v] && !has_parent[v]) ++bad;
+    free(has_parent);
+    walk_free(&w);
+    return bad;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -203,6 +203,69 @@ def test_pagerank64_vs_float64(name, n, e, directed):
     assert np.max(np.abs(f32 - want)) <= 2e-6 * max(1.0, np.max(np.abs(want)))
 
 
+@pytest.mark.parametrize("name,n,e", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("directed", [False, True])
+def test_pr_step64_sample_is_one_step(name, n, e, directed):
+    """Step k -> k+1 of the scipy FP64 recurrence at a vertex sample; the
+    iterate of another step, or the grid read the wrong way, fails."""
+    start, grid, degree = oracle.convert(e, n, directed)
+    A = _adj(n, start, grid, directed)
+    rng = np.random.default_rng(4)
+    idx = np.unique(np.concatenate([np.argsort(degree)[::-1][:16], rng.choice(n, min(n, 500), replace=False)]))
+    x4 = _pr_f64(A.T.tocsr(), n, degree, 4)
+    want = _pr_f64(A.T.tocsr(), n, degree, 5)[idx]
+    for nt in (1, 4):
+        got = oracle.pr_step64_sample(n, start, grid, degree, x4, idx, directed=directed, nthreads=nt)
+        assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+    if name == "kron11":
+        x3 = _pr_f64(A.T.tocsr(), n, degree, 3)
+        assert np.max(np.abs(oracle.pr_step64_sample(n, start, grid, degree, x3, idx, directed=directed) - want)) > 1e-6
+        if directed:
+            und = oracle.pr_step64_sample(n, start, grid, degree, x4, idx, directed=False)
+            assert np.max(np.abs(und - want)) > 1e-6
+    with pytest.raises(RuntimeError):   # the sample must be sorted and distinct
+        oracle.pr_step64_sample(n, start, grid, degree, x4, idx[::-1].copy(), directed=directed)
+
+
+@pytest.mark.parametrize("name,n,e", CASES + [("two_groups",) + graphs.two_groups()],
+                         ids=[c[0] for c in CASES] + ["two_groups"])
+@pytest.mark.parametrize("directed", [False, True])
+def test_bfs_check_accepts_bfs_and_rejects_planted_mistakes(name, n, e, directed):
+    start, grid, degree = oracle.convert(e, n, directed)
+    A = _adj(n, start, grid, directed)
+    rng = np.random.default_rng(6)
+    roots = [0, int(np.argmax(degree)), n - 1]
+    planted = 0
+    for root in roots:
+        d = _bfs_expected(A, root)
+        assert oracle.bfs_check(n, start, grid, root, d, directed) == 0, (name, root)
+        assert oracle.bfs_check(n, start, grid, root, d, directed, nthreads=4) == 0
+        reached = np.flatnonzero(d > 1)
+        if reached.size == 0:
+            continue
+        for v in rng.choice(reached, min(3, reached.size), replace=False):
+            for delta in (1, -1):
+                bad = d.copy()
+                bad[v] = int(d[v]) + delta   # d[v] >= 2, so this stays a reached depth
+                assert oracle.bfs_check(n, start, grid, root, bad, directed) > 0, (name, root, int(v), delta)
+                planted += 1
+            bad = d.copy()
+            bad[v] = 0   # a reached vertex dropped
+            assert oracle.bfs_check(n, start, grid, root, bad, directed) > 0
+        unreached = np.flatnonzero((d == 0) & (degree > 0))
+        if unreached.size:
+            bad = d.copy()
+            u = int(unreached[0])
+            bad[u] = 2   # a vertex of another component marked reached
+            assert oracle.bfs_check(n, start, grid, root, bad, directed) > 0
+            bad[u] = 1   # or a second depth-1 vertex
+            assert oracle.bfs_check(n, start, grid, root, bad, directed) > 0
+    bad = _bfs_expected(A, 0)
+    bad[0] = 2   # the root must have depth 1
+    assert oracle.bfs_check(n, start, grid, 0, bad, directed) > 0
+    assert planted > 0 or name in ("components",)
+
+
 def test_ppr_threaded_vs_scipy():
     n, e = graphs.kron(12)
     start, grid, degree = oracle.convert(e, n)
