@@ -75,15 +75,41 @@ __global__ void __launch_bounds__(SWEEP_WG) k_kc_scan(SweepArgs a, const uint32_
     sweep_chunk<true>(a, chunk, s, len, op);
 }
 
-__global__ void k_kc_count(const int32_t *__restrict__ deg, uint64_t n, unsigned long long *out) {
+// count(deg != 0) and count(deg > 0): four degrees per 16-byte load, summed
+// over the wave and the workgroup, one global atomic per workgroup and count
+// (an atomic per thread on the same two words had made this pass 0.9 ms at
+// 2^28 vertices, most of which have a non-zero final degree).
+__global__ void __launch_bounds__(256) k_kc_count(const int32_t *__restrict__ deg, uint64_t n,
+                                                  unsigned long long *out) {
+    __shared__ uint32_t s_ne, s_pos;
+    if (threadIdx.x == 0) s_ne = s_pos = 0;
+    __syncthreads();
     uint32_t ne = 0, pos = 0;
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
-        int32_t d = deg[v];
+    const uint64_t n4 = n / 4;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int4 d = reinterpret_cast<const int4 *>(deg)[i];
+        ne += (d.x != 0) + (d.y != 0) + (d.z != 0) + (d.w != 0);
+        pos += (d.x > 0) + (d.y > 0) + (d.z > 0) + (d.w > 0);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n - n4 * 4) {
+        const int32_t d = deg[n4 * 4 + threadIdx.x];
         ne += d != 0;
         pos += d > 0;
     }
-    if (ne) atomicAdd(&out[0], (unsigned long long)ne);
-    if (pos) atomicAdd(&out[1], (unsigned long long)pos);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ne += __shfl_xor(ne, o);
+        pos += __shfl_xor(pos, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&s_ne, ne);
+        atomicAdd(&s_pos, pos);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_ne) atomicAdd(&out[0], (unsigned long long)s_ne);
+        if (s_pos) atomicAdd(&out[1], (unsigned long long)s_pos);
+    }
 }
 
 // ---- peeling over the neighbour-list index (relabelled ids) -----------------
@@ -311,12 +337,14 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
         }));
     }
     GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 16, w->stream));
-    k_kc_count<<<(unsigned)std::min<uint64_t>((g->n + 255) / 256, 8192), 256, 0, w->stream>>>(w->kc_deg.ptr, g->n,
-                                                                                            w->counters.ptr);
+    k_kc_count<<<(unsigned)std::min<uint64_t>((g->n / 4 + 255) / 256 + 1, 2048), 256, 0, w->stream>>>(
+        w->kc_deg.ptr, g->n, w->counters.ptr);
     GS_HIP(hipGetLastError());
-    if (by_index) {  // relabelled degrees back to vertex order (through qdeg, N entries)
+    // the final degrees in vertex order: relabelled ones are mapped back into qdeg (N entries)
+    const int32_t *final_deg = w->kc_deg.ptr;
+    if (by_index) {
         GS_TRY(unpermute_i32(g, w->stream, w->kc_deg.ptr, w->qdeg.ptr));
-        GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, w->qdeg.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+        final_deg = w->qdeg.ptr;
     }
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_HIP(hipMemcpyAsync(w->h_counters, w->counters.ptr, 32, hipMemcpyDeviceToHost, w->stream));
@@ -335,7 +363,7 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     if (count_pos) *count_pos = w->h_counters[1];
     if (rounds) *rounds = nr;
     if (deg_out) {
-        GS_HIP(hipMemcpyAsync(deg_out, w->kc_deg.ptr, g->n * 4, hipMemcpyDeviceToHost, w->stream));
+        GS_HIP(hipMemcpyAsync(deg_out, final_deg, g->n * 4, hipMemcpyDeviceToHost, w->stream));
         GS_HIP(hipStreamSynchronize(w->stream));
     }
     return GS_OK;
