@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(256) k_bu_expand(unsigned long long *ctl, uint
             wave_sync();   // s_u is rewritten by the next step
         }
     }
-    if (checked) atomicAdd(ctl + DO_CHK, (unsigned long long)checked);
+    block_count(checked, ctl + DO_CHK);
 }
 
 // The unreached vertices with edges (clear bits of vis below nzn), listed in

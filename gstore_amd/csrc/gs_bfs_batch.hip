@@ -228,7 +228,7 @@ __device__ inline void ms_bu_light(uint64_t v0, uint64_t v1, unsigned long long 
         }
         if (acc) next[v] |= acc;   // a top-down kernel of the same level may have set other lanes' bits
     }
-    if (ck) atomicAdd(checked, (unsigned long long)ck);
+    block_count(ck, checked);
 }
 
 // Bottom-up for the heavy vertices (ids [0, nheavy), degree >= MS_HEAVY_DEG):

@@ -113,7 +113,7 @@ gs_ws::~gs_ws() {
 
 size_t gs_ws::device_bytes() const {
     return pr_prior.bytes() + pr_acc.bytes() + pr_fac.bytes() + ppr_rank.bytes() + ppr_acc.bytes() + depth.bytes() + bm_vis.bytes() + bm_front.bytes() +
-           bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + wcc_label.bytes() + wcc_cmin.bytes() + kc_deg.bytes() + ms_vis.bytes() +
+           bm_next.bytes() + rowact.bytes() + wcc_parent.bytes() + wcc_label.bytes() + wcc_cmin.bytes() + wcc_samp.bytes() + kc_deg.bytes() + ms_vis.bytes() +
            ms_front.bytes() + ms_next.bytes() + ms_roots.bytes() + ms_depth.bytes() + ms_depth_vm.bytes() + ms_counts.bytes() +
            queue.bytes() + qdeg.bytes() + qoff.bytes() + qsum.bytes() + depth_r.bytes() + counters.bytes() + ms_ctl.bytes() + do_ctl.bytes() +
            bin_cnt.bytes() + bin_off.bytes() + bin_cur.bytes() + bin_uoff.bytes() + bin_pc.bytes() + bin_pr.bytes() + bin_o.bytes() + bin_v.bytes();

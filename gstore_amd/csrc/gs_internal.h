@@ -137,6 +137,7 @@ struct gs_ws {
     gs::DevBuf<uint32_t> wcc_parent;
     gs::DevBuf<uint32_t> wcc_label;                           // labels by original id (index method)
     gs::DevBuf<uint32_t> wcc_cmin;                            // smallest original id per root (index method)
+    gs::DevBuf<uint32_t> wcc_samp;                            // roots of the sampled vertices (index method)
     gs::DevBuf<int32_t> kc_deg;
     gs::DevBuf<uint64_t> ms_vis, ms_front, ms_next, ms_roots;  // batched BFS lane masks
     gs::DevBuf<uint8_t> ms_depth;                             // lane-major depth, lanes x N
@@ -310,6 +311,21 @@ int run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr, ui
 int run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncomp);
 int run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t *count_ne0, uint64_t *count_pos,
               int *rounds);
+
+// Adds every thread's count into *ctr with one atomic per workgroup: an
+// atomic per thread on one word serialises (half a million of them cost a
+// large bottom-up level at 2^28 vertices about a tenth of its time).  Every
+// thread of the workgroup must call it (blockDim a multiple of 64).
+__device__ inline void block_count(uint64_t c, unsigned long long *ctr) {
+    __shared__ unsigned long long s_tot;
+    if (threadIdx.x == 0) s_tot = 0;
+    __syncthreads();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_tot, (unsigned long long)c);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_tot) atomicAdd(ctr, s_tot);
+}
 
 // XCD-aware chunk order: workgroup b runs on XCD b % 8; give each XCD a
 // contiguous run of chunks so its L2 sees consecutive tiles.

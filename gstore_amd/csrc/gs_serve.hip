@@ -107,7 +107,7 @@ static uint64_t need_bytes(const gs_graph *g, int kind, int lanes, bool keep_dep
         case KIND_BFS1: return base + do_bfs_bytes(g);
         case GS_Q_PPR: return base + 24 * n;  // prior, acc, factor, rank (f32) and the FP64 sums
         case GS_Q_PAGERANK: return base + 12 * n;
-        case GS_Q_WCC: return base + (g->wcc_method == 1 && !g->directed && !g->ss ? 12 : 4) * n;
+        case GS_Q_WCC: return base + (g->wcc_method == 1 && !g->directed && !g->ss ? 12 * n + 4 * 1024 : 4 * n);
         default: return base;
     }
 }
@@ -130,7 +130,7 @@ static uint64_t kind_bytes(const gs_ws *w, int kind) {
             return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes() + w->ppr_rank.bytes() +
                    w->ppr_acc.bytes();
         case GS_Q_PAGERANK: return b + w->pr_prior.bytes() + w->pr_acc.bytes() + w->pr_fac.bytes();
-        case GS_Q_WCC: return b + w->wcc_parent.bytes() + w->wcc_label.bytes() + w->wcc_cmin.bytes();
+        case GS_Q_WCC: return b + w->wcc_parent.bytes() + w->wcc_label.bytes() + w->wcc_cmin.bytes() + w->wcc_samp.bytes();
         default: return b;
     }
 }
@@ -162,6 +162,7 @@ static void trim(gs_ws *w, int kind) {
         w->wcc_parent.release();
         w->wcc_label.release();
         w->wcc_cmin.release();
+        w->wcc_samp.release();
     }
     w->kc_deg.release();
 }

@@ -121,7 +121,7 @@ constexpr uint32_t AFF_SAMPLES = 1024; // vertices sampled to find the largest c
 
 // Round r links every vertex with its r-th list entry; round 0 takes it from
 // first[] (every vertex below nzn has one), without the offsets.
-__global__ void k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+__global__ void __launch_bounds__(256) k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                             const uint32_t *__restrict__ first, uint64_t nzn, uint32_t r,
                             unsigned long long *linked) {
     uint32_t c = 0;
@@ -137,7 +137,7 @@ __global__ void k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, con
             ++c;
         }
     }
-    if (c) atomicAdd(linked, (unsigned long long)c);
+    block_count(c, linked);
 }
 
 __global__ void k_aff_sample(uint32_t *par, uint64_t n, uint32_t *out) {
@@ -151,18 +151,18 @@ __global__ void k_aff_sample(uint32_t *par, uint64_t n, uint32_t *out) {
 }
 
 // Remaining neighbours of every vertex outside the component rooted at skip.
-__global__ void k_aff_final(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+__global__ void __launch_bounds__(256) k_aff_final(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                             uint64_t nzn, uint32_t skip, unsigned long long *linked) {
-    uint32_t c = 0;
+    uint64_t c = 0;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = off[x] + AFF_ROUNDS, e = off[x + 1];
         if (b >= e) continue;
         const uint32_t u = (uint32_t)x;
         if (wcc_find(par, u) == skip) continue;
         for (uint64_t k = b; k < e; ++k) wcc_link(par, u, nbr[k]);
-        c += (uint32_t)(e - b);
+        c += e - b;
     }
-    if (c) atomicAdd(linked, (unsigned long long)c);
+    block_count(c, linked);
 }
 
 // cmin[r] = the root's own original id, for every root r of the compressed
@@ -277,12 +277,11 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
         // the most frequent root among sampled vertices: its members skip their other neighbours
         uint32_t skip = 0xFFFFFFFFu;
         if (g->nzn) {
-            DevBuf<uint32_t> samp;
-            GS_TRY(samp.alloc(AFF_SAMPLES));
-            k_aff_sample<<<(AFF_SAMPLES + 255) / 256, 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn, samp.ptr);
+            if (w->wcc_samp.n != AFF_SAMPLES) GS_TRY(w->wcc_samp.alloc(AFF_SAMPLES));
+            k_aff_sample<<<(AFF_SAMPLES + 255) / 256, 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn, w->wcc_samp.ptr);
             GS_HIP(hipGetLastError());
             std::vector<uint32_t> h(AFF_SAMPLES);
-            GS_HIP(hipMemcpyAsync(h.data(), samp.ptr, AFF_SAMPLES * 4, hipMemcpyDeviceToHost, w->stream));
+            GS_HIP(hipMemcpyAsync(h.data(), w->wcc_samp.ptr, AFF_SAMPLES * 4, hipMemcpyDeviceToHost, w->stream));
             GS_HIP(hipStreamSynchronize(w->stream));
             std::sort(h.begin(), h.end());
             uint32_t best = 0;
