@@ -239,8 +239,7 @@ __global__ void __launch_bounds__(256) k_kc_push(const uint32_t *__restrict__ qu
             for (uint32_t k = lane; k < hl; k += 64) push(nbr[b + k]);
         }
     }
-    if (cnt) atomicAdd(pushed, cnt);
-    __syncthreads();
+    block_count(cnt, pushed);
     for (uint32_t i = threadIdx.x; i < KC_HUB; i += blockDim.x)
         if (hub[i]) atomicSub(&deg[i], (int32_t)hub[i]);
 }

@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(SWEEP_WG) k_wcc_check(SweepArgs a, const uint3
 // label = root for vertices with a stored edge, 0xFFFFFFFF otherwise; count roots.
 // A vertex has a stored edge when its .degree is non-zero (undirected: endpoint
 // count) or, on a directed grid (.degree = out-degree), when it has an in-edge.
-__global__ void k_wcc_label(uint32_t *par, const int32_t *__restrict__ deg, const uint32_t *__restrict__ in_touch,
+__global__ void __launch_bounds__(256) k_wcc_label(uint32_t *par, const int32_t *__restrict__ deg, const uint32_t *__restrict__ in_touch,
                             uint64_t n, unsigned long long *ncomp) {
     uint32_t c = 0;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
@@ -100,7 +100,7 @@ __global__ void k_wcc_label(uint32_t *par, const int32_t *__restrict__ deg, cons
         if (!touched) par[v] = 0xFFFFFFFFu;
         else c += (par[v] == (uint32_t)v);
     }
-    if (c) atomicAdd(ncomp, (unsigned long long)c);
+    block_count(c, ncomp);
 }
 
 static unsigned vgrid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256) k_aff_cmin(const uint32_t *__restrict__ p
 // component's smallest original id (cmin of the root) for vertices with
 // edges (relabelled ids below nzn), 0xFFFFFFFF for the others; counts the
 // roots.
-__global__ void k_aff_label(const uint32_t *__restrict__ par, const uint32_t *__restrict__ cmin,
+__global__ void __launch_bounds__(256) k_aff_label(const uint32_t *__restrict__ par, const uint32_t *__restrict__ cmin,
                             const uint32_t *__restrict__ new_id, uint64_t nzn, uint64_t n, uint32_t *label,
                             unsigned long long *ncomp) {
     uint32_t c = 0;
@@ -235,7 +235,7 @@ __global__ void k_aff_label(const uint32_t *__restrict__ par, const uint32_t *__
             }
         }
     }
-    if (c) atomicAdd(ncomp, (unsigned long long)c);
+    block_count(c, ncomp);
 }
 
 }  // namespace gs

@@ -32,7 +32,6 @@ SCALE = 30
 BUDGET = 16 << 30
 SEGMENT = 512 << 20
 HOST_GB_NEEDED = 200     # pinned grid + the oracle's host copy + rank and depth arrays
-DEVICE_GB_NEEDED = 200   # the grid is generated and converted in HBM before it moves to the host
 
 
 def _threads():
@@ -48,13 +47,10 @@ def _say(msg):
 @pytest.fixture(scope="module")
 def kron30():
     import psutil
-    import torch
     host_gb = psutil.virtual_memory().available / 2 ** 30
     if host_gb < HOST_GB_NEEDED:
         pytest.skip(f"needs {HOST_GB_NEEDED} GiB of free host memory, {host_gb:.0f} available")
-    free, _ = torch.cuda.mem_get_info(0)
-    if free / 2 ** 30 < DEVICE_GB_NEEDED:
-        pytest.skip(f"needs {DEVICE_GB_NEEDED} GiB of free device memory, {free / 2 ** 30:.0f} available")
+    # the grid (69 GB) is generated and converted in HBM before it moves to pinned host memory
     t0 = time.time()
     _say("generating and converting Kron-30 on the device")
     g = ga.Graph.kron(SCALE)
