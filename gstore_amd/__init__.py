@@ -107,6 +107,7 @@ _SIGS = {
     "gs_get_info": (ctypes.c_int, [_P, ctypes.POINTER(_Info)]),
     "gs_set_device_budget": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64]),
     "gs_get_arrays": (ctypes.c_int, [_P, _P, _P, _P]),
+    "gs_get_host_grid": (ctypes.c_int, [_P, _P, _P]),
     "gs_open_external": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
                                         ctypes.c_uint64, ctypes.POINTER(_P)]),
     "gs_get_io_stats": (ctypes.c_int, [_P, ctypes.POINTER(_IoStats)]),
@@ -291,12 +292,26 @@ class Graph:
             pass
 
     # --- data ---------------------------------------------------------------
-    def arrays(self):
+    def arrays(self, with_grid: bool = True):
+        """(start, grid, degree) copied to the host; grid is None with with_grid=False."""
         start = np.empty(self.T + 1, np.uint64)
-        grid = np.empty(self.E, np.uint32)
+        grid = np.empty(self.E, np.uint32) if with_grid else None
         degree = np.empty(self.n, np.int32)
-        _check(_lib.gs_get_arrays(self._h, _ptr(start), _ptr(grid) if self.E else None, _ptr(degree)))
+        _check(_lib.gs_get_arrays(self._h, _ptr(start), _ptr(grid) if with_grid and self.E else None,
+                                  _ptr(degree)))
         return start, grid, degree
+
+    def host_grid(self) -> np.ndarray:
+        """Semi-external mode: a read-only view of the pinned host grid (no copy),
+        valid while the graph is open and its device budget unchanged."""
+        p = ctypes.c_void_p()
+        e = ctypes.c_uint64(0)
+        _check(_lib.gs_get_host_grid(self._h, ctypes.byref(p), ctypes.byref(e)))
+        if not e.value:
+            return np.empty(0, np.uint32)
+        view = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint32)), shape=(e.value,))
+        view.flags.writeable = False
+        return view
 
     def set_device_budget(self, budget_bytes: int, segment_bytes: int = 0) -> None:
         """Semi-external mode: keep at most budget_bytes of the grid in HBM."""

@@ -14,6 +14,10 @@ namespace gs {
 
 static thread_local std::string g_err;
 
+// Every failure path records its message here.  A failed HIP call also
+// leaves HIP's per-thread last error set until it is read; it is cleared
+// here, so that the failure is reported once and a later launch check on
+// this thread (hipGetLastError after a kernel) does not report it again.
 void set_error(const char *fmt, ...) {
     char buf[1024];
     va_list ap;
@@ -21,6 +25,7 @@ void set_error(const char *fmt, ...) {
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     g_err = buf;
+    (void)hipGetLastError();
 }
 
 const char *error_text() { return g_err.c_str(); }
@@ -734,7 +739,7 @@ int gs_get_info(gs_graph *g, gs_info *info) {
     info->device_bytes = g->grid.bytes() + g->tz_beg.bytes() + g->tz_rc.bytes() + g->chunk_tz.bytes() +
                          g->degree.bytes() + g->ws->device_bytes() + g->adj_off.bytes() + g->adj_nbr.bytes() + g->adj_first.bytes() +
                          g->new_id.bytes() + g->old_id.bytes() + g->deg_r.bytes() + g->chunkz.bytes() + gs::pb_bytes(g) +
-                         (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() : 0);
+                         (g->ss ? g->ss->seg[0].bytes() + g->ss->seg[1].bytes() + g->ss->seg[2].bytes() : 0);
     const bool built = g->adj_ready;
     info->index_entries = built ? 2 * g->E : 0;
     info->index_vertices = built ? g->nzn : 0;
@@ -771,6 +776,17 @@ int gs_get_arrays(gs_graph *g, uint64_t *start, uint32_t *grid, int32_t *degree)
         else GS_HIP(hipMemcpy(grid, g->grid.ptr, g->E * 4, hipMemcpyDeviceToHost));
     }
     if (degree) GS_HIP(hipMemcpy(degree, g->degree.ptr, g->n * 4, hipMemcpyDeviceToHost));
+    return GS_OK;
+}
+
+int gs_get_host_grid(gs_graph *g, const uint32_t **grid, uint64_t *edges) {
+    if (!g || !grid) { set_error("null argument"); return GS_ERR_ARG; }
+    if (!g->ss || g->ss->disk || !g->ss->h_grid) {
+        set_error("gs_get_host_grid: the grid has no pinned host copy (resident or external mode)");
+        return GS_ERR_UNSUPPORTED;
+    }
+    *grid = g->ss->h_grid;
+    if (edges) *edges = g->E;
     return GS_OK;
 }
 

@@ -169,9 +169,9 @@ struct gs_ws {
 
 namespace gs {
 // Semi-external mode: the grid lives in pinned host memory; the first
-// res_chunks chunks stay resident in HBM and the rest pass through two
-// device segment buffers, copied on a side stream while the previous
-// segment is being swept.
+// res_chunks chunks stay resident in HBM and the rest pass through nbuf (2
+// or 3) device segment buffers, copied on a side stream while the earlier
+// segments are being swept.
 struct DiskStage;  // external mode: the grid read from its file every sweep (gs_stream.hip)
 
 struct StreamState {
@@ -180,9 +180,11 @@ struct StreamState {
     const uint32_t *h_grid_dev = nullptr;  // the same memory as the device addresses it (direct sweeps)
     uint64_t res_chunks = 0;     // chunks [0, res_chunks) resident in gs_graph::grid
     uint64_t seg_chunks = 0;     // chunks per segment buffer
-    DevBuf<uint32_t> seg[2];
+    static constexpr int MAXBUF = 3;
+    int nbuf = 2;                // segment buffers in use
+    DevBuf<uint32_t> seg[MAXBUF];
     hipStream_t copy = nullptr;
-    hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
+    hipEvent_t ev_copied[MAXBUF] = {}, ev_done[MAXBUF] = {};
     ~StreamState();
 };
 }  // namespace gs
@@ -535,7 +537,10 @@ __device__ inline uint32_t gv1(uint32_t w, uint32_t rc) { return (rc << 16) | (w
 
 // Launch one sweep over every chunk of the grid: one launch when resident;
 // when streaming, the resident prefix first, then each segment after its
-// host-to-device copy, double-buffered so copy k+1 overlaps sweep k.
+// host-to-device copy, through nbuf segment buffers: the copy of segment
+// k + nbuf - 1 overlaps the sweep of segment k, and with three buffers a
+// segment whose sweep takes longer than a copy does not stall the copy
+// engine.
 // launch(SweepArgs, grid_size, stream) issues the kernel.  interleave (sweeps
 // whose result does not depend on the order of the chunks, such as the
 // PageRank scatter): the resident prefix is cut into segment-sized pieces
@@ -568,7 +573,7 @@ int sweep(const gs_graph *g, gs_ws *w, int kclass, F &&launch, bool direct = fal
     }
     if (ss->disk && ss->res_chunks < g->nchunk) GS_TRY(disk_pass_begin(ss));
     int k = 0;
-    for (uint64_t c0 = ss->res_chunks; c0 < g->nchunk; c0 += ss->seg_chunks, k ^= 1) {
+    for (uint64_t c0 = ss->res_chunks; c0 < g->nchunk; c0 += ss->seg_chunks, k = (k + 1) % ss->nbuf) {
         uint64_t cn = std::min(ss->seg_chunks, g->nchunk - c0);
         uint64_t e0 = c0 * g->chunk, e1 = std::min(g->E, (c0 + cn) * g->chunk);
         GS_HIP(hipStreamWaitEvent(ss->copy, ss->ev_done[k], 0));

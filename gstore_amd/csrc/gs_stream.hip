@@ -324,7 +324,7 @@ StreamState::~StreamState() {
     if (copy) (void)hipStreamSynchronize(copy);
     delete disk;   // joins its IO threads (no copy reads its staging buffers any more)
     disk = nullptr;
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < MAXBUF; ++i) {
         if (ev_copied[i]) (void)hipEventDestroy(ev_copied[i]);
         if (ev_done[i]) (void)hipEventDestroy(ev_done[i]);
     }
@@ -339,20 +339,22 @@ int stream_setup(gs_graph *g, uint32_t *h_grid, uint64_t budget, uint64_t seg_by
     const uint64_t chunk_bytes = (uint64_t)g->chunk * 4;
     if (!seg_bytes) seg_bytes = std::min<uint64_t>(512ull << 20, budget / 8);  // default: 512 MiB or budget/8
     ss->seg_chunks = std::max<uint64_t>(1, seg_bytes / chunk_bytes);
-    uint64_t two = 2 * ss->seg_chunks * chunk_bytes;
-    if (budget < two + chunk_bytes) {
+    const uint64_t segb = ss->seg_chunks * chunk_bytes;
+    if (budget < 2 * segb + chunk_bytes) {
         set_error("device budget %llu bytes is below two segments (%llu bytes) plus one chunk",
-                  (unsigned long long)budget, (unsigned long long)two);
+                  (unsigned long long)budget, (unsigned long long)(2 * segb));
         return GS_ERR_ARG;
     }
-    ss->res_chunks = std::min<uint64_t>(g->nchunk, (budget - two) / chunk_bytes);
+    // a third segment buffer when the budget holds it and some of the grid is
+    // resident beside it: it absorbs segments whose sweep outlasts a copy
+    ss->nbuf = budget >= 4 * segb + chunk_bytes ? 3 : 2;
+    ss->res_chunks = std::min<uint64_t>(g->nchunk, (budget - ss->nbuf * segb) / chunk_bytes);
     uint64_t res_edges = std::min<uint64_t>(g->E, ss->res_chunks * g->chunk);
     GS_TRY(g->grid.alloc(res_edges));
     if (res_edges) GS_HIP(hipMemcpy(g->grid.ptr, h_grid, res_edges * 4, hipMemcpyHostToDevice));
-    GS_TRY(ss->seg[0].alloc(ss->seg_chunks * g->chunk));
-    GS_TRY(ss->seg[1].alloc(ss->seg_chunks * g->chunk));
+    for (int i = 0; i < ss->nbuf; ++i) GS_TRY(ss->seg[i].alloc(ss->seg_chunks * g->chunk));
     GS_HIP(hipStreamCreateWithFlags(&ss->copy, hipStreamNonBlocking));
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < ss->nbuf; ++i) {
         GS_HIP(hipEventCreateWithFlags(&ss->ev_copied[i], hipEventDisableTiming));
         GS_HIP(hipEventCreateWithFlags(&ss->ev_done[i], hipEventDisableTiming));
         GS_HIP(hipEventRecord(ss->ev_done[i], ss->copy));  // first waits complete at once

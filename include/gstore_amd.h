@@ -159,6 +159,14 @@ int gs_get_io_stats(gs_graph *g, gs_io_stats *out);
 /* Copy the tile arrays back to host buffers (start: T+1, grid: E, degree: N). */
 int gs_get_arrays(gs_graph *g, uint64_t *start, uint32_t *grid, int32_t *degree);
 
+/* Semi-external mode only: the pinned host copy of the .grid array (E
+ * edges), read-only, owned by the graph and valid until gs_close or the next
+ * gs_set_device_budget.  Lets a host reader of a grid too large for a second
+ * copy (a checker, a writer) use it in place.  GS_ERR_UNSUPPORTED for
+ * resident and external graphs.  No counterpart in the reference (its grid
+ * lives in the process's own memory). */
+int gs_get_host_grid(gs_graph *g, const uint32_t **grid, uint64_t *edges);
+
 /* PageRank, `iters` iterations (pr_t, pr.cpp:82-303; resident driver
  * pagerank_mmap gstore.cpp:2572-2629; -j 1/11/21 -a iters, default 5). */
 int gs_pagerank(gs_graph *g, int iters, int mode, float *rank_out);

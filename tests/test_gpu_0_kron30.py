@@ -16,7 +16,12 @@ evaluated by the oracle in one pass over the engine's grid on the host:
     every reached vertex with a neighbour one level closer), and the level
     counts are its histogram.
 The generator itself is pinned bit for bit at scale-30 offsets by the
-reference vectors (tests/test_oracle.py) and at Kron-28 by the fixtures."""
+reference vectors (tests/test_oracle.py) and at Kron-28 by the fixtures.
+
+The module is named to run first among the GPU tests: the 69 GB pinned
+allocation has failed in a process that had run the external-mode tests
+before it, and succeeded in fresh processes on the same box.  Where the host
+cannot pin the grid, the tests are skipped with the reason."""
 import os
 import time
 
@@ -31,7 +36,7 @@ pytestmark = pytest.mark.gpu
 SCALE = 30
 BUDGET = 16 << 30
 SEGMENT = 512 << 20
-HOST_GB_NEEDED = 200     # pinned grid + the oracle's host copy + rank and depth arrays
+HOST_GB_NEEDED = 120     # the pinned grid (which the oracle reads in place) + start, degree, rank and depth arrays
 
 
 def _threads():
@@ -54,13 +59,24 @@ def kron30():
     t0 = time.time()
     _say("generating and converting Kron-30 on the device")
     g = ga.Graph.kron(SCALE)
-    _say(f"E={g.E} ({time.time() - t0:.0f}s); host copy of the tile arrays for the oracle")
-    start, grid, degree = g.arrays()
-    _say(f"moving the grid to pinned host memory, device budget {BUDGET >> 30} GiB ({time.time() - t0:.0f}s)")
-    g.set_device_budget(BUDGET, SEGMENT)
+    _say(f"E={g.E} ({time.time() - t0:.0f}s); moving the grid to pinned host memory, "
+         f"device budget {BUDGET >> 30} GiB")
+    try:
+        g.set_device_budget(BUDGET, SEGMENT)
+    except ga.GStoreError as e:
+        g.close()
+        if "out of memory" in str(e):
+            avail = psutil.virtual_memory().available / 2 ** 30
+            pytest.skip(f"this host cannot pin the 69 GB grid ({avail:.0f} GiB available): {e}")
+        raise
     assert not g.resident and g.resident_edges < g.E
+    # the oracle reads the pinned grid in place (a second 69 GB copy does not fit every host)
+    start, _, degree = g.arrays(with_grid=False)
+    grid = g.host_grid()
+    assert grid.size == g.E
     _say(f"resident edges {g.resident_edges} of {g.E} ({time.time() - t0:.0f}s)")
     yield g, start, grid, degree
+    del grid
     g.close()
 
 

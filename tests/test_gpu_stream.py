@@ -1,5 +1,5 @@
 """Semi-external mode: the grid in pinned host memory, a resident prefix in
-HBM and the rest streamed through two segment buffers.  Every algorithm must
+HBM and the rest streamed through two or three segment buffers.  Every algorithm must
 give the resident result (integer outputs bit for bit, PageRank compat bit for
 bit, intended within FP32 tolerance)."""
 import numpy as np
@@ -17,15 +17,19 @@ def _say(*a):
     print("[gpu-stream]", *a, flush=True)
 
 
-@pytest.mark.parametrize("res_chunks", [0, 5])
+@pytest.mark.parametrize("res_chunks", [0, 5, 12])
 def test_streaming_equals_resident(res_chunks):
+    """Two segment buffers while the budget holds fewer than four segments,
+    three from there on (res_chunks 12: the copy engine runs two segments
+    ahead of the sweep)."""
     n, e = graphs.kron(15)
     start, grid, deg = oracle.convert(e, n)
     seg = 3 * CH
     budget = 2 * seg + res_chunks * CH + CH
+    nbuf = 3 if budget >= 4 * seg + CH else 2
     with ga.Graph.from_arrays(n, start, grid, deg) as ref, ga.Graph.from_arrays(n, start, grid, deg) as g:
         g.set_device_budget(budget, seg)
-        assert not g.resident and g.resident_edges == min(g.E, (res_chunks + 1) * 4096)
+        assert not g.resident and g.resident_edges == min(g.E, (budget - nbuf * seg) // CH * 4096)
         _say("streaming, resident edges", g.resident_edges, "of", g.E)
         assert np.array_equal(g.pagerank(5, ga.PR_COMPAT), ref.pagerank(5, ga.PR_COMPAT))
         assert np.max(np.abs(g.pagerank(10, ga.PR_INTENDED) - ref.pagerank(10, ga.PR_INTENDED))) <= 1e-6
