@@ -329,6 +329,21 @@ __device__ inline void block_count(uint64_t c, unsigned long long *ctr) {
     if (threadIdx.x == 0 && s_tot) atomicAdd(ctr, s_tot);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave in six DPP adds: row_shr
+// 1, 2, 4, 8 within each row of 16 lanes, then row_bcast:15 (rows 1 and 3 add
+// the last lane of the row below) and row_bcast:31 (rows 2 and 3 add lane
+// 31).  A lane whose source is out of its row reads 0.  Every lane of the
+// wave must be active, or the inactive ones a suffix (sources are lower lanes).
+__device__ inline uint32_t wave_incl_scan(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
 // XCD-aware chunk order: workgroup b runs on XCD b % 8; give each XCD a
 // contiguous run of chunks so its L2 sees consecutive tiles.
 __device__ inline uint64_t xcd_chunk(uint64_t bid, uint64_t nchunk) {

@@ -187,34 +187,30 @@ __global__ void k_ppr_finalize(uint64_t n, uint64_t src, float *__restrict__ pri
 // Work memory per workspace: 6 bytes per contribution of a piece (two per
 // undirected edge), bin tables and 16 bytes per tile of a piece.
 
-// atomicAdd(&ctr[key], n) for each active lane, lanes of the wave with equal
-// keys combined into one atomic; returns each active lane's old value plus
-// the n of the lanes below it with the same key (its reserved position).
-// Every lane of the wave calls it.
+// atomicAdd(&ctr[key], n) for each active lane, with each run of lanes that
+// hold the same key (consecutive lanes) combined into one atomic by the run's
+// first lane; returns each active lane's reserved position: the run's old
+// value plus the n of the run's lanes below it.  The tiles of a wave come in
+// .start order, so equal tile rows sit in consecutive lanes; a key that
+// recurs after another is a run of its own (one more atomic, still correct).
+// A constant number of steps whatever the number of distinct keys: a loop
+// over them (one scan and one atomic per key) made the per-tile reservations
+// of a sparse piece, whose wave spans dozens of tile rows, 40 times slower.
+// Every lane of the wave calls it, all lanes active.
 __device__ inline uint32_t wave_reserve(uint32_t *ctr, uint32_t key, uint32_t n, bool active) {
     const int lane = threadIdx.x & 63;
-    uint32_t res = 0;
-    unsigned long long todo = __ballot(active);
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const uint32_t lk = __shfl(key, leader);
-        const unsigned long long m = __ballot(active && key == lk) & todo;
-        const bool mine = (m >> lane) & 1ull;
-        const uint32_t x = mine ? n : 0u;
-        uint32_t incl = x;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o);
-            if (lane >= o) incl += y;
-        }
-        const uint32_t tot = __shfl(incl, 63);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&ctr[lk], tot);
-        base = __shfl(base, leader);
-        if (mine) res = base + incl - x;
-        todo &= ~m;
-    }
-    return res;
+    const uint32_t x = active ? n : 0u;
+    const uint32_t prev = __shfl_up(key, 1);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    const uint32_t incl = wave_incl_scan(x), excl = incl - x;
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);   // lanes 0 .. lane
+    const int head = 63 - __clzll(heads & upto);                               // this lane's run starts here
+    const unsigned long long later = heads & ~upto;
+    const int last = later ? __ffsll((long long)later) - 2 : 63;               // for a head: its run's last lane
+    const uint32_t run_incl = __shfl(incl, last);
+    uint32_t base = 0;
+    if (lane == head && run_incl != excl) base = atomicAdd(&ctr[key], run_incl - excl);
+    return __shfl(base, head) + excl - __shfl(excl, head);
 }
 
 // Edges of tile tz inside the piece [e0, e1) and the first of them.
