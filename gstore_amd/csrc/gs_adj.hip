@@ -149,11 +149,7 @@ __device__ inline bool do_gate(const unsigned long long *ctl, unsigned long long
 // is left out -- one random read per edge fewer -- and the level end drops
 // the visited vertices from the marks.
 constexpr uint32_t TD_CHUNK = 4096;
-
-__device__ inline void td_visit(uint32_t v, const uint32_t *__restrict__ vis, uint32_t *next, bool test_vis) {
-    const uint32_t wi = v >> 5, bit = 1u << (v & 31);
-    if ((!test_vis || !(vis[wi] & bit)) && !(next[wi] & bit)) atomicOr(&next[wi], bit);
-}
+constexpr int TD_EPT = 4;   // edges per thread whose loads are in flight together
 
 __global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl, const uint32_t *__restrict__ queue,
                                                   const uint64_t *__restrict__ qoff, const uint64_t *__restrict__ off,
@@ -194,13 +190,34 @@ __global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl,
         }
         if (threadIdx.x == 0) sbeg[ns] = len;
         __syncthreads();
-        for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
-            int lo = 0, hi = ns - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+        // TD_EPT edges per thread at a time, their loads issued together: a
+        // thread walking its edges one by one had one load in flight
+        for (uint32_t t0 = threadIdx.x; t0 < len; t0 += TD_EPT * blockDim.x) {
+            uint32_t v[TD_EPT], nw[TD_EPT], vw[TD_EPT];
+#pragma unroll
+            for (int j = 0; j < TD_EPT; ++j) {
+                const uint32_t t = t0 + j * blockDim.x;
+                v[j] = 0xFFFFFFFFu;
+                if (t < len) {
+                    int lo = 0, hi = ns - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (sbeg[mid] <= t) lo = mid; else hi = mid - 1;
+                    }
+                    v[j] = nbr[sadj[lo] + (t - sbeg[lo])];
+                }
             }
-            td_visit(nbr[sadj[lo] + (t - sbeg[lo])], vis, next, test_vis);
+#pragma unroll
+            for (int j = 0; j < TD_EPT; ++j) {
+                nw[j] = v[j] != 0xFFFFFFFFu ? next[v[j] >> 5] : ~0u;
+                vw[j] = test_vis && v[j] != 0xFFFFFFFFu ? vis[v[j] >> 5] : 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < TD_EPT; ++j) {
+                if (v[j] == 0xFFFFFFFFu) continue;
+                const uint32_t bit = 1u << (v[j] & 31);
+                if (!((nw[j] | vw[j]) & bit)) atomicOr(&next[v[j] >> 5], bit);
+            }
         }
         __syncthreads();  // the staged tables are rewritten by the next step
     }
