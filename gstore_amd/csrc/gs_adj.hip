@@ -126,10 +126,11 @@ enum : unsigned long long {
     DF_ULIST = 64,    // bottom-up over the unreached list
     DF_BUFULL = 128,  // bottom-up over all vertices with edges
     DF_LEVEL = 256,   // a level runs (its level end)
-    DF_TDNV = 512     // top-down without the visited test (few vertices reached yet)
+    DF_TDNV = 512,    // top-down without the visited test (few vertices reached yet)
+    DF_TDH = 1024     // a hub level: marks of the heaviest TD_HUB_IDS vertices gathered in LDS first
 };
 constexpr int DO_LEVELS = 4;
-constexpr unsigned DO_TD_GRID = 2048;
+constexpr unsigned DO_TD_GRID = 256;   // TD_WG-thread workgroups: one per CU (LDS: staging + hub bitmap)
 constexpr unsigned DO_SCAN_GRID = 1024;
 constexpr unsigned DO_GRID = 2048;
 
@@ -150,35 +151,64 @@ __device__ inline bool do_gate(const unsigned long long *ctl, unsigned long long
 // the visited vertices from the marks.
 constexpr uint32_t TD_CHUNK = 4096;
 constexpr int TD_EPT = 4;   // edges per thread whose loads are in flight together
+constexpr unsigned TD_WG = 1024;                 // threads per top-down workgroup: one TD_CHUNK step is one batch
+constexpr uint32_t TD_HUB_IDS = 1u << 19;        // hub level: relabelled ids whose marks go through LDS
+constexpr uint32_t TD_HUB_WORDS = TD_HUB_IDS / 32;
 
-__global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl, const uint32_t *__restrict__ queue,
-                                                  const uint64_t *__restrict__ qoff, const uint64_t *__restrict__ off,
-                                                  const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
-                                                  uint32_t *next) {
+// Largest j in [lo, hi] with qoff[j] <= target, given qoff[lo] <= target
+// (qoff non-decreasing), by the 64 lanes of a wave: each step probes 64
+// evenly spaced entries and keeps the interval after the last one <= target,
+// so a queue of 2^18 entries takes 3 steps of one load per lane (a binary
+// search by one thread took 18 dependent loads per chunk, during which the
+// rest of the workgroup waited).  Every lane of the wave calls it.
+__device__ inline uint64_t wave_upper(const uint64_t *__restrict__ qoff, uint64_t lo, uint64_t hi, uint64_t target) {
+    const uint64_t lane = threadIdx.x & 63;
+    while (hi > lo) {
+        const uint64_t span = hi - lo + 1;
+        const uint64_t pos = lo + span * lane / 64;
+        const unsigned long long m = __ballot(qoff[pos] <= target);   // lane 0 holds: qoff[lo] <= target
+        const uint64_t l = 63 - __clzll(m);
+        const uint64_t nlo = lo + span * l / 64;
+        hi = l == 63 ? hi : lo + span * (l + 1) / 64 - 1;
+        lo = nlo;
+    }
+    return lo;
+}
+
+// On a hub level (DF_TDH: hundreds of millions of frontier edges while few
+// vertices are reached) about half of the edges land on the TD_HUB_IDS
+// heaviest vertices.  Their marks are or-ed into a bitmap in LDS instead of
+// the global one (no random global read, no atomic per edge), and each
+// workgroup ors its LDS words into next once at the end.  hub_words is
+// min(TD_HUB_WORDS, the bitmap words of the vertices with edges).
+__global__ void __launch_bounds__(TD_WG) k_td_edges(const unsigned long long *ctl, const uint32_t *__restrict__ queue,
+                                                    const uint64_t *__restrict__ qoff, const uint64_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ vis,
+                                                    uint32_t *next, uint32_t hub_words) {
     if (!do_gate(ctl, DF_TD)) return;
-    const bool test_vis = !(ctl[DO_FLAGS] & DF_TDNV);
+    const unsigned long long fl = ctl[DO_FLAGS];
+    const bool test_vis = !(fl & DF_TDNV), hub = (fl & DF_TDH) && hub_words;
+    const uint32_t hub_ids = hub ? hub_words * 32 : 0u;
     __shared__ uint32_t sbeg[TD_CHUNK + 2];  // first edge of staged entry k, relative to the range (clamped)
     __shared__ uint64_t sadj[TD_CHUNK + 1];  // list start of staged entry k
+    __shared__ uint32_t hb[TD_HUB_WORDS];    // hub level: marks of ids below hub_ids
     __shared__ uint64_t j_first;
     __shared__ int nstaged;
+    if (hub) {
+        for (uint32_t i = threadIdx.x; i < hub_words; i += blockDim.x) hb[i] = 0;
+        __syncthreads();
+    }
     const uint64_t nq = ctl[DO_NQ], m = ctl[DO_MQ];
     for (uint64_t c0 = blockIdx.x * (uint64_t)TD_CHUNK; c0 < m; c0 += (uint64_t)gridDim.x * TD_CHUNK) {
         const uint32_t len = (uint32_t)min<uint64_t>(TD_CHUNK, m - c0);
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 64) {
             // largest j < nq with qoff[j] <= c0, then with qoff[j] <= c0 + len - 1
-            uint64_t lo = 0, hi = nq - 1;
-            while (lo < hi) {
-                uint64_t mid = (lo + hi + 1) >> 1;
-                if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+            const uint64_t ja = wave_upper(qoff, 0, nq - 1, c0);
+            const uint64_t jb = wave_upper(qoff, ja, nq - 1, c0 + len - 1);
+            if (threadIdx.x == 0) {
+                j_first = ja;
+                nstaged = (int)(jb - ja + 1);
             }
-            const uint64_t ja = lo;
-            hi = nq - 1;
-            while (lo < hi) {
-                uint64_t mid = (lo + hi + 1) >> 1;
-                if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
-            }
-            j_first = ja;
-            nstaged = (int)(lo - ja + 1);
         }
         __syncthreads();
         const uint64_t ja = j_first;
@@ -209,6 +239,13 @@ __global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl,
             }
 #pragma unroll
             for (int j = 0; j < TD_EPT; ++j) {
+                if (v[j] < hub_ids) {   // (never on other levels: hub_ids is 0 there)
+                    atomicOr(&hb[v[j] >> 5], 1u << (v[j] & 31));
+                    v[j] = 0xFFFFFFFFu;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < TD_EPT; ++j) {
                 nw[j] = v[j] != 0xFFFFFFFFu ? next[v[j] >> 5] : ~0u;
                 vw[j] = test_vis && v[j] != 0xFFFFFFFFu ? vis[v[j] >> 5] : 0u;
             }
@@ -220,6 +257,10 @@ __global__ void __launch_bounds__(256) k_td_edges(const unsigned long long *ctl,
             }
         }
         __syncthreads();  // the staged tables are rewritten by the next step
+    }
+    if (hub) {   // (the loop ended with a barrier)
+        for (uint32_t i = threadIdx.x; i < hub_words; i += blockDim.x)
+            if (hb[i]) atomicOr(&next[i], hb[i]);
     }
 }
 
@@ -606,7 +647,7 @@ __global__ void k_do_begin(unsigned long long *ctl, uint64_t n, uint64_t nzn) {
         else if (unreached) nfl |= DF_BUFULL;
     } else if (nf && mf) {
         nfl |= DF_TD | (f & DF_HAVEQ ? 0 : DF_LIST);
-        if (reached < nzn / 64) nfl |= DF_TDNV;
+        if (reached < nzn / 64) nfl |= DF_TDNV | (mf >= nzn / 8 ? DF_TDH : 0);
     }
     ctl[DO_NFB] = nf;
     ctl[DO_NQ] = nf;
@@ -1315,8 +1356,9 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
             k_do_block_sums<<<gsc, 256, 0, w->stream>>>(ctl, w->qdeg.ptr, w->qsum.ptr);
             k_do_scan<<<1, 1024, 0, w->stream>>>(ctl, w->qsum.ptr);
             k_do_deg_scan<<<gsc, 256, 0, w->stream>>>(ctl, w->qdeg.ptr, w->qsum.ptr, w->qoff.ptr);
-            k_td_edges<<<DO_TD_GRID, 256, 0, w->stream>>>(ctl, w->queue.ptr, w->qoff.ptr, g->adj_off.ptr,
-                                                         g->adj_nbr.ptr, vis, next);
+            k_td_edges<<<DO_TD_GRID, TD_WG, 0, w->stream>>>(ctl, w->queue.ptr, w->qoff.ptr, g->adj_off.ptr,
+                                                           g->adj_nbr.ptr, vis, next,
+                                                           (uint32_t)std::min<uint64_t>(TD_HUB_WORDS, act_words));
             // bottom-up: over the unreached list (few left; the queue is free then) or over all
             k_unreached_list<<<gul, 256, 0, w->stream>>>(ctl, vis, nzn, w->queue.ptr);
             k_bu_expand<true><<<glist, 256, 0, w->stream>>>(ctl, nzn, w->queue.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
