@@ -155,26 +155,6 @@ constexpr unsigned TD_WG = 1024;                 // threads per top-down workgro
 constexpr uint32_t TD_HUB_IDS = 1u << 19;        // hub level: relabelled ids whose marks go through LDS
 constexpr uint32_t TD_HUB_WORDS = TD_HUB_IDS / 32;
 
-// Largest j in [lo, hi] with qoff[j] <= target, given qoff[lo] <= target
-// (qoff non-decreasing), by the 64 lanes of a wave: each step probes 64
-// evenly spaced entries and keeps the interval after the last one <= target,
-// so a queue of 2^18 entries takes 3 steps of one load per lane (a binary
-// search by one thread took 18 dependent loads per chunk, during which the
-// rest of the workgroup waited).  Every lane of the wave calls it.
-__device__ inline uint64_t wave_upper(const uint64_t *__restrict__ qoff, uint64_t lo, uint64_t hi, uint64_t target) {
-    const uint64_t lane = threadIdx.x & 63;
-    while (hi > lo) {
-        const uint64_t span = hi - lo + 1;
-        const uint64_t pos = lo + span * lane / 64;
-        const unsigned long long m = __ballot(qoff[pos] <= target);   // lane 0 holds: qoff[lo] <= target
-        const uint64_t l = 63 - __clzll(m);
-        const uint64_t nlo = lo + span * l / 64;
-        hi = l == 63 ? hi : lo + span * (l + 1) / 64 - 1;
-        lo = nlo;
-    }
-    return lo;
-}
-
 // On a hub level (DF_TDH: hundreds of millions of frontier edges while few
 // vertices are reached) about half of the edges land on the TD_HUB_IDS
 // heaviest vertices.  Their marks are or-ed into a bitmap in LDS instead of

@@ -153,20 +153,13 @@ __global__ void __launch_bounds__(256) k_ms_td(const uint32_t *__restrict__ queu
     const uint64_t c0 = blockIdx.x * (uint64_t)MS_CHUNK;
     if (c0 >= m) return;
     const uint32_t len = (uint32_t)min<uint64_t>(MS_CHUNK, m - c0);
-    if (threadIdx.x == 0) {
-        uint64_t lo = 0, hi = nq - 1;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+    if (threadIdx.x < 64) {   // the first wave searches the queue prefix (wave_upper)
+        const uint64_t ja = wave_upper(qoff, 0, nq - 1, c0);
+        const uint64_t jb = wave_upper(qoff, ja, nq - 1, c0 + len - 1);
+        if (threadIdx.x == 0) {
+            j_first = ja;
+            nstaged = (int)(jb - ja + 1);
         }
-        const uint64_t ja = lo;
-        hi = nq - 1;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
-        }
-        j_first = ja;
-        nstaged = (int)(lo - ja + 1);
     }
     __syncthreads();
     const uint64_t ja = j_first;
@@ -665,20 +658,13 @@ __global__ void __launch_bounds__(256) k_msg_td(const unsigned long long *ctl, c
     const unsigned long long tdl = ctl[C_TDL];
     for (uint64_t c0 = blockIdx.x * (uint64_t)MS_CHUNK; c0 < m; c0 += (uint64_t)gridDim.x * MS_CHUNK) {
         const uint32_t len = (uint32_t)min<uint64_t>(MS_CHUNK, m - c0);
-        if (threadIdx.x == 0) {
-            uint64_t lo = 0, hi = nq - 1;
-            while (lo < hi) {
-                uint64_t mid = (lo + hi + 1) >> 1;
-                if (qoff[mid] <= c0) lo = mid; else hi = mid - 1;
+        if (threadIdx.x < 64) {   // the first wave searches the queue prefix (wave_upper)
+            const uint64_t ja = wave_upper(qoff, 0, nq - 1, c0);
+            const uint64_t jb = wave_upper(qoff, ja, nq - 1, c0 + len - 1);
+            if (threadIdx.x == 0) {
+                j_first = ja;
+                nstaged = (int)(jb - ja + 1);
             }
-            const uint64_t ja = lo;
-            hi = nq - 1;
-            while (lo < hi) {
-                uint64_t mid = (lo + hi + 1) >> 1;
-                if (qoff[mid] <= c0 + len - 1) lo = mid; else hi = mid - 1;
-            }
-            j_first = ja;
-            nstaged = (int)(lo - ja + 1);
         }
         __syncthreads();
         const uint64_t ja = j_first;

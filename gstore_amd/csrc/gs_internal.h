@@ -344,6 +344,26 @@ __device__ inline uint32_t wave_incl_scan(uint32_t x) {
     return x;
 }
 
+// Largest j in [lo, hi] with qoff[j] <= target, given qoff[lo] <= target
+// (qoff non-decreasing, a queue prefix of list lengths), by the 64 lanes of a wave: each step probes 64
+// evenly spaced entries and keeps the interval after the last one <= target,
+// so a queue of 2^18 entries takes 3 steps of one load per lane (a binary
+// search by one thread took 18 dependent loads per chunk, during which the
+// rest of the workgroup waited).  Every lane of the wave calls it.
+__device__ inline uint64_t wave_upper(const uint64_t *__restrict__ qoff, uint64_t lo, uint64_t hi, uint64_t target) {
+    const uint64_t lane = threadIdx.x & 63;
+    while (hi > lo) {
+        const uint64_t span = hi - lo + 1;
+        const uint64_t pos = lo + span * lane / 64;
+        const unsigned long long m = __ballot(qoff[pos] <= target);   // lane 0 holds: qoff[lo] <= target
+        const uint64_t l = 63 - __clzll(m);
+        const uint64_t nlo = lo + span * l / 64;
+        hi = l == 63 ? hi : lo + span * (l + 1) / 64 - 1;
+        lo = nlo;
+    }
+    return lo;
+}
+
 // XCD-aware chunk order: workgroup b runs on XCD b % 8; give each XCD a
 // contiguous run of chunks so its L2 sees consecutive tiles.
 __device__ inline uint64_t xcd_chunk(uint64_t bid, uint64_t nchunk) {
