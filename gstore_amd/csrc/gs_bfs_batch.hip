@@ -252,8 +252,7 @@ __device__ inline void ms_bu_heavy(uint64_t nheavy, unsigned long long all, cons
             for (int j = 0; j < 4; ++j)
                 if (k0 + j * 64 + lane < e) acc |= f[j] & need;
             ck += min<uint64_t>(256, e - k0);
-#pragma unroll
-            for (int s = 32; s > 0; s >>= 1) acc |= __shfl_xor(acc, s);
+            acc = wave_or64(acc);
             if (acc == need) break;
         }
         if (lane == 0 && acc) next[v] |= acc;

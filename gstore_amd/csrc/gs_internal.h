@@ -344,6 +344,25 @@ __device__ inline uint32_t wave_incl_scan(uint32_t x) {
     return x;
 }
 
+// OR over the 64 lanes of a wave, returned to every lane: the DPP pattern of
+// wave_incl_scan with OR (an inclusive OR scan; lane 63 holds the total), each
+// 32-bit half on its own, then read from lane 63.  Twelve VALU operations
+// where six 64-bit shuffles took twelve LDS permutes.  All lanes active.
+__device__ inline uint32_t wave_or_scan32(uint32_t x) {
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+__device__ inline unsigned long long wave_or64(unsigned long long x) {
+    const uint32_t lo = wave_or_scan32((uint32_t)x), hi = wave_or_scan32((uint32_t)(x >> 32));
+    return (unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)hi, 63) << 32 |
+           (uint32_t)__builtin_amdgcn_readlane((int)lo, 63);
+}
+
 // Largest j in [lo, hi] with qoff[j] <= target, given qoff[lo] <= target
 // (qoff non-decreasing, a queue prefix of list lengths), by the 64 lanes of a wave: each step probes 64
 // evenly spaced entries and keeps the interval after the last one <= target,

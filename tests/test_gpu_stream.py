@@ -144,3 +144,22 @@ def test_streaming_graph_not_served():
         g.set_device_budget(2 * CH + CH, CH)
         with pytest.raises(ga.GStoreError):
             srv.add_graph(g)
+
+
+def test_host_grid_view():
+    """gs_get_host_grid: the pinned host grid of a semi-external graph, read in
+    place (equal to the grid the tile arrays return); refused for resident
+    graphs."""
+    n, e = graphs.kron(13)
+    start, grid, deg = oracle.convert(e, n)
+    with ga.Graph.from_arrays(n, start, grid, deg) as g:
+        with pytest.raises(ga.GStoreError):
+            g.host_grid()
+        g.set_device_budget(2 * 3 * CH + CH, 3 * CH)
+        assert not g.resident
+        view = g.host_grid()
+        assert view.size == g.E and not view.flags.writeable
+        assert np.array_equal(view, grid)
+        s2, g2, d2 = g.arrays(with_grid=False)
+        assert g2 is None and np.array_equal(s2, start) and np.array_equal(d2, deg)
+
