@@ -192,9 +192,13 @@ __global__ void __launch_bounds__(256) k_kc_peel_list(int32_t *deg, uint64_t nzn
 // One decrement per list entry of the listed vertices.  The KC_HUB heaviest
 // vertices (relabelled ids below KC_HUB) collect their decrements in LDS
 // first, and the workgroups (at most KC_GRID) stride over the queue, so a
-// hub's count is flushed with one global atomic per workgroup.
-constexpr uint32_t KC_HUB = 4096;
-constexpr uint32_t KC_GRID = 2048;
+// hub's count is flushed with one global atomic per workgroup.  2^15 hub
+// counters fill 128 KiB of LDS, one 1024-thread workgroup per CU: at Kron-28
+// k = 8 the pushes took 5.8 ms per run, against 6.7 with 4096 counters in
+// 256-thread workgroups (16384: 6.1, 8192: 6.4).
+constexpr uint32_t KC_HUB = 32768;
+constexpr uint32_t KC_GRID = 512;
+constexpr int KC_WG = 1024;
 
 // A lane per listed vertex pushes a list of at most KC_LANE_MAX entries
 // alone; longer lists (a vertex whose degree fell below k keeps its whole
@@ -204,7 +208,7 @@ constexpr uint32_t KC_GRID = 2048;
 // *pushed counts the entries.
 constexpr uint32_t KC_LANE_MAX = 64;
 
-__global__ void __launch_bounds__(256) k_kc_push(const uint32_t *__restrict__ queue, const int32_t *__restrict__ qlen,
+__global__ void __launch_bounds__(KC_WG) k_kc_push(const uint32_t *__restrict__ queue, const int32_t *__restrict__ qlen,
                                                  uint64_t nq, const uint64_t *__restrict__ off,
                                                  const uint32_t *__restrict__ nbr, int32_t *deg,
                                                  unsigned long long *pushed) {
@@ -275,7 +279,7 @@ static int run_kcore_index(const gs_graph *gc, gs_ws *w, int k, int *rounds_out)
             return GS_ERR_DEVICE;
         }
         w->timer.begin(GS_K_KCORE_SCAN, w->stream, &kb);
-        k_kc_push<<<(unsigned)std::min<uint64_t>((nq + 255) / 256, KC_GRID), 256, 0, w->stream>>>(
+        k_kc_push<<<(unsigned)std::min<uint64_t>((nq + KC_WG - 1) / KC_WG, KC_GRID), KC_WG, 0, w->stream>>>(
             w->queue.ptr, w->qdeg.ptr, nq, g->adj_off.ptr, g->adj_nbr.ptr, w->kc_deg.ptr, w->counters.ptr + 1);
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_KCORE_SCAN, w->stream, kb);
