@@ -947,14 +947,6 @@ static void release_adjacency(gs_graph *g) {
     g->nzn = g->npull = g->ms_nhub = 0;
 }
 
-struct EventPair {
-    hipEvent_t a = nullptr, b = nullptr;
-    ~EventPair() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-    }
-};
-
 static int build_adjacency_locked(gs_graph *g);
 
 // Every device allocation of the index build goes through here, so a test can
@@ -1291,9 +1283,10 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     const uint64_t act_words = (g->nzn + 31) / 32;
     w->timer.reset();
     w->stats = gs_stats{};
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
     GS_HIP(hipMemsetAsync(w->depth_r.ptr, 0, g->n, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_vis.ptr, 0, words * 4, w->stream));
@@ -1384,8 +1377,6 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     const int nlev = (int)hc[DO_NLEV];
     const unsigned long long checked = hc[DO_CHECKED];
     const double khz = wall_clock_khz();

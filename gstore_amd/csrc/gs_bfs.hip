@@ -115,9 +115,10 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     const uint64_t words = bitmap_words(g);
     w->timer.reset();
     w->stats = gs_stats{};
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
     GS_HIP(hipMemsetAsync(w->depth.ptr, 0, g->n, w->stream));
     GS_HIP(hipMemsetAsync(w->bm_vis.ptr, 0, words * 4, w->stream));
@@ -129,9 +130,10 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
     GS_HIP(hipGetLastError());
     int nlev = 0, status = GS_OK;
     w->levels.clear();
-    hipEvent_t la, lb;
-    GS_HIP(hipEventCreate(&la));
-    GS_HIP(hipEventCreate(&lb));
+    EventPair ev_la;
+    GS_HIP(hipEventCreate(&ev_la.a));
+    GS_HIP(hipEventCreate(&ev_la.b));
+    hipEvent_t la = ev_la.a, lb = ev_la.b;
     uint64_t prev_e = 0, prev_t = 0, front_n = 1, rows_on = 1;
     for (uint32_t level = 1;; ++level) {
         GS_HIP(hipEventRecord(la, w->stream));
@@ -179,14 +181,10 @@ int gs::run_bfs(const gs_graph *g, gs_ws *w, uint64_t root, uint8_t *depth_out, 
             break;
         }
     }
-    (void)hipEventDestroy(la);
-    (void)hipEventDestroy(lb);
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nlev;
     w->stats.edges_scanned = w->h_counters[1];

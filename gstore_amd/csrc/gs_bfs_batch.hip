@@ -1000,9 +1000,10 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     if (w->ms_counts.n < 64) GS_TRY(w->ms_counts.alloc(64));
     w->timer.reset();
     w->stats = gs_stats{};
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
     if (by_index) {
         int nlev = 0;
@@ -1013,8 +1014,6 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
         GS_TRY(sync_and_collect(w));
         float ms = 0.f;
         GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-        (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
         w->stats.ms = ms;
         w->stats.iterations = (uint64_t)nlev;
         if (levels) *levels = nlev;
@@ -1076,8 +1075,6 @@ int gs::run_bfs_batch(const gs_graph *g, gs_ws *w, const uint64_t *roots, int nr
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nlev;
     w->stats.edges_scanned = w->h_counters[0];

@@ -111,6 +111,18 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Two events destroyed with their scope, so an early error return leaks none.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() = default;
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 // Per-graph timing of kernel classes (HIP events on the graph's stream).
 struct KernelTimer {
     bool on = false;

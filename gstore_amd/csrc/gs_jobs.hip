@@ -273,9 +273,10 @@ extern "C" int gs_bfs_async(gs_graph *g, uint64_t root, uint8_t *depth_out, uint
     w->stats = gs_stats{};
     DevBuf<uint8_t> depth;
     GS_TRY(depth.alloc((g->n + 3) & ~3ull));
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
     GS_HIP(hipMemsetAsync(depth.ptr, 0, depth.n, w->stream));
     k_set_byte<<<1, 1, 0, w->stream>>>(depth.ptr, root, 1);
@@ -300,8 +301,6 @@ extern "C" int gs_bfs_async(gs_graph *g, uint64_t root, uint8_t *depth_out, uint
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)ns;
     if (sweeps) *sweeps = ns;
@@ -327,9 +326,10 @@ extern "C" int gs_traverse(gs_graph *g, uint64_t pivot, uint32_t color, uint32_t
     w->stats = gs_stats{};
     DevBuf<uint32_t> cid;
     GS_TRY(cid.alloc(g->n));
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
     k_fill_u32<<<vgrid(g->n), 256, 0, w->stream>>>(cid.ptr, g->n, INVALID_CID);
     GS_HIP(hipGetLastError());
@@ -347,8 +347,6 @@ extern "C" int gs_traverse(gs_graph *g, uint64_t pivot, uint32_t color, uint32_t
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)ns;
     if (sweeps) *sweeps = ns;

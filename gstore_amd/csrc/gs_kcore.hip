@@ -306,9 +306,10 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     }
     w->timer.reset();
     w->stats = gs_stats{};
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
     const bool by_index = g->kcore_method == 1 && !g->ss;
     int nr = 0;
@@ -354,8 +355,6 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)nr;
     if (!by_index) {

@@ -606,9 +606,10 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
     GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream, true));
     w->timer.reset();
     w->stats = gs_stats{};
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
     // the pull works in the index's relabelled ids; results are mapped back at the end
     const bool pull = use_pull(g);
@@ -647,8 +648,6 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)iters;
     w->stats.edges_scanned = g->E * (uint64_t)iters;
@@ -678,9 +677,10 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
     GS_TRY(ensure_heavy(const_cast<gs_graph *>(g), w->stream, true));
     w->timer.reset();
     w->stats = gs_stats{};
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
     const bool pull = use_pull(g);
     uint64_t s_id = src;
@@ -716,8 +716,6 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
     GS_TRY(sync_and_collect(w));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     w->stats.ms = ms;
     w->stats.iterations = (uint64_t)iters;
     w->stats.edges_scanned = g->E * (uint64_t)iters;

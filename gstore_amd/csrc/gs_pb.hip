@@ -760,9 +760,10 @@ int build_blocked(gs_graph *g) {
     const uint32_t K = (uint32_t)K64;
     gs_ws *w = g->ws;
     hipStream_t st = w->stream;
-    hipEvent_t t0, t1;
-    GS_HIP(hipEventCreate(&t0));
-    GS_HIP(hipEventCreate(&t1));
+    EventPair ev_t0;
+    GS_HIP(hipEventCreate(&ev_t0.a));
+    GS_HIP(hipEventCreate(&ev_t0.b));
+    hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, st));
     // 1. list range of every destination block, cut into build slices
     std::vector<uint64_t> beg(K + 1);
@@ -1011,8 +1012,6 @@ int build_blocked(gs_graph *g) {
     GS_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
     GS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     g->pb_k = K;
     g->pb_nu1 = u1.size();
     g->pb_nu2 = u2.size();
