@@ -120,17 +120,32 @@ constexpr uint32_t AFF_ROUNDS = 2;     // neighbours linked for every vertex
 constexpr uint32_t AFF_SAMPLES = 1024; // vertices sampled to find the largest component
 
 // Round r links every vertex with its r-th list entry; round 0 takes it from
-// first[] (every vertex below nzn has one), without the offsets.
+// first[] (every vertex below nzn has one), without the offsets.  Round 0
+// runs on the fresh forest (every vertex its own root): the first entries
+// that go to a smaller id form chains of falling ids, and each vertex follows
+// its chain to the end (a vertex whose first entry is not smaller, a root) and
+// hangs itself there with a plain store to its own parent word.  No other
+// thread writes that word and none reads the forest in this round, so the
+// links need no find and no atomic, and the forest they leave is flat (no
+// compress pass after the round).  A first entry above its vertex (the vertex
+// is heavier than every neighbour) is linked in round 1 instead.
 __global__ void __launch_bounds__(256) k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                             const uint32_t *__restrict__ first, uint64_t nzn, uint32_t r,
                             unsigned long long *linked) {
     uint32_t c = 0;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t f = first[x];
         if (r == 0) {
-            wcc_link(par, (uint32_t)x, first[x]);
+            uint32_t y = (uint32_t)x, z = f;
+            while (z < y) {   // ids fall, so the walk ends
+                y = z;
+                z = first[y];
+            }
+            if (y != (uint32_t)x) par[x] = y;
             ++c;
             continue;
         }
+        if (r == 1 && f > (uint32_t)x) wcc_link(par, (uint32_t)x, f);
         const uint64_t b = off[x];
         if (off[x + 1] - b > r) {
             wcc_link(par, (uint32_t)x, nbr[b + r]);
@@ -262,7 +277,8 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     GS_HIP(hipGetLastError());
     uint64_t passes = 0;
     if (by_index) {
-        // link AFF_ROUNDS neighbours of every vertex, compressing after each round
+        // link AFF_ROUNDS neighbours of every vertex, compressing after each round but the
+        // first (it leaves a flat forest)
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
         hipEvent_t kb;
         for (uint32_t r = 0; r < AFF_ROUNDS && g->nzn; ++r) {
@@ -271,6 +287,7 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
                                                               g->adj_first.ptr, g->nzn, r, w->counters.ptr);
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
+            if (r == 0) continue;
             w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
             k_wcc_compress<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn);
             GS_HIP(hipGetLastError());
