@@ -70,7 +70,9 @@ __global__ void __launch_bounds__(SWEEP_WG) k_wcc_hook(SweepArgs a, uint32_t *pa
     sweep_chunk<false>(a, chunk, s, len, op);
 }
 
-__global__ void k_wcc_compress(uint32_t *par, uint64_t n) {
+// (with `only_if`: nothing unless *only_if is set)
+__global__ void k_wcc_compress(uint32_t *par, uint64_t n, const unsigned long long *only_if = nullptr) {
+    if (only_if && !*only_if) return;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t r = wcc_find(par, (uint32_t)v);
         if (ld_par(par, (uint32_t)v) != r) st_par(par, (uint32_t)v, r);
@@ -126,21 +128,27 @@ constexpr uint32_t AFF_SAMPLES = 1024; // vertices sampled to find the largest c
 // its chain to the end (a vertex whose first entry is not smaller, a root) and
 // hangs itself there with a plain store to its own parent word.  No other
 // thread writes that word and none reads the forest in this round, so the
-// links need no find and no atomic, and the forest they leave is flat (no
-// compress pass after the round).  A first entry above its vertex (the vertex
-// is heavier than every neighbour) is linked in round 1 instead.
+// links need no find and no atomic.  On Kronecker graphs the chains are a few
+// steps and the forest is flat afterwards; a walk is cut after AFF_WALK steps
+// (a path-like graph would make the chains as long as the graph), hanging the
+// vertex on the chain vertex reached, and then *cut is set so the compress
+// pass after the round runs.  A first entry above its vertex (the vertex is
+// heavier than every neighbour) is linked in round 1 instead.
+constexpr int AFF_WALK = 32;
 __global__ void __launch_bounds__(256) k_aff_round(uint32_t *par, const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                             const uint32_t *__restrict__ first, uint64_t nzn, uint32_t r,
-                            unsigned long long *linked) {
+                            unsigned long long *linked, unsigned long long *cut) {
     uint32_t c = 0;
+    bool cut_here = false;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nzn; x += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t f = first[x];
         if (r == 0) {
             uint32_t y = (uint32_t)x, z = f;
-            while (z < y) {   // ids fall, so the walk ends
+            for (int s = 0; s < AFF_WALK && z < y; ++s) {
                 y = z;
                 z = first[y];
             }
+            cut_here |= z < y;   // y is on the chain, not at its end
             if (y != (uint32_t)x) par[x] = y;
             ++c;
             continue;
@@ -152,6 +160,7 @@ __global__ void __launch_bounds__(256) k_aff_round(uint32_t *par, const uint64_t
             ++c;
         }
     }
+    if (cut_here) atomicOr(cut, 1ull);
     block_count(c, linked);
 }
 
@@ -277,19 +286,21 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     GS_HIP(hipGetLastError());
     uint64_t passes = 0;
     if (by_index) {
-        // link AFF_ROUNDS neighbours of every vertex, compressing after each round but the
-        // first (it leaves a flat forest)
+        // link AFF_ROUNDS neighbours of every vertex, compressing after each round (after
+        // the first only when a chain walk was cut: it leaves a flat forest otherwise)
         GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 8, w->stream));
+        GS_HIP(hipMemsetAsync(w->counters.ptr + 3, 0, 8, w->stream));
         hipEvent_t kb;
         for (uint32_t r = 0; r < AFF_ROUNDS && g->nzn; ++r) {
             w->timer.begin(GS_K_WCC_HOOK, w->stream, &kb);
             k_aff_round<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->adj_off.ptr, g->adj_nbr.ptr,
-                                                              g->adj_first.ptr, g->nzn, r, w->counters.ptr);
+                                                              g->adj_first.ptr, g->nzn, r, w->counters.ptr,
+                                                              w->counters.ptr + 3);
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_HOOK, w->stream, kb);
-            if (r == 0) continue;
             w->timer.begin(GS_K_WCC_COMPRESS, w->stream, &kb);
-            k_wcc_compress<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn);
+            k_wcc_compress<<<vgrid(g->nzn), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->nzn,
+                                                                 r == 0 ? w->counters.ptr + 3 : nullptr);
             GS_HIP(hipGetLastError());
             w->timer.end(GS_K_WCC_COMPRESS, w->stream, kb);
         }

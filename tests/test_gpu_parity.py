@@ -161,6 +161,28 @@ def test_wcc_bit_exact(loaded, name):
     assert np.array_equal(lab, olab) and nc == onc
 
 
+@pytest.mark.parametrize("n", [2000, 300000])
+def test_wcc_long_first_entry_chains(n):
+    """Paths: all inner vertices share a degree, so the first entries chain
+    through the whole path and the first link round cuts its walks (the
+    compress after it then runs); a second path with shuffled ids and a
+    Kronecker part keep the labels and the count against the oracle."""
+    _say("wcc long chains", n)
+    rng = np.random.default_rng(n)
+    m1, p1 = graphs.path(n)
+    perm = rng.permutation(n).astype(np.uint32)
+    p2 = perm[graphs.path(n)[1]] + np.uint32(n)
+    m3, k3 = graphs.kron(10)
+    e = np.concatenate([p1, p2, k3 + np.uint32(2 * n)])
+    nn = 2 * n + m3
+    start, grid, deg = oracle.convert(e, nn)
+    with ga.Graph.from_arrays(nn, start, grid, deg) as g:
+        g.set_wcc_method(1)
+        lab, nc = g.wcc()
+    olab, onc = oracle.wcc(nn, start, grid)
+    assert np.array_equal(lab, olab) and nc == onc
+
+
 @pytest.mark.parametrize("name", IDS)
 @pytest.mark.parametrize("k", [2, 4, 8])
 def test_kcore_bit_exact(loaded, name, k):
