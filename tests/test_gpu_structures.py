@@ -1,0 +1,152 @@
+"""Graph shapes unlike the Kronecker ones, against the CPU oracle.
+
+The engine's fast paths lean on a skewed degree distribution (relabelling by
+degree, hub tables in LDS, first-entry checks and chains, heaviest-first
+lists).  These graphs take that away or push it to an extreme: a 2-D mesh
+(every inner degree equal, BFS 239 levels deep), a uniform random graph (no
+hubs), a complete bipartite graph (a few dense hubs), a broom (one hub with a
+list longer than any sorted one, and a path), many disjoint 5-cliques (k-core
+at the clique size) and a caterpillar.  Every job runs on the default methods
+(over the neighbour-list index) and is compared with the oracle as the other
+parity tests do: BFS depths and level counts, WCC labels and count, k-core
+degrees, counts and rounds, compat PageRank bit for bit, intended PageRank
+within 1e-6 max-abs and 1e-4 relative of the FP64 recurrence, and 64-root
+batched BFS lane by lane.
+"""
+import numpy as np
+import pytest
+
+import gstore_amd as ga
+import oracle
+
+pytestmark = pytest.mark.gpu
+PR_TOL = 1e-6
+
+
+def _say(*a):
+    print("[gpu-structures]", *a, flush=True)
+
+
+def mesh(w=120, h=120):
+    idx = np.arange(w * h, dtype=np.uint32).reshape(h, w)
+    e = np.concatenate([np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1),
+                        np.stack([idx[:-1, :].ravel(), idx[1:, :].ravel()], 1)])
+    return w * h, e.astype(np.uint32)
+
+
+def uniform(n=60_000, m=240_000, seed=3):
+    rng = np.random.default_rng(seed)
+    return n, rng.integers(0, n, (m, 2)).astype(np.uint32)
+
+
+def bipartite(a=48, b=3000):
+    x, y = np.meshgrid(np.arange(a), a + np.arange(b), indexing="ij")
+    return a + b, np.stack([x.ravel(), y.ravel()], 1).astype(np.uint32)
+
+
+def broom(leaves=70_000, handle=150):
+    hub = np.zeros(leaves, np.uint32)
+    e = [np.stack([hub, 1 + np.arange(leaves, dtype=np.uint32)], 1)]
+    p = leaves + 1 + np.arange(handle, dtype=np.uint32)
+    e.append(np.array([[0, p[0]]], np.uint32))
+    e.append(np.stack([p[:-1], p[1:]], 1))
+    return leaves + 1 + handle, np.concatenate(e).astype(np.uint32)
+
+
+def cliques(count=2000, size=5):
+    base = (np.arange(count, dtype=np.uint32) * size)[:, None, None]
+    i, j = np.triu_indices(size, 1)
+    e = (base + np.stack([i, j], 1)[None].astype(np.uint32)).reshape(-1, 2)
+    return count * size, e.astype(np.uint32)
+
+
+def caterpillar(spine=200, legs=5):
+    s = np.arange(spine, dtype=np.uint32)
+    e = [np.stack([s[:-1], s[1:]], 1)]
+    leg = spine + np.arange(spine * legs, dtype=np.uint32)
+    e.append(np.stack([np.repeat(s, legs), leg], 1))
+    return spine * (1 + legs), np.concatenate(e).astype(np.uint32)
+
+
+SHAPES = {"mesh": mesh, "uniform": uniform, "bipartite": bipartite, "broom": broom, "cliques": cliques,
+          "caterpillar": caterpillar}
+
+
+@pytest.fixture(scope="module")
+def loaded():
+    res = {}
+    for name, fn in SHAPES.items():
+        n, e = fn()
+        _say("load", name, "N", n, "tuples", len(e))
+        arrays = oracle.convert(e, n)
+        res[name] = (n, arrays, ga.Graph.from_arrays(n, *arrays))
+    yield res
+    for _, _, g in res.values():
+        g.close()
+
+
+def _roots(n, deg):
+    nz = np.flatnonzero(deg)
+    return sorted({0, n - 1, int(np.argmax(deg)), int(nz[len(nz) // 2]), int(nz[-1])})
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_bfs(loaded, name):
+    n, (start, grid, deg), g = loaded[name]
+    for r in _roots(n, deg):
+        _say("bfs", name, r)
+        res = g.bfs(r)
+        d, lc = oracle.bfs(n, start, grid, r)
+        assert np.array_equal(res.depth, d), (name, r)
+        assert res.level_counts == lc
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_bfs_batch(loaded, name):
+    n, (start, grid, deg), g = loaded[name]
+    rng = np.random.default_rng(len(name))
+    roots = rng.choice(np.flatnonzero(deg), min(64, int(np.count_nonzero(deg))), replace=False)
+    _say("bfs batch", name, len(roots))
+    depth, counts = g.bfs_batch(roots)
+    for i, r in enumerate(roots):
+        d, lc = oracle.bfs(n, start, grid, int(r))
+        assert np.array_equal(depth[i], d), (name, i, r)
+        assert counts[i][:len(lc)] == lc
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_wcc(loaded, name):
+    n, (start, grid, deg), g = loaded[name]
+    _say("wcc", name)
+    lab, nc = g.wcc()
+    olab, onc = oracle.wcc(n, start, grid)
+    assert np.array_equal(lab, olab) and nc == onc
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+@pytest.mark.parametrize("k", [2, 3, 5, 6])
+def test_kcore(loaded, name, k):
+    n, (start, grid, deg), g = loaded[name]
+    _say("kcore", name, k)
+    kd, ne0, pos, rounds = g.kcore(k)
+    okd, one0, opos, orounds = oracle.kcore(n, start, grid, deg, k)
+    assert np.array_equal(kd, okd) and (ne0, pos, rounds) == (one0, opos, orounds)
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_pagerank(loaded, name):
+    n, (start, grid, deg), g = loaded[name]
+    _say("pagerank", name)
+    assert np.array_equal(g.pagerank(20, ga.PR_COMPAT), oracle.pagerank(n, start, grid, deg, 20, 0))
+    got = g.pagerank(20, ga.PR_INTENDED).astype(np.float64)
+    w64 = oracle.pagerank64(n, start, grid, deg, 20)
+    d = np.abs(got - w64)
+    nz = w64 > 0
+    # 1e-6 max-abs where FP32 arithmetic can meet it; where ranks are of order 1
+    # (the broom's leaves sit near 0.52) the oracle's own FP32 sweep is 6.6e-5
+    # from the FP64 recurrence, and the engine is held to that distance instead
+    own = float(np.max(np.abs(oracle.pagerank(n, start, grid, deg, 20, 1).astype(np.float64) - w64)))
+    _say(f"  vs FP64: max-abs {d.max():.3g} (oracle FP32: {own:.3g})")
+    assert d.max() <= max(PR_TOL, own)
+    assert (float(np.max(d[nz] / w64[nz])) if nz.any() else 0.0) <= 1e-4
+    assert np.all(got[~nz] == 0)
