@@ -11,13 +11,15 @@ at the clique size) and a caterpillar.  Every job runs on the default methods
 parity tests do: BFS depths and level counts, WCC labels and count, k-core
 degrees, counts and rounds, compat PageRank bit for bit, intended PageRank
 within 1e-6 max-abs and 1e-4 relative of the FP64 recurrence, and 64-root
-batched BFS lane by lane.
+batched BFS lane by lane.  The same shapes also run as directed grids and in
+semi-external mode with nearly the whole grid streamed.
 """
 import numpy as np
 import pytest
 
 import gstore_amd as ga
 import oracle
+from tests import graphs
 
 pytestmark = pytest.mark.gpu
 PR_TOL = 1e-6
@@ -150,3 +152,48 @@ def test_pagerank(loaded, name):
     assert d.max() <= max(PR_TOL, own)
     assert (float(np.max(d[nz] / w64[nz])) if nz.any() else 0.0) <= 1e-4
     assert np.all(got[~nz] == 0)
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_directed(name):
+    """The same tuples read as a directed graph (the p x p grid): BFS along
+    the edges, WCC, PageRank by in-edges; k-core is unsupported there."""
+    n, e = SHAPES[name]()
+    start, grid, deg = oracle.convert(e, n, directed=True)
+    _say("directed", name)
+    with ga.Graph.from_arrays(n, start, grid, deg, directed=True) as g:
+        for r in _roots(n, deg):
+            d, lc = oracle.bfs(n, start, grid, r, directed=True)
+            res = g.bfs(r)
+            assert np.array_equal(res.depth, d) and res.level_counts == lc, (name, r)
+        lab, nc = g.wcc()
+        olab, onc = oracle.wcc(n, start, grid, directed=True)
+        assert np.array_equal(lab, olab) and nc == onc
+        err, bound = graphs.directed_pr_check(g.pagerank(20, ga.PR_INTENDED), n, start, grid, deg, 20)
+        _say(f"  intended vs FP64: {err:.3g} (bound {bound:.3g})")
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_streamed(name):
+    """Semi-external: one-chunk segments and a one-chunk resident prefix, so
+    nearly the whole grid streams; integer results and compat PageRank equal
+    the oracle's."""
+    n, e = SHAPES[name]()
+    start, grid, deg = oracle.convert(e, n)
+    ch = 4096 * 4
+    with ga.Graph.from_arrays(n, start, grid, deg) as g:
+        g.set_device_budget(2 * ch + ch + ch, ch)
+        if g.resident:
+            pytest.skip("the grid fits the budget")
+        _say("streamed", name, "resident edges", g.resident_edges, "of", g.E)
+        for r in _roots(n, deg)[:3]:
+            d, lc = oracle.bfs(n, start, grid, r)
+            res = g.bfs(r)
+            assert np.array_equal(res.depth, d) and res.level_counts == lc, (name, r)
+        lab, nc = g.wcc()
+        olab, onc = oracle.wcc(n, start, grid)
+        assert np.array_equal(lab, olab) and nc == onc
+        kd, ne0, pos, rounds = g.kcore(3)
+        okd, one0, opos, orounds = oracle.kcore(n, start, grid, deg, 3)
+        assert np.array_equal(kd, okd) and (ne0, pos, rounds) == (one0, opos, orounds)
+        assert np.array_equal(g.pagerank(10, ga.PR_COMPAT), oracle.pagerank(n, start, grid, deg, 10, 0))
