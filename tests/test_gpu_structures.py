@@ -11,8 +11,9 @@ at the clique size) and a caterpillar.  Every job runs on the default methods
 parity tests do: BFS depths and level counts, WCC labels and count, k-core
 degrees, counts and rounds, compat PageRank bit for bit, intended PageRank
 within 1e-6 max-abs and 1e-4 relative of the FP64 recurrence, and 64-root
-batched BFS lane by lane.  The same shapes also run as directed grids and in
-semi-external mode with nearly the whole grid streamed.
+batched BFS lane by lane.  The same shapes also run as directed grids, in
+semi-external mode with nearly the whole grid streamed, in external mode (the
+grid read from its file) and under the server, queries mixed over four of them.
 """
 import numpy as np
 import pytest
@@ -197,3 +198,63 @@ def test_streamed(name):
         okd, one0, opos, orounds = oracle.kcore(n, start, grid, deg, 3)
         assert np.array_equal(kd, okd) and (ne0, pos, rounds) == (one0, opos, orounds)
         assert np.array_equal(g.pagerank(10, ga.PR_COMPAT), oracle.pagerank(n, start, grid, deg, 10, 0))
+
+
+@pytest.mark.parametrize("name", ["mesh", "uniform", "broom"])
+def test_external(tmp_path, name):
+    """External mode: the grid read from its .grid file in every streamed pass
+    (one-chunk segments, a one-chunk resident prefix)."""
+    n, e = SHAPES[name]()
+    start, grid, deg = oracle.convert(e, n)
+    pre = str(tmp_path / name)
+    ga.convert(e, n, pre)
+    ch = 4096 * 4
+    with ga.Graph.open_external(pre, n, 2 * ch + ch + ch, segment_bytes=ch) as g:
+        assert not g.resident
+        _say("external", name, "resident edges", g.resident_edges, "of", g.E)
+        r = _roots(n, deg)[-1]
+        d, lc = oracle.bfs(n, start, grid, r)
+        res = g.bfs(r)
+        assert np.array_equal(res.depth, d) and res.level_counts == lc
+        lab, nc = g.wcc()
+        olab, onc = oracle.wcc(n, start, grid)
+        assert np.array_equal(lab, olab) and nc == onc
+        kd, ne0, pos, rounds = g.kcore(3)
+        okd, one0, opos, orounds = oracle.kcore(n, start, grid, deg, 3)
+        assert np.array_equal(kd, okd) and (ne0, pos, rounds) == (one0, opos, orounds)
+        assert np.array_equal(g.pagerank(5, ga.PR_COMPAT), oracle.pagerank(n, start, grid, deg, 5, 0))
+
+
+def test_serving_mix_over_shapes(loaded):
+    """The server over four of the shapes at once: BFS, personalised PageRank
+    and WCC queries interleaved, every answer equal to the oracle's (PPR
+    within 1e-6 L1 of the FP64 oracle, 20 iterations)."""
+    names = ["mesh", "uniform", "bipartite", "cliques"]
+    rng = np.random.default_rng(9)
+    queries = []
+    for gi, nm in enumerate(names):
+        n, (start, grid, deg), _ = loaded[nm]
+        queries += [(gi, ga.Q_BFS, int(r)) for r in rng.choice(np.flatnonzero(deg), 12, replace=False)]
+        queries += [(gi, ga.Q_PPR, int(np.argmax(deg))), (gi, ga.Q_WCC, 0)]
+    rng.shuffle(queries)
+    with ga.Server(workers=4) as srv:
+        ids = [srv.add_graph(loaded[nm][2]) for nm in names]
+        outs = {}
+        for gi, kind, arg in queries:
+            n = loaded[names[gi]][0]
+            out = np.empty(n, np.uint8 if kind == ga.Q_BFS else np.float32 if kind == ga.Q_PPR else np.uint32)
+            outs[srv.submit(ids[gi], kind, arg, out)] = (gi, kind, arg, out)
+        srv.drain()
+        res = srv.poll(4096)
+    _say("served", len(res), "batches up to", max(r["batch_size"] for r in res))
+    assert len(res) == len(queries) and all(r["status"] == 0 for r in res)
+    for r in res:
+        gi, kind, arg, out = outs[r["id"]]
+        n, (start, grid, deg), _ = loaded[names[gi]]
+        if kind == ga.Q_BFS:
+            assert np.array_equal(out, oracle.bfs(n, start, grid, arg)[0]), (names[gi], arg)
+        elif kind == ga.Q_WCC:
+            assert np.array_equal(out, oracle.wcc(n, start, grid)[0]), names[gi]
+        else:
+            want = oracle.ppr(n, start, grid, deg, arg, 20)
+            assert float(np.abs(out.astype(np.float64) - want).sum()) <= 1e-6, names[gi]
