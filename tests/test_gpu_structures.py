@@ -174,7 +174,7 @@ def test_directed(name):
         _say(f"  intended vs FP64: {err:.3g} (bound {bound:.3g})")
 
 
-@pytest.mark.parametrize("name", list(SHAPES))
+@pytest.mark.parametrize("name", [s for s in SHAPES if s != "caterpillar"])   # its grid fits four chunks
 def test_streamed(name):
     """Semi-external: one-chunk segments and a one-chunk resident prefix, so
     nearly the whole grid streams; integer results and compat PageRank equal
