@@ -1278,7 +1278,6 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     if (w->depth.n != g->n) GS_TRY(w->depth.alloc(g->n));
     if (w->depth_r.n != g->n) GS_TRY(w->depth_r.alloc(g->n));
     GS_TRY(ensure_queue(g, w));
-    const uint64_t words = bitmap_words(g);
     // bitmap words that hold vertices with edges: only they change after the root
     const uint64_t act_words = (g->nzn + 31) / 32;
     w->timer.reset();
@@ -1288,10 +1287,14 @@ int gs::run_bfs_do(const gs_graph *gc, gs_ws *w, uint64_t root, uint8_t *depth_o
     GS_HIP(hipEventCreate(&ev_t0.b));
     hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
-    GS_HIP(hipMemsetAsync(w->depth_r.ptr, 0, g->n, w->stream));
-    GS_HIP(hipMemsetAsync(w->bm_vis.ptr, 0, words * 4, w->stream));
-    GS_HIP(hipMemsetAsync(w->bm_front.ptr, 0, words * 4, w->stream));
-    GS_HIP(hipMemsetAsync(w->bm_next.ptr, 0, words * 4, w->stream));
+    // only the vertices with edges: every kernel of the search reads and writes
+    // depths and bitmap words below nzn (act_words), and the unpermute gives the
+    // others depth 0 (the root 1) without reading depth_r; what a root without
+    // edges leaves above them is never read
+    GS_HIP(hipMemsetAsync(w->depth_r.ptr, 0, g->nzn, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_vis.ptr, 0, act_words * 4, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_front.ptr, 0, act_words * 4, w->stream));
+    GS_HIP(hipMemsetAsync(w->bm_next.ptr, 0, act_words * 4, w->stream));
     if (w->do_ctl.n != DO_WORDS) GS_TRY(w->do_ctl.alloc(DO_WORDS));
     unsigned long long *ctl = w->do_ctl.ptr;
     GS_HIP(hipMemsetAsync(ctl, 0, DO_WORDS * 8, w->stream));
