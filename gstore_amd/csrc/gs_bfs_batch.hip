@@ -117,11 +117,14 @@ constexpr uint32_t MS_CHUNK = 2048;
 
 // One wave.  Roots without edges go to the end of the first queue: zero-width
 // entries there are never inside a top-down chunk's range, so a chunk stages
-// at most one entry per edge position (plus one).
+// at most one entry per edge position (plus one).  Depth rows exist only for
+// the vertices with edges (ids below nzn); the lanes of roots without edges
+// get their depth in k_ms_unpermute.  Their mask words (above nzn, never
+// cleared) are read only as a queue entry with an empty list.
 __global__ void __launch_bounds__(64) k_ms_roots(const uint64_t *__restrict__ roots, int nr,
                                                  const uint32_t *__restrict__ new_id, const int32_t *__restrict__ deg,
-                                                 unsigned long long *vis, unsigned long long *front, uint32_t *queue,
-                                                 int32_t *qdeg, uint8_t *depth_vm) {
+                                                 uint64_t nzn, unsigned long long *vis, unsigned long long *front,
+                                                 uint32_t *queue, int32_t *qdeg, uint8_t *depth_vm) {
     const int i = threadIdx.x;
     const bool in = i < nr;
     uint32_t r = 0;
@@ -131,7 +134,7 @@ __global__ void __launch_bounds__(64) k_ms_roots(const uint64_t *__restrict__ ro
         d = deg[r];
         atomicOr(&vis[r], 1ull << i);
         atomicOr(&front[r], 1ull << i);
-        if (depth_vm) depth_vm[(uint64_t)r * 64 + i] = 1;
+        if (depth_vm && r < nzn) depth_vm[(uint64_t)r * 64 + i] = 1;
     }
     const unsigned long long with_edges = __ballot(in && d > 0), without = __ballot(in && d <= 0);
     if (!in) return;
@@ -419,17 +422,29 @@ __global__ void __launch_bounds__(256) k_ms_list(uint64_t nzn, const unsigned lo
     }
 }
 
-// depth (lane-major, vertex order) from the vertex-major rows of relabelled ids.
+// depth (lane-major, vertex order) from the vertex-major rows of relabelled
+// ids; a vertex without edges (no row) is unreached in every lane but its own
+// root's, set afterwards by k_ms_root_depth.
 __global__ void k_ms_unpermute(const uint8_t *__restrict__ depth_vm, const uint32_t *__restrict__ new_id, uint64_t n,
-                               int nr, uint8_t *depth) {
+                               uint64_t nzn, int nr, uint8_t *depth) {
     for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n; o += (uint64_t)gridDim.x * blockDim.x) {
-        const uint4 *row = reinterpret_cast<const uint4 *>(depth_vm + (uint64_t)new_id[o] * 64);
-        uint4 q[4];
+        const uint32_t r = new_id[o];
+        uint4 q[4] = {};
+        if (r < nzn) {
+            const uint4 *row = reinterpret_cast<const uint4 *>(depth_vm + (uint64_t)r * 64);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) q[k] = row[k];
+            for (int k = 0; k < 4; ++k) q[k] = row[k];
+        }
         const uint8_t *b = reinterpret_cast<const uint8_t *>(q);
         for (int l = 0; l < nr; ++l) depth[(uint64_t)l * n + o] = b[l];
     }
+}
+
+// Depth 1 at the root of every lane whose root has no edges (one thread per lane).
+__global__ void k_ms_root_depth(const uint64_t *__restrict__ roots, int nr, const uint32_t *__restrict__ new_id,
+                                uint64_t n, uint64_t nzn, uint8_t *depth) {
+    const int l = threadIdx.x;
+    if (l < nr && new_id[roots[l]] >= nzn) depth[(uint64_t)l * n + roots[l]] = 1;
 }
 
 static unsigned ms_grid(uint64_t n) { return (unsigned)std::min<uint64_t>((n + 255) / 256, 8192); }
@@ -750,11 +765,12 @@ static int run_bfs_batch_index(const gs_graph *gc, gs_ws *w, const uint64_t *roo
                        *next = (unsigned long long *)w->ms_next.ptr;
     uint8_t *dvm = keep_depth ? w->ms_depth_vm.ptr : nullptr;
     GS_HIP(hipMemcpyAsync(w->ms_roots.ptr, roots, nr * 8, hipMemcpyHostToDevice, w->stream));
-    GS_HIP(hipMemsetAsync(vis, 0, n * 8, w->stream));
-    GS_HIP(hipMemsetAsync(front, 0, n * 8, w->stream));
-    GS_HIP(hipMemsetAsync(next, 0, n * 8, w->stream));
-    if (dvm) GS_HIP(hipMemsetAsync(dvm, 0, n * 64, w->stream));
-    k_ms_roots<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->new_id.ptr, g->deg_r.ptr, vis, front, w->queue.ptr,
+    // the vertices with edges only (the others keep no state; see k_ms_roots)
+    GS_HIP(hipMemsetAsync(vis, 0, nzn * 8, w->stream));
+    GS_HIP(hipMemsetAsync(front, 0, nzn * 8, w->stream));
+    GS_HIP(hipMemsetAsync(next, 0, nzn * 8, w->stream));
+    if (dvm) GS_HIP(hipMemsetAsync(dvm, 0, nzn * 64, w->stream));
+    k_ms_roots<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->new_id.ptr, g->deg_r.ptr, nzn, vis, front, w->queue.ptr,
                                         w->qdeg.ptr, dvm);
     GS_HIP(hipGetLastError());
     std::vector<unsigned long long> lc(64), scratch;
@@ -841,7 +857,8 @@ static int run_bfs_batch_index(const gs_graph *gc, gs_ws *w, const uint64_t *roo
         }
     }
     if (status == GS_OK && dvm) {
-        k_ms_unpermute<<<ms_grid(n), 256, 0, w->stream>>>(dvm, g->new_id.ptr, n, nr, w->ms_depth.ptr);
+        k_ms_unpermute<<<ms_grid(n), 256, 0, w->stream>>>(dvm, g->new_id.ptr, n, nzn, nr, w->ms_depth.ptr);
+        k_ms_root_depth<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->new_id.ptr, n, nzn, w->ms_depth.ptr);
         GS_HIP(hipGetLastError());
     }
     w->stats.edges_scanned = checked;
@@ -862,12 +879,13 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
     if (w->ms_ctl.n != C_WORDS) GS_TRY(w->ms_ctl.alloc(C_WORDS));
     unsigned long long *ctl = w->ms_ctl.ptr;
     GS_HIP(hipMemcpyAsync(w->ms_roots.ptr, roots, nr * 8, hipMemcpyHostToDevice, w->stream));
-    GS_HIP(hipMemsetAsync(vis, 0, n * 8, w->stream));
-    GS_HIP(hipMemsetAsync(front, 0, n * 8, w->stream));
-    GS_HIP(hipMemsetAsync(next, 0, n * 8, w->stream));
-    if (dvm) GS_HIP(hipMemsetAsync(dvm, 0, n * 64, w->stream));
+    // the vertices with edges only (the others keep no state; see k_ms_roots)
+    GS_HIP(hipMemsetAsync(vis, 0, nzn * 8, w->stream));
+    GS_HIP(hipMemsetAsync(front, 0, nzn * 8, w->stream));
+    GS_HIP(hipMemsetAsync(next, 0, nzn * 8, w->stream));
+    if (dvm) GS_HIP(hipMemsetAsync(dvm, 0, nzn * 64, w->stream));
     GS_HIP(hipMemsetAsync(ctl, 0, C_WORDS * 8, w->stream));
-    k_ms_roots<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->new_id.ptr, g->deg_r.ptr, vis, front, w->queue.ptr,
+    k_ms_roots<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->new_id.ptr, g->deg_r.ptr, nzn, vis, front, w->queue.ptr,
                                         w->qdeg.ptr, dvm);
     GS_HIP(hipGetLastError());
     k_msg_init<<<1, 64, 0, w->stream>>>(ctl, nr, w->qdeg.ptr, 2 * g->E, (uint32_t)g->do_alpha, (uint32_t)g->do_beta);
@@ -963,7 +981,8 @@ static int run_bfs_batch_graph(const gs_graph *gc, gs_ws *w, const uint64_t *roo
         return GS_ERR_DEPTH_OVERFLOW;
     }
     if (dvm) {
-        k_ms_unpermute<<<ms_grid(n), 256, 0, w->stream>>>(dvm, g->new_id.ptr, n, nr, w->ms_depth.ptr);
+        k_ms_unpermute<<<ms_grid(n), 256, 0, w->stream>>>(dvm, g->new_id.ptr, n, nzn, nr, w->ms_depth.ptr);
+        k_ms_root_depth<<<1, 64, 0, w->stream>>>(w->ms_roots.ptr, nr, g->new_id.ptr, n, nzn, w->ms_depth.ptr);
         GS_HIP(hipGetLastError());
     }
     return GS_OK;
