@@ -872,18 +872,21 @@ __global__ void k_gather_deg(const int32_t *__restrict__ deg, const uint32_t *__
 // 16-byte load of new_id, four gathers, a 16-byte store; out and new_id are
 // device allocations, so 16-byte aligned).
 template <class T>
-__global__ void k_unpermute(const T *__restrict__ x_r, const uint32_t *__restrict__ new_id, uint64_t n, T *out) {
+__global__ void k_unpermute(const T *__restrict__ x_r, const uint32_t *__restrict__ new_id, uint64_t n, uint64_t lim,
+                            T *out) {
     static_assert(sizeof(T) == 4, "4-byte values");
+    // relabelled ids from lim on read as 0 (k-core: only the vertices with edges hold degrees)
+    auto at = [&](uint32_t r) { return r < lim ? x_r[r] : T(0); };
     for (uint64_t v0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) * 4; v0 < n;
          v0 += (uint64_t)gridDim.x * blockDim.x * 4) {
         if (v0 + 4 <= n) {
             const uint4 q = *reinterpret_cast<const uint4 *>(new_id + v0);
-            const T a = x_r[q.x], b = x_r[q.y], c = x_r[q.z], d = x_r[q.w];
+            const T a = at(q.x), b = at(q.y), c = at(q.z), d = at(q.w);
             T *o = out + v0;
             *reinterpret_cast<uint4 *>(o) = uint4{__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
                                                   __builtin_bit_cast(uint32_t, c), __builtin_bit_cast(uint32_t, d)};
         } else {
-            for (uint64_t v = v0; v < n; ++v) out[v] = x_r[new_id[v]];
+            for (uint64_t v = v0; v < n; ++v) out[v] = at(new_id[v]);
         }
     }
 }
@@ -1214,13 +1217,13 @@ int ensure_queue(const gs_graph *g, gs_ws *w) {
 }
 
 int unpermute_i32(const gs_graph *g, hipStream_t st, const int32_t *x_r, int32_t *out) {
-    k_unpermute<int32_t><<<vgrid((g->n + 3) / 4), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, out);
+    k_unpermute<int32_t><<<vgrid((g->n + 3) / 4), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, g->nzn, out);
     GS_HIP(hipGetLastError());
     return GS_OK;
 }
 
 int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out) {
-    k_unpermute<float><<<vgrid((g->n + 3) / 4), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, out);
+    k_unpermute<float><<<vgrid((g->n + 3) / 4), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, g->n, out);
     GS_HIP(hipGetLastError());
     return GS_OK;
 }

@@ -257,7 +257,8 @@ using namespace gs;
 // k-core over the index: degrees in relabelled ids, unpermuted at the end.
 static int run_kcore_index(const gs_graph *gc, gs_ws *w, int k, int *rounds_out) {
     gs_graph *g = const_cast<gs_graph *>(gc);
-    GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, g->deg_r.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+    // the vertices with edges only: the count and the unpermute read no degree above nzn
+    GS_HIP(hipMemcpyAsync(w->kc_deg.ptr, g->deg_r.ptr, g->nzn * 4, hipMemcpyDeviceToDevice, w->stream));
     GS_HIP(hipMemsetAsync(w->counters.ptr + 1, 0, 8, w->stream));   // entries pushed, all rounds
     int nr = 0;
     for (;;) {
@@ -341,8 +342,9 @@ int gs::run_kcore(const gs_graph *g, gs_ws *w, int k, int32_t *deg_out, uint64_t
         }));
     }
     GS_HIP(hipMemsetAsync(w->counters.ptr, 0, 16, w->stream));
-    k_kc_count<<<(unsigned)std::min<uint64_t>((g->n / 4 + 255) / 256 + 1, 2048), 256, 0, w->stream>>>(
-        w->kc_deg.ptr, g->n, w->counters.ptr);
+    const uint64_t ncount = by_index ? g->nzn : g->n;   // over the index: degrees of the vertices with edges
+    k_kc_count<<<(unsigned)std::min<uint64_t>((ncount / 4 + 255) / 256 + 1, 2048), 256, 0, w->stream>>>(
+        w->kc_deg.ptr, ncount, w->counters.ptr);
     GS_HIP(hipGetLastError());
     // the final degrees in vertex order: relabelled ones are mapped back into qdeg (N entries)
     const int32_t *final_deg = w->kc_deg.ptr;

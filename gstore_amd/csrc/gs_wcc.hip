@@ -282,7 +282,8 @@ int gs::run_wcc(const gs_graph *g, gs_ws *w, uint32_t *label_out, uint64_t *ncom
     GS_HIP(hipEventCreate(&ev_t0.b));
     hipEvent_t t0 = ev_t0.a, t1 = ev_t0.b;
     GS_HIP(hipEventRecord(t0, w->stream));
-    k_wcc_init<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, g->n);
+    // over the index only the vertices with edges have a parent entry that is read
+    k_wcc_init<<<vgrid(g->n), 256, 0, w->stream>>>(w->wcc_parent.ptr, by_index ? g->nzn : g->n);
     GS_HIP(hipGetLastError());
     uint64_t passes = 0;
     if (by_index) {
