@@ -1222,8 +1222,8 @@ int unpermute_i32(const gs_graph *g, hipStream_t st, const int32_t *x_r, int32_t
     return GS_OK;
 }
 
-int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out) {
-    k_unpermute<float><<<vgrid((g->n + 3) / 4), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, g->n, out);
+int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out, uint64_t lim) {
+    k_unpermute<float><<<vgrid((g->n + 3) / 4), 256, 0, st>>>(x_r, g->new_id.ptr, g->n, lim, out);
     GS_HIP(hipGetLastError());
     return GS_OK;
 }

@@ -315,7 +315,8 @@ int scan_counts(hipStream_t st, const int32_t *cnt, uint64_t n, uint64_t *off, u
 int pr_blocked(const gs_graph *g, gs_ws *w, float *prior, const float *fac);
 int pr_blocked64(const gs_graph *g, gs_ws *w, const float *prior, double *acc);
 size_t pb_bytes(const gs_graph *g);
-int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out);
+// values of relabelled ids from `lim` on read as 0
+int unpermute_f32(const gs_graph *g, hipStream_t st, const float *x_r, float *out, uint64_t lim);
 int unpermute_i32(const gs_graph *g, hipStream_t st, const int32_t *x_r, int32_t *out);
 int ensure_queue(const gs_graph *g, gs_ws *w);  // queue, qdeg, qoff, qsum sized for N
 int queue_prefix(gs_ws *w, uint64_t nq, std::vector<unsigned long long> &scratch, uint64_t *total);

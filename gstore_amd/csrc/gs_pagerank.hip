@@ -613,12 +613,15 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
     GS_HIP(hipEventRecord(t0, w->stream));
     // the pull works in the index's relabelled ids; results are mapped back at the end
     const bool pull = use_pull(g);
-    k_pr_init<<<vgrid(g->n), 256, 0, w->stream>>>(pull ? g->deg_r.ptr : g->degree.ptr, g->n, mode,
-                                                  (float)(1.0 / (double)g->n), iters ? g->heavy_dthr : 0xFFFFFFFFu,
-                                                  w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
-    GS_HIP(hipGetLastError());
-    // the blocked passes write the new priors of the vertices with edges themselves
+    // the blocked passes write the new priors of the vertices with edges themselves and
+    // read no other; a vertex without edges ends at 0 (its factor is 0), which the
+    // unpermute below gives it without a value in the arrays
     const bool blocked = pull && g->pr_method == 2 && g->pb_k;
+    const uint64_t ninit = blocked && iters > 0 ? g->nzn : g->n;
+    k_pr_init<<<vgrid(ninit), 256, 0, w->stream>>>(pull ? g->deg_r.ptr : g->degree.ptr, ninit, mode,
+                                                   (float)(1.0 / (double)g->n), iters ? g->heavy_dthr : 0xFFFFFFFFu,
+                                                   w->pr_prior.ptr, w->pr_acc.ptr, w->pr_fac.ptr);
+    GS_HIP(hipGetLastError());
     for (int it = 0; it < iters; ++it) {
         if (blocked) {
             GS_TRY(pr_blocked(g, w, w->pr_prior.ptr, w->pr_fac.ptr));
@@ -632,17 +635,9 @@ int gs::run_pagerank(const gs_graph *g, gs_ws *w, int iters, int mode, float *ra
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
     }
-    if (blocked && iters > 0 && g->nzn < g->n) {  // vertices without edges: sum 0
-        hipEvent_t kb;
-        w->timer.begin(GS_K_PR_FINALIZE, w->stream, &kb);
-        k_pr_finalize<<<vgrid(g->n - g->nzn), 256, 0, w->stream>>>(g->n - g->nzn, w->pr_prior.ptr + g->nzn,
-                                                                    w->pr_acc.ptr + g->nzn, w->pr_fac.ptr + g->nzn, 1);
-        GS_HIP(hipGetLastError());
-        w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
-    }
-    if (pull) {  // prior (relabelled) -> acc (original order) -> prior
-        GS_TRY(unpermute_f32(g, w->stream, w->pr_prior.ptr, w->pr_acc.ptr));
-        GS_HIP(hipMemcpyAsync(w->pr_prior.ptr, w->pr_acc.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+    if (pull) {  // prior (relabelled) -> acc (original order), then the two trade places
+        GS_TRY(unpermute_f32(g, w->stream, w->pr_prior.ptr, w->pr_acc.ptr, ninit));
+        std::swap(w->pr_prior.ptr, w->pr_acc.ptr);   // both hold n values
     }
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_TRY(sync_and_collect(w));
@@ -704,13 +699,9 @@ int gs::run_ppr(const gs_graph *g, gs_ws *w, uint64_t src, int iters, float *out
         GS_HIP(hipGetLastError());
         w->timer.end(GS_K_PR_FINALIZE, w->stream, kb);
     }
-    if (pull) {  // rank (relabelled) -> acc (original order) -> rank
-        GS_TRY(unpermute_f32(g, w->stream, w->ppr_rank.ptr, w->pr_acc.ptr));
-        GS_HIP(hipMemcpyAsync(w->ppr_rank.ptr, w->pr_acc.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
-    }
-    if (pull) {  // prior (relabelled) -> acc (original order) -> prior
-        GS_TRY(unpermute_f32(g, w->stream, w->pr_prior.ptr, w->pr_acc.ptr));
-        GS_HIP(hipMemcpyAsync(w->pr_prior.ptr, w->pr_acc.ptr, g->n * 4, hipMemcpyDeviceToDevice, w->stream));
+    if (pull) {  // rank (relabelled) -> acc (original order), then the two trade places
+        GS_TRY(unpermute_f32(g, w->stream, w->ppr_rank.ptr, w->pr_acc.ptr, g->n));
+        std::swap(w->ppr_rank.ptr, w->pr_acc.ptr);   // both hold n values
     }
     GS_HIP(hipEventRecord(t1, w->stream));
     GS_TRY(sync_and_collect(w));
